@@ -1,0 +1,181 @@
+"""Benchmark: Fibonacci Circle-STARK proof at log_n = 24 on one MI355X.
+
+One step = one complete proof of the seeded Fibonacci statement (a0 = 1,
+a1 = SplitMix64(42) >> 33): device trace generation, interpolate, LDE, trace /
+quotient / FRI Blake2s commitments, device Fiat-Shamir, 80 queries,
+decommitment and the download of the finished KRAK v1 proof bytes. The
+prover's twiddle tables and buffers are built once before timing (the
+reference's ProverCache, src/prover.rs:172-218).
+
+Prints one JSON line (the driver's contract). `value` is milliseconds per
+proof (lower is better). `roofline` covers the dominant kernel, the trace-tree
+Blake2s subtree hash, which is VALU-bound; its duration comes from HIP events
+recorded around that kernel on the prover's stream. `cpu_baseline` times the
+C oracle (a single-threaded restatement of the same algorithm, same bytes) on
+a full log_n = 20 proof on this host and scales it to one log_n = 24 proof.
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Reference number for this metric: 133 ms on an RTX 5090 (README.md:37-43).
+REFERENCE_MS = 133.0
+# MI355X: 256 CUs x 64 int32 lanes x 2.4 GHz
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
+# VALU instructions of one Blake2s compression: 10 rounds x 8 G x
+# (2 add3 + 2 add + 4 xor + 4 rotate)
+OPS_PER_COMPRESSION = 10 * 8 * 12
+MASK64 = (1 << 64) - 1
+P = 0x7FFFFFFF
+
+
+def fib_a1(seed):
+    s = seed & MASK64
+    while True:
+        s = (s + 0x9E3779B97F4A7C15) & MASK64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        r = (z ^ (z >> 31)) >> 33
+        if r < P:
+            return r
+
+
+def golden_digest(log_n, blowup, pow_bits):
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
+            g = json.load(f)
+    except OSError:
+        return None
+    for p in g.get("proofs_c", []):
+        if (p["log_n"], p["blowup_bits"], p["pow_bits"]) == (log_n, blowup, pow_bits):
+            return p["sha256"]
+    return None
+
+
+def cpu_baseline(log_n_target, sample_log_n):
+    """Single-threaded oracle proof on this host, scaled to the target size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    a1 = fib_a1(42)
+    t0 = time.perf_counter()
+    pyoracle.prove(1, a1, sample_log_n)
+    secs = time.perf_counter() - t0
+    scale = float(1 << (log_n_target - sample_log_n))
+    return {
+        "value": round(secs * 1000.0 * scale, 1),
+        "unit": "ms",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"one full log_n={sample_log_n} proof ({secs:.2f} s, single thread, oracle/liboracle.so), "
+                  f"scaled x{int(scale)} (linear in the domain size) to one log_n={log_n_target} proof",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--blowup-bits", type=int, default=2)
+    ap.add_argument("--pow-bits", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import vortexstark_amd as vx
+
+    vx.load()
+    log_n, bb, pw = args.log_n, args.blowup_bits, args.pow_bits
+    a0, a1 = 1, fib_a1(42)
+    prover = vx.Prover(log_n, bb, vx.N_QUERIES, pw)
+    for _ in range(args.warmup):
+        prover.prove(a0, a1)
+    vx._lib.call("vx_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proof = prover.prove(a0, a1)
+    vx._lib.call("vx_device_sync")
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1000.0 / args.steps
+
+    # one timed proof for the stage breakdown and the dominant kernel
+    prover.enqueue(a0, a1, timed=True)
+    timed_proof = prover.wait()
+    stages = prover.stage_times()
+    digest = hashlib.sha256(proof).hexdigest()
+    want = golden_digest(log_n, bb, pw)
+    tv = time.perf_counter()
+    ok, why = vx.verify(proof, bb)
+    verify_ms = (time.perf_counter() - tv) * 1000.0
+
+    le = log_n + bb
+    N = 1 << le
+    tile = min(le, 12)
+    compressions = N + (N - (N >> tile))  # leaves + nodes inside the 4096-leaf tiles
+    k_ms = stages["trace_subtree_kernel"]
+    achieved = compressions * OPS_PER_COMPRESSION / (k_ms * 1e-3) / 1e12
+    lde_bytes = 4 * (1 << log_n) + 4 * N  # read coefficients once, write the evaluation once
+    lde_gbs = lde_bytes / (stages["lde"] * 1e-3) / 1e9
+
+    out = {
+        "metric": "Fib log_n=24 prove ms" if log_n == 24 else f"Fib log_n={log_n} prove ms",
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": round(ms / REFERENCE_MS, 4) if (log_n, bb, pw) == (24, 2, 0) else None,
+        "dtype": "u32 (M31/QM31 exact)",
+        "data": "synthetic (seeded Fibonacci statement a0=1, a1=SplitMix64(42))",
+        "config": {"workload": "fibonacci_stark_prove", "log_n": log_n, "blowup_bits": bb, "lde_log": le,
+                   "n_queries": vx.N_QUERIES, "pow_bits": pw, "proof_format": "KRAK v2" if pw else "KRAK v1",
+                   "parallelism": "single GPU"},
+        "roofline": {
+            "kernel": "k_subtree<LEAF_COLS,1,4> (trace-tree Blake2s, %d leaves)" % N,
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "TOP/s",
+            "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "traffic": None,
+            "kernel_ms": round(k_ms, 4),
+            "work": f"{compressions} Blake2s compressions x {OPS_PER_COMPRESSION} int ops",
+        },
+        "roofline_lde": {"bound": "hbm", "achieved": round(lde_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(lde_gbs / HBM_PEAK_GBS, 4), "stage_ms": round(stages["lde"], 4),
+                         "algorithmic_bytes": lde_bytes},
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "proof_bytes": len(proof),
+        "proof_sha256": digest,
+        "bit_exact_vs_oracle": (digest == want) if want else None,
+        "timed_proof_identical": timed_proof == proof,
+        "verified": ok,
+        "verify_ms": round(verify_ms, 2),
+        "device_bytes": prover.device_bytes,
+    }
+    if not ok:
+        out["verify_error"] = why
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(log_n, max(log_n - 4, 10))
+    prover.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

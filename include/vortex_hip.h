@@ -1,0 +1,229 @@
+/*
+ * vortex_hip.h -- C-ABI of the MI355X (gfx950) Circle-STARK prover hot path.
+ *
+ * Drop-in replacement for the CUDA FFI of garrick247/VortexSTARK
+ * (src/cuda/ffi.rs, resolved against libvortex_cuda.a built from the cuda/ kernels).
+ * Each entry point names the reference declaration it replaces. Conventions:
+ *   - same argument order and meaning as the reference `cuda_*` function;
+ *   - the `cuda_` prefix becomes `vx_`; a trailing hipStream_t selects the
+ *     stream (NULL = the legacy null stream);
+ *   - every function returns int: 0 = success, otherwise a vx_status code,
+ *     with a message from vx_last_error(); the reference returns void and
+ *     panics on the next failing runtime call;
+ *   - the caller owns and allocates all device buffers, outputs included;
+ *   - scalars given as `const uint32_t *alpha` ([4] QM31 words) are read from
+ *     HOST memory at call time, as in the reference (cuda/fri.cu:40-52);
+ *   - sizes are element counts (uint32_t in the reference; uint64_t is used
+ *     where 2^31-element domains need it).
+ * Arithmetic is exact over GF(2^31 - 1): results are bit-identical to the
+ * reference kernels for the same inputs.
+ */
+#ifndef VORTEX_HIP_H
+#define VORTEX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *vx_stream_t; /* == hipStream_t */
+
+enum vx_status {
+    VX_OK = 0,
+    VX_ERR_ARG = 1,
+    VX_ERR_HIP = 2,
+    VX_ERR_ALLOC = 3,
+    VX_ERR_VERIFY = 4,
+    VX_ERR_INTERNAL = 5,
+};
+
+/* Message for the last non-zero status returned on this thread. */
+const char *vx_last_error(void);
+/* Replaces cuda_device_sync (cuda/fields.cu:44-46; ffi.rs). */
+int vx_device_sync(void);
+
+/* ------------------------------------------------------------------ NTT */
+
+/* Replaces cuda_circle_ntt_evaluate (src/cuda/ffi.rs:261-269,
+ * cuda/circle_ntt.cu:248-275). Twiddle tables in the reference layout
+ * (src/circle.rs:284-347); h_layer_offsets/h_layer_sizes are HOST arrays.
+ * Like the reference, returns after the stream has finished. */
+int vx_circle_ntt_evaluate(uint32_t *d_data, const uint32_t *d_twiddles, const uint32_t *d_circle_twids,
+                           const uint32_t *h_layer_offsets, const uint32_t *h_layer_sizes,
+                           uint32_t n_line_layers, uint32_t n, vx_stream_t stream);
+
+/* Replaces cuda_circle_ntt_interpolate (ffi.rs:271-279, circle_ntt.cu:277-312),
+ * including the final 2^((30 log n) mod 31) scale. Synchronous like the reference. */
+int vx_circle_ntt_interpolate(uint32_t *d_data, const uint32_t *d_itwiddles, const uint32_t *d_circle_itwids,
+                              const uint32_t *h_layer_offsets, const uint32_t *h_layer_sizes,
+                              uint32_t n_line_layers, uint32_t n, vx_stream_t stream);
+
+/* Replaces cuda_circle_ntt_evaluate_batch / _interpolate_batch
+ * (ffi.rs:281-301; the default FORGE variants ffi.rs:407-428 take a HOST
+ * array of column pointers, as these do). Columns of equal size n. */
+int vx_circle_ntt_evaluate_batch(uint32_t *const *h_columns, const uint32_t *d_twiddles,
+                                 const uint32_t *d_circle_twids, const uint32_t *h_layer_offsets,
+                                 const uint32_t *h_layer_sizes, uint32_t n_line_layers, uint32_t n,
+                                 uint32_t n_cols, vx_stream_t stream);
+int vx_circle_ntt_interpolate_batch(uint32_t *const *h_columns, const uint32_t *d_itwiddles,
+                                    const uint32_t *d_circle_itwids, const uint32_t *h_layer_offsets,
+                                    const uint32_t *h_layer_sizes, uint32_t n_line_layers, uint32_t n,
+                                    uint32_t n_cols, vx_stream_t stream);
+
+/* Replaces cuda_circle_ntt_layer (ffi.rs:303-309, circle_ntt.cu:405-417):
+ * one butterfly layer with stride 2^layer_idx (0 = circle layer). Async. */
+int vx_circle_ntt_layer(uint32_t *d_data, const uint32_t *d_twiddles, uint32_t layer_idx, uint32_t n,
+                        int forward, vx_stream_t stream);
+
+/* Fused zero_pad + evaluate (src/prover.rs:303-318 = cuda_zero_pad then
+ * cuda_circle_ntt_evaluate): d_eval[0..n) = evaluate(coeffs zero-padded to n)
+ * on the domain whose tables are given. The first log2(n / coeff_n) layers
+ * only replicate the coefficients and are skipped. Async. */
+int vx_circle_ntt_evaluate_lde(uint32_t *d_eval, const uint32_t *d_coeffs, uint32_t coeff_n,
+                               const uint32_t *d_twiddles, const uint32_t *d_circle_twids,
+                               const uint32_t *h_layer_offsets, uint32_t n_line_layers, uint32_t n,
+                               vx_stream_t stream);
+
+/* Replaces cuda_bit_reverse_m31 (ffi.rs:311, circle_ntt.cu:234-244,422-427). Async. */
+int vx_bit_reverse_m31(uint32_t *d_data, uint32_t log_n, vx_stream_t stream);
+/* Replaces cuda_bit_reverse_qm31 (ffi.rs:1184, cuda/bit_reverse_wide.cu:21-52):
+ * out-of-place, 4 words per element. Async. */
+int vx_bit_reverse_qm31(const uint32_t *d_in, uint32_t *d_out, uint32_t n, uint32_t log_n, vx_stream_t stream);
+
+/* ------------------------------------------------------------- twiddles */
+
+/* Replaces cuda_compute_coset_points (ffi.rs:453-458, cuda/twiddles.cu:80-105,182-194). */
+int vx_compute_coset_points(uint32_t initial_x, uint32_t initial_y, uint32_t step_x, uint32_t step_y,
+                            uint32_t *d_out_x, uint32_t *d_out_y, uint32_t n, vx_stream_t stream);
+/* Replaces cuda_extract_and_squash (ffi.rs:462-467, twiddles.cu:127-142,204-213). */
+int vx_extract_and_squash(const uint32_t *d_in, uint32_t *d_twiddle_out, uint32_t *d_squash_out,
+                          uint32_t half_n, vx_stream_t stream);
+/* Replaces cuda_squash_x (ffi.rs:460, twiddles.cu:109-123,196-202). */
+int vx_squash_x(const uint32_t *d_in, uint32_t *d_out, uint32_t out_n, vx_stream_t stream);
+/* Replaces cuda_batch_inverse_m31[_stream] (ffi.rs:440,451, cuda/batch_inverse.cu:8-57);
+ * inverse(0) = 0. In-place (d_in == d_out) is allowed. */
+int vx_batch_inverse_m31(const uint32_t *d_in, uint32_t *d_out, uint32_t n, vx_stream_t stream);
+/* Replaces cuda_compute_fold_twiddle_sources[_stream] (ffi.rs:432-449,
+ * twiddles.cu:23-52): out[i] = coord(init * step^bitrev_log_n(2i)), i < n/2. */
+int vx_compute_fold_twiddle_sources(uint32_t initial_x, uint32_t initial_y, uint32_t step_x, uint32_t step_y,
+                                    uint32_t *d_out, uint32_t n, uint32_t log_n, int extract_y,
+                                    vx_stream_t stream);
+
+/* ---------------------------------------------------- AIR / constraints */
+
+/* Device counterpart of air::fibonacci_trace_parallel (src/air.rs:136-189):
+ * d_out[i] = Fib(a, b)[i] for i < 2^log_n. */
+int vx_fibonacci_trace(uint32_t a, uint32_t b, uint32_t *d_out, uint32_t log_n, vx_stream_t stream);
+/* Replaces cuda_fibonacci_quotient (ffi.rs:472-477, cuda/constraint.cu:10-36,94-105). */
+int vx_fibonacci_quotient(const uint32_t *d_trace, uint32_t *d_out0, uint32_t *d_out1, uint32_t *d_out2,
+                          uint32_t *d_out3, const uint32_t *h_alpha, uint32_t n, vx_stream_t stream);
+/* Replaces cuda_fibonacci_quotient_chunk[_stream] (ffi.rs:486-503, constraint.cu:38-64,121-153). */
+int vx_fibonacci_quotient_chunk(const uint32_t *d_trace, uint32_t *d_out0, uint32_t *d_out1, uint32_t *d_out2,
+                                uint32_t *d_out3, const uint32_t *h_alpha, uint32_t offset, uint32_t chunk_n,
+                                uint32_t global_n, vx_stream_t stream);
+/* Replaces cuda_zero_pad (ffi.rs:479-484, constraint.cu:81-90,108-117). */
+int vx_zero_pad(const uint32_t *d_src, uint32_t *d_dst, uint32_t src_n, uint32_t dst_n, vx_stream_t stream);
+
+/* ------------------------------------------------------------------ FRI */
+
+/* Replaces cuda_fold_line_soa (ffi.rs:346-352, cuda/fri.cu:58-87,130-145). */
+int vx_fold_line_soa(const uint32_t *d_in0, const uint32_t *d_in1, const uint32_t *d_in2, const uint32_t *d_in3,
+                     const uint32_t *d_twiddles, uint32_t *d_out0, uint32_t *d_out1, uint32_t *d_out2,
+                     uint32_t *d_out3, const uint32_t *h_alpha, uint32_t half_n, vx_stream_t stream);
+/* Replaces cuda_fold_circle_into_line_soa (ffi.rs:354-366, fri.cu:92-126,147-163). */
+int vx_fold_circle_into_line_soa(uint32_t *d_dst0, uint32_t *d_dst1, uint32_t *d_dst2, uint32_t *d_dst3,
+                                 const uint32_t *d_src0, const uint32_t *d_src1, const uint32_t *d_src2,
+                                 const uint32_t *d_src3, const uint32_t *d_twiddles, const uint32_t *h_alpha,
+                                 const uint32_t *h_alpha_sq, uint32_t half_n, vx_stream_t stream);
+
+/* --------------------------------------------------------------- Merkle */
+
+/* Replaces cuda_merkle_hash_leaves (ffi.rs:777-782, cuda/blake2s.cu:13-61,536-545):
+ * leaf = Blake2s(min(n_cols,16) LE words). h_columns is a HOST array of
+ * device column pointers (the reference uploads the same array first). */
+int vx_merkle_hash_leaves(const uint32_t *const *h_columns, uint32_t *d_hashes, uint32_t n_cols,
+                          uint32_t n_leaves, vx_stream_t stream);
+/* Replace cuda_merkle_hash_leaves_forge_single / _quad (ffi.rs:797-815). */
+int vx_merkle_hash_leaves_single(const uint32_t *d_col, uint32_t *d_hashes, uint32_t n_leaves, vx_stream_t stream);
+int vx_merkle_hash_leaves_quad(const uint32_t *d_c0, const uint32_t *d_c1, const uint32_t *d_c2,
+                               const uint32_t *d_c3, uint32_t *d_hashes, uint32_t n_leaves, vx_stream_t stream);
+/* Replaces cuda_merkle_hash_nodes (ffi.rs:818-822, blake2s.cu:64-87,547-555). */
+int vx_merkle_hash_nodes(const uint32_t *d_children, uint32_t *d_parents, uint32_t n_parents, vx_stream_t stream);
+/* Replaces cuda_merkle_reduce_to_root (ffi.rs:824-828, blake2s.cu:244-297,558-565):
+ * n_nodes a power of two <= 1024. */
+int vx_merkle_reduce_to_root(const uint32_t *d_nodes, uint32_t *d_root, uint32_t n_nodes, vx_stream_t stream);
+/* Replaces cuda_merkle_commit_small_soa4 (ffi.rs:830-835, blake2s.cu:92-189,569-580):
+ * root of a QM31 SoA tree of n_leaves (power of two, 4..2048). */
+int vx_merkle_commit_small_soa4(const uint32_t *d_c0, const uint32_t *d_c1, const uint32_t *d_c2,
+                                const uint32_t *d_c3, uint32_t *d_root, uint32_t n_leaves, vx_stream_t stream);
+/* Replaces cuda_merkle_hash_leaves_merge_soa4 (ffi.rs:837-842, blake2s.cu:195-240,584-596):
+ * parents[j] = H(leaf(2j), leaf(2j+1)). */
+int vx_merkle_hash_leaves_merge_soa4(const uint32_t *d_c0, const uint32_t *d_c1, const uint32_t *d_c2,
+                                     const uint32_t *d_c3, uint32_t *d_parents, uint32_t n_leaves,
+                                     vx_stream_t stream);
+/* Replace cuda_merkle_tiled_soa4[_stream] / cuda_merkle_tiled_generic
+ * (ffi.rs:844-864, blake2s.cu:305-532,600-644): the root of every 1024-leaf
+ * tile, n_leaves a multiple of 1024. */
+int vx_merkle_tiled_soa4(const uint32_t *d_c0, const uint32_t *d_c1, const uint32_t *d_c2, const uint32_t *d_c3,
+                         uint32_t *d_subtree_roots, uint32_t n_leaves, vx_stream_t stream);
+int vx_merkle_tiled_generic(const uint32_t *const *h_columns, uint32_t *d_subtree_roots, uint32_t n_cols,
+                            uint32_t n_leaves, vx_stream_t stream);
+
+/* ------------------------------------------------------------------ PoW */
+
+/* Replaces cuda_grind_pow (ffi.rs:766-772, cuda/grind.cu:26-110): tests
+ * nonces [batch_offset, batch_offset + n_threads) against the prefixed
+ * digest and atomically lowers *d_result (caller initialises it to
+ * UINT64_MAX). Unlike the reference, the result is the smallest valid nonce
+ * of the batch (no racing early exit). */
+int vx_grind_pow(const uint32_t *d_prefixed_digest, uint64_t *d_result, uint32_t pow_bits, uint64_t batch_offset,
+                 uint32_t n_threads, vx_stream_t stream);
+/* Whole deterministic grind: *d_result = smallest nonce >= 0 with
+ * trailing_zeros(Blake2s(prefix || nonce)[0..16]) >= pow_bits
+ * (Channel::verify_pow_nonce, src/channel.rs:140-168). pow_bits <= 32. */
+int vx_grind_pow_min(const uint32_t *d_prefixed_digest, uint32_t pow_bits, uint64_t *d_result, vx_stream_t stream);
+
+/* --------------------------------------------- prove / verify (src/prover.rs) */
+
+typedef struct vx_prover vx_prover;
+
+/* Size in bytes of a KRAK proof (v1 when pow_bits = 0, else v2). */
+size_t vx_proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits);
+
+/* ProverCache::new counterpart (src/prover.rs:172-218): twiddles, all
+ * intermediate buffers and the proof buffer for one size class.
+ * stream NULL = a private non-blocking stream. */
+int vx_prover_create(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits,
+                     vx_stream_t stream, vx_prover **out);
+int vx_prover_destroy(vx_prover *p);
+/* prove_cached counterpart (src/prover.rs:231-234): writes the KRAK bytes
+ * (src/bin/stark_cli.rs:1092-1120) of the proof of Fib(a, b) to out. */
+int vx_prover_prove(vx_prover *p, uint32_t a, uint32_t b, uint8_t *out, size_t out_cap, size_t *out_len);
+/* Split form: enqueue all kernels without waiting, then wait + copy. */
+int vx_prover_enqueue(vx_prover *p, uint32_t a, uint32_t b, int timed);
+int vx_prover_wait(vx_prover *p, uint8_t *out, size_t out_cap, size_t *out_len);
+/* Stage times (ms) of the last timed proof: trace, interpolate, lde,
+ * trace_commit, quotient_commit, fri, queries, decommit, download, total,
+ * and the trace-tree subtree hashing kernel alone. */
+int vx_prover_stage_times(vx_prover *p, float out[11]);
+/* Device bytes held by the prover (admission accounting). */
+uint64_t vx_prover_device_bytes(const vx_prover *p);
+/* Device buffers of the last proof, for inspection: which = 0 trace/
+ * coefficients (2^log_n), 1 LDE evaluation (2^log_eval), 2 + k FRI layer k
+ * (4 SoA columns back to back), 100 the KRAK proof buffer. */
+int vx_prover_buffer(const vx_prover *p, int which, const uint32_t **d_ptr, uint64_t *words);
+
+/* prover::prove counterpart (src/prover.rs:221-223): one-shot proof. */
+int vx_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits,
+             uint8_t *out, size_t out_cap, size_t *out_len);
+/* verifier::verify counterpart (src/verifier.rs:39-236) over KRAK bytes:
+ * VX_OK if accepted, VX_ERR_VERIFY (reason in err) if rejected. Host only. */
+int vx_verify(const uint8_t *proof, size_t len, uint32_t blowup_bits, uint32_t n_queries, char *err,
+              size_t err_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VORTEX_HIP_H */

@@ -1,0 +1,301 @@
+"""ctypes wrapper for the CPU oracle (oracle/liboracle*.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker. The product package never
+imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+P = 0x7FFFFFFF
+SUBGROUP, HALF_COSET, HALF_ODDS = 0, 1, 2
+
+_libs = {}
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc only)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class Channel(C.Structure):
+    _fields_ = [("state", C.c_uint8 * 32), ("n_draws", C.c_uint32)]
+
+
+def lib(omp=False):
+    name = "liboracle_omp.so" if omp else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    u32, u64, sz = C.c_uint32, C.c_uint64, C.c_size_t
+    sig = {
+        "or_m31_add": (u32, [u32, u32]),
+        "or_m31_sub": (u32, [u32, u32]),
+        "or_m31_mul": (u32, [u32, u32]),
+        "or_m31_inv": (u32, [u32]),
+        "or_cm31_mul": (None, [_u32p, _u32p, _u32p]),
+        "or_qm31_mul": (None, [_u32p, _u32p, _u32p]),
+        "or_qm31_inv": (None, [_u32p, _u32p]),
+        "or_coset": (None, [C.c_int, u32, _u32p]),
+        "or_eval_domain": (None, [u32, _u32p]),
+        "or_coset_at": (None, [_u32p, u64, _u32p]),
+        "or_circle_mul": (None, [_u32p, _u32p, _u32p]),
+        "or_blake2s": (None, [C.c_char_p, sz, _u8p]),
+        "or_blake2s_single": (None, [C.c_char_p, sz, _u8p]),
+        "or_compute_twiddles": (None, [_u32p, _u32p, _u32p, _u32p, _u32p]),
+        "or_batch_inverse": (None, [_u32p, _u32p, sz]),
+        "or_fold_twiddles": (None, [_u32p, C.c_int, _u32p]),
+        "or_ntt_layer": (None, [_u32p, _u32p, u32, u64, C.c_int]),
+        "or_ntt_evaluate": (None, [_u32p, _u32p, _u32p, _u32p, u32, u64]),
+        "or_ntt_interpolate": (None, [_u32p, _u32p, _u32p, _u32p, u32, u64]),
+        "or_bit_reverse_m31": (None, [_u32p, u32]),
+        "or_fibonacci_trace": (None, [u32, u32, u32, _u32p]),
+        "or_zero_pad": (None, [_u32p, _u32p, u64, u64]),
+        "or_fibonacci_quotient": (None, [_u32p, _u32p, _u32p, _u32p, _u32p, _u32p, u64]),
+        "or_fold_line": (None, [C.c_void_p, _u32p, C.c_void_p, _u32p, u64]),
+        "or_fold_circle_into_line": (None, [C.c_void_p, C.c_void_p, _u32p, _u32p, _u32p, u64]),
+        "or_merkle_leaf_hashes": (None, [C.c_void_p, u32, u64, _u32p]),
+        "or_merkle_hash_nodes": (None, [_u32p, _u32p, u64]),
+        "or_merkle_root": (None, [C.c_void_p, u32, u32, _u32p]),
+        "or_verify_auth_path": (C.c_int, [_u32p, _u32p, u64, _u32p, u32]),
+        "or_channel_init": (None, [C.POINTER(Channel)]),
+        "or_channel_mix_digest": (None, [C.POINTER(Channel), _u32p]),
+        "or_channel_mix_felts": (None, [C.POINTER(Channel), _u32p, sz]),
+        "or_channel_draw_felt": (None, [C.POINTER(Channel), _u32p]),
+        "or_channel_draw_number": (u64, [C.POINTER(Channel), u64]),
+        "or_channel_mix_u64": (None, [C.POINTER(Channel), u64]),
+        "or_channel_verify_pow_nonce": (C.c_int, [C.POINTER(Channel), u32, u64]),
+        "or_grind_pow": (u64, [C.POINTER(Channel), u32]),
+        "or_prove": (C.c_int, [u32, u32, u32, u32, u32, u32,
+                               C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz)]),
+        "or_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
+        "or_proof_size": (sz, [u32, u32, u32, u32]),
+        "or_free": (None, [C.c_void_p]),
+    }
+    for name_, (res, args) in sig.items():
+        f = getattr(L, name_)
+        f.restype = res
+        f.argtypes = args
+    _libs[name] = L
+    return L
+
+
+def u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _ptrs(cols):
+    arr = (C.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+    return arr
+
+
+# ---------------- convenience wrappers ----------------
+
+def coset(kind, log_size):
+    out = np.zeros(5, np.uint32)
+    lib().or_coset(kind, log_size, out)
+    return out
+
+
+def eval_domain(log_size):
+    out = np.zeros(5, np.uint32)
+    lib().or_eval_domain(log_size, out)
+    return out
+
+
+def coset_at(c, i):
+    out = np.zeros(2, np.uint32)
+    lib().or_coset_at(u32(c), i, out)
+    return int(out[0]), int(out[1])
+
+
+def blake2s(data: bytes) -> bytes:
+    out = np.zeros(32, np.uint8)
+    lib().or_blake2s(data, len(data), out)
+    return out.tobytes()
+
+
+def twiddles(c):
+    """Reference-layout forward twiddles of a coset: (line, circle, offsets, sizes)."""
+    c = u32(c)
+    log_n = int(c[4])
+    n = 1 << log_n
+    line = np.zeros(max(n - 1, 1), np.uint32)
+    circ = np.zeros(max(n // 2, 1), np.uint32)
+    offs = np.zeros(32, np.uint32)
+    sizes = np.zeros(32, np.uint32)
+    lib().or_compute_twiddles(c, line, circ, offs, sizes)
+    nl = max(log_n - 1, 0)
+    return line, circ, offs[:nl].copy(), sizes[:nl].copy()
+
+
+def batch_inverse(x):
+    x = u32(x)
+    out = np.zeros_like(x)
+    lib().or_batch_inverse(x, out, x.size)
+    return out
+
+
+def inverse_twiddles(c):
+    line, circ, offs, sizes = twiddles(c)
+    n_used = int(sizes.sum())
+    iline = line.copy()
+    iline[:n_used] = batch_inverse(line[:n_used])
+    return iline, batch_inverse(circ), offs, sizes
+
+
+def fold_twiddles(c, extract_y):
+    c = u32(c)
+    out = np.zeros((1 << int(c[4])) // 2, np.uint32)
+    lib().or_fold_twiddles(c, 1 if extract_y else 0, out)
+    return out
+
+
+def evaluate(data, c):
+    d = u32(data).copy()
+    line, circ, offs, _ = twiddles(c)
+    lib().or_ntt_evaluate(d, line, circ, u32(offs) if offs.size else np.zeros(1, np.uint32),
+                          offs.size, d.size)
+    return d
+
+
+def interpolate(data, c):
+    d = u32(data).copy()
+    line, circ, offs, _ = inverse_twiddles(c)
+    lib().or_ntt_interpolate(d, line, circ, u32(offs) if offs.size else np.zeros(1, np.uint32),
+                             offs.size, d.size)
+    return d
+
+
+def ntt_layer(data, tw, layer_idx, forward):
+    d = u32(data).copy()
+    lib().or_ntt_layer(d, u32(tw), layer_idx, d.size, 1 if forward else 0)
+    return d
+
+
+def bit_reverse(data):
+    d = u32(data).copy()
+    lib().or_bit_reverse_m31(d, int(d.size).bit_length() - 1)
+    return d
+
+
+def fibonacci_trace(a, b, log_n):
+    out = np.zeros(1 << log_n, np.uint32)
+    lib().or_fibonacci_trace(a, b, log_n, out)
+    return out
+
+
+def quotient(trace, alpha):
+    t = u32(trace)
+    outs = [np.zeros_like(t) for _ in range(4)]
+    lib().or_fibonacci_quotient(t, *outs, u32(alpha), t.size)
+    return outs
+
+
+def fold_line(cols, tw, alpha):
+    cols = [u32(c) for c in cols]
+    half = cols[0].size // 2
+    outs = [np.zeros(half, np.uint32) for _ in range(4)]
+    lib().or_fold_line(_ptrs(cols), u32(tw), _ptrs(outs), u32(alpha), half)
+    return outs
+
+
+def fold_circle_into_line(dst, src, tw, alpha, alpha_sq):
+    dst = [u32(c).copy() for c in dst]
+    src = [u32(c) for c in src]
+    lib().or_fold_circle_into_line(_ptrs(dst), _ptrs(src), u32(tw), u32(alpha), u32(alpha_sq),
+                                   dst[0].size)
+    return dst
+
+
+def leaf_hashes(cols):
+    cols = [u32(c) for c in cols]
+    out = np.zeros((cols[0].size, 8), np.uint32)
+    lib().or_merkle_leaf_hashes(_ptrs(cols), len(cols), cols[0].size, out)
+    return out
+
+
+def hash_nodes(children):
+    ch = u32(children).reshape(-1)
+    out = np.zeros((ch.size // 16, 8), np.uint32)
+    lib().or_merkle_hash_nodes(ch, out, ch.size // 16)
+    return out
+
+
+def merkle_root(cols):
+    cols = [u32(c) for c in cols]
+    out = np.zeros(8, np.uint32)
+    lib().or_merkle_root(_ptrs(cols), len(cols), int(cols[0].size).bit_length() - 1, out)
+    return out
+
+
+def qm31_mul(a, b):
+    out = np.zeros(4, np.uint32)
+    lib().or_qm31_mul(u32(a), u32(b), out)
+    return out
+
+
+def prove(a, b, log_n, blowup_bits=2, n_queries=80, pow_bits=0, omp=False):
+    L = lib(omp)
+    p = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    rc = L.or_prove(a, b, log_n, blowup_bits, n_queries, pow_bits, C.byref(p), C.byref(n))
+    if rc != 0:
+        raise ValueError(f"or_prove rejected arguments (rc={rc})")
+    data = C.string_at(p, n.value)
+    L.or_free(p)
+    return data
+
+
+def verify(proof: bytes, blowup_bits=2, n_queries=80):
+    err = C.create_string_buffer(256)
+    ok = lib().or_verify(proof, len(proof), blowup_bits, n_queries, err, 256)
+    return bool(ok), err.value.decode()
+
+
+def proof_size(log_n, blowup_bits=2, n_queries=80, pow_bits=0):
+    return lib().or_proof_size(log_n, blowup_bits, n_queries, pow_bits)
+
+
+class PyChannel:
+    def __init__(self):
+        self.c = Channel()
+        lib().or_channel_init(C.byref(self.c))
+
+    @property
+    def state(self):
+        return bytes(self.c.state)
+
+    def mix_digest(self, d):
+        lib().or_channel_mix_digest(C.byref(self.c), u32(d))
+
+    def mix_felts(self, felts):
+        f = u32(felts).reshape(-1)
+        lib().or_channel_mix_felts(C.byref(self.c), f, f.size // 4)
+
+    def draw_felt(self):
+        out = np.zeros(4, np.uint32)
+        lib().or_channel_draw_felt(C.byref(self.c), out)
+        return out
+
+    def draw_number(self, bound):
+        return int(lib().or_channel_draw_number(C.byref(self.c), bound))
+
+    def mix_u64(self, v):
+        lib().or_channel_mix_u64(C.byref(self.c), v)
+
+    def verify_pow_nonce(self, bits, nonce):
+        return bool(lib().or_channel_verify_pow_nonce(C.byref(self.c), bits, nonce))
+
+    def grind(self, bits):
+        return int(lib().or_grind_pow(C.byref(self.c), bits))

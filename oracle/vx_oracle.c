@@ -1,0 +1,1041 @@
+/*
+ * vx_oracle.c -- CPU restatement of the VortexSTARK Fibonacci prove path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the correctness oracle and the timed
+ * CPU baseline. It is loaded by tests/, by __graft_entry__.smoke() and by the
+ * cpu_baseline leg of bench.py, always as the checker and never as the thing
+ * measured on the GPU. The product library (vortexstark_amd) does not link it.
+ *
+ * The reference is Rust + CUDA and has no CPU path for this prover; this file
+ * restates its algorithm from the source text. Each function names the
+ * reference file:line it follows. Arithmetic is exact over GF(2^31 - 1), so
+ * evaluation order never changes a result.
+ *
+ * Built by oracle/Makefile into oracle/liboracle.so (single-threaded) and
+ * oracle/liboracle_omp.so (OpenMP over the data-parallel loops; same bytes).
+ */
+#include "vx_oracle.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define P OR_P
+
+/* ======================================================================
+ * Field: src/field/m31.rs:24-35,86-113; cm31.rs:76-86; qm31.rs:113-132
+ * ====================================================================== */
+static inline uint32_t m31_add(uint32_t a, uint32_t b) {
+    uint32_t r = a + b;
+    return r >= P ? r - P : r;
+}
+static inline uint32_t m31_sub(uint32_t a, uint32_t b) {
+    return a >= b ? a - b : a + P - b;
+}
+static inline uint32_t m31_reduce(uint64_t x) {
+    uint32_t lo = (uint32_t)(x & P);
+    uint32_t hi = (uint32_t)(x >> 31);
+    uint32_t r = lo + hi;
+    return r >= P ? r - P : r;
+}
+static inline uint32_t m31_mul(uint32_t a, uint32_t b) {
+    return m31_reduce((uint64_t)a * (uint64_t)b);
+}
+static inline uint32_t m31_neg(uint32_t a) { return a == 0 ? 0 : P - a; }
+static uint32_t m31_pow(uint32_t base, uint32_t e) {
+    uint32_t r = 1;
+    while (e) {
+        if (e & 1) r = m31_mul(r, base);
+        base = m31_mul(base, base);
+        e >>= 1;
+    }
+    return r;
+}
+/* m31.rs:32-35: Fermat inverse, so inv(0) = 0 */
+static inline uint32_t m31_inv(uint32_t a) { return m31_pow(a, P - 2); }
+
+uint32_t or_m31_add(uint32_t a, uint32_t b) { return m31_add(a, b); }
+uint32_t or_m31_sub(uint32_t a, uint32_t b) { return m31_sub(a, b); }
+uint32_t or_m31_mul(uint32_t a, uint32_t b) { return m31_mul(a, b); }
+uint32_t or_m31_inv(uint32_t a) { return m31_inv(a); }
+
+typedef struct { uint32_t a, b; } cm31;
+typedef struct { uint32_t v[4]; } qm31; /* [a.re, a.im, b.re, b.im] */
+
+static inline cm31 cm_add(cm31 x, cm31 y) { return (cm31){m31_add(x.a, y.a), m31_add(x.b, y.b)}; }
+static inline cm31 cm_sub(cm31 x, cm31 y) { return (cm31){m31_sub(x.a, y.a), m31_sub(x.b, y.b)}; }
+/* cm31.rs:76-86: (a+bi)(c+di) = (ac-bd) + (ad+bc)i */
+static inline cm31 cm_mul(cm31 x, cm31 y) {
+    return (cm31){m31_sub(m31_mul(x.a, y.a), m31_mul(x.b, y.b)),
+                  m31_add(m31_mul(x.a, y.b), m31_mul(x.b, y.a))};
+}
+static inline cm31 cm_scale(cm31 x, uint32_t s) { return (cm31){m31_mul(x.a, s), m31_mul(x.b, s)}; }
+/* cm31.rs:38-51 */
+static cm31 cm_inv(cm31 x) {
+    uint32_t inv_norm = m31_inv(m31_add(m31_mul(x.a, x.a), m31_mul(x.b, x.b)));
+    return (cm31){m31_mul(x.a, inv_norm), m31_mul(m31_neg(x.b), inv_norm)};
+}
+/* (p+qi)(2+i) = (2p-q) + (p+2q)i ; qm31.rs:120-124 */
+static inline cm31 cm_mul_u2(cm31 x) {
+    return (cm31){m31_sub(m31_add(x.a, x.a), x.b), m31_add(x.a, m31_add(x.b, x.b))};
+}
+static inline cm31 q_a(qm31 x) { return (cm31){x.v[0], x.v[1]}; }
+static inline cm31 q_b(qm31 x) { return (cm31){x.v[2], x.v[3]}; }
+static inline qm31 q_make(cm31 a, cm31 b) { return (qm31){{a.a, a.b, b.a, b.b}}; }
+static inline qm31 q_add(qm31 x, qm31 y) {
+    qm31 r;
+    for (int i = 0; i < 4; i++) r.v[i] = m31_add(x.v[i], y.v[i]);
+    return r;
+}
+static inline qm31 q_sub(qm31 x, qm31 y) {
+    qm31 r;
+    for (int i = 0; i < 4; i++) r.v[i] = m31_sub(x.v[i], y.v[i]);
+    return r;
+}
+/* qm31.rs:113-132: (a+bu)(c+du) = (ac + bd(2+i)) + (ad+bc)u */
+static inline qm31 q_mul(qm31 x, qm31 y) {
+    cm31 ac = cm_mul(q_a(x), q_a(y)), bd = cm_mul(q_b(x), q_b(y));
+    cm31 ad = cm_mul(q_a(x), q_b(y)), bc = cm_mul(q_b(x), q_a(y));
+    return q_make(cm_add(ac, cm_mul_u2(bd)), cm_add(ad, bc));
+}
+static inline qm31 q_scale(qm31 x, uint32_t s) {
+    qm31 r;
+    for (int i = 0; i < 4; i++) r.v[i] = m31_mul(x.v[i], s);
+    return r;
+}
+/* qm31.rs:67-91: conj / norm */
+static qm31 q_inv(qm31 x) {
+    cm31 a2 = cm_mul(q_a(x), q_a(x)), b2 = cm_mul(q_b(x), q_b(x));
+    cm31 inv_norm = cm_inv(cm_sub(a2, cm_mul_u2(b2)));
+    cm31 nb = (cm31){m31_neg(x.v[2]), m31_neg(x.v[3])};
+    return q_make(cm_mul(q_a(x), inv_norm), cm_mul(nb, inv_norm));
+}
+static inline qm31 q_load(const uint32_t *p) { return (qm31){{p[0], p[1], p[2], p[3]}}; }
+
+void or_cm31_mul(const uint32_t a[2], const uint32_t b[2], uint32_t out[2]) {
+    cm31 r = cm_mul((cm31){a[0], a[1]}, (cm31){b[0], b[1]});
+    out[0] = r.a;
+    out[1] = r.b;
+}
+void or_qm31_mul(const uint32_t a[4], const uint32_t b[4], uint32_t out[4]) {
+    qm31 r = q_mul(q_load(a), q_load(b));
+    memcpy(out, r.v, 16);
+}
+void or_qm31_inv(const uint32_t a[4], uint32_t out[4]) {
+    qm31 r = q_inv(q_load(a));
+    memcpy(out, r.v, 16);
+}
+
+/* ======================================================================
+ * Circle group: src/circle.rs:12-185
+ * ====================================================================== */
+typedef struct { uint32_t x, y; } cpt;
+static const cpt C_ID = {1, 0};
+static const cpt C_GEN = {2, 1268011823u}; /* circle.rs:26-29 */
+
+/* circle.rs:33-38 */
+static inline cpt c_mul(cpt a, cpt b) {
+    return (cpt){m31_sub(m31_mul(a.x, b.x), m31_mul(a.y, b.y)),
+                 m31_add(m31_mul(a.x, b.y), m31_mul(a.y, b.x))};
+}
+static cpt c_rdouble(cpt p, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) p = c_mul(p, p);
+    return p;
+}
+/* circle.rs:65-76 (u32 scalar in the reference; u64 here, identical below 2^32) */
+static cpt c_pow(cpt base, uint64_t k) {
+    cpt r = C_ID;
+    while (k) {
+        if (k & 1) r = c_mul(r, base);
+        base = c_mul(base, base);
+        k >>= 1;
+    }
+    return r;
+}
+
+typedef struct { cpt init, step; uint32_t log_size; } coset_t;
+
+static coset_t coset_unpack(const uint32_t c[5]) {
+    coset_t r = {{c[0], c[1]}, {c[2], c[3]}, c[4]};
+    return r;
+}
+static void coset_pack(coset_t c, uint32_t out[5]) {
+    out[0] = c.init.x; out[1] = c.init.y; out[2] = c.step.x; out[3] = c.step.y; out[4] = c.log_size;
+}
+/* circle.rs:101-143 */
+static coset_t coset_make(int kind, uint32_t L) {
+    coset_t c;
+    c.log_size = L;
+    c.step = c_rdouble(C_GEN, 31 - L);
+    if (kind == OR_SUBGROUP) c.init = C_ID;
+    else if (kind == OR_HALF_COSET) c.init = c_rdouble(C_GEN, 30 - L);
+    else c.init = c_rdouble(C_GEN, L >= 29 ? 0 : 29 - L); /* half_odds: saturating_sub */
+    return c;
+}
+/* prover.rs:655-659: the eval domain is subgroup(31) when log_eval=31 */
+static coset_t eval_domain(uint32_t L) {
+    return L <= 30 ? coset_make(OR_HALF_COSET, L) : coset_make(OR_SUBGROUP, L);
+}
+/* circle.rs:258-267 */
+static cpt coset_at(coset_t c, uint64_t i) { return c_mul(c.init, c_pow(c.step, i)); }
+
+void or_coset(int kind, uint32_t log_size, uint32_t out[5]) { coset_pack(coset_make(kind, log_size), out); }
+void or_eval_domain(uint32_t log_size, uint32_t out[5]) { coset_pack(eval_domain(log_size), out); }
+void or_coset_at(const uint32_t c[5], uint64_t i, uint32_t out[2]) {
+    cpt p = coset_at(coset_unpack(c), i);
+    out[0] = p.x;
+    out[1] = p.y;
+}
+void or_circle_mul(const uint32_t a[2], const uint32_t b[2], uint32_t out[2]) {
+    cpt p = c_mul((cpt){a[0], a[1]}, (cpt){b[0], b[1]});
+    out[0] = p.x;
+    out[1] = p.y;
+}
+
+/* ======================================================================
+ * Blake2s-256 (RFC 7693), as src/channel.rs:216-298 and cuda/include/
+ * blake2s.cuh:53-136: h0 ^= 0x01010020, t0 = byte length, final flag.
+ * ====================================================================== */
+static const uint32_t B2_IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                  0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+static const uint8_t B2_SIGMA[10][16] = {
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15},
+    {14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3},
+    {11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4},
+    {7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8},
+    {9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13},
+    {2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9},
+    {12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11},
+    {13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10},
+    {6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5},
+    {10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0},
+};
+static inline uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+static void b2_compress(uint32_t h[8], const uint32_t m[16], uint64_t t, int last) {
+    uint32_t v[16];
+    for (int i = 0; i < 8; i++) { v[i] = h[i]; v[i + 8] = B2_IV[i]; }
+    v[12] ^= (uint32_t)t;
+    v[13] ^= (uint32_t)(t >> 32);
+    if (last) v[14] ^= 0xFFFFFFFFu;
+#define G2(a, b, c, d, x, y)                 \
+    do {                                     \
+        v[a] = v[a] + v[b] + (x);            \
+        v[d] = rotr32(v[d] ^ v[a], 16);      \
+        v[c] = v[c] + v[d];                  \
+        v[b] = rotr32(v[b] ^ v[c], 12);      \
+        v[a] = v[a] + v[b] + (y);            \
+        v[d] = rotr32(v[d] ^ v[a], 8);       \
+        v[c] = v[c] + v[d];                  \
+        v[b] = rotr32(v[b] ^ v[c], 7);       \
+    } while (0)
+    for (int r = 0; r < 10; r++) {
+        const uint8_t *s = B2_SIGMA[r];
+        G2(0, 4, 8, 12, m[s[0]], m[s[1]]);
+        G2(1, 5, 9, 13, m[s[2]], m[s[3]]);
+        G2(2, 6, 10, 14, m[s[4]], m[s[5]]);
+        G2(3, 7, 11, 15, m[s[6]], m[s[7]]);
+        G2(0, 5, 10, 15, m[s[8]], m[s[9]]);
+        G2(1, 6, 11, 12, m[s[10]], m[s[11]]);
+        G2(2, 7, 8, 13, m[s[12]], m[s[13]]);
+        G2(3, 4, 9, 14, m[s[14]], m[s[15]]);
+    }
+#undef G2
+    for (int i = 0; i < 8; i++) h[i] ^= v[i] ^ v[i + 8];
+}
+static inline uint32_t le32(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+static inline void put32(uint8_t *p, uint32_t v) {
+    p[0] = (uint8_t)v; p[1] = (uint8_t)(v >> 8); p[2] = (uint8_t)(v >> 16); p[3] = (uint8_t)(v >> 24);
+}
+
+/* Standard unkeyed Blake2s-256 over any length (the `blake2` crate path used
+ * by Channel::mix_felts, src/channel.rs:46-59). */
+void or_blake2s(const uint8_t *in, size_t len, uint8_t out[32]) {
+    uint32_t h[8], m[16];
+    memcpy(h, B2_IV, sizeof h);
+    h[0] ^= 0x01010020u;
+    uint64_t t = 0;
+    size_t off = 0;
+    while (len - off > 64) {
+        for (int i = 0; i < 16; i++) m[i] = le32(in + off + 4 * i);
+        t += 64;
+        b2_compress(h, m, t, 0);
+        off += 64;
+    }
+    uint8_t blk[64] = {0};
+    memcpy(blk, in + off, len - off);
+    t += len - off;
+    for (int i = 0; i < 16; i++) m[i] = le32(blk + 4 * i);
+    b2_compress(h, m, t, 1);
+    for (int i = 0; i < 8; i++) put32(out + 4 * i, h[i]);
+}
+/* src/channel.rs:254-257: the single-block helper hashes min(len, 64) bytes. */
+void or_blake2s_single(const uint8_t *in, size_t len, uint8_t out[32]) {
+    or_blake2s(in, len < 64 ? len : 64, out);
+}
+
+/* Word-level helpers matching cuda/blake2s.cu:13-87 (leaf of n<=16 words,
+ * node of two 8-word children). */
+static void hash_words(const uint32_t *w, uint32_t nw, uint32_t out[8]) {
+    uint32_t h[8], m[16] = {0};
+    memcpy(h, B2_IV, sizeof h);
+    h[0] ^= 0x01010020u;
+    if (nw > 16) nw = 16;
+    for (uint32_t i = 0; i < nw; i++) m[i] = w[i];
+    b2_compress(h, m, 4u * nw, 1);
+    memcpy(out, h, 32);
+}
+static void hash_node(const uint32_t *l, const uint32_t *r, uint32_t out[8]) {
+    uint32_t m[16];
+    memcpy(m, l, 32);
+    memcpy(m + 8, r, 32);
+    hash_words(m, 16, out);
+}
+
+/* ======================================================================
+ * Twiddles: src/circle.rs:284-347 (compute_twiddles), cuda/twiddles.cu:
+ * 80-142 (coset points, extract_and_squash), cuda/batch_inverse.cu:8-41.
+ * ====================================================================== */
+void or_compute_twiddles(const uint32_t c5[5], uint32_t *line_tw, uint32_t *circle_tw,
+                         uint32_t *offsets, uint32_t *sizes) {
+    coset_t c = coset_unpack(c5);
+    uint64_t n = 1ull << c.log_size;
+    uint32_t *xs = (uint32_t *)malloc(n * 4);
+    /* coset points p_i = init * step^i (twiddles.cu:80-105); sequential
+     * multiplication gives identical exact values. */
+    cpt p = c.init;
+    for (uint64_t i = 0; i < n; i++) {
+        xs[i] = p.x;
+        if (i < n / 2) circle_tw[i] = p.y;
+        p = c_mul(p, c.step);
+    }
+    uint32_t n_line_layers = c.log_size > 0 ? c.log_size - 1 : 0;
+    uint64_t cur_n = n, off = 0;
+    for (uint32_t l = 0; l < n_line_layers; l++) {
+        uint64_t half = cur_n / 2;
+        offsets[l] = (uint32_t)off;
+        sizes[l] = (uint32_t)half;
+        /* extract_and_squash: tw[i] = x[2i]; next[i] = 2 x[2i]^2 - 1 */
+        for (uint64_t i = 0; i < half; i++) {
+            uint32_t x = xs[2 * i];
+            line_tw[off + i] = x;
+            uint32_t x2 = m31_mul(x, x);
+            xs[i] = m31_sub(m31_add(x2, x2), 1);
+        }
+        off += half;
+        cur_n = half;
+    }
+    free(xs);
+}
+
+/* batch_inverse.cu:8-41: inverse with inverse(0) := 0 (chunking does not
+ * change exact results). */
+void or_batch_inverse(const uint32_t *in, uint32_t *out, size_t n) {
+    for (size_t i = 0; i < n; i++) out[i] = in[i] == 0 ? 0 : m31_inv(in[i]);
+}
+
+static inline uint64_t bitrev(uint64_t x, uint32_t bits) {
+    uint64_t r = 0;
+    for (uint32_t i = 0; i < bits; i++) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+/* twiddles.cu:23-52 + batch inverse (src/fri.rs:112-139):
+ * out[i] = coord(D.at(bitrev_L(2i)))^-1 for i < |D|/2 */
+void or_fold_twiddles(const uint32_t c5[5], int extract_y, uint32_t *out) {
+    coset_t c = coset_unpack(c5);
+    uint64_t half = (1ull << c.log_size) / 2;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)half; i++) {
+        cpt p = coset_at(c, bitrev((uint64_t)i << 1, c.log_size));
+        uint32_t v = extract_y ? p.y : p.x;
+        out[i] = v == 0 ? 0 : m31_inv(v);
+    }
+}
+
+/* ======================================================================
+ * Twisted circle NTT: cuda/circle_ntt.cu:74-117 (butterflies and layer
+ * indexing), :157-165 (scale), :248-312 (layer order).
+ * ====================================================================== */
+void or_ntt_layer(uint32_t *d, const uint32_t *tw, uint32_t li, uint64_t n, int forward) {
+    uint64_t half = n / 2, stride = 1ull << li;
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < (int64_t)half; t++) {
+        uint64_t h = (uint64_t)t >> li, l = (uint64_t)t & (stride - 1);
+        uint64_t i0 = (h << (li + 1)) + l, i1 = i0 + stride;
+        uint32_t v0 = d[i0], v1 = d[i1], w = tw[h];
+        if (forward) {
+            uint32_t tmp = m31_mul(v1, w);
+            d[i0] = m31_add(v0, tmp);
+            d[i1] = m31_sub(v0, tmp);
+        } else {
+            d[i0] = m31_add(v0, v1);
+            d[i1] = m31_mul(m31_sub(v0, v1), w);
+        }
+    }
+}
+void or_ntt_evaluate(uint32_t *d, const uint32_t *line_tw, const uint32_t *circle_tw,
+                     const uint32_t *offsets, uint32_t n_line_layers, uint64_t n) {
+    for (int l = (int)n_line_layers - 1; l >= 0; l--)
+        or_ntt_layer(d, line_tw + offsets[l], (uint32_t)l + 1, n, 1);
+    or_ntt_layer(d, circle_tw, 0, n, 1);
+}
+void or_ntt_interpolate(uint32_t *d, const uint32_t *line_itw, const uint32_t *circle_itw,
+                        const uint32_t *offsets, uint32_t n_line_layers, uint64_t n) {
+    or_ntt_layer(d, circle_itw, 0, n, 0);
+    for (uint32_t l = 0; l < n_line_layers; l++)
+        or_ntt_layer(d, line_itw + offsets[l], l + 1, n, 0);
+    uint32_t log_n = 0;
+    for (uint64_t t = n; t > 1; t >>= 1) log_n++;
+    uint32_t e = (30u * log_n) % 31u;
+    uint32_t inv_n = e == 0 ? 1u : (1u << e);
+    for (uint64_t i = 0; i < n; i++) d[i] = m31_mul(d[i], inv_n);
+}
+/* circle_ntt.cu:234-244 */
+void or_bit_reverse_m31(uint32_t *d, uint32_t log_n) {
+    uint64_t n = 1ull << log_n;
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t j = bitrev(i, log_n);
+        if (i < j) { uint32_t t = d[i]; d[i] = d[j]; d[j] = t; }
+    }
+}
+
+/* ======================================================================
+ * AIR: src/air.rs:30-45 ; constraint: cuda/constraint.cu:10-36, 81-90
+ * ====================================================================== */
+void or_fibonacci_trace(uint32_t a, uint32_t b, uint32_t log_n, uint32_t *out) {
+    uint64_t n = 1ull << log_n;
+    out[0] = a;
+    out[1] = b;
+    for (uint64_t i = 2; i < n; i++) out[i] = m31_add(out[i - 1], out[i - 2]);
+}
+void or_zero_pad(const uint32_t *src, uint32_t *dst, uint64_t src_n, uint64_t dst_n) {
+    for (uint64_t i = 0; i < dst_n; i++) dst[i] = i < src_n ? src[i] : 0;
+}
+void or_fibonacci_quotient(const uint32_t *t, uint32_t *o0, uint32_t *o1, uint32_t *o2,
+                           uint32_t *o3, const uint32_t alpha[4], uint64_t n) {
+    qm31 al = q_load(alpha);
+#pragma omp parallel for schedule(static)
+    for (int64_t si = 0; si < (int64_t)n; si++) {
+        uint64_t i = (uint64_t)si;
+        uint32_t c = m31_sub(t[(i + 2) % n], m31_add(t[(i + 1) % n], t[i]));
+        qm31 r = q_scale(al, c);
+        o0[i] = r.v[0]; o1[i] = r.v[1]; o2[i] = r.v[2]; o3[i] = r.v[3];
+    }
+}
+
+/* ======================================================================
+ * FRI folds: cuda/fri.cu:58-87 (fold_line), :92-126 (fold_circle_into_line)
+ * ====================================================================== */
+static inline qm31 fold_pair(qm31 f0, qm31 f1, qm31 alpha, uint32_t tw) {
+    return q_add(q_add(f0, f1), q_mul(alpha, q_scale(q_sub(f0, f1), tw)));
+}
+void or_fold_line(const uint32_t *const in[4], const uint32_t *tw, uint32_t *const out[4],
+                  const uint32_t alpha[4], uint64_t half_n) {
+    qm31 al = q_load(alpha);
+#pragma omp parallel for schedule(static)
+    for (int64_t si = 0; si < (int64_t)half_n; si++) {
+        uint64_t i = (uint64_t)si;
+        qm31 f0 = {{in[0][2 * i], in[1][2 * i], in[2][2 * i], in[3][2 * i]}};
+        qm31 f1 = {{in[0][2 * i + 1], in[1][2 * i + 1], in[2][2 * i + 1], in[3][2 * i + 1]}};
+        qm31 r = fold_pair(f0, f1, al, tw[i]);
+        for (int k = 0; k < 4; k++) out[k][i] = r.v[k];
+    }
+}
+void or_fold_circle_into_line(uint32_t *const dst[4], const uint32_t *const src[4],
+                              const uint32_t *tw, const uint32_t alpha[4],
+                              const uint32_t alpha_sq[4], uint64_t half_n) {
+    qm31 al = q_load(alpha), al2 = q_load(alpha_sq);
+#pragma omp parallel for schedule(static)
+    for (int64_t si = 0; si < (int64_t)half_n; si++) {
+        uint64_t i = (uint64_t)si;
+        qm31 f0 = {{src[0][2 * i], src[1][2 * i], src[2][2 * i], src[3][2 * i]}};
+        qm31 f1 = {{src[0][2 * i + 1], src[1][2 * i + 1], src[2][2 * i + 1], src[3][2 * i + 1]}};
+        qm31 prev = {{dst[0][i], dst[1][i], dst[2][i], dst[3][i]}};
+        qm31 r = q_add(q_mul(prev, al2), fold_pair(f0, f1, al, tw[i]));
+        for (int k = 0; k < 4; k++) dst[k][i] = r.v[k];
+    }
+}
+
+/* ======================================================================
+ * Merkle: cuda/blake2s.cu:13-87, src/merkle.rs:520-593, 841-884
+ * ====================================================================== */
+void or_merkle_leaf_hashes(const uint32_t *const *cols, uint32_t n_cols, uint64_t n_leaves,
+                           uint32_t *hashes) {
+    uint32_t nc = n_cols > 16 ? 16 : n_cols;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n_leaves; i++) {
+        uint32_t w[16];
+        for (uint32_t c = 0; c < nc; c++) w[c] = cols[c][i];
+        hash_words(w, nc, hashes + 8 * (uint64_t)i);
+    }
+}
+void or_merkle_hash_nodes(const uint32_t *children, uint32_t *parents, uint64_t n_parents) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n_parents; i++)
+        hash_node(children + 16 * (uint64_t)i, children + 16 * (uint64_t)i + 8, parents + 8 * (uint64_t)i);
+}
+
+/* A full tree: layers[0] = leaf hashes (n), layers[k] = n >> k nodes. */
+typedef struct {
+    uint32_t log_n;
+    uint32_t *nodes;      /* all layers concatenated, 8 words per node */
+    uint64_t off[40];     /* word offset of each layer */
+} tree_t;
+
+static void tree_build_from_leaves(tree_t *t, uint32_t log_n, uint32_t *leaf_hashes_owned) {
+    uint64_t n = 1ull << log_n;
+    t->log_n = log_n;
+    t->nodes = (uint32_t *)realloc(leaf_hashes_owned, (2 * n - 1) * 32);
+    uint64_t o = 0, cnt = n;
+    for (uint32_t k = 0; k <= log_n; k++) {
+        t->off[k] = o;
+        o += cnt * 8;
+        cnt >>= 1;
+    }
+    for (uint32_t k = 1; k <= log_n; k++)
+        or_merkle_hash_nodes(t->nodes + t->off[k - 1], t->nodes + t->off[k], n >> k);
+}
+static void tree_commit(tree_t *t, const uint32_t *const *cols, uint32_t n_cols, uint32_t log_n) {
+    uint64_t n = 1ull << log_n;
+    uint32_t *leaves = (uint32_t *)malloc(n * 32);
+    or_merkle_leaf_hashes(cols, n_cols, n, leaves);
+    tree_build_from_leaves(t, log_n, leaves);
+}
+static const uint32_t *tree_root(const tree_t *t) { return t->nodes + t->off[t->log_n]; }
+static void tree_free(tree_t *t) { free(t->nodes); t->nodes = NULL; }
+
+void or_merkle_root(const uint32_t *const *cols, uint32_t n_cols, uint32_t log_n, uint32_t root[8]) {
+    tree_t t;
+    tree_commit(&t, cols, n_cols, log_n);
+    memcpy(root, tree_root(&t), 32);
+    tree_free(&t);
+}
+
+/* src/merkle.rs:520-566 */
+int or_verify_auth_path(const uint32_t root[8], const uint32_t leaf[8], uint64_t index,
+                        const uint32_t *path, uint32_t depth) {
+    uint32_t cur[8];
+    memcpy(cur, leaf, 32);
+    for (uint32_t k = 0; k < depth; k++) {
+        const uint32_t *sib = path + 8 * (uint64_t)k;
+        if (index % 2 == 0) hash_node(cur, sib, cur);
+        else hash_node(sib, cur, cur);
+        index /= 2;
+    }
+    return memcmp(cur, root, 32) == 0;
+}
+
+/* ======================================================================
+ * Channel: src/channel.rs:15-185
+ * ====================================================================== */
+void or_channel_init(or_channel *ch) { memset(ch, 0, sizeof *ch); }
+
+/* channel.rs:30-38 */
+void or_channel_mix_digest(or_channel *ch, const uint32_t d[8]) {
+    uint8_t in[64];
+    memcpy(in, ch->state, 32);
+    for (int i = 0; i < 8; i++) put32(in + 32 + 4 * i, d[i]);
+    or_blake2s_single(in, 64, ch->state);
+    ch->n_draws = 0;
+}
+/* channel.rs:46-59 */
+void or_channel_mix_felts(or_channel *ch, const uint32_t *felts, size_t n_felts) {
+    if (n_felts == 0) return;
+    size_t len = 32 + 16 * n_felts;
+    uint8_t *in = (uint8_t *)malloc(len);
+    memcpy(in, ch->state, 32);
+    for (size_t i = 0; i < 4 * n_felts; i++) put32(in + 32 + 4 * i, felts[i]);
+    or_blake2s(in, len, ch->state);
+    free(in);
+    ch->n_draws = 0;
+}
+/* channel.rs:177-184 */
+static void ch_squeeze(or_channel *ch, uint8_t out[32]) {
+    uint8_t in[37];
+    memcpy(in, ch->state, 32);
+    put32(in + 32, ch->n_draws);
+    in[36] = 0;
+    ch->n_draws++;
+    or_blake2s_single(in, 37, out);
+}
+/* channel.rs:66-83 */
+void or_channel_draw_felt(or_channel *ch, uint32_t out[4]) {
+    for (;;) {
+        uint8_t b[32];
+        ch_squeeze(ch, b);
+        int ok = 1;
+        for (int i = 0; i < 8; i++)
+            if (le32(b + 4 * i) >= 2 * P) ok = 0;
+        if (ok) {
+            for (int i = 0; i < 4; i++) out[i] = le32(b + 4 * i) % P;
+            return;
+        }
+    }
+}
+/* channel.rs:91-95 */
+uint64_t or_channel_draw_number(or_channel *ch, uint64_t bound) {
+    uint8_t b[32];
+    ch_squeeze(ch, b);
+    uint64_t raw = (uint64_t)le32(b) | ((uint64_t)le32(b + 4) << 32);
+    return raw % bound;
+}
+/* channel.rs:120-129 */
+void or_channel_mix_u64(or_channel *ch, uint64_t v) {
+    uint8_t in[40];
+    memcpy(in, ch->state, 32);
+    put32(in + 32, (uint32_t)v);
+    put32(in + 36, (uint32_t)(v >> 32));
+    or_blake2s_single(in, 40, ch->state);
+    ch->n_draws = 0;
+}
+/* channel.rs:140-168 */
+static void pow_prefix(const or_channel *ch, uint32_t pow_bits, uint8_t prefix[32]) {
+    uint8_t in[52] = {0};
+    put32(in, 0x12345678u);
+    memcpy(in + 16, ch->state, 32);
+    put32(in + 48, pow_bits);
+    or_blake2s_single(in, 52, prefix);
+}
+static uint32_t pow_tz(const uint8_t prefix[32], uint64_t nonce) {
+    uint8_t in[40], r[32];
+    memcpy(in, prefix, 32);
+    put32(in + 32, (uint32_t)nonce);
+    put32(in + 36, (uint32_t)(nonce >> 32));
+    or_blake2s_single(in, 40, r);
+    uint64_t lo = (uint64_t)le32(r) | ((uint64_t)le32(r + 4) << 32);
+    uint64_t hi = (uint64_t)le32(r + 8) | ((uint64_t)le32(r + 12) << 32);
+    if (lo != 0) return (uint32_t)__builtin_ctzll(lo);
+    return 64 + (hi ? (uint32_t)__builtin_ctzll(hi) : 64);
+}
+int or_channel_verify_pow_nonce(const or_channel *ch, uint32_t pow_bits, uint64_t nonce) {
+    uint8_t prefix[32];
+    pow_prefix(ch, pow_bits, prefix);
+    return pow_tz(prefix, nonce) >= pow_bits;
+}
+/* Deterministic grind: the smallest nonce in 0,1,2,... that passes
+ * verify_pow_nonce (the search order the GPU grind reproduces). */
+uint64_t or_grind_pow(const or_channel *ch, uint32_t pow_bits) {
+    uint8_t prefix[32];
+    pow_prefix(ch, pow_bits, prefix);
+    for (uint64_t n = 0;; n++)
+        if (pow_tz(prefix, n) >= pow_bits) return n;
+}
+
+/* ======================================================================
+ * Proof bytes: KRAK v1 (src/bin/stark_cli.rs:1051-1110). pow_bits > 0 selects
+ * the KRAK v2 extension: header gains blowup_bits, pow_bits, nonce(lo,hi);
+ * everything after the header is unchanged.
+ * ====================================================================== */
+#define KRAK_MAGIC 0x4B52414Bu
+
+typedef struct { uint8_t *p; size_t len, cap; } wbuf;
+static void wb_u32(wbuf *w, uint32_t v) {
+    if (w->len + 4 > w->cap) {
+        w->cap = w->cap ? 2 * w->cap : 1 << 16;
+        w->p = (uint8_t *)realloc(w->p, w->cap);
+    }
+    put32(w->p + w->len, v);
+    w->len += 4;
+}
+static void wb_words(wbuf *w, const uint32_t *v, size_t n) {
+    for (size_t i = 0; i < n; i++) wb_u32(w, v[i]);
+}
+/* auth path of leaf `idx` from a full tree (src/prover.rs:2262-2273) */
+static void wb_path(wbuf *w, const tree_t *t, uint64_t idx) {
+    wb_u32(w, t->log_n);
+    for (uint32_t k = 0; k < t->log_n; k++) {
+        wb_words(w, t->nodes + t->off[k] + 8 * (idx ^ 1), 8);
+        idx >>= 1;
+    }
+}
+/* write_decommitment_{u32,4} (stark_cli.rs:1061-1089) for data held as
+ * `width` SoA columns. */
+static void wb_decommit(wbuf *w, const tree_t *t, const uint32_t *const *cols, uint32_t width,
+                        const uint64_t *idx, uint32_t nq) {
+    wb_u32(w, nq);
+    for (uint32_t q = 0; q < nq; q++)
+        for (uint32_t c = 0; c < width; c++) wb_u32(w, cols[c][idx[q]]);
+    for (uint32_t q = 0; q < nq; q++)
+        for (uint32_t c = 0; c < width; c++) wb_u32(w, cols[c][idx[q] ^ 1]);
+    for (uint32_t q = 0; q < nq; q++) wb_path(w, t, idx[q]);
+    for (uint32_t q = 0; q < nq; q++) wb_path(w, t, idx[q] ^ 1);
+}
+
+size_t or_proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits) {
+    uint32_t le = log_n + blowup_bits;
+    uint32_t n_fri = le - 3;
+    size_t words = (pow_bits ? 9 : 5) + 16 + 1 + 8 * (size_t)n_fri + 1 + 32 + 1 + nq;
+    /* one decommitment: n, values, siblings, 2*nq paths of (1 + 8*depth) */
+    words += 1 + 2 * (size_t)nq + 2 * (size_t)nq * (1 + 8 * (size_t)le);           /* trace */
+    words += 1 + 8 * (size_t)nq + 2 * (size_t)nq * (1 + 8 * (size_t)le);           /* quotient */
+    words += 1;
+    for (uint32_t k = 0; k < n_fri; k++) {
+        uint32_t L = le - 1 - k;
+        words += 1 + 8 * (size_t)nq + 2 * (size_t)nq * (1 + 8 * (size_t)L);
+    }
+    return 4 * words;
+}
+
+/* src/prover.rs:264-619 (prove_lean_inner); prove_lean_fused and
+ * prove_with_cache produce the same bytes. */
+int or_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint32_t nq,
+             uint32_t pow_bits, uint8_t **out, size_t *out_len) {
+    if (log_n < 4 || blowup_bits < 1 || log_n + blowup_bits > 31 || a >= P || b >= P || nq == 0)
+        return -1;
+    if (log_n + blowup_bits > 31) return -1;
+    uint32_t le = log_n + blowup_bits;
+    uint64_t n = 1ull << log_n, N = 1ull << le;
+
+    /* trace + interpolate on half_coset(log_n) (prover.rs:280-300) */
+    uint32_t *coef = (uint32_t *)malloc(n * 4);
+    or_fibonacci_trace(a, b, log_n, coef);
+    {
+        uint32_t c5[5];
+        coset_pack(coset_make(OR_HALF_COSET, log_n), c5);
+        uint32_t *ltw = (uint32_t *)malloc(n * 4), *ctw = (uint32_t *)malloc(n / 2 * 4);
+        uint32_t offs[32], sizes[32];
+        or_compute_twiddles(c5, ltw, ctw, offs, sizes);
+        or_batch_inverse(ltw, ltw, n - 2);
+        or_batch_inverse(ctw, ctw, n / 2);
+        or_ntt_interpolate(coef, ltw, ctw, offs, log_n - 1, n);
+        free(ltw);
+        free(ctw);
+    }
+    /* zero-pad + evaluate on the eval domain (prover.rs:303-318, 655-659) */
+    uint32_t *ev = (uint32_t *)malloc(N * 4);
+    or_zero_pad(coef, ev, n, N);
+    free(coef);
+    coset_t edom = eval_domain(le);
+    {
+        uint32_t c5[5];
+        coset_pack(edom, c5);
+        uint32_t *ltw = (uint32_t *)malloc(N * 4), *ctw = (uint32_t *)malloc(N / 2 * 4);
+        uint32_t offs[32], sizes[32];
+        or_compute_twiddles(c5, ltw, ctw, offs, sizes);
+        or_ntt_evaluate(ev, ltw, ctw, offs, le - 1, N);
+        free(ltw);
+        free(ctw);
+    }
+    /* trace commitment (prover.rs:321-324) */
+    tree_t ttree;
+    const uint32_t *evc[1] = {ev};
+    tree_commit(&ttree, evc, 1, le);
+
+    /* quotient (prover.rs:330-352) */
+    or_channel ch;
+    or_channel_init(&ch);
+    or_channel_mix_digest(&ch, tree_root(&ttree));
+    uint32_t alpha[4];
+    or_channel_draw_felt(&ch, alpha);
+    uint32_t *q[4];
+    for (int k = 0; k < 4; k++) q[k] = (uint32_t *)malloc(N * 4);
+    or_fibonacci_quotient(ev, q[0], q[1], q[2], q[3], alpha, N);
+    tree_t qtree;
+    tree_commit(&qtree, (const uint32_t *const *)q, 4, le);
+    or_channel_mix_digest(&ch, tree_root(&qtree));
+
+    /* FRI (prover.rs:380-548) */
+    uint32_t n_fri = le - 3;
+    uint32_t *(*fl)[4] = (uint32_t * (*)[4]) calloc(n_fri, sizeof *fl);
+    tree_t *ftree = (tree_t *)calloc(n_fri, sizeof(tree_t));
+    uint32_t *tw = (uint32_t *)malloc(N / 2 * 4);
+    {
+        uint32_t fa[4], fa2[4], c5[5];
+        or_channel_draw_felt(&ch, fa);
+        or_qm31_mul(fa, fa, fa2);
+        coset_pack(edom, c5);
+        or_fold_twiddles(c5, 1, tw);
+        for (int k = 0; k < 4; k++) fl[0][k] = (uint32_t *)calloc(N / 2, 4);
+        or_fold_circle_into_line(fl[0], (const uint32_t *const *)q, tw, fa, fa2, N / 2);
+        tree_commit(&ftree[0], (const uint32_t *const *)fl[0], 4, le - 1);
+        or_channel_mix_digest(&ch, tree_root(&ftree[0]));
+    }
+    uint32_t cur = le - 1;
+    for (uint32_t k = 1; k < n_fri; k++) {
+        uint32_t fa[4], c5[5];
+        or_channel_draw_felt(&ch, fa);
+        coset_pack(coset_make(OR_HALF_ODDS, cur), c5);
+        or_fold_twiddles(c5, 0, tw);
+        uint64_t half = 1ull << (cur - 1);
+        for (int c = 0; c < 4; c++) fl[k][c] = (uint32_t *)malloc(half * 4);
+        or_fold_line((const uint32_t *const *)fl[k - 1], tw, fl[k], fa, half);
+        cur--;
+        tree_commit(&ftree[k], (const uint32_t *const *)fl[k], 4, cur);
+        or_channel_mix_digest(&ch, tree_root(&ftree[k]));
+    }
+    free(tw);
+
+    /* queries (prover.rs:554-557) + optional PoW before the draws */
+    uint32_t last[32];
+    for (int i = 0; i < 8; i++)
+        for (int c = 0; c < 4; c++) last[4 * i + c] = fl[n_fri - 1][c][i];
+    or_channel_mix_felts(&ch, last, 8);
+    uint64_t nonce = 0;
+    if (pow_bits) {
+        nonce = or_grind_pow(&ch, pow_bits);
+        or_channel_mix_u64(&ch, nonce);
+    }
+    uint64_t *qi = (uint64_t *)malloc(nq * 8), *fi = (uint64_t *)malloc(nq * 8);
+    for (uint32_t i = 0; i < nq; i++) qi[i] = or_channel_draw_number(&ch, N);
+
+    /* serialize (stark_cli.rs:1092-1120) */
+    wbuf w = {0};
+    wb_u32(&w, KRAK_MAGIC);
+    wb_u32(&w, pow_bits ? 2 : 1);
+    wb_u32(&w, log_n);
+    wb_u32(&w, a);
+    wb_u32(&w, b);
+    if (pow_bits) {
+        wb_u32(&w, blowup_bits);
+        wb_u32(&w, pow_bits);
+        wb_u32(&w, (uint32_t)nonce);
+        wb_u32(&w, (uint32_t)(nonce >> 32));
+    }
+    wb_words(&w, tree_root(&ttree), 8);
+    wb_words(&w, tree_root(&qtree), 8);
+    wb_u32(&w, n_fri);
+    for (uint32_t k = 0; k < n_fri; k++) wb_words(&w, tree_root(&ftree[k]), 8);
+    wb_u32(&w, 8);
+    wb_words(&w, last, 32);
+    wb_u32(&w, nq);
+    for (uint32_t i = 0; i < nq; i++) wb_u32(&w, (uint32_t)qi[i]);
+    wb_decommit(&w, &ttree, evc, 1, qi, nq);
+    wb_decommit(&w, &qtree, (const uint32_t *const *)q, 4, qi, nq);
+    wb_u32(&w, n_fri);
+    for (uint32_t i = 0; i < nq; i++) fi[i] = qi[i] / 2;
+    for (uint32_t k = 0; k < n_fri; k++) {
+        wb_decommit(&w, &ftree[k], (const uint32_t *const *)fl[k], 4, fi, nq);
+        for (uint32_t i = 0; i < nq; i++) fi[i] /= 2;
+    }
+
+    tree_free(&ttree);
+    tree_free(&qtree);
+    for (uint32_t k = 0; k < n_fri; k++) {
+        tree_free(&ftree[k]);
+        for (int c = 0; c < 4; c++) free(fl[k][c]);
+    }
+    free(ftree);
+    free(fl);
+    for (int k = 0; k < 4; k++) free(q[k]);
+    free(ev);
+    free(qi);
+    free(fi);
+    *out = w.p;
+    *out_len = w.len;
+    return 0;
+}
+
+void or_free(void *p) { free(p); }
+
+/* ======================================================================
+ * Verifier: src/verifier.rs:39-310 over KRAK bytes (stark_cli.rs:1189-1230)
+ * ====================================================================== */
+typedef struct { const uint8_t *p; size_t len, pos; int bad; } rbuf;
+static uint32_t rb_u32(rbuf *r) {
+    if (r->pos + 4 > r->len) { r->bad = 1; return 0; }
+    uint32_t v = le32(r->p + r->pos);
+    r->pos += 4;
+    return v;
+}
+typedef struct {
+    uint32_t n, width;
+    uint32_t *vals, *sibs;        /* n * width */
+    uint32_t *depth, *sdepth;     /* n */
+    uint32_t **paths, **spaths;   /* n pointers to depth*8 words */
+} decom_t;
+static void decom_free(decom_t *d) {
+    if (d->paths)
+        for (uint32_t i = 0; i < d->n; i++) { free(d->paths[i]); free(d->spaths[i]); }
+    free(d->vals); free(d->sibs); free(d->depth); free(d->sdepth); free(d->paths); free(d->spaths);
+    memset(d, 0, sizeof *d);
+}
+static int rb_decom(rbuf *r, decom_t *d, uint32_t width) {
+    memset(d, 0, sizeof *d);
+    d->width = width;
+    d->n = rb_u32(r);
+    if (r->bad || d->n > (1u << 20) || (size_t)d->n * width * 8 > r->len) return 0;
+    uint32_t n = d->n;
+    d->vals = (uint32_t *)calloc((size_t)n * width + 1, 4);
+    d->sibs = (uint32_t *)calloc((size_t)n * width + 1, 4);
+    d->depth = (uint32_t *)calloc(n + 1, 4);
+    d->sdepth = (uint32_t *)calloc(n + 1, 4);
+    d->paths = (uint32_t **)calloc(n + 1, sizeof(uint32_t *));
+    d->spaths = (uint32_t **)calloc(n + 1, sizeof(uint32_t *));
+    for (size_t i = 0; i < (size_t)n * width; i++) d->vals[i] = rb_u32(r);
+    for (size_t i = 0; i < (size_t)n * width; i++) d->sibs[i] = rb_u32(r);
+    for (int s = 0; s < 2; s++)
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t dep = rb_u32(r);
+            if (r->bad || dep > 64) return 0;
+            uint32_t *pth = (uint32_t *)calloc((size_t)dep * 8 + 1, 4);
+            for (uint32_t k = 0; k < 8 * dep; k++) pth[k] = rb_u32(r);
+            if (s == 0) { d->depth[i] = dep; d->paths[i] = pth; }
+            else { d->sdepth[i] = dep; d->spaths[i] = pth; }
+        }
+    return !r->bad;
+}
+/* verifier.rs:274-310 */
+static int check_decom(const decom_t *d, const uint32_t root[8], const uint64_t *idx, uint32_t nq) {
+    if (d->n != nq) return 0;
+    for (uint32_t q = 0; q < nq; q++) {
+        uint32_t h[8];
+        hash_words(d->vals + (size_t)q * d->width, d->width, h);
+        if (!or_verify_auth_path(root, h, idx[q], d->paths[q], d->depth[q])) return 0;
+        hash_words(d->sibs + (size_t)q * d->width, d->width, h);
+        if (!or_verify_auth_path(root, h, idx[q] ^ 1, d->spaths[q], d->sdepth[q])) return 0;
+    }
+    return 1;
+}
+/* verifier.rs:30-36 */
+static uint32_t fold_twiddle_at(coset_t dom, uint64_t folded, int circle) {
+    cpt p = coset_at(dom, bitrev(folded << 1, dom.log_size));
+    return m31_inv(circle ? p.y : p.x);
+}
+static qm31 pair_fold(const decom_t *d, uint32_t q, uint64_t idx, qm31 alpha, uint32_t tw) {
+    qm31 v = q_load(d->vals + 4 * (size_t)q), s = q_load(d->sibs + 4 * (size_t)q);
+    return idx % 2 == 0 ? fold_pair(v, s, alpha, tw) : fold_pair(s, v, alpha, tw);
+}
+
+#define VFAIL(...) do { snprintf(err, err_cap, __VA_ARGS__); goto done; } while (0)
+
+int or_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits, uint32_t nq,
+              char *err, size_t err_cap) {
+    int ok = 0;
+    rbuf r = {bytes, len, 0, 0};
+    decom_t td = {0}, qd = {0}, *fd = NULL;
+    uint32_t n_fd = 0, n_fri = 0, n_last = 0, n_q = 0;
+    uint32_t (*froots)[8] = NULL, *lastw = NULL;
+    uint64_t *qidx = NULL, *fidx = NULL;
+    qm31 *alphas = NULL;
+    char dummy[8];
+    if (!err || err_cap == 0) { err = dummy; err_cap = sizeof dummy; }
+    err[0] = 0;
+
+    if (rb_u32(&r) != KRAK_MAGIC) VFAIL("bad magic");
+    uint32_t version = rb_u32(&r);
+    if (version != 1 && version != 2) VFAIL("unsupported proof version %u", version);
+    uint32_t log_n = rb_u32(&r), a = rb_u32(&r), b = rb_u32(&r);
+    uint32_t pow_bits = 0;
+    uint64_t nonce = 0;
+    if (version == 2) {
+        uint32_t bb = rb_u32(&r);
+        pow_bits = rb_u32(&r);
+        nonce = rb_u32(&r);
+        nonce |= (uint64_t)rb_u32(&r) << 32;
+        if (bb != blowup_bits) VFAIL("blowup mismatch");
+        if (pow_bits > 128) VFAIL("bad pow bits");
+    }
+    uint32_t troot[8], qroot[8];
+    for (int i = 0; i < 8; i++) troot[i] = rb_u32(&r);
+    for (int i = 0; i < 8; i++) qroot[i] = rb_u32(&r);
+    n_fri = rb_u32(&r);
+    if (r.bad || n_fri > 64) VFAIL("truncated proof");
+    froots = (uint32_t(*)[8])calloc(n_fri + 1, 32);
+    for (uint32_t k = 0; k < n_fri; k++)
+        for (int i = 0; i < 8; i++) froots[k][i] = rb_u32(&r);
+    n_last = rb_u32(&r);
+    if (r.bad || n_last > 4096) VFAIL("truncated proof");
+    lastw = (uint32_t *)calloc(4 * (size_t)n_last + 1, 4);
+    for (size_t i = 0; i < 4 * (size_t)n_last; i++) lastw[i] = rb_u32(&r);
+    n_q = rb_u32(&r);
+    if (r.bad || n_q > (1u << 20)) VFAIL("truncated proof");
+    qidx = (uint64_t *)calloc(n_q + 1, 8);
+    for (uint32_t i = 0; i < n_q; i++) qidx[i] = rb_u32(&r);
+    if (!rb_decom(&r, &td, 1)) VFAIL("truncated proof");
+    if (!rb_decom(&r, &qd, 4)) VFAIL("truncated proof");
+    n_fd = rb_u32(&r);
+    if (r.bad || n_fd > 64) VFAIL("truncated proof");
+    fd = (decom_t *)calloc(n_fd + 1, sizeof(decom_t));
+    for (uint32_t k = 0; k < n_fd; k++)
+        if (!rb_decom(&r, &fd[k], 4)) VFAIL("truncated proof");
+
+    uint32_t le = log_n + blowup_bits;
+    if (log_n < 1 || le > 31) VFAIL("bad log size");
+    uint64_t N = 1ull << le;
+    /* verifier.rs:46-53 */
+    if (a >= P || b >= P) VFAIL("Public inputs out of M31 range");
+    if (n_q != nq) VFAIL("Expected %u queries, got %u", nq, n_q);
+    /* verifier.rs:56-70 */
+    or_channel ch;
+    or_channel_init(&ch);
+    or_channel_mix_digest(&ch, troot);
+    uint32_t tmp[4];
+    or_channel_draw_felt(&ch, tmp);
+    or_channel_mix_digest(&ch, qroot);
+    alphas = (qm31 *)calloc(n_fri + 2, sizeof(qm31));
+    or_channel_draw_felt(&ch, alphas[0].v);
+    for (uint32_t k = 0; k < n_fri; k++) {
+        or_channel_mix_digest(&ch, froots[k]);
+        or_channel_draw_felt(&ch, alphas[k + 1].v);
+    }
+    /* verifier.rs:73-87 */
+    if (n_fri != (le >= 3 ? le - 3 : 0)) VFAIL("Expected %u FRI layers, got %u", le - 3, n_fri);
+    if (n_last != 8) VFAIL("Expected 8 FRI last layer values, got %u", n_last);
+    /* verifier.rs:90-96 (+ KRAK v2 PoW check before the query draws) */
+    or_channel_mix_felts(&ch, lastw, n_last);
+    if (version == 2) {
+        if (!or_channel_verify_pow_nonce(&ch, pow_bits, nonce)) VFAIL("PoW nonce invalid");
+        or_channel_mix_u64(&ch, nonce);
+    }
+    for (uint32_t i = 0; i < nq; i++)
+        if (or_channel_draw_number(&ch, N) != qidx[i])
+            VFAIL("Query indices don't match Fiat-Shamir derivation");
+    /* verifier.rs:99-104 */
+    {
+        int tz = 1, qz = 1;
+        for (int i = 0; i < 8; i++) { tz &= troot[i] == 0; qz &= qroot[i] == 0; }
+        if (tz) VFAIL("Trace commitment is zero");
+        if (qz) VFAIL("Quotient commitment is zero");
+    }
+    /* verifier.rs:107-144 */
+    if (!check_decom(&td, troot, qidx, nq)) VFAIL("trace auth path invalid");
+    if (!check_decom(&qd, qroot, qidx, nq)) VFAIL("quotient auth path invalid");
+    fidx = (uint64_t *)calloc(nq + 1, 8);
+    for (uint32_t i = 0; i < nq; i++) fidx[i] = qidx[i] / 2;
+    for (uint32_t k = 0; k < n_fd && k < n_fri; k++) {
+        if (!check_decom(&fd[k], froots[k], fidx, nq)) VFAIL("FRI layer %u auth path invalid", k);
+        for (uint32_t i = 0; i < nq; i++) fidx[i] /= 2;
+    }
+    /* verifier.rs:153-233: fold equations */
+    if (n_fd == 0) VFAIL("missing FRI decommitments");
+    for (uint32_t k = 0; k < n_fd; k++)
+        if (fd[k].n != nq) VFAIL("FRI layer %u decommitment size mismatch", k);
+    for (uint32_t q = 0; q < nq; q++) {
+        uint64_t ci = qidx[q];
+        uint32_t cl = le;
+        coset_t dom = eval_domain(cl);
+        qm31 f0 = q_load(qd.vals + 4 * (size_t)q), f1 = q_load(qd.sibs + 4 * (size_t)q);
+        if (ci % 2) { qm31 t = f0; f0 = f1; f1 = t; }
+        qm31 e = fold_pair(f0, f1, alphas[0], fold_twiddle_at(dom, ci / 2, 1));
+        if (memcmp(e.v, fd[0].vals + 4 * (size_t)q, 16)) VFAIL("Circle fold mismatch at query %u", q);
+        ci /= 2;
+        cl--;
+        for (uint32_t k = 0; k + 1 < n_fd; k++) {
+            coset_t ld = coset_make(OR_HALF_ODDS, cl);
+            qm31 ex = pair_fold(&fd[k], q, ci, alphas[k + 1], fold_twiddle_at(ld, ci / 2, 0));
+            if (memcmp(ex.v, fd[k + 1].vals + 4 * (size_t)q, 16))
+                VFAIL("Line fold mismatch at query %u, layer %u", q, k);
+            ci /= 2;
+            cl--;
+        }
+        if (ci >= n_last) VFAIL("FRI last layer index out of bounds");
+        if (memcmp(fd[n_fd - 1].vals + 4 * (size_t)q, lastw + 4 * ci, 16))
+            VFAIL("FRI last layer mismatch at query %u", q);
+    }
+    ok = 1;
+done:
+    decom_free(&td);
+    decom_free(&qd);
+    if (fd)
+        for (uint32_t k = 0; k < n_fd; k++) decom_free(&fd[k]);
+    free(fd);
+    free(froots);
+    free(lastw);
+    free(qidx);
+    free(fidx);
+    free(alphas);
+    return ok;
+}

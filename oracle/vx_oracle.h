@@ -1,0 +1,125 @@
+/*
+ * vx_oracle.h -- CPU restatement of the VortexSTARK Fibonacci Circle-STARK
+ * prove path, used ONLY as the correctness oracle and CPU baseline.
+ *
+ * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker (or the
+ * timed CPU baseline). The product (vortexstark_amd) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates. All arithmetic
+ * is exact over GF(2^31-1), so results are bit-exact by construction.
+ *
+ * Pinning status (what checks each function; see tests/test_oracle_*.py):
+ *   field / circle        reference literal KATs, Python big-int identities
+ *   blake2s / channel     Python hashlib.blake2s KATs (standard RFC 7693)
+ *   twiddles / NTT        independent Python big-int restatement fixtures,
+ *                         closed form at n=4, roundtrip, layer equivalence,
+ *                         LDE replication property
+ *   quotient / folds      closed form per element (Python big-int)
+ *   merkle                Python hashlib tree, auth-path verification
+ *   prove / verify        independent pure-Python prover byte-equality
+ *                         (log_n 4..6), verifier accept + tamper-reject
+ *   pow                   brute-force smallest nonce via hashlib
+ */
+#ifndef VX_ORACLE_H
+#define VX_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_P 0x7FFFFFFFu
+
+/* ---- field (src/field/m31.rs, cm31.rs, qm31.rs) ---- */
+uint32_t or_m31_add(uint32_t a, uint32_t b);
+uint32_t or_m31_sub(uint32_t a, uint32_t b);
+uint32_t or_m31_mul(uint32_t a, uint32_t b);
+uint32_t or_m31_inv(uint32_t a);
+void or_cm31_mul(const uint32_t a[2], const uint32_t b[2], uint32_t out[2]);
+void or_qm31_mul(const uint32_t a[4], const uint32_t b[4], uint32_t out[4]);
+void or_qm31_inv(const uint32_t a[4], uint32_t out[4]);
+
+/* ---- circle (src/circle.rs) ----
+ * A coset is passed as uint32_t[5] = {init.x, init.y, step.x, step.y, log_size}. */
+enum { OR_SUBGROUP = 0, OR_HALF_COSET = 1, OR_HALF_ODDS = 2 };
+void or_coset(int kind, uint32_t log_size, uint32_t coset[5]);
+void or_eval_domain(uint32_t log_size, uint32_t coset[5]);
+void or_coset_at(const uint32_t coset[5], uint64_t i, uint32_t out[2]);
+void or_circle_mul(const uint32_t a[2], const uint32_t b[2], uint32_t out[2]);
+
+/* ---- blake2s (src/channel.rs:216-298; standard RFC 7693 Blake2s-256) ---- */
+void or_blake2s(const uint8_t *in, size_t len, uint8_t out[32]);
+void or_blake2s_single(const uint8_t *in, size_t len, uint8_t out[32]);
+
+/* ---- twiddles (src/circle.rs:284-347, cuda/twiddles.cu:80-142) ---- */
+void or_compute_twiddles(const uint32_t coset[5], uint32_t *line_tw,
+                         uint32_t *circle_tw, uint32_t *offsets, uint32_t *sizes);
+void or_batch_inverse(const uint32_t *in, uint32_t *out, size_t n);
+void or_fold_twiddles(const uint32_t coset[5], int extract_y, uint32_t *out);
+
+/* ---- NTT (cuda/circle_ntt.cu:74-117,157-165,248-312) ---- */
+void or_ntt_layer(uint32_t *data, const uint32_t *tw, uint32_t layer_idx,
+                  uint64_t n, int forward);
+void or_ntt_evaluate(uint32_t *data, const uint32_t *line_tw,
+                     const uint32_t *circle_tw, const uint32_t *offsets,
+                     uint32_t n_line_layers, uint64_t n);
+void or_ntt_interpolate(uint32_t *data, const uint32_t *line_itw,
+                        const uint32_t *circle_itw, const uint32_t *offsets,
+                        uint32_t n_line_layers, uint64_t n);
+void or_bit_reverse_m31(uint32_t *data, uint32_t log_n);
+
+/* ---- AIR + constraint (src/air.rs:16-45, cuda/constraint.cu:10-36,81-90) ---- */
+void or_fibonacci_trace(uint32_t a, uint32_t b, uint32_t log_n, uint32_t *out);
+void or_zero_pad(const uint32_t *src, uint32_t *dst, uint64_t src_n, uint64_t dst_n);
+void or_fibonacci_quotient(const uint32_t *trace, uint32_t *o0, uint32_t *o1,
+                           uint32_t *o2, uint32_t *o3, const uint32_t alpha[4],
+                           uint64_t n);
+
+/* ---- FRI (cuda/fri.cu:58-126) ---- */
+void or_fold_line(const uint32_t *const in[4], const uint32_t *tw,
+                  uint32_t *const out[4], const uint32_t alpha[4], uint64_t half_n);
+void or_fold_circle_into_line(uint32_t *const dst[4], const uint32_t *const src[4],
+                              const uint32_t *tw, const uint32_t alpha[4],
+                              const uint32_t alpha_sq[4], uint64_t half_n);
+
+/* ---- Merkle (cuda/blake2s.cu:13-87, src/merkle.rs:520-593,841-884) ---- */
+void or_merkle_leaf_hashes(const uint32_t *const *cols, uint32_t n_cols,
+                           uint64_t n_leaves, uint32_t *hashes);
+void or_merkle_hash_nodes(const uint32_t *children, uint32_t *parents,
+                          uint64_t n_parents);
+void or_merkle_root(const uint32_t *const *cols, uint32_t n_cols,
+                    uint32_t log_n_leaves, uint32_t root[8]);
+int or_verify_auth_path(const uint32_t root[8], const uint32_t leaf_hash[8],
+                        uint64_t index, const uint32_t *path, uint32_t depth);
+
+/* ---- channel (src/channel.rs:15-185) ---- */
+typedef struct {
+    uint8_t state[32];
+    uint32_t n_draws;
+} or_channel;
+void or_channel_init(or_channel *ch);
+void or_channel_mix_digest(or_channel *ch, const uint32_t digest[8]);
+void or_channel_mix_felts(or_channel *ch, const uint32_t *felts, size_t n_felts);
+void or_channel_draw_felt(or_channel *ch, uint32_t out[4]);
+uint64_t or_channel_draw_number(or_channel *ch, uint64_t bound);
+void or_channel_mix_u64(or_channel *ch, uint64_t value);
+int or_channel_verify_pow_nonce(const or_channel *ch, uint32_t pow_bits, uint64_t nonce);
+uint64_t or_grind_pow(const or_channel *ch, uint32_t pow_bits);
+
+/* ---- prove / verify (src/prover.rs:264-619, src/verifier.rs:39-310,
+ *      src/bin/stark_cli.rs:1051-1110) ---- */
+int or_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits,
+             uint32_t n_queries, uint32_t pow_bits, uint8_t **out, size_t *out_len);
+int or_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits,
+              uint32_t n_queries, char *err, size_t err_cap);
+size_t or_proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries,
+                     uint32_t pow_bits);
+void or_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

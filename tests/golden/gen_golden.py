@@ -1,0 +1,146 @@
+"""Writes tests/golden/golden.json: known-answer vectors for the prove path.
+
+Sources, per section:
+  blake2s      Python hashlib.blake2s (standard Blake2s-256)
+  twiddles,    tests/golden/pyref.py, an independent big-int restatement that
+  ntt, folds   derives twiddles from closed-form point powers
+  proofs_py    pyref.prove (independent of the C oracle), SHA-256 of the bytes
+  proofs_c     the C oracle (oracle/liboracle_omp.so) at sizes pyref cannot
+               reach; pinned transitively by proofs_py agreement at small sizes
+  pow          brute-force smallest nonce with hashlib
+
+Run from the repo root:  python tests/golden/gen_golden.py [--big]
+(--big adds the log_n=20 and log_n=24 oracle digests, which take minutes.)
+"""
+import hashlib
+import json
+import os
+import random
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+
+import pyref  # noqa: E402
+
+P = pyref.P
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed):
+    s = seed & MASK64
+    while True:
+        s = (s + 0x9E3779B97F4A7C15) & MASK64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        yield z ^ (z >> 31)
+
+
+def fib_a1(seed):
+    """a1 of the seeded trace recipe: r = next() >> 33, rejected while r >= P."""
+    for z in splitmix64(seed):
+        r = z >> 33
+        if r < P:
+            return r
+
+
+def main():
+    big = "--big" in sys.argv
+    g = {}
+    rng = random.Random(20261202)
+
+    g["blake2s"] = []
+    for n in [0, 1, 4, 16, 37, 40, 52, 63, 64, 65, 127, 128, 129, 160, 300]:
+        data = bytes(rng.randrange(256) for _ in range(n))
+        g["blake2s"].append({"in": data.hex(), "out": hashlib.blake2s(data).hexdigest()})
+
+    g["twiddles"] = []
+    for L in [2, 3, 4, 5]:
+        for name, c in [("half_coset", pyref.half_coset(L)), ("half_odds", pyref.half_odds(L)),
+                        ("subgroup", pyref.subgroup(L))]:
+            line, circle = pyref.twiddle_tables(c)
+            g["twiddles"].append({"kind": name, "log": L, "line": line, "circle": circle})
+
+    g["ntt_basis"] = []
+    for L in [2, 3, 4]:
+        c = pyref.half_coset(L)
+        n = 1 << L
+        ev = [pyref.evaluate([int(i == k) for i in range(n)], c) for k in range(n)]
+        g["ntt_basis"].append({"log": L, "evaluate_unit": ev})
+
+    g["ntt_random"] = []
+    for L in [5, 6, 8]:
+        n = 1 << L
+        v = [rng.randrange(P) for _ in range(n)]
+        c = pyref.half_coset(L)
+        g["ntt_random"].append({"log": L, "in": v, "evaluate": pyref.evaluate(v, c),
+                                "interpolate": pyref.interpolate(v, c)})
+
+    g["fold_twiddles"] = []
+    for L in [3, 4, 5, 6]:
+        for name, c, use_y in [("half_coset_y", pyref.half_coset(L), True),
+                               ("half_odds_x", pyref.half_odds(L), False)]:
+            g["fold_twiddles"].append({"kind": name, "log": L,
+                                       "tw": [pyref.fold_tw(c, i, use_y) for i in range(1 << (L - 1))]})
+
+    g["channel"] = []
+    ch = pyref.Channel()
+    ops = []
+    digest = [rng.randrange(1 << 32) for _ in range(8)]
+    ch.mix_digest(digest)
+    ops.append({"op": "mix_digest", "digest": digest, "state": ch.state.hex()})
+    for _ in range(3):
+        ops.append({"op": "draw_felt", "out": list(ch.draw_felt())})
+    felts = [tuple(rng.randrange(P) for _ in range(4)) for _ in range(8)]
+    ch.mix_felts(felts)
+    ops.append({"op": "mix_felts", "felts": [list(f) for f in felts], "state": ch.state.hex()})
+    ops.append({"op": "draw_number", "bound": 1 << 26, "out": ch.draw_number(1 << 26)})
+    ch.mix_u64(0x123456789ABCDEF)
+    ops.append({"op": "mix_u64", "v": 0x123456789ABCDEF, "state": ch.state.hex()})
+    g["channel"] = ops
+
+    g["pow"] = []
+    for bits in [4, 8, 12]:
+        c2 = pyref.Channel()
+        c2.mix_digest([bits] * 8)
+        nonce = 0
+        while not c2.pow_ok(bits, nonce):
+            nonce += 1
+        g["pow"].append({"digest": [bits] * 8, "bits": bits, "nonce": nonce})
+
+    g["proofs_py"] = []
+    for a, b, log_n, bb, pw in [(1, 1, 4, 2, 0), (1, 1, 5, 2, 0), (1, 1, 6, 2, 0), (2, 3, 6, 2, 0),
+                                (42, 99, 6, 2, 0), (1000, 2000, 6, 2, 0), (1, 1, 8, 2, 0),
+                                (1, 1, 6, 1, 0), (1, 1, 5, 3, 0), (5, 8, 5, 2, 6)]:
+        data = pyref.prove(a, b, log_n, blowup_bits=bb, pow_bits=pw)
+        g["proofs_py"].append({"a": a, "b": b, "log_n": log_n, "blowup_bits": bb, "pow_bits": pw,
+                               "len": len(data), "sha256": hashlib.sha256(data).hexdigest()})
+
+    import pyoracle
+    seed = 42
+    a1 = fib_a1(seed)
+    g["workload_seed"] = {"seed": seed, "a0": 1, "a1": a1}
+    sizes = [(10, 2, 0), (12, 2, 0), (16, 2, 0), (16, 2, 10), (14, 1, 0)]
+    if big:
+        sizes += [(20, 2, 0), (24, 2, 0)]
+    g["proofs_c"] = []
+    for log_n, bb, pw in sizes:
+        data = pyoracle.prove(1, a1, log_n, blowup_bits=bb, pow_bits=pw, omp=True)
+        g["proofs_c"].append({"a": 1, "b": a1, "log_n": log_n, "blowup_bits": bb, "pow_bits": pw,
+                              "len": len(data), "sha256": hashlib.sha256(data).hexdigest()})
+        print("oracle digest", log_n, bb, pw, len(data), flush=True)
+
+    out = os.path.join(HERE, "golden.json")
+    if not big and os.path.exists(out):
+        old = json.load(open(out))
+        keep = [p for p in old.get("proofs_c", []) if p["log_n"] >= 20]
+        g["proofs_c"] += keep
+    with open(out, "w") as f:
+        json.dump(g, f, separators=(",", ":"))
+    print("wrote", out, os.path.getsize(out), "bytes")
+
+
+if __name__ == "__main__":
+    main()

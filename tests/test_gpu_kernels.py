@@ -1,0 +1,300 @@
+"""Per-kernel parity of the HIP path against the CPU oracle, through the C-ABI.
+
+Inputs are seeded; every output must be bit-identical to the oracle's.
+Sizes run from the smallest shapes the reference allows up to 2^22-2^24 (the
+oracle finishes these in seconds)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import vortexstark_amd as vx
+from gpu_util import d2h, dev, empty, host, ptr, sync
+from vortexstark_amd._lib import call
+
+pytestmark = pytest.mark.gpu
+P = po.P
+NULL = None
+
+
+def rnd(n, seed):
+    return np.random.default_rng(seed).integers(0, P, int(n), dtype=np.uint32)
+
+
+def hp(a):
+    """host pointer of a numpy array"""
+    return C.c_void_p(a.ctypes.data)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib_loaded():
+    vx.load()
+    print("gpu kernel parity tests: library loaded", flush=True)
+
+
+@pytest.mark.parametrize("log_n,a,b", [(4, 1, 1), (5, 3, 7), (10, 123, 456), (16, 1, 2 ** 30), (22, P - 1, P - 2)])
+def test_fibonacci_trace(log_n, a, b):
+    out = empty(1 << log_n)
+    call("vx_fibonacci_trace", a, b, ptr(out), log_n, NULL)
+    assert np.array_equal(host(out), po.fibonacci_trace(a, b, log_n))
+
+
+@pytest.mark.parametrize("kind,log_n", [(po.HALF_COSET, 3), (po.HALF_COSET, 11), (po.HALF_ODDS, 10),
+                                        (po.SUBGROUP, 12), (po.HALF_COSET, 20)])
+def test_coset_points_and_squash(kind, log_n):
+    c = po.coset(kind, log_n)
+    n = 1 << log_n
+    xs, ys = empty(n), empty(n)
+    call("vx_compute_coset_points", int(c[0]), int(c[1]), int(c[2]), int(c[3]), ptr(xs), ptr(ys), n, NULL)
+    hx, hy = host(xs), host(ys)
+    line, circ, offs, sizes = po.twiddles(c)
+    assert np.array_equal(hy[:n // 2], circ[:n // 2])
+    # one extract_and_squash layer reproduces line layer 0 and feeds layer 1
+    tw, sq = empty(n // 2), empty(n // 2)
+    call("vx_extract_and_squash", ptr(xs), ptr(tw), ptr(sq), n // 2, NULL)
+    assert np.array_equal(host(tw), line[:n // 2])
+    tw2, sq2 = empty(n // 4), empty(n // 4)
+    call("vx_extract_and_squash", ptr(sq), ptr(tw2), ptr(sq2), n // 4, NULL)
+    assert np.array_equal(host(tw2), line[offs[1]:offs[1] + sizes[1]])
+    sq3 = empty(n // 2)
+    call("vx_squash_x", ptr(xs), ptr(sq3), n // 2, NULL)
+    assert np.array_equal(host(sq3), host(sq))
+    assert np.array_equal(hx[:4], [po.coset_at(c, i)[0] for i in range(4)])
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 1000, 1 << 20])
+def test_batch_inverse(n):
+    x = rnd(n, n)
+    x[::7] = 0
+    d = dev(x)
+    out = empty(n)
+    call("vx_batch_inverse_m31", ptr(d), ptr(out), n, NULL)
+    assert np.array_equal(host(out), po.batch_inverse(x))
+    call("vx_batch_inverse_m31", ptr(d), ptr(d), n, NULL)  # in place
+    assert np.array_equal(host(d), po.batch_inverse(x))
+
+
+@pytest.mark.parametrize("kind,log_n,use_y", [(po.HALF_COSET, 4, True), (po.HALF_COSET, 18, True),
+                                              (po.HALF_ODDS, 9, False), (po.HALF_ODDS, 20, False),
+                                              (po.SUBGROUP, 12, True)])
+def test_fold_twiddle_sources(kind, log_n, use_y):
+    c = po.coset(kind, log_n)
+    n = 1 << log_n
+    out = empty(n // 2)
+    call("vx_compute_fold_twiddle_sources", int(c[0]), int(c[1]), int(c[2]), int(c[3]), ptr(out), n, log_n,
+         1 if use_y else 0, NULL)
+    src = host(out)
+    inv = empty(n // 2)
+    call("vx_batch_inverse_m31", ptr(out), ptr(inv), n // 2, NULL)
+    assert np.array_equal(host(inv), po.fold_twiddles(c, use_y))
+    assert src[0] == (po.coset_at(c, 0)[1] if use_y else po.coset_at(c, 0)[0])
+
+
+def _tables(c):
+    line, circ, offs, sizes = po.twiddles(c)
+    return dev(line), dev(circ), np.ascontiguousarray(offs, np.uint32), np.ascontiguousarray(sizes, np.uint32)
+
+
+def _itables(c):
+    line, circ, offs, sizes = po.inverse_twiddles(c)
+    return dev(line), dev(circ), np.ascontiguousarray(offs, np.uint32), np.ascontiguousarray(sizes, np.uint32)
+
+
+NTT_LOGS = [2, 3, 4, 5, 8, 12, 13, 14, 15, 16, 18, 21, 24]
+
+
+@pytest.mark.parametrize("log_n", NTT_LOGS)
+def test_ntt_evaluate(log_n):
+    c = po.coset(po.HALF_COSET, log_n)
+    v = rnd(1 << log_n, 100 + log_n)
+    line, circ, offs, sizes = _tables(c)
+    d = dev(v)
+    call("vx_circle_ntt_evaluate", ptr(d), ptr(line), ptr(circ), hp(offs), hp(sizes), log_n - 1, 1 << log_n, NULL)
+    assert np.array_equal(host(d), po.evaluate(v, c))
+
+
+@pytest.mark.parametrize("log_n", NTT_LOGS)
+def test_ntt_interpolate(log_n):
+    c = po.coset(po.HALF_COSET, log_n)
+    v = rnd(1 << log_n, 200 + log_n)
+    line, circ, offs, sizes = _itables(c)
+    d = dev(v)
+    call("vx_circle_ntt_interpolate", ptr(d), ptr(line), ptr(circ), hp(offs), hp(sizes), log_n - 1, 1 << log_n,
+         NULL)
+    assert np.array_equal(host(d), po.interpolate(v, c))
+
+
+def test_ntt_roundtrip_2p26():
+    # size-independent property at the LDE size of the log_n = 24 proof
+    log_n = 26
+    c = po.coset(po.HALF_COSET, log_n)
+    fl, fc, offs, sizes = _tables(c)
+    il, ic, _, _ = _itables(c)
+    v = rnd(1 << log_n, 26)
+    d = dev(v)
+    call("vx_circle_ntt_evaluate", ptr(d), ptr(fl), ptr(fc), hp(offs), hp(sizes), log_n - 1, 1 << log_n, NULL)
+    assert not np.array_equal(host(d), v)
+    call("vx_circle_ntt_interpolate", ptr(d), ptr(il), ptr(ic), hp(offs), hp(sizes), log_n - 1, 1 << log_n, NULL)
+    assert np.array_equal(host(d), v)
+
+
+@pytest.mark.parametrize("log_n,blowup", [(3, 2), (4, 2), (8, 1), (12, 2), (14, 2), (16, 3), (20, 2)])
+def test_lde_matches_zero_pad_then_evaluate(log_n, blowup):
+    le = log_n + blowup
+    c = po.eval_domain(le)
+    coef = rnd(1 << log_n, le)
+    padded = np.zeros(1 << le, np.uint32)
+    padded[:1 << log_n] = coef
+    line, circ, offs, _ = _tables(c)
+    dc, out = dev(coef), empty(1 << le)
+    call("vx_circle_ntt_evaluate_lde", ptr(out), ptr(dc), 1 << log_n, ptr(line), ptr(circ), hp(offs), le - 1,
+         1 << le, NULL)
+    assert np.array_equal(host(out), po.evaluate(padded, c))
+
+
+@pytest.mark.parametrize("log_n,li", [(6, 0), (6, 3), (10, 9), (16, 7)])
+@pytest.mark.parametrize("forward", [1, 0])
+def test_ntt_single_layer(log_n, li, forward):
+    c = po.coset(po.HALF_COSET, log_n)
+    line, circ, offs, _ = po.twiddles(c)
+    tw = circ if li == 0 else line[offs[li - 1]:]
+    v = rnd(1 << log_n, li + 7 * forward)
+    d, dt = dev(v), dev(tw)
+    call("vx_circle_ntt_layer", ptr(d), ptr(dt), li, 1 << log_n, forward, NULL)
+    assert np.array_equal(host(d), po.ntt_layer(v, tw, li, forward))
+
+
+@pytest.mark.parametrize("log_n", [1, 5, 12, 20])
+def test_bit_reverse(log_n):
+    v = rnd(1 << log_n, log_n)
+    d = dev(v)
+    call("vx_bit_reverse_m31", ptr(d), log_n, NULL)
+    assert np.array_equal(host(d), po.bit_reverse(v))
+    q = rnd(4 << log_n, log_n + 1)
+    dq, out = dev(q), empty(4 << log_n)
+    call("vx_bit_reverse_qm31", ptr(dq), ptr(out), 1 << log_n, log_n, NULL)
+    idx = np.array([int(format(i, f"0{log_n}b")[::-1], 2) for i in range(1 << log_n)])
+    want = np.zeros_like(q).reshape(-1, 4)
+    want[idx] = q.reshape(-1, 4)
+    assert np.array_equal(host(out), want.reshape(-1))
+
+
+@pytest.mark.parametrize("n", [8, 1000, 1 << 20])
+def test_zero_pad_and_quotient(n):
+    t = rnd(n, n)
+    alpha = rnd(4, n + 1)
+    dt = dev(t)
+    outs = [empty(n) for _ in range(4)]
+    call("vx_fibonacci_quotient", ptr(dt), *[ptr(o) for o in outs], hp(alpha), n, NULL)
+    want = po.quotient(t, alpha)
+    for o, w in zip(outs, want):
+        assert np.array_equal(host(o), w)
+    # chunked variant over the tail, including the wrap-around rows
+    off, chunk = n // 2, n - n // 2
+    outs2 = [empty(chunk) for _ in range(4)]
+    call("vx_fibonacci_quotient_chunk", ptr(dt), *[ptr(o) for o in outs2], hp(alpha), off, chunk, n, NULL)
+    for o, w in zip(outs2, want):
+        assert np.array_equal(host(o), w[off:])
+    dst = empty(2 * n)
+    call("vx_zero_pad", ptr(dt), ptr(dst), n, 2 * n, NULL)
+    assert np.array_equal(host(dst), np.concatenate([t, np.zeros(n, np.uint32)]))
+
+
+@pytest.mark.parametrize("half_n", [4, 1000, 1 << 19])
+def test_folds(half_n):
+    cols = [rnd(2 * half_n, half_n + k) for k in range(4)]
+    tw = rnd(half_n, 5)
+    alpha = rnd(4, 6)
+    a2 = po.qm31_mul(alpha, alpha)
+    dcols, dtw = [dev(c) for c in cols], dev(tw)
+    outs = [empty(half_n) for _ in range(4)]
+    call("vx_fold_line_soa", *[ptr(c) for c in dcols], ptr(dtw), *[ptr(o) for o in outs], hp(alpha), half_n, NULL)
+    for o, w in zip(outs, po.fold_line(cols, tw, alpha)):
+        assert np.array_equal(host(o), w)
+    dst = [rnd(half_n, 50 + k) for k in range(4)]
+    ddst = [dev(d) for d in dst]
+    call("vx_fold_circle_into_line_soa", *[ptr(d) for d in ddst], *[ptr(c) for c in dcols], ptr(dtw), hp(alpha),
+         hp(np.ascontiguousarray(a2)), half_n, NULL)
+    for o, w in zip(ddst, po.fold_circle_into_line(dst, cols, tw, alpha, a2)):
+        assert np.array_equal(host(o), w)
+
+
+def _col_ptr_array(dcols):
+    return (C.c_void_p * len(dcols))(*[t.data_ptr() for t in dcols])
+
+
+@pytest.mark.parametrize("ncols,n", [(1, 64), (4, 1000), (7, 4096), (16, 256), (19, 128)])
+def test_merkle_leaves_and_nodes(ncols, n):
+    cols = [rnd(n, ncols * 10 + k) for k in range(ncols)]
+    dcols = [dev(c) for c in cols]
+    hashes = empty(8 * n)
+    call("vx_merkle_hash_leaves", _col_ptr_array(dcols), ptr(hashes), ncols, n, NULL)
+    want = po.leaf_hashes(cols[:16])
+    assert np.array_equal(host(hashes).reshape(-1, 8), want)
+    par = empty(8 * (n // 2))
+    call("vx_merkle_hash_nodes", ptr(hashes), ptr(par), n // 2, NULL)
+    assert np.array_equal(host(par).reshape(-1, 8), po.hash_nodes(want))
+
+
+@pytest.mark.parametrize("n_nodes", [1, 2, 64, 512, 1024])
+def test_merkle_reduce_to_root(n_nodes):
+    nodes = rnd(8 * n_nodes, n_nodes)
+    d, root = dev(nodes), empty(8)
+    call("vx_merkle_reduce_to_root", ptr(d), ptr(root), n_nodes, NULL)
+    level = nodes.reshape(-1, 8)
+    while len(level) > 1:
+        level = po.hash_nodes(level)
+    assert np.array_equal(host(root), level[0])
+
+
+@pytest.mark.parametrize("n", [4, 8, 256, 2048])
+def test_merkle_small_soa4_and_merge(n):
+    cols = [rnd(n, 300 + n + k) for k in range(4)]
+    dcols = [dev(c) for c in cols]
+    root = empty(8)
+    call("vx_merkle_commit_small_soa4", *[ptr(c) for c in dcols], ptr(root), n, NULL)
+    assert np.array_equal(host(root), po.merkle_root(cols))
+    par = empty(8 * (n // 2))
+    call("vx_merkle_hash_leaves_merge_soa4", *[ptr(c) for c in dcols], ptr(par), n, NULL)
+    assert np.array_equal(host(par).reshape(-1, 8), po.hash_nodes(po.leaf_hashes(cols)))
+
+
+@pytest.mark.parametrize("ncols,log_n", [(1, 10), (1, 16), (4, 12), (4, 18), (3, 11)])
+def test_merkle_tiled(ncols, log_n):
+    n = 1 << log_n
+    cols = [rnd(n, 400 + log_n + k) for k in range(ncols)]
+    dcols = [dev(c) for c in cols]
+    roots = empty(8 * (n // 1024))
+    if ncols == 4:
+        call("vx_merkle_tiled_soa4", *[ptr(c) for c in dcols], ptr(roots), n, NULL)
+    else:
+        call("vx_merkle_tiled_generic", _col_ptr_array(dcols), ptr(roots), ncols, n, NULL)
+    level = po.leaf_hashes(cols)
+    for _ in range(10):
+        level = po.hash_nodes(level)
+    assert np.array_equal(host(roots).reshape(-1, 8), level)
+
+
+@pytest.mark.parametrize("bits", [1, 8, 12, 16, 20])
+def test_grind_smallest_nonce(bits):
+    ch = po.PyChannel()
+    ch.mix_digest([bits, 1, 2, 3, 4, 5, 6, 7])
+    prefix = np.zeros(8, np.uint32)
+    st = np.frombuffer(ch.state, dtype=np.uint32)
+    # prefix digest = Blake2s(0x12345678 || 0^12 || state || bits)  (src/channel.rs:140-150)
+    msg = np.zeros(13, np.uint32)
+    msg[0] = 0x12345678
+    msg[4:12] = st
+    msg[12] = bits
+    prefix[:] = np.frombuffer(po.blake2s(msg.tobytes()), dtype=np.uint32)
+    dp, res = dev(prefix), empty(2)
+    call("vx_grind_pow_min", ptr(dp), bits, ptr(res), NULL)
+    got = host(res)
+    nonce = int(got[0]) | (int(got[1]) << 32)
+    assert nonce == ch.grind(bits)
+    # reference-style batch over [0, nonce + 1) gives the same minimum
+    res2 = dev(np.array([0xFFFFFFFF, 0xFFFFFFFF], np.uint32))
+    call("vx_grind_pow", ptr(dp), ptr(res2), bits, 0, nonce + 1, NULL)
+    got2 = host(res2)
+    assert int(got2[0]) | (int(got2[1]) << 32) == nonce

@@ -1,0 +1,93 @@
+"""vortexstark_amd: MI355X (gfx950) Circle-STARK prover hot path.
+
+Python mirror of the reference's prove/verify API (src/prover.rs:221-262,
+src/verifier.rs:39) over the C-ABI in include/vortex_hip.h. Proofs are KRAK
+byte strings (src/bin/stark_cli.rs:1092-1120); v1 when pow_bits == 0.
+
+    p = Prover(log_n=24)          # ProverCache::new: tables + buffers, once
+    proof = p.prove(1, 1)         # prove_cached(a, b)
+    ok, why = verify(proof)       # verifier::verify
+"""
+import ctypes as C
+
+from . import _lib
+from ._lib import VortexError, load
+
+BLOWUP_BITS = 2   # src/prover.rs:106-109
+N_QUERIES = 80    # src/prover.rs:114-117
+P = 0x7FFFFFFF
+
+STAGES = ("trace", "interpolate", "lde", "trace_commit", "quotient_commit", "fri", "queries", "decommit",
+          "download", "total", "trace_subtree_kernel")
+
+
+def proof_size(log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0):
+    return int(load().vx_proof_size(log_n, blowup_bits, n_queries, pow_bits))
+
+
+class Prover:
+    """Cached prover for one size class (ProverCache + prove_cached)."""
+
+    def __init__(self, log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0, stream=None):
+        self.log_n, self.blowup_bits, self.n_queries, self.pow_bits = log_n, blowup_bits, n_queries, pow_bits
+        self._h = C.c_void_p()
+        _lib.call("vx_prover_create", log_n, blowup_bits, n_queries, pow_bits, stream, C.byref(self._h))
+        self.size = proof_size(log_n, blowup_bits, n_queries, pow_bits)
+        self._buf = (C.c_uint8 * self.size)()
+
+    def prove(self, a, b):
+        n = C.c_size_t()
+        _lib.call("vx_prover_prove", self._h, a, b, self._buf, self.size, C.byref(n))
+        return bytes(self._buf[:n.value])
+
+    def enqueue(self, a, b, timed=False):
+        _lib.call("vx_prover_enqueue", self._h, a, b, 1 if timed else 0)
+
+    def wait(self):
+        n = C.c_size_t()
+        _lib.call("vx_prover_wait", self._h, self._buf, self.size, C.byref(n))
+        return bytes(self._buf[:n.value])
+
+    def stage_times(self):
+        out = (C.c_float * 11)()
+        _lib.call("vx_prover_stage_times", self._h, out)
+        return dict(zip(STAGES, list(out)))
+
+    @property
+    def device_bytes(self):
+        return int(load().vx_prover_device_bytes(self._h))
+
+    def buffer(self, which):
+        """(device pointer, words) of an internal buffer of the last proof."""
+        ptr, words = C.c_void_p(), C.c_uint64()
+        _lib.call("vx_prover_buffer", self._h, which, C.byref(ptr), C.byref(words))
+        return ptr.value, words.value
+
+    def close(self):
+        if self._h:
+            _lib.call("vx_prover_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prove(a, b, log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0):
+    """prover::prove (src/prover.rs:221): one-shot proof bytes."""
+    size = proof_size(log_n, blowup_bits, n_queries, pow_bits)
+    buf = (C.c_uint8 * size)()
+    n = C.c_size_t()
+    _lib.call("vx_prove", a, b, log_n, blowup_bits, n_queries, pow_bits, buf, size, C.byref(n))
+    return bytes(buf[:n.value])
+
+
+def verify(proof, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES):
+    """verifier::verify (src/verifier.rs:39): (accepted, reason). Host code."""
+    err = C.create_string_buffer(512)
+    rc = load().vx_verify(proof, len(proof), blowup_bits, n_queries, err, 512)
+    if rc not in (0, 4):
+        raise VortexError("vx_verify", rc, err.value.decode())
+    return rc == 0, err.value.decode()

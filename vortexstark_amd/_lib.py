@@ -1,0 +1,112 @@
+"""ctypes binding of libvortex_hip.so (the C-ABI declared in include/vortex_hip.h).
+
+The library is built in-tree by vortexstark_amd/csrc/Makefile (see
+__graft_entry__.build()). Loading fails loudly when it is missing: there is no
+CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvortex_hip.so")
+HEADER = os.path.join(HERE, "..", "include", "vortex_hip.h")
+
+u32, u64, sz, ip = C.c_uint32, C.c_uint64, C.c_size_t, C.c_int
+vp = C.c_void_p
+u32p = C.POINTER(C.c_uint32)
+u8p = C.POINTER(C.c_uint8)
+
+# name -> (restype, argtypes). Device pointers and streams are passed as ints.
+SIGNATURES = {
+    "vx_last_error": (C.c_char_p, []),
+    "vx_device_sync": (ip, []),
+    "vx_circle_ntt_evaluate": (ip, [vp, vp, vp, vp, vp, u32, u32, vp]),
+    "vx_circle_ntt_interpolate": (ip, [vp, vp, vp, vp, vp, u32, u32, vp]),
+    "vx_circle_ntt_evaluate_batch": (ip, [vp, vp, vp, vp, vp, u32, u32, u32, vp]),
+    "vx_circle_ntt_interpolate_batch": (ip, [vp, vp, vp, vp, vp, u32, u32, u32, vp]),
+    "vx_circle_ntt_layer": (ip, [vp, vp, u32, u32, ip, vp]),
+    "vx_circle_ntt_evaluate_lde": (ip, [vp, vp, u32, vp, vp, vp, u32, u32, vp]),
+    "vx_bit_reverse_m31": (ip, [vp, u32, vp]),
+    "vx_bit_reverse_qm31": (ip, [vp, vp, u32, u32, vp]),
+    "vx_compute_coset_points": (ip, [u32, u32, u32, u32, vp, vp, u32, vp]),
+    "vx_extract_and_squash": (ip, [vp, vp, vp, u32, vp]),
+    "vx_squash_x": (ip, [vp, vp, u32, vp]),
+    "vx_batch_inverse_m31": (ip, [vp, vp, u32, vp]),
+    "vx_compute_fold_twiddle_sources": (ip, [u32, u32, u32, u32, vp, u32, u32, ip, vp]),
+    "vx_fibonacci_trace": (ip, [u32, u32, vp, u32, vp]),
+    "vx_fibonacci_quotient": (ip, [vp, vp, vp, vp, vp, vp, u32, vp]),
+    "vx_fibonacci_quotient_chunk": (ip, [vp, vp, vp, vp, vp, vp, u32, u32, u32, vp]),
+    "vx_zero_pad": (ip, [vp, vp, u32, u32, vp]),
+    "vx_fold_line_soa": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp]),
+    "vx_fold_circle_into_line_soa": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp]),
+    "vx_merkle_hash_leaves": (ip, [vp, vp, u32, u32, vp]),
+    "vx_merkle_hash_leaves_single": (ip, [vp, vp, u32, vp]),
+    "vx_merkle_hash_leaves_quad": (ip, [vp, vp, vp, vp, vp, u32, vp]),
+    "vx_merkle_hash_nodes": (ip, [vp, vp, u32, vp]),
+    "vx_merkle_reduce_to_root": (ip, [vp, vp, u32, vp]),
+    "vx_merkle_commit_small_soa4": (ip, [vp, vp, vp, vp, vp, u32, vp]),
+    "vx_merkle_hash_leaves_merge_soa4": (ip, [vp, vp, vp, vp, vp, u32, vp]),
+    "vx_merkle_tiled_soa4": (ip, [vp, vp, vp, vp, vp, u32, vp]),
+    "vx_merkle_tiled_generic": (ip, [vp, vp, u32, u32, vp]),
+    "vx_grind_pow": (ip, [vp, vp, u32, u64, u32, vp]),
+    "vx_grind_pow_min": (ip, [vp, u32, vp, vp]),
+    "vx_proof_size": (sz, [u32, u32, u32, u32]),
+    "vx_prover_create": (ip, [u32, u32, u32, u32, vp, C.POINTER(vp)]),
+    "vx_prover_destroy": (ip, [vp]),
+    "vx_prover_prove": (ip, [vp, u32, u32, vp, sz, C.POINTER(sz)]),
+    "vx_prover_enqueue": (ip, [vp, u32, u32, ip]),
+    "vx_prover_wait": (ip, [vp, vp, sz, C.POINTER(sz)]),
+    "vx_prover_stage_times": (ip, [vp, C.POINTER(C.c_float)]),
+    "vx_prover_device_bytes": (u64, [vp]),
+    "vx_prover_buffer": (ip, [vp, ip, C.POINTER(vp), C.POINTER(u64)]),
+    "vx_prove": (ip, [u32, u32, u32, u32, u32, u32, vp, sz, C.POINTER(sz)]),
+    "vx_verify": (ip, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
+}
+
+_lib = None
+
+
+class VortexError(RuntimeError):
+    """Non-zero status from a vx_* entry point."""
+
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def header_symbols():
+    """Function names declared in include/vortex_hip.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vx_[a-z0-9_]+)\s*\(", text)))
+
+
+def load():
+    """Load libvortex_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the HIP path has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(fn, rc):
+    if rc != 0:
+        raise VortexError(fn, rc, load().vx_last_error().decode(errors="replace"))
+
+
+def call(fn, *args):
+    """Call a status-returning entry point and raise on failure."""
+    rc = getattr(load(), fn)(*args)
+    check(fn, rc)
+    return rc

@@ -1,0 +1,446 @@
+// extern "C" entry points of include/vortex_hip.h. Each shim validates its
+// arguments on the host (shapes that the kernels assume), launches on the
+// given stream and converts exceptions into a status code + vx_last_error().
+#include "../../include/vortex_hip.h"
+
+#include <cstring>
+#include <string>
+
+#include "prover.h"
+
+using namespace vx;
+
+namespace vx {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+}  // namespace vx
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    f();
+    return VX_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return VX_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return VX_ERR_INTERNAL;
+  }
+}
+
+static hipStream_t S(vx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+static bool pow2(uint64_t n) { return n && (n & (n - 1)) == 0; }
+static uint32_t log2u(uint64_t n) {
+  uint32_t l = 0;
+  while ((1ull << l) < n) l++;
+  return l;
+}
+static qm31 host_q(const uint32_t* a) {
+  require(a != nullptr, "alpha must be a host [4] array");
+  for (int i = 0; i < 4; i++) require(a[i] < P, "alpha words must be canonical M31");
+  return {{a[0], a[1], a[2], a[3]}};
+}
+
+// Reference twiddle tables for a transform of n = 2^log_n points.
+static TwiddleRef tw_ref(const uint32_t* line, const uint32_t* circle, const uint32_t* h_offsets,
+                         uint32_t n_line_layers, uint32_t n) {
+  require(line && circle && pow2(n) && n >= 2, "ntt: n must be a power of two >= 2 with twiddles");
+  uint32_t log_n = log2u(n);
+  require(n_line_layers == log_n - 1, "ntt: n_line_layers must be log2(n) - 1");
+  require(n_line_layers == 0 || h_offsets != nullptr, "ntt: null layer offsets");
+  TwiddleRef r{};
+  r.line = line;
+  r.circle = circle;
+  r.log_n = log_n;
+  for (uint32_t l = 0; l < n_line_layers; l++) r.off[l] = h_offsets[l];
+  return r;
+}
+
+extern "C" {
+
+const char* vx_last_error(void) { return g_last_error.c_str(); }
+
+int vx_device_sync(void) {
+  return guard([&] { VX_HIP(hipDeviceSynchronize()); });
+}
+
+int vx_circle_ntt_evaluate(uint32_t* d, const uint32_t* tw, const uint32_t* ctw, const uint32_t* offs,
+                           const uint32_t* sizes, uint32_t nl, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(d != nullptr, "null data");
+    TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
+    ntt_forward(d, d, n - 1, r, r.log_n, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_circle_ntt_interpolate(uint32_t* d, const uint32_t* tw, const uint32_t* ctw, const uint32_t* offs,
+                              const uint32_t* sizes, uint32_t nl, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(d != nullptr, "null data");
+    TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
+    ntt_inverse(d, r, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_circle_ntt_evaluate_batch(uint32_t* const* cols, const uint32_t* tw, const uint32_t* ctw,
+                                 const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
+                                 uint32_t n_cols, vx_stream_t st) {
+  return guard([&] {
+    require(n_cols == 0 || cols != nullptr, "null column array");
+    TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
+    for (uint32_t c = 0; c < n_cols; c++) {
+      require(cols[c] != nullptr, "null column");
+      ntt_forward(cols[c], cols[c], n - 1, r, r.log_n, S(st));
+    }
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_circle_ntt_interpolate_batch(uint32_t* const* cols, const uint32_t* tw, const uint32_t* ctw,
+                                    const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
+                                    uint32_t n_cols, vx_stream_t st) {
+  return guard([&] {
+    require(n_cols == 0 || cols != nullptr, "null column array");
+    TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
+    for (uint32_t c = 0; c < n_cols; c++) {
+      require(cols[c] != nullptr, "null column");
+      ntt_inverse(cols[c], r, S(st));
+    }
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_circle_ntt_layer(uint32_t* d, const uint32_t* tw, uint32_t li, uint32_t n, int fwd, vx_stream_t st) {
+  return guard([&] {
+    require(d && tw && pow2(n) && n >= 2 && li < log2u(n), "ntt_layer: bad arguments");
+    ntt_layer(d, tw, li, n, fwd != 0, S(st));
+  });
+}
+
+int vx_circle_ntt_evaluate_lde(uint32_t* d_eval, const uint32_t* d_coeffs, uint32_t coeff_n, const uint32_t* tw,
+                               const uint32_t* ctw, const uint32_t* offs, uint32_t nl, uint32_t n,
+                               vx_stream_t st) {
+  return guard([&] {
+    require(d_eval && d_coeffs && pow2(coeff_n) && coeff_n >= 2 && coeff_n <= n, "lde: bad sizes");
+    require(d_eval != d_coeffs, "lde: output must not alias the coefficients");
+    TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
+    ntt_forward(d_eval, d_coeffs, coeff_n - 1, r, log2u(coeff_n), S(st));
+  });
+}
+
+int vx_bit_reverse_m31(uint32_t* d, uint32_t log_n, vx_stream_t st) {
+  return guard([&] {
+    require(d && log_n <= 31, "bit_reverse: bad arguments");
+    bit_reverse_m31(d, log_n, S(st));
+  });
+}
+
+int vx_bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t log_n, vx_stream_t st) {
+  return guard([&] {
+    require(in && out && log_n <= 31 && n == (1u << log_n), "bit_reverse_qm31: n must be 2^log_n");
+    bit_reverse_qm31(in, out, log_n, S(st));
+  });
+}
+
+int vx_compute_coset_points(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* ox, uint32_t* oy,
+                            uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(ox || oy, "coset_points: no output");
+    coset_points({ix, iy}, {sx, sy}, ox, oy, n, S(st));
+  });
+}
+
+int vx_extract_and_squash(const uint32_t* in, uint32_t* tw, uint32_t* sq, uint32_t half_n, vx_stream_t st) {
+  return guard([&] {
+    require(in && tw && sq, "extract_and_squash: null pointer");
+    extract_and_squash(in, tw, sq, half_n, S(st));
+  });
+}
+
+int vx_squash_x(const uint32_t* in, uint32_t* out, uint32_t out_n, vx_stream_t st) {
+  return guard([&] {
+    require(in && out, "squash_x: null pointer");
+    extract_and_squash(in, nullptr, out, out_n, S(st));
+  });
+}
+
+int vx_batch_inverse_m31(const uint32_t* in, uint32_t* out, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(in && out, "batch_inverse: null pointer");
+    batch_inverse(in, out, n, S(st));
+  });
+}
+
+int vx_compute_fold_twiddle_sources(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* out, uint32_t n,
+                                    uint32_t log_n, int extract_y, vx_stream_t st) {
+  return guard([&] {
+    require(out != nullptr, "fold_twiddle_sources: null output");
+    fold_twiddle_sources({ix, iy}, {sx, sy}, out, n, log_n, extract_y != 0, S(st));
+  });
+}
+
+int vx_fibonacci_trace(uint32_t a, uint32_t b, uint32_t* out, uint32_t log_n, vx_stream_t st) {
+  return guard([&] {
+    require(out != nullptr, "fibonacci_trace: null output");
+    fib_trace(a, b, out, log_n, S(st));
+  });
+}
+
+int vx_fibonacci_quotient(const uint32_t* t, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3,
+                          const uint32_t* alpha, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(t && o0 && o1 && o2 && o3, "quotient: null pointer");
+    fib_quotient(t, o0, o1, o2, o3, host_q(alpha), 0, n, n, S(st));
+  });
+}
+
+int vx_fibonacci_quotient_chunk(const uint32_t* t, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3,
+                                const uint32_t* alpha, uint32_t offset, uint32_t chunk_n, uint32_t global_n,
+                                vx_stream_t st) {
+  return guard([&] {
+    require(t && o0 && o1 && o2 && o3, "quotient_chunk: null pointer");
+    fib_quotient(t, o0, o1, o2, o3, host_q(alpha), offset, chunk_n, global_n, S(st));
+  });
+}
+
+int vx_zero_pad(const uint32_t* src, uint32_t* dst, uint32_t src_n, uint32_t dst_n, vx_stream_t st) {
+  return guard([&] {
+    require(dst && (src || src_n == 0), "zero_pad: null pointer");
+    zero_pad(src, dst, src_n, dst_n, S(st));
+  });
+}
+
+int vx_fold_line_soa(const uint32_t* i0, const uint32_t* i1, const uint32_t* i2, const uint32_t* i3,
+                     const uint32_t* tw, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3,
+                     const uint32_t* alpha, uint32_t half_n, vx_stream_t st) {
+  return guard([&] {
+    require(i0 && i1 && i2 && i3 && tw && o0 && o1 && o2 && o3, "fold_line: null pointer");
+    const uint32_t* in[4] = {i0, i1, i2, i3};
+    uint32_t* out[4] = {o0, o1, o2, o3};
+    fold_line(in, tw, out, host_q(alpha), half_n, S(st));
+  });
+}
+
+int vx_fold_circle_into_line_soa(uint32_t* d0, uint32_t* d1, uint32_t* d2, uint32_t* d3, const uint32_t* s0,
+                                 const uint32_t* s1, const uint32_t* s2, const uint32_t* s3, const uint32_t* tw,
+                                 const uint32_t* alpha, const uint32_t* alpha_sq, uint32_t half_n,
+                                 vx_stream_t st) {
+  return guard([&] {
+    require(d0 && d1 && d2 && d3 && s0 && s1 && s2 && s3 && tw, "fold_circle: null pointer");
+    uint32_t* dst[4] = {d0, d1, d2, d3};
+    const uint32_t* src[4] = {s0, s1, s2, s3};
+    fold_circle_into_line(dst, src, tw, host_q(alpha), host_q(alpha_sq), half_n, S(st));
+  });
+}
+
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+static LeafSrc cols_leaf(const uint32_t* const* cols, uint32_t n_cols) {
+  require(cols != nullptr && n_cols >= 1, "merkle: need at least one column");
+  LeafSrc L{};
+  L.kind = LEAF_COLS;
+  L.n_cols = n_cols > 16 ? 16 : n_cols;
+  for (uint32_t c = 0; c < L.n_cols; c++) {
+    require(cols[c] != nullptr, "merkle: null column");
+    L.col[c] = cols[c];
+  }
+  // t = 4 * min(n_cols, 16): extra columns are ignored (cuda/blake2s.cu:50-54)
+  return L;
+}
+
+int vx_merkle_hash_leaves(const uint32_t* const* cols, uint32_t* hashes, uint32_t n_cols, uint32_t n_leaves,
+                          vx_stream_t st) {
+  return guard([&] {
+    require(hashes != nullptr, "hash_leaves: null output");
+    merkle_leaf_hashes(cols_leaf(cols, n_cols), n_leaves, hashes, S(st));
+  });
+}
+
+int vx_merkle_hash_leaves_single(const uint32_t* col, uint32_t* hashes, uint32_t n_leaves, vx_stream_t st) {
+  const uint32_t* cols[1] = {col};
+  return vx_merkle_hash_leaves(cols, hashes, 1, n_leaves, st);
+}
+
+int vx_merkle_hash_leaves_quad(const uint32_t* c0, const uint32_t* c1, const uint32_t* c2, const uint32_t* c3,
+                               uint32_t* hashes, uint32_t n_leaves, vx_stream_t st) {
+  const uint32_t* cols[4] = {c0, c1, c2, c3};
+  return vx_merkle_hash_leaves(cols, hashes, 4, n_leaves, st);
+}
+
+int vx_merkle_hash_nodes(const uint32_t* ch, uint32_t* par, uint32_t n_parents, vx_stream_t st) {
+  return guard([&] {
+    require(ch && par, "hash_nodes: null pointer");
+    merkle_hash_nodes(ch, par, n_parents, S(st));
+  });
+}
+
+int vx_merkle_reduce_to_root(const uint32_t* nodes, uint32_t* root, uint32_t n_nodes, vx_stream_t st) {
+  return guard([&] {
+    require(nodes && root, "reduce_to_root: null pointer");
+    merkle_reduce_to_root(nodes, root, n_nodes, S(st));
+  });
+}
+
+int vx_merkle_commit_small_soa4(const uint32_t* c0, const uint32_t* c1, const uint32_t* c2, const uint32_t* c3,
+                                uint32_t* root, uint32_t n_leaves, vx_stream_t st) {
+  return guard([&] {
+    require(root && pow2(n_leaves) && n_leaves >= 4 && n_leaves <= 2048, "commit_small: 4..2048 leaves");
+    const uint32_t* cols[4] = {c0, c1, c2, c3};
+    LeafSrc L = cols_leaf(cols, 4);
+    for (int c = 0; c < 4; c++) require(aligned16(L.col[c]), "commit_small: columns must be 16-byte aligned");
+    merkle_root_small(L, log2u(n_leaves), root, S(st));
+  });
+}
+
+int vx_merkle_hash_leaves_merge_soa4(const uint32_t* c0, const uint32_t* c1, const uint32_t* c2, const uint32_t* c3,
+                                     uint32_t* parents, uint32_t n_leaves, vx_stream_t st) {
+  return guard([&] {
+    require(parents != nullptr, "leaves_merge: null output");
+    const uint32_t* cols[4] = {c0, c1, c2, c3};
+    merkle_leaf_pairs(cols_leaf(cols, 4), n_leaves, parents, S(st));
+  });
+}
+
+int vx_merkle_tiled_generic(const uint32_t* const* cols, uint32_t* roots, uint32_t n_cols, uint32_t n_leaves,
+                            vx_stream_t st) {
+  return guard([&] {
+    require(roots && pow2(n_leaves) && n_leaves >= 1024, "tiled: n_leaves must be a power of two >= 1024");
+    LeafSrc L = cols_leaf(cols, n_cols);
+    for (uint32_t c = 0; c < L.n_cols; c++) require(aligned16(L.col[c]), "tiled: columns must be 16-byte aligned");
+    merkle_tiled(L, log2u(n_leaves), 10, roots, S(st));
+  });
+}
+
+int vx_merkle_tiled_soa4(const uint32_t* c0, const uint32_t* c1, const uint32_t* c2, const uint32_t* c3,
+                         uint32_t* roots, uint32_t n_leaves, vx_stream_t st) {
+  const uint32_t* cols[4] = {c0, c1, c2, c3};
+  return vx_merkle_tiled_generic(cols, roots, 4, n_leaves, st);
+}
+
+int vx_grind_pow(const uint32_t* prefix, uint64_t* result, uint32_t pow_bits, uint64_t batch_offset,
+                 uint32_t n_threads, vx_stream_t st) {
+  return guard([&] {
+    require(prefix && result && n_threads > 0, "grind_pow: bad arguments");
+    grind_pow_batch(prefix, result, pow_bits, batch_offset, n_threads, S(st));
+  });
+}
+
+int vx_grind_pow_min(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
+  return guard([&] {
+    require(prefix && result, "grind_pow_min: null pointer");
+    grind_pow(prefix, pow_bits, result, S(st));
+  });
+}
+
+// ----------------------------------------------------------- prove / verify
+struct vx_prover {
+  FibProver impl;
+  vx_prover(uint32_t l, uint32_t b, uint32_t q, uint32_t p, hipStream_t s) : impl(l, b, q, p, s) {}
+};
+
+size_t vx_proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits) {
+  if (log_n < 1 || blowup_bits < 1 || log_n + blowup_bits > 31 || log_n + blowup_bits < 4) return 0;
+  return proof_size(log_n, blowup_bits, nq, pow_bits);
+}
+
+int vx_prover_create(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits, vx_stream_t st,
+                     vx_prover** out) {
+  return guard([&] {
+    require(out != nullptr, "prover_create: null out");
+    *out = nullptr;
+    *out = new vx_prover(log_n, blowup_bits, nq, pow_bits, S(st));
+  });
+}
+
+int vx_prover_destroy(vx_prover* p) {
+  return guard([&] { delete p; });
+}
+
+int vx_prover_enqueue(vx_prover* p, uint32_t a, uint32_t b, int timed) {
+  return guard([&] {
+    require(p != nullptr, "null prover");
+    p->impl.enqueue(a, b, timed != 0);
+  });
+}
+
+int vx_prover_wait(vx_prover* p, uint8_t* out, size_t cap, size_t* len) {
+  return guard([&] {
+    require(p != nullptr, "null prover");
+    size_t n = 0;
+    const uint8_t* src = p->impl.wait(&n);
+    if (len) *len = n;
+    if (out) {
+      require(cap >= n, "output buffer smaller than the proof (see vx_proof_size)");
+      std::memcpy(out, src, n);
+    }
+  });
+}
+
+int vx_prover_prove(vx_prover* p, uint32_t a, uint32_t b, uint8_t* out, size_t cap, size_t* len) {
+  int rc = vx_prover_enqueue(p, a, b, 0);
+  if (rc) return rc;
+  return vx_prover_wait(p, out, cap, len);
+}
+
+int vx_prover_stage_times(vx_prover* p, float out[11]) {
+  return guard([&] {
+    require(p && out, "null argument");
+    StageTimes t = p->impl.last_times();
+    float v[11] = {t.trace_ms, t.interpolate_ms, t.lde_ms,     t.trace_commit_ms, t.quotient_commit_ms, t.fri_ms,
+                   t.queries_ms, t.decommit_ms,  t.download_ms, t.total_ms,       t.trace_subtree_ms};
+    std::memcpy(out, v, sizeof v);
+  });
+}
+
+uint64_t vx_prover_device_bytes(const vx_prover* p) { return p ? p->impl.device_bytes() : 0; }
+
+int vx_prover_buffer(const vx_prover* p, int which, const uint32_t** ptr, uint64_t* words) {
+  return guard([&] {
+    require(p && ptr && words, "null argument");
+    const ProofLayout& L = p->impl.layout();
+    if (which == 0) {
+      *ptr = p->impl.d_trace();
+      *words = 1ull << L.log_n;
+    } else if (which == 1) {
+      *ptr = p->impl.d_eval();
+      *words = 1ull << L.le;
+    } else if (which == 100) {
+      *ptr = p->impl.d_proof();
+      *words = L.total_words;
+    } else {
+      require(false, "prover_buffer: unknown buffer");
+    }
+  });
+}
+
+int vx_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits,
+             uint8_t* out, size_t cap, size_t* len) {
+  vx_prover* p = nullptr;
+  int rc = vx_prover_create(log_n, blowup_bits, nq, pow_bits, nullptr, &p);
+  if (rc) return rc;
+  rc = vx_prover_prove(p, a, b, out, cap, len);
+  vx_prover_destroy(p);
+  return rc;
+}
+
+int vx_verify(const uint8_t* proof, size_t len, uint32_t blowup_bits, uint32_t nq, char* err, size_t err_cap) {
+  int rc = guard([&] {
+    require(proof != nullptr || len == 0, "verify: null proof");
+    std::string why;
+    if (!verify_proof(proof, len, blowup_bits, nq, why)) throw Error(ERR_VERIFY, why);
+  });
+  if (err && err_cap) {
+    std::string m = rc == VX_OK ? "" : g_last_error;
+    std::strncpy(err, m.c_str(), err_cap - 1);
+    err[err_cap - 1] = 0;
+  }
+  return rc;
+}
+
+}  // extern "C"

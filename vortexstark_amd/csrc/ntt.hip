@@ -1,0 +1,237 @@
+// Twisted circle NTT for gfx950 (interpolate / evaluate / LDE).
+//
+// Semantics: cuda/circle_ntt.cu:74-117 (butterflies, layer indexing) and
+// :248-312 (layer order, 1/n scale). Bit j of the index is butterflied by
+// reference layer j (j = 0 is the circle layer, twiddle C[i >> 1]; j >= 1 is
+// line layer j - 1, twiddle T_{j-1}[i >> (j+1)]). Evaluate runs bits
+// high -> low, interpolate low -> high.
+//
+// MI355X design: instead of one HBM round trip per layer, a pass transforms
+// k consecutive bits [j0, j0 + k) of a 2^k x 2^cbits tile held in LDS
+// (64 KiB per workgroup, 16384 words). Rows are 2^j0 apart in HBM; the
+// 2^cbits consecutive columns of a row give 64-byte (or longer) contiguous
+// runs. The lowest pass is contiguous (j0 = 0, 14 bits), so a 2^24 transform
+// takes two passes. The LDE's first log(N/n) evaluate layers only copy the
+// zero-padded coefficients, so the first pass reads coefficient i & (n-1)
+// directly and those layers are never executed.
+#include "vx_internal.h"
+
+namespace vx {
+
+static constexpr uint32_t NTT_THREADS = 512;
+static constexpr uint32_t NTT_TILE_LOG = 14;   // words per tile (64 KiB of LDS)
+static constexpr uint32_t NTT_STRIDED_MAX = 10; // bits per strided pass (>= 16 columns)
+
+struct PassArgs {
+  const uint32_t* tw[16];  // twiddle table of bit j0 + b, for local bit b
+  uint32_t j0, k, cbits;
+  uint64_t src_mask;
+  uint32_t scale;          // output multiplier (1 = none)
+};
+
+template <bool FWD>
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(uint32_t* dst, const uint32_t* src, PassArgs a) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t cmask = (1u << a.cbits) - 1;
+  const uint32_t E = 1u << (a.k + a.cbits);
+  const uint32_t low_tiles_log = a.j0 - a.cbits;
+  const uint64_t tile = blockIdx.x;
+  const uint64_t low_base = (tile & ((1ull << low_tiles_log) - 1)) << a.cbits;
+  const uint64_t hi = tile >> low_tiles_log;
+  const uint64_t hi_base = hi << (a.j0 + a.k);
+
+  for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+    uint64_t gi = hi_base | ((uint64_t)(e >> a.cbits) << a.j0) | (low_base + (e & cmask));
+    lds[e] = src[gi & a.src_mask];
+  }
+  __syncthreads();
+
+  const uint32_t half = E >> 1;
+  for (uint32_t s = 0; s < a.k; s++) {
+    const uint32_t b = FWD ? (a.k - 1 - s) : s;
+    const uint32_t* tw = a.tw[b];
+    const uint32_t bmask = (1u << b) - 1;
+    for (uint32_t u = threadIdx.x; u < half; u += blockDim.x) {
+      const uint32_t c = u & cmask, rr = u >> a.cbits;
+      const uint32_t r0 = ((rr >> b) << (b + 1)) | (rr & bmask);
+      const uint64_t h = (hi << (a.k - b - 1)) | (r0 >> (b + 1));
+      const uint32_t w = tw[h];
+      const uint32_t i0 = (r0 << a.cbits) | c, i1 = ((r0 | (1u << b)) << a.cbits) | c;
+      const uint32_t v0 = lds[i0], v1 = lds[i1];
+      if (FWD) {
+        const uint32_t t = m31_mul(v1, w);
+        lds[i0] = m31_add(v0, t);
+        lds[i1] = m31_sub(v0, t);
+      } else {
+        lds[i0] = m31_add(v0, v1);
+        lds[i1] = m31_mul(m31_sub(v0, v1), w);
+      }
+    }
+    __syncthreads();
+  }
+
+  for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+    uint64_t gi = hi_base | ((uint64_t)(e >> a.cbits) << a.j0) | (low_base + (e & cmask));
+    uint32_t v = lds[e];
+    if (a.scale != 1u) v = m31_mul(v, a.scale);
+    dst[gi] = v;
+  }
+}
+
+static const uint32_t* bit_table(const TwiddleRef& tw, uint32_t j) {
+  return j == 0 ? tw.circle : tw.line + tw.off[j - 1];
+}
+
+static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_mask,
+                        const TwiddleRef& tw, uint32_t j0, uint32_t k, uint32_t cbits, uint32_t scale,
+                        hipStream_t s) {
+  const uint32_t log_n = tw.log_n;
+  require(k >= 1 && k <= 16 && k + cbits <= NTT_TILE_LOG && k + cbits <= log_n && cbits <= j0 &&
+              j0 + k <= log_n,
+          "ntt: invalid pass shape");
+  PassArgs a{};
+  for (uint32_t b = 0; b < k; b++) a.tw[b] = bit_table(tw, j0 + b);
+  a.j0 = j0;
+  a.k = k;
+  a.cbits = cbits;
+  a.src_mask = src_mask;
+  a.scale = scale;
+  const uint32_t e_log = k + cbits;
+  const uint64_t grid = 1ull << (log_n - e_log);
+  const uint32_t threads = e_log >= 10 ? NTT_THREADS : (e_log >= 7 ? (1u << (e_log - 1)) : 64u);
+  const size_t lds = (size_t)4 << e_log;
+  if (fwd)
+    hipLaunchKernelGGL(k_ntt_pass<true>, dim3((uint32_t)grid), dim3(threads), lds, s, dst, src, a);
+  else
+    hipLaunchKernelGGL(k_ntt_pass<false>, dim3((uint32_t)grid), dim3(threads), lds, s, dst, src, a);
+  VX_HIP(hipGetLastError());
+}
+
+void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
+                 uint32_t eff_bits, hipStream_t s) {
+  const uint32_t log_n = tw.log_n;
+  require(log_n >= 1 && log_n <= 31 && eff_bits >= 1 && eff_bits <= log_n, "ntt_forward: bad sizes");
+  require(((src_mask + 1) & src_mask) == 0 && src_mask + 1 >= (1ull << eff_bits),
+          "ntt_forward: source length must be a power of two covering the transformed bits");
+  // Strided passes cover [NTT_TILE_LOG, eff_bits) from the top, then one
+  // contiguous pass covers the low bits.
+  uint32_t top = eff_bits;
+  const uint32_t contig = eff_bits < NTT_TILE_LOG ? eff_bits : NTT_TILE_LOG;
+  const uint32_t strided_bits = eff_bits - contig;
+  const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
+  const uint32_t* cur = src;
+  uint64_t mask = src_mask;
+  for (uint32_t p = 0; p < n_strided; p++) {
+    uint32_t k = (top - contig) / (n_strided - p);  // balanced chunks
+    uint32_t j0 = top - k;
+    launch_pass(true, data, cur, mask, tw, j0, k, NTT_TILE_LOG - k, 1u, s);
+    top = j0;
+    cur = data;
+    mask = (1ull << log_n) - 1;
+  }
+  launch_pass(true, data, cur, mask, tw, 0, contig, 0, 1u, s);
+}
+
+void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s) {
+  const uint32_t log_n = tw.log_n;
+  require(log_n >= 1 && log_n <= 31, "ntt_inverse: bad size");
+  const uint32_t e = (30u * log_n) % 31u;
+  const uint32_t inv_n = e == 0 ? 1u : (1u << e);
+  const uint64_t mask = (1ull << log_n) - 1;
+  const uint32_t contig = log_n < NTT_TILE_LOG ? log_n : NTT_TILE_LOG;
+  const uint32_t strided_bits = log_n - contig;
+  const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
+  launch_pass(false, data, data, mask, tw, 0, contig, 0, n_strided == 0 ? inv_n : 1u, s);
+  uint32_t bot = contig;
+  for (uint32_t p = 0; p < n_strided; p++) {
+    uint32_t k = (log_n - bot) / (n_strided - p);
+    bool last = p + 1 == n_strided;
+    launch_pass(false, data, data, mask, tw, bot, k, NTT_TILE_LOG - k, last ? inv_n : 1u, s);
+    bot += k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single reference layer (cuda/circle_ntt.cu:89-117), for the layer-wise API.
+__global__ void k_ntt_layer(uint32_t* data, const uint32_t* tw, uint32_t li, uint64_t half_n, int fwd) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= half_n) return;
+  uint64_t stride = 1ull << li;
+  uint64_t h = t >> li, l = t & (stride - 1);
+  uint64_t i0 = (h << (li + 1)) + l, i1 = i0 + stride;
+  uint32_t v0 = data[i0], v1 = data[i1], w = tw[h];
+  if (fwd) {
+    uint32_t m = m31_mul(v1, w);
+    data[i0] = m31_add(v0, m);
+    data[i1] = m31_sub(v0, m);
+  } else {
+    data[i0] = m31_add(v0, v1);
+    data[i1] = m31_mul(m31_sub(v0, v1), w);
+  }
+}
+
+void ntt_layer(uint32_t* data, const uint32_t* tw, uint32_t li, uint64_t n, bool forward, hipStream_t s) {
+  require(n >= 2 && (n & (n - 1)) == 0 && (1ull << (li + 1)) <= n, "ntt_layer: bad shape");
+  uint64_t half = n / 2;
+  hipLaunchKernelGGL(k_ntt_layer, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, s, data, tw, li, half,
+                     forward ? 1 : 0);
+  VX_HIP(hipGetLastError());
+}
+
+__global__ void k_scale(uint32_t* d, uint32_t sc, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = m31_mul(d[i], sc);
+}
+void m31_scale(uint32_t* d, uint32_t sc, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_scale, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, d, sc, n);
+  VX_HIP(hipGetLastError());
+}
+
+// In-place i <-> bitrev(i) swap (cuda/circle_ntt.cu:234-244).
+__global__ void k_bitrev_m31(uint32_t* d, uint32_t log_n, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t j = log_n ? (__brevll(i) >> (64 - log_n)) : 0;
+  if (i < j) {
+    uint32_t t = d[i];
+    d[i] = d[j];
+    d[j] = t;
+  }
+}
+void bit_reverse_m31(uint32_t* d, uint32_t log_n, hipStream_t s) {
+  require(log_n <= 31, "bit_reverse: log_n too large");
+  uint64_t n = 1ull << log_n;
+  hipLaunchKernelGGL(k_bitrev_m31, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, d, log_n, n);
+  VX_HIP(hipGetLastError());
+}
+
+// Out-of-place QM31 (4 words per element, AoS) bit reverse
+// (cuda/bit_reverse_wide.cu:21-52).
+__global__ void k_bitrev_qm31(const uint4* in, uint4* out, uint32_t log_n, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t j = log_n ? (__brevll(i) >> (64 - log_n)) : 0;
+  out[j] = in[i];
+}
+void bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint32_t log_n, hipStream_t s) {
+  require(log_n <= 31 && in != out, "bit_reverse_qm31: bad arguments");
+  uint64_t n = 1ull << log_n;
+  hipLaunchKernelGGL(k_bitrev_qm31, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, (const uint4*)in,
+                     (uint4*)out, log_n, n);
+  VX_HIP(hipGetLastError());
+}
+
+// dst[i] = i < src_n ? src[i] : 0 (cuda/constraint.cu:81-90).
+__global__ void k_zero_pad(const uint32_t* src, uint32_t* dst, uint64_t src_n, uint64_t dst_n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < dst_n) dst[i] = i < src_n ? src[i] : 0u;
+}
+void zero_pad(const uint32_t* src, uint32_t* dst, uint64_t src_n, uint64_t dst_n, hipStream_t s) {
+  if (dst_n == 0) return;
+  hipLaunchKernelGGL(k_zero_pad, dim3((uint32_t)((dst_n + 255) / 256)), dim3(256), 0, s, src, dst, src_n,
+                     dst_n);
+  VX_HIP(hipGetLastError());
+}
+
+}  // namespace vx

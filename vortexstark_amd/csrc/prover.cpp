@@ -1,0 +1,357 @@
+// Fibonacci Circle-STARK prover driver (src/prover.rs:172-619 counterpart).
+//
+// The transcript order is the reference's (prover.rs:321-557):
+//   commit trace LDE -> mix root, draw alpha -> commit quotient -> mix root,
+//   draw alpha_0 -> circle fold, commit layer 0 -> mix -> { draw alpha_k,
+//   line fold, commit, mix } down to 8 values -> mix_felts(last layer) ->
+//   [PoW: grind, mix_u64(nonce)] -> 80 query draws -> decommit.
+// Everything, transcript included, runs on one HIP stream; the only host
+// transfer is the finished KRAK proof.
+#include "prover.h"
+
+#include <cstring>
+#include <string>
+
+namespace vx {
+
+static constexpr uint32_t KRAK_MAGIC = 0x4B52414Bu;
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? ERR_ALLOC : ERR_HIP;
+    throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+DeviceBuf::DeviceBuf(size_t n) : bytes(n) {
+  if (n) VX_HIP(hipMalloc(&p, n));
+}
+DeviceBuf::~DeviceBuf() {
+  if (p) (void)hipFree(p);
+}
+DeviceBuf& DeviceBuf::operator=(DeviceBuf&& o) noexcept {
+  if (this != &o) {
+    if (p) (void)hipFree(p);
+    p = o.p;
+    bytes = o.bytes;
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  return *this;
+}
+
+ProofLayout::ProofLayout(uint32_t log_n_, uint32_t blowup_bits_, uint32_t nq_, uint32_t pow_bits_)
+    : log_n(log_n_), blowup_bits(blowup_bits_), nq(nq_), pow_bits(pow_bits_) {
+  le = log_n + blowup_bits;
+  n_fri = le - 3;
+  header_words = pow_bits ? 9 : 5;
+  w_troot = header_words;
+  w_qroot = w_troot + 8;
+  w_nfri = w_qroot + 8;
+  w_froot0 = w_nfri + 1;
+  w_last = w_froot0 + 8ull * n_fri;
+  w_nq = w_last + 1 + 32;
+  w_tdec = w_nq + 1 + nq;
+  w_qdec = w_tdec + 1 + 2ull * nq + 2ull * nq * (1 + 8ull * le);
+  w_nfdec = w_qdec + 1 + 8ull * nq + 2ull * nq * (1 + 8ull * le);
+  uint64_t w = w_nfdec + 1;
+  for (uint32_t k = 0; k < n_fri; k++) {
+    w_fdec.push_back(w);
+    uint32_t L = le - 1 - k;
+    w += 1 + 8ull * nq + 2ull * nq * (1 + 8ull * L);
+  }
+  total_words = w;
+}
+
+size_t proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits) {
+  return ProofLayout(log_n, blowup_bits, nq, pow_bits).total_words * 4;
+}
+
+// src/circle.rs:284-347: coset points, circle twiddles = y of the first half,
+// line layer l = even x of layer l-1's squashed x (2x^2 - 1).
+TwiddleSet::TwiddleSet(const coset& c, bool inverse, hipStream_t s) {
+  const uint32_t L = c.log_size;
+  const uint64_t n = 1ull << L;
+  line = DeviceBuf(4 * (n > 2 ? n : 2));
+  circle = DeviceBuf(4 * (n / 2 > 0 ? n / 2 : 1));
+  DeviceBuf xs(4 * n), ys(4 * n), tmp(4 * (n / 2 > 0 ? n / 2 : 1));
+  coset_points(c.init, c.step, xs.u32(), ys.u32(), n, s);
+  VX_HIP(hipMemcpyAsync(circle.p, ys.p, 4 * (n / 2), hipMemcpyDeviceToDevice, s));
+  uint64_t off = 0, cur_n = n;
+  uint32_t* cur = xs.u32();
+  uint32_t* nxt = tmp.u32();
+  for (uint32_t l = 0; l + 1 < L; l++) {
+    uint64_t half = cur_n / 2;
+    ref.off[l] = (uint32_t)off;
+    extract_and_squash(cur, line.u32() + off, nxt, half, s);
+    std::swap(cur, nxt);
+    off += half;
+    cur_n = half;
+  }
+  if (inverse) {
+    batch_inverse(line.u32(), line.u32(), off, s);
+    batch_inverse(circle.u32(), circle.u32(), n / 2, s);
+  }
+  VX_HIP(hipStreamSynchronize(s));
+  ref.line = line.u32();
+  ref.circle = circle.u32();
+  ref.log_n = L;
+}
+
+FibProver::FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, hipStream_t stream)
+    : layout_((require(log_n >= 4 && blowup_bits >= 1 && log_n + blowup_bits <= 31, "prover: need log_n >= 4, "
+                                                                                       "blowup >= 1, log_n + blowup <= 31"),
+               log_n),
+              blowup_bits, n_queries, pow_bits),
+      log_n_(log_n),
+      blowup_(blowup_bits),
+      nq_(n_queries),
+      pow_bits_(pow_bits),
+      le_(log_n + blowup_bits),
+      n_fri_(log_n + blowup_bits - 3),
+      stream_(stream) {
+  require(n_queries >= 1 && n_queries <= 65536, "prover: n_queries must be 1..65536");
+  require(pow_bits <= 32, "prover: pow_bits must be <= 32");
+  if (!stream_) {
+    VX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  const uint64_t n = 1ull << log_n_, N = 1ull << le_;
+  trace_ = DeviceBuf(4 * n);
+  eval_ = DeviceBuf(4 * N);
+  device_bytes_ += trace_.bytes + eval_.bytes;
+  for (uint32_t k = 0; k < n_fri_; k++) {
+    uint64_t words = 1ull << (le_ - 1 - k);
+    layers_.emplace_back(4 * 4 * words);
+    layer_col_words_.push_back(words);
+    device_bytes_ += layers_.back().bytes;
+  }
+  for (uint32_t t = 0; t < 2 + n_fri_; t++) {
+    uint32_t L = t < 2 ? le_ : le_ - 1 - (t - 2);
+    uint32_t lo = L < 4 ? L : 4;
+    std::vector<uint64_t> off(MAX_LEVELS, 0);
+    uint64_t w = 0;
+    for (uint32_t l = lo; l <= L; l++) {
+      off[l] = w;
+      w += 8ull << (L - l);
+    }
+    trees_.emplace_back(4 * w);
+    tree_log_.push_back(L);
+    tree_lo_.push_back(lo);
+    tree_off_.push_back(off);
+    device_bytes_ += trees_.back().bytes;
+  }
+  // channel (64 B) | alphas | queries | prefix | nonce
+  const size_t alpha_words = 4ull * (n_fri_ + 2);
+  const size_t query_words = (nq_ + 3) / 4 * 4;  // keeps the prefix/nonce slots 16-byte aligned
+  const size_t small_bytes = 64 + 4 * alpha_words + 4 * query_words + 64 + 64;
+  small_ = DeviceBuf(small_bytes);
+  auto* base = static_cast<uint8_t*>(small_.p);
+  d_channel_ = reinterpret_cast<Channel*>(base);
+  d_alphas_ = reinterpret_cast<uint32_t*>(base + 64);
+  d_queries_ = d_alphas_ + alpha_words;
+  d_prefix_ = reinterpret_cast<uint32_t*>(base + 64 + 4 * alpha_words + 4 * query_words);
+  d_nonce_ = reinterpret_cast<uint64_t*>(reinterpret_cast<uint8_t*>(d_prefix_) + 64);
+  proof_ = DeviceBuf(layout_.total_words * 4);
+  VX_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_proof_), layout_.total_words * 4, hipHostMallocDefault));
+  device_bytes_ += small_.bytes + proof_.bytes;
+  for (auto& e : ev_) VX_HIP(hipEventCreate(&e));
+  build_tables();
+  build_descs();
+}
+
+FibProver::~FibProver() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (host_proof_) (void)hipHostFree(host_proof_);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+void FibProver::build_tables() {
+  hipStream_t s = stream_;
+  trace_itw_ = std::make_unique<TwiddleSet>(make_coset(HALF_COSET, log_n_), true, s);
+  eval_tw_ = std::make_unique<TwiddleSet>(eval_domain(le_), false, s);
+  device_bytes_ += trace_itw_->line.bytes + trace_itw_->circle.bytes + eval_tw_->line.bytes + eval_tw_->circle.bytes;
+  // Fold twiddles (src/fri.rs:112-160): layer 0 = circle fold on the eval
+  // domain (inverse y), layer k >= 1 = line fold on half_odds(le - k)
+  // (inverse x); entry i uses the point at bitrev(2i).
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < n_fri_; k++) {
+    fold_tw_off_.push_back(total);
+    total += 1ull << (le_ - 1 - k);
+  }
+  fold_tw_ = DeviceBuf(4 * total);
+  device_bytes_ += fold_tw_.bytes;
+  for (uint32_t k = 0; k < n_fri_; k++) {
+    coset d = k == 0 ? eval_domain(le_) : make_coset(HALF_ODDS, le_ - k);
+    fold_twiddle_sources(d.init, d.step, fold_tw_.u32() + fold_tw_off_[k], 1ull << d.log_size, d.log_size, k == 0, s);
+  }
+  batch_inverse(fold_tw_.u32(), fold_tw_.u32(), total, s);
+  VX_HIP(hipStreamSynchronize(s));
+}
+
+TreeLevels FibProver::tree_levels(uint32_t t) const {
+  TreeLevels T{};
+  T.base = trees_[t].u32();
+  T.log_n = tree_log_[t];
+  T.lo = tree_lo_[t];
+  for (int l = 0; l < MAX_LEVELS; l++) T.off[l] = tree_off_[t][l];
+  return T;
+}
+
+void FibProver::build_descs() {
+  std::vector<DecomDesc> d(2 + n_fri_);
+  for (uint32_t t = 0; t < 2 + n_fri_; t++) {
+    DecomDesc& x = d[t];
+    std::memset(&x, 0, sizeof x);
+    x.log_n = tree_log_[t];
+    x.lo = tree_lo_[t];
+    x.tree = trees_[t].u32();
+    for (int l = 0; l < MAX_LEVELS; l++) x.off[l] = tree_off_[t][l];
+    if (t == 0) {
+      x.kind = DECOM_M31;
+      x.col[0] = eval_.u32();
+      x.shift = 0;
+      x.proof_word = layout_.w_tdec;
+    } else if (t == 1) {
+      x.kind = DECOM_QUOTIENT;
+      x.col[0] = eval_.u32();
+      x.alpha = d_alphas_;
+      x.shift = 0;
+      x.proof_word = layout_.w_qdec;
+    } else {
+      uint32_t k = t - 2;
+      x.kind = DECOM_SOA4;
+      for (int c = 0; c < 4; c++) x.col[c] = layers_[k].u32() + c * layer_col_words_[k];
+      x.shift = k + 1;
+      x.proof_word = layout_.w_fdec[k];
+    }
+  }
+  descs_ = DeviceBuf(sizeof(DecomDesc) * d.size());
+  VX_HIP(hipMemcpy(descs_.p, d.data(), sizeof(DecomDesc) * d.size(), hipMemcpyHostToDevice));
+  device_bytes_ += descs_.bytes;
+}
+
+void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
+  require(a < P && b < P, "prove: public inputs must be canonical M31");
+  hipStream_t s = stream_;
+  timed_ = timed;
+  const ProofLayout& L = layout_;
+  uint32_t* proof = proof_.u32();
+  const uint64_t n = 1ull << log_n_, N = 1ull << le_;
+  auto mark = [&](int i) {
+    if (timed) VX_HIP(hipEventRecord(ev_[i], s));
+  };
+  auto alpha = [&](uint32_t i) { return d_alphas_ + 4 * i; };
+  auto layer_col = [&](uint32_t k, int c) { return layers_[k].u32() + c * layer_col_words_[k]; };
+
+  mark(0);
+  std::vector<uint32_t> hw = {0, KRAK_MAGIC, 1, pow_bits_ ? 2u : 1u, 2, log_n_, 3, a, 4, b};
+  if (pow_bits_) {
+    hw.insert(hw.end(), {5, blowup_, 6, pow_bits_});
+  }
+  hw.insert(hw.end(), {(uint32_t)L.w_nfri, n_fri_, (uint32_t)L.w_nfdec, n_fri_});
+  write_header(proof, hw.data(), (uint32_t)hw.size() / 2, s);
+  fib_trace(a, b, trace_.u32(), log_n_, s);
+  mark(1);
+  ntt_inverse(trace_.u32(), trace_itw_->ref, s);
+  mark(2);
+  ntt_forward(eval_.u32(), trace_.u32(), n - 1, eval_tw_->ref, log_n_, s);
+  mark(3);
+
+  LeafSrc lt{};
+  lt.kind = LEAF_COLS;
+  lt.n_cols = 1;
+  lt.col[0] = eval_.u32();
+  merkle_commit(lt, tree_levels(0), tree_lo_[0], proof + L.w_troot, s, timed ? ev_[10] : nullptr,
+                timed ? ev_[11] : nullptr);
+  channel_step(d_channel_, true, proof + L.w_troot, alpha(0), nullptr, s);
+  mark(4);
+
+  LeafSrc lq{};
+  lq.kind = LEAF_QUOTIENT;
+  lq.col[0] = eval_.u32();
+  lq.alpha_q = alpha(0);
+  lq.src_mask = N - 1;
+  merkle_commit(lq, tree_levels(1), tree_lo_[1], proof + L.w_qroot, s);
+  channel_step(d_channel_, false, proof + L.w_qroot, alpha(1), nullptr, s);
+  mark(5);
+
+  LeafSrc lc{};
+  lc.kind = LEAF_FOLD_CIRCLE_Q;
+  lc.col[0] = eval_.u32();
+  lc.alpha_q = alpha(0);
+  lc.alpha_f = alpha(1);
+  lc.tw = fold_tw_.u32() + fold_tw_off_[0];
+  lc.src_mask = N - 1;
+  for (int c = 0; c < 4; c++) lc.out[c] = layer_col(0, c);
+  merkle_commit(lc, tree_levels(2), tree_lo_[2], proof + L.w_froot0, s);
+  channel_step(d_channel_, false, proof + L.w_froot0, n_fri_ > 1 ? alpha(2) : nullptr, nullptr, s);
+  for (uint32_t k = 1; k < n_fri_; k++) {
+    LeafSrc ll{};
+    ll.kind = LEAF_FOLD_LINE;
+    for (int c = 0; c < 4; c++) {
+      ll.col[c] = layer_col(k - 1, c);
+      ll.out[c] = layer_col(k, c);
+    }
+    ll.tw = fold_tw_.u32() + fold_tw_off_[k];
+    ll.alpha_f = alpha(k + 1);
+    uint32_t* root = proof + L.w_froot0 + 8ull * k;
+    merkle_commit(ll, tree_levels(2 + k), tree_lo_[2 + k], root, s);
+    channel_step(d_channel_, false, root, k + 1 < n_fri_ ? alpha(k + 2) : nullptr, nullptr, s);
+  }
+  mark(6);
+
+  QueryArgs q{};
+  q.ch = d_channel_;
+  for (int c = 0; c < 4; c++) q.last[c] = layer_col(n_fri_ - 1, c);
+  q.pow_bits = pow_bits_;
+  q.nonce = d_nonce_;
+  q.nq = nq_;
+  q.domain = N;
+  q.queries = d_queries_;
+  q.proof = proof;
+  q.last_word = L.w_last;
+  q.query_word = L.w_nq;
+  q.nonce_word = pow_bits_ ? 7 : 0;
+  channel_mix_last(q, d_prefix_, s);
+  if (pow_bits_) grind_pow(d_prefix_, pow_bits_, d_nonce_, s);
+  channel_draw_queries(q, s);
+  mark(7);
+  decommit(reinterpret_cast<const DecomDesc*>(descs_.p), 2 + n_fri_, d_queries_, nq_, proof, s);
+  mark(8);
+  VX_HIP(hipMemcpyAsync(host_proof_, proof, layout_.total_words * 4, hipMemcpyDeviceToHost, s));
+  mark(9);
+}
+
+const uint8_t* FibProver::wait(size_t* len) {
+  VX_HIP(hipStreamSynchronize(stream_));
+  if (len) *len = layout_.total_words * 4;
+  return host_proof_;
+}
+
+StageTimes FibProver::last_times() {
+  StageTimes t{};
+  if (!timed_) return t;
+  VX_HIP(hipEventSynchronize(ev_[9]));
+  auto el = [&](int a, int b) {
+    float ms = 0;
+    VX_HIP(hipEventElapsedTime(&ms, ev_[a], ev_[b]));
+    return ms;
+  };
+  t.trace_ms = el(0, 1);
+  t.interpolate_ms = el(1, 2);
+  t.lde_ms = el(2, 3);
+  t.trace_commit_ms = el(3, 4);
+  t.quotient_commit_ms = el(4, 5);
+  t.fri_ms = el(5, 6);
+  t.queries_ms = el(6, 7);
+  t.decommit_ms = el(7, 8);
+  t.download_ms = el(8, 9);
+  t.total_ms = el(0, 9);
+  t.trace_subtree_ms = el(10, 11);
+  return t;
+}
+
+}  // namespace vx

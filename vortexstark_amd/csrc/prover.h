@@ -1,0 +1,110 @@
+// Host-side Fibonacci prover over the gfx950 kernels: the C++ counterpart of
+// src/prover.rs (prove / prove_cached / ProverCache). One FibProver owns the
+// twiddles, every intermediate buffer and the KRAK output buffer for one
+// (log_n, blowup, queries, pow) size class, so repeated proofs allocate
+// nothing and run as a single stream of kernels ending in one D2H copy.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "vx_internal.h"
+
+namespace vx {
+
+struct DeviceBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DeviceBuf() = default;
+  explicit DeviceBuf(size_t n);
+  ~DeviceBuf();
+  DeviceBuf(const DeviceBuf&) = delete;
+  DeviceBuf& operator=(const DeviceBuf&) = delete;
+  DeviceBuf(DeviceBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DeviceBuf& operator=(DeviceBuf&& o) noexcept;
+  uint32_t* u32() const { return static_cast<uint32_t*>(p); }
+};
+
+// Word offsets of every variable field of a KRAK proof
+// (src/bin/stark_cli.rs:1092-1120).
+struct ProofLayout {
+  uint32_t log_n, blowup_bits, nq, pow_bits, le, n_fri;
+  uint64_t header_words, w_troot, w_qroot, w_nfri, w_froot0, w_last, w_nq, w_tdec, w_qdec, w_nfdec;
+  std::vector<uint64_t> w_fdec;
+  uint64_t total_words;
+  ProofLayout(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits);
+};
+
+// Reference-layout twiddle tables of one coset, resident on the device.
+struct TwiddleSet {
+  DeviceBuf line, circle;
+  TwiddleRef ref{};
+  // forward tables of `c`; `inverse` replaces them with their inverses
+  TwiddleSet(const coset& c, bool inverse, hipStream_t s);
+};
+
+struct StageTimes {
+  float trace_ms, interpolate_ms, lde_ms, trace_commit_ms, quotient_commit_ms, fri_ms, queries_ms, decommit_ms,
+      download_ms, total_ms, trace_subtree_ms;
+};
+
+class FibProver {
+ public:
+  FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, hipStream_t stream);
+  ~FibProver();
+  // Enqueue one proof of Fib(a, b) on the stream (no host synchronisation).
+  void enqueue(uint32_t a, uint32_t b, bool timed = false);
+  // Wait for the last enqueued proof; returns its KRAK bytes.
+  const uint8_t* wait(size_t* len);
+  size_t proof_bytes() const { return layout_.total_words * 4; }
+  const ProofLayout& layout() const { return layout_; }
+  hipStream_t stream() const { return stream_; }
+  uint64_t device_bytes() const { return device_bytes_; }
+  StageTimes last_times();
+
+  // device buffers for tests / introspection
+  const uint32_t* d_eval() const { return eval_.u32(); }
+  const uint32_t* d_trace() const { return trace_.u32(); }
+  const uint32_t* d_proof() const { return proof_.u32(); }
+
+ private:
+  void build_tables();
+  void build_descs();
+  TreeLevels tree_levels(uint32_t idx) const;
+
+  ProofLayout layout_;
+  uint32_t log_n_, blowup_, nq_, pow_bits_, le_, n_fri_;
+  hipStream_t stream_;
+  bool own_stream_ = false;
+  uint64_t device_bytes_ = 0;
+
+  std::unique_ptr<TwiddleSet> trace_itw_, eval_tw_;
+  DeviceBuf fold_tw_;
+  std::vector<uint64_t> fold_tw_off_;
+  DeviceBuf trace_, eval_;
+  std::vector<DeviceBuf> layers_;  // FRI layer k: 4 SoA columns of 2^(le-1-k)
+  std::vector<uint64_t> layer_col_words_;
+  std::vector<DeviceBuf> trees_;   // 0 = trace, 1 = quotient, 2 + k = FRI layer k
+  std::vector<uint32_t> tree_log_, tree_lo_;
+  std::vector<std::vector<uint64_t>> tree_off_;
+  DeviceBuf small_;                // channel | alphas | queries | prefix | nonce
+  Channel* d_channel_ = nullptr;
+  uint32_t* d_alphas_ = nullptr;
+  uint32_t* d_queries_ = nullptr;
+  uint32_t* d_prefix_ = nullptr;
+  uint64_t* d_nonce_ = nullptr;
+  DeviceBuf descs_;
+  DeviceBuf proof_;
+  uint8_t* host_proof_ = nullptr;
+  hipEvent_t ev_[12] = {};
+  bool timed_ = false;
+};
+
+// Host verifier (src/verifier.rs:39-310) over KRAK bytes; returns true and
+// leaves `err` empty when the proof is accepted.
+bool verify_proof(const uint8_t* bytes, size_t len, uint32_t blowup_bits, uint32_t n_queries, std::string& err);
+size_t proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits);
+
+}  // namespace vx

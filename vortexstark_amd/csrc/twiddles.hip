@@ -1,0 +1,100 @@
+// Twiddle construction for the circle NTT and FRI folds.
+//
+// Reference behaviour: cuda/twiddles.cu:23-52 (fold twiddle sources),
+// :80-105 (coset points), :127-142 (extract_and_squash), and
+// cuda/batch_inverse.cu:8-41 (Montgomery batch inverse with inverse(0) = 0).
+// All values are exact, so the chunking below gives identical results.
+#include "vx_internal.h"
+
+namespace vx {
+
+// p_i = init * step^i. Each thread squares-and-multiplies once for its first
+// point and then steps by `step` for the rest of a 16-point run.
+static constexpr uint32_t PTS_PER_THREAD = 16;
+
+__global__ void k_coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n) {
+  uint64_t first = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * PTS_PER_THREAD;
+  if (first >= n) return;
+  cpt p = c_mul(init, c_pow(step, first));
+  uint64_t end = first + PTS_PER_THREAD < n ? first + PTS_PER_THREAD : n;
+  for (uint64_t i = first; i < end; i++) {
+    if (xs) xs[i] = p.x;
+    if (ys) ys[i] = p.y;
+    p = c_mul(p, step);
+  }
+}
+void coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  uint64_t threads = (n + PTS_PER_THREAD - 1) / PTS_PER_THREAD;
+  hipLaunchKernelGGL(k_coset_points, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, init, step, xs,
+                     ys, n);
+  VX_HIP(hipGetLastError());
+}
+
+// tw[i] = x[2i]; sq[i] = 2 x[2i]^2 - 1.
+__global__ void k_extract_and_squash(const uint32_t* in, uint32_t* tw, uint32_t* sq, uint64_t half_n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= half_n) return;
+  uint32_t x = in[2 * i];
+  if (tw) tw[i] = x;
+  uint32_t x2 = m31_mul(x, x);
+  sq[i] = m31_sub(m31_add(x2, x2), 1u);
+}
+void extract_and_squash(const uint32_t* in, uint32_t* tw, uint32_t* sq, uint64_t half_n, hipStream_t s) {
+  if (half_n == 0) return;
+  hipLaunchKernelGGL(k_extract_and_squash, dim3((uint32_t)((half_n + 255) / 256)), dim3(256), 0, s, in, tw, sq,
+                     half_n);
+  VX_HIP(hipGetLastError());
+}
+
+// Batch inverse, 16 elements per thread: prefix products over the non-zero
+// entries, one Fermat inversion, then unwind. Zeros map to zero.
+static constexpr uint32_t INV_CHUNK = 16;
+__global__ void k_batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n) {
+  uint64_t start = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * INV_CHUNK;
+  if (start >= n) return;
+  uint32_t cnt = (uint32_t)(n - start < INV_CHUNK ? n - start : INV_CHUNK);
+  uint32_t v[INV_CHUNK], pre[INV_CHUNK];
+  uint32_t acc = 1;
+#pragma unroll
+  for (uint32_t i = 0; i < INV_CHUNK; i++) {
+    v[i] = i < cnt ? in[start + i] : 0u;
+    pre[i] = acc;  // product of the non-zero entries before i
+    if (v[i] != 0) acc = m31_mul(acc, v[i]);
+  }
+  uint32_t inv = m31_inv(acc);  // acc >= 1 is never zero
+#pragma unroll
+  for (int i = INV_CHUNK - 1; i >= 0; i--) {
+    uint32_t r = 0;
+    if (v[i] != 0) {
+      r = m31_mul(inv, pre[i]);
+      inv = m31_mul(inv, v[i]);
+    }
+    if ((uint32_t)i < cnt) out[start + i] = r;
+  }
+}
+void batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  uint64_t threads = (n + INV_CHUNK - 1) / INV_CHUNK;
+  hipLaunchKernelGGL(k_batch_inverse, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, in, out, n);
+  VX_HIP(hipGetLastError());
+}
+
+// out[i] = coord(init * step^(bitrev_log_n(2i))) for i < n/2.
+__global__ void k_fold_tw_sources(cpt init, cpt step, uint32_t* out, uint64_t half_n, uint32_t log_n, int use_y) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= half_n) return;
+  uint64_t idx = __brevll(i << 1) >> (64 - log_n);
+  cpt p = c_mul(init, c_pow(step, idx));
+  out[i] = use_y ? p.y : p.x;
+}
+void fold_twiddle_sources(cpt init, cpt step, uint32_t* out, uint64_t n, uint32_t log_n, bool extract_y,
+                          hipStream_t s) {
+  require(log_n >= 1 && log_n <= 31 && n == (1ull << log_n), "fold_twiddle_sources: n must be 2^log_n");
+  uint64_t half = n / 2;
+  hipLaunchKernelGGL(k_fold_tw_sources, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, s, init, step, out,
+                     half, log_n, extract_y ? 1 : 0);
+  VX_HIP(hipGetLastError());
+}
+
+}  // namespace vx

@@ -1,0 +1,174 @@
+// Internal interfaces between the gfx950 kernel translation units and the
+// C++ host layer (prover, C-ABI shims). Not part of the public C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "vx_channel.h"
+#include "vx_circle.h"
+#include "vx_field.h"
+
+namespace vx {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+enum Status : int {
+  OK = 0,
+  ERR_ARG = 1,      // invalid argument (shape, size, null pointer)
+  ERR_HIP = 2,      // HIP runtime error
+  ERR_ALLOC = 3,    // device allocation failed
+  ERR_VERIFY = 4,   // proof rejected
+  ERR_INTERNAL = 5,
+};
+void hip_check(hipError_t e, const char* what);
+#define VX_HIP(call) ::vx::hip_check((call), #call)
+inline void require(bool cond, const char* what) {
+  if (!cond) throw Error(ERR_ARG, what);
+}
+// Last error message of the calling thread (C-ABI vx_last_error).
+void set_last_error(const std::string& msg);
+
+// --------------------------------------------------------- NTT (ntt.hip)
+// Reference-layout twiddle tables of a coset (src/circle.rs:284-347):
+//   line: layer l occupies [off[l], off[l] + (N >> (l+1))), circle: N/2 y-coords.
+struct TwiddleRef {
+  const uint32_t* line;
+  const uint32_t* circle;
+  uint32_t off[32];
+  uint32_t log_n;
+};
+
+// Multi-bit butterfly pass plan entry.
+struct NttPass {
+  uint32_t j0;     // lowest bit of the pass
+  uint32_t k;      // number of bits
+  uint32_t cbits;  // log2(columns per tile)
+};
+
+// Runs the twisted circle NTT over all log_n bits, in the reference layer
+// order (cuda/circle_ntt.cu:248-312), using fused multi-bit LDS passes.
+//   forward: evaluate; src (may alias data) is read with index mask
+//            src_mask, which replicates a shorter coefficient vector into
+//            every block (LDE, the first log_n - log(src) layers are copies).
+//            Only bits below `eff_bits` are transformed.
+//   inverse: interpolate, then scale by 2^((30 log_n) mod 31).
+void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
+                 uint32_t eff_bits, hipStream_t s);
+void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s);
+// One reference layer (cuda/circle_ntt.cu:89-117).
+void ntt_layer(uint32_t* data, const uint32_t* tw, uint32_t layer_idx, uint64_t n, bool forward,
+               hipStream_t s);
+void m31_scale(uint32_t* data, uint32_t scale, uint64_t n, hipStream_t s);
+void bit_reverse_m31(uint32_t* data, uint32_t log_n, hipStream_t s);
+void bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint32_t log_n, hipStream_t s);
+void zero_pad(const uint32_t* src, uint32_t* dst, uint64_t src_n, uint64_t dst_n, hipStream_t s);
+
+// ----------------------------------------------- twiddles (twiddles.hip)
+void coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n, hipStream_t s);
+void extract_and_squash(const uint32_t* in, uint32_t* tw_out, uint32_t* sq_out, uint64_t half_n,
+                        hipStream_t s);
+void batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s);
+void fold_twiddle_sources(cpt init, cpt step, uint32_t* out, uint64_t n, uint32_t log_n,
+                          bool extract_y, hipStream_t s);
+
+// ------------------------------------------------- misc kernels (fri.hip)
+void fib_trace(uint32_t a, uint32_t b, uint32_t* out, uint32_t log_n, hipStream_t s);
+void fib_quotient(const uint32_t* trace, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3,
+                  qm31 alpha, uint64_t offset, uint64_t chunk_n, uint64_t global_n, hipStream_t s);
+void fold_line(const uint32_t* const in[4], const uint32_t* tw, uint32_t* const out[4], qm31 alpha,
+               uint64_t half_n, hipStream_t s);
+void fold_circle_into_line(uint32_t* const dst[4], const uint32_t* const src[4], const uint32_t* tw,
+                           qm31 alpha, qm31 alpha_sq, uint64_t half_n, hipStream_t s);
+
+// --------------------------------------------------- Merkle (merkle.hip)
+constexpr int MAX_LEVELS = 33;
+// Stored levels [lo, log_n] of one Merkle tree; level l has 2^(log_n - l)
+// nodes of 8 words at base + off[l].
+struct TreeLevels {
+  uint32_t* base;
+  uint32_t log_n;
+  uint32_t lo;
+  uint64_t off[MAX_LEVELS];
+};
+// Leaf producers fused into the subtree kernel.
+enum LeafKind : uint32_t {
+  LEAF_COLS = 0,          // n_cols (1..16) M31 columns, leaf = Blake2s(row words)
+  LEAF_QUOTIENT = 1,      // Fibonacci quotient of an eval column (cuda/constraint.cu:10-36)
+  LEAF_FOLD_CIRCLE_Q = 2, // circle fold of that quotient (cuda/fri.cu:92-126), writes the layer
+  LEAF_FOLD_LINE = 3,     // line fold of a QM31 SoA layer (cuda/fri.cu:58-87), writes the layer
+};
+struct LeafSrc {
+  uint32_t kind;
+  uint32_t n_cols;
+  const uint32_t* col[16];   // LEAF_COLS: columns; QUOTIENT/FOLD_CIRCLE_Q: col[0] = eval;
+                             // FOLD_LINE: col[0..3] = previous layer
+  uint32_t* out[4];          // fold outputs (the new layer, SoA)
+  const uint32_t* tw;        // fold twiddles
+  const uint32_t* alpha_q;   // device QM31: quotient alpha
+  const uint32_t* alpha_f;   // device QM31: fold alpha
+  uint64_t src_mask;         // eval length - 1 (quotient wraparound)
+};
+// Hashes all 2^log_n leaves and stores tree levels [store_lo, log_n] into
+// `tl` (store_lo >= tl.lo). If root_out is non-null the root is copied there.
+// Optional events are recorded right before and after the subtree kernel.
+void merkle_commit(const LeafSrc& leaf, const TreeLevels& tl, uint32_t store_lo, uint32_t* root_out,
+                   hipStream_t s, hipEvent_t sub_begin = nullptr, hipEvent_t sub_end = nullptr);
+// Tiled commit (cuda/blake2s.cu:305-532): roots of each 2^tile_log-leaf tile.
+void merkle_tiled(const LeafSrc& leaf, uint32_t log_n, uint32_t tile_log, uint32_t* subtree_roots,
+                  hipStream_t s);
+void merkle_leaf_hashes(const LeafSrc& leaf, uint64_t n_leaves, uint32_t* hashes, hipStream_t s);
+void merkle_hash_nodes(const uint32_t* children, uint32_t* parents, uint64_t n_parents, hipStream_t s);
+void merkle_reduce_to_root(const uint32_t* nodes, uint32_t* root, uint32_t n_nodes, hipStream_t s);
+void merkle_root_small(const LeafSrc& leaf, uint32_t log_n, uint32_t* root_out, hipStream_t s);
+void merkle_leaf_pairs(const LeafSrc& leaf, uint64_t n_leaves, uint32_t* parents, hipStream_t s);
+
+// Decommitment of one tree at the query positions (src/prover.rs:551-618),
+// written straight into the KRAK byte buffer.
+enum DecomKind : uint32_t { DECOM_M31 = 0, DECOM_SOA4 = 1, DECOM_QUOTIENT = 2 };
+struct DecomDesc {
+  uint32_t kind;
+  uint32_t log_n;
+  uint32_t lo;           // lowest stored tree level
+  uint32_t shift;        // leaf index = query >> shift
+  const uint32_t* col[4];
+  const uint32_t* alpha; // DECOM_QUOTIENT
+  const uint32_t* tree;
+  uint64_t off[MAX_LEVELS];
+  uint64_t proof_word;   // word offset of this decommitment ("n") in the proof
+};
+void decommit(const DecomDesc* d_descs, uint32_t n_trees, const uint32_t* d_queries, uint32_t nq,
+              uint32_t* proof, hipStream_t s);
+
+// ------------------------------------------------- channel (channel.hip)
+// Single-lane transcript steps over a device-resident Channel.
+void channel_step(Channel* ch, bool init, const uint32_t* digest, uint32_t* alpha_out,
+                  uint32_t* alpha_sq_out, hipStream_t s);
+struct QueryArgs {
+  Channel* ch;
+  const uint32_t* last[4];  // last FRI layer, SoA, 8 values
+  uint32_t pow_bits;
+  const uint64_t* nonce;    // device, when pow_bits > 0
+  uint32_t nq;
+  uint64_t domain;          // eval domain size
+  uint32_t* queries;        // out, nq indices
+  uint32_t* proof;          // KRAK buffer
+  uint64_t last_word;       // word offset of n_last
+  uint64_t query_word;      // word offset of n_queries
+  uint64_t nonce_word;      // word offset of nonce (v2) or 0
+};
+void channel_mix_last(const QueryArgs& q, uint32_t* prefix_out, hipStream_t s);
+void channel_draw_queries(const QueryArgs& q, hipStream_t s);
+// Deterministic PoW: smallest nonce >= 0 (linear order) whose Blake2s has
+// >= pow_bits trailing zeros (src/channel.rs:140-168).
+void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, hipStream_t s);
+void grind_pow_batch(const uint32_t* d_prefix, uint64_t* d_result, uint32_t pow_bits,
+                     uint64_t batch_offset, uint64_t n_nonces, hipStream_t s);
+void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_t s);
+
+}  // namespace vx
