@@ -44,6 +44,20 @@ const char *vx_last_error(void);
 /* Replaces cuda_device_sync (cuda/fields.cu:44-46; ffi.rs). */
 int vx_device_sync(void);
 
+/* Runtime helpers replacing the CUDA runtime bindings the host side uses
+ * (src/cuda/ffi.rs:6-28: cudaMalloc, cudaFree, cudaMemcpy[Async],
+ * cudaMemset, cudaStreamCreate/Synchronize/Destroy), so callers share this
+ * library's HIP runtime. kind: 1 = host->device, 2 = device->host,
+ * 3 = device->device (cudaMemcpyKind / hipMemcpyKind values). */
+int vx_malloc(void **d_ptr, size_t bytes);
+int vx_free(void *d_ptr);
+int vx_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int vx_memcpy_async(void *dst, const void *src, size_t bytes, int kind, vx_stream_t stream);
+int vx_memset(void *d_ptr, int value, size_t bytes);
+int vx_stream_create(vx_stream_t *out);
+int vx_stream_synchronize(vx_stream_t stream);
+int vx_stream_destroy(vx_stream_t stream);
+
 /* ------------------------------------------------------------------ NTT */
 
 /* Replaces cuda_circle_ntt_evaluate (src/cuda/ffi.rs:261-269,

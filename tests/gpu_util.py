@@ -1,58 +1,70 @@
-"""Device-buffer plumbing for the GPU parity tests (torch allocates, the
-vx_* C-ABI computes)."""
+"""Device-buffer plumbing for the GPU parity tests.
+
+Buffers are allocated through the library's own runtime helpers (vx_malloc /
+vx_memcpy, the C-ABI's counterparts of the reference's cudaMalloc/cudaMemcpy
+bindings), so a test process holds exactly one HIP runtime: the one
+libvortex_hip.so links."""
 import ctypes as C
 
 import numpy as np
 
-_torch = None
+import vortexstark_amd as vx
+from vortexstark_amd._lib import call
+
+H2D, D2H = 1, 2
 
 
-def torch():
-    global _torch
-    if _torch is None:
-        import torch as t
+class DevBuf:
+    def __init__(self, words):
+        self.words = int(words)
+        self._p = C.c_void_p()
+        call("vx_malloc", C.byref(self._p), max(4 * self.words, 4))
+        self.ptr = self._p.value
 
-        if not t.cuda.is_available():
-            raise RuntimeError("no HIP device visible")
-        _torch = t
-    return _torch
+    def data_ptr(self):
+        return self.ptr
+
+    def __del__(self):
+        try:
+            if self._p:
+                vx.load().vx_free(self._p)
+                self._p = C.c_void_p()
+        except Exception:
+            pass
 
 
 def dev(arr):
-    """numpy uint32 -> device int32 tensor (same bits)."""
-    t = torch()
+    """numpy uint32 -> device buffer (same bits)."""
     a = np.ascontiguousarray(np.asarray(arr, dtype=np.uint32))
-    return t.from_numpy(a.view(np.int32).copy()).to("cuda")
+    b = DevBuf(a.size)
+    if a.size:
+        call("vx_memcpy", C.c_void_p(b.ptr), C.c_void_p(a.ctypes.data), a.nbytes, H2D)
+    return b
 
 
 def empty(n):
-    return torch().zeros(int(n), dtype=torch().int32, device="cuda")
+    b = DevBuf(n)
+    call("vx_memset", C.c_void_p(b.ptr), 0, 4 * b.words)
+    return b
 
 
-def host(t):
-    torch().cuda.synchronize()
-    return t.cpu().numpy().view(np.uint32).copy()
+def host(b):
+    out = np.zeros(b.words, np.uint32)
+    if b.words:
+        call("vx_memcpy", C.c_void_p(out.ctypes.data), C.c_void_p(b.ptr), out.nbytes, D2H)
+    return out
 
 
-def ptr(t):
-    return C.c_void_p(t.data_ptr())
+def ptr(b):
+    return C.c_void_p(b.ptr)
 
 
 def sync():
-    torch().cuda.synchronize()
-
-
-_hip = None
+    call("vx_device_sync")
 
 
 def d2h(dptr, words):
     """Copy `words` uint32 from a raw device pointer (library-owned buffers)."""
-    global _hip
-    if _hip is None:
-        _hip = C.CDLL("libamdhip64.so")
-        _hip.hipMemcpy.restype = C.c_int
-        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     out = np.zeros(int(words), np.uint32)
-    rc = _hip.hipMemcpy(out.ctypes.data, C.c_void_p(dptr), out.nbytes, 2)  # hipMemcpyDeviceToHost
-    assert rc == 0, f"hipMemcpy failed: {rc}"
+    call("vx_memcpy", C.c_void_p(out.ctypes.data), C.c_void_p(dptr), out.nbytes, D2H)
     return out

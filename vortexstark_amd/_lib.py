@@ -21,6 +21,14 @@ u8p = C.POINTER(C.c_uint8)
 SIGNATURES = {
     "vx_last_error": (C.c_char_p, []),
     "vx_device_sync": (ip, []),
+    "vx_malloc": (ip, [C.POINTER(vp), sz]),
+    "vx_free": (ip, [vp]),
+    "vx_memcpy": (ip, [vp, vp, sz, ip]),
+    "vx_memcpy_async": (ip, [vp, vp, sz, ip, vp]),
+    "vx_memset": (ip, [vp, ip, sz]),
+    "vx_stream_create": (ip, [C.POINTER(vp)]),
+    "vx_stream_synchronize": (ip, [vp]),
+    "vx_stream_destroy": (ip, [vp]),
     "vx_circle_ntt_evaluate": (ip, [vp, vp, vp, vp, vp, u32, u32, vp]),
     "vx_circle_ntt_interpolate": (ip, [vp, vp, vp, vp, vp, u32, u32, vp]),
     "vx_circle_ntt_evaluate_batch": (ip, [vp, vp, vp, vp, vp, u32, u32, u32, vp]),

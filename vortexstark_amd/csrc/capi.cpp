@@ -68,6 +68,61 @@ int vx_device_sync(void) {
   return guard([&] { VX_HIP(hipDeviceSynchronize()); });
 }
 
+int vx_malloc(void** d, size_t bytes) {
+  return guard([&] {
+    require(d != nullptr, "malloc: null out");
+    *d = nullptr;
+    VX_HIP(hipMalloc(d, bytes));
+  });
+}
+
+int vx_free(void* d) {
+  return guard([&] { VX_HIP(hipFree(d)); });
+}
+
+static hipMemcpyKind copy_kind(int kind) {
+  require(kind >= 1 && kind <= 3, "memcpy: kind must be 1 (H2D), 2 (D2H) or 3 (D2D)");
+  return static_cast<hipMemcpyKind>(kind);
+}
+
+int vx_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  return guard([&] {
+    require((dst && src) || bytes == 0, "memcpy: null pointer");
+    VX_HIP(hipMemcpy(dst, src, bytes, copy_kind(kind)));
+  });
+}
+
+int vx_memcpy_async(void* dst, const void* src, size_t bytes, int kind, vx_stream_t st) {
+  return guard([&] {
+    require((dst && src) || bytes == 0, "memcpy: null pointer");
+    VX_HIP(hipMemcpyAsync(dst, src, bytes, copy_kind(kind), S(st)));
+  });
+}
+
+int vx_memset(void* d, int value, size_t bytes) {
+  return guard([&] {
+    require(d != nullptr || bytes == 0, "memset: null pointer");
+    VX_HIP(hipMemset(d, value, bytes));
+  });
+}
+
+int vx_stream_create(vx_stream_t* out) {
+  return guard([&] {
+    require(out != nullptr, "stream_create: null out");
+    hipStream_t s = nullptr;
+    VX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = reinterpret_cast<vx_stream_t>(s);
+  });
+}
+
+int vx_stream_synchronize(vx_stream_t st) {
+  return guard([&] { VX_HIP(hipStreamSynchronize(S(st))); });
+}
+
+int vx_stream_destroy(vx_stream_t st) {
+  return guard([&] { VX_HIP(hipStreamDestroy(S(st))); });
+}
+
 int vx_circle_ntt_evaluate(uint32_t* d, const uint32_t* tw, const uint32_t* ctw, const uint32_t* offs,
                            const uint32_t* sizes, uint32_t nl, uint32_t n, vx_stream_t st) {
   return guard([&] {
