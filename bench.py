@@ -80,6 +80,22 @@ def cpu_baseline(log_n_target, sample_log_n):
     }
 
 
+def copy_calibration(vx, mib=256):
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    counters scale the per-kernel HBM traffic reported from rocprofv3."""
+    import ctypes as C
+
+    src, dst = C.c_void_p(), C.c_void_p()
+    nbytes = mib << 20
+    vx._lib.call("vx_malloc", C.byref(src), nbytes)
+    vx._lib.call("vx_malloc", C.byref(dst), nbytes)
+    vx._lib.call("vx_memset", src, 1, nbytes)
+    vx._lib.call("vx_memcpy", dst, src, nbytes, 3)
+    vx._lib.call("vx_device_sync")
+    vx._lib.call("vx_free", src)
+    vx._lib.call("vx_free", dst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +105,12 @@ def main():
     ap.add_argument("--blowup-bits", type=int, default=2)
     ap.add_argument("--pow-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: one 256 MiB device copy (counter calibration), "
+                         "1 warmup + 2 timed proofs, no CPU baseline")
     args = ap.parse_args()
+    if args.profile:
+        args.steps, args.warmup, args.no_cpu_baseline = 2, 1, True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
@@ -98,6 +119,8 @@ def main():
     import vortexstark_amd as vx
 
     vx.load()
+    if args.profile:
+        copy_calibration(vx)
     log_n, bb, pw = args.log_n, args.blowup_bits, args.pow_bits
     a0, a1 = 1, fib_a1(42)
     prover = vx.Prover(log_n, bb, vx.N_QUERIES, pw)

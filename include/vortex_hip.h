@@ -224,9 +224,10 @@ int vx_prover_wait(vx_prover *p, uint8_t *out, size_t out_cap, size_t *out_len);
 int vx_prover_stage_times(vx_prover *p, float out[11]);
 /* Device bytes held by the prover (admission accounting). */
 uint64_t vx_prover_device_bytes(const vx_prover *p);
-/* Device buffers of the last proof, for inspection: which = 0 trace/
+/* Buffers of the last proof, for inspection: which = 0 trace/
  * coefficients (2^log_n), 1 LDE evaluation (2^log_eval), 2 + k FRI layer k
- * (4 SoA columns back to back), 100 the KRAK proof buffer. */
+ * (4 SoA columns back to back), 100 the KRAK proof buffer, 101 the pinned
+ * HOST copy of the last proof (valid after vx_prover_wait). */
 int vx_prover_buffer(const vx_prover *p, int which, const uint32_t **d_ptr, uint64_t *words);
 
 /* prover::prove counterpart (src/prover.rs:221-223): one-shot proof. */

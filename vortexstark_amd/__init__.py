@@ -33,20 +33,20 @@ class Prover:
         self._h = C.c_void_p()
         _lib.call("vx_prover_create", log_n, blowup_bits, n_queries, pow_bits, stream, C.byref(self._h))
         self.size = proof_size(log_n, blowup_bits, n_queries, pow_bits)
-        self._buf = (C.c_uint8 * self.size)()
 
     def prove(self, a, b):
-        n = C.c_size_t()
-        _lib.call("vx_prover_prove", self._h, a, b, self._buf, self.size, C.byref(n))
-        return bytes(self._buf[:n.value])
+        self.enqueue(a, b)
+        return self.wait()
 
     def enqueue(self, a, b, timed=False):
         _lib.call("vx_prover_enqueue", self._h, a, b, 1 if timed else 0)
 
     def wait(self):
+        """Wait for the last enqueued proof; one copy out of pinned memory."""
         n = C.c_size_t()
-        _lib.call("vx_prover_wait", self._h, self._buf, self.size, C.byref(n))
-        return bytes(self._buf[:n.value])
+        _lib.call("vx_prover_wait", self._h, None, 0, C.byref(n))
+        ptr, _ = self.buffer(101)
+        return C.string_at(ptr, n.value)
 
     def stage_times(self):
         out = (C.c_float * 11)()
@@ -78,10 +78,10 @@ class Prover:
 def prove(a, b, log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0):
     """prover::prove (src/prover.rs:221): one-shot proof bytes."""
     size = proof_size(log_n, blowup_bits, n_queries, pow_bits)
-    buf = (C.c_uint8 * size)()
+    buf = C.create_string_buffer(size)
     n = C.c_size_t()
     _lib.call("vx_prove", a, b, log_n, blowup_bits, n_queries, pow_bits, buf, size, C.byref(n))
-    return bytes(buf[:n.value])
+    return buf.raw[:n.value]
 
 
 def verify(proof, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES):

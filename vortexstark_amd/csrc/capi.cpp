@@ -468,6 +468,12 @@ int vx_prover_buffer(const vx_prover* p, int which, const uint32_t** ptr, uint64
     } else if (which == 100) {
       *ptr = p->impl.d_proof();
       *words = L.total_words;
+    } else if (which == 101) {
+      *ptr = reinterpret_cast<const uint32_t*>(p->impl.host_proof());
+      *words = L.total_words;
+    } else if (which >= 2 && (uint32_t)(which - 2) < L.n_fri) {
+      *ptr = p->impl.d_layer(which - 2);
+      *words = 4ull << (L.le - 1 - (which - 2));
     } else {
       require(false, "prover_buffer: unknown buffer");
     }

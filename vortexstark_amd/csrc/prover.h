@@ -68,6 +68,8 @@ class FibProver {
   const uint32_t* d_eval() const { return eval_.u32(); }
   const uint32_t* d_trace() const { return trace_.u32(); }
   const uint32_t* d_proof() const { return proof_.u32(); }
+  const uint8_t* host_proof() const { return host_proof_; }
+  const uint32_t* d_layer(uint32_t k) const { return layers_[k].u32(); }
 
  private:
   void build_tables();
