@@ -4,29 +4,45 @@
 // leaf = Blake2s(row words, t = 4 * n_cols), node = Blake2s(left || right,
 // t = 64), leaves in natural order, parent j = H(2j, 2j+1).
 //
-// MI355X design. One workgroup commits a 4096-leaf tile: each of 256 lanes
-// hashes 16 consecutive leaves and folds them to a level-4 node entirely in
-// VGPRs (31 compressions, state + message in registers), then levels 5..12
-// are merged through LDS. Only levels >= 4 are written to HBM (about 4 bytes
-// per leaf), which is all the decommitment needs: it re-hashes the 16-leaf
-// group around each query. The leaf words can be produced on the fly by the
-// fused producers below, so the Fibonacci quotient is never materialised
-// and each FRI fold writes its layer once while hashing it:
+// MI355X design. A 256-lane workgroup commits one tile of leaves: each lane
+// hashes 2^LPT consecutive leaves and folds them to one node entirely in
+// VGPRs (state and message in registers), then the tile's upper levels are
+// merged through LDS. Big trees (>= 2^21 leaves) use 16 leaves per lane and
+// 4096-leaf tiles (throughput); smaller ones use 4 leaves per lane and
+// 1024-leaf tiles, which cuts a workgroup's dependent chain from 39 to 15
+// compressions (latency). Only levels >= 4 are written to HBM (about 4 bytes
+// per leaf); the decommitment re-hashes the 16-leaf group around a query.
+// Leaf words can be produced on the fly by fused producers, so the Fibonacci
+// quotient is never materialised and each FRI fold writes its layer once
+// while hashing it:
 //   LEAF_QUOTIENT       q_i = alpha (e[i+2] - e[i+1] - e[i])  (cuda/constraint.cu:10-36)
 //   LEAF_FOLD_CIRCLE_Q  circle fold of that quotient           (cuda/fri.cu:92-126)
 //   LEAF_FOLD_LINE      line fold of the previous layer        (cuda/fri.cu:58-87)
-// Blake2s is VALU-bound here; loads are 16-byte vectors and hide under it.
+// The loads of the next 4-leaf group are issued before the current group is
+// hashed. The kernel that produces a root also runs the transcript step
+// (mix the root, draw the next alpha), so no separate channel launch is
+// needed. FRI layers of <= 2^10 values are committed by one workgroup in a
+// single launch (k_fri_tail). Blake2s is VALU-bound here.
 #include "vx_internal.h"
 
 namespace vx {
 
 static constexpr uint32_t SUB_THREADS = 256;
-static constexpr uint32_t SUB_LPT_LOG = 4;   // leaves per lane (log)
-static constexpr uint32_t SUB_TILE_LOG = 12; // leaves per workgroup (log)
-static constexpr uint32_t RED_NLEV_MAX = 9;  // tree levels per reduce workgroup
+static constexpr uint32_t BIG_TREE_LOG = 21;  // >= this: 16 leaves per lane, 4096-leaf tiles
+static constexpr uint32_t MAX_TILE_LOG = 12;
+static constexpr uint32_t RED_THREADS = 512;  // tree-top reduce: 1024 nodes, 10 levels per workgroup
+static constexpr uint32_t RED_NLEV_MAX = 10;
+static constexpr uint32_t TAIL_THREADS = 256;
 
 struct G4 {
   uint32_t w[4][4];  // words of 4 consecutive leaves (W <= 4)
+};
+// Raw loads of one 4-leaf group (kept apart from the arithmetic so the next
+// group's loads can be in flight while this group is hashed).
+struct Raw {
+  uint4 a[8];
+  uint4 t;
+  uint32_t h0, h1;
 };
 
 __device__ __forceinline__ uint4 ld4(const uint32_t* p) { return *reinterpret_cast<const uint4*>(p); }
@@ -34,78 +50,107 @@ __device__ __forceinline__ void st4(uint32_t* p, uint32_t a, uint32_t b, uint32_
   *reinterpret_cast<uint4*>(p) = make_uint4(a, b, c, d);
 }
 __device__ __forceinline__ qm31 ldq(const uint32_t* p) { return {{p[0], p[1], p[2], p[3]}}; }
+__device__ __forceinline__ void u4w(const uint4& v, uint32_t* w) {
+  w[0] = v.x;
+  w[1] = v.y;
+  w[2] = v.z;
+  w[3] = v.w;
+}
 
-// Fibonacci constraint for 8 consecutive positions from e[0..10).
+// Fibonacci constraint at position k of a window e[] (cuda/constraint.cu:22-30).
 __device__ __forceinline__ uint32_t fib_c(const uint32_t* e, int k) {
   return m31_sub(e[k + 2], m31_add(e[k + 1], e[k]));
 }
 
-// Produce the words of leaves [i, i+4) (and write fold outputs).
 template <uint32_t KIND, uint32_t NC>
-__device__ __forceinline__ void leaf_group(const LeafSrc& L, uint64_t i, G4& g) {
+__device__ __forceinline__ void leaf_load(const LeafSrc& L, uint64_t i, Raw& r) {
   if constexpr (KIND == LEAF_COLS && NC == 1) {
-    uint4 v = ld4(L.col[0] + i);
-    g.w[0][0] = v.x;
-    g.w[1][0] = v.y;
-    g.w[2][0] = v.z;
-    g.w[3][0] = v.w;
+    r.a[0] = ld4(L.col[0] + i);
+  } else if constexpr (KIND == LEAF_COLS && NC == 4) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) r.a[c] = ld4(L.col[c] + i);
+  } else if constexpr (KIND == LEAF_QUOTIENT) {
+    r.a[0] = ld4(L.col[0] + i);
+    r.h0 = L.col[0][(i + 4) & L.src_mask];
+    r.h1 = L.col[0][(i + 5) & L.src_mask];
+  } else if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) {
+    const uint64_t b = 2 * i;
+    r.a[0] = ld4(L.col[0] + b);
+    r.a[1] = ld4(L.col[0] + b + 4);
+    r.h0 = L.col[0][(b + 8) & L.src_mask];
+    r.h1 = L.col[0][(b + 9) & L.src_mask];
+    r.t = ld4(L.tw + i);
+  } else if constexpr (KIND == LEAF_FOLD_LINE) {
+    const uint64_t b = 2 * i;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      r.a[2 * c] = ld4(L.col[c] + b);
+      r.a[2 * c + 1] = ld4(L.col[c] + b + 4);
+    }
+    r.t = ld4(L.tw + i);
+  }
+}
+
+// Words of leaves [i, i+4) from their raw loads (fold producers also store
+// the new layer).
+template <uint32_t KIND, uint32_t NC>
+__device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const Raw& r, const qm31& aq,
+                                             const qm31& af, G4& g) {
+  if constexpr (KIND == LEAF_COLS && NC == 1) {
+    g.w[0][0] = r.a[0].x;
+    g.w[1][0] = r.a[0].y;
+    g.w[2][0] = r.a[0].z;
+    g.w[3][0] = r.a[0].w;
   } else if constexpr (KIND == LEAF_COLS && NC == 4) {
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      uint4 v = ld4(L.col[c] + i);
-      g.w[0][c] = v.x;
-      g.w[1][c] = v.y;
-      g.w[2][c] = v.z;
-      g.w[3][c] = v.w;
+      g.w[0][c] = r.a[c].x;
+      g.w[1][c] = r.a[c].y;
+      g.w[2][c] = r.a[c].z;
+      g.w[3][c] = r.a[c].w;
     }
   } else if constexpr (KIND == LEAF_QUOTIENT) {
-    const uint32_t* ev = L.col[0];
-    uint4 v = ld4(ev + i);
-    uint32_t e[6] = {v.x, v.y, v.z, v.w, ev[(i + 4) & L.src_mask], ev[(i + 5) & L.src_mask]};
-    qm31 a = ldq(L.alpha_q);
+    uint32_t e[6];
+    u4w(r.a[0], e);
+    e[4] = r.h0;
+    e[5] = r.h1;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      qm31 q = qm_scale(a, fib_c(e, j));
+      qm31 q = qm_scale(aq, fib_c(e, j));
 #pragma unroll
       for (int c = 0; c < 4; c++) g.w[j][c] = q.v[c];
     }
   } else if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) {
-    const uint32_t* ev = L.col[0];
-    const uint64_t b = 2 * i;
-    uint4 v0 = ld4(ev + b), v1 = ld4(ev + b + 4);
-    uint32_t e[10] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, ev[(b + 8) & L.src_mask],
-                      ev[(b + 9) & L.src_mask]};
-    qm31 aq = ldq(L.alpha_q), af = ldq(L.alpha_f);
-    uint4 tw = ld4(L.tw + i);
-    uint32_t t[4] = {tw.x, tw.y, tw.z, tw.w};
+    uint32_t e[10], t[4];
+    u4w(r.a[0], e);
+    u4w(r.a[1], e + 4);
+    e[8] = r.h0;
+    e[9] = r.h1;
+    u4w(r.t, t);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       qm31 f0 = qm_scale(aq, fib_c(e, 2 * j)), f1 = qm_scale(aq, fib_c(e, 2 * j + 1));
-      qm31 r = fri_fold_pair(f0, f1, af, t[j]);
+      qm31 o = fri_fold_pair(f0, f1, af, t[j]);
 #pragma unroll
-      for (int c = 0; c < 4; c++) g.w[j][c] = r.v[c];
+      for (int c = 0; c < 4; c++) g.w[j][c] = o.v[c];
     }
 #pragma unroll
     for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
   } else if constexpr (KIND == LEAF_FOLD_LINE) {
-    const uint64_t b = 2 * i;
-    uint32_t in[4][8];
+    uint32_t in[4][8], t[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      uint4 v0 = ld4(L.col[c] + b), v1 = ld4(L.col[c] + b + 4);
-      in[c][0] = v0.x; in[c][1] = v0.y; in[c][2] = v0.z; in[c][3] = v0.w;
-      in[c][4] = v1.x; in[c][5] = v1.y; in[c][6] = v1.z; in[c][7] = v1.w;
+      u4w(r.a[2 * c], in[c]);
+      u4w(r.a[2 * c + 1], in[c] + 4);
     }
-    qm31 af = ldq(L.alpha_f);
-    uint4 tw = ld4(L.tw + i);
-    uint32_t t[4] = {tw.x, tw.y, tw.z, tw.w};
+    u4w(r.t, t);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       qm31 f0 = {{in[0][2 * j], in[1][2 * j], in[2][2 * j], in[3][2 * j]}};
       qm31 f1 = {{in[0][2 * j + 1], in[1][2 * j + 1], in[2][2 * j + 1], in[3][2 * j + 1]}};
-      qm31 r = fri_fold_pair(f0, f1, af, t[j]);
+      qm31 o = fri_fold_pair(f0, f1, af, t[j]);
 #pragma unroll
-      for (int c = 0; c < 4; c++) g.w[j][c] = r.v[c];
+      for (int c = 0; c < 4; c++) g.w[j][c] = o.v[c];
     }
 #pragma unroll
     for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
@@ -136,12 +181,22 @@ __device__ __forceinline__ void cp8(uint32_t d[8], const uint32_t s[8]) {
 template <uint32_t KIND, uint32_t NC, uint32_t LPT_LOG>
 __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, uint32_t out[8]) {
   static_assert(LPT_LOG >= 2 && LPT_LOG <= 4, "lane subtree covers 4..16 leaves");
+  constexpr bool kGeneric = KIND == LEAF_COLS && NC == 0;
+  constexpr uint32_t G = 1u << (LPT_LOG - 2);
+  qm31 aq{}, af{};
+  if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
+  if constexpr (KIND == LEAF_FOLD_CIRCLE_Q || KIND == LEAF_FOLD_LINE) af = ldq(L.alpha_f);
+  Raw cur, nxt;
+  if constexpr (!kGeneric) leaf_load<KIND, NC>(L, first, cur);
   uint32_t s2[8], s3[8];
 #pragma unroll
-  for (uint32_t gi = 0; gi < (1u << (LPT_LOG - 2)); gi++) {
+  for (uint32_t gi = 0; gi < G; gi++) {
     const uint64_t i = first + 4 * gi;
     G4 g;
-    if constexpr (!(KIND == LEAF_COLS && NC == 0)) leaf_group<KIND, NC>(L, i, g);
+    if constexpr (!kGeneric) {
+      if (gi + 1 < G) leaf_load<KIND, NC>(L, i + 4, nxt);
+      leaf_produce<KIND, NC>(L, i, cur, aq, af, g);
+    }
     uint32_t a[8], b[8], n1a[8], n1b[8], n2[8];
     leaf_hash<KIND, NC>(L, g, i, 0, a);
     leaf_hash<KIND, NC>(L, g, i, 1, b);
@@ -166,6 +221,9 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
         }
       }
     }
+    if constexpr (!kGeneric) {
+      if (gi + 1 < G) cur = nxt;
+    }
   }
 }
 
@@ -181,6 +239,24 @@ __device__ __forceinline__ void lds_get(const uint4* s, uint32_t idx, uint32_t n
   uint4 a = s[2 * idx], b = s[2 * idx + 1];
   n[0] = a.x; n[1] = a.y; n[2] = a.z; n[3] = a.w;
   n[4] = b.x; n[5] = b.y; n[6] = b.z; n[7] = b.w;
+}
+
+// Transcript step fused into the kernel that produced a root
+// (src/prover.rs:321-334 and the mix/draw pairs that follow each commit).
+__device__ __forceinline__ void finish_root(const RootHook& h, const uint32_t node[8]) {
+  if (h.root_out)
+    for (int k = 0; k < 8; k++) h.root_out[k] = node[k];
+  if (h.ch) {
+    Channel c;
+    if (h.init) ch_init(c);
+    else c = *h.ch;
+    ch_mix_digest(c, node);
+    if (h.alpha_out) {
+      qm31 a = ch_draw_felt(c);
+      for (int k = 0; k < 4; k++) h.alpha_out[k] = a.v[k];
+    }
+    *h.ch = c;
+  }
 }
 
 // Merges `nthr` nodes held one per lane in `buf[0]` up to one node, storing
@@ -209,9 +285,12 @@ __device__ __forceinline__ void lds_merge(uint4 (*buf)[2 * CAP], uint32_t nthr, 
   }
 }
 
-template <uint32_t KIND, uint32_t NC, uint32_t LPT_LOG>
+// ROOT instances (the whole tree in one workgroup) carry the transcript step;
+// the throughput instances do not, which keeps their VGPR count (and so their
+// occupancy) at what the hashing itself needs.
+template <uint32_t KIND, uint32_t NC, uint32_t LPT_LOG, bool ROOT>
 __global__ __launch_bounds__(SUB_THREADS) void k_subtree(LeafSrc L, TreeLevels T, uint32_t tile_log,
-                                                         uint32_t store_lo, uint32_t* root_out) {
+                                                         uint32_t store_lo, RootHook hook) {
   __shared__ uint4 buf[2][2 * SUB_THREADS];
   const uint32_t nthr = 1u << (tile_log - LPT_LOG);
   const uint32_t tid = threadIdx.x;
@@ -226,16 +305,17 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(LeafSrc L, TreeLevels T
   }
   __syncthreads();
   lds_merge<SUB_THREADS>(buf, nthr, LPT_LOG, T, tile, tile_log, store_lo, node);
-  if (root_out && tid == 0 && tile_log == T.log_n) {
-    for (int k = 0; k < 8; k++) root_out[k] = node[k];
+  if constexpr (ROOT) {
+    if (tid == 0) finish_root(hook, node);
   }
 }
 
 // Upper tree levels: each workgroup hashes 2^nlev level-`from` nodes down to
 // one node, storing levels from+1 .. from+nlev.
-__global__ __launch_bounds__(SUB_THREADS) void k_tree_reduce(TreeLevels T, uint32_t from, uint32_t nlev,
-                                                             uint32_t* root_out) {
-  __shared__ uint4 buf[2][2 * SUB_THREADS];
+template <bool ROOT>
+__global__ __launch_bounds__(RED_THREADS) void k_tree_reduce(TreeLevels T, uint32_t from, uint32_t nlev,
+                                                             RootHook hook) {
+  __shared__ uint4 buf[2][2 * RED_THREADS];
   const uint32_t tid = threadIdx.x;
   const uint32_t nthr = 1u << (nlev - 1);
   const uint64_t wg = blockIdx.x;
@@ -252,27 +332,39 @@ __global__ __launch_bounds__(SUB_THREADS) void k_tree_reduce(TreeLevels T, uint3
     store_node(T.base + T.off[from + 1] + 8 * ((wg << (nlev - 1)) + tid), node);
   }
   __syncthreads();
-  lds_merge<SUB_THREADS>(buf, nthr, from + 1, T, wg, from + nlev, 0, node);
-  if (root_out && tid == 0 && from + nlev == T.log_n) {
-    for (int k = 0; k < 8; k++) root_out[k] = node[k];
+  lds_merge<RED_THREADS>(buf, nthr, from + 1, T, wg, from + nlev, 0, node);
+  if constexpr (ROOT) {
+    if (tid == 0) finish_root(hook, node);
   }
+}
+
+// Leaves per lane (log) for a tile of 2^tile_log leaves: 256 lanes when the
+// tile allows it, between 4 and 16 leaves each.
+static uint32_t lane_log_for(uint32_t tile_log) {
+  uint32_t l = tile_log > 8 ? tile_log - 8 : 0;
+  return l < 2 ? 2 : (l > 4 ? 4 : l);
+}
+
+template <uint32_t KIND, uint32_t NC, uint32_t LPT>
+static void launch_subtree_lpt(const LeafSrc& L, const TreeLevels& T, uint32_t tile_log, uint32_t store_lo,
+                               const RootHook& hook, hipStream_t s) {
+  const uint64_t grid = 1ull << (T.log_n - tile_log);
+  if (tile_log == T.log_n)
+    hipLaunchKernelGGL((k_subtree<KIND, NC, LPT, true>), dim3((uint32_t)grid), dim3(SUB_THREADS), 0, s, L, T,
+                       tile_log, store_lo, hook);
+  else
+    hipLaunchKernelGGL((k_subtree<KIND, NC, LPT, false>), dim3((uint32_t)grid), dim3(SUB_THREADS), 0, s, L, T,
+                       tile_log, store_lo, hook);
 }
 
 template <uint32_t KIND, uint32_t NC>
 static void launch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t tile_log, uint32_t store_lo,
-                           uint32_t* root_out, hipStream_t s) {
-  const uint64_t grid = 1ull << (T.log_n - tile_log);
-  const uint32_t lpt = tile_log < SUB_LPT_LOG ? tile_log : SUB_LPT_LOG;
-  require(lpt >= 3 || (KIND == LEAF_COLS && lpt >= 2), "merkle: tree too small");
-  if (lpt == 4)
-    hipLaunchKernelGGL((k_subtree<KIND, NC, 4>), dim3((uint32_t)grid), dim3(SUB_THREADS), 0, s, L, T, tile_log,
-                       store_lo, root_out);
-  else if (lpt == 3)
-    hipLaunchKernelGGL((k_subtree<KIND, NC, 3>), dim3((uint32_t)grid), dim3(SUB_THREADS), 0, s, L, T, tile_log,
-                       store_lo, root_out);
-  else
-    hipLaunchKernelGGL((k_subtree<KIND, NC, 2>), dim3((uint32_t)grid), dim3(SUB_THREADS), 0, s, L, T, tile_log,
-                       store_lo, root_out);
+                           const RootHook& hook, hipStream_t s) {
+  const uint32_t lpt = lane_log_for(tile_log);
+  require(tile_log >= lpt && (tile_log - lpt) <= 8, "merkle: tile shape out of range");
+  if (lpt == 4) launch_subtree_lpt<KIND, NC, 4>(L, T, tile_log, store_lo, hook, s);
+  else if (lpt == 3) launch_subtree_lpt<KIND, NC, 3>(L, T, tile_log, store_lo, hook, s);
+  else launch_subtree_lpt<KIND, NC, 2>(L, T, tile_log, store_lo, hook, s);
   VX_HIP(hipGetLastError());
 }
 
@@ -301,34 +393,38 @@ static void check_leaf(const LeafSrc& L, uint32_t log_n) {
 }
 
 static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t tile_log, uint32_t store_lo,
-                             uint32_t* root_out, hipStream_t s) {
+                             const RootHook& hook, hipStream_t s) {
   switch (L.kind) {
     case LEAF_COLS:
-      if (L.n_cols == 1) launch_subtree<LEAF_COLS, 1>(L, T, tile_log, store_lo, root_out, s);
-      else if (L.n_cols == 4) launch_subtree<LEAF_COLS, 4>(L, T, tile_log, store_lo, root_out, s);
-      else launch_subtree<LEAF_COLS, 0>(L, T, tile_log, store_lo, root_out, s);
+      if (L.n_cols == 1) launch_subtree<LEAF_COLS, 1>(L, T, tile_log, store_lo, hook, s);
+      else if (L.n_cols == 4) launch_subtree<LEAF_COLS, 4>(L, T, tile_log, store_lo, hook, s);
+      else launch_subtree<LEAF_COLS, 0>(L, T, tile_log, store_lo, hook, s);
       break;
-    case LEAF_QUOTIENT: launch_subtree<LEAF_QUOTIENT, 4>(L, T, tile_log, store_lo, root_out, s); break;
-    case LEAF_FOLD_CIRCLE_Q: launch_subtree<LEAF_FOLD_CIRCLE_Q, 4>(L, T, tile_log, store_lo, root_out, s); break;
-    case LEAF_FOLD_LINE: launch_subtree<LEAF_FOLD_LINE, 4>(L, T, tile_log, store_lo, root_out, s); break;
+    case LEAF_QUOTIENT: launch_subtree<LEAF_QUOTIENT, 4>(L, T, tile_log, store_lo, hook, s); break;
+    case LEAF_FOLD_CIRCLE_Q: launch_subtree<LEAF_FOLD_CIRCLE_Q, 4>(L, T, tile_log, store_lo, hook, s); break;
+    case LEAF_FOLD_LINE: launch_subtree<LEAF_FOLD_LINE, 4>(L, T, tile_log, store_lo, hook, s); break;
   }
 }
 
-void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, uint32_t* root_out, hipStream_t s,
+void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, const RootHook& hook, hipStream_t s,
                    hipEvent_t sub_begin, hipEvent_t sub_end) {
   check_leaf(L, T.log_n);
   require(T.base != nullptr, "merkle_commit: null tree buffer");
-  const uint32_t tile_log = T.log_n < SUB_TILE_LOG ? T.log_n : SUB_TILE_LOG;
-  const uint32_t lpt = tile_log < SUB_LPT_LOG ? tile_log : SUB_LPT_LOG;
-  require(T.lo <= lpt && store_lo >= T.lo && store_lo <= lpt, "merkle_commit: stored levels must start at the lane level");
+  const uint32_t tile_log = T.log_n >= BIG_TREE_LOG ? MAX_TILE_LOG : (T.log_n < 10 ? T.log_n : 10);
+  const uint32_t lpt = lane_log_for(tile_log);
+  require(store_lo >= T.lo && store_lo >= lpt && store_lo <= tile_log,
+          "merkle_commit: stored levels must start at or above the lane level");
   if (sub_begin) VX_HIP(hipEventRecord(sub_begin, s));
-  dispatch_subtree(L, T, tile_log, store_lo, root_out, s);
+  dispatch_subtree(L, T, tile_log, store_lo, hook, s);
   if (sub_end) VX_HIP(hipEventRecord(sub_end, s));
   uint32_t from = tile_log;
   while (from < T.log_n) {
     uint32_t nlev = T.log_n - from < RED_NLEV_MAX ? T.log_n - from : RED_NLEV_MAX;
     uint64_t grid = 1ull << (T.log_n - from - nlev);
-    hipLaunchKernelGGL(k_tree_reduce, dim3((uint32_t)grid), dim3(SUB_THREADS), 0, s, T, from, nlev, root_out);
+    if (from + nlev == T.log_n)
+      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((uint32_t)grid), dim3(RED_THREADS), 0, s, T, from, nlev, hook);
+    else
+      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((uint32_t)grid), dim3(RED_THREADS), 0, s, T, from, nlev, hook);
     VX_HIP(hipGetLastError());
     from += nlev;
   }
@@ -336,13 +432,109 @@ void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, uin
 
 void merkle_tiled(const LeafSrc& L, uint32_t log_n, uint32_t tile_log, uint32_t* subtree_roots, hipStream_t s) {
   check_leaf(L, log_n);
-  require(tile_log >= 4 && tile_log <= SUB_TILE_LOG && tile_log <= log_n, "merkle_tiled: tile size out of range");
+  require(tile_log >= 4 && tile_log <= MAX_TILE_LOG && tile_log <= log_n, "merkle_tiled: tile size out of range");
   TreeLevels T{};
   T.base = subtree_roots;
   T.log_n = log_n;
   T.lo = tile_log;
   T.off[tile_log] = 0;
-  dispatch_subtree(L, T, tile_log, tile_log, nullptr, s);
+  dispatch_subtree(L, T, tile_log, tile_log, RootHook{}, s);
+}
+
+// ------------------------------------------------------------ FRI tail
+// FRI line-fold layers of at most 2^TAIL_MAX_LOG values, committed one after
+// another by a single workgroup: fold (previous layer from LDS), leaf hashes,
+// tree levels in LDS (levels >= lo stored for the decommitment), root into
+// the proof, then mix the root and draw the next alpha (src/prover.rs:
+// 527-544). Each layer's dependent chain is about L + 4 compressions.
+__global__ __launch_bounds__(TAIL_THREADS) void k_fri_tail(FriTailArgs a) {
+  __shared__ uint32_t vals[2][4][1u << TAIL_MAX_LOG];
+  __shared__ uint4 hashes[2u << TAIL_MAX_LOG];
+  __shared__ uint32_t s_alpha[4];
+  const uint32_t tid = threadIdx.x;
+  uint32_t cur = 0;
+  for (uint32_t li = 0; li < a.n_layers; li++) {
+    const FriTailLayer& T = a.layer[li];
+    const uint32_t L = T.log_n, n = 1u << L;
+    if (tid == 0)
+      for (int k = 0; k < 4; k++) s_alpha[k] = T.alpha[k];
+    __syncthreads();
+    const qm31 al = {{s_alpha[0], s_alpha[1], s_alpha[2], s_alpha[3]}};
+    for (uint32_t i = tid; i < n; i += TAIL_THREADS) {
+      qm31 f0, f1;
+      if (li == 0) {
+        f0 = {{a.in[0][2 * i], a.in[1][2 * i], a.in[2][2 * i], a.in[3][2 * i]}};
+        f1 = {{a.in[0][2 * i + 1], a.in[1][2 * i + 1], a.in[2][2 * i + 1], a.in[3][2 * i + 1]}};
+      } else {
+        const uint32_t p = cur ^ 1;
+        f0 = {{vals[p][0][2 * i], vals[p][1][2 * i], vals[p][2][2 * i], vals[p][3][2 * i]}};
+        f1 = {{vals[p][0][2 * i + 1], vals[p][1][2 * i + 1], vals[p][2][2 * i + 1], vals[p][3][2 * i + 1]}};
+      }
+      const qm31 r = fri_fold_pair(f0, f1, al, T.tw[i]);
+      for (int c = 0; c < 4; c++) {
+        vals[cur][c][i] = r.v[c];
+        T.out[c][i] = r.v[c];
+      }
+      uint32_t h[8];
+      b2_leaf4(r.v[0], r.v[1], r.v[2], r.v[3], h);
+      lds_put(hashes, i, h);
+    }
+    __syncthreads();
+    for (uint32_t lvl = 1; lvl <= L; lvl++) {
+      const uint32_t cnt = n >> lvl;  // <= 512 = 2 parents per lane
+      const uint32_t i0 = tid, i1 = tid + TAIL_THREADS;
+      uint32_t p0[8], p1[8], l[8], r[8];
+      if (i0 < cnt) {
+        lds_get(hashes, 2 * i0, l);
+        lds_get(hashes, 2 * i0 + 1, r);
+        b2_hash_node(l, r, p0);
+      }
+      if (i1 < cnt) {
+        lds_get(hashes, 2 * i1, l);
+        lds_get(hashes, 2 * i1 + 1, r);
+        b2_hash_node(l, r, p1);
+      }
+      __syncthreads();  // every child read before any parent overwrites it
+      if (i0 < cnt) {
+        lds_put(hashes, i0, p0);
+        if (lvl >= T.lo) store_node(T.tree + T.off[lvl] + 8 * i0, p0);
+      }
+      if (i1 < cnt) {
+        lds_put(hashes, i1, p1);
+        if (lvl >= T.lo) store_node(T.tree + T.off[lvl] + 8 * i1, p1);
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      uint32_t root[8];
+      lds_get(hashes, 0, root);
+      for (int k = 0; k < 8; k++) T.root_out[k] = root[k];
+      Channel c = *a.ch;
+      ch_mix_digest(c, root);
+      if (T.next_alpha) {
+        const qm31 nx = ch_draw_felt(c);
+        for (int k = 0; k < 4; k++) T.next_alpha[k] = nx.v[k];
+      }
+      *a.ch = c;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+void fri_tail(const FriTailArgs& a, hipStream_t s) {
+  require(a.n_layers >= 1 && a.n_layers <= TAIL_MAX_LAYERS && a.ch, "fri_tail: bad layer count");
+  uint32_t prev = 0;
+  for (uint32_t li = 0; li < a.n_layers; li++) {
+    const FriTailLayer& T = a.layer[li];
+    require(T.log_n >= 1 && T.log_n <= TAIL_MAX_LOG && T.lo <= T.log_n && T.tw && T.tree && T.root_out && T.alpha,
+            "fri_tail: bad layer");
+    require(li == 0 || T.log_n + 1 == prev, "fri_tail: layers must halve");
+    for (int c = 0; c < 4; c++) require(T.out[c] != nullptr && (li > 0 || a.in[c] != nullptr), "fri_tail: null column");
+    prev = T.log_n;
+  }
+  hipLaunchKernelGGL(k_fri_tail, dim3(1), dim3(TAIL_THREADS), 0, s, a);
+  VX_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------- reference-API kernels
@@ -415,15 +607,17 @@ void merkle_reduce_to_root(const uint32_t* nodes, uint32_t* root, uint32_t n_nod
   VX_HIP(hipGetLastError());
 }
 
-// Root of a tree of at most 2^SUB_TILE_LOG leaves in one workgroup, nothing
+// Root of a tree of at most 2^MAX_TILE_LOG leaves in one workgroup, nothing
 // stored (cuda_merkle_commit_small_soa4, cuda/blake2s.cu:92-189).
 void merkle_root_small(const LeafSrc& L, uint32_t log_n, uint32_t* root_out, hipStream_t s) {
   check_leaf(L, log_n);
-  require(log_n <= SUB_TILE_LOG && root_out, "merkle_root_small: at most 4096 leaves");
+  require(log_n <= MAX_TILE_LOG && root_out, "merkle_root_small: at most 4096 leaves");
   TreeLevels T{};
   T.base = nullptr;
   T.log_n = log_n;
-  dispatch_subtree(L, T, log_n, MAX_LEVELS, root_out, s);
+  RootHook hook{};
+  hook.root_out = root_out;
+  dispatch_subtree(L, T, log_n, MAX_LEVELS, hook, s);
 }
 
 // parents[j] = H(leaf(2j), leaf(2j+1)) (cuda/blake2s.cu:195-240).

@@ -78,6 +78,112 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(uint32_t* dst, const u
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-radix pass for 16384-element tiles (k + cbits = 14): 512 lanes x
+// 32 elements. A round butterflies up to 5 bits in VGPRs: lane t holds the 32
+// elements whose tile index e has bits [pos, pos+5) = s (register index) and
+// the other 9 bits = t. Between rounds the tile moves through LDS once, with
+// an XOR swizzle that keeps all three access patterns bank-conflict free.
+// For register bit q of a round the twiddle index is
+//   h = (H << (13 - pos - q)) | ((t >> pos) << (4 - q)) | (s0 >> (q + 1)),
+// so a lane needs only the 2^(4-q) consecutive words starting at the first
+// term; they are loaded once per round instead of once per butterfly.
+static constexpr uint32_t RX_THREADS = 512;
+static constexpr uint32_t RX_E_BITS = 14;
+
+struct RadixRound {
+  uint32_t rb0, nb, pos;  // row bits [rb0, rb0 + nb) at register window [pos, pos + 5)
+};
+struct RadixArgs {
+  const uint32_t* tw[16];  // twiddle table of row bit b (global bit j0 + b)
+  uint32_t j0, k, cbits;
+  uint32_t n_rounds;
+  RadixRound round[4];     // in execution order
+  uint64_t src_mask;
+  uint32_t scale;
+};
+
+__device__ __forceinline__ uint32_t rx_own(uint32_t t, uint32_t s, uint32_t pos) {
+  return (t & ((1u << pos) - 1)) | (s << pos) | ((t >> pos) << (pos + 5));
+}
+__device__ __forceinline__ uint32_t rx_swz(uint32_t e) { return e ^ ((e >> 5) & 31u); }
+
+template <bool FWD, int Q>
+__device__ __forceinline__ void rx_bit(uint32_t (&v)[32], const uint32_t* table, uint64_t base) {
+  constexpr uint32_t NW = 1u << (4 - Q);
+  uint32_t w[NW];
+#pragma unroll
+  for (uint32_t m = 0; m < NW; m++) w[m] = table[base + m];
+#pragma unroll
+  for (uint32_t s0 = 0; s0 < 32; s0++) {
+    if (s0 & (1u << Q)) continue;
+    const uint32_t s1 = s0 | (1u << Q);
+    const uint32_t tw = w[s0 >> (Q + 1)];
+    if (FWD) {
+      const uint32_t m = m31_mul(v[s1], tw);
+      v[s1] = m31_sub(v[s0], m);
+      v[s0] = m31_add(v[s0], m);
+    } else {
+      const uint32_t d = m31_sub(v[s0], v[s1]);
+      v[s0] = m31_add(v[s0], v[s1]);
+      v[s1] = m31_mul(d, tw);
+    }
+  }
+}
+
+template <bool FWD>
+__global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const uint32_t* src, RadixArgs a) {
+  __shared__ uint32_t lds[1u << RX_E_BITS];
+  const uint32_t t = threadIdx.x;
+  const uint32_t cmask = (1u << a.cbits) - 1;
+  const uint32_t low_tiles_log = a.j0 - a.cbits;
+  const uint64_t tile = blockIdx.x;
+  const uint64_t low_base = (tile & ((1ull << low_tiles_log) - 1)) << a.cbits;
+  const uint64_t H = tile >> low_tiles_log;
+  const uint64_t hi_base = H << (a.j0 + a.k);
+  uint32_t v[32];
+  for (uint32_t ri = 0; ri < a.n_rounds; ri++) {
+    const RadixRound R = a.round[ri];
+    const uint32_t pos = R.pos;
+    if (ri == 0) {
+#pragma unroll
+      for (uint32_t s = 0; s < 32; s++) {
+        const uint32_t e = rx_own(t, s, pos);
+        const uint64_t gi = hi_base | ((uint64_t)(e >> a.cbits) << a.j0) | (low_base + (e & cmask));
+        v[s] = src[gi & a.src_mask];
+      }
+    } else {
+#pragma unroll
+      for (uint32_t s = 0; s < 32; s++) v[s] = lds[rx_swz(rx_own(t, s, pos))];
+    }
+    const uint32_t qlo = R.rb0 + a.cbits - pos, qhi = qlo + R.nb;  // register bits to butterfly
+#define VX_RX_BIT(Q)                                                                              \
+  if ((Q) >= qlo && (Q) < qhi) {                                                                  \
+    const uint32_t epos = pos + (Q);                                                              \
+    const uint64_t base = (H << (13 - epos)) | ((uint64_t)(t >> pos) << (4 - (Q)));               \
+    rx_bit<FWD, (Q)>(v, a.tw[epos - a.cbits], base);                                              \
+  }
+    if (FWD) {
+      VX_RX_BIT(4) VX_RX_BIT(3) VX_RX_BIT(2) VX_RX_BIT(1) VX_RX_BIT(0)
+    } else {
+      VX_RX_BIT(0) VX_RX_BIT(1) VX_RX_BIT(2) VX_RX_BIT(3) VX_RX_BIT(4)
+    }
+#undef VX_RX_BIT
+    if (ri + 1 < a.n_rounds) {
+#pragma unroll
+      for (uint32_t s = 0; s < 32; s++) lds[rx_swz(rx_own(t, s, pos))] = v[s];
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (uint32_t s = 0; s < 32; s++) {
+        const uint32_t e = rx_own(t, s, pos);
+        const uint64_t gi = hi_base | ((uint64_t)(e >> a.cbits) << a.j0) | (low_base + (e & cmask));
+        dst[gi] = a.scale != 1u ? m31_mul(v[s], a.scale) : v[s];
+      }
+    }
+  }
+}
+
 static const uint32_t* bit_table(const TwiddleRef& tw, uint32_t j) {
   return j == 0 ? tw.circle : tw.line + tw.off[j - 1];
 }
@@ -98,6 +204,33 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
   a.scale = scale;
   const uint32_t e_log = k + cbits;
   const uint64_t grid = 1ull << (log_n - e_log);
+  if (e_log == RX_E_BITS) {
+    RadixArgs r{};
+    for (uint32_t b = 0; b < k; b++) r.tw[b] = a.tw[b];
+    r.j0 = j0;
+    r.k = k;
+    r.cbits = cbits;
+    r.src_mask = src_mask;
+    r.scale = scale;
+    // rounds of <= 5 row bits from the top; evaluate runs them top-down,
+    // interpolate bottom-up
+    RadixRound rr[4];
+    uint32_t nr = 0;
+    for (int32_t hi = (int32_t)k; hi > 0; hi -= 5) {
+      uint32_t rb0 = hi >= 5 ? (uint32_t)hi - 5 : 0;
+      uint32_t nb = (uint32_t)hi - rb0;
+      uint32_t pos = rb0 + cbits < 9 ? rb0 + cbits : 9;
+      rr[nr++] = {rb0, nb, pos};
+    }
+    r.n_rounds = nr;
+    for (uint32_t i = 0; i < nr; i++) r.round[i] = fwd ? rr[i] : rr[nr - 1 - i];
+    if (fwd)
+      hipLaunchKernelGGL(k_ntt_radix<true>, dim3((uint32_t)grid), dim3(RX_THREADS), 0, s, dst, src, r);
+    else
+      hipLaunchKernelGGL(k_ntt_radix<false>, dim3((uint32_t)grid), dim3(RX_THREADS), 0, s, dst, src, r);
+    VX_HIP(hipGetLastError());
+    return;
+  }
   const uint32_t threads = e_log >= 10 ? NTT_THREADS : (e_log >= 7 ? (1u << (e_log - 1)) : 64u);
   const size_t lds = (size_t)4 << e_log;
   if (fwd)

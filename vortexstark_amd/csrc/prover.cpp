@@ -264,9 +264,8 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   lt.kind = LEAF_COLS;
   lt.n_cols = 1;
   lt.col[0] = eval_.u32();
-  merkle_commit(lt, tree_levels(0), tree_lo_[0], proof + L.w_troot, s, timed ? ev_[10] : nullptr,
-                timed ? ev_[11] : nullptr);
-  channel_step(d_channel_, true, proof + L.w_troot, alpha(0), nullptr, s);
+  merkle_commit(lt, tree_levels(0), tree_lo_[0], RootHook{d_channel_, proof + L.w_troot, alpha(0), 1}, s,
+                timed ? ev_[10] : nullptr, timed ? ev_[11] : nullptr);
   mark(4);
 
   LeafSrc lq{};
@@ -274,8 +273,7 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   lq.col[0] = eval_.u32();
   lq.alpha_q = alpha(0);
   lq.src_mask = N - 1;
-  merkle_commit(lq, tree_levels(1), tree_lo_[1], proof + L.w_qroot, s);
-  channel_step(d_channel_, false, proof + L.w_qroot, alpha(1), nullptr, s);
+  merkle_commit(lq, tree_levels(1), tree_lo_[1], RootHook{d_channel_, proof + L.w_qroot, alpha(1), 0}, s);
   mark(5);
 
   LeafSrc lc{};
@@ -286,9 +284,12 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   lc.tw = fold_tw_.u32() + fold_tw_off_[0];
   lc.src_mask = N - 1;
   for (int c = 0; c < 4; c++) lc.out[c] = layer_col(0, c);
-  merkle_commit(lc, tree_levels(2), tree_lo_[2], proof + L.w_froot0, s);
-  channel_step(d_channel_, false, proof + L.w_froot0, n_fri_ > 1 ? alpha(2) : nullptr, nullptr, s);
-  for (uint32_t k = 1; k < n_fri_; k++) {
+  merkle_commit(lc, tree_levels(2), tree_lo_[2],
+                RootHook{d_channel_, proof + L.w_froot0, n_fri_ > 1 ? alpha(2) : nullptr, 0}, s);
+  // Line folds: big layers through the subtree kernels, the last layers of
+  // <= 2^TAIL_MAX_LOG values in one k_fri_tail launch.
+  uint32_t k = 1;
+  for (; k < n_fri_ && le_ - 1 - k > TAIL_MAX_LOG; k++) {
     LeafSrc ll{};
     ll.kind = LEAF_FOLD_LINE;
     for (int c = 0; c < 4; c++) {
@@ -297,9 +298,27 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
     }
     ll.tw = fold_tw_.u32() + fold_tw_off_[k];
     ll.alpha_f = alpha(k + 1);
-    uint32_t* root = proof + L.w_froot0 + 8ull * k;
-    merkle_commit(ll, tree_levels(2 + k), tree_lo_[2 + k], root, s);
-    channel_step(d_channel_, false, root, k + 1 < n_fri_ ? alpha(k + 2) : nullptr, nullptr, s);
+    merkle_commit(ll, tree_levels(2 + k), tree_lo_[2 + k],
+                  RootHook{d_channel_, proof + L.w_froot0 + 8ull * k, k + 1 < n_fri_ ? alpha(k + 2) : nullptr, 0}, s);
+  }
+  if (k < n_fri_) {
+    FriTailArgs ta{};
+    for (int c = 0; c < 4; c++) ta.in[c] = layer_col(k - 1, c);
+    ta.ch = d_channel_;
+    ta.n_layers = n_fri_ - k;
+    for (uint32_t j = k; j < n_fri_; j++) {
+      FriTailLayer& T = ta.layer[j - k];
+      for (int c = 0; c < 4; c++) T.out[c] = layer_col(j, c);
+      T.tw = fold_tw_.u32() + fold_tw_off_[j];
+      T.tree = trees_[2 + j].u32();
+      T.log_n = le_ - 1 - j;
+      T.lo = tree_lo_[2 + j];
+      for (uint32_t l = 0; l <= T.log_n; l++) T.off[l] = (uint32_t)tree_off_[2 + j][l];
+      T.root_out = proof + L.w_froot0 + 8ull * j;
+      T.alpha = alpha(j + 1);
+      T.next_alpha = j + 1 < n_fri_ ? alpha(j + 2) : nullptr;
+    }
+    fri_tail(ta, s);
   }
   mark(6);
 

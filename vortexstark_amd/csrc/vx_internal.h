@@ -114,11 +114,41 @@ struct LeafSrc {
   const uint32_t* alpha_f;   // device QM31: fold alpha
   uint64_t src_mask;         // eval length - 1 (quotient wraparound)
 };
+// What the kernel that produces a root does with it: copy it to root_out,
+// and, with a channel, mix it (after ch_init when `init`) and draw the next
+// alpha into alpha_out.
+struct RootHook {
+  Channel* ch;
+  uint32_t* root_out;
+  uint32_t* alpha_out;
+  uint32_t init;
+};
 // Hashes all 2^log_n leaves and stores tree levels [store_lo, log_n] into
-// `tl` (store_lo >= tl.lo). If root_out is non-null the root is copied there.
-// Optional events are recorded right before and after the subtree kernel.
-void merkle_commit(const LeafSrc& leaf, const TreeLevels& tl, uint32_t store_lo, uint32_t* root_out,
+// `tl` (store_lo >= tl.lo, and >= the lane level: 4 for trees of >= 2^21
+// leaves, else 2). Optional events bracket the subtree kernel.
+void merkle_commit(const LeafSrc& leaf, const TreeLevels& tl, uint32_t store_lo, const RootHook& hook,
                    hipStream_t s, hipEvent_t sub_begin = nullptr, hipEvent_t sub_end = nullptr);
+// FRI line-fold layers of <= 2^TAIL_MAX_LOG values, committed by one
+// workgroup in one launch.
+constexpr uint32_t TAIL_MAX_LOG = 10;
+constexpr uint32_t TAIL_MAX_LAYERS = TAIL_MAX_LOG;
+struct FriTailLayer {
+  uint32_t* out[4];            // layer values (SoA), 2^log_n each
+  const uint32_t* tw;          // fold twiddles of the fold that produces this layer
+  uint32_t* tree;              // stored tree levels [lo, log_n]
+  uint32_t off[TAIL_MAX_LOG + 1];
+  uint32_t* root_out;          // proof slot of the root
+  const uint32_t* alpha;       // device QM31 used by this fold
+  uint32_t* next_alpha;        // where to draw the next fold's alpha (null: none)
+  uint32_t log_n, lo;
+};
+struct FriTailArgs {
+  const uint32_t* in[4];       // layer folded into the first tail layer
+  Channel* ch;
+  uint32_t n_layers;
+  FriTailLayer layer[TAIL_MAX_LAYERS];
+};
+void fri_tail(const FriTailArgs& a, hipStream_t s);
 // Tiled commit (cuda/blake2s.cu:305-532): roots of each 2^tile_log-leaf tile.
 void merkle_tiled(const LeafSrc& leaf, uint32_t log_n, uint32_t tile_log, uint32_t* subtree_roots,
                   hipStream_t s);
