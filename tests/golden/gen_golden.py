@@ -10,7 +10,8 @@ Sources, per section:
   pow          brute-force smallest nonce with hashlib
 
 Run from the repo root:  python tests/golden/gen_golden.py [--big]
-(--big adds the log_n=20 and log_n=24 oracle digests, which take minutes.)
+(--big adds the log_n = 20, 24 and 25 (blowup 1) oracle digests and the
+log_n = 24 PoW-26 digest, which take minutes.)
 """
 import hashlib
 import json
@@ -124,7 +125,7 @@ def main():
     g["workload_seed"] = {"seed": seed, "a0": 1, "a1": a1}
     sizes = [(10, 2, 0), (12, 2, 0), (16, 2, 0), (16, 2, 10), (14, 1, 0)]
     if big:
-        sizes += [(20, 2, 0), (24, 2, 0)]
+        sizes += [(20, 2, 0), (24, 2, 0), (25, 1, 0), (24, 2, 26)]
     g["proofs_c"] = []
     for log_n, bb, pw in sizes:
         data = pyoracle.prove(1, a1, log_n, blowup_bits=bb, pow_bits=pw, omp=True)

@@ -121,6 +121,8 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
       for (int c = 0; c < 4; c++) g.w[j][c] = q.v[c];
     }
   } else if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) {
+    // f_k = aq * c_k, so (f0 + f1) + af * (tw * (f0 - f1))
+    //   = aq * (c0 + c1) + (af * aq) * (tw * (c0 - c1))   (exact field identity)
     uint32_t e[10], t[4];
     u4w(r.a[0], e);
     u4w(r.a[1], e + 4);
@@ -128,9 +130,9 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
     e[9] = r.h1;
     u4w(r.t, t);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      qm31 f0 = qm_scale(aq, fib_c(e, 2 * j)), f1 = qm_scale(aq, fib_c(e, 2 * j + 1));
-      qm31 o = fri_fold_pair(f0, f1, af, t[j]);
+    for (int j = 0; j < 4; j++) {  // `af` holds af * aq here (see lane_subtree)
+      const uint32_t c0 = fib_c(e, 2 * j), c1 = fib_c(e, 2 * j + 1);
+      qm31 o = qm_add(qm_scale(aq, m31_add(c0, c1)), qm_scale(af, m31_mul(t[j], m31_sub(c0, c1))));
 #pragma unroll
       for (int c = 0; c < 4; c++) g.w[j][c] = o.v[c];
     }
@@ -185,7 +187,8 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
   constexpr uint32_t G = 1u << (LPT_LOG - 2);
   qm31 aq{}, af{};
   if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
-  if constexpr (KIND == LEAF_FOLD_CIRCLE_Q || KIND == LEAF_FOLD_LINE) af = ldq(L.alpha_f);
+  if constexpr (KIND == LEAF_FOLD_LINE) af = ldq(L.alpha_f);
+  if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) af = qm_mul(ldq(L.alpha_f), aq);
   Raw cur, nxt;
   if constexpr (!kGeneric) leaf_load<KIND, NC>(L, first, cur);
   uint32_t s2[8], s3[8];
