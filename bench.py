@@ -80,6 +80,98 @@ def cpu_baseline(log_n_target, sample_log_n):
     }
 
 
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def serving_requests(vx, n, seed):
+    """The serving mix: Fibonacci proofs with log_n uniform in 14..22, blowup
+    4, 80 queries, PoW 20; public inputs (1, SplitMix64 stream)."""
+    import random
+
+    rng = random.Random(seed)
+    reqs = []
+    for _ in range(n):
+        log_n = rng.randint(14, 22)
+        reqs.append(vx.request(log_n, a=1, b=rng.randrange(P), blowup_bits=2, pow_bits=20))
+    return reqs
+
+
+def run_server(vx, srv, reqs, arrivals):
+    """Replay the trace; returns latencies (s), throughput, proof digests."""
+    proofs, times = srv.run_trace(reqs, arrivals, keep=True)
+    digests = [hashlib.sha256(p).digest() for p in proofs]
+    sample = proofs[:: max(1, len(proofs) // 16)]
+    verified = all(vx.verify(p, 2)[0] for p in sample)
+    del proofs
+    lat = [t[2] - a for t, a in zip(times, arrivals)]
+    span = max(t[2] for t in times) - min(arrivals)
+    return {
+        "proofs_per_s": round(len(reqs) / span, 1),
+        "p50_ms": round(percentile(lat, 50) * 1e3, 3),
+        "p99_ms": round(percentile(lat, 99) * 1e3, 3),
+        "max_ms": round(max(lat) * 1e3, 3),
+        "span_s": round(span, 3),
+        "verified_sample": verified,
+    }, digests
+
+
+def serving(vx, n_requests, slots, seed=42, load=0.8):
+    """Continuous batching vs the one-at-a-time loop on one Poisson stream.
+
+    1. Saturation: all requests queued at t = 0, once per server: the plain
+       loop's capacity C_plain and the batching server's capacity.
+    2. Open loop: Poisson arrivals at lambda = load * C_plain; both servers
+       replay the same arrival trace; p50 / p99 of (completion - arrival).
+    Every proof of the batching server must equal, byte for byte, the proof
+    of the same request made alone by the plain loop."""
+    import random
+
+    reqs = serving_requests(vx, n_requests, seed)
+    plain = vx.Server(n_slots=1)
+    batch = vx.Server(n_slots=slots)
+    classes = sorted({(r.log_n, r.blowup_bits, r.pow_bits) for r in reqs})
+    for (ln, bb, pw) in classes:
+        plain.prepare(vx.request(ln, blowup_bits=bb, pow_bits=pw))
+        batch.prepare(vx.request(ln, blowup_bits=bb, pow_bits=pw))
+    # warm every class once on every slot (first replays, page-in of pinned buffers)
+    warm = [vx.request(ln, blowup_bits=bb, pow_bits=pw) for (ln, bb, pw) in classes]
+    plain.run_trace(warm, [0.0] * len(warm), keep=False)
+    batch.run_trace(warm * slots, [0.0] * (len(warm) * slots), keep=False)
+
+    zero = [0.0] * n_requests
+    sat_plain, d_plain = run_server(vx, plain, reqs, zero)
+    sat_batch, d_batch = run_server(vx, batch, reqs, zero)
+    lam = load * sat_plain["proofs_per_s"]
+    rng = random.Random(seed + 1)
+    t, arrivals = 0.0, []
+    for _ in reqs:
+        t += rng.expovariate(lam)
+        arrivals.append(t)
+    open_plain, o_plain = run_server(vx, plain, reqs, arrivals)
+    open_batch, o_batch = run_server(vx, batch, reqs, arrivals)
+    identical = d_plain == d_batch == o_plain == o_batch
+    resident = batch.resident_bytes
+    plain.close()
+    batch.close()
+    return {
+        "requests": n_requests,
+        "mix": "Fibonacci, log_n uniform 14..22, blowup 4, 80 queries, PoW 20 (KRAK v2)",
+        "slots": slots,
+        "lambda_per_s": round(lam, 1),
+        "plain_loop": {"saturation": sat_plain, "poisson": open_plain},
+        "continuous_batching": {"saturation": sat_batch, "poisson": open_batch},
+        "capacity_ratio": round(sat_batch["proofs_per_s"] / sat_plain["proofs_per_s"], 3),
+        "p99_ratio": round(open_batch["p99_ms"] / open_plain["p99_ms"], 3),
+        "identical_to_alone": identical,
+        "resident_bytes": resident,
+    }
+
+
 def copy_calibration(vx, mib=256):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     counters scale the per-kernel HBM traffic reported from rocprofv3."""
@@ -105,12 +197,16 @@ def main():
     ap.add_argument("--blowup-bits", type=int, default=2)
     ap.add_argument("--pow-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: one 256 MiB device copy (counter calibration), "
                          "1 warmup + 2 timed proofs, no CPU baseline")
     args = ap.parse_args()
     if args.profile:
-        args.steps, args.warmup, args.no_cpu_baseline = 2, 1, True
+        args.steps, args.warmup, args.no_cpu_baseline, args.no_serve = 2, 1, True, True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
@@ -119,6 +215,9 @@ def main():
     import vortexstark_amd as vx
 
     vx.load()
+    if args.serve_only:
+        print(json.dumps({"serving": serving(vx, args.serve_requests, args.slots)}), flush=True)
+        return
     if args.profile:
         copy_calibration(vx)
     log_n, bb, pw = args.log_n, args.blowup_bits, args.pow_bits
@@ -194,6 +293,8 @@ def main():
     }
     if not ok:
         out["verify_error"] = why
+    if not args.no_serve and not args.profile:
+        out["serving"] = serving(vx, args.serve_requests, args.slots)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(log_n, max(log_n - 4, 10))
     prover.close()

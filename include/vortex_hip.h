@@ -230,6 +230,11 @@ uint64_t vx_prover_device_bytes(const vx_prover *p);
  * HOST copy of the last proof (valid after vx_prover_wait). */
 int vx_prover_buffer(const vx_prover *p, int which, const uint32_t **d_ptr, uint64_t *words);
 
+/* Untimed proofs replay a HIP graph of the whole proof captured on first
+ * use (default on); 0 launches the kernels one by one instead. Same bytes.
+ * No counterpart in the reference (its prove_cached launches per call). */
+int vx_prover_set_graphs(vx_prover *p, int on);
+
 /* prover::prove counterpart (src/prover.rs:221-223): one-shot proof. */
 int vx_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits,
              uint8_t *out, size_t out_cap, size_t *out_len);
@@ -237,6 +242,43 @@ int vx_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint3
  * VX_OK if accepted, VX_ERR_VERIFY (reason in err) if rejected. Host only. */
 int vx_verify(const uint8_t *proof, size_t len, uint32_t blowup_bits, uint32_t n_queries, char *err,
               size_t err_cap);
+
+/* ------------------------------------------------------------------ serving
+ * Replaces the reference's one-worker proof queue (src/bin/dashboard.rs:
+ * 66-96, drained one proof at a time; ProverPipeline::prove_batch,
+ * src/prover.rs:2278-2319, is the same serial loop) with continuous
+ * batching: n_slots HIP streams, each request starts on a free slot as soon
+ * as one frees up and overlaps on the device with the others. Each slot keeps
+ * a resident prover per size class, admitted against mem_budget_bytes of
+ * device memory (idle provers are evicted least-recently-used). n_slots = 1
+ * is the one-at-a-time loop. Proof bytes are identical to vx_prover_prove. */
+typedef struct vx_server vx_server;
+typedef struct vx_request {
+  uint32_t air; /* 0 = Fibonacci */
+  uint32_t log_n, blowup_bits, n_queries, pow_bits;
+  uint32_t a, b; /* Fibonacci public inputs */
+  uint32_t reserved;
+  uint64_t seed;
+} vx_request;
+
+int vx_server_create(uint32_t n_slots, uint64_t mem_budget_bytes, int use_graphs, vx_server **out);
+int vx_server_destroy(vx_server *s);
+/* Create the request's size class on every slot ahead of traffic. */
+int vx_server_prepare(vx_server *s, const vx_request *r);
+/* Queue a request (non-blocking); out (host, out_cap >= vx_proof_size) receives
+ * the proof before the ticket completes. */
+int vx_server_submit(vx_server *s, const vx_request *r, uint8_t *out, size_t out_cap, uint64_t *ticket);
+/* Block until the ticket completes and release it; times = submit, start
+ * and completion (seconds on the server's clock). */
+int vx_server_wait(vx_server *s, uint64_t ticket, size_t *out_len, double times[3]);
+/* 1 if the ticket has completed, 0 if not, < 0 on error. */
+int vx_server_done(vx_server *s, uint64_t ticket);
+uint64_t vx_server_resident_bytes(const vx_server *s);
+/* Replay an arrival trace: request i is submitted arrival_s[i] seconds
+ * (sorted) after the call starts; returns when all completed. times[3i..3i+2]
+ * = submit, start, completion, in seconds from the call's start. */
+int vx_server_run_trace(vx_server *s, const vx_request *reqs, const double *arrival_s, uint32_t n,
+                        uint8_t *const *outs, const size_t *caps, size_t *lens, double *times);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,109 @@
+"""Graph replay and the continuous-batching server (vx_server_*).
+
+Every proof produced through a HIP graph or through the server, alone or in a
+mix of concurrent requests, must equal byte for byte the proof of the same
+request made alone by the kernel-by-kernel prover; that prover in turn equals
+the oracle (test_gpu_prove.py)."""
+import random
+
+import pytest
+
+import pyoracle as po
+import vortexstark_amd as vx
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(log_n, a, b, bb=2, pw=0):
+    p = vx.Prover(log_n, bb, 80, pw, graphs=False)
+    out = p.prove(a, b)
+    p.close()
+    return out
+
+
+@pytest.mark.parametrize("log_n,bb,pw", [(8, 2, 0), (14, 2, 0), (12, 1, 12), (18, 2, 0)])
+def test_graph_replays_equal_eager(log_n, bb, pw):
+    g = vx.Prover(log_n, bb, 80, pw, graphs=True)
+    e = vx.Prover(log_n, bb, 80, pw, graphs=False)
+    for i, (a, b) in enumerate([(1, 1), (5, 7), (1, 1), (123456789, 42), (2, 3)]):
+        print("replay", log_n, bb, pw, i, flush=True)
+        want = e.prove(a, b)            # eager work between the replays
+        got = g.prove(a, b)
+        assert got == want, (log_n, i)
+    assert g.prove(3, 4) == po.prove(3, 4, log_n, blowup_bits=bb, pow_bits=pw, omp=True)
+    g.close()
+    e.close()
+
+
+def test_timed_proof_after_graph_replays():
+    p = vx.Prover(12)
+    a = p.prove(1, 2)
+    p.enqueue(1, 2, timed=True)
+    assert p.wait() == a
+    assert p.stage_times()["total"] > 0
+    assert p.prove(1, 2) == a
+    p.close()
+
+
+def mixed_requests(n, seed, sizes=(8, 10, 12, 14, 16), pw=(0, 10)):
+    rng = random.Random(seed)
+    reqs = []
+    for _ in range(n):
+        reqs.append(vx.request(rng.choice(sizes), a=rng.randrange(vx.P), b=rng.randrange(vx.P),
+                               pow_bits=rng.choice(pw)))
+    return reqs
+
+
+@pytest.mark.parametrize("slots", [1, 4])
+def test_server_mix_equals_alone(slots):
+    reqs = mixed_requests(24, seed=slots)
+    srv = vx.Server(n_slots=slots)
+    tickets = [srv.submit(r) for r in reqs]
+    got = [srv.wait(t) for t in tickets]
+    srv.close()
+    for r, (proof, (ts, t0, t1)) in zip(reqs, got):
+        assert ts <= t0 <= t1
+        assert proof == alone(r.log_n, r.a, r.b, r.blowup_bits, r.pow_bits), (r.log_n, r.a, r.b, r.pow_bits)
+        assert vx.verify(proof)[0]
+
+
+def test_server_run_trace_equals_alone():
+    reqs = mixed_requests(40, seed=7, sizes=(10, 12, 14, 16, 18))
+    rng = random.Random(1)
+    t, arrivals = 0.0, []
+    for _ in reqs:
+        t += rng.expovariate(2000.0)
+        arrivals.append(t)
+    srv = vx.Server(n_slots=4)
+    for k in {(r.log_n, r.pow_bits) for r in reqs}:
+        srv.prepare(vx.request(k[0], pow_bits=k[1]))
+    proofs, times = srv.run_trace(reqs, arrivals)
+    srv.close()
+    for r, p, (ts, t0, t1), at in zip(reqs, proofs, times, arrivals):
+        assert at - 1e-3 <= ts <= t0 <= t1
+        assert p == alone(r.log_n, r.a, r.b, r.blowup_bits, r.pow_bits)
+
+
+def test_server_evicts_to_fit_budget():
+    # budget for about one log_n=16 class: switching classes must evict
+    budget = 3 * vx.Prover(16).device_bytes
+    srv = vx.Server(n_slots=2, mem_budget=budget)
+    reqs = [vx.request(n, a=n, b=n + 1) for n in (16, 14, 16, 12, 16, 14)]
+    tickets = [srv.submit(r) for r in reqs]
+    for r, t in zip(reqs, tickets):
+        proof, _ = srv.wait(t)
+        assert proof == alone(r.log_n, r.a, r.b)
+    assert srv.resident_bytes <= budget
+    srv.close()
+
+
+def test_server_rejects_what_cannot_fit():
+    srv = vx.Server(n_slots=1, mem_budget=1 << 20)
+    t = srv.submit(vx.request(16))
+    with pytest.raises(vx.VortexError, match="budget"):
+        srv.wait(t)
+    # the server keeps serving what fits
+    srv.close()
+    with pytest.raises(vx.VortexError):
+        srv2 = vx.Server(n_slots=1)
+        srv2.submit(vx.request(16, a=vx.P))  # non-canonical input

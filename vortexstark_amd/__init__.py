@@ -26,13 +26,17 @@ def proof_size(log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0):
 
 
 class Prover:
-    """Cached prover for one size class (ProverCache + prove_cached)."""
+    """Cached prover for one size class (ProverCache + prove_cached).
 
-    def __init__(self, log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0, stream=None):
+    graphs=True replays a HIP graph of the whole proof (captured on the first
+    untimed proof); False launches kernel by kernel. The bytes are the same."""
+
+    def __init__(self, log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0, stream=None, graphs=True):
         self.log_n, self.blowup_bits, self.n_queries, self.pow_bits = log_n, blowup_bits, n_queries, pow_bits
         self._h = C.c_void_p()
         _lib.call("vx_prover_create", log_n, blowup_bits, n_queries, pow_bits, stream, C.byref(self._h))
         self.size = proof_size(log_n, blowup_bits, n_queries, pow_bits)
+        _lib.call("vx_prover_set_graphs", self._h, 1 if graphs else 0)
 
     def prove(self, a, b):
         self.enqueue(a, b)
@@ -91,3 +95,81 @@ def verify(proof, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES):
     if rc not in (0, 4):
         raise VortexError("vx_verify", rc, err.value.decode())
     return rc == 0, err.value.decode()
+
+
+AIR_FIBONACCI = 0
+
+
+def request(log_n, a=1, b=1, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0, air=AIR_FIBONACCI, seed=0):
+    """A proof request (vx_request)."""
+    return _lib.Request(air, log_n, blowup_bits, n_queries, pow_bits, a, b, 0, seed)
+
+
+class Server:
+    """Continuous-batching proof server (vx_server_*): n_slots HIP streams,
+    resident provers per size class admitted against mem_budget bytes of
+    device memory. n_slots=1 is the one-proof-at-a-time loop of the
+    reference's worker (src/bin/dashboard.rs:66-96)."""
+
+    def __init__(self, n_slots=4, mem_budget=64 << 30, graphs=True):
+        self._h = C.c_void_p()
+        _lib.call("vx_server_create", n_slots, mem_budget, 1 if graphs else 0, C.byref(self._h))
+        self.n_slots = n_slots
+        self._bufs = {}
+
+    def prepare(self, req):
+        _lib.call("vx_server_prepare", self._h, C.byref(req))
+
+    def submit(self, req):
+        size = proof_size(req.log_n, req.blowup_bits, req.n_queries, req.pow_bits)
+        buf = C.create_string_buffer(size)
+        t = C.c_uint64()
+        _lib.call("vx_server_submit", self._h, C.byref(req), buf, size, C.byref(t))
+        self._bufs[t.value] = buf
+        return t.value
+
+    def done(self, ticket):
+        rc = load().vx_server_done(self._h, ticket)
+        if rc < 0:
+            _lib.check("vx_server_done", rc)
+        return rc == 1
+
+    def wait(self, ticket):
+        """(proof bytes, (submit, start, done) seconds on the server clock)."""
+        n = C.c_size_t()
+        times = (C.c_double * 3)()
+        buf = self._bufs.pop(ticket)
+        _lib.call("vx_server_wait", self._h, ticket, C.byref(n), times)
+        return buf.raw[:n.value], tuple(times)
+
+    def run_trace(self, reqs, arrivals, keep=True):
+        """Replay an arrival trace (seconds, sorted). Returns (proofs or
+        None, times[n][3] = submit/start/done seconds from the trace start)."""
+        n = len(reqs)
+        arr = (_lib.Request * n)(*reqs)
+        at = (C.c_double * n)(*arrivals)
+        sizes = [proof_size(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits) for r in reqs]
+        bufs = [C.create_string_buffer(sz) for sz in sizes]
+        outs = (C.c_void_p * n)(*[C.addressof(b) for b in bufs])
+        caps = (C.c_size_t * n)(*sizes)
+        lens = (C.c_size_t * n)()
+        times = (C.c_double * (3 * n))()
+        _lib.call("vx_server_run_trace", self._h, arr, at, n, outs, caps, lens, times)
+        t = [tuple(times[3 * i:3 * i + 3]) for i in range(n)]
+        proofs = [bufs[i].raw[:lens[i]] for i in range(n)] if keep else None
+        return proofs, t
+
+    @property
+    def resident_bytes(self):
+        return int(load().vx_server_resident_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            _lib.call("vx_server_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

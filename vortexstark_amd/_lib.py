@@ -70,7 +70,23 @@ SIGNATURES = {
     "vx_prover_buffer": (ip, [vp, ip, C.POINTER(vp), C.POINTER(u64)]),
     "vx_prove": (ip, [u32, u32, u32, u32, u32, u32, vp, sz, C.POINTER(sz)]),
     "vx_verify": (ip, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
+    "vx_prover_set_graphs": (ip, [vp, ip]),
+    "vx_server_create": (ip, [u32, u64, ip, C.POINTER(vp)]),
+    "vx_server_destroy": (ip, [vp]),
+    "vx_server_prepare": (ip, [vp, vp]),
+    "vx_server_submit": (ip, [vp, vp, vp, sz, C.POINTER(u64)]),
+    "vx_server_wait": (ip, [vp, u64, C.POINTER(sz), C.POINTER(C.c_double)]),
+    "vx_server_done": (ip, [vp, u64]),
+    "vx_server_resident_bytes": (u64, [vp]),
+    "vx_server_run_trace": (ip, [vp, vp, C.POINTER(C.c_double), u32, C.POINTER(vp), C.POINTER(sz),
+                                 C.POINTER(sz), C.POINTER(C.c_double)]),
 }
+
+
+class Request(C.Structure):
+    """vx_request (include/vortex_hip.h)."""
+    _fields_ = [("air", u32), ("log_n", u32), ("blowup_bits", u32), ("n_queries", u32), ("pow_bits", u32),
+                ("a", u32), ("b", u32), ("reserved", u32), ("seed", u64)]
 
 _lib = None
 

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "prover.h"
+#include "server.h"
 
 using namespace vx;
 
@@ -502,6 +503,115 @@ int vx_verify(const uint8_t* proof, size_t len, uint32_t blowup_bits, uint32_t n
     err[err_cap - 1] = 0;
   }
   return rc;
+}
+
+int vx_prover_set_graphs(vx_prover* p, int on) {
+  return guard([&] {
+    require(p != nullptr, "null prover");
+    p->impl.set_graphs(on != 0);
+  });
+}
+
+// ------------------------------------------------------------------ serving
+struct vx_server {
+  ProofServer impl;
+  vx_server(uint32_t n, uint64_t budget, bool graphs) : impl(n, budget, graphs) {}
+};
+
+static ProofRequest to_request(const vx_request* r) {
+  require(r != nullptr, "server: null request");
+  ProofRequest q{};
+  q.air = r->air;
+  q.log_n = r->log_n;
+  q.blowup_bits = r->blowup_bits;
+  q.n_queries = r->n_queries;
+  q.pow_bits = r->pow_bits;
+  q.a = r->a;
+  q.b = r->b;
+  q.seed = r->seed;
+  return q;
+}
+
+int vx_server_create(uint32_t n_slots, uint64_t mem_budget_bytes, int use_graphs, vx_server** out) {
+  return guard([&] {
+    require(out != nullptr, "server_create: null out");
+    *out = nullptr;
+    *out = new vx_server(n_slots, mem_budget_bytes, use_graphs != 0);
+  });
+}
+
+int vx_server_destroy(vx_server* s) {
+  return guard([&] { delete s; });
+}
+
+int vx_server_prepare(vx_server* s, const vx_request* r) {
+  return guard([&] {
+    require(s != nullptr, "null server");
+    s->impl.prepare(to_request(r));
+  });
+}
+
+int vx_server_submit(vx_server* s, const vx_request* r, uint8_t* out, size_t out_cap, uint64_t* ticket) {
+  return guard([&] {
+    require(s && ticket, "null argument");
+    *ticket = s->impl.submit(to_request(r), out, out_cap);
+  });
+}
+
+int vx_server_wait(vx_server* s, uint64_t ticket, size_t* out_len, double times[3]) {
+  return guard([&] {
+    require(s != nullptr, "null server");
+    JobTimes t{};
+    std::string err;
+    bool ok = s->impl.wait(ticket, out_len, &t, err);
+    if (times) {
+      times[0] = t.submit_s;
+      times[1] = t.start_s;
+      times[2] = t.done_s;
+    }
+    if (!ok) throw Error(ERR_ARG, err);
+  });
+}
+
+int vx_server_done(vx_server* s, uint64_t ticket) {
+  int done = 0;
+  int rc = guard([&] {
+    require(s != nullptr, "null server");
+    done = s->impl.done(ticket) ? 1 : 0;
+  });
+  return rc ? rc : done;
+}
+
+uint64_t vx_server_resident_bytes(const vx_server* s) { return s ? s->impl.resident_bytes() : 0; }
+
+int vx_server_run_trace(vx_server* s, const vx_request* reqs, const double* arrival_s, uint32_t n,
+                        uint8_t* const* outs, const size_t* caps, size_t* lens, double* times) {
+  return guard([&] {
+    require(s && reqs && arrival_s && outs && caps && lens && times, "null argument");
+    for (uint32_t i = 1; i < n; i++) require(arrival_s[i] >= arrival_s[i - 1], "run_trace: arrivals must be sorted");
+    std::vector<uint64_t> tickets(n);
+    const double t0 = s->impl.now();
+    for (uint32_t i = 0; i < n; i++) {
+      const double due = t0 + arrival_s[i];
+      for (;;) {  // sleep through long gaps, spin the last 200 us
+        double left = due - s->impl.now();
+        if (left <= 0) break;
+        if (left > 3e-4) std::this_thread::sleep_for(std::chrono::duration<double>(left - 2e-4));
+      }
+      tickets[i] = s->impl.submit(to_request(&reqs[i]), outs[i], caps[i]);
+    }
+    std::string first_err;
+    for (uint32_t i = 0; i < n; i++) {
+      JobTimes t{};
+      std::string err;
+      bool ok = s->impl.wait(tickets[i], &lens[i], &t, err);
+      if (!ok && first_err.empty()) first_err = "request " + std::to_string(i) + ": " + err;
+      times[3 * i + 0] = t.submit_s - t0;
+      times[3 * i + 1] = t.start_s - t0;
+      times[3 * i + 2] = t.done_s - t0;
+    }
+    if (!first_err.empty()) throw Error(ERR_ARG, first_err);
+  });
 }
 
 }  // extern "C"

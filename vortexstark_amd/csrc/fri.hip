@@ -31,7 +31,7 @@ __device__ __forceinline__ void fib_pair(uint64_t k, uint32_t& fk, uint32_t& fk1
   fk1 = g;
 }
 
-__global__ void k_fib_trace(uint32_t a, uint32_t b, uint32_t* out, uint64_t n) {
+__device__ __forceinline__ void fib_fill(uint32_t a, uint32_t b, uint32_t* out, uint64_t n) {
   const uint64_t first = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * TRACE_PER_LANE;
   if (first >= n) return;
   uint32_t fk, fk1;
@@ -53,11 +53,43 @@ __global__ void k_fib_trace(uint32_t a, uint32_t b, uint32_t* out, uint64_t n) {
   }
 }
 
+__global__ void k_fib_trace(uint32_t a, uint32_t b, uint32_t* out, uint64_t n) { fib_fill(a, b, out, n); }
+
+// Public inputs read from device memory, so a captured graph replays with new (a, b).
+__global__ void k_fib_trace_dev(const uint32_t* ab, uint32_t* out, uint64_t n) { fib_fill(ab[0], ab[1], out, n); }
+
+static uint32_t trace_blocks(uint64_t n) {
+  uint64_t lanes = (n + TRACE_PER_LANE - 1) / TRACE_PER_LANE;
+  return (uint32_t)((lanes + 255) / 256);
+}
+
 void fib_trace(uint32_t a, uint32_t b, uint32_t* out, uint32_t log_n, hipStream_t s) {
   require(log_n >= 1 && log_n <= 31 && a < P && b < P, "fib_trace: bad arguments");
   uint64_t n = 1ull << log_n;
-  uint64_t lanes = (n + TRACE_PER_LANE - 1) / TRACE_PER_LANE;
-  hipLaunchKernelGGL(k_fib_trace, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, s, a, b, out, n);
+  hipLaunchKernelGGL(k_fib_trace, dim3(trace_blocks(n)), dim3(256), 0, s, a, b, out, n);
+  VX_HIP(hipGetLastError());
+}
+
+void fib_trace_dev(const uint32_t* ab, uint32_t* out, uint32_t log_n, hipStream_t s) {
+  require(log_n >= 1 && log_n <= 31, "fib_trace: bad log_n");
+  uint64_t n = 1ull << log_n;
+  hipLaunchKernelGGL(k_fib_trace_dev, dim3(trace_blocks(n)), dim3(256), 0, s, ab, out, n);
+  VX_HIP(hipGetLastError());
+}
+
+// inputs[0..1] = (a, b) and the proof header words that carry them.
+__global__ void k_set_inputs(uint32_t* inputs, uint32_t* proof, uint32_t word_a, uint32_t a, uint32_t b) {
+  if (threadIdx.x == 0) {
+    inputs[0] = a;
+    inputs[1] = b;
+    proof[word_a] = a;
+    proof[word_a + 1] = b;
+  }
+}
+
+void set_inputs(uint32_t* inputs, uint32_t* proof, uint32_t word_a, uint32_t a, uint32_t b, hipStream_t s) {
+  require(a < P && b < P, "prove: public inputs must be canonical M31");
+  hipLaunchKernelGGL(k_set_inputs, dim3(1), dim3(64), 0, s, inputs, proof, word_a, a, b);
   VX_HIP(hipGetLastError());
 }
 

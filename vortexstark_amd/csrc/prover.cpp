@@ -10,7 +10,10 @@
 #include "prover.h"
 
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 
 namespace vx {
 
@@ -98,6 +101,63 @@ TwiddleSet::TwiddleSet(const coset& c, bool inverse, hipStream_t s) {
   ref.log_n = L;
 }
 
+// Fold twiddles (src/fri.rs:112-160): layer 0 = circle fold on the eval
+// domain (inverse y), layer k >= 1 = line fold on half_odds(le - k)
+// (inverse x); entry i uses the point at bitrev(2i).
+ProverTables::ProverTables(uint32_t log_n_, uint32_t blowup_bits, hipStream_t s)
+    : log_n(log_n_), le(log_n_ + blowup_bits), n_fri(log_n_ + blowup_bits - 3) {
+  trace_itw = std::make_unique<TwiddleSet>(make_coset(HALF_COSET, log_n), true, s);
+  eval_tw = std::make_unique<TwiddleSet>(eval_domain(le), false, s);
+  bytes += trace_itw->line.bytes + trace_itw->circle.bytes + eval_tw->line.bytes + eval_tw->circle.bytes;
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < n_fri; k++) {
+    fold_tw_off.push_back(total);
+    total += 1ull << (le - 1 - k);
+  }
+  fold_tw = DeviceBuf(4 * total);
+  bytes += fold_tw.bytes;
+  for (uint32_t k = 0; k < n_fri; k++) {
+    coset d = k == 0 ? eval_domain(le) : make_coset(HALF_ODDS, le - k);
+    fold_twiddle_sources(d.init, d.step, fold_tw.u32() + fold_tw_off[k], 1ull << d.log_size, d.log_size, k == 0, s);
+  }
+  batch_inverse(fold_tw.u32(), fold_tw.u32(), total, s);
+  VX_HIP(hipStreamSynchronize(s));
+}
+
+uint64_t ProverTables::estimate_bytes(uint32_t log_n, uint32_t blowup_bits) {
+  const uint64_t n = 1ull << log_n, N = n << blowup_bits;
+  // line (n) + circle (n/2) per twiddle set, fold tables sum to ~N
+  return 4 * (n + n / 2) + 4 * (N + N / 2) + 4 * N;
+}
+
+std::shared_ptr<const ProverTables> ProverTables::get(uint32_t log_n, uint32_t blowup_bits, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, uint32_t, uint32_t>, std::weak_ptr<const ProverTables>> cache;
+  int dev = 0;
+  VX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(dev, log_n, blowup_bits);
+  if (auto sp = cache[key].lock()) return sp;
+  auto sp = std::make_shared<const ProverTables>(log_n, blowup_bits, s);
+  cache[key] = sp;
+  return sp;
+}
+
+uint64_t FibProver::estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits) {
+  const uint32_t le = log_n + blowup_bits, n_fri = le - 3;
+  uint64_t b = 4ull << log_n;          // trace
+  b += 4ull << le;                     // eval
+  for (uint32_t k = 0; k < n_fri; k++) b += 16ull << (le - 1 - k);   // FRI layers
+  for (uint32_t t = 0; t < 2 + n_fri; t++) {                         // trees, levels >= 4
+    uint32_t L = t < 2 ? le : le - 1 - (t - 2);
+    uint32_t lo = L < 4 ? L : 4;
+    for (uint32_t l = lo; l <= L; l++) b += 32ull << (L - l);
+  }
+  b += ProofLayout(log_n, blowup_bits, nq, pow_bits).total_words * 4;  // device proof
+  b += 64 + 16ull * (n_fri + 2) + 4ull * (nq + 3) + 256 + sizeof(DecomDesc) * (2 + n_fri);
+  return b;
+}
+
 FibProver::FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, hipStream_t stream)
     : layout_((require(log_n >= 4 && blowup_bits >= 1 && log_n + blowup_bits <= 31, "prover: need log_n >= 4, "
                                                                                        "blowup >= 1, log_n + blowup <= 31"),
@@ -141,10 +201,10 @@ FibProver::FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, u
     tree_off_.push_back(off);
     device_bytes_ += trees_.back().bytes;
   }
-  // channel (64 B) | alphas | queries | prefix | nonce
+  // channel (64 B) | alphas | queries | prefix (64 B) | nonce (64 B) | inputs (64 B)
   const size_t alpha_words = 4ull * (n_fri_ + 2);
   const size_t query_words = (nq_ + 3) / 4 * 4;  // keeps the prefix/nonce slots 16-byte aligned
-  const size_t small_bytes = 64 + 4 * alpha_words + 4 * query_words + 64 + 64;
+  const size_t small_bytes = 64 + 4 * alpha_words + 4 * query_words + 64 + 64 + 64;
   small_ = DeviceBuf(small_bytes);
   auto* base = static_cast<uint8_t*>(small_.p);
   d_channel_ = reinterpret_cast<Channel*>(base);
@@ -152,43 +212,24 @@ FibProver::FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, u
   d_queries_ = d_alphas_ + alpha_words;
   d_prefix_ = reinterpret_cast<uint32_t*>(base + 64 + 4 * alpha_words + 4 * query_words);
   d_nonce_ = reinterpret_cast<uint64_t*>(reinterpret_cast<uint8_t*>(d_prefix_) + 64);
+  d_inputs_ = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(d_prefix_) + 128);
   proof_ = DeviceBuf(layout_.total_words * 4);
-  VX_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_proof_), layout_.total_words * 4, hipHostMallocDefault));
+  for (auto& h : host_proof_)
+    VX_HIP(hipHostMalloc(reinterpret_cast<void**>(&h), layout_.total_words * 4, hipHostMallocDefault));
   device_bytes_ += small_.bytes + proof_.bytes;
   for (auto& e : ev_) VX_HIP(hipEventCreate(&e));
-  build_tables();
+  tab_ = ProverTables::get(log_n_, blowup_, stream_);
   build_descs();
 }
 
 FibProver::~FibProver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_) (void)hipGraphExecDestroy(graph_);
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
-  if (host_proof_) (void)hipHostFree(host_proof_);
+  for (auto& h : host_proof_)
+    if (h) (void)hipHostFree(h);
   if (own_stream_) (void)hipStreamDestroy(stream_);
-}
-
-void FibProver::build_tables() {
-  hipStream_t s = stream_;
-  trace_itw_ = std::make_unique<TwiddleSet>(make_coset(HALF_COSET, log_n_), true, s);
-  eval_tw_ = std::make_unique<TwiddleSet>(eval_domain(le_), false, s);
-  device_bytes_ += trace_itw_->line.bytes + trace_itw_->circle.bytes + eval_tw_->line.bytes + eval_tw_->circle.bytes;
-  // Fold twiddles (src/fri.rs:112-160): layer 0 = circle fold on the eval
-  // domain (inverse y), layer k >= 1 = line fold on half_odds(le - k)
-  // (inverse x); entry i uses the point at bitrev(2i).
-  uint64_t total = 0;
-  for (uint32_t k = 0; k < n_fri_; k++) {
-    fold_tw_off_.push_back(total);
-    total += 1ull << (le_ - 1 - k);
-  }
-  fold_tw_ = DeviceBuf(4 * total);
-  device_bytes_ += fold_tw_.bytes;
-  for (uint32_t k = 0; k < n_fri_; k++) {
-    coset d = k == 0 ? eval_domain(le_) : make_coset(HALF_ODDS, le_ - k);
-    fold_twiddle_sources(d.init, d.step, fold_tw_.u32() + fold_tw_off_[k], 1ull << d.log_size, d.log_size, k == 0, s);
-  }
-  batch_inverse(fold_tw_.u32(), fold_tw_.u32(), total, s);
-  VX_HIP(hipStreamSynchronize(s));
 }
 
 TreeLevels FibProver::tree_levels(uint32_t t) const {
@@ -233,10 +274,10 @@ void FibProver::build_descs() {
   device_bytes_ += descs_.bytes;
 }
 
-void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
-  require(a < P && b < P, "prove: public inputs must be canonical M31");
-  hipStream_t s = stream_;
-  timed_ = timed;
+// The whole proof after the public inputs are set: kernels only (no host
+// synchronisation, no memset), so it can be captured into a graph.
+void FibProver::record(bool timed, hipStream_t s) {
+  const ProverTables& tb = *tab_;
   const ProofLayout& L = layout_;
   uint32_t* proof = proof_.u32();
   const uint64_t n = 1ull << log_n_, N = 1ull << le_;
@@ -247,17 +288,17 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   auto layer_col = [&](uint32_t k, int c) { return layers_[k].u32() + c * layer_col_words_[k]; };
 
   mark(0);
-  std::vector<uint32_t> hw = {0, KRAK_MAGIC, 1, pow_bits_ ? 2u : 1u, 2, log_n_, 3, a, 4, b};
+  std::vector<uint32_t> hw = {0, KRAK_MAGIC, 1, pow_bits_ ? 2u : 1u, 2, log_n_};  // words 3, 4: set_inputs
   if (pow_bits_) {
     hw.insert(hw.end(), {5, blowup_, 6, pow_bits_});
   }
   hw.insert(hw.end(), {(uint32_t)L.w_nfri, n_fri_, (uint32_t)L.w_nfdec, n_fri_});
   write_header(proof, hw.data(), (uint32_t)hw.size() / 2, s);
-  fib_trace(a, b, trace_.u32(), log_n_, s);
+  fib_trace_dev(d_inputs_, trace_.u32(), log_n_, s);
   mark(1);
-  ntt_inverse(trace_.u32(), trace_itw_->ref, s);
+  ntt_inverse(trace_.u32(), tb.trace_itw->ref, s);
   mark(2);
-  ntt_forward(eval_.u32(), trace_.u32(), n - 1, eval_tw_->ref, log_n_, s);
+  ntt_forward(eval_.u32(), trace_.u32(), n - 1, tb.eval_tw->ref, log_n_, s);
   mark(3);
 
   LeafSrc lt{};
@@ -281,7 +322,7 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   lc.col[0] = eval_.u32();
   lc.alpha_q = alpha(0);
   lc.alpha_f = alpha(1);
-  lc.tw = fold_tw_.u32() + fold_tw_off_[0];
+  lc.tw = tb.fold_tw.u32() + tb.fold_tw_off[0];
   lc.src_mask = N - 1;
   for (int c = 0; c < 4; c++) lc.out[c] = layer_col(0, c);
   merkle_commit(lc, tree_levels(2), tree_lo_[2],
@@ -296,7 +337,7 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
       ll.col[c] = layer_col(k - 1, c);
       ll.out[c] = layer_col(k, c);
     }
-    ll.tw = fold_tw_.u32() + fold_tw_off_[k];
+    ll.tw = tb.fold_tw.u32() + tb.fold_tw_off[k];
     ll.alpha_f = alpha(k + 1);
     merkle_commit(ll, tree_levels(2 + k), tree_lo_[2 + k],
                   RootHook{d_channel_, proof + L.w_froot0 + 8ull * k, k + 1 < n_fri_ ? alpha(k + 2) : nullptr, 0}, s);
@@ -309,7 +350,7 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
     for (uint32_t j = k; j < n_fri_; j++) {
       FriTailLayer& T = ta.layer[j - k];
       for (int c = 0; c < 4; c++) T.out[c] = layer_col(j, c);
-      T.tw = fold_tw_.u32() + fold_tw_off_[j];
+      T.tw = tb.fold_tw.u32() + tb.fold_tw_off[j];
       T.tree = trees_[2 + j].u32();
       T.log_n = le_ - 1 - j;
       T.lo = tree_lo_[2 + j];
@@ -340,14 +381,50 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   mark(7);
   decommit(reinterpret_cast<const DecomDesc*>(descs_.p), 2 + n_fri_, d_queries_, nq_, proof, s);
   mark(8);
-  VX_HIP(hipMemcpyAsync(host_proof_, proof, layout_.total_words * 4, hipMemcpyDeviceToHost, s));
-  mark(9);
+}
+
+// Captured on a private stream, so work that other threads put on the
+// prover's stream meanwhile cannot end up in the graph.
+void FibProver::capture() {
+  hipStream_t cs = nullptr;
+  VX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    try {
+      record(false, cs);
+    } catch (...) {
+      if (hipStreamEndCapture(cs, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      (void)hipStreamDestroy(cs);
+      throw;
+    }
+    e = hipStreamEndCapture(cs, &g);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  VX_HIP(e);
+}
+
+void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
+  require(a < P && b < P, "prove: public inputs must be canonical M31");
+  timed_ = timed;
+  set_inputs(d_inputs_, proof_.u32(), 3, a, b, stream_);
+  if (timed || !use_graph_) {
+    record(timed, stream_);
+  } else {
+    if (!graph_) capture();
+    VX_HIP(hipGraphLaunch(graph_, stream_));
+  }
+  cur_ ^= 1;
+  VX_HIP(hipMemcpyAsync(host_proof_[cur_], proof_.p, layout_.total_words * 4, hipMemcpyDeviceToHost, stream_));
+  if (timed) VX_HIP(hipEventRecord(ev_[9], stream_));
 }
 
 const uint8_t* FibProver::wait(size_t* len) {
   VX_HIP(hipStreamSynchronize(stream_));
   if (len) *len = layout_.total_words * 4;
-  return host_proof_;
+  return host_proof_[cur_];
 }
 
 StageTimes FibProver::last_times() {

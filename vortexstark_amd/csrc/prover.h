@@ -50,11 +50,28 @@ struct StageTimes {
       download_ms, total_ms, trace_subtree_ms;
 };
 
+// Read-only tables of one (log_n, blowup) size class: interpolation twiddles
+// (inverse, trace domain), evaluation twiddles (eval domain) and the inverse
+// FRI fold twiddles of every layer. Shared by every prover of the class on a
+// device (ProverTables::get), built once.
+struct ProverTables {
+  uint32_t log_n, le, n_fri;
+  std::unique_ptr<TwiddleSet> trace_itw, eval_tw;
+  DeviceBuf fold_tw;
+  std::vector<uint64_t> fold_tw_off;
+  uint64_t bytes = 0;
+  ProverTables(uint32_t log_n, uint32_t blowup_bits, hipStream_t s);
+  static std::shared_ptr<const ProverTables> get(uint32_t log_n, uint32_t blowup_bits, hipStream_t s);
+  static uint64_t estimate_bytes(uint32_t log_n, uint32_t blowup_bits);
+};
+
 class FibProver {
  public:
   FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, hipStream_t stream);
   ~FibProver();
   // Enqueue one proof of Fib(a, b) on the stream (no host synchronisation).
+  // Untimed proofs replay a HIP graph captured on the first one (when graphs
+  // are enabled); the public inputs reach it through device memory.
   void enqueue(uint32_t a, uint32_t b, bool timed = false);
   // Wait for the last enqueued proof; returns its KRAK bytes.
   const uint8_t* wait(size_t* len);
@@ -63,17 +80,28 @@ class FibProver {
   hipStream_t stream() const { return stream_; }
   uint64_t device_bytes() const { return device_bytes_; }
   StageTimes last_times();
+  void set_graphs(bool on) { use_graph_ = on; }
+  // Capture the proof graph now rather than on the first untimed proof.
+  void prepare_graph() {
+    if (use_graph_ && !graph_) capture();
+  }
+  bool graphs() const { return use_graph_; }
+  // Device bytes a prover of this class allocates for itself (tables excluded).
+  static uint64_t estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits);
 
   // device buffers for tests / introspection
   const uint32_t* d_eval() const { return eval_.u32(); }
   const uint32_t* d_trace() const { return trace_.u32(); }
   const uint32_t* d_proof() const { return proof_.u32(); }
-  const uint8_t* host_proof() const { return host_proof_; }
+  // host (pinned) copy of the last proof; two buffers alternate, so the bytes
+  // of proof k stay valid while proof k + 1 runs
+  const uint8_t* host_proof() const { return host_proof_[cur_]; }
   const uint32_t* d_layer(uint32_t k) const { return layers_[k].u32(); }
 
  private:
-  void build_tables();
   void build_descs();
+  void record(bool timed, hipStream_t s);
+  void capture();
   TreeLevels tree_levels(uint32_t idx) const;
 
   ProofLayout layout_;
@@ -82,26 +110,28 @@ class FibProver {
   bool own_stream_ = false;
   uint64_t device_bytes_ = 0;
 
-  std::unique_ptr<TwiddleSet> trace_itw_, eval_tw_;
-  DeviceBuf fold_tw_;
-  std::vector<uint64_t> fold_tw_off_;
+  std::shared_ptr<const ProverTables> tab_;
   DeviceBuf trace_, eval_;
   std::vector<DeviceBuf> layers_;  // FRI layer k: 4 SoA columns of 2^(le-1-k)
   std::vector<uint64_t> layer_col_words_;
   std::vector<DeviceBuf> trees_;   // 0 = trace, 1 = quotient, 2 + k = FRI layer k
   std::vector<uint32_t> tree_log_, tree_lo_;
   std::vector<std::vector<uint64_t>> tree_off_;
-  DeviceBuf small_;                // channel | alphas | queries | prefix | nonce
+  DeviceBuf small_;                // channel | alphas | queries | prefix | nonce | inputs
   Channel* d_channel_ = nullptr;
   uint32_t* d_alphas_ = nullptr;
   uint32_t* d_queries_ = nullptr;
   uint32_t* d_prefix_ = nullptr;
   uint64_t* d_nonce_ = nullptr;
+  uint32_t* d_inputs_ = nullptr;
   DeviceBuf descs_;
   DeviceBuf proof_;
-  uint8_t* host_proof_ = nullptr;
+  uint8_t* host_proof_[2] = {nullptr, nullptr};
+  uint32_t cur_ = 0;
   hipEvent_t ev_[12] = {};
   bool timed_ = false;
+  bool use_graph_ = true;
+  hipGraphExec_t graph_ = nullptr;
 };
 
 // Host verifier (src/verifier.rs:39-310) over KRAK bytes; returns true and

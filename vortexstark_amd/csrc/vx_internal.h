@@ -79,6 +79,8 @@ void fold_twiddle_sources(cpt init, cpt step, uint32_t* out, uint64_t n, uint32_
 
 // ------------------------------------------------- misc kernels (fri.hip)
 void fib_trace(uint32_t a, uint32_t b, uint32_t* out, uint32_t log_n, hipStream_t s);
+void fib_trace_dev(const uint32_t* ab, uint32_t* out, uint32_t log_n, hipStream_t s);
+void set_inputs(uint32_t* inputs, uint32_t* proof, uint32_t word_a, uint32_t a, uint32_t b, hipStream_t s);
 void fib_quotient(const uint32_t* trace, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3,
                   qm31 alpha, uint64_t offset, uint64_t chunk_n, uint64_t global_n, hipStream_t s);
 void fold_line(const uint32_t* const in[4], const uint32_t* tw, uint32_t* const out[4], qm31 alpha,
