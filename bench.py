@@ -88,16 +88,25 @@ def percentile(xs, q):
     return xs[k]
 
 
+SERVING_MIX = ("50% Fibonacci (log_n uniform 14..22), 50% synthetic 34-column AIR with 35 degree-2 "
+               "constraints (log_n uniform 14..20); blowup 4, 80 queries, PoW 20")
+
+
 def serving_requests(vx, n, seed):
-    """The serving mix: Fibonacci proofs with log_n uniform in 14..22, blowup
-    4, 80 queries, PoW 20; public inputs (1, SplitMix64 stream)."""
+    """The serving mix: half Fibonacci proofs (log_n uniform in 14..22,
+    inputs (1, random)), half 34-column synthetic-AIR proofs (log_n uniform
+    in 14..20, random trace seed, 35 constraints); blowup 4, 80 queries,
+    PoW 20. Drawn from Python's seeded Mersenne Twister."""
     import random
 
     rng = random.Random(seed)
     reqs = []
     for _ in range(n):
-        log_n = rng.randint(14, 22)
-        reqs.append(vx.request(log_n, a=1, b=rng.randrange(P), blowup_bits=2, pow_bits=20))
+        if rng.random() < 0.5:
+            reqs.append(vx.request(rng.randint(14, 22), a=1, b=rng.randrange(P), blowup_bits=2, pow_bits=20))
+        else:
+            reqs.append(vx.request(rng.randint(14, 20), air=vx.AIR_D2, seed=rng.getrandbits(64), blowup_bits=2,
+                                   pow_bits=20))
     return reqs
 
 
@@ -107,6 +116,9 @@ def run_server(vx, srv, reqs, arrivals):
     digests = [hashlib.sha256(p).digest() for p in proofs]
     sample = proofs[:: max(1, len(proofs) // 16)]
     verified = all(vx.verify(p, 2)[0] for p in sample)
+    by_air = {}
+    for r, t, a in zip(reqs, times, arrivals):
+        by_air.setdefault("fibonacci" if r.air == vx.AIR_FIBONACCI else "d2_34col", []).append(t[2] - a)
     del proofs
     lat = [t[2] - a for t, a in zip(times, arrivals)]
     span = max(t[2] for t in times) - min(arrivals)
@@ -116,6 +128,7 @@ def run_server(vx, srv, reqs, arrivals):
         "p99_ms": round(percentile(lat, 99) * 1e3, 3),
         "max_ms": round(max(lat) * 1e3, 3),
         "span_s": round(span, 3),
+        "p99_ms_by_air": {k: round(percentile(v, 99) * 1e3, 3) for k, v in sorted(by_air.items())},
         "verified_sample": verified,
     }, digests
 
@@ -134,12 +147,12 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
     reqs = serving_requests(vx, n_requests, seed)
     plain = vx.Server(n_slots=1)
     batch = vx.Server(n_slots=slots)
-    classes = sorted({(r.log_n, r.blowup_bits, r.pow_bits) for r in reqs})
-    for (ln, bb, pw) in classes:
-        plain.prepare(vx.request(ln, blowup_bits=bb, pow_bits=pw))
-        batch.prepare(vx.request(ln, blowup_bits=bb, pow_bits=pw))
+    classes = sorted({(r.air, r.log_n, r.blowup_bits, r.pow_bits) for r in reqs})
+    warm = [vx.request(ln, air=air, blowup_bits=bb, pow_bits=pw) for (air, ln, bb, pw) in classes]
+    for w in warm:
+        plain.prepare(w)
+        batch.prepare(w)
     # warm every class once on every slot (first replays, page-in of pinned buffers)
-    warm = [vx.request(ln, blowup_bits=bb, pow_bits=pw) for (ln, bb, pw) in classes]
     plain.run_trace(warm, [0.0] * len(warm), keep=False)
     batch.run_trace(warm * slots, [0.0] * (len(warm) * slots), keep=False)
 
@@ -160,7 +173,7 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
     batch.close()
     return {
         "requests": n_requests,
-        "mix": "Fibonacci, log_n uniform 14..22, blowup 4, 80 queries, PoW 20 (KRAK v2)",
+        "mix": SERVING_MIX,
         "slots": slots,
         "lambda_per_s": round(lam, 1),
         "plain_loop": {"saturation": sat_plain, "poisson": open_plain},

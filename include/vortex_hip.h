@@ -230,6 +230,17 @@ uint64_t vx_prover_device_bytes(const vx_prover *p);
  * HOST copy of the last proof (valid after vx_prover_wait). */
 int vx_prover_buffer(const vx_prover *p, int which, const uint32_t **d_ptr, uint64_t *words);
 
+/* Synthetic multi-column AIR (the serving workload's 34-column request; no
+ * reference counterpart, modelled on the reference's 34-column Cairo trace,
+ * src/cairo_air/prover.rs:341-350): n_cols columns of seeded random M31
+ * values, n_cons degree-2 transition constraints c t_a t_b' + d t_e with
+ * coefficients from air_seed, trace committed in trees of <= 16 columns,
+ * KRAK v3 bytes (checked by vx_verify). The proof is made by the same
+ * vx_prover_* calls, with (a, b) = the trace seed's low and high words. */
+int vx_d2_prover_create(uint32_t log_n, uint32_t n_cols, uint32_t n_cons, uint64_t air_seed, uint32_t blowup_bits,
+                        uint32_t n_queries, uint32_t pow_bits, vx_stream_t stream, vx_prover **out);
+size_t vx_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries);
+
 /* Untimed proofs replay a HIP graph of the whole proof captured on first
  * use (default on); 0 launches the kernels one by one instead. Same bytes.
  * No counterpart in the reference (its prove_cached launches per call). */
@@ -254,11 +265,13 @@ int vx_verify(const uint8_t *proof, size_t len, uint32_t blowup_bits, uint32_t n
  * is the one-at-a-time loop. Proof bytes are identical to vx_prover_prove. */
 typedef struct vx_server vx_server;
 typedef struct vx_request {
-  uint32_t air; /* 0 = Fibonacci */
+  uint32_t air; /* 0 = Fibonacci, 1 = synthetic multi-column AIR */
   uint32_t log_n, blowup_bits, n_queries, pow_bits;
-  uint32_t a, b; /* Fibonacci public inputs */
+  uint32_t a, b;           /* Fibonacci public inputs */
+  uint32_t n_cols, n_cons; /* air 1: trace columns, constraints */
   uint32_t reserved;
-  uint64_t seed;
+  uint64_t seed;           /* air 1: trace seed */
+  uint64_t air_seed;       /* air 1: constraint coefficient seed */
 } vx_request;
 
 int vx_server_create(uint32_t n_slots, uint64_t mem_budget_bytes, int use_graphs, vx_server **out);

@@ -81,6 +81,14 @@ def lib(omp=False):
         "or_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
         "or_proof_size": (sz, [u32, u32, u32, u32]),
         "or_free": (None, [C.c_void_p]),
+        "or_d2_value": (u32, [u64, u32, u64]),
+        "or_d2_trace": (None, [u64, u32, u32, _u32p]),
+        "or_d2_coeffs": (None, [u64, u32, _u32p]),
+        "or_d2_quotient": (None, [_u32p, u32, u32, u32, u64, _u32p, _u32p, _u32p, _u32p, _u32p]),
+        "or_d2_prove": (C.c_int, [u64, u32, u32, u32, u64, u32, u32, u32,
+                                  C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz)]),
+        "or_d2_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
+        "or_d2_proof_size": (sz, [u32, u32, u32, u32]),
     }
     for name_, (res, args) in sig.items():
         f = getattr(L, name_)
@@ -299,3 +307,51 @@ class PyChannel:
 
     def grind(self, bits):
         return int(lib().or_grind_pow(C.byref(self.c), bits))
+
+
+# ---- synthetic degree-2 multi-column AIR (KRAK v3) ----
+D2_COLS, D2_CONS = 34, 35
+
+
+def d2_trace(seed, n_cols, log_n):
+    out = np.zeros(n_cols << log_n, np.uint32)
+    lib().or_d2_trace(seed, n_cols, log_n, out)
+    return out.reshape(n_cols, 1 << log_n)
+
+
+def d2_coeffs(air_seed, n_cons):
+    out = np.zeros(2 * n_cons, np.uint32)
+    lib().or_d2_coeffs(air_seed, n_cons, out)
+    return out
+
+
+def d2_quotient(eval_cols, n_cons, air_seed, alpha):
+    ev = np.ascontiguousarray(eval_cols, dtype=np.uint32)
+    n_cols, N = ev.shape
+    q = [np.zeros(N, np.uint32) for _ in range(4)]
+    lib().or_d2_quotient(ev.reshape(-1), n_cols, N.bit_length() - 1, n_cons, air_seed, u32(alpha), *q)
+    return q
+
+
+def d2_prove(seed, log_n, n_cols=D2_COLS, n_cons=D2_CONS, air_seed=0, blowup_bits=2, n_queries=80, pow_bits=0,
+             omp=False):
+    L = lib(omp)
+    p = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    rc = L.or_d2_prove(seed, log_n, n_cols, n_cons, air_seed, blowup_bits, n_queries, pow_bits, C.byref(p),
+                       C.byref(n))
+    if rc != 0:
+        raise ValueError(f"or_d2_prove rejected arguments (rc={rc})")
+    data = C.string_at(p, n.value)
+    L.or_free(p)
+    return data
+
+
+def d2_verify(proof: bytes, blowup_bits=2, n_queries=80):
+    err = C.create_string_buffer(256)
+    ok = lib().or_d2_verify(proof, len(proof), blowup_bits, n_queries, err, 256)
+    return bool(ok), err.value.decode()
+
+
+def d2_proof_size(log_n, n_cols=D2_COLS, blowup_bits=2, n_queries=80):
+    return lib().or_d2_proof_size(log_n, n_cols, blowup_bits, n_queries)

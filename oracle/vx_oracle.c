@@ -679,6 +679,128 @@ size_t or_proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t
     return 4 * words;
 }
 
+/* FRI commit phase and query draws (src/prover.rs:380-557), shared by the
+ * Fibonacci prover and the synthetic multi-column AIR below: circle fold of
+ * the quotient q (4 SoA columns on the eval domain of size 2^le), line folds
+ * down to 8 values, mix_felts(last layer), optional PoW, nq query draws. */
+typedef struct {
+    uint32_t le, n_fri, nq;
+    uint32_t *(*fl)[4];
+    tree_t *ftree;
+    uint32_t last[32];
+    uint64_t nonce;
+    uint64_t *qi;
+} fri_run_t;
+
+static void fri_commit_and_query(or_channel *ch, uint32_t *const q[4], uint32_t le, uint32_t nq,
+                                 uint32_t pow_bits, fri_run_t *f) {
+    uint64_t N = 1ull << le;
+    coset_t edom = eval_domain(le);
+    f->le = le;
+    f->nq = nq;
+    f->n_fri = le - 3;
+    uint32_t n_fri = f->n_fri;
+    f->fl = (uint32_t * (*)[4]) calloc(n_fri, sizeof *f->fl);
+    f->ftree = (tree_t *)calloc(n_fri, sizeof(tree_t));
+    uint32_t *(*fl)[4] = f->fl;
+    tree_t *ftree = f->ftree;
+    uint32_t *tw = (uint32_t *)malloc(N / 2 * 4);
+    {
+        uint32_t fa[4], fa2[4], c5[5];
+        or_channel_draw_felt(ch, fa);
+        or_qm31_mul(fa, fa, fa2);
+        coset_pack(edom, c5);
+        or_fold_twiddles(c5, 1, tw);
+        for (int k = 0; k < 4; k++) fl[0][k] = (uint32_t *)calloc(N / 2, 4);
+        or_fold_circle_into_line(fl[0], (const uint32_t *const *)q, tw, fa, fa2, N / 2);
+        tree_commit(&ftree[0], (const uint32_t *const *)fl[0], 4, le - 1);
+        or_channel_mix_digest(ch, tree_root(&ftree[0]));
+    }
+    uint32_t cur = le - 1;
+    for (uint32_t k = 1; k < n_fri; k++) {
+        uint32_t fa[4], c5[5];
+        or_channel_draw_felt(ch, fa);
+        coset_pack(coset_make(OR_HALF_ODDS, cur), c5);
+        or_fold_twiddles(c5, 0, tw);
+        uint64_t half = 1ull << (cur - 1);
+        for (int c = 0; c < 4; c++) fl[k][c] = (uint32_t *)malloc(half * 4);
+        or_fold_line((const uint32_t *const *)fl[k - 1], tw, fl[k], fa, half);
+        cur--;
+        tree_commit(&ftree[k], (const uint32_t *const *)fl[k], 4, cur);
+        or_channel_mix_digest(ch, tree_root(&ftree[k]));
+    }
+    free(tw);
+    for (int i = 0; i < 8; i++)
+        for (int c = 0; c < 4; c++) f->last[4 * i + c] = fl[n_fri - 1][c][i];
+    or_channel_mix_felts(ch, f->last, 8);
+    f->nonce = 0;
+    if (pow_bits) {
+        f->nonce = or_grind_pow(ch, pow_bits);
+        or_channel_mix_u64(ch, f->nonce);
+    }
+    f->qi = (uint64_t *)malloc((nq + 1) * 8);
+    for (uint32_t i = 0; i < nq; i++) f->qi[i] = or_channel_draw_number(ch, N);
+}
+/* n_fri, FRI roots, last layer, queries (stark_cli.rs:1100-1106) */
+static void fri_write_roots_and_queries(wbuf *w, const fri_run_t *f) {
+    wb_u32(w, f->n_fri);
+    for (uint32_t k = 0; k < f->n_fri; k++) wb_words(w, tree_root(&f->ftree[k]), 8);
+    wb_u32(w, 8);
+    wb_words(w, f->last, 32);
+    wb_u32(w, f->nq);
+    for (uint32_t i = 0; i < f->nq; i++) wb_u32(w, (uint32_t)f->qi[i]);
+}
+/* n_fri, then each FRI layer's decommitment at query >> (k + 1) */
+static void fri_write_decommits(wbuf *w, const fri_run_t *f) {
+    uint64_t *fi = (uint64_t *)malloc((f->nq + 1) * 8);
+    wb_u32(w, f->n_fri);
+    for (uint32_t i = 0; i < f->nq; i++) fi[i] = f->qi[i] / 2;
+    for (uint32_t k = 0; k < f->n_fri; k++) {
+        wb_decommit(w, &f->ftree[k], (const uint32_t *const *)f->fl[k], 4, fi, f->nq);
+        for (uint32_t i = 0; i < f->nq; i++) fi[i] /= 2;
+    }
+    free(fi);
+}
+static void fri_free(fri_run_t *f) {
+    for (uint32_t k = 0; k < f->n_fri; k++) {
+        tree_free(&f->ftree[k]);
+        for (int c = 0; c < 4; c++) free(f->fl[k][c]);
+    }
+    free(f->ftree);
+    free(f->fl);
+    free(f->qi);
+}
+
+/* In place: the first 2^log_n words of `col` (trace values on
+ * half_coset(log_n)) become the 2^le-point evaluation on the eval domain of
+ * the interpolated polynomial (src/prover.rs:280-318, 655-659). */
+static void lde_column(uint32_t *col, uint32_t log_n, uint32_t le) {
+    uint64_t n = 1ull << log_n, N = 1ull << le;
+    {
+        uint32_t c5[5];
+        coset_pack(coset_make(OR_HALF_COSET, log_n), c5);
+        uint32_t *ltw = (uint32_t *)malloc(n * 4), *ctw = (uint32_t *)malloc(n / 2 * 4);
+        uint32_t offs[32], sizes[32];
+        or_compute_twiddles(c5, ltw, ctw, offs, sizes);
+        or_batch_inverse(ltw, ltw, n - 2);
+        or_batch_inverse(ctw, ctw, n / 2);
+        or_ntt_interpolate(col, ltw, ctw, offs, log_n - 1, n);
+        free(ltw);
+        free(ctw);
+    }
+    for (uint64_t i = n; i < N; i++) col[i] = 0; /* zero_pad */
+    {
+        uint32_t c5[5];
+        coset_pack(eval_domain(le), c5);
+        uint32_t *ltw = (uint32_t *)malloc(N * 4), *ctw = (uint32_t *)malloc(N / 2 * 4);
+        uint32_t offs[32], sizes[32];
+        or_compute_twiddles(c5, ltw, ctw, offs, sizes);
+        or_ntt_evaluate(col, ltw, ctw, offs, le - 1, N);
+        free(ltw);
+        free(ctw);
+    }
+}
+
 /* src/prover.rs:264-619 (prove_lean_inner); prove_lean_fused and
  * prove_with_cache produce the same bytes. */
 int or_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint32_t nq,
@@ -689,36 +811,12 @@ int or_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint3
     uint32_t le = log_n + blowup_bits;
     uint64_t n = 1ull << log_n, N = 1ull << le;
 
-    /* trace + interpolate on half_coset(log_n) (prover.rs:280-300) */
-    uint32_t *coef = (uint32_t *)malloc(n * 4);
-    or_fibonacci_trace(a, b, log_n, coef);
-    {
-        uint32_t c5[5];
-        coset_pack(coset_make(OR_HALF_COSET, log_n), c5);
-        uint32_t *ltw = (uint32_t *)malloc(n * 4), *ctw = (uint32_t *)malloc(n / 2 * 4);
-        uint32_t offs[32], sizes[32];
-        or_compute_twiddles(c5, ltw, ctw, offs, sizes);
-        or_batch_inverse(ltw, ltw, n - 2);
-        or_batch_inverse(ctw, ctw, n / 2);
-        or_ntt_interpolate(coef, ltw, ctw, offs, log_n - 1, n);
-        free(ltw);
-        free(ctw);
-    }
-    /* zero-pad + evaluate on the eval domain (prover.rs:303-318, 655-659) */
+    /* trace, interpolate on half_coset(log_n), zero-pad, evaluate on the
+     * eval domain (prover.rs:280-318, 655-659) */
     uint32_t *ev = (uint32_t *)malloc(N * 4);
-    or_zero_pad(coef, ev, n, N);
-    free(coef);
-    coset_t edom = eval_domain(le);
-    {
-        uint32_t c5[5];
-        coset_pack(edom, c5);
-        uint32_t *ltw = (uint32_t *)malloc(N * 4), *ctw = (uint32_t *)malloc(N / 2 * 4);
-        uint32_t offs[32], sizes[32];
-        or_compute_twiddles(c5, ltw, ctw, offs, sizes);
-        or_ntt_evaluate(ev, ltw, ctw, offs, le - 1, N);
-        free(ltw);
-        free(ctw);
-    }
+    or_fibonacci_trace(a, b, log_n, ev);
+    lde_column(ev, log_n, le);
+    (void)n;
     /* trace commitment (prover.rs:321-324) */
     tree_t ttree;
     const uint32_t *evc[1] = {ev};
@@ -737,49 +835,10 @@ int or_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint3
     tree_commit(&qtree, (const uint32_t *const *)q, 4, le);
     or_channel_mix_digest(&ch, tree_root(&qtree));
 
-    /* FRI (prover.rs:380-548) */
-    uint32_t n_fri = le - 3;
-    uint32_t *(*fl)[4] = (uint32_t * (*)[4]) calloc(n_fri, sizeof *fl);
-    tree_t *ftree = (tree_t *)calloc(n_fri, sizeof(tree_t));
-    uint32_t *tw = (uint32_t *)malloc(N / 2 * 4);
-    {
-        uint32_t fa[4], fa2[4], c5[5];
-        or_channel_draw_felt(&ch, fa);
-        or_qm31_mul(fa, fa, fa2);
-        coset_pack(edom, c5);
-        or_fold_twiddles(c5, 1, tw);
-        for (int k = 0; k < 4; k++) fl[0][k] = (uint32_t *)calloc(N / 2, 4);
-        or_fold_circle_into_line(fl[0], (const uint32_t *const *)q, tw, fa, fa2, N / 2);
-        tree_commit(&ftree[0], (const uint32_t *const *)fl[0], 4, le - 1);
-        or_channel_mix_digest(&ch, tree_root(&ftree[0]));
-    }
-    uint32_t cur = le - 1;
-    for (uint32_t k = 1; k < n_fri; k++) {
-        uint32_t fa[4], c5[5];
-        or_channel_draw_felt(&ch, fa);
-        coset_pack(coset_make(OR_HALF_ODDS, cur), c5);
-        or_fold_twiddles(c5, 0, tw);
-        uint64_t half = 1ull << (cur - 1);
-        for (int c = 0; c < 4; c++) fl[k][c] = (uint32_t *)malloc(half * 4);
-        or_fold_line((const uint32_t *const *)fl[k - 1], tw, fl[k], fa, half);
-        cur--;
-        tree_commit(&ftree[k], (const uint32_t *const *)fl[k], 4, cur);
-        or_channel_mix_digest(&ch, tree_root(&ftree[k]));
-    }
-    free(tw);
-
-    /* queries (prover.rs:554-557) + optional PoW before the draws */
-    uint32_t last[32];
-    for (int i = 0; i < 8; i++)
-        for (int c = 0; c < 4; c++) last[4 * i + c] = fl[n_fri - 1][c][i];
-    or_channel_mix_felts(&ch, last, 8);
-    uint64_t nonce = 0;
-    if (pow_bits) {
-        nonce = or_grind_pow(&ch, pow_bits);
-        or_channel_mix_u64(&ch, nonce);
-    }
-    uint64_t *qi = (uint64_t *)malloc(nq * 8), *fi = (uint64_t *)malloc(nq * 8);
-    for (uint32_t i = 0; i < nq; i++) qi[i] = or_channel_draw_number(&ch, N);
+    /* FRI + queries (prover.rs:380-557) */
+    fri_run_t fr;
+    fri_commit_and_query(&ch, q, le, nq, pow_bits, &fr);
+    uint64_t nonce = fr.nonce, *qi = fr.qi;
 
     /* serialize (stark_cli.rs:1092-1120) */
     wbuf w = {0};
@@ -796,33 +855,16 @@ int or_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint3
     }
     wb_words(&w, tree_root(&ttree), 8);
     wb_words(&w, tree_root(&qtree), 8);
-    wb_u32(&w, n_fri);
-    for (uint32_t k = 0; k < n_fri; k++) wb_words(&w, tree_root(&ftree[k]), 8);
-    wb_u32(&w, 8);
-    wb_words(&w, last, 32);
-    wb_u32(&w, nq);
-    for (uint32_t i = 0; i < nq; i++) wb_u32(&w, (uint32_t)qi[i]);
+    fri_write_roots_and_queries(&w, &fr);
     wb_decommit(&w, &ttree, evc, 1, qi, nq);
     wb_decommit(&w, &qtree, (const uint32_t *const *)q, 4, qi, nq);
-    wb_u32(&w, n_fri);
-    for (uint32_t i = 0; i < nq; i++) fi[i] = qi[i] / 2;
-    for (uint32_t k = 0; k < n_fri; k++) {
-        wb_decommit(&w, &ftree[k], (const uint32_t *const *)fl[k], 4, fi, nq);
-        for (uint32_t i = 0; i < nq; i++) fi[i] /= 2;
-    }
+    fri_write_decommits(&w, &fr);
 
     tree_free(&ttree);
     tree_free(&qtree);
-    for (uint32_t k = 0; k < n_fri; k++) {
-        tree_free(&ftree[k]);
-        for (int c = 0; c < 4; c++) free(fl[k][c]);
-    }
-    free(ftree);
-    free(fl);
+    fri_free(&fr);
     for (int k = 0; k < 4; k++) free(q[k]);
     free(ev);
-    free(qi);
-    free(fi);
     *out = w.p;
     *out_len = w.len;
     return 0;
@@ -1037,5 +1079,408 @@ done:
     free(qidx);
     free(fidx);
     free(alphas);
+    return ok;
+}
+
+/* ======================================================================
+ * Synthetic degree-2 multi-column AIR ("d2"): the serving workload's
+ * 34-column request. No reference counterpart; its multi-column commitment
+ * follows the reference's Cairo trace (columns in Merkle trees of <= 16
+ * columns, src/cairo_air/prover.rs:341-350, because the leaf hash takes at
+ * most 16 words, cuda/blake2s.cu:48-53) and its verifier checks the
+ * constraints at the query rows from values at q and q + 1, as the Cairo
+ * verifier does (trace_values_at_queries[_next], prover.rs:415-419).
+ *
+ *   trace     t_c[i] = mix64(seed + G * ((c << 32 | i) + 1)) mod P,
+ *             G = 0x9E3779B97F4A7C15, mix64 = SplitMix64 finaliser
+ *   AIR       constraint k < K:  C_k(i) = c_k t_a[i] t_b[i+1] + d_k t_e[i],
+ *             a = k mod W, b = (5k + 1) mod W, e = (11k + 7) mod W,
+ *             (c_0, d_0, c_1, d_1, ...) = SplitMix64(air_seed) outputs mod P
+ *   quotient  Q(i) = sum_k alpha^(k+1) C_k(i) on the LDE (row i + 1 taken
+ *             mod N in eval order, as the Fibonacci quotient does)
+ *   transcript  mix the G = ceil(W / 16) trace roots, draw alpha, commit Q,
+ *             then exactly the Fibonacci FRI / PoW / query phase
+ *   bytes     KRAK v3: magic, 3, log_n, W, K, blowup_bits, pow_bits,
+ *             nonce lo/hi, seed lo/hi, air_seed lo/hi, G, G roots, Q root,
+ *             FRI roots, last layer, queries, per group: n, values at q,
+ *             values at q + 1, paths at q, paths at q + 1; then the
+ *             quotient and FRI decommitments as in KRAK v1.
+ * ====================================================================== */
+static inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+#define D2_GOLDEN 0x9E3779B97F4A7C15ull
+
+uint32_t or_d2_value(uint64_t seed, uint32_t col, uint64_t row) {
+    uint64_t key = ((uint64_t)col << 32) | row;
+    return (uint32_t)(mix64(seed + D2_GOLDEN * (key + 1)) % P);
+}
+void or_d2_trace(uint64_t seed, uint32_t n_cols, uint32_t log_n, uint32_t *out) {
+    uint64_t n = 1ull << log_n;
+    for (uint32_t c = 0; c < n_cols; c++)
+        for (uint64_t i = 0; i < n; i++) out[c * n + i] = or_d2_value(seed, c, i);
+}
+void or_d2_coeffs(uint64_t air_seed, uint32_t n_cons, uint32_t *cd) {
+    uint64_t s = air_seed;
+    for (uint32_t k = 0; k < 2 * n_cons; k++) {
+        s += D2_GOLDEN;
+        cd[k] = (uint32_t)(mix64(s) % P);
+    }
+}
+static inline uint32_t d2_a(uint32_t k, uint32_t w) { return k % w; }
+static inline uint32_t d2_b(uint32_t k, uint32_t w) { return (5 * k + 1) % w; }
+static inline uint32_t d2_e(uint32_t k, uint32_t w) { return (11 * k + 7) % w; }
+
+/* Q at row i of the evaluation (cols: W columns of N values). */
+static qm31 d2_q_at(const uint32_t *const *cols, uint32_t w, uint32_t k_cons, const uint32_t *cd,
+                    const qm31 *beta, uint64_t i, uint64_t mask) {
+    qm31 acc = {{0, 0, 0, 0}};
+    uint64_t j = (i + 1) & mask;
+    for (uint32_t k = 0; k < k_cons; k++) {
+        uint32_t m = m31_add(m31_mul(cd[2 * k], m31_mul(cols[d2_a(k, w)][i], cols[d2_b(k, w)][j])),
+                             m31_mul(cd[2 * k + 1], cols[d2_e(k, w)][i]));
+        acc = q_add(acc, q_scale(beta[k], m));
+    }
+    return acc;
+}
+/* Same from the values of one row and of the next (verifier side). */
+static qm31 d2_q_rows(const uint32_t *row, const uint32_t *next, uint32_t w, uint32_t k_cons,
+                      const uint32_t *cd, const qm31 *beta) {
+    qm31 acc = {{0, 0, 0, 0}};
+    for (uint32_t k = 0; k < k_cons; k++) {
+        uint32_t m = m31_add(m31_mul(cd[2 * k], m31_mul(row[d2_a(k, w)], next[d2_b(k, w)])),
+                             m31_mul(cd[2 * k + 1], row[d2_e(k, w)]));
+        acc = q_add(acc, q_scale(beta[k], m));
+    }
+    return acc;
+}
+static void d2_powers(const uint32_t alpha[4], uint32_t k_cons, qm31 *beta) {
+    qm31 a = q_load(alpha), p = a;
+    for (uint32_t k = 0; k < k_cons; k++) {
+        beta[k] = p;
+        p = q_mul(p, a);
+    }
+}
+void or_d2_quotient(const uint32_t *eval, uint32_t n_cols, uint32_t le, uint32_t n_cons, uint64_t air_seed,
+                    const uint32_t alpha[4], uint32_t *q0, uint32_t *q1, uint32_t *q2, uint32_t *q3) {
+    uint64_t N = 1ull << le;
+    const uint32_t **cols = (const uint32_t **)malloc(n_cols * sizeof(uint32_t *));
+    for (uint32_t c = 0; c < n_cols; c++) cols[c] = eval + c * N;
+    uint32_t *cd = (uint32_t *)malloc(8 * n_cons + 8);
+    or_d2_coeffs(air_seed, n_cons, cd);
+    qm31 *beta = (qm31 *)malloc(sizeof(qm31) * (n_cons + 1));
+    d2_powers(alpha, n_cons, beta);
+    uint32_t *o[4] = {q0, q1, q2, q3};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)N; i++) {
+        qm31 v = d2_q_at(cols, n_cols, n_cons, cd, beta, (uint64_t)i, N - 1);
+        for (int c = 0; c < 4; c++) o[c][i] = v.v[c];
+    }
+    free(cols);
+    free(cd);
+    free(beta);
+}
+
+static uint32_t d2_groups(uint32_t w) { return (w + 15) / 16; }
+static uint32_t d2_group_cols(uint32_t w, uint32_t g) { return w - 16 * g < 16 ? w - 16 * g : 16; }
+#define KRAK_D2_HEADER 13
+
+size_t or_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, uint32_t nq) {
+    uint32_t le = log_n + blowup_bits, n_fri = le - 3, G = d2_groups(n_cols);
+    size_t words = KRAK_D2_HEADER + 1 + 8 * (size_t)G + 8 + 1 + 8 * (size_t)n_fri + 1 + 32 + 1 + nq;
+    for (uint32_t g = 0; g < G; g++)
+        words += 1 + 2 * (size_t)nq * d2_group_cols(n_cols, g) + 2 * (size_t)nq * (1 + 8 * (size_t)le);
+    words += 1 + 8 * (size_t)nq + 2 * (size_t)nq * (1 + 8 * (size_t)le);
+    words += 1;
+    for (uint32_t k = 0; k < n_fri; k++) {
+        uint32_t L = le - 1 - k;
+        words += 1 + 8 * (size_t)nq + 2 * (size_t)nq * (1 + 8 * (size_t)L);
+    }
+    return 4 * words;
+}
+
+int or_d2_prove(uint64_t seed, uint32_t log_n, uint32_t n_cols, uint32_t n_cons, uint64_t air_seed,
+                uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits, uint8_t **out, size_t *out_len) {
+    if (log_n < 4 || blowup_bits < 1 || log_n + blowup_bits > 31 || nq == 0 || n_cols < 1 || n_cols > 256 ||
+        n_cons < 1 || n_cons > 1024)
+        return -1;
+    uint32_t le = log_n + blowup_bits, G = d2_groups(n_cols);
+    uint64_t n = 1ull << log_n, N = 1ull << le;
+    uint32_t *ev = (uint32_t *)malloc((size_t)n_cols * N * 4);
+    for (uint32_t c = 0; c < n_cols; c++) {
+        uint32_t *col = ev + c * N;
+        for (uint64_t i = 0; i < n; i++) col[i] = or_d2_value(seed, c, i);
+        lde_column(col, log_n, le);
+    }
+    /* trace trees, one per group of <= 16 columns */
+    tree_t *tt = (tree_t *)calloc(G, sizeof(tree_t));
+    const uint32_t **cols = (const uint32_t **)malloc(n_cols * sizeof(uint32_t *));
+    for (uint32_t c = 0; c < n_cols; c++) cols[c] = ev + c * N;
+    or_channel ch;
+    or_channel_init(&ch);
+    for (uint32_t g = 0; g < G; g++) {
+        tree_commit(&tt[g], cols + 16 * g, d2_group_cols(n_cols, g), le);
+        or_channel_mix_digest(&ch, tree_root(&tt[g]));
+    }
+    uint32_t alpha[4];
+    or_channel_draw_felt(&ch, alpha);
+    uint32_t *q[4];
+    for (int k = 0; k < 4; k++) q[k] = (uint32_t *)malloc(N * 4);
+    or_d2_quotient(ev, n_cols, le, n_cons, air_seed, alpha, q[0], q[1], q[2], q[3]);
+    tree_t qtree;
+    tree_commit(&qtree, (const uint32_t *const *)q, 4, le);
+    or_channel_mix_digest(&ch, tree_root(&qtree));
+    fri_run_t fr;
+    fri_commit_and_query(&ch, q, le, nq, pow_bits, &fr);
+
+    wbuf w = {0};
+    wb_u32(&w, KRAK_MAGIC);
+    wb_u32(&w, 3);
+    wb_u32(&w, log_n);
+    wb_u32(&w, n_cols);
+    wb_u32(&w, n_cons);
+    wb_u32(&w, blowup_bits);
+    wb_u32(&w, pow_bits);
+    wb_u32(&w, (uint32_t)fr.nonce);
+    wb_u32(&w, (uint32_t)(fr.nonce >> 32));
+    wb_u32(&w, (uint32_t)seed);
+    wb_u32(&w, (uint32_t)(seed >> 32));
+    wb_u32(&w, (uint32_t)air_seed);
+    wb_u32(&w, (uint32_t)(air_seed >> 32));
+    wb_u32(&w, G);
+    for (uint32_t g = 0; g < G; g++) wb_words(&w, tree_root(&tt[g]), 8);
+    wb_words(&w, tree_root(&qtree), 8);
+    fri_write_roots_and_queries(&w, &fr);
+    uint64_t *nx = (uint64_t *)malloc((nq + 1) * 8);
+    for (uint32_t i = 0; i < nq; i++) nx[i] = (fr.qi[i] + 1) & (N - 1);
+    for (uint32_t g = 0; g < G; g++) {
+        uint32_t wg = d2_group_cols(n_cols, g);
+        const uint32_t *const *gc = cols + 16 * g;
+        wb_u32(&w, nq);
+        for (uint32_t i = 0; i < nq; i++)
+            for (uint32_t c = 0; c < wg; c++) wb_u32(&w, gc[c][fr.qi[i]]);
+        for (uint32_t i = 0; i < nq; i++)
+            for (uint32_t c = 0; c < wg; c++) wb_u32(&w, gc[c][nx[i]]);
+        for (uint32_t i = 0; i < nq; i++) wb_path(&w, &tt[g], fr.qi[i]);
+        for (uint32_t i = 0; i < nq; i++) wb_path(&w, &tt[g], nx[i]);
+    }
+    wb_decommit(&w, &qtree, (const uint32_t *const *)q, 4, fr.qi, nq);
+    fri_write_decommits(&w, &fr);
+
+    for (uint32_t g = 0; g < G; g++) tree_free(&tt[g]);
+    free(tt);
+    tree_free(&qtree);
+    fri_free(&fr);
+    for (int k = 0; k < 4; k++) free(q[k]);
+    free(cols);
+    free(ev);
+    free(nx);
+    *out = w.p;
+    *out_len = w.len;
+    return 0;
+}
+
+/* Trace decommitment of one group: values at q and at q + 1, auth paths of both. */
+typedef struct {
+    uint32_t n, width;
+    uint32_t *vals, *next;
+    uint32_t **paths, **npaths;
+    uint32_t *depth, *ndepth;
+} d2dec_t;
+static void d2dec_free(d2dec_t *d) {
+    if (d->paths)
+        for (uint32_t i = 0; i < d->n; i++) { free(d->paths[i]); free(d->npaths[i]); }
+    free(d->vals); free(d->next); free(d->paths); free(d->npaths); free(d->depth); free(d->ndepth);
+    memset(d, 0, sizeof *d);
+}
+static int rb_d2dec(rbuf *r, d2dec_t *d, uint32_t width) {
+    memset(d, 0, sizeof *d);
+    d->width = width;
+    d->n = rb_u32(r);
+    if (r->bad || d->n > (1u << 20) || (size_t)d->n * width * 8 > r->len) return 0;
+    uint32_t n = d->n;
+    d->vals = (uint32_t *)calloc((size_t)n * width + 1, 4);
+    d->next = (uint32_t *)calloc((size_t)n * width + 1, 4);
+    d->paths = (uint32_t **)calloc(n + 1, sizeof(uint32_t *));
+    d->npaths = (uint32_t **)calloc(n + 1, sizeof(uint32_t *));
+    d->depth = (uint32_t *)calloc(n + 1, 4);
+    d->ndepth = (uint32_t *)calloc(n + 1, 4);
+    for (size_t i = 0; i < (size_t)n * width; i++) d->vals[i] = rb_u32(r);
+    for (size_t i = 0; i < (size_t)n * width; i++) d->next[i] = rb_u32(r);
+    for (int s = 0; s < 2; s++)
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t dep = rb_u32(r);
+            if (r->bad || dep > 64) return 0;
+            uint32_t *pth = (uint32_t *)calloc((size_t)dep * 8 + 1, 4);
+            for (uint32_t k = 0; k < 8 * dep; k++) pth[k] = rb_u32(r);
+            if (s == 0) { d->depth[i] = dep; d->paths[i] = pth; }
+            else { d->ndepth[i] = dep; d->npaths[i] = pth; }
+        }
+    return !r->bad;
+}
+
+int or_d2_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits, uint32_t nq, char *err, size_t err_cap) {
+    int ok = 0;
+    rbuf r = {bytes, len, 0, 0};
+    decom_t qd = {0}, *fd = NULL;
+    d2dec_t *td = NULL;
+    uint32_t G = 0, n_fd = 0, n_fri = 0, n_last = 0, n_q = 0;
+    uint32_t (*troots)[8] = NULL, (*froots)[8] = NULL, *lastw = NULL, *cd = NULL, *row = NULL, *nrow = NULL;
+    uint64_t *qidx = NULL, *fidx = NULL;
+    qm31 *alphas = NULL, *beta = NULL;
+    char dummy[8];
+    if (!err || err_cap == 0) { err = dummy; err_cap = sizeof dummy; }
+    err[0] = 0;
+
+    if (rb_u32(&r) != KRAK_MAGIC) VFAIL("bad magic");
+    if (rb_u32(&r) != 3) VFAIL("unsupported proof version");
+    uint32_t log_n = rb_u32(&r), W = rb_u32(&r), K = rb_u32(&r), bb = rb_u32(&r), pow_bits = rb_u32(&r);
+    uint64_t nonce = rb_u32(&r);
+    nonce |= (uint64_t)rb_u32(&r) << 32;
+    uint64_t seed = rb_u32(&r);
+    seed |= (uint64_t)rb_u32(&r) << 32;
+    uint64_t air_seed = rb_u32(&r);
+    air_seed |= (uint64_t)rb_u32(&r) << 32;
+    (void)seed;
+    if (r.bad) VFAIL("truncated proof");
+    if (bb != blowup_bits) VFAIL("blowup mismatch");
+    if (W < 1 || W > 256 || K < 1 || K > 1024) VFAIL("bad AIR shape");
+    if (pow_bits > 128) VFAIL("bad pow bits");
+    G = rb_u32(&r);
+    if (r.bad || G != d2_groups(W)) VFAIL("bad trace group count");
+    troots = (uint32_t(*)[8])calloc(G, 32);
+    for (uint32_t g = 0; g < G; g++)
+        for (int i = 0; i < 8; i++) troots[g][i] = rb_u32(&r);
+    uint32_t qroot[8];
+    for (int i = 0; i < 8; i++) qroot[i] = rb_u32(&r);
+    n_fri = rb_u32(&r);
+    if (r.bad || n_fri > 64) VFAIL("truncated proof");
+    froots = (uint32_t(*)[8])calloc(n_fri + 1, 32);
+    for (uint32_t k = 0; k < n_fri; k++)
+        for (int i = 0; i < 8; i++) froots[k][i] = rb_u32(&r);
+    n_last = rb_u32(&r);
+    if (r.bad || n_last > 4096) VFAIL("truncated proof");
+    lastw = (uint32_t *)calloc(4 * (size_t)n_last + 1, 4);
+    for (size_t i = 0; i < 4 * (size_t)n_last; i++) lastw[i] = rb_u32(&r);
+    n_q = rb_u32(&r);
+    if (r.bad || n_q > (1u << 20)) VFAIL("truncated proof");
+    qidx = (uint64_t *)calloc(n_q + 1, 8);
+    for (uint32_t i = 0; i < n_q; i++) qidx[i] = rb_u32(&r);
+    td = (d2dec_t *)calloc(G, sizeof(d2dec_t));
+    for (uint32_t g = 0; g < G; g++)
+        if (!rb_d2dec(&r, &td[g], d2_group_cols(W, g))) VFAIL("truncated proof");
+    if (!rb_decom(&r, &qd, 4)) VFAIL("truncated proof");
+    n_fd = rb_u32(&r);
+    if (r.bad || n_fd > 64) VFAIL("truncated proof");
+    fd = (decom_t *)calloc(n_fd + 1, sizeof(decom_t));
+    for (uint32_t k = 0; k < n_fd; k++)
+        if (!rb_decom(&r, &fd[k], 4)) VFAIL("truncated proof");
+    if (r.pos != r.len) VFAIL("trailing bytes");
+
+    uint32_t le = log_n + blowup_bits;
+    if (log_n < 1 || le > 31) VFAIL("bad log size");
+    uint64_t N = 1ull << le;
+    if (n_q != nq) VFAIL("Expected %u queries, got %u", nq, n_q);
+    or_channel ch;
+    or_channel_init(&ch);
+    for (uint32_t g = 0; g < G; g++) or_channel_mix_digest(&ch, troots[g]);
+    uint32_t alpha[4];
+    or_channel_draw_felt(&ch, alpha);
+    or_channel_mix_digest(&ch, qroot);
+    alphas = (qm31 *)calloc(n_fri + 2, sizeof(qm31));
+    or_channel_draw_felt(&ch, alphas[0].v);
+    for (uint32_t k = 0; k < n_fri; k++) {
+        or_channel_mix_digest(&ch, froots[k]);
+        or_channel_draw_felt(&ch, alphas[k + 1].v);
+    }
+    if (n_fri != (le >= 3 ? le - 3 : 0)) VFAIL("Expected %u FRI layers, got %u", le - 3, n_fri);
+    if (n_last != 8) VFAIL("Expected 8 FRI last layer values, got %u", n_last);
+    or_channel_mix_felts(&ch, lastw, n_last);
+    if (pow_bits) {
+        if (!or_channel_verify_pow_nonce(&ch, pow_bits, nonce)) VFAIL("PoW nonce invalid");
+        or_channel_mix_u64(&ch, nonce);
+    } else if (nonce != 0) {
+        VFAIL("nonce without PoW");
+    }
+    for (uint32_t i = 0; i < nq; i++)
+        if (or_channel_draw_number(&ch, N) != qidx[i]) VFAIL("Query indices don't match Fiat-Shamir derivation");
+    /* trace groups: auth paths at q and q + 1 */
+    for (uint32_t g = 0; g < G; g++) {
+        const d2dec_t *d = &td[g];
+        if (d->n != nq) VFAIL("trace group %u decommitment size mismatch", g);
+        for (uint32_t q = 0; q < nq; q++) {
+            uint32_t h[8];
+            uint64_t nx = (qidx[q] + 1) & (N - 1);
+            hash_words(d->vals + (size_t)q * d->width, d->width, h);
+            if (d->depth[q] != le || !or_verify_auth_path(troots[g], h, qidx[q], d->paths[q], d->depth[q]))
+                VFAIL("trace group %u auth path invalid", g);
+            hash_words(d->next + (size_t)q * d->width, d->width, h);
+            if (d->ndepth[q] != le || !or_verify_auth_path(troots[g], h, nx, d->npaths[q], d->ndepth[q]))
+                VFAIL("trace group %u next-row auth path invalid", g);
+        }
+    }
+    if (!check_decom(&qd, qroot, qidx, nq)) VFAIL("quotient auth path invalid");
+    /* the AIR: Q(q) from the committed rows q and q + 1 */
+    cd = (uint32_t *)malloc(8 * (size_t)K + 8);
+    or_d2_coeffs(air_seed, K, cd);
+    beta = (qm31 *)malloc(sizeof(qm31) * (K + 1));
+    d2_powers(alpha, K, beta);
+    row = (uint32_t *)malloc(4 * (size_t)W);
+    nrow = (uint32_t *)malloc(4 * (size_t)W);
+    for (uint32_t q = 0; q < nq; q++) {
+        for (uint32_t g = 0; g < G; g++)
+            for (uint32_t c = 0; c < td[g].width; c++) {
+                row[16 * g + c] = td[g].vals[(size_t)q * td[g].width + c];
+                nrow[16 * g + c] = td[g].next[(size_t)q * td[g].width + c];
+            }
+        qm31 e = d2_q_rows(row, nrow, W, K, cd, beta);
+        if (memcmp(e.v, qd.vals + 4 * (size_t)q, 16)) VFAIL("Constraint quotient mismatch at query %u", q);
+    }
+    fidx = (uint64_t *)calloc(nq + 1, 8);
+    for (uint32_t i = 0; i < nq; i++) fidx[i] = qidx[i] / 2;
+    for (uint32_t k = 0; k < n_fd && k < n_fri; k++) {
+        if (!check_decom(&fd[k], froots[k], fidx, nq)) VFAIL("FRI layer %u auth path invalid", k);
+        for (uint32_t i = 0; i < nq; i++) fidx[i] /= 2;
+    }
+    if (n_fd != n_fri || n_fd == 0) VFAIL("FRI decommitment count mismatch");
+    for (uint32_t q = 0; q < nq; q++) {
+        uint64_t ci = qidx[q];
+        uint32_t cl = le;
+        qm31 f0 = q_load(qd.vals + 4 * (size_t)q), f1 = q_load(qd.sibs + 4 * (size_t)q);
+        if (ci % 2) { qm31 t = f0; f0 = f1; f1 = t; }
+        qm31 e = fold_pair(f0, f1, alphas[0], fold_twiddle_at(eval_domain(cl), ci / 2, 1));
+        if (memcmp(e.v, fd[0].vals + 4 * (size_t)q, 16)) VFAIL("Circle fold mismatch at query %u", q);
+        ci /= 2;
+        cl--;
+        for (uint32_t k = 0; k + 1 < n_fd; k++) {
+            qm31 ex = pair_fold(&fd[k], q, ci, alphas[k + 1], fold_twiddle_at(coset_make(OR_HALF_ODDS, cl), ci / 2, 0));
+            if (memcmp(ex.v, fd[k + 1].vals + 4 * (size_t)q, 16)) VFAIL("Line fold mismatch at query %u, layer %u", q, k);
+            ci /= 2;
+            cl--;
+        }
+        if (ci >= n_last) VFAIL("FRI last layer index out of bounds");
+        if (memcmp(fd[n_fd - 1].vals + 4 * (size_t)q, lastw + 4 * ci, 16)) VFAIL("FRI last layer mismatch at query %u", q);
+    }
+    ok = 1;
+done:
+    if (td)
+        for (uint32_t g = 0; g < G; g++) d2dec_free(&td[g]);
+    free(td);
+    decom_free(&qd);
+    if (fd)
+        for (uint32_t k = 0; k < n_fd; k++) decom_free(&fd[k]);
+    free(fd);
+    free(troots);
+    free(froots);
+    free(lastw);
+    free(qidx);
+    free(fidx);
+    free(alphas);
+    free(beta);
+    free(cd);
+    free(row);
+    free(nrow);
     return ok;
 }

@@ -119,6 +119,20 @@ size_t or_proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries,
                      uint32_t pow_bits);
 void or_free(void *p);
 
+/* ---- synthetic degree-2 multi-column AIR (build-defined; no reference
+ *      counterpart): W columns, K constraints C_k = c_k t_a t_b' + d_k t_e,
+ *      proof bytes KRAK v3 (definition above or_d2_value in vx_oracle.c) ---- */
+uint32_t or_d2_value(uint64_t seed, uint32_t col, uint64_t row);
+void or_d2_trace(uint64_t seed, uint32_t n_cols, uint32_t log_n, uint32_t *out);
+void or_d2_coeffs(uint64_t air_seed, uint32_t n_cons, uint32_t *cd);
+void or_d2_quotient(const uint32_t *eval, uint32_t n_cols, uint32_t le, uint32_t n_cons, uint64_t air_seed,
+                    const uint32_t alpha[4], uint32_t *q0, uint32_t *q1, uint32_t *q2, uint32_t *q3);
+int or_d2_prove(uint64_t seed, uint32_t log_n, uint32_t n_cols, uint32_t n_cons, uint64_t air_seed,
+                uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, uint8_t **out, size_t *out_len);
+int or_d2_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits, uint32_t n_queries, char *err,
+                 size_t err_cap);
+size_t or_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries);
+
 #ifdef __cplusplus
 }
 #endif

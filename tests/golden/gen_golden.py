@@ -8,10 +8,14 @@ Sources, per section:
   proofs_c     the C oracle (oracle/liboracle_omp.so) at sizes pyref cannot
                reach; pinned transitively by proofs_py agreement at small sizes
   pow          brute-force smallest nonce with hashlib
+  proofs_d2_py pyref.prove_d2 (synthetic 34-column AIR, KRAK v3), independent
+               of the C oracle
+  proofs_d2_c  the C oracle's multi-column AIR proofs at larger sizes
 
 Run from the repo root:  python tests/golden/gen_golden.py [--big]
-(--big adds the log_n = 20, 24 and 25 (blowup 1) oracle digests and the
-log_n = 24 PoW-26 digest, which take minutes.)
+(--big adds the log_n = 20, 24 and 25 (blowup 1) oracle digests, the
+log_n = 24 PoW-26 digest and the log_n = 20 multi-column AIR digest, which
+take minutes.)
 """
 import hashlib
 import json
@@ -133,11 +137,30 @@ def main():
                               "len": len(data), "sha256": hashlib.sha256(data).hexdigest()})
         print("oracle digest", log_n, bb, pw, len(data), flush=True)
 
+    # synthetic degree-2 multi-column AIR (KRAK v3): the Python restatement
+    # at small sizes, the C oracle at larger ones
+    g["proofs_d2_py"] = []
+    for sd, log_n, w, k, air, bb, nq, pw in [(7, 4, 34, 35, 99, 2, 80, 0), (7, 4, 3, 2, 99, 1, 5, 0),
+                                             (11, 5, 17, 4, 5, 2, 10, 6), (3, 5, 34, 35, 0xD2A1, 2, 80, 4)]:
+        data = pyref.prove_d2(sd, log_n, w, k, air, bb, nq, pw)
+        g["proofs_d2_py"].append({"seed": sd, "log_n": log_n, "n_cols": w, "n_cons": k, "air_seed": air,
+                                  "blowup_bits": bb, "n_queries": nq, "pow_bits": pw, "len": len(data),
+                                  "sha256": hashlib.sha256(data).hexdigest()})
+    g["proofs_d2_c"] = []
+    d2_sizes = [(10, 0), (14, 20), (16, 0)] + ([(20, 20)] if big else [])
+    for log_n, pw in d2_sizes:
+        data = pyoracle.d2_prove(seed, log_n, 34, 35, 0xD2A1, 2, 80, pw, omp=True)
+        g["proofs_d2_c"].append({"seed": seed, "log_n": log_n, "n_cols": 34, "n_cons": 35, "air_seed": 0xD2A1,
+                                 "blowup_bits": 2, "n_queries": 80, "pow_bits": pw, "len": len(data),
+                                 "sha256": hashlib.sha256(data).hexdigest()})
+        print("oracle d2 digest", log_n, pw, len(data), flush=True)
+
     out = os.path.join(HERE, "golden.json")
     if not big and os.path.exists(out):
         old = json.load(open(out))
         keep = [p for p in old.get("proofs_c", []) if p["log_n"] >= 20]
         g["proofs_c"] += keep
+        g["proofs_d2_c"] += [p for p in old.get("proofs_d2_c", []) if p["log_n"] >= 20]
     with open(out, "w") as f:
         json.dump(g, f, separators=(",", ":"))
     print("wrote", out, os.path.getsize(out), "bytes")

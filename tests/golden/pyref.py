@@ -298,3 +298,119 @@ def prove(a, b, log_n, blowup_bits=2, n_queries=80, pow_bits=0):
         out += decom(vals, layers, idx)
         idx = [i // 2 for i in idx]
     return le_words(out)
+
+
+# ---------------- synthetic degree-2 multi-column AIR (build-defined) ----------------
+# Restated from the definition (not from oracle/vx_oracle.c): W columns of
+# SplitMix64-derived values, K constraints C_k = c_k t_a t_b' + d_k t_e with
+# a = k mod W, b = (5k + 1) mod W, e = (11k + 7) mod W; Q = sum alpha^(k+1) C_k;
+# trace committed in trees of <= 16 columns; KRAK v3 bytes.
+
+M64 = (1 << 64) - 1
+GOLDEN = 0x9E3779B97F4A7C15
+
+
+def mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def d2_value(seed, col, row):
+    return mix64((seed + GOLDEN * (((col << 32) | row) + 1)) & M64) % P
+
+
+def d2_coeffs(air_seed, k_cons):
+    s, out = air_seed, []
+    for _ in range(2 * k_cons):
+        s = (s + GOLDEN) & M64
+        out.append(mix64(s) % P)
+    return out
+
+
+def d2_q(row, nxt, w, k_cons, cd, alpha):
+    acc, beta = (0, 0, 0, 0), alpha
+    for k in range(k_cons):
+        m = (cd[2 * k] * row[k % w] * nxt[(5 * k + 1) % w] + cd[2 * k + 1] * row[(11 * k + 7) % w]) % P
+        acc = qm_add(acc, qm_scale(beta, m))
+        beta = qm_mul(beta, alpha)
+    return acc
+
+
+def prove_d2(seed, log_n, w=34, k_cons=35, air_seed=0, blowup_bits=2, n_queries=80, pow_bits=0):
+    le = log_n + blowup_bits
+    N = 1 << le
+    hc, ed = half_coset(log_n), eval_domain(le)
+    cols = []
+    for c in range(w):
+        coeffs = interpolate([d2_value(seed, c, i) for i in range(1 << log_n)], hc)
+        cols.append(evaluate(coeffs + [0] * (N - len(coeffs)), ed))
+    groups = [list(range(g, min(g + 16, w))) for g in range(0, w, 16)]
+    t_layers = [merkle_layers([[cols[c][i] for c in grp] for i in range(N)]) for grp in groups]
+    ch = Channel()
+    for tl in t_layers:
+        ch.mix_digest(words(tl[-1][0]))
+    alpha = ch.draw_felt()
+    cd = d2_coeffs(air_seed, k_cons)
+    q = [d2_q([cols[c][i] for c in range(w)], [cols[c][(i + 1) % N] for c in range(w)], w, k_cons, cd, alpha)
+         for i in range(N)]
+    q_layers = merkle_layers(q)
+    ch.mix_digest(words(q_layers[-1][0]))
+    alpha0 = ch.draw_felt()
+    layer = [fold(q[2 * i], q[2 * i + 1], alpha0, fold_tw(ed, i, True)) for i in range(N // 2)]
+    fri = [(layer, merkle_layers(layer))]
+    ch.mix_digest(words(fri[-1][1][-1][0]))
+    cur = le - 1
+    while cur > 3:
+        al = ch.draw_felt()
+        d = half_odds(cur)
+        layer = [fold(layer[2 * i], layer[2 * i + 1], al, fold_tw(d, i, False)) for i in range(len(layer) // 2)]
+        cur -= 1
+        fri.append((layer, merkle_layers(layer)))
+        ch.mix_digest(words(fri[-1][1][-1][0]))
+    ch.mix_felts(layer)
+    nonce = 0
+    if pow_bits:
+        while not ch.pow_ok(pow_bits, nonce):
+            nonce += 1
+        ch.mix_u64(nonce)
+    qis = [ch.draw_number(N) for _ in range(n_queries)]
+
+    out = [0x4B52414B, 3, log_n, w, k_cons, blowup_bits, pow_bits, nonce & 0xFFFFFFFF, nonce >> 32,
+           seed & 0xFFFFFFFF, seed >> 32, air_seed & 0xFFFFFFFF, air_seed >> 32, len(groups)]
+    for tl in t_layers:
+        out += words(tl[-1][0])
+    out += words(q_layers[-1][0])
+    out += [len(fri)] + sum((words(f[1][-1][0]) for f in fri), [])
+    out += [len(layer)] + sum((list(v) for v in layer), [])
+    out += [len(qis)] + qis
+    for grp, tl in zip(groups, t_layers):
+        nx = [(i + 1) % N for i in qis]
+        out += [len(qis)]
+        for idxs in (qis, nx):
+            for i in idxs:
+                out += [cols[c][i] for c in grp]
+        for idxs in (qis, nx):
+            for i in idxs:
+                pth = auth_path(tl, i)
+                out += [len(pth)] + sum(pth, [])
+
+    def decom(vals, layers, idxs):
+        res = [len(idxs)]
+        for i in idxs:
+            res += list(vals[i])
+        for i in idxs:
+            res += list(vals[i ^ 1])
+        for sel in (0, 1):
+            for i in idxs:
+                pth = auth_path(layers, i ^ sel)
+                res += [len(pth)] + sum(pth, [])
+        return res
+
+    out += decom(q, q_layers, qis)
+    out += [len(fri)]
+    idx = [i // 2 for i in qis]
+    for vals, layers in fri:
+        out += decom(vals, layers, idx)
+        idx = [i // 2 for i in idx]
+    return le_words(out)

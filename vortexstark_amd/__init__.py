@@ -79,6 +79,35 @@ class Prover:
             pass
 
 
+D2_COLS, D2_CONS, D2_AIR_SEED = 34, 35, 0xD2A1
+
+
+def d2_proof_size(log_n, n_cols=D2_COLS, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES):
+    return int(load().vx_d2_proof_size(log_n, n_cols, blowup_bits, n_queries))
+
+
+class D2Prover(Prover):
+    """Cached prover of the synthetic multi-column AIR (KRAK v3): n_cols
+    seeded random M31 columns, n_cons degree-2 transition constraints."""
+
+    def __init__(self, log_n, n_cols=D2_COLS, n_cons=D2_CONS, air_seed=D2_AIR_SEED, blowup_bits=BLOWUP_BITS,
+                 n_queries=N_QUERIES, pow_bits=0, stream=None, graphs=True):
+        self.log_n, self.blowup_bits, self.n_queries, self.pow_bits = log_n, blowup_bits, n_queries, pow_bits
+        self.n_cols, self.n_cons, self.air_seed = n_cols, n_cons, air_seed
+        self._h = C.c_void_p()
+        _lib.call("vx_d2_prover_create", log_n, n_cols, n_cons, air_seed, blowup_bits, n_queries, pow_bits, stream,
+                  C.byref(self._h))
+        self.size = d2_proof_size(log_n, n_cols, blowup_bits, n_queries)
+        _lib.call("vx_prover_set_graphs", self._h, 1 if graphs else 0)
+
+    def prove(self, seed):
+        self.enqueue(seed)
+        return self.wait()
+
+    def enqueue(self, seed, timed=False):
+        _lib.call("vx_prover_enqueue", self._h, seed & 0xFFFFFFFF, seed >> 32, 1 if timed else 0)
+
+
 def prove(a, b, log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0):
     """prover::prove (src/prover.rs:221): one-shot proof bytes."""
     size = proof_size(log_n, blowup_bits, n_queries, pow_bits)
@@ -97,12 +126,22 @@ def verify(proof, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES):
     return rc == 0, err.value.decode()
 
 
-AIR_FIBONACCI = 0
+AIR_FIBONACCI, AIR_D2 = 0, 1
 
 
-def request(log_n, a=1, b=1, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0, air=AIR_FIBONACCI, seed=0):
-    """A proof request (vx_request)."""
-    return _lib.Request(air, log_n, blowup_bits, n_queries, pow_bits, a, b, 0, seed)
+def request(log_n, a=1, b=1, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0, air=AIR_FIBONACCI, seed=0,
+            n_cols=D2_COLS, n_cons=D2_CONS, air_seed=D2_AIR_SEED):
+    """A proof request (vx_request): Fibonacci (a, b), or with air=AIR_D2 the
+    synthetic multi-column AIR over the trace of `seed`."""
+    d2 = air == AIR_D2
+    return _lib.Request(air, log_n, blowup_bits, n_queries, pow_bits, a if not d2 else 0, b if not d2 else 0,
+                        n_cols if d2 else 0, n_cons if d2 else 0, 0, seed if d2 else 0, air_seed if d2 else 0)
+
+
+def request_size(req):
+    if req.air == AIR_D2:
+        return d2_proof_size(req.log_n, req.n_cols, req.blowup_bits, req.n_queries)
+    return proof_size(req.log_n, req.blowup_bits, req.n_queries, req.pow_bits)
 
 
 class Server:
@@ -121,7 +160,7 @@ class Server:
         _lib.call("vx_server_prepare", self._h, C.byref(req))
 
     def submit(self, req):
-        size = proof_size(req.log_n, req.blowup_bits, req.n_queries, req.pow_bits)
+        size = request_size(req)
         buf = C.create_string_buffer(size)
         t = C.c_uint64()
         _lib.call("vx_server_submit", self._h, C.byref(req), buf, size, C.byref(t))
@@ -148,7 +187,7 @@ class Server:
         n = len(reqs)
         arr = (_lib.Request * n)(*reqs)
         at = (C.c_double * n)(*arrivals)
-        sizes = [proof_size(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits) for r in reqs]
+        sizes = [request_size(r) for r in reqs]
         bufs = [C.create_string_buffer(sz) for sz in sizes]
         outs = (C.c_void_p * n)(*[C.addressof(b) for b in bufs])
         caps = (C.c_size_t * n)(*sizes)

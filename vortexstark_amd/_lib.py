@@ -71,6 +71,8 @@ SIGNATURES = {
     "vx_prove": (ip, [u32, u32, u32, u32, u32, u32, vp, sz, C.POINTER(sz)]),
     "vx_verify": (ip, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
     "vx_prover_set_graphs": (ip, [vp, ip]),
+    "vx_d2_prover_create": (ip, [u32, u32, u32, u64, u32, u32, u32, vp, C.POINTER(vp)]),
+    "vx_d2_proof_size": (sz, [u32, u32, u32, u32]),
     "vx_server_create": (ip, [u32, u64, ip, C.POINTER(vp)]),
     "vx_server_destroy": (ip, [vp]),
     "vx_server_prepare": (ip, [vp, vp]),
@@ -86,7 +88,8 @@ SIGNATURES = {
 class Request(C.Structure):
     """vx_request (include/vortex_hip.h)."""
     _fields_ = [("air", u32), ("log_n", u32), ("blowup_bits", u32), ("n_queries", u32), ("pow_bits", u32),
-                ("a", u32), ("b", u32), ("reserved", u32), ("seed", u64)]
+                ("a", u32), ("b", u32), ("n_cols", u32), ("n_cons", u32), ("reserved", u32), ("seed", u64),
+                ("air_seed", u64)]
 
 _lib = None
 

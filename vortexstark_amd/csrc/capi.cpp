@@ -8,6 +8,7 @@
 
 #include "prover.h"
 #include "server.h"
+#include "vx_d2.h"
 
 using namespace vx;
 
@@ -396,8 +397,9 @@ int vx_grind_pow_min(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result
 
 // ----------------------------------------------------------- prove / verify
 struct vx_prover {
-  FibProver impl;
-  vx_prover(uint32_t l, uint32_t b, uint32_t q, uint32_t p, hipStream_t s) : impl(l, b, q, p, s) {}
+  StarkProver impl;
+  vx_prover(uint32_t l, uint32_t b, uint32_t q, uint32_t p, hipStream_t s, const AirSpec& air = {})
+      : impl(l, b, q, p, s, air) {}
 };
 
 size_t vx_proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits) {
@@ -412,6 +414,30 @@ int vx_prover_create(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t
     *out = nullptr;
     *out = new vx_prover(log_n, blowup_bits, nq, pow_bits, S(st));
   });
+}
+
+int vx_d2_prover_create(uint32_t log_n, uint32_t n_cols, uint32_t n_cons, uint64_t air_seed, uint32_t blowup_bits,
+                        uint32_t nq, uint32_t pow_bits, vx_stream_t st, vx_prover** out) {
+  return guard([&] {
+    require(out != nullptr, "prover_create: null out");
+    *out = nullptr;
+    AirSpec air;
+    air.kind = AIR_D2;
+    air.n_cols = n_cols;
+    air.n_cons = n_cons;
+    air.air_seed = air_seed;
+    *out = new vx_prover(log_n, blowup_bits, nq, pow_bits, S(st), air);
+  });
+}
+
+size_t vx_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, uint32_t nq) {
+  if (log_n < 1 || blowup_bits < 1 || log_n + blowup_bits > 31 || log_n + blowup_bits < 4 || n_cols < 1 ||
+      n_cols > D2_MAX_COLS)
+    return 0;
+  AirSpec air;
+  air.kind = AIR_D2;
+  air.n_cols = n_cols;
+  return proof_size(log_n, blowup_bits, nq, 1, air);
 }
 
 int vx_prover_destroy(vx_prover* p) {
@@ -528,7 +554,10 @@ static ProofRequest to_request(const vx_request* r) {
   q.pow_bits = r->pow_bits;
   q.a = r->a;
   q.b = r->b;
+  q.n_cols = r->n_cols;
+  q.n_cons = r->n_cons;
   q.seed = r->seed;
+  q.air_seed = r->air_seed;
   return q;
 }
 

@@ -1,0 +1,59 @@
+// Kernels of the synthetic degree-2 multi-column AIR (definition in vx_d2.h):
+// trace generation and the powers of the composition challenge. The
+// quotient itself is produced inside the Merkle subtree kernel
+// (LEAF_D2_QUOTIENT in merkle.hip), which also stores it for the circle fold.
+#include "vx_d2.h"
+#include "vx_internal.h"
+
+namespace vx {
+
+static constexpr uint32_t D2_ROWS_PER_LANE = 4;
+
+// Column blockIdx.y, rows [4 lane, 4 lane + 4); the 64-bit trace seed is
+// read from device memory so a captured proof graph replays with new seeds.
+__global__ void k_d2_trace(const uint32_t* seed_words, uint32_t* out, uint64_t n, uint64_t stride) {
+  const uint64_t first = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * D2_ROWS_PER_LANE;
+  if (first >= n) return;
+  const uint64_t seed = (uint64_t)seed_words[0] | ((uint64_t)seed_words[1] << 32);
+  const uint32_t c = blockIdx.y;
+  uint32_t v[D2_ROWS_PER_LANE];
+#pragma unroll
+  for (uint32_t j = 0; j < D2_ROWS_PER_LANE; j++) v[j] = d2_value(seed, c, first + j);
+  uint32_t* col = out + c * stride;
+  if (first + D2_ROWS_PER_LANE <= n) {
+    *reinterpret_cast<uint4*>(col + first) = make_uint4(v[0], v[1], v[2], v[3]);
+  } else {
+    for (uint32_t j = 0; first + j < n; j++) col[first + j] = v[j];
+  }
+}
+
+void d2_trace(const uint32_t* seed_words, uint32_t* out, uint32_t n_cols, uint32_t log_n, uint64_t stride,
+              hipStream_t s) {
+  require(log_n >= 1 && log_n <= 31 && n_cols >= 1 && n_cols <= D2_MAX_COLS && stride >= (1ull << log_n),
+          "d2_trace: bad shape");
+  const uint64_t n = 1ull << log_n;
+  const uint64_t lanes = (n + D2_ROWS_PER_LANE - 1) / D2_ROWS_PER_LANE;
+  hipLaunchKernelGGL(k_d2_trace, dim3((uint32_t)((lanes + 255) / 256), n_cols), dim3(256), 0, s, seed_words, out,
+                     n, stride);
+  VX_HIP(hipGetLastError());
+}
+
+// beta[k] = alpha^(k+1), k < K (one lane; K <= 1024 QM31 products).
+__global__ void k_d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons) {
+  if (threadIdx.x != 0) return;
+  const qm31 a = {{alpha[0], alpha[1], alpha[2], alpha[3]}};
+  qm31 p = a;
+  for (uint32_t k = 0; k < k_cons; k++) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) beta[4 * k + c] = p.v[c];
+    p = qm_mul(p, a);
+  }
+}
+
+void d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons, hipStream_t s) {
+  require(k_cons >= 1 && k_cons <= D2_MAX_CONS, "d2_powers: bad constraint count");
+  hipLaunchKernelGGL(k_d2_powers, dim3(1), dim3(64), 0, s, alpha, beta, k_cons);
+  VX_HIP(hipGetLastError());
+}
+
+}  // namespace vx

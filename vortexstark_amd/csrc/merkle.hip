@@ -159,6 +159,42 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
   }
 }
 
+// Synthetic AIR quotient of rows [i, i+4) (vx_d2.h): Q(r) = sum_k beta_k
+// (c_k t_a[r] t_b[r+1] + d_k t_e[r]); the constraint table and beta are
+// wave-uniform (scalar loads), the rows are one 16-byte load per column use
+// plus the wrapped row i + 4 of column b. Q is stored (SoA) for the fold.
+__device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g) {
+  const uint32_t* ev = L.col[0];
+  const uint64_t st = L.col_stride;
+  const uint64_t i4 = (i + 4) & L.src_mask;
+  qm31 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) acc[j] = qm31{{0, 0, 0, 0}};
+  for (uint32_t k = 0; k < L.n_cons; k++) {
+    const uint32_t* t = L.cons + 8 * k;
+    const uint32_t c = t[0], d = t[1];
+    const uint32_t* ca = ev + t[2] * st;
+    const uint32_t* cb = ev + t[3] * st;
+    const uint32_t* ce = ev + t[4] * st;
+    const uint4 va = ld4(ca + i), vb = ld4(cb + i), ve = ld4(ce + i);
+    const uint32_t b4 = cb[i4];
+    const qm31 be = ldq(L.beta + 4 * k);
+    const uint32_t A[4] = {va.x, va.y, va.z, va.w}, B[5] = {vb.x, vb.y, vb.z, vb.w, b4};
+    const uint32_t E[4] = {ve.x, ve.y, ve.z, ve.w};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t m = m31_add(m31_mul(c, m31_mul(A[j], B[j + 1])), m31_mul(d, E[j]));
+      acc[j] = qm_add(acc[j], qm_scale(be, m));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) g.w[j][c] = acc[j].v[c];
+#pragma unroll
+  for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+}
+
 template <uint32_t KIND, uint32_t NC>
 __device__ __forceinline__ void leaf_hash(const LeafSrc& L, const G4& g, uint64_t i, int j, uint32_t h[8]) {
   if constexpr (KIND == LEAF_COLS && NC == 1) {
@@ -184,22 +220,24 @@ template <uint32_t KIND, uint32_t NC, uint32_t LPT_LOG>
 __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, uint32_t out[8]) {
   static_assert(LPT_LOG >= 2 && LPT_LOG <= 4, "lane subtree covers 4..16 leaves");
   constexpr bool kGeneric = KIND == LEAF_COLS && NC == 0;
+  constexpr bool kDirect = kGeneric || KIND == LEAF_D2_QUOTIENT;  // no raw-load prefetch
   constexpr uint32_t G = 1u << (LPT_LOG - 2);
   qm31 aq{}, af{};
   if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
   if constexpr (KIND == LEAF_FOLD_LINE) af = ldq(L.alpha_f);
   if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) af = qm_mul(ldq(L.alpha_f), aq);
   Raw cur, nxt;
-  if constexpr (!kGeneric) leaf_load<KIND, NC>(L, first, cur);
+  if constexpr (!kDirect) leaf_load<KIND, NC>(L, first, cur);
   uint32_t s2[8], s3[8];
 #pragma unroll
   for (uint32_t gi = 0; gi < G; gi++) {
     const uint64_t i = first + 4 * gi;
     G4 g;
-    if constexpr (!kGeneric) {
+    if constexpr (!kDirect) {
       if (gi + 1 < G) leaf_load<KIND, NC>(L, i + 4, nxt);
       leaf_produce<KIND, NC>(L, i, cur, aq, af, g);
     }
+    if constexpr (KIND == LEAF_D2_QUOTIENT) d2_produce(L, i, g);
     uint32_t a[8], b[8], n1a[8], n1b[8], n2[8];
     leaf_hash<KIND, NC>(L, g, i, 0, a);
     leaf_hash<KIND, NC>(L, g, i, 1, b);
@@ -224,7 +262,7 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
         }
       }
     }
-    if constexpr (!kGeneric) {
+    if constexpr (!kDirect) {
       if (gi + 1 < G) cur = nxt;
     }
   }
@@ -390,6 +428,12 @@ static void check_leaf(const LeafSrc& L, uint32_t log_n) {
       require(L.alpha_f && L.tw, "merkle: bad line-fold source");
       for (int c = 0; c < 4; c++) require(L.col[c] && L.out[c], "merkle: null fold column");
       break;
+    case LEAF_D2_QUOTIENT:
+      require(L.col[0] && L.cons && L.beta && L.n_cols >= 1 && L.n_cons >= 1 &&
+                  L.src_mask == (1ull << log_n) - 1 && L.col_stride >= (1ull << log_n),
+              "merkle: bad multi-column quotient source");
+      for (int c = 0; c < 4; c++) require(L.out[c] != nullptr, "merkle: null quotient output");
+      break;
     default:
       require(false, "merkle: unknown leaf kind");
   }
@@ -406,6 +450,7 @@ static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t til
     case LEAF_QUOTIENT: launch_subtree<LEAF_QUOTIENT, 4>(L, T, tile_log, store_lo, hook, s); break;
     case LEAF_FOLD_CIRCLE_Q: launch_subtree<LEAF_FOLD_CIRCLE_Q, 4>(L, T, tile_log, store_lo, hook, s); break;
     case LEAF_FOLD_LINE: launch_subtree<LEAF_FOLD_LINE, 4>(L, T, tile_log, store_lo, hook, s); break;
+    case LEAF_D2_QUOTIENT: launch_subtree<LEAF_D2_QUOTIENT, 4>(L, T, tile_log, store_lo, hook, s); break;
   }
 }
 
@@ -648,13 +693,18 @@ void merkle_leaf_pairs(const LeafSrc& L, uint64_t n_leaves, uint32_t* parents, h
 }
 
 // ------------------------------------------------------------- decommitment
-// One 64-lane workgroup per (tree, query): re-hash the 2^lo-leaf group that
-// holds the query, take siblings below level `lo` from it and the rest from
-// the stored levels, and write values, sibling values and both auth paths
-// into the KRAK buffer (src/bin/stark_cli.rs:1061-1089 layout).
-__device__ __forceinline__ void decom_value(const DecomDesc& d, uint64_t i, uint32_t w[4]) {
-  if (d.kind == DECOM_M31) {
-    w[0] = d.col[0][i];
+// One 64-lane workgroup per (tree, query, role): role 0 opens the query leaf,
+// role 1 its partner (the sibling, or the next row). Each re-hashes the
+// 2^lo-leaf group that holds its leaf, takes the path below level `lo` from
+// it and the rest from the stored levels, and writes its values and auth
+// path into the proof (src/bin/stark_cli.rs:1061-1089 layout: n, values,
+// partner values, paths, partner paths).
+__device__ __forceinline__ uint32_t decom_width(const DecomDesc& d) {
+  return d.kind == DECOM_COLS ? d.n_cols : 4u;
+}
+__device__ __forceinline__ void decom_value(const DecomDesc& d, uint64_t i, uint32_t w[16]) {
+  if (d.kind == DECOM_COLS) {
+    for (uint32_t c = 0; c < d.n_cols; c++) w[c] = d.col[c][i];
   } else if (d.kind == DECOM_SOA4) {
     for (int c = 0; c < 4; c++) w[c] = d.col[c][i];
   } else {
@@ -669,15 +719,19 @@ __device__ __forceinline__ void decom_value(const DecomDesc& d, uint64_t i, uint
 __global__ __launch_bounds__(64) void k_decommit(const DecomDesc* descs, const uint32_t* queries, uint32_t nq,
                                                  uint32_t* proof) {
   __shared__ uint32_t nodes[32 * 8];  // level v of the group at node offset 32 - 2^(lo - v + 1)
-  const uint32_t tree = blockIdx.x / nq, q = blockIdx.x % nq;
+  const uint32_t role = blockIdx.x & 1, tq = blockIdx.x >> 1;
+  const uint32_t tree = tq / nq, q = tq % nq;
   const DecomDesc& d = descs[tree];
   const uint32_t lane = threadIdx.x;
-  const uint32_t lo = d.lo, D = d.log_n, W = d.kind == DECOM_M31 ? 1u : 4u;
-  const uint64_t idx = (uint64_t)queries[q] >> d.shift;
+  const uint32_t lo = d.lo, D = d.log_n, W = decom_width(d);
+  const uint64_t mask = (1ull << D) - 1;
+  const uint64_t qidx = (uint64_t)queries[q] >> d.shift;
+  const uint64_t idx = role == 0 ? qidx : (d.pair == PAIR_NEXT ? (qidx + 1) & mask : qidx ^ 1);
   const uint64_t first = (idx >> lo) << lo;
   // level v occupies nodes [base(v), base(v) + 2^(lo-v)), base(v) = 32 - 2^(lo-v+1)
   if (lane < (1u << lo)) {
-    uint32_t w[4] = {0, 0, 0, 0}, h[8];
+    uint32_t w[16], h[8];
+    for (int k = 0; k < 16; k++) w[k] = 0;
     decom_value(d, first + lane, w);
     b2_hash_words(w, W, h);
     for (int k = 0; k < 8; k++) nodes[8 * (32 - (2u << lo) + lane) + k] = h[k];
@@ -700,17 +754,13 @@ __global__ __launch_bounds__(64) void k_decommit(const DecomDesc* descs, const u
   const uint64_t base = d.proof_word;
   const uint64_t pbase = base + 1 + 2ull * nq * W;
   const uint64_t plen = 1 + 8ull * D;
-  uint32_t* path = proof + pbase + q * plen;
-  uint32_t* spath = proof + pbase + (uint64_t)nq * plen + q * plen;
+  uint32_t* path = proof + pbase + ((uint64_t)role * nq + q) * plen;
   if (lane == 0) {
-    uint32_t w[4] = {0, 0, 0, 0};
-    if (q == 0) proof[base] = nq;
+    uint32_t w[16];
+    if (q == 0 && role == 0) proof[base] = nq;
     decom_value(d, idx, w);
-    for (uint32_t k = 0; k < W; k++) proof[base + 1 + (uint64_t)q * W + k] = w[k];
-    decom_value(d, idx ^ 1, w);
-    for (uint32_t k = 0; k < W; k++) proof[base + 1 + (uint64_t)nq * W + (uint64_t)q * W + k] = w[k];
+    for (uint32_t k = 0; k < W; k++) proof[base + 1 + ((uint64_t)role * nq + q) * W + k] = w[k];
     path[0] = D;
-    spath[0] = D;
   }
   if (lane < D) {
     const uint32_t v = lane;
@@ -722,16 +772,14 @@ __global__ __launch_bounds__(64) void k_decommit(const DecomDesc* descs, const u
       sib = d.tree + d.off[v] + 8 * ((idx >> v) ^ 1);
     }
     for (int k = 0; k < 8; k++) path[1 + 8 * v + k] = sib[k];
-    const uint32_t* ssib = sib;
-    if (v == 0) ssib = &nodes[8 * (32 - (2u << lo) + (uint32_t)(idx - first))];
-    for (int k = 0; k < 8; k++) spath[1 + 8 * v + k] = ssib[k];
   }
 }
 
 void decommit(const DecomDesc* d_descs, uint32_t n_trees, const uint32_t* d_queries, uint32_t nq, uint32_t* proof,
               hipStream_t s) {
   if (n_trees == 0 || nq == 0) return;
-  hipLaunchKernelGGL(k_decommit, dim3(n_trees * nq), dim3(64), 0, s, d_descs, d_queries, nq, proof);
+  require((uint64_t)n_trees * nq * 2 < (1ull << 31), "decommit: too many openings");
+  hipLaunchKernelGGL(k_decommit, dim3(2 * n_trees * nq), dim3(64), 0, s, d_descs, d_queries, nq, proof);
   VX_HIP(hipGetLastError());
 }
 

@@ -27,11 +27,14 @@ struct PassArgs {
   uint32_t j0, k, cbits;
   uint64_t src_mask;
   uint32_t scale;          // output multiplier (1 = none)
+  uint64_t dst_stride, src_stride;  // words between the columns of a batch (blockIdx.y)
 };
 
 template <bool FWD>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(uint32_t* dst, const uint32_t* src, PassArgs a) {
   extern __shared__ uint32_t lds[];
+  dst += blockIdx.y * a.dst_stride;
+  src += blockIdx.y * a.src_stride;
   const uint32_t cmask = (1u << a.cbits) - 1;
   const uint32_t E = 1u << (a.k + a.cbits);
   const uint32_t low_tiles_log = a.j0 - a.cbits;
@@ -101,6 +104,7 @@ struct RadixArgs {
   RadixRound round[4];     // in execution order
   uint64_t src_mask;
   uint32_t scale;
+  uint64_t dst_stride, src_stride;
 };
 
 __device__ __forceinline__ uint32_t rx_own(uint32_t t, uint32_t s, uint32_t pos) {
@@ -134,6 +138,8 @@ __device__ __forceinline__ void rx_bit(uint32_t (&v)[32], const uint32_t* table,
 template <bool FWD>
 __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const uint32_t* src, RadixArgs a) {
   __shared__ uint32_t lds[1u << RX_E_BITS];
+  dst += blockIdx.y * a.dst_stride;
+  src += blockIdx.y * a.src_stride;
   const uint32_t t = threadIdx.x;
   const uint32_t cmask = (1u << a.cbits) - 1;
   const uint32_t low_tiles_log = a.j0 - a.cbits;
@@ -188,9 +194,14 @@ static const uint32_t* bit_table(const TwiddleRef& tw, uint32_t j) {
   return j == 0 ? tw.circle : tw.line + tw.off[j - 1];
 }
 
+struct Batch {
+  uint32_t n_cols;
+  uint64_t dst_stride, src_stride;
+};
+
 static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_mask,
                         const TwiddleRef& tw, uint32_t j0, uint32_t k, uint32_t cbits, uint32_t scale,
-                        hipStream_t s) {
+                        const Batch& bt, hipStream_t s) {
   const uint32_t log_n = tw.log_n;
   require(k >= 1 && k <= 16 && k + cbits <= NTT_TILE_LOG && k + cbits <= log_n && cbits <= j0 &&
               j0 + k <= log_n,
@@ -202,6 +213,9 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
   a.cbits = cbits;
   a.src_mask = src_mask;
   a.scale = scale;
+  a.dst_stride = bt.dst_stride;
+  a.src_stride = bt.src_stride;
+  require(bt.n_cols >= 1 && bt.n_cols <= 65535, "ntt: 1..65535 columns per batch");
   const uint32_t e_log = k + cbits;
   const uint64_t grid = 1ull << (log_n - e_log);
   if (e_log == RX_E_BITS) {
@@ -212,6 +226,8 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
     r.cbits = cbits;
     r.src_mask = src_mask;
     r.scale = scale;
+    r.dst_stride = bt.dst_stride;
+    r.src_stride = bt.src_stride;
     // rounds of <= 5 row bits from the top; evaluate runs them top-down,
     // interpolate bottom-up
     RadixRound rr[4];
@@ -225,27 +241,29 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
     r.n_rounds = nr;
     for (uint32_t i = 0; i < nr; i++) r.round[i] = fwd ? rr[i] : rr[nr - 1 - i];
     if (fwd)
-      hipLaunchKernelGGL(k_ntt_radix<true>, dim3((uint32_t)grid), dim3(RX_THREADS), 0, s, dst, src, r);
+      hipLaunchKernelGGL(k_ntt_radix<true>, dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0, s, dst, src, r);
     else
-      hipLaunchKernelGGL(k_ntt_radix<false>, dim3((uint32_t)grid), dim3(RX_THREADS), 0, s, dst, src, r);
+      hipLaunchKernelGGL(k_ntt_radix<false>, dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0, s, dst, src, r);
     VX_HIP(hipGetLastError());
     return;
   }
   const uint32_t threads = e_log >= 10 ? NTT_THREADS : (e_log >= 7 ? (1u << (e_log - 1)) : 64u);
   const size_t lds = (size_t)4 << e_log;
   if (fwd)
-    hipLaunchKernelGGL(k_ntt_pass<true>, dim3((uint32_t)grid), dim3(threads), lds, s, dst, src, a);
+    hipLaunchKernelGGL(k_ntt_pass<true>, dim3((uint32_t)grid, bt.n_cols), dim3(threads), lds, s, dst, src, a);
   else
-    hipLaunchKernelGGL(k_ntt_pass<false>, dim3((uint32_t)grid), dim3(threads), lds, s, dst, src, a);
+    hipLaunchKernelGGL(k_ntt_pass<false>, dim3((uint32_t)grid, bt.n_cols), dim3(threads), lds, s, dst, src, a);
   VX_HIP(hipGetLastError());
 }
 
 void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
-                 uint32_t eff_bits, hipStream_t s) {
+                 uint32_t eff_bits, hipStream_t s, uint32_t n_cols, uint64_t dst_stride, uint64_t src_stride) {
   const uint32_t log_n = tw.log_n;
   require(log_n >= 1 && log_n <= 31 && eff_bits >= 1 && eff_bits <= log_n, "ntt_forward: bad sizes");
   require(((src_mask + 1) & src_mask) == 0 && src_mask + 1 >= (1ull << eff_bits),
           "ntt_forward: source length must be a power of two covering the transformed bits");
+  require(n_cols == 1 || (dst_stride >= (1ull << log_n) && src_stride >= src_mask + 1),
+          "ntt_forward: batch columns must not overlap");
   // Strided passes cover [NTT_TILE_LOG, eff_bits) from the top, then one
   // contiguous pass covers the low bits.
   uint32_t top = eff_bits;
@@ -254,32 +272,36 @@ void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const T
   const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
   const uint32_t* cur = src;
   uint64_t mask = src_mask;
+  Batch bt{n_cols, dst_stride, src_stride};
   for (uint32_t p = 0; p < n_strided; p++) {
     uint32_t k = (top - contig) / (n_strided - p);  // balanced chunks
     uint32_t j0 = top - k;
-    launch_pass(true, data, cur, mask, tw, j0, k, NTT_TILE_LOG - k, 1u, s);
+    launch_pass(true, data, cur, mask, tw, j0, k, NTT_TILE_LOG - k, 1u, bt, s);
     top = j0;
     cur = data;
     mask = (1ull << log_n) - 1;
+    bt.src_stride = dst_stride;
   }
-  launch_pass(true, data, cur, mask, tw, 0, contig, 0, 1u, s);
+  launch_pass(true, data, cur, mask, tw, 0, contig, 0, 1u, bt, s);
 }
 
-void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s) {
+void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n_cols, uint64_t stride) {
   const uint32_t log_n = tw.log_n;
   require(log_n >= 1 && log_n <= 31, "ntt_inverse: bad size");
+  require(n_cols == 1 || stride >= (1ull << log_n), "ntt_inverse: batch columns must not overlap");
   const uint32_t e = (30u * log_n) % 31u;
   const uint32_t inv_n = e == 0 ? 1u : (1u << e);
   const uint64_t mask = (1ull << log_n) - 1;
   const uint32_t contig = log_n < NTT_TILE_LOG ? log_n : NTT_TILE_LOG;
   const uint32_t strided_bits = log_n - contig;
   const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
-  launch_pass(false, data, data, mask, tw, 0, contig, 0, n_strided == 0 ? inv_n : 1u, s);
+  const Batch bt{n_cols, stride, stride};
+  launch_pass(false, data, data, mask, tw, 0, contig, 0, n_strided == 0 ? inv_n : 1u, bt, s);
   uint32_t bot = contig;
   for (uint32_t p = 0; p < n_strided; p++) {
     uint32_t k = (log_n - bot) / (n_strided - p);
     bool last = p + 1 == n_strided;
-    launch_pass(false, data, data, mask, tw, bot, k, NTT_TILE_LOG - k, last ? inv_n : 1u, s);
+    launch_pass(false, data, data, mask, tw, bot, k, NTT_TILE_LOG - k, last ? inv_n : 1u, bt, s);
     bot += k;
   }
 }

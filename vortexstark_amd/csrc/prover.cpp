@@ -9,6 +9,8 @@
 // transfer is the finished KRAK proof.
 #include "prover.h"
 
+#include "vx_d2.h"
+
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -43,21 +45,37 @@ DeviceBuf& DeviceBuf::operator=(DeviceBuf&& o) noexcept {
   return *this;
 }
 
-ProofLayout::ProofLayout(uint32_t log_n_, uint32_t blowup_bits_, uint32_t nq_, uint32_t pow_bits_)
+ProofLayout::ProofLayout(uint32_t log_n_, uint32_t blowup_bits_, uint32_t nq_, uint32_t pow_bits_,
+                         const AirSpec& air)
     : log_n(log_n_), blowup_bits(blowup_bits_), nq(nq_), pow_bits(pow_bits_) {
   le = log_n + blowup_bits;
   n_fri = le - 3;
-  header_words = pow_bits ? 9 : 5;
-  w_troot = header_words;
-  w_qroot = w_troot + 8;
+  const bool d2 = air.kind == AIR_D2;
+  n_groups = d2 ? d2_groups(air.n_cols) : 1;
+  // v1: magic, version, log_n, a, b; v2 adds blowup, pow, nonce lo/hi;
+  // v3: magic, 3, log_n, W, K, blowup, pow, nonce lo/hi, seed lo/hi,
+  //     air_seed lo/hi, G
+  header_words = d2 ? 14 : (pow_bits ? 9 : 5);
+  w_nonce = d2 ? 7 : (pow_bits ? 7 : 0);
+  uint64_t w = header_words;
+  for (uint32_t g = 0; g < n_groups; g++) {
+    w_troot.push_back(w);
+    w += 8;
+  }
+  w_qroot = w;
   w_nfri = w_qroot + 8;
   w_froot0 = w_nfri + 1;
   w_last = w_froot0 + 8ull * n_fri;
   w_nq = w_last + 1 + 32;
-  w_tdec = w_nq + 1 + nq;
-  w_qdec = w_tdec + 1 + 2ull * nq + 2ull * nq * (1 + 8ull * le);
+  w = w_nq + 1 + nq;
+  for (uint32_t g = 0; g < n_groups; g++) {
+    const uint64_t width = d2 ? d2_group_cols(air.n_cols, g) : 1;
+    w_tdec.push_back(w);
+    w += 1 + 2ull * nq * width + 2ull * nq * (1 + 8ull * le);
+  }
+  w_qdec = w;
   w_nfdec = w_qdec + 1 + 8ull * nq + 2ull * nq * (1 + 8ull * le);
-  uint64_t w = w_nfdec + 1;
+  w = w_nfdec + 1;
   for (uint32_t k = 0; k < n_fri; k++) {
     w_fdec.push_back(w);
     uint32_t L = le - 1 - k;
@@ -66,8 +84,8 @@ ProofLayout::ProofLayout(uint32_t log_n_, uint32_t blowup_bits_, uint32_t nq_, u
   total_words = w;
 }
 
-size_t proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits) {
-  return ProofLayout(log_n, blowup_bits, nq, pow_bits).total_words * 4;
+size_t proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits, const AirSpec& air) {
+  return ProofLayout(log_n, blowup_bits, nq, pow_bits, air).total_words * 4;
 }
 
 // src/circle.rs:284-347: coset points, circle twiddles = y of the first half,
@@ -143,32 +161,46 @@ std::shared_ptr<const ProverTables> ProverTables::get(uint32_t log_n, uint32_t b
   return sp;
 }
 
-uint64_t FibProver::estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits) {
+static void check_air(const AirSpec& air) {
+  if (air.kind == AIR_FIBONACCI) return;
+  require(air.kind == AIR_D2, "prover: unknown AIR");
+  require(air.n_cols >= 1 && air.n_cols <= D2_MAX_COLS && air.n_cons >= 1 && air.n_cons <= D2_MAX_CONS,
+          "prover: multi-column AIR needs 1..256 columns and 1..1024 constraints");
+}
+
+uint64_t StarkProver::estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits,
+                                     const AirSpec& air) {
   const uint32_t le = log_n + blowup_bits, n_fri = le - 3;
-  uint64_t b = 4ull << log_n;          // trace
-  b += 4ull << le;                     // eval
+  const bool d2 = air.kind == AIR_D2;
+  const uint64_t W = d2 ? air.n_cols : 1, G = d2 ? d2_groups(air.n_cols) : 1;
+  uint64_t b = (4ull << log_n) * W;    // trace
+  b += (4ull << le) * W;               // eval
+  if (d2) b += (16ull << le) + 32ull * air.n_cons + 16ull * air.n_cons;  // quotient, table, powers
   for (uint32_t k = 0; k < n_fri; k++) b += 16ull << (le - 1 - k);   // FRI layers
-  for (uint32_t t = 0; t < 2 + n_fri; t++) {                         // trees, levels >= 4
-    uint32_t L = t < 2 ? le : le - 1 - (t - 2);
+  for (uint32_t t = 0; t < G + 1 + n_fri; t++) {                     // trees, levels >= 4
+    uint32_t L = t <= G ? le : le - 1 - (t - G - 1);
     uint32_t lo = L < 4 ? L : 4;
     for (uint32_t l = lo; l <= L; l++) b += 32ull << (L - l);
   }
-  b += ProofLayout(log_n, blowup_bits, nq, pow_bits).total_words * 4;  // device proof
-  b += 64 + 16ull * (n_fri + 2) + 4ull * (nq + 3) + 256 + sizeof(DecomDesc) * (2 + n_fri);
+  b += ProofLayout(log_n, blowup_bits, nq, pow_bits, air).total_words * 4;  // device proof
+  b += 64 + 16ull * (n_fri + 2) + 4ull * (nq + 3) + 256 + sizeof(DecomDesc) * (G + 1 + n_fri);
   return b;
 }
 
-FibProver::FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, hipStream_t stream)
-    : layout_((require(log_n >= 4 && blowup_bits >= 1 && log_n + blowup_bits <= 31, "prover: need log_n >= 4, "
+StarkProver::StarkProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits,
+                         hipStream_t stream, const AirSpec& air)
+    : air_((check_air(air), air)),
+      layout_((require(log_n >= 4 && blowup_bits >= 1 && log_n + blowup_bits <= 31, "prover: need log_n >= 4, "
                                                                                        "blowup >= 1, log_n + blowup <= 31"),
                log_n),
-              blowup_bits, n_queries, pow_bits),
+              blowup_bits, n_queries, pow_bits, air),
       log_n_(log_n),
       blowup_(blowup_bits),
       nq_(n_queries),
       pow_bits_(pow_bits),
       le_(log_n + blowup_bits),
       n_fri_(log_n + blowup_bits - 3),
+      n_groups_(layout_.n_groups),
       stream_(stream) {
   require(n_queries >= 1 && n_queries <= 65536, "prover: n_queries must be 1..65536");
   require(pow_bits <= 32, "prover: pow_bits must be <= 32");
@@ -176,18 +208,36 @@ FibProver::FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, u
     VX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     own_stream_ = true;
   }
+  const bool d2 = air_.kind == AIR_D2;
+  const uint64_t W = d2 ? air_.n_cols : 1;
   const uint64_t n = 1ull << log_n_, N = 1ull << le_;
-  trace_ = DeviceBuf(4 * n);
-  eval_ = DeviceBuf(4 * N);
+  trace_ = DeviceBuf(4 * n * W);
+  eval_ = DeviceBuf(4 * N * W);
   device_bytes_ += trace_.bytes + eval_.bytes;
+  if (d2) {
+    quot_ = DeviceBuf(16 * N);
+    std::vector<uint32_t> cd(2 * air_.n_cons), tab(8 * air_.n_cons, 0);
+    d2_coeffs(air_.air_seed, air_.n_cons, cd.data());
+    for (uint32_t k = 0; k < air_.n_cons; k++) {
+      tab[8 * k + 0] = cd[2 * k];
+      tab[8 * k + 1] = cd[2 * k + 1];
+      tab[8 * k + 2] = d2_col_a(k, air_.n_cols);
+      tab[8 * k + 3] = d2_col_b(k, air_.n_cols);
+      tab[8 * k + 4] = d2_col_e(k, air_.n_cols);
+    }
+    cons_ = DeviceBuf(4 * tab.size());
+    VX_HIP(hipMemcpy(cons_.p, tab.data(), 4 * tab.size(), hipMemcpyHostToDevice));
+    beta_ = DeviceBuf(16ull * air_.n_cons);
+    device_bytes_ += quot_.bytes + cons_.bytes + beta_.bytes;
+  }
   for (uint32_t k = 0; k < n_fri_; k++) {
     uint64_t words = 1ull << (le_ - 1 - k);
     layers_.emplace_back(4 * 4 * words);
     layer_col_words_.push_back(words);
     device_bytes_ += layers_.back().bytes;
   }
-  for (uint32_t t = 0; t < 2 + n_fri_; t++) {
-    uint32_t L = t < 2 ? le_ : le_ - 1 - (t - 2);
+  for (uint32_t t = 0; t < n_groups_ + 1 + n_fri_; t++) {
+    uint32_t L = t <= n_groups_ ? le_ : le_ - 1 - (t - n_groups_ - 1);
     uint32_t lo = L < 4 ? L : 4;
     std::vector<uint64_t> off(MAX_LEVELS, 0);
     uint64_t w = 0;
@@ -222,7 +272,7 @@ FibProver::FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, u
   build_descs();
 }
 
-FibProver::~FibProver() {
+StarkProver::~StarkProver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (graph_) (void)hipGraphExecDestroy(graph_);
   for (auto& e : ev_)
@@ -232,7 +282,7 @@ FibProver::~FibProver() {
   if (own_stream_) (void)hipStreamDestroy(stream_);
 }
 
-TreeLevels FibProver::tree_levels(uint32_t t) const {
+TreeLevels StarkProver::tree_levels(uint32_t t) const {
   TreeLevels T{};
   T.base = trees_[t].u32();
   T.log_n = tree_log_[t];
@@ -241,28 +291,38 @@ TreeLevels FibProver::tree_levels(uint32_t t) const {
   return T;
 }
 
-void FibProver::build_descs() {
-  std::vector<DecomDesc> d(2 + n_fri_);
-  for (uint32_t t = 0; t < 2 + n_fri_; t++) {
+void StarkProver::build_descs() {
+  const bool d2 = air_.kind == AIR_D2;
+  const uint64_t N = 1ull << le_;
+  const uint32_t G = n_groups_;
+  std::vector<DecomDesc> d(G + 1 + n_fri_);
+  for (uint32_t t = 0; t < G + 1 + n_fri_; t++) {
     DecomDesc& x = d[t];
     std::memset(&x, 0, sizeof x);
     x.log_n = tree_log_[t];
     x.lo = tree_lo_[t];
     x.tree = trees_[t].u32();
     for (int l = 0; l < MAX_LEVELS; l++) x.off[l] = tree_off_[t][l];
-    if (t == 0) {
-      x.kind = DECOM_M31;
-      x.col[0] = eval_.u32();
+    if (t < G) {  // trace group t
+      x.kind = DECOM_COLS;
+      x.n_cols = d2 ? d2_group_cols(air_.n_cols, t) : 1;
+      for (uint32_t c = 0; c < x.n_cols; c++) x.col[c] = eval_.u32() + (D2_GROUP * t + c) * N;
+      x.pair = d2 ? PAIR_NEXT : PAIR_SIBLING;
       x.shift = 0;
-      x.proof_word = layout_.w_tdec;
-    } else if (t == 1) {
-      x.kind = DECOM_QUOTIENT;
-      x.col[0] = eval_.u32();
-      x.alpha = d_alphas_;
+      x.proof_word = layout_.w_tdec[t];
+    } else if (t == G) {
+      if (d2) {
+        x.kind = DECOM_SOA4;
+        for (int c = 0; c < 4; c++) x.col[c] = quot_.u32() + c * N;
+      } else {
+        x.kind = DECOM_QUOTIENT;
+        x.col[0] = eval_.u32();
+        x.alpha = d_alphas_;
+      }
       x.shift = 0;
       x.proof_word = layout_.w_qdec;
     } else {
-      uint32_t k = t - 2;
+      uint32_t k = t - G - 1;
       x.kind = DECOM_SOA4;
       for (int c = 0; c < 4; c++) x.col[c] = layers_[k].u32() + c * layer_col_words_[k];
       x.shift = k + 1;
@@ -276,7 +336,7 @@ void FibProver::build_descs() {
 
 // The whole proof after the public inputs are set: kernels only (no host
 // synchronisation, no memset), so it can be captured into a graph.
-void FibProver::record(bool timed, hipStream_t s) {
+void StarkProver::record(bool timed, hipStream_t s) {
   const ProverTables& tb = *tab_;
   const ProofLayout& L = layout_;
   uint32_t* proof = proof_.u32();
@@ -287,45 +347,88 @@ void FibProver::record(bool timed, hipStream_t s) {
   auto alpha = [&](uint32_t i) { return d_alphas_ + 4 * i; };
   auto layer_col = [&](uint32_t k, int c) { return layers_[k].u32() + c * layer_col_words_[k]; };
 
+  const bool d2 = air_.kind == AIR_D2;
+  const uint32_t G = n_groups_;
+  const uint32_t QT = G;  // quotient tree index; FRI layer k is tree G + 1 + k
   mark(0);
-  std::vector<uint32_t> hw = {0, KRAK_MAGIC, 1, pow_bits_ ? 2u : 1u, 2, log_n_};  // words 3, 4: set_inputs
-  if (pow_bits_) {
-    hw.insert(hw.end(), {5, blowup_, 6, pow_bits_});
+  std::vector<uint32_t> hw;
+  if (!d2) {  // words 3, 4 (a, b) are written by set_inputs
+    hw = {0, KRAK_MAGIC, 1, pow_bits_ ? 2u : 1u, 2, log_n_};
+    if (pow_bits_) hw.insert(hw.end(), {5, blowup_, 6, pow_bits_});
+  } else {    // words 9, 10 (seed) by set_inputs, 7, 8 (nonce) by the query step or stay 0
+    hw = {0, KRAK_MAGIC, 1, 3, 2, log_n_, 3, air_.n_cols, 4, air_.n_cons, 5, blowup_, 6, pow_bits_,
+          11, (uint32_t)air_.air_seed, 12, (uint32_t)(air_.air_seed >> 32), 13, G};
+    if (!pow_bits_) hw.insert(hw.end(), {7, 0, 8, 0});
   }
   hw.insert(hw.end(), {(uint32_t)L.w_nfri, n_fri_, (uint32_t)L.w_nfdec, n_fri_});
   write_header(proof, hw.data(), (uint32_t)hw.size() / 2, s);
-  fib_trace_dev(d_inputs_, trace_.u32(), log_n_, s);
-  mark(1);
-  ntt_inverse(trace_.u32(), tb.trace_itw->ref, s);
-  mark(2);
-  ntt_forward(eval_.u32(), trace_.u32(), n - 1, tb.eval_tw->ref, log_n_, s);
+  if (!d2) {
+    fib_trace_dev(d_inputs_, trace_.u32(), log_n_, s);
+    mark(1);
+    ntt_inverse(trace_.u32(), tb.trace_itw->ref, s);
+    mark(2);
+    ntt_forward(eval_.u32(), trace_.u32(), n - 1, tb.eval_tw->ref, log_n_, s);
+  } else {
+    const uint32_t W = air_.n_cols;
+    d2_trace(d_inputs_, trace_.u32(), W, log_n_, n, s);
+    mark(1);
+    ntt_inverse(trace_.u32(), tb.trace_itw->ref, s, W, n);
+    mark(2);
+    ntt_forward(eval_.u32(), trace_.u32(), n - 1, tb.eval_tw->ref, log_n_, s, W, N, n);
+  }
   mark(3);
 
-  LeafSrc lt{};
-  lt.kind = LEAF_COLS;
-  lt.n_cols = 1;
-  lt.col[0] = eval_.u32();
-  merkle_commit(lt, tree_levels(0), tree_lo_[0], RootHook{d_channel_, proof + L.w_troot, alpha(0), 1}, s,
-                timed ? ev_[10] : nullptr, timed ? ev_[11] : nullptr);
+  // trace commitment: one tree per group of <= 16 columns; the channel mixes
+  // every root and draws the quotient challenge after the last
+  for (uint32_t g = 0; g < G; g++) {
+    LeafSrc lt{};
+    lt.kind = LEAF_COLS;
+    lt.n_cols = d2 ? d2_group_cols(air_.n_cols, g) : 1;
+    for (uint32_t c = 0; c < lt.n_cols; c++) lt.col[c] = eval_.u32() + (D2_GROUP * g + c) * N;
+    const bool last = g + 1 == G;
+    merkle_commit(lt, tree_levels(g), tree_lo_[g],
+                  RootHook{d_channel_, proof + L.w_troot[g], last ? alpha(0) : nullptr, g == 0 ? 1u : 0u}, s,
+                  timed && g == 0 ? ev_[10] : nullptr, timed && g == 0 ? ev_[11] : nullptr);
+  }
   mark(4);
 
-  LeafSrc lq{};
-  lq.kind = LEAF_QUOTIENT;
-  lq.col[0] = eval_.u32();
-  lq.alpha_q = alpha(0);
-  lq.src_mask = N - 1;
-  merkle_commit(lq, tree_levels(1), tree_lo_[1], RootHook{d_channel_, proof + L.w_qroot, alpha(1), 0}, s);
+  if (!d2) {
+    LeafSrc lq{};
+    lq.kind = LEAF_QUOTIENT;
+    lq.col[0] = eval_.u32();
+    lq.alpha_q = alpha(0);
+    lq.src_mask = N - 1;
+    merkle_commit(lq, tree_levels(QT), tree_lo_[QT], RootHook{d_channel_, proof + L.w_qroot, alpha(1), 0}, s);
+  } else {
+    d2_powers(alpha(0), beta_.u32(), air_.n_cons, s);
+    LeafSrc lq{};
+    lq.kind = LEAF_D2_QUOTIENT;
+    lq.col[0] = eval_.u32();
+    lq.n_cols = air_.n_cols;
+    lq.col_stride = N;
+    lq.n_cons = air_.n_cons;
+    lq.cons = cons_.u32();
+    lq.beta = beta_.u32();
+    lq.src_mask = N - 1;
+    for (int c = 0; c < 4; c++) lq.out[c] = quot_.u32() + c * N;
+    merkle_commit(lq, tree_levels(QT), tree_lo_[QT], RootHook{d_channel_, proof + L.w_qroot, alpha(1), 0}, s);
+  }
   mark(5);
 
   LeafSrc lc{};
-  lc.kind = LEAF_FOLD_CIRCLE_Q;
-  lc.col[0] = eval_.u32();
-  lc.alpha_q = alpha(0);
+  if (!d2) {  // circle fold of the quotient, recomputed from the eval column
+    lc.kind = LEAF_FOLD_CIRCLE_Q;
+    lc.col[0] = eval_.u32();
+    lc.alpha_q = alpha(0);
+    lc.src_mask = N - 1;
+  } else {    // circle fold of the stored quotient: same pair formula as a line fold
+    lc.kind = LEAF_FOLD_LINE;
+    for (int c = 0; c < 4; c++) lc.col[c] = quot_.u32() + c * N;
+  }
   lc.alpha_f = alpha(1);
   lc.tw = tb.fold_tw.u32() + tb.fold_tw_off[0];
-  lc.src_mask = N - 1;
   for (int c = 0; c < 4; c++) lc.out[c] = layer_col(0, c);
-  merkle_commit(lc, tree_levels(2), tree_lo_[2],
+  merkle_commit(lc, tree_levels(QT + 1), tree_lo_[QT + 1],
                 RootHook{d_channel_, proof + L.w_froot0, n_fri_ > 1 ? alpha(2) : nullptr, 0}, s);
   // Line folds: big layers through the subtree kernels, the last layers of
   // <= 2^TAIL_MAX_LOG values in one k_fri_tail launch.
@@ -339,7 +442,7 @@ void FibProver::record(bool timed, hipStream_t s) {
     }
     ll.tw = tb.fold_tw.u32() + tb.fold_tw_off[k];
     ll.alpha_f = alpha(k + 1);
-    merkle_commit(ll, tree_levels(2 + k), tree_lo_[2 + k],
+    merkle_commit(ll, tree_levels(QT + 1 + k), tree_lo_[QT + 1 + k],
                   RootHook{d_channel_, proof + L.w_froot0 + 8ull * k, k + 1 < n_fri_ ? alpha(k + 2) : nullptr, 0}, s);
   }
   if (k < n_fri_) {
@@ -351,10 +454,10 @@ void FibProver::record(bool timed, hipStream_t s) {
       FriTailLayer& T = ta.layer[j - k];
       for (int c = 0; c < 4; c++) T.out[c] = layer_col(j, c);
       T.tw = tb.fold_tw.u32() + tb.fold_tw_off[j];
-      T.tree = trees_[2 + j].u32();
+      T.tree = trees_[QT + 1 + j].u32();
       T.log_n = le_ - 1 - j;
-      T.lo = tree_lo_[2 + j];
-      for (uint32_t l = 0; l <= T.log_n; l++) T.off[l] = (uint32_t)tree_off_[2 + j][l];
+      T.lo = tree_lo_[QT + 1 + j];
+      for (uint32_t l = 0; l <= T.log_n; l++) T.off[l] = (uint32_t)tree_off_[QT + 1 + j][l];
       T.root_out = proof + L.w_froot0 + 8ull * j;
       T.alpha = alpha(j + 1);
       T.next_alpha = j + 1 < n_fri_ ? alpha(j + 2) : nullptr;
@@ -374,18 +477,18 @@ void FibProver::record(bool timed, hipStream_t s) {
   q.proof = proof;
   q.last_word = L.w_last;
   q.query_word = L.w_nq;
-  q.nonce_word = pow_bits_ ? 7 : 0;
+  q.nonce_word = pow_bits_ ? L.w_nonce : 0;
   channel_mix_last(q, d_prefix_, s);
   if (pow_bits_) grind_pow(d_prefix_, pow_bits_, d_nonce_, s);
   channel_draw_queries(q, s);
   mark(7);
-  decommit(reinterpret_cast<const DecomDesc*>(descs_.p), 2 + n_fri_, d_queries_, nq_, proof, s);
+  decommit(reinterpret_cast<const DecomDesc*>(descs_.p), G + 1 + n_fri_, d_queries_, nq_, proof, s);
   mark(8);
 }
 
 // Captured on a private stream, so work that other threads put on the
 // prover's stream meanwhile cannot end up in the graph.
-void FibProver::capture() {
+void StarkProver::capture() {
   hipStream_t cs = nullptr;
   VX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipGraph_t g = nullptr;
@@ -406,10 +509,11 @@ void FibProver::capture() {
   VX_HIP(e);
 }
 
-void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
-  require(a < P && b < P, "prove: public inputs must be canonical M31");
+void StarkProver::enqueue(uint32_t a, uint32_t b, bool timed) {
+  const bool d2 = air_.kind == AIR_D2;
+  require(d2 || (a < P && b < P), "prove: public inputs must be canonical M31");
   timed_ = timed;
-  set_inputs(d_inputs_, proof_.u32(), 3, a, b, stream_);
+  set_inputs(d_inputs_, proof_.u32(), d2 ? 9 : 3, a, b, stream_);
   if (timed || !use_graph_) {
     record(timed, stream_);
   } else {
@@ -421,13 +525,13 @@ void FibProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   if (timed) VX_HIP(hipEventRecord(ev_[9], stream_));
 }
 
-const uint8_t* FibProver::wait(size_t* len) {
+const uint8_t* StarkProver::wait(size_t* len) {
   VX_HIP(hipStreamSynchronize(stream_));
   if (len) *len = layout_.total_words * 4;
   return host_proof_[cur_];
 }
 
-StageTimes FibProver::last_times() {
+StageTimes StarkProver::last_times() {
   StageTimes t{};
   if (!timed_) return t;
   VX_HIP(hipEventSynchronize(ev_[9]));

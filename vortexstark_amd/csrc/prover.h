@@ -1,8 +1,10 @@
-// Host-side Fibonacci prover over the gfx950 kernels: the C++ counterpart of
-// src/prover.rs (prove / prove_cached / ProverCache). One FibProver owns the
-// twiddles, every intermediate buffer and the KRAK output buffer for one
-// (log_n, blowup, queries, pow) size class, so repeated proofs allocate
+// Host-side prover over the gfx950 kernels: the C++ counterpart of
+// src/prover.rs (prove / prove_cached / ProverCache). One StarkProver owns
+// the twiddles, every intermediate buffer and the proof buffer for one size
+// class (AIR, log_n, blowup, queries, pow), so repeated proofs allocate
 // nothing and run as a single stream of kernels ending in one D2H copy.
+// AIRs: the reference's Fibonacci AIR (KRAK v1/v2 bytes) and the synthetic
+// multi-column degree-2 AIR of the serving workload (vx_d2.h, KRAK v3).
 #pragma once
 #include <memory>
 #include <vector>
@@ -27,14 +29,23 @@ struct DeviceBuf {
   uint32_t* u32() const { return static_cast<uint32_t*>(p); }
 };
 
-// Word offsets of every variable field of a KRAK proof
-// (src/bin/stark_cli.rs:1092-1120).
+enum AirKind : uint32_t { AIR_FIBONACCI = 0, AIR_D2 = 1 };
+struct AirSpec {
+  uint32_t kind = AIR_FIBONACCI;
+  uint32_t n_cols = 1;    // AIR_D2: trace columns W
+  uint32_t n_cons = 1;    // AIR_D2: constraints K
+  uint64_t air_seed = 0;  // AIR_D2: constraint coefficient seed
+};
+
+// Word offsets of every variable field of a proof: KRAK v1/v2
+// (src/bin/stark_cli.rs:1092-1120) for Fibonacci, KRAK v3 for AIR_D2
+// (header, one root and one next-row decommitment per trace group).
 struct ProofLayout {
-  uint32_t log_n, blowup_bits, nq, pow_bits, le, n_fri;
-  uint64_t header_words, w_troot, w_qroot, w_nfri, w_froot0, w_last, w_nq, w_tdec, w_qdec, w_nfdec;
-  std::vector<uint64_t> w_fdec;
+  uint32_t log_n, blowup_bits, nq, pow_bits, le, n_fri, n_groups;
+  uint64_t header_words, w_qroot, w_nfri, w_froot0, w_last, w_nq, w_qdec, w_nfdec, w_nonce;
+  std::vector<uint64_t> w_troot, w_tdec, w_fdec;
   uint64_t total_words;
-  ProofLayout(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits);
+  ProofLayout(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits, const AirSpec& air = {});
 };
 
 // Reference-layout twiddle tables of one coset, resident on the device.
@@ -65,13 +76,15 @@ struct ProverTables {
   static uint64_t estimate_bytes(uint32_t log_n, uint32_t blowup_bits);
 };
 
-class FibProver {
+class StarkProver {
  public:
-  FibProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, hipStream_t stream);
-  ~FibProver();
-  // Enqueue one proof of Fib(a, b) on the stream (no host synchronisation).
-  // Untimed proofs replay a HIP graph captured on the first one (when graphs
-  // are enabled); the public inputs reach it through device memory.
+  StarkProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits, hipStream_t stream,
+              const AirSpec& air = {});
+  ~StarkProver();
+  // Enqueue one proof on the stream (no host synchronisation): Fib(a, b),
+  // or for AIR_D2 the trace of seed (a | b << 32). Untimed proofs replay a
+  // HIP graph captured on the first one (when graphs are enabled); the
+  // public inputs reach it through device memory.
   void enqueue(uint32_t a, uint32_t b, bool timed = false);
   // Wait for the last enqueued proof; returns its KRAK bytes.
   const uint8_t* wait(size_t* len);
@@ -87,7 +100,9 @@ class FibProver {
   }
   bool graphs() const { return use_graph_; }
   // Device bytes a prover of this class allocates for itself (tables excluded).
-  static uint64_t estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits);
+  static uint64_t estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits,
+                                 const AirSpec& air = {});
+  const AirSpec& air() const { return air_; }
 
   // device buffers for tests / introspection
   const uint32_t* d_eval() const { return eval_.u32(); }
@@ -104,17 +119,19 @@ class FibProver {
   void capture();
   TreeLevels tree_levels(uint32_t idx) const;
 
+  AirSpec air_;
   ProofLayout layout_;
-  uint32_t log_n_, blowup_, nq_, pow_bits_, le_, n_fri_;
+  uint32_t log_n_, blowup_, nq_, pow_bits_, le_, n_fri_, n_groups_;
   hipStream_t stream_;
   bool own_stream_ = false;
   uint64_t device_bytes_ = 0;
 
   std::shared_ptr<const ProverTables> tab_;
-  DeviceBuf trace_, eval_;
+  DeviceBuf trace_, eval_;         // W columns of 2^log_n / 2^le (W = 1 for Fibonacci)
+  DeviceBuf quot_, cons_, beta_;   // AIR_D2: stored quotient (SoA), constraint table, challenge powers
   std::vector<DeviceBuf> layers_;  // FRI layer k: 4 SoA columns of 2^(le-1-k)
   std::vector<uint64_t> layer_col_words_;
-  std::vector<DeviceBuf> trees_;   // 0 = trace, 1 = quotient, 2 + k = FRI layer k
+  std::vector<DeviceBuf> trees_;   // g < G trace groups, G = quotient, G + 1 + k = FRI layer k
   std::vector<uint32_t> tree_log_, tree_lo_;
   std::vector<std::vector<uint64_t>> tree_off_;
   DeviceBuf small_;                // channel | alphas | queries | prefix | nonce | inputs
@@ -137,6 +154,6 @@ class FibProver {
 // Host verifier (src/verifier.rs:39-310) over KRAK bytes; returns true and
 // leaves `err` empty when the proof is accepted.
 bool verify_proof(const uint8_t* bytes, size_t len, uint32_t blowup_bits, uint32_t n_queries, std::string& err);
-size_t proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits);
+size_t proof_size(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits, const AirSpec& air = {});
 
 }  // namespace vx

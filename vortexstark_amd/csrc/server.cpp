@@ -4,32 +4,50 @@
 #include <algorithm>
 #include <cstring>
 
+#include "vx_d2.h"
+
 namespace vx {
 
 namespace {
 
-class FibServed final : public ServedProver {
+AirSpec air_of(const ProofRequest& r) {
+  AirSpec a;
+  a.kind = r.air;
+  if (r.air == AIR_D2) {
+    a.n_cols = r.n_cols;
+    a.n_cons = r.n_cons;
+    a.air_seed = r.air_seed;
+  }
+  return a;
+}
+
+class Served final : public ServedProver {
  public:
-  FibServed(const ProofRequest& r, hipStream_t s, bool graphs)
-      : p_(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, s) {
+  Served(const ProofRequest& r, hipStream_t s, bool graphs)
+      : p_(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, s, air_of(r)) {
     p_.set_graphs(graphs);
     if (graphs) p_.prepare_graph();
   }
-  void start(const ProofRequest& r) override { p_.enqueue(r.a, r.b, false); }
+  void start(const ProofRequest& r) override {
+    if (r.air == AIR_D2) p_.enqueue((uint32_t)r.seed, (uint32_t)(r.seed >> 32), false);
+    else p_.enqueue(r.a, r.b, false);
+  }
   const uint8_t* result() const override { return p_.host_proof(); }
   size_t result_bytes() const override { return p_.proof_bytes(); }
   uint64_t device_bytes() const override { return p_.device_bytes(); }
 
  private:
-  FibProver p_;
+  StarkProver p_;
 };
 
 void check_request(const ProofRequest& r) {
-  require(r.air == AIR_FIBONACCI, "server: unknown AIR");
+  require(r.air == AIR_FIBONACCI || r.air == AIR_D2, "server: unknown AIR");
   require(r.log_n >= 4 && r.blowup_bits >= 1 && r.log_n + r.blowup_bits <= 31,
           "server: need log_n >= 4, blowup >= 1, log_n + blowup <= 31");
   require(r.n_queries >= 1 && r.n_queries <= 65536 && r.pow_bits <= 32, "server: bad n_queries / pow_bits");
-  require(r.a < P && r.b < P, "server: public inputs must be canonical M31");
+  if (r.air == AIR_FIBONACCI) require(r.a < P && r.b < P, "server: public inputs must be canonical M31");
+  else require(r.n_cols >= 1 && r.n_cols <= D2_MAX_COLS && r.n_cons >= 1 && r.n_cons <= D2_MAX_CONS,
+               "server: multi-column AIR needs 1..256 columns and 1..1024 constraints");
 }
 
 }  // namespace
@@ -66,15 +84,17 @@ double ProofServer::now() const {
 }
 
 ProofServer::Key ProofServer::key_of(const ProofRequest& r) {
-  return Key{r.air, r.log_n, r.blowup_bits, r.n_queries, r.pow_bits};
+  const bool d2 = r.air == AIR_D2;
+  return Key{r.air, r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, d2 ? r.n_cols : 0, d2 ? r.n_cons : 0,
+             d2 ? r.air_seed : 0};
 }
 
 uint64_t ProofServer::estimate(const ProofRequest& r) {
-  return FibProver::estimate_bytes(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits);
+  return StarkProver::estimate_bytes(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, air_of(r));
 }
 
 std::unique_ptr<ServedProver> ProofServer::create(const ProofRequest& r, hipStream_t s, uint64_t* bytes) {
-  auto p = std::make_unique<FibServed>(r, s, graphs_);
+  auto p = std::make_unique<Served>(r, s, graphs_);
   *bytes = p->device_bytes();
   return p;
 }
@@ -154,7 +174,7 @@ void ProofServer::prepare(const ProofRequest& r) {
 uint64_t ProofServer::submit(const ProofRequest& r, uint8_t* out, size_t cap) {
   check_request(r);
   require(out != nullptr, "server: null output buffer");
-  const size_t need = proof_size(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits);
+  const size_t need = proof_size(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, air_of(r));
   require(cap >= need, "server: output buffer smaller than the proof (see vx_proof_size)");
   uint64_t t;
   {

@@ -28,16 +28,16 @@
 
 namespace vx {
 
-enum : uint32_t { AIR_FIBONACCI = 0 };
-
 struct ProofRequest {
-  uint32_t air;          // AIR_FIBONACCI
+  uint32_t air;          // AIR_FIBONACCI or AIR_D2
   uint32_t log_n;
   uint32_t blowup_bits;
   uint32_t n_queries;
   uint32_t pow_bits;
   uint32_t a, b;         // Fibonacci public inputs
-  uint64_t seed;         // reserved for seeded AIRs
+  uint32_t n_cols, n_cons;  // AIR_D2 shape
+  uint64_t seed;         // AIR_D2 trace seed
+  uint64_t air_seed;     // AIR_D2 constraint coefficients
 };
 
 // What the server needs of a resident prover of one size class.
@@ -79,7 +79,7 @@ class ProofServer {
   uint32_t n_slots() const { return (uint32_t)slots_.size(); }
 
  private:
-  using Key = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>;
+  using Key = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t>;
   struct Job {
     ProofRequest req;
     uint8_t* out;

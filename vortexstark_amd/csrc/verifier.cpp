@@ -1,14 +1,16 @@
-// Fibonacci STARK verifier over KRAK proof bytes (src/verifier.rs:39-310,
+// STARK verifier over KRAK proof bytes (src/verifier.rs:39-310,
 // deserialisation src/bin/stark_cli.rs:1189-1230). Host code: it replays
 // the transcript, re-derives the queries, checks every Merkle path and every
 // FRI fold equation. KRAK v2 (PoW) additionally checks the nonce before the
-// query draws.
+// query draws; KRAK v3 (the synthetic multi-column AIR) also checks the
+// constraint composition at every query row.
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "prover.h"
+#include "vx_d2.h"
 
 namespace vx {
 
@@ -95,6 +97,189 @@ uint32_t fold_twiddle_at(const coset& dom, uint64_t folded, bool circle) {
 
 qm31 q_at(const std::vector<uint32_t>& v, size_t q) { return {{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]}}; }
 
+// The FRI part of a proof, shared by KRAK v1/v2 (Fibonacci) and v3
+// (multi-column AIR): quotient root, FRI roots, last layer, queries, then
+// (after the trace decommitments) the quotient and FRI decommitments.
+struct FriPart {
+  uint32_t qroot[8];
+  uint32_t n_fri = 0, n_last = 0, n_q = 0;
+  std::vector<uint32_t> froots, last;
+  std::vector<uint64_t> qidx;
+  Decom qd;
+  std::vector<Decom> fd;
+  std::vector<qm31> alphas;
+};
+
+bool read_fri_head(Reader& r, FriPart& f) {
+  for (auto& w : f.qroot) w = r.u32();
+  f.n_fri = r.u32();
+  if (r.bad || f.n_fri > 64) return false;
+  f.froots.resize(8ull * f.n_fri);
+  for (auto& w : f.froots) w = r.u32();
+  f.n_last = r.u32();
+  if (r.bad || f.n_last > 4096) return false;
+  f.last.resize(4ull * f.n_last);
+  for (auto& w : f.last) w = r.u32();
+  f.n_q = r.u32();
+  if (r.bad || f.n_q > (1u << 20)) return false;
+  f.qidx.resize(f.n_q);
+  for (auto& v : f.qidx) v = r.u32();
+  return !r.bad;
+}
+
+bool read_fri_tail(Reader& r, FriPart& f) {
+  if (!read_decom(r, f.qd, 4)) return false;
+  const uint32_t n_fd = r.u32();
+  if (r.bad || n_fd > 64) return false;
+  f.fd.resize(n_fd);
+  for (auto& d : f.fd)
+    if (!read_decom(r, d, 4)) return false;
+  return true;
+}
+
+// verifier.rs:56-96 from the quotient root on: FRI challenges, layer count,
+// last layer, optional PoW, query derivation. Returns "" or the failure.
+std::string replay_fri(Channel& ch, FriPart& f, uint32_t le, uint32_t nq, bool pow, uint32_t pow_bits,
+                       uint64_t nonce) {
+  const uint64_t N = 1ull << le;
+  ch_mix_digest(ch, f.qroot);
+  f.alphas.push_back(ch_draw_felt(ch));
+  for (uint32_t k = 0; k < f.n_fri; k++) {
+    ch_mix_digest(ch, f.froots.data() + 8 * k);
+    f.alphas.push_back(ch_draw_felt(ch));
+  }
+  if (f.n_fri != le - 3) return "Expected " + std::to_string(le - 3) + " FRI layers, got " + std::to_string(f.n_fri);
+  if (f.n_last != 8) return "Expected 8 FRI last layer values, got " + std::to_string(f.n_last);
+  ch_mix_felts(ch, f.last.data(), f.n_last);
+  if (pow) {
+    uint32_t prefix[8];
+    ch_pow_prefix(ch, pow_bits, prefix);
+    if (pow_trailing_zeros(prefix, nonce) < pow_bits) return "PoW nonce invalid";
+    ch_mix_u64(ch, nonce);
+  }
+  for (uint32_t i = 0; i < nq; i++)
+    if (ch_draw_number(ch, N) != f.qidx[i]) return "Query indices don't match Fiat-Shamir derivation";
+  return "";
+}
+
+// verifier.rs:119-144,153-233: FRI auth paths and fold equations.
+std::string check_fri_layers(const FriPart& f, uint32_t le, uint32_t nq) {
+  const uint32_t n_fd = (uint32_t)f.fd.size();
+  std::vector<uint64_t> fidx(nq);
+  for (uint32_t i = 0; i < nq; i++) fidx[i] = f.qidx[i] / 2;
+  for (uint32_t k = 0; k < n_fd && k < f.n_fri; k++) {
+    if (!check_decom(f.fd[k], f.froots.data() + 8 * k, fidx, nq))
+      return "FRI layer " + std::to_string(k) + " auth path invalid";
+    for (auto& v : fidx) v /= 2;
+  }
+  if (n_fd == 0) return "missing FRI decommitments";
+  for (uint32_t k = 0; k < n_fd; k++)
+    if (f.fd[k].n() != nq) return "FRI layer decommitment size mismatch";
+  for (uint32_t q = 0; q < nq; q++) {
+    uint64_t ci = f.qidx[q];
+    uint32_t cl = le;
+    qm31 f0 = q_at(f.qd.vals, q), f1 = q_at(f.qd.sibs, q);
+    if (ci % 2) std::swap(f0, f1);
+    qm31 e = fri_fold_pair(f0, f1, f.alphas[0], fold_twiddle_at(eval_domain(cl), ci / 2, true));
+    if (!qm_eq(e, q_at(f.fd[0].vals, q))) return "Circle fold mismatch at query " + std::to_string(q);
+    ci /= 2;
+    cl--;
+    for (uint32_t k = 0; k + 1 < n_fd; k++) {
+      qm31 v = q_at(f.fd[k].vals, q), s = q_at(f.fd[k].sibs, q);
+      if (ci % 2) std::swap(v, s);
+      qm31 ex = fri_fold_pair(v, s, f.alphas[k + 1], fold_twiddle_at(make_coset(HALF_ODDS, cl), ci / 2, false));
+      if (!qm_eq(ex, q_at(f.fd[k + 1].vals, q)))
+        return "Line fold mismatch at query " + std::to_string(q) + ", layer " + std::to_string(k);
+      ci /= 2;
+      cl--;
+    }
+    if (ci >= f.n_last) return "FRI last layer index out of bounds";
+    if (!qm_eq(q_at(f.fd[n_fd - 1].vals, q), q_at(f.last, ci)))
+      return "FRI last layer mismatch at query " + std::to_string(q);
+  }
+  return "";
+}
+
+// KRAK v3: the synthetic multi-column AIR (vx_d2.h). Per trace group:
+// n, values at q, values at q + 1, paths at q, paths at q + 1. Beyond the
+// Fibonacci checks, the quotient value at each query must equal the AIR's
+// composition of the committed rows q and q + 1.
+bool verify_d2(Reader& r, uint32_t blowup_bits, uint32_t nq, std::string& err) {
+  auto fail = [&](const std::string& m) {
+    err = m;
+    return false;
+  };
+  const uint32_t log_n = r.u32(), W = r.u32(), K = r.u32(), bb = r.u32(), pow_bits = r.u32();
+  uint64_t nonce = r.u32();
+  nonce |= (uint64_t)r.u32() << 32;
+  (void)r.u32();  // trace seed: public metadata, not re-derived
+  (void)r.u32();
+  uint64_t air_seed = r.u32();
+  air_seed |= (uint64_t)r.u32() << 32;
+  if (r.bad) return fail("truncated proof");
+  if (bb != blowup_bits) return fail("blowup mismatch");
+  if (W < 1 || W > D2_MAX_COLS || K < 1 || K > D2_MAX_CONS) return fail("bad AIR shape");
+  if (pow_bits > 128) return fail("bad pow bits");
+  const uint32_t G = r.u32();
+  if (r.bad || G != d2_groups(W)) return fail("bad trace group count");
+  std::vector<uint32_t> troots(8ull * G);
+  for (auto& w : troots) w = r.u32();
+  FriPart f;
+  if (!read_fri_head(r, f)) return fail("truncated proof");
+  std::vector<Decom> td(G);  // vals = rows q, sibs = rows q + 1
+  for (uint32_t g = 0; g < G; g++)
+    if (!read_decom(r, td[g], d2_group_cols(W, g))) return fail("truncated proof");
+  if (!read_fri_tail(r, f)) return fail("truncated proof");
+  if (r.pos != r.len) return fail("trailing bytes");
+
+  const uint32_t le = log_n + blowup_bits;
+  if (log_n < 1 || le > 31) return fail("bad log size");
+  const uint64_t mask = (1ull << le) - 1;
+  if (f.n_q != nq) return fail("Expected " + std::to_string(nq) + " queries, got " + std::to_string(f.n_q));
+  Channel ch;
+  ch_init(ch);
+  for (uint32_t g = 0; g < G; g++) ch_mix_digest(ch, troots.data() + 8 * g);
+  const qm31 alpha = ch_draw_felt(ch);
+  if (!pow_bits && nonce) return fail("nonce without PoW");
+  std::string e = replay_fri(ch, f, le, nq, pow_bits != 0, pow_bits, nonce);
+  if (!e.empty()) return fail(e);
+  for (uint32_t g = 0; g < G; g++) {
+    const Decom& d = td[g];
+    if (d.n() != nq || d.paths.size() != nq) return fail("trace group " + std::to_string(g) + " size mismatch");
+    for (uint32_t q = 0; q < nq; q++) {
+      uint32_t h[8];
+      leaf_hash(d.vals.data() + (size_t)q * d.width, d.width, h);
+      if (d.paths[q].size() != 8ull * le || !check_path(troots.data() + 8 * g, h, f.qidx[q], d.paths[q]))
+        return fail("trace group " + std::to_string(g) + " auth path invalid");
+      leaf_hash(d.sibs.data() + (size_t)q * d.width, d.width, h);
+      if (d.spaths[q].size() != 8ull * le ||
+          !check_path(troots.data() + 8 * g, h, (f.qidx[q] + 1) & mask, d.spaths[q]))
+        return fail("trace group " + std::to_string(g) + " next-row auth path invalid");
+    }
+  }
+  if (!check_decom(f.qd, f.qroot, f.qidx, nq)) return fail("quotient auth path invalid");
+  std::vector<uint32_t> cd(2ull * K), row(W), nxt(W);
+  d2_coeffs(air_seed, K, cd.data());
+  for (uint32_t q = 0; q < nq; q++) {
+    for (uint32_t g = 0; g < G; g++)
+      for (uint32_t c = 0; c < td[g].width; c++) {
+        row[D2_GROUP * g + c] = td[g].vals[(size_t)q * td[g].width + c];
+        nxt[D2_GROUP * g + c] = td[g].sibs[(size_t)q * td[g].width + c];
+      }
+    qm31 acc{{0, 0, 0, 0}}, beta = alpha;
+    for (uint32_t k = 0; k < K; k++) {
+      uint32_t m = m31_add(m31_mul(cd[2 * k], m31_mul(row[d2_col_a(k, W)], nxt[d2_col_b(k, W)])),
+                           m31_mul(cd[2 * k + 1], row[d2_col_e(k, W)]));
+      acc = qm_add(acc, qm_scale(beta, m));
+      beta = qm_mul(beta, alpha);
+    }
+    if (!qm_eq(acc, q_at(f.qd.vals, q))) return fail("Constraint quotient mismatch at query " + std::to_string(q));
+  }
+  e = check_fri_layers(f, le, nq);
+  if (!e.empty()) return fail(e);
+  return true;
+}
+
 }  // namespace
 
 bool verify_proof(const uint8_t* bytes, size_t len, uint32_t blowup_bits, uint32_t nq, std::string& err) {
@@ -106,6 +291,7 @@ bool verify_proof(const uint8_t* bytes, size_t len, uint32_t blowup_bits, uint32
   Reader r{bytes, len};
   if (r.u32() != 0x4B52414Bu) return fail("bad magic");
   const uint32_t version = r.u32();
+  if (version == 3) return verify_d2(r, blowup_bits, nq, err);
   if (version != 1 && version != 2) return fail("unsupported proof version " + std::to_string(version));
   const uint32_t log_n = r.u32(), a = r.u32(), b = r.u32();
   uint32_t pow_bits = 0;
@@ -118,104 +304,38 @@ bool verify_proof(const uint8_t* bytes, size_t len, uint32_t blowup_bits, uint32
     if (bb != blowup_bits) return fail("blowup mismatch");
     if (pow_bits > 128) return fail("bad pow bits");
   }
-  uint32_t troot[8], qroot[8];
+  uint32_t troot[8];
   for (auto& w : troot) w = r.u32();
-  for (auto& w : qroot) w = r.u32();
-  const uint32_t n_fri = r.u32();
-  if (r.bad || n_fri > 64) return fail("truncated proof");
-  std::vector<uint32_t> froots(8ull * n_fri);
-  for (auto& w : froots) w = r.u32();
-  const uint32_t n_last = r.u32();
-  if (r.bad || n_last > 4096) return fail("truncated proof");
-  std::vector<uint32_t> last(4ull * n_last);
-  for (auto& w : last) w = r.u32();
-  const uint32_t n_q = r.u32();
-  if (r.bad || n_q > (1u << 20)) return fail("truncated proof");
-  std::vector<uint64_t> qidx(n_q);
-  for (auto& v : qidx) v = r.u32();
-  Decom td, qd;
-  if (!read_decom(r, td, 1) || !read_decom(r, qd, 4)) return fail("truncated proof");
-  const uint32_t n_fd = r.u32();
-  if (r.bad || n_fd > 64) return fail("truncated proof");
-  std::vector<Decom> fd(n_fd);
-  for (auto& d : fd)
-    if (!read_decom(r, d, 4)) return fail("truncated proof");
+  FriPart f;
+  if (!read_fri_head(r, f)) return fail("truncated proof");
+  Decom td;
+  if (!read_decom(r, td, 1) || !read_fri_tail(r, f)) return fail("truncated proof");
 
   const uint32_t le = log_n + blowup_bits;
   if (log_n < 1 || le > 31) return fail("bad log size");
-  const uint64_t N = 1ull << le;
   // verifier.rs:46-53
   if (a >= P || b >= P) return fail("Public inputs out of M31 range");
-  if (n_q != nq) return fail("Expected " + std::to_string(nq) + " queries, got " + std::to_string(n_q));
+  if (f.n_q != nq) return fail("Expected " + std::to_string(nq) + " queries, got " + std::to_string(f.n_q));
   // verifier.rs:56-70: transcript replay
   Channel ch;
   ch_init(ch);
   ch_mix_digest(ch, troot);
   (void)ch_draw_felt(ch);
-  ch_mix_digest(ch, qroot);
-  std::vector<qm31> alphas;
-  alphas.push_back(ch_draw_felt(ch));
-  for (uint32_t k = 0; k < n_fri; k++) {
-    ch_mix_digest(ch, froots.data() + 8 * k);
-    alphas.push_back(ch_draw_felt(ch));
-  }
-  // verifier.rs:73-87
-  if (n_fri != le - 3) return fail("Expected " + std::to_string(le - 3) + " FRI layers, got " + std::to_string(n_fri));
-  if (n_last != 8) return fail("Expected 8 FRI last layer values, got " + std::to_string(n_last));
-  // verifier.rs:90-96
-  ch_mix_felts(ch, last.data(), n_last);
-  if (version == 2) {
-    uint32_t prefix[8];
-    ch_pow_prefix(ch, pow_bits, prefix);
-    if (pow_trailing_zeros(prefix, nonce) < pow_bits) return fail("PoW nonce invalid");
-    ch_mix_u64(ch, nonce);
-  }
-  for (uint32_t i = 0; i < nq; i++)
-    if (ch_draw_number(ch, N) != qidx[i]) return fail("Query indices don't match Fiat-Shamir derivation");
+  std::string e = replay_fri(ch, f, le, nq, version == 2, pow_bits, nonce);
+  if (!e.empty()) return fail(e);
   // verifier.rs:99-104
   bool tz = true, qz = true;
   for (int i = 0; i < 8; i++) {
     tz = tz && troot[i] == 0;
-    qz = qz && qroot[i] == 0;
+    qz = qz && f.qroot[i] == 0;
   }
   if (tz) return fail("Trace commitment is zero");
   if (qz) return fail("Quotient commitment is zero");
   // verifier.rs:107-144
-  if (!check_decom(td, troot, qidx, nq)) return fail("trace auth path invalid");
-  if (!check_decom(qd, qroot, qidx, nq)) return fail("quotient auth path invalid");
-  std::vector<uint64_t> fidx(nq);
-  for (uint32_t i = 0; i < nq; i++) fidx[i] = qidx[i] / 2;
-  for (uint32_t k = 0; k < n_fd && k < n_fri; k++) {
-    if (!check_decom(fd[k], froots.data() + 8 * k, fidx, nq))
-      return fail("FRI layer " + std::to_string(k) + " auth path invalid");
-    for (auto& v : fidx) v /= 2;
-  }
-  // verifier.rs:153-233: fold equations
-  if (n_fd == 0) return fail("missing FRI decommitments");
-  for (uint32_t k = 0; k < n_fd; k++)
-    if (fd[k].n() != nq) return fail("FRI layer decommitment size mismatch");
-  for (uint32_t q = 0; q < nq; q++) {
-    uint64_t ci = qidx[q];
-    uint32_t cl = le;
-    qm31 f0 = q_at(qd.vals, q), f1 = q_at(qd.sibs, q);
-    if (ci % 2) std::swap(f0, f1);
-    qm31 e = fri_fold_pair(f0, f1, alphas[0], fold_twiddle_at(eval_domain(cl), ci / 2, true));
-    if (!qm_eq(e, q_at(fd[0].vals, q))) return fail("Circle fold mismatch at query " + std::to_string(q));
-    ci /= 2;
-    cl--;
-    for (uint32_t k = 0; k + 1 < n_fd; k++) {
-      qm31 v = q_at(fd[k].vals, q), s = q_at(fd[k].sibs, q);
-      if (ci % 2) std::swap(v, s);
-      qm31 ex = fri_fold_pair(v, s, alphas[k + 1], fold_twiddle_at(make_coset(HALF_ODDS, cl), ci / 2, false));
-      if (!qm_eq(ex, q_at(fd[k + 1].vals, q)))
-        return fail("Line fold mismatch at query " + std::to_string(q) + ", layer " + std::to_string(k));
-      ci /= 2;
-      cl--;
-    }
-    if (ci >= n_last) return fail("FRI last layer index out of bounds");
-    if (!qm_eq(q_at(fd[n_fd - 1].vals, q), q_at(last, ci)))
-      return fail("FRI last layer mismatch at query " + std::to_string(q));
-  }
+  if (!check_decom(td, troot, f.qidx, nq)) return fail("trace auth path invalid");
+  if (!check_decom(f.qd, f.qroot, f.qidx, nq)) return fail("quotient auth path invalid");
+  e = check_fri_layers(f, le, nq);
+  if (!e.empty()) return fail(e);
   return true;
 }
 

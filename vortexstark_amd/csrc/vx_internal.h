@@ -58,9 +58,12 @@ struct NttPass {
 //            every block (LDE, the first log_n - log(src) layers are copies).
 //            Only bits below `eff_bits` are transformed.
 //   inverse: interpolate, then scale by 2^((30 log_n) mod 31).
+// Batched over n_cols columns (one per blockIdx.y): column c of data starts
+// at data + c * dst_stride (stride for the inverse), of src at src + c * src_stride.
 void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
-                 uint32_t eff_bits, hipStream_t s);
-void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s);
+                 uint32_t eff_bits, hipStream_t s, uint32_t n_cols = 1, uint64_t dst_stride = 0,
+                 uint64_t src_stride = 0);
+void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n_cols = 1, uint64_t stride = 0);
 // One reference layer (cuda/circle_ntt.cu:89-117).
 void ntt_layer(uint32_t* data, const uint32_t* tw, uint32_t layer_idx, uint64_t n, bool forward,
                hipStream_t s);
@@ -104,6 +107,7 @@ enum LeafKind : uint32_t {
   LEAF_QUOTIENT = 1,      // Fibonacci quotient of an eval column (cuda/constraint.cu:10-36)
   LEAF_FOLD_CIRCLE_Q = 2, // circle fold of that quotient (cuda/fri.cu:92-126), writes the layer
   LEAF_FOLD_LINE = 3,     // line fold of a QM31 SoA layer (cuda/fri.cu:58-87), writes the layer
+  LEAF_D2_QUOTIENT = 4,   // synthetic AIR quotient of W eval columns (vx_d2.h), writes Q (SoA)
 };
 struct LeafSrc {
   uint32_t kind;
@@ -115,6 +119,13 @@ struct LeafSrc {
   const uint32_t* alpha_q;   // device QM31: quotient alpha
   const uint32_t* alpha_f;   // device QM31: fold alpha
   uint64_t src_mask;         // eval length - 1 (quotient wraparound)
+  // LEAF_D2_QUOTIENT: col[0] = first of n_cols eval columns, col_stride
+  // words apart; per constraint 8 words {c, d, a, b, e, 0, 0, 0}; beta =
+  // n_cons QM31 powers of the challenge
+  uint64_t col_stride;
+  uint32_t n_cons;
+  const uint32_t* cons;
+  const uint32_t* beta;
 };
 // What the kernel that produces a root does with it: copy it to root_out,
 // and, with a channel, mix it (after ch_init when `init`) and draw the next
@@ -162,13 +173,20 @@ void merkle_leaf_pairs(const LeafSrc& leaf, uint64_t n_leaves, uint32_t* parents
 
 // Decommitment of one tree at the query positions (src/prover.rs:551-618),
 // written straight into the KRAK byte buffer.
-enum DecomKind : uint32_t { DECOM_M31 = 0, DECOM_SOA4 = 1, DECOM_QUOTIENT = 2 };
+// Values of a leaf: DECOM_COLS = n_cols (<= 16) M31 columns, DECOM_SOA4 = a
+// QM31 SoA layer, DECOM_QUOTIENT = the Fibonacci quotient recomputed from
+// the eval column. The partner leaf of query leaf i is i ^ 1 (PAIR_SIBLING,
+// the KRAK layout) or (i + 1) mod 2^log_n (PAIR_NEXT, next-row openings).
+enum DecomKind : uint32_t { DECOM_COLS = 0, DECOM_SOA4 = 1, DECOM_QUOTIENT = 2 };
+enum DecomPair : uint32_t { PAIR_SIBLING = 0, PAIR_NEXT = 1 };
 struct DecomDesc {
   uint32_t kind;
+  uint32_t n_cols;       // DECOM_COLS
+  uint32_t pair;
   uint32_t log_n;
   uint32_t lo;           // lowest stored tree level
   uint32_t shift;        // leaf index = query >> shift
-  const uint32_t* col[4];
+  const uint32_t* col[16];
   const uint32_t* alpha; // DECOM_QUOTIENT
   const uint32_t* tree;
   uint64_t off[MAX_LEVELS];
@@ -202,5 +220,10 @@ void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, 
 void grind_pow_batch(const uint32_t* d_prefix, uint64_t* d_result, uint32_t pow_bits,
                      uint64_t batch_offset, uint64_t n_nonces, hipStream_t s);
 void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_t s);
+
+// ------------------------------------- synthetic multi-column AIR (d2.hip)
+void d2_trace(const uint32_t* seed_words, uint32_t* out, uint32_t n_cols, uint32_t log_n, uint64_t stride,
+              hipStream_t s);
+void d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons, hipStream_t s);
 
 }  // namespace vx
