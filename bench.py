@@ -185,8 +185,60 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
     }
 
 
-def copy_calibration(vx, mib=256):
-    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+def batched(vx, steps, n_cols=96, seed=42, blowup_bits=2):
+    """The batched uneven-commit workload: n_cols columns with log sizes drawn
+    from {14..22} by the seed, values from the synthetic generator; one step =
+    work-list LDE at blowup 4 + batched bit reverse + lifted Blake2s
+    commitment of all columns into one tree (inputs resident in HBM)."""
+    logs = vx.batch_log_sizes(n_cols, seed)
+    b = vx.Batch(logs, blowup_bits, True)
+    b.fill_random(seed)
+    root = b.run()
+    vx._lib.call("vx_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.enqueue()
+    root = b.wait()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    b.run(timed=True)
+    st = b.stage_times()
+    S = logs[-1] + blowup_bits
+    n_in = sum(1 << L for L in logs)
+    n_ev = sum(1 << (L + blowup_bits) for L in logs)
+    blocks = (n_cols + 15) // 16
+    compressions = blocks * (1 << S) + (1 << S) - 1
+    lde_bytes = 4 * n_in + 4 * n_in + 4 * n_ev   # read inputs, write coefficients, write evaluations
+    br_bytes = 8 * n_ev
+    want = None
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
+            g = json.load(f)
+        want = next((x["root"] for x in g.get("batch_c", []) if x["n_cols"] == n_cols and x["seed"] == seed), None)
+    except OSError:
+        pass
+    out = {
+        "workload": f"{n_cols} columns, log sizes 14..22 (seed {seed}), blowup 4, bit-reversed, one lifted tree",
+        "ms_per_batch": round(ms, 3),
+        "stages_ms": {k: round(v, 4) for k, v in st.items()},
+        "columns_per_s": round(n_cols / (ms * 1e-3), 1),
+        "input_elements": n_in,
+        "lde_elements": n_ev,
+        "lde_gbs": round(lde_bytes / (st["lde"] * 1e-3) / 1e9, 1),
+        "lde_hbm_frac": round(lde_bytes / (st["lde"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "bit_reverse_gbs": round(br_bytes / (st["bit_reverse"] * 1e-3) / 1e9, 1),
+        "bit_reverse_hbm_frac": round(br_bytes / (st["bit_reverse"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "commit_tops": round(compressions * OPS_PER_COMPRESSION / (st["commit"] * 1e-3) / 1e12, 2),
+        "commit_valu_frac": round(compressions * OPS_PER_COMPRESSION / (st["commit"] * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
+        "lifted_root_matches_oracle": (root == want) if want else None,
+        "device_bytes": b.device_bytes,
+    }
+    b.close()
+    return out
+
+
+def copy_calibration(vx, mib=1024):
+    """A plain device-to-device copy of known size (1 GiB, four times the
+    256 MB MALL, so the source streams from HBM): its FETCH_SIZE / WRITE_SIZE
     counters scale the per-kernel HBM traffic reported from rocprofv3."""
     import ctypes as C
 
@@ -214,12 +266,14 @@ def main():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: one 256 MiB device copy (counter calibration), "
+                    help="short run for rocprofv3: one 1 GiB device copy (counter calibration), "
                          "1 warmup + 2 timed proofs, no CPU baseline")
     args = ap.parse_args()
     if args.profile:
-        args.steps, args.warmup, args.no_cpu_baseline, args.no_serve = 2, 1, True, True
+        args.steps, args.warmup, args.no_cpu_baseline, args.no_serve, args.no_batch = 2, 1, True, True, True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
@@ -230,6 +284,9 @@ def main():
     vx.load()
     if args.serve_only:
         print(json.dumps({"serving": serving(vx, args.serve_requests, args.slots)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batched_commit": batched(vx, args.steps)}), flush=True)
         return
     if args.profile:
         copy_calibration(vx)
@@ -308,6 +365,8 @@ def main():
         out["verify_error"] = why
     if not args.no_serve and not args.profile:
         out["serving"] = serving(vx, args.serve_requests, args.slots)
+    if not args.no_batch and not args.profile:
+        out["batched_commit"] = batched(vx, args.steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(log_n, max(log_n - 4, 10))
     prover.close()

@@ -254,6 +254,61 @@ int vx_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint3
 int vx_verify(const uint8_t *proof, size_t len, uint32_t blowup_bits, uint32_t n_queries, char *err,
               size_t err_cap);
 
+/* ------------------------------------------------- batched uneven columns
+ * Many M31 columns of different log sizes through one launch per phase,
+ * driven by device work lists. The reference batches only equal-size
+ * columns, one launch per layer (src/ntt.rs:191-281,
+ * cuda/circle_ntt.cu:314-404). Twiddle tables are built by the library per
+ * distinct size (shared with provers of that size). */
+
+/* Synthetic data: d[i] = mix64(seed + G ((col << 32 | i) + 1)) mod P for
+ * i < 2^log_n (G = 0x9E3779B97F4A7C15, mix64 = the SplitMix64 finaliser):
+ * column `col` of the synthetic AIR's trace generator. Synchronises. */
+int vx_m31_fill_random(uint32_t *d_data, uint32_t log_n, uint64_t seed, uint32_t col, vx_stream_t stream);
+
+/* Interpolate column c (2^log_sizes[c] values at h_in[c], 14..24) on
+ * half_coset(log), write the coefficients to h_coef[c] and the evaluation on
+ * the 2^blowup-times larger eval domain to h_eval[c]: the same values as
+ * vx_circle_ntt_interpolate + vx_circle_ntt_evaluate of the zero-padded
+ * coefficients (src/prover.rs:293-318), per column. Host arrays of device
+ * pointers; synchronises the stream. */
+int vx_circle_ntt_lde_batch(const uint32_t *const *h_in, uint32_t *const *h_coef, uint32_t *const *h_eval,
+                            const uint32_t *h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, vx_stream_t stream);
+/* Out-of-place dst[c][bitrev(i)] = src[c][i], log sizes >= 10
+ * (cuda_bit_reverse_m31, src/cuda/ffi.rs:311, for many columns at once). */
+int vx_bit_reverse_m31_batch(const uint32_t *const *h_src, uint32_t *const *h_dst, const uint32_t *h_log_sizes,
+                             uint32_t n_cols, vx_stream_t stream);
+/* cuda_build_leaves_lifted (src/cuda/ffi.rs:937-947,
+ * cuda/merkle_leaves_lifted.cu): leaf i of a 2^lifting_log_size tree =
+ * Blake2s of every column's value at its lifted row
+ * ((i >> (S - L + 1)) << 1 | (i & 1) for a column of log size L < S), columns
+ * sorted by ascending size. The schedule of same-size chunks is derived from
+ * h_log_sizes. d_hashes: 8 words per leaf. */
+int vx_build_leaves_lifted(const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
+                           uint32_t lifting_log_size, uint32_t *d_hashes, vx_stream_t stream);
+/* Root of that lifted tree (lifting size = the largest column), nodes
+ * Blake2s(left || right). */
+int vx_merkle_root_lifted(const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
+                          uint32_t *d_root, vx_stream_t stream);
+
+/* Resident batch (the batched-commit workload): inputs, coefficients and
+ * evaluations of n_cols columns (log sizes sorted ascending, 14..24) kept in
+ * HBM; each run is four work-list LDE launches, one bit-reverse launch and one
+ * lifted commitment. */
+typedef struct vx_batch vx_batch;
+int vx_batch_create(const uint32_t *h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, int bit_reverse,
+                    vx_stream_t stream, vx_batch **out);
+int vx_batch_destroy(vx_batch *b);
+/* which: 0 input values (2^L), 1 coefficients (2^L), 2 evaluation
+ * (2^(L+blowup), bit-reversed when enabled). */
+int vx_batch_column(const vx_batch *b, uint32_t col, int which, uint32_t **d_ptr, uint64_t *words);
+int vx_batch_enqueue(vx_batch *b, int timed);
+/* Wait for the run; h_root (host, may be NULL) receives the lifted root. */
+int vx_batch_wait(vx_batch *b, uint32_t h_root[8]);
+/* ms of the last timed run: lde, bit reverse, commit, total. */
+int vx_batch_stage_times(vx_batch *b, float out[4]);
+uint64_t vx_batch_device_bytes(const vx_batch *b);
+
 /* ------------------------------------------------------------------ serving
  * Replaces the reference's one-worker proof queue (src/bin/dashboard.rs:
  * 66-96, drained one proof at a time; ProverPipeline::prove_batch,

@@ -89,6 +89,10 @@ def lib(omp=False):
                                   C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz)]),
         "or_d2_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
         "or_d2_proof_size": (sz, [u32, u32, u32, u32]),
+        "or_lde": (None, [_u32p, u32, u32, _u32p]),
+        "or_d2_column": (None, [u64, u32, u32, _u32p]),
+        "or_build_leaves_lifted": (None, [C.c_void_p, _u32p, u32, u32, _u32p]),
+        "or_merkle_root_lifted": (None, [C.c_void_p, _u32p, u32, _u32p]),
     }
     for name_, (res, args) in sig.items():
         f = getattr(L, name_)
@@ -355,3 +359,47 @@ def d2_verify(proof: bytes, blowup_bits=2, n_queries=80):
 
 def d2_proof_size(log_n, n_cols=D2_COLS, blowup_bits=2, n_queries=80):
     return lib().or_d2_proof_size(log_n, n_cols, blowup_bits, n_queries)
+
+
+# ---- batched uneven columns ----
+def lde(values, blowup_bits=2):
+    v = u32(values)
+    log_n = v.size.bit_length() - 1
+    out = np.zeros(v.size << blowup_bits, np.uint32)
+    lib().or_lde(v, log_n, blowup_bits, out)
+    return out
+
+
+def leaves_lifted(cols, lifting_log, omp=True):
+    cs = [u32(c) for c in cols]
+    logs = u32([c.size.bit_length() - 1 for c in cs])
+    out = np.zeros(8 << lifting_log, np.uint32)
+    lib(omp).or_build_leaves_lifted(_ptrs(cs), logs, len(cs), lifting_log, out)
+    return out.reshape(-1, 8)
+
+
+def merkle_root_lifted(cols, omp=True):
+    cs = [u32(c) for c in cols]
+    logs = u32([c.size.bit_length() - 1 for c in cs])
+    out = np.zeros(8, np.uint32)
+    lib(omp).or_merkle_root_lifted(_ptrs(cs), logs, len(cs), out)
+    return out
+
+
+def d2_column(seed, col, log_n, omp=True):
+    out = np.zeros(1 << log_n, np.uint32)
+    lib(omp).or_d2_column(seed, col, log_n, out)
+    return out
+
+
+def batch_root(log_sizes, seed, blowup_bits=2, bit_reverse=True, omp=True):
+    """Lifted root of the batched workload: column c = d2_column(seed, c,
+    L_c), LDE at 2^blowup_bits, bit reversed, lifted tree."""
+    evs = []
+    for c, L in enumerate(log_sizes):
+        e = np.zeros(1 << (L + blowup_bits), np.uint32)
+        lib(omp).or_lde(d2_column(seed, c, L, omp), L, blowup_bits, e)
+        if bit_reverse:
+            lib(omp).or_bit_reverse_m31(e, L + blowup_bits)
+        evs.append(e)
+    return merkle_root_lifted(evs, omp)

@@ -1484,3 +1484,51 @@ done:
     free(nrow);
     return ok;
 }
+
+/* ======================================================================
+ * Batched uneven columns: per-column LDE (the Fibonacci prover's
+ * interpolate + zero-pad + evaluate, src/prover.rs:280-318) and the lifted
+ * Merkle tree of the reference's stwo backend (cuda/merkle_leaves_lifted.cu:
+ * 26-98, columns sorted by ascending size, src/../merkle_ops.rs schedule).
+ * ====================================================================== */
+void or_lde(const uint32_t *in, uint32_t log_n, uint32_t blowup_bits, uint32_t *out) {
+    uint64_t n = 1ull << log_n;
+    memcpy(out, in, n * 4);
+    lde_column(out, log_n, log_n + blowup_bits);
+}
+
+static uint64_t lifted_row(uint64_t idx, uint32_t S, uint32_t L) {
+    return L == S ? idx : (((idx >> (S - L + 1)) << 1) | (idx & 1));
+}
+/* leaf i = Blake2s(all columns' words at their lifted rows), multi-block */
+void or_build_leaves_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
+                            uint32_t lifting_log, uint32_t *hashes) {
+    uint64_t n = 1ull << lifting_log;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        uint8_t *buf = (uint8_t *)malloc(4 * (size_t)n_cols);
+        for (uint32_t c = 0; c < n_cols; c++)
+            put32(buf + 4 * c, cols[c][lifted_row((uint64_t)i, lifting_log, log_sizes[c])]);
+        uint8_t h[32];
+        or_blake2s(buf, 4 * (size_t)n_cols, h);
+        for (int k = 0; k < 8; k++) hashes[8 * (uint64_t)i + k] = le32(h + 4 * k);
+        free(buf);
+    }
+}
+void or_merkle_root_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
+                           uint32_t root[8]) {
+    uint32_t S = log_sizes[n_cols - 1];
+    uint64_t n = 1ull << S;
+    uint32_t *leaves = (uint32_t *)malloc(n * 32);
+    or_build_leaves_lifted(cols, log_sizes, n_cols, S, leaves);
+    tree_t t;
+    tree_build_from_leaves(&t, S, leaves);
+    memcpy(root, tree_root(&t), 32);
+    tree_free(&t);
+}
+/* column `col` of the synthetic generator (or_d2_value) */
+void or_d2_column(uint64_t seed, uint32_t col, uint32_t log_n, uint32_t *out) {
+    uint64_t n = 1ull << log_n;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) out[i] = or_d2_value(seed, col, (uint64_t)i);
+}

@@ -20,6 +20,10 @@
  *   prove / verify        independent pure-Python prover byte-equality
  *                         (log_n 4..6), verifier accept + tamper-reject
  *   pow                   brute-force smallest nonce via hashlib
+ *   d2 AIR (build-defined) independent Python restatement (pyref.prove_d2),
+ *                         SplitMix64 reference output, closed forms
+ *   lifted Merkle         Python hashlib over the lifted rows; the LDE is the
+ *                         pinned single-column LDE per column
  */
 #ifndef VX_ORACLE_H
 #define VX_ORACLE_H
@@ -124,6 +128,7 @@ void or_free(void *p);
  *      proof bytes KRAK v3 (definition above or_d2_value in vx_oracle.c) ---- */
 uint32_t or_d2_value(uint64_t seed, uint32_t col, uint64_t row);
 void or_d2_trace(uint64_t seed, uint32_t n_cols, uint32_t log_n, uint32_t *out);
+void or_d2_column(uint64_t seed, uint32_t col, uint32_t log_n, uint32_t *out);
 void or_d2_coeffs(uint64_t air_seed, uint32_t n_cons, uint32_t *cd);
 void or_d2_quotient(const uint32_t *eval, uint32_t n_cols, uint32_t le, uint32_t n_cons, uint64_t air_seed,
                     const uint32_t alpha[4], uint32_t *q0, uint32_t *q1, uint32_t *q2, uint32_t *q3);
@@ -132,6 +137,14 @@ int or_d2_prove(uint64_t seed, uint32_t log_n, uint32_t n_cols, uint32_t n_cons,
 int or_d2_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits, uint32_t n_queries, char *err,
                  size_t err_cap);
 size_t or_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries);
+
+/* ---- batched uneven columns: per-column LDE (src/prover.rs:280-318) and the
+ *      lifted Merkle tree (cuda/merkle_leaves_lifted.cu:26-98) ---- */
+void or_lde(const uint32_t *in, uint32_t log_n, uint32_t blowup_bits, uint32_t *out);
+void or_build_leaves_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
+                            uint32_t lifting_log, uint32_t *hashes);
+void or_merkle_root_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
+                           uint32_t root[8]);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,8 @@ Sources, per section:
   proofs_d2_py pyref.prove_d2 (synthetic 34-column AIR, KRAK v3), independent
                of the C oracle
   proofs_d2_c  the C oracle's multi-column AIR proofs at larger sizes
+  batch_c      the C oracle's lifted root of the batched uneven-column
+               workload (per-column LDE, bit reverse, lifted tree)
 
 Run from the repo root:  python tests/golden/gen_golden.py [--big]
 (--big adds the log_n = 20, 24 and 25 (blowup 1) oracle digests, the
@@ -26,6 +28,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+sys.path.insert(0, os.path.join(HERE, "..", ".."))
 
 import pyref  # noqa: E402
 
@@ -155,12 +158,24 @@ def main():
                                  "sha256": hashlib.sha256(data).hexdigest()})
         print("oracle d2 digest", log_n, pw, len(data), flush=True)
 
+    # batched uneven commitment: 96 columns, sizes 2^14..2^22 from the seed,
+    # blowup 4, bit reversed, lifted tree (C oracle)
+    import vortexstark_amd as vx_sizes  # only the size schedule (pure Python)
+    g["batch_c"] = []
+    for n_cols, sd in [(12, 7)] + ([(96, seed)] if big else []):
+        logs = vx_sizes.batch_log_sizes(n_cols, sd)
+        root = pyoracle.batch_root(logs, sd, 2, True)
+        g["batch_c"].append({"n_cols": n_cols, "seed": sd, "log_sizes": logs, "blowup_bits": 2,
+                             "bit_reverse": True, "root": [int(x) for x in root]})
+        print("oracle batch root", n_cols, flush=True)
+
     out = os.path.join(HERE, "golden.json")
     if not big and os.path.exists(out):
         old = json.load(open(out))
         keep = [p for p in old.get("proofs_c", []) if p["log_n"] >= 20]
         g["proofs_c"] += keep
         g["proofs_d2_c"] += [p for p in old.get("proofs_d2_c", []) if p["log_n"] >= 20]
+        g["batch_c"] += [p for p in old.get("batch_c", []) if p["n_cols"] >= 96]
     with open(out, "w") as f:
         json.dump(g, f, separators=(",", ":"))
     print("wrote", out, os.path.getsize(out), "bytes")

@@ -165,6 +165,16 @@ def merkle_layers(leaves_words):
     return layers
 
 
+def merkle_layers_from_hashes(leaf_hashes):
+    """Tree over precomputed 32-byte leaf hashes (node = Blake2s(left || right))."""
+    layer = list(leaf_hashes)
+    layers = [layer]
+    while len(layer) > 1:
+        layer = [b2(layer[2 * i] + layer[2 * i + 1]) for i in range(len(layer) // 2)]
+        layers.append(layer)
+    return layers
+
+
 def auth_path(layers, idx):
     path = []
     for layer in layers[:-1]:

@@ -1,0 +1,118 @@
+"""Batched uneven columns on the GPU: the work-list LDE, the batched bit
+reverse and the lifted Merkle tree against the oracle, column by column and
+root by root; the resident batch against the committed workload digest."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import vortexstark_amd as vx
+from gpu_util import d2h, dev, empty, host, sync
+
+pytestmark = pytest.mark.gpu
+
+
+def ptrs(bufs):
+    return (C.c_void_p * len(bufs))(*[b.ptr for b in bufs])
+
+
+def u32s(xs):
+    return (C.c_uint32 * len(xs))(*xs)
+
+
+def bitrev(a):
+    n = a.size
+    bits = n.bit_length() - 1
+    idx = np.arange(n, dtype=np.uint64)
+    r = np.zeros(n, dtype=np.uint64)
+    for b in range(bits):
+        r |= ((idx >> np.uint64(b)) & np.uint64(1)) << np.uint64(bits - 1 - b)
+    out = np.empty_like(a)
+    out[r.astype(np.int64)] = a
+    return out
+
+
+def test_lde_batch_matches_oracle_per_column():
+    rng = np.random.default_rng(11)
+    logs = [14, 16, 14, 15, 17, 14]   # uneven, unsorted on purpose (no ordering needed here)
+    bb = 2
+    vals = [rng.integers(0, po.P, size=1 << L, dtype=np.uint32) for L in logs]
+    ins = [dev(v) for v in vals]
+    coefs = [empty(1 << L) for L in logs]
+    evals = [empty(1 << (L + bb)) for L in logs]
+    vx._lib.call("vx_circle_ntt_lde_batch", ptrs(ins), ptrs(coefs), ptrs(evals), u32s(logs), len(logs), bb, None)
+    for i, L in enumerate(logs):
+        print("column", i, L, flush=True)
+        want_c = po.interpolate(vals[i], po.coset(po.HALF_COSET, L))
+        assert np.array_equal(host(coefs[i]), want_c)
+        assert np.array_equal(host(evals[i]), po.lde(vals[i], bb))
+        assert np.array_equal(host(ins[i]), vals[i])  # inputs kept intact
+
+
+def test_bit_reverse_batch():
+    rng = np.random.default_rng(12)
+    logs = [10, 13, 11, 16]
+    vals = [rng.integers(0, po.P, size=1 << L, dtype=np.uint32) for L in logs]
+    src = [dev(v) for v in vals]
+    dst = [empty(1 << L) for L in logs]
+    vx._lib.call("vx_bit_reverse_m31_batch", ptrs(src), ptrs(dst), u32s(logs), len(logs), None)
+    for v, d in zip(vals, dst):
+        assert np.array_equal(host(d), bitrev(v))
+
+
+@pytest.mark.parametrize("logs", [[4], [3, 3, 4, 6], [2] * 7 + [4] * 9 + [6] * 16, [5, 6, 8, 8, 9, 12, 12, 13]])
+def test_lifted_leaves_and_root(logs):
+    rng = np.random.default_rng(len(logs))
+    cols = [rng.integers(0, po.P, size=1 << L, dtype=np.uint32) for L in logs]
+    bufs = [dev(c) for c in cols]
+    S = logs[-1]
+    out = empty(8 << S)
+    vx._lib.call("vx_build_leaves_lifted", ptrs(bufs), u32s(logs), len(logs), S, out.ptr, None)
+    assert np.array_equal(host(out).reshape(-1, 8), po.leaves_lifted(cols, S))
+    if S >= 4:
+        root = empty(8)
+        vx._lib.call("vx_merkle_root_lifted", ptrs(bufs), u32s(logs), len(logs), root.ptr, None)
+        assert np.array_equal(host(root), po.merkle_root_lifted(cols))
+
+
+def test_resident_batch_matches_oracle(golden):
+    g = next(x for x in golden["batch_c"] if x["n_cols"] == 12)
+    b = vx.Batch(g["log_sizes"], g["blowup_bits"], g["bit_reverse"])
+    b.fill_random(g["seed"])
+    for c, L in enumerate(g["log_sizes"][:3]):
+        ptr, words = b.column(c, 0)
+        assert np.array_equal(d2h(ptr, words), po.d2_column(g["seed"], c, L))
+    root = b.run()
+    assert root == g["root"]
+    # every output column equals the oracle's bit-reversed LDE
+    for c in (0, 5, 11):
+        L = g["log_sizes"][c]
+        ptr, words = b.column(c, 2)
+        e = po.lde(po.d2_column(g["seed"], c, L), g["blowup_bits"])
+        po.lib().or_bit_reverse_m31(e, L + g["blowup_bits"])
+        assert np.array_equal(d2h(ptr, words), e)
+    # a second run gives the same root (inputs are not consumed)
+    assert b.run() == root
+    b.close()
+
+
+def test_batched_workload_digest(golden):
+    g = next((x for x in golden["batch_c"] if x["n_cols"] == 96), None)
+    if g is None:
+        pytest.skip("96-column digest not in golden.json (gen_golden.py --big)")
+    b = vx.Batch(g["log_sizes"], g["blowup_bits"], g["bit_reverse"])
+    b.fill_random(g["seed"])
+    assert b.run(timed=True) == g["root"]
+    t = b.stage_times()
+    print("batch stage ms", t, flush=True)
+    assert t["total"] > 0
+    b.close()
+
+
+def test_batch_rejects_bad_shapes():
+    with pytest.raises(vx.VortexError, match="ascending"):
+        vx.Batch([15, 14])
+    with pytest.raises(vx.VortexError, match="14..24"):
+        vx.Batch([12, 14])
+    sync()

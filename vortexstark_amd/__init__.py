@@ -108,6 +108,84 @@ class D2Prover(Prover):
         _lib.call("vx_prover_enqueue", self._h, seed & 0xFFFFFFFF, seed >> 32, 1 if timed else 0)
 
 
+def _u32_array(xs):
+    return (C.c_uint32 * len(xs))(*xs)
+
+
+def _ptr_array(ptrs):
+    return (C.c_void_p * len(ptrs))(*ptrs)
+
+
+class Batch:
+    """Resident batch of M31 columns of uneven log sizes (sorted ascending,
+    14..24): one run = work-list LDE at 2^blowup_bits, bit reverse, lifted
+    Blake2s commitment (vx_batch_*)."""
+
+    def __init__(self, log_sizes, blowup_bits=BLOWUP_BITS, bit_reverse=True, stream=None):
+        self.log_sizes = list(log_sizes)
+        self._h = C.c_void_p()
+        _lib.call("vx_batch_create", _u32_array(self.log_sizes), len(self.log_sizes), blowup_bits,
+                  1 if bit_reverse else 0, stream, C.byref(self._h))
+
+    def column(self, c, which):
+        """(device pointer, words): which 0 input, 1 coefficients, 2 evaluation."""
+        ptr, words = C.c_void_p(), C.c_uint64()
+        _lib.call("vx_batch_column", self._h, c, which, C.byref(ptr), C.byref(words))
+        return ptr.value, words.value
+
+    def fill_random(self, seed):
+        """Input column c = the synthetic generator's column c for `seed`."""
+        for c, L in enumerate(self.log_sizes):
+            _lib.call("vx_m31_fill_random", self.column(c, 0)[0], L, seed, c, None)
+
+    def run(self, timed=False):
+        self.enqueue(timed)
+        return self.wait()
+
+    def enqueue(self, timed=False):
+        _lib.call("vx_batch_enqueue", self._h, 1 if timed else 0)
+
+    def wait(self):
+        root = (C.c_uint32 * 8)()
+        _lib.call("vx_batch_wait", self._h, root)
+        return list(root)
+
+    def stage_times(self):
+        out = (C.c_float * 4)()
+        _lib.call("vx_batch_stage_times", self._h, out)
+        return dict(zip(("lde", "bit_reverse", "commit", "total"), list(out)))
+
+    @property
+    def device_bytes(self):
+        return int(load().vx_batch_device_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            _lib.call("vx_batch_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_log_sizes(n_cols, seed, lo=14, hi=22):
+    """The batched-commit workload's column sizes: SplitMix64(seed) outputs
+    mod (hi - lo + 1) + lo, sorted ascending."""
+    M = (1 << 64) - 1
+    s, out = seed, []
+    for _ in range(n_cols):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out.append(lo + z % (hi - lo + 1))
+    return sorted(out)
+
+
 def prove(a, b, log_n, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0):
     """prover::prove (src/prover.rs:221): one-shot proof bytes."""
     size = proof_size(log_n, blowup_bits, n_queries, pow_bits)

@@ -73,6 +73,18 @@ SIGNATURES = {
     "vx_prover_set_graphs": (ip, [vp, ip]),
     "vx_d2_prover_create": (ip, [u32, u32, u32, u64, u32, u32, u32, vp, C.POINTER(vp)]),
     "vx_d2_proof_size": (sz, [u32, u32, u32, u32]),
+    "vx_m31_fill_random": (ip, [vp, u32, u64, u32, vp]),
+    "vx_circle_ntt_lde_batch": (ip, [vp, vp, vp, vp, u32, u32, vp]),
+    "vx_bit_reverse_m31_batch": (ip, [vp, vp, vp, u32, vp]),
+    "vx_build_leaves_lifted": (ip, [vp, vp, u32, u32, vp, vp]),
+    "vx_merkle_root_lifted": (ip, [vp, vp, u32, vp, vp]),
+    "vx_batch_create": (ip, [vp, u32, u32, ip, vp, C.POINTER(vp)]),
+    "vx_batch_destroy": (ip, [vp]),
+    "vx_batch_column": (ip, [vp, u32, ip, C.POINTER(vp), C.POINTER(u64)]),
+    "vx_batch_enqueue": (ip, [vp, ip]),
+    "vx_batch_wait": (ip, [vp, vp]),
+    "vx_batch_stage_times": (ip, [vp, C.POINTER(C.c_float)]),
+    "vx_batch_device_bytes": (u64, [vp]),
     "vx_server_create": (ip, [u32, u64, ip, C.POINTER(vp)]),
     "vx_server_destroy": (ip, [vp]),
     "vx_server_prepare": (ip, [vp, vp]),

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "prover.h"
+#include "batch.h"
 #include "server.h"
 #include "vx_d2.h"
 
@@ -535,6 +536,167 @@ int vx_prover_set_graphs(vx_prover* p, int on) {
   return guard([&] {
     require(p != nullptr, "null prover");
     p->impl.set_graphs(on != 0);
+  });
+}
+
+// ------------------------------------------------------- batched columns
+struct vx_batch {
+  BatchCommitter impl;
+  vx_batch(const uint32_t* l, uint32_t n, uint32_t b, bool br, hipStream_t s) : impl(l, n, b, br, s) {}
+};
+
+int vx_batch_create(const uint32_t* h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, int bit_reverse,
+                    vx_stream_t st, vx_batch** out) {
+  return guard([&] {
+    require(out != nullptr, "batch_create: null out");
+    *out = nullptr;
+    *out = new vx_batch(h_log_sizes, n_cols, blowup_bits, bit_reverse != 0, S(st));
+  });
+}
+
+int vx_batch_destroy(vx_batch* b) {
+  return guard([&] { delete b; });
+}
+
+int vx_batch_column(const vx_batch* b, uint32_t col, int which, uint32_t** d_ptr, uint64_t* words) {
+  return guard([&] {
+    require(b && d_ptr && words, "null argument");
+    *d_ptr = b->impl.column(col, which);
+    *words = 1ull << (b->impl.log_size(col) + (which == 2 ? b->impl.blowup() : 0));
+  });
+}
+
+int vx_batch_enqueue(vx_batch* b, int timed) {
+  return guard([&] {
+    require(b != nullptr, "null batch");
+    b->impl.enqueue(timed != 0);
+  });
+}
+
+int vx_batch_wait(vx_batch* b, uint32_t h_root[8]) {
+  return guard([&] {
+    require(b != nullptr, "null batch");
+    b->impl.wait(h_root);
+  });
+}
+
+int vx_batch_stage_times(vx_batch* b, float out[4]) {
+  return guard([&] {
+    require(b && out, "null argument");
+    b->impl.stage_times(out);
+  });
+}
+
+uint64_t vx_batch_device_bytes(const vx_batch* b) { return b ? b->impl.device_bytes() : 0; }
+
+int vx_m31_fill_random(uint32_t* d, uint32_t log_n, uint64_t seed, uint32_t col, vx_stream_t st) {
+  return guard([&] {
+    require(d != nullptr && log_n >= 2 && log_n <= 31, "fill_random: bad arguments");
+    // column `col` of the synthetic AIR trace generator with this seed
+    uint32_t* sw = nullptr;
+    uint32_t words[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    DeviceBuf dsw(16);
+    VX_HIP(hipMemcpy(dsw.p, words, 8, hipMemcpyHostToDevice));
+    sw = dsw.u32();
+    d2_trace_col(sw, d, col, log_n, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_circle_ntt_lde_batch(const uint32_t* const* h_in, uint32_t* const* h_coef, uint32_t* const* h_eval,
+                            const uint32_t* h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, vx_stream_t st) {
+  return guard([&] {
+    require(h_in && h_coef && h_eval && h_log_sizes && n_cols >= 1, "lde_batch: null argument");
+    hipStream_t s = S(st);
+    std::vector<std::shared_ptr<const ProverTables>> keep;
+    std::vector<LdeBatchCol> cols(n_cols);
+    for (uint32_t c = 0; c < n_cols; c++) {
+      require(h_log_sizes[c] >= 14 && h_log_sizes[c] <= 24 && h_log_sizes[c] + blowup_bits <= 31 &&
+                  blowup_bits >= 1,
+              "lde_batch: log sizes must be 14..24");
+      keep.push_back(ProverTables::get(h_log_sizes[c], blowup_bits, s));
+      cols[c] = {h_log_sizes[c], h_in[c], h_coef[c], h_eval[c], keep.back()->trace_itw->ref,
+                 keep.back()->eval_tw->ref};
+    }
+    std::vector<uint8_t> jobs;
+    LdeBatchStages stg{};
+    lde_batch_build(cols.data(), n_cols, blowup_bits, jobs, stg);
+    DeviceBuf dj(jobs.size());
+    VX_HIP(hipMemcpy(dj.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));
+    lde_batch_launch(static_cast<const uint8_t*>(dj.p), stg, s);
+    VX_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int vx_bit_reverse_m31_batch(const uint32_t* const* h_src, uint32_t* const* h_dst, const uint32_t* h_log_sizes,
+                             uint32_t n_cols, vx_stream_t st) {
+  return guard([&] {
+    require(h_src && h_dst && h_log_sizes && n_cols >= 1, "bit_reverse_batch: null argument");
+    std::vector<uint8_t> jobs;
+    uint32_t count = 0;
+    uint64_t tiles = 0;
+    bitrev_batch_build(h_src, h_dst, h_log_sizes, n_cols, jobs, count, tiles);
+    DeviceBuf dj(jobs.size());
+    VX_HIP(hipMemcpy(dj.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));
+    bitrev_batch_launch(static_cast<const uint8_t*>(dj.p), count, tiles, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+static LeafSrc lifted_src(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                          uint32_t lifting_log, DeviceBuf& table) {
+  require(h_cols && h_log_sizes && n_cols >= 1 && lifting_log >= 1 && lifting_log <= 31, "lifted: bad arguments");
+  std::vector<LiftCol> lc(n_cols);
+  for (uint32_t c = 0; c < n_cols; c++) {
+    require(h_cols[c] && h_log_sizes[c] >= 1 && h_log_sizes[c] <= lifting_log, "lifted: column larger than the tree");
+    require(c == 0 || h_log_sizes[c] >= h_log_sizes[c - 1], "lifted: columns must be sorted by ascending size");
+    lc[c] = {h_cols[c], h_log_sizes[c], 0};
+  }
+  table = DeviceBuf(sizeof(LiftCol) * n_cols);
+  VX_HIP(hipMemcpy(table.p, lc.data(), sizeof(LiftCol) * n_cols, hipMemcpyHostToDevice));
+  LeafSrc L{};
+  L.kind = LEAF_LIFTED;
+  L.lift = static_cast<const LiftCol*>(table.p);
+  L.lift_n = n_cols;
+  L.lift_log = lifting_log;
+  return L;
+}
+
+int vx_build_leaves_lifted(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                           uint32_t lifting_log_size, uint32_t* d_hashes, vx_stream_t st) {
+  return guard([&] {
+    require(d_hashes != nullptr, "lifted: null output");
+    DeviceBuf table;
+    LeafSrc L = lifted_src(h_cols, h_log_sizes, n_cols, lifting_log_size, table);
+    merkle_lifted_leaves(L, d_hashes, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_merkle_root_lifted(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                          uint32_t* d_root, vx_stream_t st) {
+  return guard([&] {
+    require(d_root != nullptr && h_log_sizes != nullptr && n_cols >= 1, "lifted: null argument");
+    const uint32_t lg = h_log_sizes[n_cols - 1];
+    require(lg >= 4, "lifted: the largest column needs >= 16 rows");
+    DeviceBuf table;
+    LeafSrc L = lifted_src(h_cols, h_log_sizes, n_cols, lg, table);
+    // only the levels from the tile size up are stored (the tile roots and above)
+    const uint32_t tile = lg >= 21 ? 12 : (lg < 10 ? lg : 10);
+    TreeLevels T{};
+    uint64_t w = 0;
+    T.log_n = lg;
+    T.lo = tile;
+    for (uint32_t l = T.lo; l <= lg; l++) {
+      T.off[l] = w;
+      w += 8ull << (lg - l);
+    }
+    DeviceBuf tree(4 * w);
+    T.base = tree.u32();
+    RootHook hook{};
+    hook.root_out = d_root;
+    merkle_commit(L, T, tile, hook, S(st), nullptr, nullptr);
+    VX_HIP(hipStreamSynchronize(S(st)));
   });
 }
 

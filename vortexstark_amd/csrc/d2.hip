@@ -38,6 +38,24 @@ void d2_trace(const uint32_t* seed_words, uint32_t* out, uint32_t n_cols, uint32
   VX_HIP(hipGetLastError());
 }
 
+// One column (index `col` of the generator) of 2^log_n values.
+__global__ void k_d2_trace_col(const uint32_t* seed_words, uint32_t* out, uint32_t col, uint64_t n) {
+  const uint64_t first = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * D2_ROWS_PER_LANE;
+  if (first >= n) return;
+  const uint64_t seed = (uint64_t)seed_words[0] | ((uint64_t)seed_words[1] << 32);
+#pragma unroll
+  for (uint32_t j = 0; j < D2_ROWS_PER_LANE; j++)
+    if (first + j < n) out[first + j] = d2_value(seed, col, first + j);
+}
+
+void d2_trace_col(const uint32_t* seed_words, uint32_t* out, uint32_t col, uint32_t log_n, hipStream_t s) {
+  require(log_n >= 1 && log_n <= 31, "d2_trace_col: bad log_n");
+  const uint64_t n = 1ull << log_n;
+  const uint64_t lanes = (n + D2_ROWS_PER_LANE - 1) / D2_ROWS_PER_LANE;
+  hipLaunchKernelGGL(k_d2_trace_col, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, s, seed_words, out, col, n);
+  VX_HIP(hipGetLastError());
+}
+
 // beta[k] = alpha^(k+1), k < K (one lane; K <= 1024 QM31 products).
 __global__ void k_d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons) {
   if (threadIdx.x != 0) return;

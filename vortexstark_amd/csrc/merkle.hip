@@ -195,9 +195,39 @@ __device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g) 
   for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
 }
 
+// Lifted leaf (cuda/merkle_leaves_lifted.cu:26-98): Blake2s of the words of
+// every column, in column order (ascending size), where a column of log
+// size L < S contributes its row ((idx >> (S - L + 1)) << 1) | (idx & 1).
+// 16 words per compression; the last block is the final one.
+__device__ __forceinline__ uint64_t lifted_row(uint64_t idx, uint32_t S, uint32_t L) {
+  return L == S ? idx : (((idx >> (S - L + 1)) << 1) | (idx & 1));
+}
+__device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint32_t h[8]) {
+  b2_init(h);
+  const uint32_t n = L.lift_n, S = L.lift_log;
+  uint32_t t = 0;
+  for (uint32_t base = 0; base < n; base += 16) {
+    uint32_t m[16];
+#pragma unroll
+    for (uint32_t j = 0; j < 16; j++) {
+      const uint32_t c = base + j;
+      uint32_t v = 0;
+      if (c < n) {
+        const LiftCol col = L.lift[c];
+        v = col.p[lifted_row(idx, S, col.log_n)];
+      }
+      m[j] = v;
+    }
+    t += 4 * (n - base < 16 ? n - base : 16);
+    b2_compress(h, m, t, base + 16 >= n);
+  }
+}
+
 template <uint32_t KIND, uint32_t NC>
 __device__ __forceinline__ void leaf_hash(const LeafSrc& L, const G4& g, uint64_t i, int j, uint32_t h[8]) {
-  if constexpr (KIND == LEAF_COLS && NC == 1) {
+  if constexpr (KIND == LEAF_LIFTED) {
+    lifted_leaf(L, i + j, h);
+  } else if constexpr (KIND == LEAF_COLS && NC == 1) {
     b2_leaf1(g.w[j][0], h);
   } else if constexpr (KIND == LEAF_COLS && NC == 0) {
     // generic 1..16 columns, read per leaf
@@ -220,7 +250,7 @@ template <uint32_t KIND, uint32_t NC, uint32_t LPT_LOG>
 __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, uint32_t out[8]) {
   static_assert(LPT_LOG >= 2 && LPT_LOG <= 4, "lane subtree covers 4..16 leaves");
   constexpr bool kGeneric = KIND == LEAF_COLS && NC == 0;
-  constexpr bool kDirect = kGeneric || KIND == LEAF_D2_QUOTIENT;  // no raw-load prefetch
+  constexpr bool kDirect = kGeneric || KIND == LEAF_D2_QUOTIENT || KIND == LEAF_LIFTED;  // no raw-load prefetch
   constexpr uint32_t G = 1u << (LPT_LOG - 2);
   qm31 aq{}, af{};
   if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
@@ -428,6 +458,9 @@ static void check_leaf(const LeafSrc& L, uint32_t log_n) {
       require(L.alpha_f && L.tw, "merkle: bad line-fold source");
       for (int c = 0; c < 4; c++) require(L.col[c] && L.out[c], "merkle: null fold column");
       break;
+    case LEAF_LIFTED:
+      require(L.lift && L.lift_n >= 1 && L.lift_log == log_n, "merkle: bad lifted source");
+      break;
     case LEAF_D2_QUOTIENT:
       require(L.col[0] && L.cons && L.beta && L.n_cols >= 1 && L.n_cons >= 1 &&
                   L.src_mask == (1ull << log_n) - 1 && L.col_stride >= (1ull << log_n),
@@ -451,6 +484,7 @@ static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t til
     case LEAF_FOLD_CIRCLE_Q: launch_subtree<LEAF_FOLD_CIRCLE_Q, 4>(L, T, tile_log, store_lo, hook, s); break;
     case LEAF_FOLD_LINE: launch_subtree<LEAF_FOLD_LINE, 4>(L, T, tile_log, store_lo, hook, s); break;
     case LEAF_D2_QUOTIENT: launch_subtree<LEAF_D2_QUOTIENT, 4>(L, T, tile_log, store_lo, hook, s); break;
+    case LEAF_LIFTED: launch_subtree<LEAF_LIFTED, 0>(L, T, tile_log, store_lo, hook, s); break;
   }
 }
 
@@ -689,6 +723,20 @@ void merkle_leaf_pairs(const LeafSrc& L, uint64_t n_leaves, uint32_t* parents, h
   uint64_t np = n_leaves / 2;
   if (np == 0) return;
   hipLaunchKernelGGL(k_leaf_pairs, dim3((uint32_t)((np + 255) / 256)), dim3(256), 0, s, L, np, parents);
+  VX_HIP(hipGetLastError());
+}
+
+__global__ void k_lifted_leaves(LeafSrc L, uint64_t n, uint32_t* hashes) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t h[8];
+  lifted_leaf(L, i, h);
+  store_node(hashes + 8 * i, h);
+}
+void merkle_lifted_leaves(const LeafSrc& L, uint32_t* hashes, hipStream_t s) {
+  check_leaf(L, L.lift_log);
+  const uint64_t n = 1ull << L.lift_log;
+  hipLaunchKernelGGL(k_lifted_leaves, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, L, n, hashes);
   VX_HIP(hipGetLastError());
 }
 

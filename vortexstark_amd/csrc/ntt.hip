@@ -135,15 +135,14 @@ __device__ __forceinline__ void rx_bit(uint32_t (&v)[32], const uint32_t* table,
   }
 }
 
+// One 16384-element tile of a pass; `a` may live in kernel arguments or in
+// global memory (work-list jobs), `lds` holds 2^14 words.
 template <bool FWD>
-__global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const uint32_t* src, RadixArgs a) {
-  __shared__ uint32_t lds[1u << RX_E_BITS];
-  dst += blockIdx.y * a.dst_stride;
-  src += blockIdx.y * a.src_stride;
+__device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, const RadixArgs& a, uint64_t tile,
+                                           uint32_t* lds) {
   const uint32_t t = threadIdx.x;
   const uint32_t cmask = (1u << a.cbits) - 1;
   const uint32_t low_tiles_log = a.j0 - a.cbits;
-  const uint64_t tile = blockIdx.x;
   const uint64_t low_base = (tile & ((1ull << low_tiles_log) - 1)) << a.cbits;
   const uint64_t H = tile >> low_tiles_log;
   const uint64_t hi_base = H << (a.j0 + a.k);
@@ -190,6 +189,38 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const u
   }
 }
 
+template <bool FWD>
+__global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const uint32_t* src, RadixArgs a) {
+  __shared__ uint32_t lds[1u << RX_E_BITS];
+  dst += blockIdx.y * a.dst_stride;
+  src += blockIdx.y * a.src_stride;
+  radix_tile<FWD>(dst, src, a, blockIdx.x, lds);
+}
+
+// Work list: the tiles of many passes (columns of different sizes) in one
+// launch. Workgroup b runs tile b - first_tile of the last job whose
+// first_tile <= b (jobs sorted by first_tile).
+struct RadixJob {
+  uint32_t* dst;
+  const uint32_t* src;
+  uint64_t first_tile;
+  RadixArgs a;
+};
+
+template <bool FWD>
+__global__ __launch_bounds__(RX_THREADS) void k_ntt_radix_list(const RadixJob* jobs, uint32_t n_jobs) {
+  __shared__ uint32_t lds[1u << RX_E_BITS];
+  const uint64_t b = blockIdx.x;
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (jobs[mid].first_tile <= b) lo = mid;
+    else hi = mid;
+  }
+  const RadixJob& J = jobs[lo];
+  radix_tile<FWD>(J.dst, J.src, J.a, b - J.first_tile, lds);
+}
+
 static const uint32_t* bit_table(const TwiddleRef& tw, uint32_t j) {
   return j == 0 ? tw.circle : tw.line + tw.off[j - 1];
 }
@@ -198,6 +229,33 @@ struct Batch {
   uint32_t n_cols;
   uint64_t dst_stride, src_stride;
 };
+
+// Arguments of a 16384-element radix pass over bits [j0, j0 + k) with
+// 2^cbits columns per row: rounds of <= 5 row bits from the top; evaluate
+// runs them top-down, interpolate bottom-up.
+static RadixArgs make_radix_args(bool fwd, const TwiddleRef& tw, uint32_t j0, uint32_t k, uint32_t cbits,
+                                 uint64_t src_mask, uint32_t scale) {
+  require(k >= 1 && k <= 14 && k + cbits == RX_E_BITS && cbits <= j0 && j0 + k <= tw.log_n,
+          "ntt: invalid radix pass shape");
+  RadixArgs r{};
+  for (uint32_t b = 0; b < k; b++) r.tw[b] = bit_table(tw, j0 + b);
+  r.j0 = j0;
+  r.k = k;
+  r.cbits = cbits;
+  r.src_mask = src_mask;
+  r.scale = scale;
+  RadixRound rr[4];
+  uint32_t nr = 0;
+  for (int32_t hi = (int32_t)k; hi > 0; hi -= 5) {
+    uint32_t rb0 = hi >= 5 ? (uint32_t)hi - 5 : 0;
+    uint32_t nb = (uint32_t)hi - rb0;
+    uint32_t pos = rb0 + cbits < 9 ? rb0 + cbits : 9;
+    rr[nr++] = {rb0, nb, pos};
+  }
+  r.n_rounds = nr;
+  for (uint32_t i = 0; i < nr; i++) r.round[i] = fwd ? rr[i] : rr[nr - 1 - i];
+  return r;
+}
 
 static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_mask,
                         const TwiddleRef& tw, uint32_t j0, uint32_t k, uint32_t cbits, uint32_t scale,
@@ -219,27 +277,9 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
   const uint32_t e_log = k + cbits;
   const uint64_t grid = 1ull << (log_n - e_log);
   if (e_log == RX_E_BITS) {
-    RadixArgs r{};
-    for (uint32_t b = 0; b < k; b++) r.tw[b] = a.tw[b];
-    r.j0 = j0;
-    r.k = k;
-    r.cbits = cbits;
-    r.src_mask = src_mask;
-    r.scale = scale;
+    RadixArgs r = make_radix_args(fwd, tw, j0, k, cbits, src_mask, scale);
     r.dst_stride = bt.dst_stride;
     r.src_stride = bt.src_stride;
-    // rounds of <= 5 row bits from the top; evaluate runs them top-down,
-    // interpolate bottom-up
-    RadixRound rr[4];
-    uint32_t nr = 0;
-    for (int32_t hi = (int32_t)k; hi > 0; hi -= 5) {
-      uint32_t rb0 = hi >= 5 ? (uint32_t)hi - 5 : 0;
-      uint32_t nb = (uint32_t)hi - rb0;
-      uint32_t pos = rb0 + cbits < 9 ? rb0 + cbits : 9;
-      rr[nr++] = {rb0, nb, pos};
-    }
-    r.n_rounds = nr;
-    for (uint32_t i = 0; i < nr; i++) r.round[i] = fwd ? rr[i] : rr[nr - 1 - i];
     if (fwd)
       hipLaunchKernelGGL(k_ntt_radix<true>, dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0, s, dst, src, r);
     else
@@ -304,6 +344,140 @@ void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n
     launch_pass(false, data, data, mask, tw, bot, k, NTT_TILE_LOG - k, last ? inv_n : 1u, bt, s);
     bot += k;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Uneven batched LDE: columns of log sizes 14..24 (interpolate on their trace
+// domain, evaluate on their 2^blowup-times larger eval domain) in four
+// work-list launches, whatever the number and sizes of the columns:
+//   0 inverse contiguous 14-bit pass (in place on the coefficients)
+//   1 inverse strided pass over bits [14, L) (columns with L > 14), 1/n
+//   2 forward strided pass over bits [14, L) of the evaluation, reading
+//     coefficient i & (n - 1) (the replication layers are never executed)
+//   3 forward contiguous 14-bit pass
+// The per-column passes are exactly those of ntt_inverse / ntt_forward, so
+// every column's result equals the single-column transform.
+size_t lde_batch_job_bytes() { return sizeof(RadixJob); }
+
+void lde_batch_build(const LdeBatchCol* cols, uint32_t n, uint32_t blowup_bits, std::vector<uint8_t>& out,
+                     LdeBatchStages& st) {
+  std::vector<RadixJob> stage[4];
+  for (uint32_t c = 0; c < n; c++) {
+    const LdeBatchCol& C = cols[c];
+    const uint32_t L = C.log_n, E = L + blowup_bits;
+    require(L >= RX_E_BITS && L <= 24 && E <= 31 && C.itw.log_n == L && C.etw.log_n == E && C.coef && C.eval,
+            "lde_batch: columns need 14 <= log_n <= 24 and matching twiddles");
+    const uint32_t e = (30u * L) % 31u;
+    const uint32_t inv_n = e == 0 ? 1u : (1u << e);
+    const uint64_t nmask = (1ull << L) - 1, Emask = (1ull << E) - 1;
+    const uint32_t sb = L - RX_E_BITS;  // strided bits
+    stage[0].push_back({C.coef, C.in ? C.in : C.coef, 0,
+                        make_radix_args(false, C.itw, 0, RX_E_BITS, 0, nmask, sb == 0 ? inv_n : 1u)});
+    if (sb) {
+      stage[1].push_back({C.coef, C.coef, 0, make_radix_args(false, C.itw, RX_E_BITS, sb, RX_E_BITS - sb, nmask, inv_n)});
+      stage[2].push_back({C.eval, C.coef, 0, make_radix_args(true, C.etw, RX_E_BITS, sb, RX_E_BITS - sb, nmask, 1u)});
+      stage[3].push_back({C.eval, C.eval, 0, make_radix_args(true, C.etw, 0, RX_E_BITS, 0, Emask, 1u)});
+    } else {
+      stage[3].push_back({C.eval, C.coef, 0, make_radix_args(true, C.etw, 0, RX_E_BITS, 0, nmask, 1u)});
+    }
+  }
+  // tile counts: stages 0, 1 run on the 2^L-element transform, 2, 3 on 2^E
+  out.clear();
+  for (int k = 0; k < 4; k++) {
+    uint64_t tiles = 0;
+    uint32_t idx = 0;
+    for (uint32_t c = 0; c < n; c++) {
+      const uint32_t L = cols[c].log_n, E = L + blowup_bits;
+      if (!(k == 0 || k == 3 || L > RX_E_BITS)) continue;
+      RadixJob& j = stage[k][idx++];
+      j.first_tile = tiles;
+      tiles += 1ull << ((k < 2 ? L : E) - RX_E_BITS);
+    }
+    st.count[k] = (uint32_t)stage[k].size();
+    st.tiles[k] = tiles;
+    st.offset[k] = out.size();
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(stage[k].data());
+    out.insert(out.end(), p, p + stage[k].size() * sizeof(RadixJob));
+  }
+}
+
+void lde_batch_launch(const uint8_t* d_jobs, const LdeBatchStages& st, hipStream_t s) {
+  for (int k = 0; k < 4; k++) {
+    if (!st.count[k]) continue;
+    require(st.tiles[k] < (1ull << 31), "lde_batch: too many tiles");
+    const RadixJob* jobs = reinterpret_cast<const RadixJob*>(d_jobs + st.offset[k]);
+    if (k >= 2)
+      hipLaunchKernelGGL(k_ntt_radix_list<true>, dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s, jobs,
+                         st.count[k]);
+    else
+      hipLaunchKernelGGL(k_ntt_radix_list<false>, dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s, jobs,
+                         st.count[k]);
+    VX_HIP(hipGetLastError());
+  }
+}
+
+// Out-of-place batched bit reverse, dst[rev(i)] = src[i], for columns of
+// log size >= 10: a workgroup moves a 32 x 32 tile (the 1024 indices with
+// one middle-bit value m) through LDS, reading 32 runs of 32 consecutive
+// words and writing 32 runs of 32 consecutive words (rev(a, m, b) =
+// (rev5 b, rev m, rev5 a)).
+struct BitrevJob {
+  const uint32_t* src;
+  uint32_t* dst;
+  uint32_t log_n;
+  uint64_t first_tile;
+};
+static constexpr uint32_t BR_T = 5;
+
+__global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint32_t n_jobs) {
+  __shared__ uint32_t tile[32][33];
+  const uint64_t b = blockIdx.x;
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (jobs[mid].first_tile <= b) lo = mid;
+    else hi = mid;
+  }
+  const BitrevJob& J = jobs[lo];
+  const uint32_t E = J.log_n, mb = E - 2 * BR_T;
+  const uint64_t m = b - J.first_tile;
+  const uint64_t rm = mb ? (__brevll(m) >> (64 - mb)) : 0;
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (uint32_t r = 0; r < 4; r++) {
+    const uint32_t a = (t >> 5) + 8 * r, bb = t & 31;
+    tile[a][bb] = J.src[((uint64_t)a << (E - BR_T)) | (m << BR_T) | bb];
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < 4; r++) {
+    const uint32_t bb = (t >> 5) + 8 * r, a2 = t & 31;  // a2 = rev5(a)
+    const uint32_t a = __brev(a2) >> (32 - BR_T), rb = __brev(bb) >> (32 - BR_T);
+    J.dst[((uint64_t)rb << (E - BR_T)) | (rm << BR_T) | a2] = tile[a][bb];
+  }
+}
+
+void bitrev_batch_build(const uint32_t* const* src, uint32_t* const* dst, const uint32_t* log_n, uint32_t n,
+                        std::vector<uint8_t>& out, uint32_t& count, uint64_t& tiles) {
+  std::vector<BitrevJob> jobs;
+  tiles = 0;
+  for (uint32_t c = 0; c < n; c++) {
+    require(log_n[c] >= 2 * BR_T && log_n[c] <= 31 && src[c] && dst[c] && src[c] != dst[c],
+            "bitrev_batch: out-of-place columns of log size >= 10");
+    jobs.push_back({src[c], dst[c], log_n[c], tiles});
+    tiles += 1ull << (log_n[c] - 2 * BR_T);
+  }
+  count = n;
+  out.assign(reinterpret_cast<const uint8_t*>(jobs.data()),
+             reinterpret_cast<const uint8_t*>(jobs.data()) + jobs.size() * sizeof(BitrevJob));
+}
+
+void bitrev_batch_launch(const uint8_t* d_jobs, uint32_t count, uint64_t tiles, hipStream_t s) {
+  if (!count) return;
+  require(tiles < (1ull << 31), "bitrev_batch: too many tiles");
+  hipLaunchKernelGGL(k_bitrev_list, dim3((uint32_t)tiles), dim3(256), 0, s, reinterpret_cast<const BitrevJob*>(d_jobs),
+                     count);
+  VX_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

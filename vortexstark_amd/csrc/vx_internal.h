@@ -6,6 +6,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "vx_channel.h"
 #include "vx_circle.h"
@@ -64,6 +65,29 @@ void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const T
                  uint32_t eff_bits, hipStream_t s, uint32_t n_cols = 1, uint64_t dst_stride = 0,
                  uint64_t src_stride = 0);
 void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n_cols = 1, uint64_t stride = 0);
+// Uneven batched LDE (columns of log sizes 14..24): host-built work lists
+// (lde_batch_build) uploaded by the caller, then four launches.
+struct LdeBatchCol {
+  uint32_t log_n;
+  const uint32_t* in;  // 2^log_n trace values (null: read from coef, in place)
+  uint32_t* coef;   // out: coefficients
+  uint32_t* eval;   // out: 2^(log_n + blowup) evaluation
+  TwiddleRef itw;   // inverse tables of half_coset(log_n)
+  TwiddleRef etw;   // forward tables of the eval domain
+};
+struct LdeBatchStages {
+  uint32_t count[4];
+  uint64_t tiles[4];
+  uint64_t offset[4];  // byte offset of each stage's jobs
+};
+size_t lde_batch_job_bytes();
+void lde_batch_build(const LdeBatchCol* cols, uint32_t n, uint32_t blowup_bits, std::vector<uint8_t>& jobs,
+                     LdeBatchStages& st);
+void lde_batch_launch(const uint8_t* d_jobs, const LdeBatchStages& st, hipStream_t s);
+// Out-of-place batched bit reverse of columns of log size >= 10.
+void bitrev_batch_build(const uint32_t* const* src, uint32_t* const* dst, const uint32_t* log_n, uint32_t n,
+                        std::vector<uint8_t>& jobs, uint32_t& count, uint64_t& tiles);
+void bitrev_batch_launch(const uint8_t* d_jobs, uint32_t count, uint64_t tiles, hipStream_t s);
 // One reference layer (cuda/circle_ntt.cu:89-117).
 void ntt_layer(uint32_t* data, const uint32_t* tw, uint32_t layer_idx, uint64_t n, bool forward,
                hipStream_t s);
@@ -108,6 +132,13 @@ enum LeafKind : uint32_t {
   LEAF_FOLD_CIRCLE_Q = 2, // circle fold of that quotient (cuda/fri.cu:92-126), writes the layer
   LEAF_FOLD_LINE = 3,     // line fold of a QM31 SoA layer (cuda/fri.cu:58-87), writes the layer
   LEAF_D2_QUOTIENT = 4,   // synthetic AIR quotient of W eval columns (vx_d2.h), writes Q (SoA)
+  LEAF_LIFTED = 5,        // lifted multi-size leaf (cuda/merkle_leaves_lifted.cu:1-98)
+};
+// A column of a lifted tree: 2^log_n values, log_n <= the tree's log size.
+struct LiftCol {
+  const uint32_t* p;
+  uint32_t log_n;
+  uint32_t pad;
 };
 struct LeafSrc {
   uint32_t kind;
@@ -126,6 +157,10 @@ struct LeafSrc {
   uint32_t n_cons;
   const uint32_t* cons;
   const uint32_t* beta;
+  // LEAF_LIFTED: lift_n columns sorted by ascending size, tree of 2^lift_log leaves
+  const LiftCol* lift;
+  uint32_t lift_n;
+  uint32_t lift_log;
 };
 // What the kernel that produces a root does with it: copy it to root_out,
 // and, with a channel, mix it (after ch_init when `init`) and draw the next
@@ -170,6 +205,8 @@ void merkle_hash_nodes(const uint32_t* children, uint32_t* parents, uint64_t n_p
 void merkle_reduce_to_root(const uint32_t* nodes, uint32_t* root, uint32_t n_nodes, hipStream_t s);
 void merkle_root_small(const LeafSrc& leaf, uint32_t log_n, uint32_t* root_out, hipStream_t s);
 void merkle_leaf_pairs(const LeafSrc& leaf, uint64_t n_leaves, uint32_t* parents, hipStream_t s);
+// Every leaf hash of a lifted tree (cuda_build_leaves_lifted semantics).
+void merkle_lifted_leaves(const LeafSrc& leaf, uint32_t* hashes, hipStream_t s);
 
 // Decommitment of one tree at the query positions (src/prover.rs:551-618),
 // written straight into the KRAK byte buffer.
@@ -225,5 +262,6 @@ void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_
 void d2_trace(const uint32_t* seed_words, uint32_t* out, uint32_t n_cols, uint32_t log_n, uint64_t stride,
               hipStream_t s);
 void d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons, hipStream_t s);
+void d2_trace_col(const uint32_t* seed_words, uint32_t* out, uint32_t col, uint32_t log_n, hipStream_t s);
 
 }  // namespace vx
