@@ -48,6 +48,24 @@ def fib_a1(seed):
             return r
 
 
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "fib_log24_prove_kernels.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the trace-tree kernel at log_n = 24, blowup 4,
+    from the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE
+    runs of `bench.py --profile`, each scaled by what the counter reads for a
+    1 GiB device copy); None if the summary is absent."""
+    import re
+
+    try:
+        with open(PROFILE_SUMMARY) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", f.read())
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
+
+
 def golden_digest(log_n, blowup, pow_bits):
     try:
         with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
@@ -216,6 +234,29 @@ def batched(vx, steps, n_cols=96, seed=42, blowup_bits=2):
         want = next((x["root"] for x in g.get("batch_c", []) if x["n_cols"] == n_cols and x["seed"] == seed), None)
     except OSError:
         pass
+    # the whole batched-columns proof (LDE, bit reverse, lifted tree, RLC
+    # quotient, FRI, queries, decommitment), KRAK v4
+    p = vx.BatchProver(logs, blowup_bits, vx.N_QUERIES, 0)
+    proof = p.prove(seed)
+    vx._lib.call("vx_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        proof = p.prove(seed)
+    proof_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    p.enqueue(seed, timed=True)
+    p.wait()
+    proof_stages = {k: round(v, 4) for k, v in p.stage_times().items() if k != "trace_subtree_kernel"}
+    ok, why = vx.verify(proof, blowup_bits)
+    digest = hashlib.sha256(proof).hexdigest()
+    want_proof = None
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
+            g = json.load(f)
+        want_proof = next((x["sha256"] for x in g.get("proofs_batch_c", [])
+                           if x["n_cols"] == n_cols and x["seed"] == seed and x["pow_bits"] == 0), None)
+    except OSError:
+        pass
+    p.close()
     out = {
         "workload": f"{n_cols} columns, log sizes 14..22 (seed {seed}), blowup 4, bit-reversed, one lifted tree",
         "ms_per_batch": round(ms, 3),
@@ -231,6 +272,9 @@ def batched(vx, steps, n_cols=96, seed=42, blowup_bits=2):
         "commit_valu_frac": round(compressions * OPS_PER_COMPRESSION / (st["commit"] * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
         "lifted_root_matches_oracle": (root == want) if want else None,
         "device_bytes": b.device_bytes,
+        "full_proof": {"ms_per_proof": round(proof_ms, 3), "stages_ms": proof_stages, "proof_bytes": len(proof),
+                       "format": "KRAK v4 (RLC quotient + FRI, 80 queries)", "verified": ok,
+                       "bit_exact_vs_oracle": (digest == want_proof) if want_proof else None},
     }
     b.close()
     return out
@@ -345,7 +389,7 @@ def main():
             "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "TOP/s",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic() if (log_n, bb) == (24, 2) else None,
             "kernel_ms": round(k_ms, 4),
             "work": f"{compressions} Blake2s compressions x {OPS_PER_COMPRESSION} int ops",
         },

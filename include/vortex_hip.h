@@ -241,6 +241,18 @@ int vx_d2_prover_create(uint32_t log_n, uint32_t n_cols, uint32_t n_cons, uint64
                         uint32_t n_queries, uint32_t pow_bits, vx_stream_t stream, vx_prover **out);
 size_t vx_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries);
 
+/* Batched-columns proof (KRAK v4; the batched uneven-commit workload as a
+ * full proof): n_cols columns of log sizes h_log_sizes (sorted ascending,
+ * 14..24) with values from the synthetic generator, bit-reversed LDE, one
+ * lifted tree (cuda/merkle_leaves_lifted.cu), a random-linear-combination
+ * quotient (the transcript mixes Blake2s of the log sizes, then the lifted
+ * root, then draws the combination challenge), then the Fibonacci FRI / PoW
+ * / queries. Proved by vx_prover_*
+ * with (a, b) = the trace seed's low and high words; checked by vx_verify. */
+int vx_batch_prover_create(const uint32_t *h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries,
+                           uint32_t pow_bits, vx_stream_t stream, vx_prover **out);
+size_t vx_batch_proof_size(const uint32_t *h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries);
+
 /* Untimed proofs replay a HIP graph of the whole proof captured on first
  * use (default on); 0 launches the kernels one by one instead. Same bytes.
  * No counterpart in the reference (its prove_cached launches per call). */

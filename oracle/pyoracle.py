@@ -91,6 +91,10 @@ def lib(omp=False):
         "or_d2_proof_size": (sz, [u32, u32, u32, u32]),
         "or_lde": (None, [_u32p, u32, u32, _u32p]),
         "or_d2_column": (None, [u64, u32, u32, _u32p]),
+        "or_batch_prove": (C.c_int, [u64, _u32p, u32, u32, u32, u32, C.POINTER(C.POINTER(C.c_uint8)),
+                                     C.POINTER(sz)]),
+        "or_batch_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
+        "or_batch_proof_size": (sz, [_u32p, u32, u32, u32]),
         "or_build_leaves_lifted": (None, [C.c_void_p, _u32p, u32, u32, _u32p]),
         "or_merkle_root_lifted": (None, [C.c_void_p, _u32p, u32, _u32p]),
     }
@@ -403,3 +407,27 @@ def batch_root(log_sizes, seed, blowup_bits=2, bit_reverse=True, omp=True):
             lib(omp).or_bit_reverse_m31(e, L + blowup_bits)
         evs.append(e)
     return merkle_root_lifted(evs, omp)
+
+
+# ---- batched-columns proof (KRAK v4) ----
+def batch_prove(seed, log_sizes, blowup_bits=2, n_queries=80, pow_bits=0, omp=True):
+    L = lib(omp)
+    p = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    rc = L.or_batch_prove(seed, u32(log_sizes), len(log_sizes), blowup_bits, n_queries, pow_bits, C.byref(p),
+                          C.byref(n))
+    if rc != 0:
+        raise ValueError(f"or_batch_prove rejected arguments (rc={rc})")
+    data = C.string_at(p, n.value)
+    L.or_free(p)
+    return data
+
+
+def batch_verify(proof: bytes, blowup_bits=2, n_queries=80):
+    err = C.create_string_buffer(256)
+    ok = lib().or_batch_verify(proof, len(proof), blowup_bits, n_queries, err, 256)
+    return bool(ok), err.value.decode()
+
+
+def batch_proof_size(log_sizes, blowup_bits=2, n_queries=80):
+    return lib().or_batch_proof_size(u32(log_sizes), len(log_sizes), blowup_bits, n_queries)

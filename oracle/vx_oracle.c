@@ -1532,3 +1532,288 @@ void or_d2_column(uint64_t seed, uint32_t col, uint32_t log_n, uint32_t *out) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; i++) out[i] = or_d2_value(seed, col, (uint64_t)i);
 }
+
+/* ======================================================================
+ * Batched-columns proof (KRAK v4): the batched uneven-commit workload as a
+ * full proof. No reference counterpart as a whole; its pieces are the
+ * reference's: per-column LDE (src/prover.rs:280-318), bit reverse
+ * (cuda/circle_ntt.cu:234-244), the lifted tree (cuda/merkle_leaves_lifted.cu),
+ * and the Fibonacci quotient-commitment / FRI / query phase.
+ *   columns   c < C, log sizes L_c sorted ascending, t_c[i] = or_d2_value(seed, c, i)
+ *   trace     E_c = L_c + blowup, bit-reversed LDE; lifted tree of 2^S leaves, S = max E_c
+ *   quotient  Q(i) = sum_c alpha^(c+1) col_c[lifted_row(i, S, E_c)], alpha drawn
+ *             after mixing Blake2s(log sizes) and the lifted root; then
+ *             exactly the Fibonacci FRI
+ *   bytes     magic, 4, C, blowup, pow, nonce lo/hi, seed lo/hi, C log sizes,
+ *             lifted root, Q root, FRI roots, last layer, queries, trace
+ *             decommitment (n, C values at q, C values at q ^ 1, paths at q,
+ *             paths at q ^ 1), quotient and FRI decommitments as in KRAK v1.
+ * ====================================================================== */
+#define KRAK_BATCH_HEADER 9
+
+size_t or_batch_proof_size(const uint32_t *log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t nq) {
+    uint32_t S = log_sizes[n_cols - 1] + blowup_bits, n_fri = S - 3;
+    size_t words = KRAK_BATCH_HEADER + n_cols + 8 + 8 + 1 + 8 * (size_t)n_fri + 1 + 32 + 1 + nq;
+    words += 1 + 2 * (size_t)nq * n_cols + 2 * (size_t)nq * (1 + 8 * (size_t)S);
+    words += 1 + 8 * (size_t)nq + 2 * (size_t)nq * (1 + 8 * (size_t)S);
+    words += 1;
+    for (uint32_t k = 0; k < n_fri; k++) {
+        uint32_t L = S - 1 - k;
+        words += 1 + 8 * (size_t)nq + 2 * (size_t)nq * (1 + 8 * (size_t)L);
+    }
+    return 4 * words;
+}
+
+/* the column sizes enter the transcript first: Blake2s of their LE words */
+static void batch_sizes_digest(const uint32_t *log_sizes, uint32_t n_cols, uint32_t out[8]) {
+    uint8_t *buf = (uint8_t *)malloc(4 * (size_t)n_cols);
+    for (uint32_t c = 0; c < n_cols; c++) put32(buf + 4 * c, log_sizes[c]);
+    uint8_t h[32];
+    or_blake2s(buf, 4 * (size_t)n_cols, h);
+    for (int k = 0; k < 8; k++) out[k] = le32(h + 4 * k);
+    free(buf);
+}
+
+int or_batch_prove(uint64_t seed, const uint32_t *log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t nq,
+                   uint32_t pow_bits, uint8_t **out, size_t *out_len) {
+    if (n_cols < 1 || n_cols > 4096 || blowup_bits < 1 || nq == 0) return -1;
+    for (uint32_t c = 0; c < n_cols; c++)
+        if (log_sizes[c] < 2 || log_sizes[c] + blowup_bits > 31 || (c && log_sizes[c] < log_sizes[c - 1]))
+            return -1;
+    const uint32_t S = log_sizes[n_cols - 1] + blowup_bits;
+    if (S < 4) return -1;
+    const uint64_t N = 1ull << S;
+    uint32_t **cols = (uint32_t **)calloc(n_cols, sizeof(uint32_t *));
+    uint32_t *elog = (uint32_t *)malloc(4 * (size_t)n_cols);
+    for (uint32_t c = 0; c < n_cols; c++) {
+        uint32_t L = log_sizes[c];
+        elog[c] = L + blowup_bits;
+        cols[c] = (uint32_t *)malloc(4ull << elog[c]);
+        or_d2_column(seed, c, L, cols[c]);
+        lde_column(cols[c], L, elog[c]);
+        or_bit_reverse_m31(cols[c], elog[c]);
+    }
+    /* lifted tree */
+    uint32_t *leaves = (uint32_t *)malloc(N * 32);
+    or_build_leaves_lifted((const uint32_t *const *)cols, elog, n_cols, S, leaves);
+    tree_t tt;
+    tree_build_from_leaves(&tt, S, leaves);
+    or_channel ch;
+    or_channel_init(&ch);
+    uint32_t sd[8];
+    batch_sizes_digest(log_sizes, n_cols, sd);
+    or_channel_mix_digest(&ch, sd);
+    or_channel_mix_digest(&ch, tree_root(&tt));
+    uint32_t alpha[4];
+    or_channel_draw_felt(&ch, alpha);
+    /* RLC quotient */
+    qm31 *beta = (qm31 *)malloc(sizeof(qm31) * n_cols);
+    d2_powers(alpha, n_cols, beta);
+    uint32_t *q[4];
+    for (int k = 0; k < 4; k++) q[k] = (uint32_t *)malloc(N * 4);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)N; i++) {
+        qm31 acc = {{0, 0, 0, 0}};
+        for (uint32_t c = 0; c < n_cols; c++)
+            acc = q_add(acc, q_scale(beta[c], cols[c][lifted_row((uint64_t)i, S, elog[c])]));
+        for (int k = 0; k < 4; k++) q[k][i] = acc.v[k];
+    }
+    tree_t qtree;
+    tree_commit(&qtree, (const uint32_t *const *)q, 4, S);
+    or_channel_mix_digest(&ch, tree_root(&qtree));
+    fri_run_t fr;
+    fri_commit_and_query(&ch, q, S, nq, pow_bits, &fr);
+
+    wbuf w = {0};
+    wb_u32(&w, KRAK_MAGIC);
+    wb_u32(&w, 4);
+    wb_u32(&w, n_cols);
+    wb_u32(&w, blowup_bits);
+    wb_u32(&w, pow_bits);
+    wb_u32(&w, (uint32_t)fr.nonce);
+    wb_u32(&w, (uint32_t)(fr.nonce >> 32));
+    wb_u32(&w, (uint32_t)seed);
+    wb_u32(&w, (uint32_t)(seed >> 32));
+    wb_words(&w, log_sizes, n_cols);
+    wb_words(&w, tree_root(&tt), 8);
+    wb_words(&w, tree_root(&qtree), 8);
+    fri_write_roots_and_queries(&w, &fr);
+    wb_u32(&w, nq);
+    for (int side = 0; side < 2; side++)
+        for (uint32_t i = 0; i < nq; i++) {
+            uint64_t idx = fr.qi[i] ^ (uint64_t)side;
+            for (uint32_t c = 0; c < n_cols; c++) wb_u32(&w, cols[c][lifted_row(idx, S, elog[c])]);
+        }
+    for (uint32_t i = 0; i < nq; i++) wb_path(&w, &tt, fr.qi[i]);
+    for (uint32_t i = 0; i < nq; i++) wb_path(&w, &tt, fr.qi[i] ^ 1);
+    wb_decommit(&w, &qtree, (const uint32_t *const *)q, 4, fr.qi, nq);
+    fri_write_decommits(&w, &fr);
+
+    tree_free(&tt);
+    tree_free(&qtree);
+    fri_free(&fr);
+    for (int k = 0; k < 4; k++) free(q[k]);
+    for (uint32_t c = 0; c < n_cols; c++) free(cols[c]);
+    free(cols);
+    free(elog);
+    free(beta);
+    *out = w.p;
+    *out_len = w.len;
+    return 0;
+}
+
+static void lifted_hash_words(const uint32_t *vals, uint32_t n, uint32_t out[8]) {
+    uint8_t *buf = (uint8_t *)malloc(4 * (size_t)n + 4);
+    for (uint32_t c = 0; c < n; c++) put32(buf + 4 * c, vals[c]);
+    uint8_t h[32];
+    or_blake2s(buf, 4 * (size_t)n, h);
+    for (int k = 0; k < 8; k++) out[k] = le32(h + 4 * k);
+    free(buf);
+}
+
+int or_batch_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits, uint32_t nq, char *err, size_t err_cap) {
+    int ok = 0;
+    rbuf r = {bytes, len, 0, 0};
+    decom_t qd = {0}, td = {0}, *fd = NULL;
+    uint32_t n_fd = 0, n_fri = 0, n_last = 0, n_q = 0, C = 0;
+    uint32_t *logs = NULL, (*froots)[8] = NULL, *lastw = NULL;
+    uint64_t *qidx = NULL, *fidx = NULL;
+    qm31 *alphas = NULL, *beta = NULL;
+    char dummy[8];
+    if (!err || err_cap == 0) { err = dummy; err_cap = sizeof dummy; }
+    err[0] = 0;
+
+    if (rb_u32(&r) != KRAK_MAGIC) VFAIL("bad magic");
+    if (rb_u32(&r) != 4) VFAIL("unsupported proof version");
+    C = rb_u32(&r);
+    uint32_t bb = rb_u32(&r), pow_bits = rb_u32(&r);
+    uint64_t nonce = rb_u32(&r);
+    nonce |= (uint64_t)rb_u32(&r) << 32;
+    (void)rb_u32(&r);
+    (void)rb_u32(&r);
+    if (r.bad || C < 1 || C > 4096) VFAIL("bad column count");
+    if (bb != blowup_bits) VFAIL("blowup mismatch");
+    if (pow_bits > 128) VFAIL("bad pow bits");
+    logs = (uint32_t *)calloc(C, 4);
+    for (uint32_t c = 0; c < C; c++) {
+        logs[c] = rb_u32(&r);
+        if (r.bad || logs[c] < 1 || logs[c] + bb > 31 || (c && logs[c] < logs[c - 1])) VFAIL("bad column sizes");
+    }
+    uint32_t S = logs[C - 1] + bb;
+    uint32_t troot[8], qroot[8];
+    for (int i = 0; i < 8; i++) troot[i] = rb_u32(&r);
+    for (int i = 0; i < 8; i++) qroot[i] = rb_u32(&r);
+    n_fri = rb_u32(&r);
+    if (r.bad || n_fri > 64) VFAIL("truncated proof");
+    froots = (uint32_t(*)[8])calloc(n_fri + 1, 32);
+    for (uint32_t k = 0; k < n_fri; k++)
+        for (int i = 0; i < 8; i++) froots[k][i] = rb_u32(&r);
+    n_last = rb_u32(&r);
+    if (r.bad || n_last > 4096) VFAIL("truncated proof");
+    lastw = (uint32_t *)calloc(4 * (size_t)n_last + 1, 4);
+    for (size_t i = 0; i < 4 * (size_t)n_last; i++) lastw[i] = rb_u32(&r);
+    n_q = rb_u32(&r);
+    if (r.bad || n_q > (1u << 20)) VFAIL("truncated proof");
+    qidx = (uint64_t *)calloc(n_q + 1, 8);
+    for (uint32_t i = 0; i < n_q; i++) qidx[i] = rb_u32(&r);
+    if (!rb_decom(&r, &td, C)) VFAIL("truncated proof");
+    if (!rb_decom(&r, &qd, 4)) VFAIL("truncated proof");
+    n_fd = rb_u32(&r);
+    if (r.bad || n_fd > 64) VFAIL("truncated proof");
+    fd = (decom_t *)calloc(n_fd + 1, sizeof(decom_t));
+    for (uint32_t k = 0; k < n_fd; k++)
+        if (!rb_decom(&r, &fd[k], 4)) VFAIL("truncated proof");
+    if (r.pos != r.len) VFAIL("trailing bytes");
+
+    uint64_t N = 1ull << S;
+    if (n_q != nq) VFAIL("Expected %u queries, got %u", nq, n_q);
+    or_channel ch;
+    or_channel_init(&ch);
+    uint32_t sd[8];
+    batch_sizes_digest(logs, C, sd);
+    or_channel_mix_digest(&ch, sd);
+    or_channel_mix_digest(&ch, troot);
+    uint32_t alpha[4];
+    or_channel_draw_felt(&ch, alpha);
+    or_channel_mix_digest(&ch, qroot);
+    alphas = (qm31 *)calloc(n_fri + 2, sizeof(qm31));
+    or_channel_draw_felt(&ch, alphas[0].v);
+    for (uint32_t k = 0; k < n_fri; k++) {
+        or_channel_mix_digest(&ch, froots[k]);
+        or_channel_draw_felt(&ch, alphas[k + 1].v);
+    }
+    if (n_fri != S - 3) VFAIL("Expected %u FRI layers, got %u", S - 3, n_fri);
+    if (n_last != 8) VFAIL("Expected 8 FRI last layer values, got %u", n_last);
+    or_channel_mix_felts(&ch, lastw, n_last);
+    if (pow_bits) {
+        if (!or_channel_verify_pow_nonce(&ch, pow_bits, nonce)) VFAIL("PoW nonce invalid");
+        or_channel_mix_u64(&ch, nonce);
+    } else if (nonce != 0) {
+        VFAIL("nonce without PoW");
+    }
+    for (uint32_t i = 0; i < nq; i++)
+        if (or_channel_draw_number(&ch, N) != qidx[i]) VFAIL("Query indices don't match Fiat-Shamir derivation");
+    /* lifted trace openings (leaf = Blake2s of all C values) */
+    if (td.n != nq) VFAIL("trace decommitment size mismatch");
+    for (uint32_t q = 0; q < nq; q++) {
+        uint32_t h[8];
+        lifted_hash_words(td.vals + (size_t)q * C, C, h);
+        if (td.depth[q] != S || !or_verify_auth_path(troot, h, qidx[q], td.paths[q], td.depth[q]))
+            VFAIL("trace auth path invalid");
+        lifted_hash_words(td.sibs + (size_t)q * C, C, h);
+        if (td.sdepth[q] != S || !or_verify_auth_path(troot, h, qidx[q] ^ 1, td.spaths[q], td.sdepth[q]))
+            VFAIL("trace auth path invalid");
+    }
+    if (!check_decom(&qd, qroot, qidx, nq)) VFAIL("quotient auth path invalid");
+    /* the random linear combination at q and q ^ 1 */
+    beta = (qm31 *)malloc(sizeof(qm31) * C);
+    d2_powers(alpha, C, beta);
+    for (uint32_t q = 0; q < nq; q++)
+        for (int side = 0; side < 2; side++) {
+            const uint32_t *v = (side ? td.sibs : td.vals) + (size_t)q * C;
+            qm31 acc = {{0, 0, 0, 0}};
+            for (uint32_t c = 0; c < C; c++) acc = q_add(acc, q_scale(beta[c], v[c]));
+            if (memcmp(acc.v, (side ? qd.sibs : qd.vals) + 4 * (size_t)q, 16))
+                VFAIL("Quotient combination mismatch at query %u", q);
+        }
+    fidx = (uint64_t *)calloc(nq + 1, 8);
+    for (uint32_t i = 0; i < nq; i++) fidx[i] = qidx[i] / 2;
+    for (uint32_t k = 0; k < n_fd && k < n_fri; k++) {
+        if (!check_decom(&fd[k], froots[k], fidx, nq)) VFAIL("FRI layer %u auth path invalid", k);
+        for (uint32_t i = 0; i < nq; i++) fidx[i] /= 2;
+    }
+    if (n_fd != n_fri || n_fd == 0) VFAIL("FRI decommitment count mismatch");
+    for (uint32_t q = 0; q < nq; q++) {
+        uint64_t ci = qidx[q];
+        uint32_t cl = S;
+        qm31 f0 = q_load(qd.vals + 4 * (size_t)q), f1 = q_load(qd.sibs + 4 * (size_t)q);
+        if (ci % 2) { qm31 t = f0; f0 = f1; f1 = t; }
+        qm31 e = fold_pair(f0, f1, alphas[0], fold_twiddle_at(eval_domain(cl), ci / 2, 1));
+        if (memcmp(e.v, fd[0].vals + 4 * (size_t)q, 16)) VFAIL("Circle fold mismatch at query %u", q);
+        ci /= 2;
+        cl--;
+        for (uint32_t k = 0; k + 1 < n_fd; k++) {
+            qm31 ex = pair_fold(&fd[k], q, ci, alphas[k + 1], fold_twiddle_at(coset_make(OR_HALF_ODDS, cl), ci / 2, 0));
+            if (memcmp(ex.v, fd[k + 1].vals + 4 * (size_t)q, 16)) VFAIL("Line fold mismatch at query %u, layer %u", q, k);
+            ci /= 2;
+            cl--;
+        }
+        if (ci >= n_last) VFAIL("FRI last layer index out of bounds");
+        if (memcmp(fd[n_fd - 1].vals + 4 * (size_t)q, lastw + 4 * ci, 16)) VFAIL("FRI last layer mismatch at query %u", q);
+    }
+    ok = 1;
+done:
+    decom_free(&td);
+    decom_free(&qd);
+    if (fd)
+        for (uint32_t k = 0; k < n_fd; k++) decom_free(&fd[k]);
+    free(fd);
+    free(logs);
+    free(froots);
+    free(lastw);
+    free(qidx);
+    free(fidx);
+    free(alphas);
+    free(beta);
+    return ok;
+}

@@ -146,6 +146,14 @@ void or_build_leaves_lifted(const uint32_t *const *cols, const uint32_t *log_siz
 void or_merkle_root_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
                            uint32_t root[8]);
 
+/* ---- batched-columns proof (KRAK v4): per-column bit-reversed LDE, lifted
+ *      tree, random-linear-combination quotient, the Fibonacci FRI ---- */
+int or_batch_prove(uint64_t seed, const uint32_t *log_sizes, uint32_t n_cols, uint32_t blowup_bits,
+                   uint32_t n_queries, uint32_t pow_bits, uint8_t **out, size_t *out_len);
+int or_batch_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits, uint32_t n_queries, char *err,
+                    size_t err_cap);
+size_t or_batch_proof_size(const uint32_t *log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,9 @@ Sources, per section:
   proofs_d2_c  the C oracle's multi-column AIR proofs at larger sizes
   batch_c      the C oracle's lifted root of the batched uneven-column
                workload (per-column LDE, bit reverse, lifted tree)
+  proofs_batch_py / proofs_batch_c
+               the batched-columns proof (KRAK v4): pyref.prove_batch at tiny
+               sizes, the C oracle at the workload's sizes
 
 Run from the repo root:  python tests/golden/gen_golden.py [--big]
 (--big adds the log_n = 20, 24 and 25 (blowup 1) oracle digests, the
@@ -169,6 +172,23 @@ def main():
                              "bit_reverse": True, "root": [int(x) for x in root]})
         print("oracle batch root", n_cols, flush=True)
 
+    # batched-columns proof (KRAK v4): Python restatement at tiny sizes, the C
+    # oracle at the workload's sizes
+    g["proofs_batch_py"] = []
+    for sd, logs, bb, nq, pw in [(5, [2, 3, 3, 4], 2, 8, 0), (9, [3], 1, 5, 0),
+                                 (1, [2, 2, 2, 3, 3] + [4] * 14, 2, 80, 0), (3, [3, 5], 2, 16, 6)]:
+        data = pyref.prove_batch(sd, logs, bb, nq, pw)
+        g["proofs_batch_py"].append({"seed": sd, "log_sizes": logs, "blowup_bits": bb, "n_queries": nq,
+                                     "pow_bits": pw, "len": len(data), "sha256": hashlib.sha256(data).hexdigest()})
+    g["proofs_batch_c"] = []
+    for n_cols, sd, pw in [(12, 7, 0)] + ([(96, seed, 0), (96, seed, 20)] if big else []):
+        logs = vx_sizes.batch_log_sizes(n_cols, sd)
+        data = pyoracle.batch_prove(sd, logs, 2, 80, pw)
+        g["proofs_batch_c"].append({"n_cols": n_cols, "seed": sd, "log_sizes": logs, "blowup_bits": 2,
+                                    "n_queries": 80, "pow_bits": pw, "len": len(data),
+                                    "sha256": hashlib.sha256(data).hexdigest()})
+        print("oracle batch proof", n_cols, pw, len(data), flush=True)
+
     out = os.path.join(HERE, "golden.json")
     if not big and os.path.exists(out):
         old = json.load(open(out))
@@ -176,6 +196,7 @@ def main():
         g["proofs_c"] += keep
         g["proofs_d2_c"] += [p for p in old.get("proofs_d2_c", []) if p["log_n"] >= 20]
         g["batch_c"] += [p for p in old.get("batch_c", []) if p["n_cols"] >= 96]
+        g["proofs_batch_c"] += [p for p in old.get("proofs_batch_c", []) if p["n_cols"] >= 96]
     with open(out, "w") as f:
         json.dump(g, f, separators=(",", ":"))
     print("wrote", out, os.path.getsize(out), "bytes")

@@ -424,3 +424,100 @@ def prove_d2(seed, log_n, w=34, k_cons=35, air_seed=0, blowup_bits=2, n_queries=
         out += decom(vals, layers, idx)
         idx = [i // 2 for i in idx]
     return le_words(out)
+
+
+# ---------------- batched-columns proof (KRAK v4, build-defined) ----------------
+# Columns c of log sizes L_c (ascending), values d2_value(seed, c, i), LDE at
+# blowup, bit reversed; lifted tree (leaf = Blake2s of every column at its
+# lifted row); the channel mixes Blake2s(log sizes) then the root; Q(i) =
+# sum_c alpha^(c+1) col_c[lifted row]; then the FRI phase.
+
+def lifted_row(i, S, L):
+    return i if L == S else ((i >> (S - L + 1)) << 1) | (i & 1)
+
+
+def prove_batch(seed, log_sizes, blowup_bits=2, n_queries=80, pow_bits=0):
+    elog = [L + blowup_bits for L in log_sizes]
+    S = elog[-1]
+    N = 1 << S
+    cols = []
+    for c, L in enumerate(log_sizes):
+        coeffs = interpolate([d2_value(seed, c, i) for i in range(1 << L)], half_coset(L))
+        ev = evaluate(coeffs + [0] * ((1 << elog[c]) - len(coeffs)), eval_domain(elog[c]))
+        cols.append([ev[bitrev(i, elog[c])] for i in range(len(ev))])
+
+    def leaf_words(i):
+        return [cols[c][lifted_row(i, S, elog[c])] for c in range(len(cols))]
+
+    t_layers = merkle_layers_from_hashes([b2(le_words(leaf_words(i))) for i in range(N)])
+    ch = Channel()
+    ch.mix_digest(words(b2(le_words(list(log_sizes)))))   # the sizes enter the transcript first
+    ch.mix_digest(words(t_layers[-1][0]))
+    alpha = ch.draw_felt()
+    betas, b = [], alpha
+    for _ in cols:
+        betas.append(b)
+        b = qm_mul(b, alpha)
+    q = []
+    for i in range(N):
+        acc = (0, 0, 0, 0)
+        for be, v in zip(betas, leaf_words(i)):
+            acc = qm_add(acc, qm_scale(be, v))
+        q.append(acc)
+    q_layers = merkle_layers(q)
+    ch.mix_digest(words(q_layers[-1][0]))
+    alpha0 = ch.draw_felt()
+    ed = eval_domain(S)
+    layer = [fold(q[2 * i], q[2 * i + 1], alpha0, fold_tw(ed, i, True)) for i in range(N // 2)]
+    fri = [(layer, merkle_layers(layer))]
+    ch.mix_digest(words(fri[-1][1][-1][0]))
+    cur = S - 1
+    while cur > 3:
+        al = ch.draw_felt()
+        d = half_odds(cur)
+        layer = [fold(layer[2 * i], layer[2 * i + 1], al, fold_tw(d, i, False)) for i in range(len(layer) // 2)]
+        cur -= 1
+        fri.append((layer, merkle_layers(layer)))
+        ch.mix_digest(words(fri[-1][1][-1][0]))
+    ch.mix_felts(layer)
+    nonce = 0
+    if pow_bits:
+        while not ch.pow_ok(pow_bits, nonce):
+            nonce += 1
+        ch.mix_u64(nonce)
+    qis = [ch.draw_number(N) for _ in range(n_queries)]
+
+    out = [0x4B52414B, 4, len(cols), blowup_bits, pow_bits, nonce & 0xFFFFFFFF, nonce >> 32,
+           seed & 0xFFFFFFFF, seed >> 32] + list(log_sizes)
+    out += words(t_layers[-1][0]) + words(q_layers[-1][0])
+    out += [len(fri)] + sum((words(f[1][-1][0]) for f in fri), [])
+    out += [len(layer)] + sum((list(v) for v in layer), [])
+    out += [len(qis)] + qis
+    out += [len(qis)]
+    for side in (0, 1):
+        for i in qis:
+            out += leaf_words(i ^ side)
+    for side in (0, 1):
+        for i in qis:
+            pth = auth_path(t_layers, i ^ side)
+            out += [len(pth)] + sum(pth, [])
+
+    def decom(vals, layers, idxs):
+        res = [len(idxs)]
+        for i in idxs:
+            res += list(vals[i])
+        for i in idxs:
+            res += list(vals[i ^ 1])
+        for sel in (0, 1):
+            for i in idxs:
+                pth = auth_path(layers, i ^ sel)
+                res += [len(pth)] + sum(pth, [])
+        return res
+
+    out += decom(q, q_layers, qis)
+    out += [len(fri)]
+    idx = [i // 2 for i in qis]
+    for vals, layers in fri:
+        out += decom(vals, layers, idx)
+        idx = [i // 2 for i in idx]
+    return le_words(out)

@@ -2,6 +2,7 @@
 reverse and the lifted Merkle tree against the oracle, column by column and
 root by root; the resident batch against the committed workload digest."""
 import ctypes as C
+import hashlib
 
 import numpy as np
 import pytest
@@ -116,3 +117,40 @@ def test_batch_rejects_bad_shapes():
     with pytest.raises(vx.VortexError, match="14..24"):
         vx.Batch([12, 14])
     sync()
+
+
+@pytest.mark.parametrize("logs,bb,pw", [([14], 1, 0), ([14, 14, 15, 16], 2, 0), ([14] * 17 + [15], 2, 8),
+                                        ([15, 15, 16, 17, 17, 17], 1, 0)])
+def test_batch_proofs_match_oracle(logs, bb, pw):
+    print("batch proof", logs, bb, pw, flush=True)
+    p = vx.BatchProver(logs, bb, 80, pw)
+    for seed in (7, 2**40 + 3):
+        got = p.prove(seed)
+        want = po.batch_prove(seed, logs, bb, 80, pw)
+        assert len(got) == len(want) == p.size
+        assert got == want, (logs, seed)
+    assert vx.verify(got, bb) == (True, "")
+    p.close()
+
+
+def test_batch_proof_graph_replay_equals_eager():
+    logs = [14, 15, 15, 16]
+    g = vx.BatchProver(logs, graphs=True)
+    e = vx.BatchProver(logs, graphs=False)
+    for seed in (1, 2, 1, 99):
+        assert g.prove(seed) == e.prove(seed)
+    g.close()
+    e.close()
+
+
+@pytest.mark.parametrize("n_cols,pw", [(12, 0), (96, 0), (96, 20)])
+def test_batch_proof_workload_digests(golden, n_cols, pw):
+    g = next((x for x in golden["proofs_batch_c"] if (x["n_cols"], x["pow_bits"]) == (n_cols, pw)), None)
+    if g is None:
+        pytest.skip("digest not in golden.json (gen_golden.py --big)")
+    p = vx.BatchProver(g["log_sizes"], g["blowup_bits"], g["n_queries"], pw)
+    got = p.prove(g["seed"])
+    assert len(got) == g["len"]
+    assert hashlib.sha256(got).hexdigest() == g["sha256"]
+    assert vx.verify(got) == (True, "")
+    p.close()

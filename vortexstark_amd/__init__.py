@@ -171,6 +171,33 @@ class Batch:
             pass
 
 
+def batch_proof_size(log_sizes, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES):
+    return int(load().vx_batch_proof_size(_u32_array(list(log_sizes)), len(log_sizes), blowup_bits, n_queries))
+
+
+class BatchProver(Prover):
+    """Cached prover of the batched-columns proof (KRAK v4): columns of the
+    given log sizes (sorted ascending, 14..24) from the synthetic generator,
+    bit-reversed LDE, lifted tree, random-linear-combination quotient, FRI."""
+
+    def __init__(self, log_sizes, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_bits=0, stream=None,
+                 graphs=True):
+        self.log_sizes = list(log_sizes)
+        self.log_n, self.blowup_bits, self.n_queries, self.pow_bits = self.log_sizes[-1], blowup_bits, n_queries, pow_bits
+        self._h = C.c_void_p()
+        _lib.call("vx_batch_prover_create", _u32_array(self.log_sizes), len(self.log_sizes), blowup_bits, n_queries,
+                  pow_bits, stream, C.byref(self._h))
+        self.size = batch_proof_size(self.log_sizes, blowup_bits, n_queries)
+        _lib.call("vx_prover_set_graphs", self._h, 1 if graphs else 0)
+
+    def prove(self, seed):
+        self.enqueue(seed)
+        return self.wait()
+
+    def enqueue(self, seed, timed=False):
+        _lib.call("vx_prover_enqueue", self._h, seed & 0xFFFFFFFF, seed >> 32, 1 if timed else 0)
+
+
 def batch_log_sizes(n_cols, seed, lo=14, hi=22):
     """The batched-commit workload's column sizes: SplitMix64(seed) outputs
     mod (hi - lo + 1) + lo, sorted ascending."""

@@ -74,6 +74,8 @@ SIGNATURES = {
     "vx_d2_prover_create": (ip, [u32, u32, u32, u64, u32, u32, u32, vp, C.POINTER(vp)]),
     "vx_d2_proof_size": (sz, [u32, u32, u32, u32]),
     "vx_m31_fill_random": (ip, [vp, u32, u64, u32, vp]),
+    "vx_batch_prover_create": (ip, [vp, u32, u32, u32, u32, vp, C.POINTER(vp)]),
+    "vx_batch_proof_size": (sz, [vp, u32, u32, u32]),
     "vx_circle_ntt_lde_batch": (ip, [vp, vp, vp, vp, u32, u32, vp]),
     "vx_bit_reverse_m31_batch": (ip, [vp, vp, vp, u32, vp]),
     "vx_build_leaves_lifted": (ip, [vp, vp, u32, u32, vp, vp]),

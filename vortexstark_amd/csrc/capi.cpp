@@ -441,6 +441,30 @@ size_t vx_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, u
   return proof_size(log_n, blowup_bits, nq, 1, air);
 }
 
+int vx_batch_prover_create(const uint32_t* h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t nq,
+                           uint32_t pow_bits, vx_stream_t st, vx_prover** out) {
+  return guard([&] {
+    require(out != nullptr && h_log_sizes != nullptr && n_cols >= 1, "batch_prover_create: bad arguments");
+    *out = nullptr;
+    AirSpec air;
+    air.kind = AIR_BATCH;
+    air.n_cols = n_cols;
+    air.log_sizes.assign(h_log_sizes, h_log_sizes + n_cols);
+    *out = new vx_prover(h_log_sizes[n_cols - 1], blowup_bits, nq, pow_bits, S(st), air);
+  });
+}
+
+size_t vx_batch_proof_size(const uint32_t* h_log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t nq) {
+  if (!h_log_sizes || n_cols < 1 || n_cols > 4096 || blowup_bits < 1) return 0;
+  const uint32_t S = h_log_sizes[n_cols - 1] + blowup_bits;
+  if (S < 4 || S > 31) return 0;
+  AirSpec air;
+  air.kind = AIR_BATCH;
+  air.n_cols = n_cols;
+  air.log_sizes.assign(h_log_sizes, h_log_sizes + n_cols);
+  return proof_size(h_log_sizes[n_cols - 1], blowup_bits, nq, 1, air);
+}
+
 int vx_prover_destroy(vx_prover* p) {
   return guard([&] { delete p; });
 }

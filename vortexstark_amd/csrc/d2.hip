@@ -56,6 +56,35 @@ void d2_trace_col(const uint32_t* seed_words, uint32_t* out, uint32_t col, uint3
   VX_HIP(hipGetLastError());
 }
 
+// Columns stored back to back: column c occupies [off[c], off[c+1]) (every
+// length a multiple of 4) and holds the generator's column c. One launch
+// for all columns; a lane finds its column by binary search over off[].
+__global__ void k_d2_fill_columns(const uint32_t* seed_words, uint32_t* out, const uint64_t* off, uint32_t n_cols) {
+  const uint64_t first = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * D2_ROWS_PER_LANE;
+  if (first >= off[n_cols]) return;
+  uint32_t lo = 0, hi = n_cols;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= first) lo = mid;
+    else hi = mid;
+  }
+  const uint64_t seed = (uint64_t)seed_words[0] | ((uint64_t)seed_words[1] << 32);
+  const uint64_t row = first - off[lo];
+  uint32_t v[D2_ROWS_PER_LANE];
+#pragma unroll
+  for (uint32_t j = 0; j < D2_ROWS_PER_LANE; j++) v[j] = d2_value(seed, lo, row + j);
+  *reinterpret_cast<uint4*>(out + first) = make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+void d2_fill_columns(const uint32_t* seed_words, uint32_t* out, const uint64_t* d_off, uint32_t n_cols,
+                     uint64_t total, hipStream_t s) {
+  require(total % D2_ROWS_PER_LANE == 0 && n_cols >= 1, "d2_fill_columns: bad shape");
+  const uint64_t lanes = total / D2_ROWS_PER_LANE;
+  hipLaunchKernelGGL(k_d2_fill_columns, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, s, seed_words, out,
+                     d_off, n_cols);
+  VX_HIP(hipGetLastError());
+}
+
 // beta[k] = alpha^(k+1), k < K (one lane; K <= 1024 QM31 products).
 __global__ void k_d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons) {
   if (threadIdx.x != 0) return;
@@ -69,7 +98,7 @@ __global__ void k_d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_co
 }
 
 void d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons, hipStream_t s) {
-  require(k_cons >= 1 && k_cons <= D2_MAX_CONS, "d2_powers: bad constraint count");
+  require(k_cons >= 1 && k_cons <= 4096, "d2_powers: bad count");
   hipLaunchKernelGGL(k_d2_powers, dim3(1), dim3(64), 0, s, alpha, beta, k_cons);
   VX_HIP(hipGetLastError());
 }

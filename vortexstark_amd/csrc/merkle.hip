@@ -23,6 +23,7 @@
 // (mix the root, draw the next alpha), so no separate channel launch is
 // needed. FRI layers of <= 2^10 values are committed by one workgroup in a
 // single launch (k_fri_tail). Blake2s is VALU-bound here.
+#include "vx_blake2s_x4.h"
 #include "vx_internal.h"
 
 namespace vx {
@@ -202,9 +203,9 @@ __device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g) 
 __device__ __forceinline__ uint64_t lifted_row(uint64_t idx, uint32_t S, uint32_t L) {
   return L == S ? idx : (((idx >> (S - L + 1)) << 1) | (idx & 1));
 }
-__device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint32_t h[8]) {
+__device__ __forceinline__ void lifted_leaf_cols(const LiftCol* cols, uint32_t n, uint32_t S, uint64_t idx,
+                                                 uint32_t h[8]) {
   b2_init(h);
-  const uint32_t n = L.lift_n, S = L.lift_log;
   uint32_t t = 0;
   for (uint32_t base = 0; base < n; base += 16) {
     uint32_t m[16];
@@ -213,7 +214,7 @@ __device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint
       const uint32_t c = base + j;
       uint32_t v = 0;
       if (c < n) {
-        const LiftCol col = L.lift[c];
+        const LiftCol col = cols[c];
         v = col.p[lifted_row(idx, S, col.log_n)];
       }
       m[j] = v;
@@ -221,6 +222,42 @@ __device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint
     t += 4 * (n - base < 16 ? n - base : 16);
     b2_compress(h, m, t, base + 16 >= n);
   }
+}
+__device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint32_t h[8]) {
+  lifted_leaf_cols(L.lift, L.lift_n, L.lift_log, idx, h);
+}
+
+// Random linear combination of the lifted rows of leaves [i, i+4):
+// Q(r) = sum_c beta_c col_c[lifted_row(r)]. A column of the tree's size
+// gives rows i..i+3 (one 16-byte load); a smaller one gives rows
+// b, b+1, b, b+1 with b = (i >> shift) << 1 (one 8-byte load). Q is stored
+// (SoA) for the fold.
+__device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i, G4& g) {
+  const uint32_t S = L.lift_log;
+  qm31 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) acc[j] = qm31{{0, 0, 0, 0}};
+  for (uint32_t c = 0; c < L.lift_n; c++) {
+    const LiftCol col = L.lift[c];
+    const qm31 be = ldq(L.beta + 4 * c);
+    uint32_t v[4];
+    if (col.log_n == S) {
+      u4w(ld4(col.p + i), v);
+    } else {
+      const uint64_t b = (i >> (S - col.log_n + 1)) << 1;
+      const uint2 x = *reinterpret_cast<const uint2*>(col.p + b);
+      v[0] = v[2] = x.x;
+      v[1] = v[3] = x.y;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] = qm_add(acc[j], qm_scale(be, v[j]));
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) g.w[j][c] = acc[j].v[c];
+#pragma unroll
+  for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
 }
 
 template <uint32_t KIND, uint32_t NC>
@@ -250,7 +287,8 @@ template <uint32_t KIND, uint32_t NC, uint32_t LPT_LOG>
 __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, uint32_t out[8]) {
   static_assert(LPT_LOG >= 2 && LPT_LOG <= 4, "lane subtree covers 4..16 leaves");
   constexpr bool kGeneric = KIND == LEAF_COLS && NC == 0;
-  constexpr bool kDirect = kGeneric || KIND == LEAF_D2_QUOTIENT || KIND == LEAF_LIFTED;  // no raw-load prefetch
+  constexpr bool kDirect = kGeneric || KIND == LEAF_D2_QUOTIENT || KIND == LEAF_LIFTED ||
+                           KIND == LEAF_LIFTED_RLC;  // no raw-load prefetch
   constexpr uint32_t G = 1u << (LPT_LOG - 2);
   qm31 aq{}, af{};
   if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
@@ -268,6 +306,7 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
       leaf_produce<KIND, NC>(L, i, cur, aq, af, g);
     }
     if constexpr (KIND == LEAF_D2_QUOTIENT) d2_produce(L, i, g);
+    if constexpr (KIND == LEAF_LIFTED_RLC) lifted_rlc_produce(L, i, g);
     uint32_t a[8], b[8], n1a[8], n1b[8], n2[8];
     leaf_hash<KIND, NC>(L, g, i, 0, a);
     leaf_hash<KIND, NC>(L, g, i, 1, b);
@@ -333,7 +372,12 @@ __device__ __forceinline__ void finish_root(const RootHook& h, const uint32_t no
 // Merges `nthr` nodes held one per lane in `buf[0]` up to one node, storing
 // each produced level (lvl0+1 ...) at global index base_idx(level) + lane.
 // Returns the top node in lane 0's `node`.
-template <uint32_t CAP>
+// With QUAD, levels with at most CAP / 4 nodes hash each node with the four
+// lanes of a quad (b2_node_x4): those levels are a dependent chain with most
+// lanes idle, so halving the instructions per compression halves their
+// latency. (The throughput instances do without: the quad path raises their
+// VGPR count.) Lane 0 returns the top node.
+template <uint32_t CAP, bool QUAD>
 __device__ __forceinline__ void lds_merge(uint4 (*buf)[2 * CAP], uint32_t nthr, uint32_t lvl0,
                                           const TreeLevels& T, uint64_t group, uint32_t group_log,
                                           uint32_t store_lo, uint32_t node[8]) {
@@ -342,7 +386,25 @@ __device__ __forceinline__ void lds_merge(uint4 (*buf)[2 * CAP], uint32_t nthr, 
   uint32_t cnt = nthr;
   for (uint32_t lvl = lvl0 + 1; cnt > 1; lvl++) {
     cnt >>= 1;
-    if (tid < cnt) {
+    if (QUAD && 4 * cnt <= CAP) {
+      const uint32_t nid = tid >> 2, q = tid & 3;
+      if (nid < cnt) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(buf[cur]) + 16 * nid;
+        uint32_t m[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) m[k] = src[k];
+        uint32_t lo, hi;
+        b2_node_x4(m, q, lo, hi);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(buf[cur ^ 1]) + 8 * nid;
+        dst[q] = lo;
+        dst[4 + q] = hi;
+        if (T.base && lvl >= store_lo) {
+          uint32_t* g = T.base + T.off[lvl] + 8 * ((group << (group_log - lvl)) + nid);
+          g[q] = lo;
+          g[4 + q] = hi;
+        }
+      }
+    } else if (tid < cnt) {
       uint32_t l[8], r[8];
       lds_get(buf[cur], 2 * tid, l);
       lds_get(buf[cur], 2 * tid + 1, r);
@@ -354,6 +416,7 @@ __device__ __forceinline__ void lds_merge(uint4 (*buf)[2 * CAP], uint32_t nthr, 
     cur ^= 1;
     __syncthreads();
   }
+  if (tid == 0) lds_get(buf[cur], 0, node);
 }
 
 // ROOT instances (the whole tree in one workgroup) carry the transcript step;
@@ -375,7 +438,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(LeafSrc L, TreeLevels T
       store_node(T.base + T.off[LPT_LOG] + 8 * ((tile << (tile_log - LPT_LOG)) + tid), node);
   }
   __syncthreads();
-  lds_merge<SUB_THREADS>(buf, nthr, LPT_LOG, T, tile, tile_log, store_lo, node);
+  lds_merge<SUB_THREADS, ROOT || LPT_LOG == 2>(buf, nthr, LPT_LOG, T, tile, tile_log, store_lo, node);
   if constexpr (ROOT) {
     if (tid == 0) finish_root(hook, node);
   }
@@ -403,7 +466,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_tree_reduce(TreeLevels T, uint3
     store_node(T.base + T.off[from + 1] + 8 * ((wg << (nlev - 1)) + tid), node);
   }
   __syncthreads();
-  lds_merge<RED_THREADS>(buf, nthr, from + 1, T, wg, from + nlev, 0, node);
+  lds_merge<RED_THREADS, true>(buf, nthr, from + 1, T, wg, from + nlev, 0, node);
   if constexpr (ROOT) {
     if (tid == 0) finish_root(hook, node);
   }
@@ -461,6 +524,10 @@ static void check_leaf(const LeafSrc& L, uint32_t log_n) {
     case LEAF_LIFTED:
       require(L.lift && L.lift_n >= 1 && L.lift_log == log_n, "merkle: bad lifted source");
       break;
+    case LEAF_LIFTED_RLC:
+      require(L.lift && L.lift_n >= 1 && L.lift_log == log_n && L.beta, "merkle: bad lifted combination source");
+      for (int c = 0; c < 4; c++) require(L.out[c] != nullptr, "merkle: null quotient output");
+      break;
     case LEAF_D2_QUOTIENT:
       require(L.col[0] && L.cons && L.beta && L.n_cols >= 1 && L.n_cons >= 1 &&
                   L.src_mask == (1ull << log_n) - 1 && L.col_stride >= (1ull << log_n),
@@ -485,6 +552,7 @@ static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t til
     case LEAF_FOLD_LINE: launch_subtree<LEAF_FOLD_LINE, 4>(L, T, tile_log, store_lo, hook, s); break;
     case LEAF_D2_QUOTIENT: launch_subtree<LEAF_D2_QUOTIENT, 4>(L, T, tile_log, store_lo, hook, s); break;
     case LEAF_LIFTED: launch_subtree<LEAF_LIFTED, 0>(L, T, tile_log, store_lo, hook, s); break;
+    case LEAF_LIFTED_RLC: launch_subtree<LEAF_LIFTED_RLC, 4>(L, T, tile_log, store_lo, hook, s); break;
   }
 }
 
@@ -678,7 +746,7 @@ __global__ __launch_bounds__(RTR_THREADS) void k_reduce_to_root(const uint32_t* 
   }
   __syncthreads();
   TreeLevels none{};
-  lds_merge<RTR_THREADS>(buf, nthr, 0, none, 0, 0, 0, node);
+  lds_merge<RTR_THREADS, true>(buf, nthr, 0, none, 0, 0, 0, node);
   if (tid == 0)
     for (int k = 0; k < 8; k++) root[k] = node[k];
 }
@@ -748,7 +816,7 @@ void merkle_lifted_leaves(const LeafSrc& L, uint32_t* hashes, hipStream_t s) {
 // path into the proof (src/bin/stark_cli.rs:1061-1089 layout: n, values,
 // partner values, paths, partner paths).
 __device__ __forceinline__ uint32_t decom_width(const DecomDesc& d) {
-  return d.kind == DECOM_COLS ? d.n_cols : 4u;
+  return (d.kind == DECOM_COLS || d.kind == DECOM_LIFTED) ? d.n_cols : 4u;
 }
 __device__ __forceinline__ void decom_value(const DecomDesc& d, uint64_t i, uint32_t w[16]) {
   if (d.kind == DECOM_COLS) {
@@ -778,10 +846,15 @@ __global__ __launch_bounds__(64) void k_decommit(const DecomDesc* descs, const u
   const uint64_t first = (idx >> lo) << lo;
   // level v occupies nodes [base(v), base(v) + 2^(lo-v)), base(v) = 32 - 2^(lo-v+1)
   if (lane < (1u << lo)) {
-    uint32_t w[16], h[8];
-    for (int k = 0; k < 16; k++) w[k] = 0;
-    decom_value(d, first + lane, w);
-    b2_hash_words(w, W, h);
+    uint32_t h[8];
+    if (d.kind == DECOM_LIFTED) {
+      lifted_leaf_cols(d.lift, d.n_cols, D, first + lane, h);
+    } else {
+      uint32_t w[16];
+      for (int k = 0; k < 16; k++) w[k] = 0;
+      decom_value(d, first + lane, w);
+      b2_hash_words(w, W, h);
+    }
     for (int k = 0; k < 8; k++) nodes[8 * (32 - (2u << lo) + lane) + k] = h[k];
   }
   __syncthreads();
@@ -803,7 +876,14 @@ __global__ __launch_bounds__(64) void k_decommit(const DecomDesc* descs, const u
   const uint64_t pbase = base + 1 + 2ull * nq * W;
   const uint64_t plen = 1 + 8ull * D;
   uint32_t* path = proof + pbase + ((uint64_t)role * nq + q) * plen;
-  if (lane == 0) {
+  if (d.kind == DECOM_LIFTED) {  // the opened row: every column at its lifted row
+    for (uint32_t c = lane; c < W; c += 64)
+      proof[base + 1 + ((uint64_t)role * nq + q) * W + c] = d.lift[c].p[lifted_row(idx, D, d.lift[c].log_n)];
+    if (lane == 0) {
+      if (q == 0 && role == 0) proof[base] = nq;
+      path[0] = D;
+    }
+  } else if (lane == 0) {
     uint32_t w[16];
     if (q == 0 && role == 0) proof[base] = nq;
     decom_value(d, idx, w);

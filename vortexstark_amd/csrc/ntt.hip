@@ -112,12 +112,31 @@ __device__ __forceinline__ uint32_t rx_own(uint32_t t, uint32_t s, uint32_t pos)
 }
 __device__ __forceinline__ uint32_t rx_swz(uint32_t e) { return e ^ ((e >> 5) & 31u); }
 
-template <bool FWD, int Q>
-__device__ __forceinline__ void rx_bit(uint32_t (&v)[32], const uint32_t* table, uint64_t base) {
+// Twiddles of register bit Q for this lane: the 2^(4-Q) consecutive words
+// from `base` (a multiple of 2^(4-Q)); 16-byte loads when aligned.
+template <int Q>
+__device__ __forceinline__ void rx_load_tw(const uint32_t* table, uint64_t base, uint32_t (&w)[1u << (4 - Q)]) {
   constexpr uint32_t NW = 1u << (4 - Q);
-  uint32_t w[NW];
+  const uint32_t* p = table + base;
+  if constexpr (NW >= 4) {
+    if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
 #pragma unroll
-  for (uint32_t m = 0; m < NW; m++) w[m] = table[base + m];
+      for (uint32_t m = 0; m < NW; m += 4) {
+        const uint4 x = *reinterpret_cast<const uint4*>(p + m);
+        w[m] = x.x;
+        w[m + 1] = x.y;
+        w[m + 2] = x.z;
+        w[m + 3] = x.w;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < NW; m++) w[m] = p[m];
+}
+
+template <bool FWD, int Q>
+__device__ __forceinline__ void rx_apply(uint32_t (&v)[32], const uint32_t (&w)[1u << (4 - Q)]) {
 #pragma unroll
   for (uint32_t s0 = 0; s0 < 32; s0++) {
     if (s0 & (1u << Q)) continue;
@@ -150,7 +169,25 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   for (uint32_t ri = 0; ri < a.n_rounds; ri++) {
     const RadixRound R = a.round[ri];
     const uint32_t pos = R.pos;
-    if (ri == 0) {
+    // A contiguous pass (cbits = 0) whose round keeps each lane's 32 elements
+    // next to each other (pos < 4) would read or write HBM with a 128-byte
+    // stride between lanes; it goes through LDS instead, so that every
+    // 16-byte access of a wavefront is part of one contiguous 1 KiB run.
+    const bool staged = a.cbits == 0 && a.j0 == 0 && pos < 4;
+    if (ri == 0 && staged) {
+#pragma unroll
+      for (uint32_t g = 0; g < 8; g++) {
+        const uint32_t e = (g << 11) | (t << 2);
+        const uint4 x = *reinterpret_cast<const uint4*>(src + ((hi_base | e) & a.src_mask));
+        lds[rx_swz(e)] = x.x;
+        lds[rx_swz(e + 1)] = x.y;
+        lds[rx_swz(e + 2)] = x.z;
+        lds[rx_swz(e + 3)] = x.w;
+      }
+      __syncthreads();
+#pragma unroll
+      for (uint32_t s = 0; s < 32; s++) v[s] = lds[rx_swz(rx_own(t, s, pos))];
+    } else if (ri == 0) {
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) {
         const uint32_t e = rx_own(t, s, pos);
@@ -162,22 +199,42 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
       for (uint32_t s = 0; s < 32; s++) v[s] = lds[rx_swz(rx_own(t, s, pos))];
     }
     const uint32_t qlo = R.rb0 + a.cbits - pos, qhi = qlo + R.nb;  // register bits to butterfly
-#define VX_RX_BIT(Q)                                                                              \
+    // all of the round's twiddles are requested before the first butterfly,
+    // so their latencies overlap (one wait per round instead of one per bit)
+    uint32_t w4[1], w3[2], w2[4], w1[8], w0[16];
+#define VX_RX_LOAD(Q, W)                                                                          \
   if ((Q) >= qlo && (Q) < qhi) {                                                                  \
     const uint32_t epos = pos + (Q);                                                              \
     const uint64_t base = (H << (13 - epos)) | ((uint64_t)(t >> pos) << (4 - (Q)));               \
-    rx_bit<FWD, (Q)>(v, a.tw[epos - a.cbits], base);                                              \
+    rx_load_tw<(Q)>(a.tw[epos - a.cbits], base, W);                                               \
   }
+    VX_RX_LOAD(4, w4) VX_RX_LOAD(3, w3) VX_RX_LOAD(2, w2) VX_RX_LOAD(1, w1) VX_RX_LOAD(0, w0)
+#undef VX_RX_LOAD
+#define VX_RX_APPLY(Q, W) \
+  if ((Q) >= qlo && (Q) < qhi) rx_apply<FWD, (Q)>(v, W);
     if (FWD) {
-      VX_RX_BIT(4) VX_RX_BIT(3) VX_RX_BIT(2) VX_RX_BIT(1) VX_RX_BIT(0)
+      VX_RX_APPLY(4, w4) VX_RX_APPLY(3, w3) VX_RX_APPLY(2, w2) VX_RX_APPLY(1, w1) VX_RX_APPLY(0, w0)
     } else {
-      VX_RX_BIT(0) VX_RX_BIT(1) VX_RX_BIT(2) VX_RX_BIT(3) VX_RX_BIT(4)
+      VX_RX_APPLY(0, w0) VX_RX_APPLY(1, w1) VX_RX_APPLY(2, w2) VX_RX_APPLY(3, w3) VX_RX_APPLY(4, w4)
     }
-#undef VX_RX_BIT
+#undef VX_RX_APPLY
     if (ri + 1 < a.n_rounds) {
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) lds[rx_swz(rx_own(t, s, pos))] = v[s];
       __syncthreads();
+    } else if (staged) {
+      // each lane rewrites exactly the words it read this round, so no
+      // barrier is needed before these stores
+#pragma unroll
+      for (uint32_t s = 0; s < 32; s++)
+        lds[rx_swz(rx_own(t, s, pos))] = a.scale != 1u ? m31_mul(v[s], a.scale) : v[s];
+      __syncthreads();
+#pragma unroll
+      for (uint32_t g = 0; g < 8; g++) {
+        const uint32_t e = (g << 11) | (t << 2);
+        *reinterpret_cast<uint4*>(dst + (hi_base | e)) =
+            make_uint4(lds[rx_swz(e)], lds[rx_swz(e + 1)], lds[rx_swz(e + 2)], lds[rx_swz(e + 3)]);
+      }
     } else {
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) {
@@ -296,9 +353,13 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
   VX_HIP(hipGetLastError());
 }
 
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
                  uint32_t eff_bits, hipStream_t s, uint32_t n_cols, uint64_t dst_stride, uint64_t src_stride) {
   const uint32_t log_n = tw.log_n;
+  require(aligned16(data) && aligned16(src) && dst_stride % 4 == 0 && src_stride % 4 == 0,
+          "ntt: columns must be 16-byte aligned");
   require(log_n >= 1 && log_n <= 31 && eff_bits >= 1 && eff_bits <= log_n, "ntt_forward: bad sizes");
   require(((src_mask + 1) & src_mask) == 0 && src_mask + 1 >= (1ull << eff_bits),
           "ntt_forward: source length must be a power of two covering the transformed bits");
@@ -327,6 +388,7 @@ void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const T
 
 void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n_cols, uint64_t stride) {
   const uint32_t log_n = tw.log_n;
+  require(aligned16(data) && stride % 4 == 0, "ntt: columns must be 16-byte aligned");
   require(log_n >= 1 && log_n <= 31, "ntt_inverse: bad size");
   require(n_cols == 1 || stride >= (1ull << log_n), "ntt_inverse: batch columns must not overlap");
   const uint32_t e = (30u * log_n) % 31u;
@@ -367,6 +429,8 @@ void lde_batch_build(const LdeBatchCol* cols, uint32_t n, uint32_t blowup_bits, 
     const uint32_t L = C.log_n, E = L + blowup_bits;
     require(L >= RX_E_BITS && L <= 24 && E <= 31 && C.itw.log_n == L && C.etw.log_n == E && C.coef && C.eval,
             "lde_batch: columns need 14 <= log_n <= 24 and matching twiddles");
+    require(aligned16(C.coef) && aligned16(C.eval) && (!C.in || aligned16(C.in)),
+            "lde_batch: columns must be 16-byte aligned");
     const uint32_t e = (30u * L) % 31u;
     const uint32_t inv_n = e == 0 ? 1u : (1u << e);
     const uint64_t nmask = (1ull << L) - 1, Emask = (1ull << E) - 1;

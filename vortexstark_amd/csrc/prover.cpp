@@ -50,13 +50,14 @@ ProofLayout::ProofLayout(uint32_t log_n_, uint32_t blowup_bits_, uint32_t nq_, u
     : log_n(log_n_), blowup_bits(blowup_bits_), nq(nq_), pow_bits(pow_bits_) {
   le = log_n + blowup_bits;
   n_fri = le - 3;
-  const bool d2 = air.kind == AIR_D2;
+  const bool d2 = air.kind == AIR_D2, batch = air.kind == AIR_BATCH;
   n_groups = d2 ? d2_groups(air.n_cols) : 1;
   // v1: magic, version, log_n, a, b; v2 adds blowup, pow, nonce lo/hi;
   // v3: magic, 3, log_n, W, K, blowup, pow, nonce lo/hi, seed lo/hi,
-  //     air_seed lo/hi, G
-  header_words = d2 ? 14 : (pow_bits ? 9 : 5);
-  w_nonce = d2 ? 7 : (pow_bits ? 7 : 0);
+  //     air_seed lo/hi, G;  v4: magic, 4, C, blowup, pow, nonce lo/hi,
+  //     seed lo/hi, C log sizes
+  header_words = batch ? 9 + air.n_cols : (d2 ? 14 : (pow_bits ? 9 : 5));
+  w_nonce = batch ? 5 : (d2 ? 7 : (pow_bits ? 7 : 0));
   uint64_t w = header_words;
   for (uint32_t g = 0; g < n_groups; g++) {
     w_troot.push_back(w);
@@ -69,7 +70,7 @@ ProofLayout::ProofLayout(uint32_t log_n_, uint32_t blowup_bits_, uint32_t nq_, u
   w_nq = w_last + 1 + 32;
   w = w_nq + 1 + nq;
   for (uint32_t g = 0; g < n_groups; g++) {
-    const uint64_t width = d2 ? d2_group_cols(air.n_cols, g) : 1;
+    const uint64_t width = d2 ? d2_group_cols(air.n_cols, g) : (batch ? air.n_cols : 1);
     w_tdec.push_back(w);
     w += 1 + 2ull * nq * width + 2ull * nq * (1 + 8ull * le);
   }
@@ -163,6 +164,14 @@ std::shared_ptr<const ProverTables> ProverTables::get(uint32_t log_n, uint32_t b
 
 static void check_air(const AirSpec& air) {
   if (air.kind == AIR_FIBONACCI) return;
+  if (air.kind == AIR_BATCH) {
+    require(air.n_cols >= 1 && air.n_cols <= 4096 && air.log_sizes.size() == air.n_cols,
+            "prover: batch needs 1..4096 columns");
+    for (uint32_t c = 0; c < air.n_cols; c++)
+      require(air.log_sizes[c] >= 14 && air.log_sizes[c] <= 24 && (c == 0 || air.log_sizes[c] >= air.log_sizes[c - 1]),
+              "prover: batch log sizes must be 14..24, sorted ascending");
+    return;
+  }
   require(air.kind == AIR_D2, "prover: unknown AIR");
   require(air.n_cols >= 1 && air.n_cols <= D2_MAX_COLS && air.n_cons >= 1 && air.n_cons <= D2_MAX_CONS,
           "prover: multi-column AIR needs 1..256 columns and 1..1024 constraints");
@@ -171,10 +180,16 @@ static void check_air(const AirSpec& air) {
 uint64_t StarkProver::estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t nq, uint32_t pow_bits,
                                      const AirSpec& air) {
   const uint32_t le = log_n + blowup_bits, n_fri = le - 3;
-  const bool d2 = air.kind == AIR_D2;
+  const bool d2 = air.kind == AIR_D2, batch = air.kind == AIR_BATCH;
   const uint64_t W = d2 ? air.n_cols : 1, G = d2 ? d2_groups(air.n_cols) : 1;
-  uint64_t b = (4ull << log_n) * W;    // trace
-  b += (4ull << le) * W;               // eval
+  uint64_t b = 0;
+  if (batch) {
+    for (uint32_t L : air.log_sizes) b += (8ull << L) + (8ull << (L + blowup_bits));  // in, coef, eval, bit-reversed
+    b += (16ull << le) + 16ull * air.n_cols + 32ull * air.n_cols;  // quotient, powers, lifted table
+  } else {
+    b += (4ull << log_n) * W;            // trace
+    b += (4ull << le) * W;               // eval
+  }
   if (d2) b += (16ull << le) + 32ull * air.n_cons + 16ull * air.n_cons;  // quotient, table, powers
   for (uint32_t k = 0; k < n_fri; k++) b += 16ull << (le - 1 - k);   // FRI layers
   for (uint32_t t = 0; t < G + 1 + n_fri; t++) {                     // trees, levels >= 4
@@ -208,12 +223,29 @@ StarkProver::StarkProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_querie
     VX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     own_stream_ = true;
   }
-  const bool d2 = air_.kind == AIR_D2;
+  const bool d2 = air_.kind == AIR_D2, batch = air_.kind == AIR_BATCH;
   const uint64_t W = d2 ? air_.n_cols : 1;
   const uint64_t n = 1ull << log_n_, N = 1ull << le_;
-  trace_ = DeviceBuf(4 * n * W);
-  eval_ = DeviceBuf(4 * N * W);
-  device_bytes_ += trace_.bytes + eval_.bytes;
+  if (batch) {
+    require(log_n_ == air_.log_sizes.back(), "prover: batch log_n must be the largest column's");
+    uint64_t win = 0, wev = 0;
+    for (uint32_t L : air_.log_sizes) {
+      win += 1ull << L;
+      wev += 1ull << (L + blowup_bits);
+    }
+    n_in_ = win;
+    trace_ = DeviceBuf(4 * win);
+    coef_ = DeviceBuf(4 * win);
+    eval_ = DeviceBuf(4 * wev);
+    evr_ = DeviceBuf(4 * wev);
+    quot_ = DeviceBuf(16 * N);
+    beta_ = DeviceBuf(16ull * air_.n_cols);
+    device_bytes_ += trace_.bytes + coef_.bytes + eval_.bytes + evr_.bytes + quot_.bytes + beta_.bytes;
+  } else {
+    trace_ = DeviceBuf(4 * n * W);
+    eval_ = DeviceBuf(4 * N * W);
+    device_bytes_ += trace_.bytes + eval_.bytes;
+  }
   if (d2) {
     quot_ = DeviceBuf(16 * N);
     std::vector<uint32_t> cd(2 * air_.n_cons), tab(8 * air_.n_cons, 0);
@@ -269,7 +301,66 @@ StarkProver::StarkProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_querie
   device_bytes_ += small_.bytes + proof_.bytes;
   for (auto& e : ev_) VX_HIP(hipEventCreate(&e));
   tab_ = ProverTables::get(log_n_, blowup_, stream_);
+  if (batch) build_batch();
   build_descs();
+}
+
+// Work lists, lifted column table, column offsets and the constant header
+// words of the batched-columns proof.
+void StarkProver::build_batch() {
+  const uint32_t C = air_.n_cols;
+  std::vector<LdeBatchCol> cols(C);
+  std::vector<uint64_t> off(C + 1, 0);
+  std::vector<const uint32_t*> src(C);
+  std::vector<uint32_t*> dst(C);
+  std::vector<uint32_t> elog(C);
+  std::vector<LiftCol> lift(C);
+  uint64_t oin = 0, oev = 0;
+  for (uint32_t c = 0; c < C; c++) {
+    const uint32_t L = air_.log_sizes[c], E = L + blowup_;
+    if (c == 0 || L != air_.log_sizes[c - 1]) {
+      col_tables_.push_back(L == log_n_ ? tab_ : ProverTables::get(L, blowup_, stream_));
+      if (L != log_n_) device_bytes_ += col_tables_.back()->bytes;
+    }
+    const ProverTables& t = *col_tables_.back();
+    cols[c] = {L, trace_.u32() + oin, coef_.u32() + oin, eval_.u32() + oev, t.trace_itw->ref, t.eval_tw->ref};
+    off[c] = oin;
+    src[c] = eval_.u32() + oev;
+    dst[c] = evr_.u32() + oev;
+    elog[c] = E;
+    lift[c] = {evr_.u32() + oev, E, 0};
+    oin += 1ull << L;
+    oev += 1ull << E;
+  }
+  off[C] = oin;
+  std::vector<uint8_t> jobs;
+  lde_batch_build(cols.data(), C, blowup_, jobs, lde_st_);
+  lde_jobs_ = DeviceBuf(jobs.size());
+  VX_HIP(hipMemcpy(lde_jobs_.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));
+  bitrev_batch_build(src.data(), dst.data(), elog.data(), C, jobs, br_count_, br_tiles_);
+  br_jobs_ = DeviceBuf(jobs.size());
+  VX_HIP(hipMemcpy(br_jobs_.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));
+  lift_ = DeviceBuf(sizeof(LiftCol) * C);
+  VX_HIP(hipMemcpy(lift_.p, lift.data(), sizeof(LiftCol) * C, hipMemcpyHostToDevice));
+  uint32_t sd[8];
+  b2_hash_long(air_.log_sizes.data(), 4 * C, sd);
+  size_digest_ = DeviceBuf(32);
+  VX_HIP(hipMemcpy(size_digest_.p, sd, 32, hipMemcpyHostToDevice));
+  fill_off_ = DeviceBuf(8ull * (C + 1));
+  VX_HIP(hipMemcpy(fill_off_.p, off.data(), 8ull * (C + 1), hipMemcpyHostToDevice));
+  device_bytes_ += lde_jobs_.bytes + br_jobs_.bytes + lift_.bytes + fill_off_.bytes;
+  // header words that never change: magic, version, C, blowup, pow, nonce
+  // (zero unless the query step writes it), C log sizes, layer counts
+  std::vector<uint32_t> hdr(layout_.header_words, 0);
+  hdr[0] = KRAK_MAGIC;
+  hdr[1] = 4;
+  hdr[2] = C;
+  hdr[3] = blowup_;
+  hdr[4] = pow_bits_;
+  for (uint32_t c = 0; c < C; c++) hdr[9 + c] = air_.log_sizes[c];
+  VX_HIP(hipMemcpy(proof_.p, hdr.data(), 4 * hdr.size(), hipMemcpyHostToDevice));
+  VX_HIP(hipMemcpy(proof_.u32() + layout_.w_nfri, &n_fri_, 4, hipMemcpyHostToDevice));
+  VX_HIP(hipMemcpy(proof_.u32() + layout_.w_nfdec, &n_fri_, 4, hipMemcpyHostToDevice));
 }
 
 StarkProver::~StarkProver() {
@@ -303,7 +394,14 @@ void StarkProver::build_descs() {
     x.lo = tree_lo_[t];
     x.tree = trees_[t].u32();
     for (int l = 0; l < MAX_LEVELS; l++) x.off[l] = tree_off_[t][l];
-    if (t < G) {  // trace group t
+    if (t < G && air_.kind == AIR_BATCH) {  // the lifted tree
+      x.kind = DECOM_LIFTED;
+      x.n_cols = air_.n_cols;
+      x.lift = static_cast<const LiftCol*>(lift_.p);
+      x.pair = PAIR_SIBLING;
+      x.shift = 0;
+      x.proof_word = layout_.w_tdec[t];
+    } else if (t < G) {  // trace group t
       x.kind = DECOM_COLS;
       x.n_cols = d2 ? d2_group_cols(air_.n_cols, t) : 1;
       for (uint32_t c = 0; c < x.n_cols; c++) x.col[c] = eval_.u32() + (D2_GROUP * t + c) * N;
@@ -311,7 +409,7 @@ void StarkProver::build_descs() {
       x.shift = 0;
       x.proof_word = layout_.w_tdec[t];
     } else if (t == G) {
-      if (d2) {
+      if (d2 || air_.kind == AIR_BATCH) {
         x.kind = DECOM_SOA4;
         for (int c = 0; c < 4; c++) x.col[c] = quot_.u32() + c * N;
       } else {
@@ -347,12 +445,42 @@ void StarkProver::record(bool timed, hipStream_t s) {
   auto alpha = [&](uint32_t i) { return d_alphas_ + 4 * i; };
   auto layer_col = [&](uint32_t k, int c) { return layers_[k].u32() + c * layer_col_words_[k]; };
 
-  const bool d2 = air_.kind == AIR_D2;
+  const bool d2 = air_.kind == AIR_D2, batch = air_.kind == AIR_BATCH;
   const uint32_t G = n_groups_;
   const uint32_t QT = G;  // quotient tree index; FRI layer k is tree G + 1 + k
   mark(0);
+  if (batch) {
+    // the header is constant (written at construction) apart from the seed
+    // (set_inputs) and the nonce (query step)
+    d2_fill_columns(d_inputs_, trace_.u32(), reinterpret_cast<const uint64_t*>(fill_off_.p), air_.n_cols, n_in_, s);
+    mark(1);
+    mark(2);
+    lde_batch_launch(static_cast<const uint8_t*>(lde_jobs_.p), lde_st_, s);
+    bitrev_batch_launch(static_cast<const uint8_t*>(br_jobs_.p), br_count_, br_tiles_, s);
+    mark(3);
+    LeafSrc lt{};
+    lt.kind = LEAF_LIFTED;
+    lt.lift = static_cast<const LiftCol*>(lift_.p);
+    lt.lift_n = air_.n_cols;
+    lt.lift_log = le_;
+    channel_step(d_channel_, true, size_digest_.u32(), nullptr, nullptr, s);  // sizes first
+    merkle_commit(lt, tree_levels(0), tree_lo_[0], RootHook{d_channel_, proof + L.w_troot[0], alpha(0), 0u}, s,
+                  timed ? ev_[10] : nullptr, timed ? ev_[11] : nullptr);
+    mark(4);
+    d2_powers(alpha(0), beta_.u32(), air_.n_cols, s);
+    LeafSrc lq{};
+    lq.kind = LEAF_LIFTED_RLC;
+    lq.lift = lt.lift;
+    lq.lift_n = air_.n_cols;
+    lq.lift_log = le_;
+    lq.beta = beta_.u32();
+    for (int c = 0; c < 4; c++) lq.out[c] = quot_.u32() + c * N;
+    merkle_commit(lq, tree_levels(QT), tree_lo_[QT], RootHook{d_channel_, proof + L.w_qroot, alpha(1), 0}, s);
+    mark(5);
+  }
   std::vector<uint32_t> hw;
-  if (!d2) {  // words 3, 4 (a, b) are written by set_inputs
+  if (batch) {
+  } else if (!d2) {  // words 3, 4 (a, b) are written by set_inputs
     hw = {0, KRAK_MAGIC, 1, pow_bits_ ? 2u : 1u, 2, log_n_};
     if (pow_bits_) hw.insert(hw.end(), {5, blowup_, 6, pow_bits_});
   } else {    // words 9, 10 (seed) by set_inputs, 7, 8 (nonce) by the query step or stay 0
@@ -360,9 +488,12 @@ void StarkProver::record(bool timed, hipStream_t s) {
           11, (uint32_t)air_.air_seed, 12, (uint32_t)(air_.air_seed >> 32), 13, G};
     if (!pow_bits_) hw.insert(hw.end(), {7, 0, 8, 0});
   }
-  hw.insert(hw.end(), {(uint32_t)L.w_nfri, n_fri_, (uint32_t)L.w_nfdec, n_fri_});
-  write_header(proof, hw.data(), (uint32_t)hw.size() / 2, s);
-  if (!d2) {
+  if (!batch) {
+    hw.insert(hw.end(), {(uint32_t)L.w_nfri, n_fri_, (uint32_t)L.w_nfdec, n_fri_});
+    write_header(proof, hw.data(), (uint32_t)hw.size() / 2, s);
+  }
+  if (batch) {
+  } else if (!d2) {
     fib_trace_dev(d_inputs_, trace_.u32(), log_n_, s);
     mark(1);
     ntt_inverse(trace_.u32(), tb.trace_itw->ref, s);
@@ -376,11 +507,11 @@ void StarkProver::record(bool timed, hipStream_t s) {
     mark(2);
     ntt_forward(eval_.u32(), trace_.u32(), n - 1, tb.eval_tw->ref, log_n_, s, W, N, n);
   }
-  mark(3);
+  if (!batch) mark(3);
 
   // trace commitment: one tree per group of <= 16 columns; the channel mixes
   // every root and draws the quotient challenge after the last
-  for (uint32_t g = 0; g < G; g++) {
+  for (uint32_t g = 0; g < G && !batch; g++) {
     LeafSrc lt{};
     lt.kind = LEAF_COLS;
     lt.n_cols = d2 ? d2_group_cols(air_.n_cols, g) : 1;
@@ -390,9 +521,10 @@ void StarkProver::record(bool timed, hipStream_t s) {
                   RootHook{d_channel_, proof + L.w_troot[g], last ? alpha(0) : nullptr, g == 0 ? 1u : 0u}, s,
                   timed && g == 0 ? ev_[10] : nullptr, timed && g == 0 ? ev_[11] : nullptr);
   }
-  mark(4);
+  if (!batch) mark(4);
 
-  if (!d2) {
+  if (batch) {
+  } else if (!d2) {
     LeafSrc lq{};
     lq.kind = LEAF_QUOTIENT;
     lq.col[0] = eval_.u32();
@@ -413,10 +545,10 @@ void StarkProver::record(bool timed, hipStream_t s) {
     for (int c = 0; c < 4; c++) lq.out[c] = quot_.u32() + c * N;
     merkle_commit(lq, tree_levels(QT), tree_lo_[QT], RootHook{d_channel_, proof + L.w_qroot, alpha(1), 0}, s);
   }
-  mark(5);
+  if (!batch) mark(5);
 
   LeafSrc lc{};
-  if (!d2) {  // circle fold of the quotient, recomputed from the eval column
+  if (!d2 && !batch) {  // circle fold of the quotient, recomputed from the eval column
     lc.kind = LEAF_FOLD_CIRCLE_Q;
     lc.col[0] = eval_.u32();
     lc.alpha_q = alpha(0);
@@ -510,10 +642,10 @@ void StarkProver::capture() {
 }
 
 void StarkProver::enqueue(uint32_t a, uint32_t b, bool timed) {
-  const bool d2 = air_.kind == AIR_D2;
-  require(d2 || (a < P && b < P), "prove: public inputs must be canonical M31");
+  const bool seeded = air_.kind != AIR_FIBONACCI;  // (a, b) = the trace seed's words
+  require(seeded || (a < P && b < P), "prove: public inputs must be canonical M31");
   timed_ = timed;
-  set_inputs(d_inputs_, proof_.u32(), d2 ? 9 : 3, a, b, stream_);
+  set_inputs(d_inputs_, proof_.u32(), air_.kind == AIR_D2 ? 9 : (air_.kind == AIR_BATCH ? 7 : 3), a, b, stream_);
   if (timed || !use_graph_) {
     record(timed, stream_);
   } else {

@@ -29,12 +29,16 @@ struct DeviceBuf {
   uint32_t* u32() const { return static_cast<uint32_t*>(p); }
 };
 
-enum AirKind : uint32_t { AIR_FIBONACCI = 0, AIR_D2 = 1 };
+// AIR_BATCH: the batched-columns proof (KRAK v4): columns of uneven log
+// sizes (sorted ascending, 14..24), bit-reversed LDE, one lifted tree, a
+// random-linear-combination quotient, then the Fibonacci FRI.
+enum AirKind : uint32_t { AIR_FIBONACCI = 0, AIR_D2 = 1, AIR_BATCH = 2 };
 struct AirSpec {
   uint32_t kind = AIR_FIBONACCI;
-  uint32_t n_cols = 1;    // AIR_D2: trace columns W
+  uint32_t n_cols = 1;    // AIR_D2: trace columns W; AIR_BATCH: columns
   uint32_t n_cons = 1;    // AIR_D2: constraints K
   uint64_t air_seed = 0;  // AIR_D2: constraint coefficient seed
+  std::vector<uint32_t> log_sizes;  // AIR_BATCH: per-column log sizes
 };
 
 // Word offsets of every variable field of a proof: KRAK v1/v2
@@ -115,6 +119,7 @@ class StarkProver {
 
  private:
   void build_descs();
+  void build_batch();
   void record(bool timed, hipStream_t s);
   void capture();
   TreeLevels tree_levels(uint32_t idx) const;
@@ -129,6 +134,13 @@ class StarkProver {
   std::shared_ptr<const ProverTables> tab_;
   DeviceBuf trace_, eval_;         // W columns of 2^log_n / 2^le (W = 1 for Fibonacci)
   DeviceBuf quot_, cons_, beta_;   // AIR_D2: stored quotient (SoA), constraint table, challenge powers
+  // AIR_BATCH: coefficients, bit-reversed LDE, work lists, lifted columns,
+  // column offsets (trace_ holds the inputs, eval_ the natural-order LDE)
+  DeviceBuf coef_, evr_, lde_jobs_, br_jobs_, lift_, fill_off_, size_digest_;
+  std::vector<std::shared_ptr<const ProverTables>> col_tables_;
+  LdeBatchStages lde_st_{};
+  uint32_t br_count_ = 0;
+  uint64_t br_tiles_ = 0, n_in_ = 0;
   std::vector<DeviceBuf> layers_;  // FRI layer k: 4 SoA columns of 2^(le-1-k)
   std::vector<uint64_t> layer_col_words_;
   std::vector<DeviceBuf> trees_;   // g < G trace groups, G = quotient, G + 1 + k = FRI layer k

@@ -280,6 +280,75 @@ bool verify_d2(Reader& r, uint32_t blowup_bits, uint32_t nq, std::string& err) {
   return true;
 }
 
+// KRAK v4: the batched-columns proof (lifted tree of C columns, RLC
+// quotient). The trace opening of a query is every column at its lifted row,
+// for q and q ^ 1; the leaf is their multi-block Blake2s; the quotient at
+// both rows must equal sum_c alpha^(c+1) value_c.
+bool verify_batch(Reader& r, uint32_t blowup_bits, uint32_t nq, std::string& err) {
+  auto fail = [&](const std::string& m) {
+    err = m;
+    return false;
+  };
+  const uint32_t C = r.u32(), bb = r.u32(), pow_bits = r.u32();
+  uint64_t nonce = r.u32();
+  nonce |= (uint64_t)r.u32() << 32;
+  (void)r.u32();  // trace seed: public metadata
+  (void)r.u32();
+  if (r.bad || C < 1 || C > 4096) return fail("bad column count");
+  if (bb != blowup_bits) return fail("blowup mismatch");
+  if (pow_bits > 128) return fail("bad pow bits");
+  std::vector<uint32_t> logs(C);
+  for (uint32_t c = 0; c < C; c++) {
+    logs[c] = r.u32();
+    if (r.bad || logs[c] < 1 || logs[c] + bb > 31 || (c && logs[c] < logs[c - 1])) return fail("bad column sizes");
+  }
+  const uint32_t S = logs[C - 1] + bb;
+  uint32_t troot[8];
+  for (auto& w : troot) w = r.u32();
+  FriPart f;
+  if (!read_fri_head(r, f)) return fail("truncated proof");
+  Decom td;
+  if (!read_decom(r, td, C) || !read_fri_tail(r, f)) return fail("truncated proof");
+  if (r.pos != r.len) return fail("trailing bytes");
+  if (S < 4) return fail("bad log size");
+  if (f.n_q != nq) return fail("Expected " + std::to_string(nq) + " queries, got " + std::to_string(f.n_q));
+  Channel ch;
+  ch_init(ch);
+  uint32_t sd[8];
+  b2_hash_long(logs.data(), 4 * C, sd);  // the sizes enter the transcript first
+  ch_mix_digest(ch, sd);
+  ch_mix_digest(ch, troot);
+  const qm31 alpha = ch_draw_felt(ch);
+  if (!pow_bits && nonce) return fail("nonce without PoW");
+  std::string e = replay_fri(ch, f, S, nq, pow_bits != 0, pow_bits, nonce);
+  if (!e.empty()) return fail(e);
+  if (td.n() != nq || td.paths.size() != nq) return fail("trace decommitment size mismatch");
+  for (uint32_t q = 0; q < nq; q++) {
+    uint32_t h[8];
+    b2_hash_long(td.vals.data() + (size_t)q * C, 4 * C, h);
+    if (td.paths[q].size() != 8ull * S || !check_path(troot, h, f.qidx[q], td.paths[q]))
+      return fail("trace auth path invalid");
+    b2_hash_long(td.sibs.data() + (size_t)q * C, 4 * C, h);
+    if (td.spaths[q].size() != 8ull * S || !check_path(troot, h, f.qidx[q] ^ 1, td.spaths[q]))
+      return fail("trace auth path invalid");
+  }
+  if (!check_decom(f.qd, f.qroot, f.qidx, nq)) return fail("quotient auth path invalid");
+  for (uint32_t q = 0; q < nq; q++)
+    for (int side = 0; side < 2; side++) {
+      const uint32_t* v = (side ? td.sibs : td.vals).data() + (size_t)q * C;
+      qm31 acc{{0, 0, 0, 0}}, beta = alpha;
+      for (uint32_t c = 0; c < C; c++) {
+        acc = qm_add(acc, qm_scale(beta, v[c]));
+        beta = qm_mul(beta, alpha);
+      }
+      if (!qm_eq(acc, q_at(side ? f.qd.sibs : f.qd.vals, q)))
+        return fail("Quotient combination mismatch at query " + std::to_string(q));
+    }
+  e = check_fri_layers(f, S, nq);
+  if (!e.empty()) return fail(e);
+  return true;
+}
+
 }  // namespace
 
 bool verify_proof(const uint8_t* bytes, size_t len, uint32_t blowup_bits, uint32_t nq, std::string& err) {
@@ -292,6 +361,7 @@ bool verify_proof(const uint8_t* bytes, size_t len, uint32_t blowup_bits, uint32
   if (r.u32() != 0x4B52414Bu) return fail("bad magic");
   const uint32_t version = r.u32();
   if (version == 3) return verify_d2(r, blowup_bits, nq, err);
+  if (version == 4) return verify_batch(r, blowup_bits, nq, err);
   if (version != 1 && version != 2) return fail("unsupported proof version " + std::to_string(version));
   const uint32_t log_n = r.u32(), a = r.u32(), b = r.u32();
   uint32_t pow_bits = 0;

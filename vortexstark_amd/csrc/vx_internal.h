@@ -133,6 +133,7 @@ enum LeafKind : uint32_t {
   LEAF_FOLD_LINE = 3,     // line fold of a QM31 SoA layer (cuda/fri.cu:58-87), writes the layer
   LEAF_D2_QUOTIENT = 4,   // synthetic AIR quotient of W eval columns (vx_d2.h), writes Q (SoA)
   LEAF_LIFTED = 5,        // lifted multi-size leaf (cuda/merkle_leaves_lifted.cu:1-98)
+  LEAF_LIFTED_RLC = 6,    // random linear combination of the lifted rows, writes Q (SoA)
 };
 // A column of a lifted tree: 2^log_n values, log_n <= the tree's log size.
 struct LiftCol {
@@ -214,7 +215,7 @@ void merkle_lifted_leaves(const LeafSrc& leaf, uint32_t* hashes, hipStream_t s);
 // QM31 SoA layer, DECOM_QUOTIENT = the Fibonacci quotient recomputed from
 // the eval column. The partner leaf of query leaf i is i ^ 1 (PAIR_SIBLING,
 // the KRAK layout) or (i + 1) mod 2^log_n (PAIR_NEXT, next-row openings).
-enum DecomKind : uint32_t { DECOM_COLS = 0, DECOM_SOA4 = 1, DECOM_QUOTIENT = 2 };
+enum DecomKind : uint32_t { DECOM_COLS = 0, DECOM_SOA4 = 1, DECOM_QUOTIENT = 2, DECOM_LIFTED = 3 };
 enum DecomPair : uint32_t { PAIR_SIBLING = 0, PAIR_NEXT = 1 };
 struct DecomDesc {
   uint32_t kind;
@@ -225,6 +226,7 @@ struct DecomDesc {
   uint32_t shift;        // leaf index = query >> shift
   const uint32_t* col[16];
   const uint32_t* alpha; // DECOM_QUOTIENT
+  const LiftCol* lift;   // DECOM_LIFTED: n_cols columns, tree of 2^log_n leaves
   const uint32_t* tree;
   uint64_t off[MAX_LEVELS];
   uint64_t proof_word;   // word offset of this decommitment ("n") in the proof
@@ -263,5 +265,7 @@ void d2_trace(const uint32_t* seed_words, uint32_t* out, uint32_t n_cols, uint32
               hipStream_t s);
 void d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons, hipStream_t s);
 void d2_trace_col(const uint32_t* seed_words, uint32_t* out, uint32_t col, uint32_t log_n, hipStream_t s);
+void d2_fill_columns(const uint32_t* seed_words, uint32_t* out, const uint64_t* d_off, uint32_t n_cols,
+                     uint64_t total, hipStream_t s);
 
 }  // namespace vx
