@@ -103,4 +103,28 @@ void d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons, hipStream
   VX_HIP(hipGetLastError());
 }
 
+// Quotient weights of the K constraints: w[8k..8k+3] = beta_k c_k and
+// w[8k+4..8k+7] = beta_k d_k with beta_k = alpha^(k+1) and (c_k, d_k) from the
+// constraint table (8 words per constraint, c and d first). One lane.
+__global__ void k_d2_weights(const uint32_t* alpha, const uint32_t* cons, uint32_t* w, uint32_t k_cons) {
+  if (threadIdx.x != 0) return;
+  const qm31 a = {{alpha[0], alpha[1], alpha[2], alpha[3]}};
+  qm31 p = a;
+  for (uint32_t k = 0; k < k_cons; k++) {
+    const qm31 g = qm_scale(p, cons[8 * k]), d = qm_scale(p, cons[8 * k + 1]);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      w[8 * k + c] = g.v[c];
+      w[8 * k + 4 + c] = d.v[c];
+    }
+    p = qm_mul(p, a);
+  }
+}
+
+void d2_weights(const uint32_t* alpha, const uint32_t* cons, uint32_t* w, uint32_t k_cons, hipStream_t s) {
+  require(k_cons >= 1 && k_cons <= D2_MAX_CONS, "d2_weights: bad count");
+  hipLaunchKernelGGL(k_d2_weights, dim3(1), dim3(64), 0, s, alpha, cons, w, k_cons);
+  VX_HIP(hipGetLastError());
+}
+
 }  // namespace vx

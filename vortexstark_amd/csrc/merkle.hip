@@ -24,6 +24,7 @@
 // needed. FRI layers of <= 2^10 values are committed by one workgroup in a
 // single launch (k_fri_tail). Blake2s is VALU-bound here.
 #include "vx_blake2s_x4.h"
+#include "vx_d2.h"
 #include "vx_internal.h"
 
 namespace vx {
@@ -160,38 +161,61 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
   }
 }
 
-// Synthetic AIR quotient of rows [i, i+4) (vx_d2.h): Q(r) = sum_k beta_k
-// (c_k t_a[r] t_b[r+1] + d_k t_e[r]); the constraint table and beta are
-// wave-uniform (scalar loads), the rows are one 16-byte load per column use
-// plus the wrapped row i + 4 of column b. Q is stored (SoA) for the fold.
+// Synthetic AIR quotient of rows [i, i+4) (vx_d2.h):
+//   Q(r) = sum_k gamma_k t_a[r] t_b[r+1] + delta_k t_e[r],
+// gamma_k = beta_k c_k and delta_k = beta_k d_k (QM31, precomputed once per
+// proof by d2_weights). Each QM31 component is a dot product accumulated in
+// 64 bits without reduction: every product is below P^2, an accumulator
+// folded to lo + 2 hi (below 3 * 2^32, congruent mod P since 2^32 = 2 mod P)
+// takes four more products below 2^64, so it is folded after every pair of
+// constraints and reduced once at the end. The constraint table and weights
+// are wave-uniform (scalar loads); the rows are one 16-byte load per column
+// use plus the wrapped row i + 4 of column b. Q is stored (SoA) for the fold.
+__device__ __forceinline__ uint64_t fold64(uint64_t x) { return (x & 0xFFFFFFFFull) + ((x >> 32) << 1); }
+
 __device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g) {
   const uint32_t* ev = L.col[0];
   const uint64_t st = L.col_stride;
   const uint64_t i4 = (i + 4) & L.src_mask;
-  qm31 acc[4];
+  const uint32_t K = L.n_cons;
+  uint64_t acc[4][4];
 #pragma unroll
-  for (int j = 0; j < 4; j++) acc[j] = qm31{{0, 0, 0, 0}};
-  for (uint32_t k = 0; k < L.n_cons; k++) {
-    const uint32_t* t = L.cons + 8 * k;
-    const uint32_t c = t[0], d = t[1];
-    const uint32_t* ca = ev + t[2] * st;
-    const uint32_t* cb = ev + t[3] * st;
-    const uint32_t* ce = ev + t[4] * st;
-    const uint4 va = ld4(ca + i), vb = ld4(cb + i), ve = ld4(ce + i);
-    const uint32_t b4 = cb[i4];
-    const qm31 be = ldq(L.beta + 4 * k);
-    const uint32_t A[4] = {va.x, va.y, va.z, va.w}, B[5] = {vb.x, vb.y, vb.z, vb.w, b4};
-    const uint32_t E[4] = {ve.x, ve.y, ve.z, ve.w};
+  for (int j = 0; j < 4; j++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t m = m31_add(m31_mul(c, m31_mul(A[j], B[j + 1])), m31_mul(d, E[j]));
-      acc[j] = qm_add(acc[j], qm_scale(be, m));
+    for (int c = 0; c < 4; c++) acc[j][c] = 0;
+#pragma unroll 2
+  for (uint32_t k0 = 0; k0 < K; k0 += 2) {
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+      const uint32_t k = k0 + h;
+      if (k < K) {
+        const uint32_t* t = L.cons + 8 * k;
+        const uint4 wg = ld_uniform(reinterpret_cast<const uint4*>(L.beta + 8 * k));
+        const uint4 wd = ld_uniform(reinterpret_cast<const uint4*>(L.beta + 8 * k + 4));
+        const uint32_t w[8] = {wg.x, wg.y, wg.z, wg.w, wd.x, wd.y, wd.z, wd.w};
+        const uint32_t* cb = ev + ld_uniform(t + 3) * st;
+        const uint4 va = ld4(ev + ld_uniform(t + 2) * st + i), vb = ld4(cb + i);
+        const uint4 ve = ld4(ev + ld_uniform(t + 4) * st + i);
+        const uint32_t b4 = cb[i4];
+        const uint32_t A[4] = {va.x, va.y, va.z, va.w}, B[5] = {vb.x, vb.y, vb.z, vb.w, b4};
+        const uint32_t E[4] = {ve.x, ve.y, ve.z, ve.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint64_t ab = m31_mul(A[j], B[j + 1]);
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[j][c] += (uint64_t)w[c] * ab + (uint64_t)w[4 + c] * E[j];
+        }
+      }
     }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[j][c] = fold64(acc[j][c]);
   }
 #pragma unroll
   for (int j = 0; j < 4; j++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) g.w[j][c] = acc[j].v[c];
+    for (int c = 0; c < 4; c++) g.w[j][c] = d2_mod_p(acc[j][c]);
 #pragma unroll
   for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
 }
@@ -214,8 +238,8 @@ __device__ __forceinline__ void lifted_leaf_cols(const LiftCol* cols, uint32_t n
       const uint32_t c = base + j;
       uint32_t v = 0;
       if (c < n) {
-        const LiftCol col = cols[c];
-        v = col.p[lifted_row(idx, S, col.log_n)];
+        const LiftCol col = ld_uniform(cols + c);
+        v = as_global(col.p)[lifted_row(idx, S, col.log_n)];
       }
       m[j] = v;
     }
@@ -230,48 +254,76 @@ __device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint
 // Random linear combination of the lifted rows of leaves [i, i+4):
 // Q(r) = sum_c beta_c col_c[lifted_row(r)]. A column of the tree's size
 // gives rows i..i+3 (one 16-byte load); a smaller one gives rows
-// b, b+1, b, b+1 with b = (i >> shift) << 1 (one 8-byte load). Q is stored
-// (SoA) for the fold.
+// b, b+1, b, b+1 with b = (i >> shift) << 1 (one 8-byte load). The QM31
+// components accumulate in 64 bits as in d2_produce (fold after every four
+// columns); four columns' loads are in flight together. Q is stored (SoA)
+// for the fold.
 __device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i, G4& g) {
-  const uint32_t S = L.lift_log;
-  qm31 acc[4];
+  const uint32_t S = L.lift_log, n = L.lift_n;
+  uint64_t acc[4][4];
 #pragma unroll
-  for (int j = 0; j < 4; j++) acc[j] = qm31{{0, 0, 0, 0}};
-  for (uint32_t c = 0; c < L.lift_n; c++) {
-    const LiftCol col = L.lift[c];
-    const qm31 be = ldq(L.beta + 4 * c);
-    uint32_t v[4];
-    if (col.log_n == S) {
-      u4w(ld4(col.p + i), v);
-    } else {
-      const uint64_t b = (i >> (S - col.log_n + 1)) << 1;
-      const uint2 x = *reinterpret_cast<const uint2*>(col.p + b);
-      v[0] = v[2] = x.x;
-      v[1] = v[3] = x.y;
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[j][c] = 0;
+  for (uint32_t c0 = 0; c0 < n; c0 += 4) {
+#pragma unroll
+    for (uint32_t h = 0; h < 4; h++) {
+      const uint32_t c = c0 + h;
+      if (c < n) {
+        const LiftCol col = ld_uniform(L.lift + c);
+        const uint4 bq = ld_uniform(reinterpret_cast<const uint4*>(L.beta + 4 * c));
+        const uint32_t be[4] = {bq.x, bq.y, bq.z, bq.w};
+        uint32_t v[4];
+        if (col.log_n == S) {
+          u4w(ld4_global(col.p + i), v);
+        } else {
+          const uint64_t b = (i >> (S - col.log_n + 1)) << 1;
+          const uint2 x = ld2_global(col.p + b);
+          v[0] = v[2] = x.x;
+          v[1] = v[3] = x.y;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int k = 0; k < 4; k++) acc[j][k] += (uint64_t)be[k] * v[j];
+      }
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[j] = qm_add(acc[j], qm_scale(be, v[j]));
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[j][k] = fold64(acc[j][k]);
   }
 #pragma unroll
   for (int j = 0; j < 4; j++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) g.w[j][c] = acc[j].v[c];
+    for (int c = 0; c < 4; c++) g.w[j][c] = d2_mod_p(acc[j][c]);
 #pragma unroll
   for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
 }
 
+// Rows [i, i+4) of up to 16 columns, one 16-byte load per column, all
+// issued before the first of the four leaves is hashed.
+struct GCols {
+  uint4 c[16];
+};
+__device__ __forceinline__ void cols_load(const LeafSrc& L, uint64_t i, GCols& gc) {
+#pragma unroll
+  for (uint32_t c = 0; c < 16; c++) gc.c[c] = c < L.n_cols ? ld4(L.col[c] + i) : make_uint4(0, 0, 0, 0);
+}
+
 template <uint32_t KIND, uint32_t NC>
-__device__ __forceinline__ void leaf_hash(const LeafSrc& L, const G4& g, uint64_t i, int j, uint32_t h[8]) {
+__device__ __forceinline__ void leaf_hash(const LeafSrc& L, const G4& g, const GCols& gc, uint64_t i, int j,
+                                          uint32_t h[8]) {
   if constexpr (KIND == LEAF_LIFTED) {
     lifted_leaf(L, i + j, h);
   } else if constexpr (KIND == LEAF_COLS && NC == 1) {
     b2_leaf1(g.w[j][0], h);
   } else if constexpr (KIND == LEAF_COLS && NC == 0) {
-    // generic 1..16 columns, read per leaf
+    // generic 1..16 columns: row i + j of each
     uint32_t w[16];
 #pragma unroll
-    for (uint32_t c = 0; c < 16; c++) w[c] = c < L.n_cols ? L.col[c][i + j] : 0u;
-    b2_hash_words(w, L.n_cols > 16 ? 16 : L.n_cols, h);
+    for (uint32_t c = 0; c < 16; c++) w[c] = j == 0 ? gc.c[c].x : j == 1 ? gc.c[c].y : j == 2 ? gc.c[c].z : gc.c[c].w;
+    b2_hash_words(w, L.n_cols, h);
   } else {
     b2_leaf4(g.w[j][0], g.w[j][1], g.w[j][2], g.w[j][3], h);
   }
@@ -301,18 +353,20 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
   for (uint32_t gi = 0; gi < G; gi++) {
     const uint64_t i = first + 4 * gi;
     G4 g;
+    GCols gc;
     if constexpr (!kDirect) {
       if (gi + 1 < G) leaf_load<KIND, NC>(L, i + 4, nxt);
       leaf_produce<KIND, NC>(L, i, cur, aq, af, g);
     }
+    if constexpr (kGeneric) cols_load(L, i, gc);
     if constexpr (KIND == LEAF_D2_QUOTIENT) d2_produce(L, i, g);
     if constexpr (KIND == LEAF_LIFTED_RLC) lifted_rlc_produce(L, i, g);
     uint32_t a[8], b[8], n1a[8], n1b[8], n2[8];
-    leaf_hash<KIND, NC>(L, g, i, 0, a);
-    leaf_hash<KIND, NC>(L, g, i, 1, b);
+    leaf_hash<KIND, NC>(L, g, gc, i, 0, a);
+    leaf_hash<KIND, NC>(L, g, gc, i, 1, b);
     b2_hash_node(a, b, n1a);
-    leaf_hash<KIND, NC>(L, g, i, 2, a);
-    leaf_hash<KIND, NC>(L, g, i, 3, b);
+    leaf_hash<KIND, NC>(L, g, gc, i, 2, a);
+    leaf_hash<KIND, NC>(L, g, gc, i, 3, b);
     b2_hash_node(a, b, n1b);
     b2_hash_node(n1a, n1b, n2);
     if constexpr (LPT_LOG == 2) {
@@ -507,7 +561,9 @@ static void check_leaf(const LeafSrc& L, uint32_t log_n) {
   switch (L.kind) {
     case LEAF_COLS:
       require(L.n_cols >= 1 && L.n_cols <= 16, "merkle: n_cols must be 1..16");
-      for (uint32_t c = 0; c < L.n_cols; c++) require(L.col[c] != nullptr, "merkle: null column");
+      for (uint32_t c = 0; c < L.n_cols; c++)
+        require(L.col[c] != nullptr && (reinterpret_cast<uintptr_t>(L.col[c]) & 15u) == 0,
+                "merkle: columns must be non-null and 16-byte aligned");
       break;
     case LEAF_QUOTIENT:
       require(L.col[0] && L.alpha_q && L.src_mask == (1ull << log_n) - 1, "merkle: bad quotient source");

@@ -156,8 +156,8 @@ __device__ __forceinline__ void rx_apply(uint32_t (&v)[32], const uint32_t (&w)[
 
 // One 16384-element tile of a pass; `a` may live in kernel arguments or in
 // global memory (work-list jobs), `lds` holds 2^14 words.
-template <bool FWD>
-__device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, const RadixArgs& a, uint64_t tile,
+template <bool FWD, typename Args>
+__device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, const Args& a, uint64_t tile,
                                            uint32_t* lds) {
   const uint32_t t = threadIdx.x;
   const uint32_t cmask = (1u << a.cbits) - 1;
@@ -167,8 +167,7 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   const uint64_t hi_base = H << (a.j0 + a.k);
   uint32_t v[32];
   for (uint32_t ri = 0; ri < a.n_rounds; ri++) {
-    const RadixRound R = a.round[ri];
-    const uint32_t pos = R.pos;
+    const uint32_t pos = a.round[ri].pos, r_rb0 = a.round[ri].rb0, r_nb = a.round[ri].nb;
     // A contiguous pass (cbits = 0) whose round keeps each lane's 32 elements
     // next to each other (pos < 4) would read or write HBM with a 128-byte
     // stride between lanes; it goes through LDS instead, so that every
@@ -198,7 +197,7 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) v[s] = lds[rx_swz(rx_own(t, s, pos))];
     }
-    const uint32_t qlo = R.rb0 + a.cbits - pos, qhi = qlo + R.nb;  // register bits to butterfly
+    const uint32_t qlo = r_rb0 + a.cbits - pos, qhi = qlo + r_nb;  // register bits to butterfly
     // all of the round's twiddles are requested before the first butterfly,
     // so their latencies overlap (one wait per round instead of one per bit)
     uint32_t w4[1], w3[2], w2[4], w1[8], w0[16];
@@ -271,10 +270,10 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix_list(const RadixJob* j
   uint32_t lo = 0, hi = n_jobs;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (jobs[mid].first_tile <= b) lo = mid;
+    if (ld_uniform(&jobs[mid].first_tile) <= b) lo = mid;
     else hi = mid;
   }
-  const RadixJob& J = jobs[lo];
+  uniform_t<RadixJob>& J = as_uniform(jobs + lo);
   radix_tile<FWD>(J.dst, J.src, J.a, b - J.first_tile, lds);
 }
 
@@ -499,10 +498,10 @@ __global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint
   uint32_t lo = 0, hi = n_jobs;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (jobs[mid].first_tile <= b) lo = mid;
+    if (ld_uniform(&jobs[mid].first_tile) <= b) lo = mid;
     else hi = mid;
   }
-  const BitrevJob& J = jobs[lo];
+  const BitrevJob J = ld_uniform(jobs + lo);
   const uint32_t E = J.log_n, mb = E - 2 * BR_T;
   const uint64_t m = b - J.first_tile;
   const uint64_t rm = mb ? (__brevll(m) >> (64 - mb)) : 0;

@@ -259,7 +259,7 @@ StarkProver::StarkProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_querie
     }
     cons_ = DeviceBuf(4 * tab.size());
     VX_HIP(hipMemcpy(cons_.p, tab.data(), 4 * tab.size(), hipMemcpyHostToDevice));
-    beta_ = DeviceBuf(16ull * air_.n_cons);
+    beta_ = DeviceBuf(32ull * air_.n_cons);
     device_bytes_ += quot_.bytes + cons_.bytes + beta_.bytes;
   }
   for (uint32_t k = 0; k < n_fri_; k++) {
@@ -532,7 +532,7 @@ void StarkProver::record(bool timed, hipStream_t s) {
     lq.src_mask = N - 1;
     merkle_commit(lq, tree_levels(QT), tree_lo_[QT], RootHook{d_channel_, proof + L.w_qroot, alpha(1), 0}, s);
   } else {
-    d2_powers(alpha(0), beta_.u32(), air_.n_cons, s);
+    d2_weights(alpha(0), cons_.u32(), beta_.u32(), air_.n_cons, s);
     LeafSrc lq{};
     lq.kind = LEAF_D2_QUOTIENT;
     lq.col[0] = eval_.u32();

@@ -35,6 +35,41 @@ inline void require(bool cond, const char* what) {
 // Last error message of the calling thread (C-ABI vx_last_error).
 void set_last_error(const std::string& msg);
 
+// Read of wave-uniform data that no launch modifies while it runs (job
+// lists, constraint and column tables): through the constant address space,
+// so the compiler issues scalar (SMEM) loads and keeps the values in SGPRs
+// instead of one vector load per lane, each a full memory round trip.
+template <typename T>
+using uniform_t = const __attribute__((address_space(4))) T;
+template <typename T>
+__device__ __forceinline__ uniform_t<T>& as_uniform(const T* p) {
+  return *(uniform_t<T>*)p;
+}
+// A pointer known to address global memory (e.g. one read from a table),
+// so its loads are global rather than flat loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ uint4 ld4_global(const uint32_t* p) {
+  const __attribute__((address_space(1))) uint32_t* q = as_global(p);
+  return make_uint4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ uint2 ld2_global(const uint32_t* p) {
+  const __attribute__((address_space(1))) uint32_t* q = as_global(p);
+  return make_uint2(q[0], q[1]);
+}
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "ld_uniform: whole words only");
+  T r;
+  uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+  uniform_t<uint32_t>* q = (uniform_t<uint32_t>*)p;
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; i++) d[i] = q[i];
+  return r;
+}
+
 // --------------------------------------------------------- NTT (ntt.hip)
 // Reference-layout twiddle tables of a coset (src/circle.rs:284-347):
 //   line: layer l occupies [off[l], off[l] + (N >> (l+1))), circle: N/2 y-coords.
@@ -153,7 +188,8 @@ struct LeafSrc {
   uint64_t src_mask;         // eval length - 1 (quotient wraparound)
   // LEAF_D2_QUOTIENT: col[0] = first of n_cols eval columns, col_stride
   // words apart; per constraint 8 words {c, d, a, b, e, 0, 0, 0}; beta =
-  // n_cons QM31 powers of the challenge
+  // per constraint 8 words {beta_k c_k, beta_k d_k} (QM31 each, d2_weights).
+  // LEAF_LIFTED_RLC: beta = lift_n QM31 powers of the challenge
   uint64_t col_stride;
   uint32_t n_cons;
   const uint32_t* cons;
@@ -264,6 +300,7 @@ void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_
 void d2_trace(const uint32_t* seed_words, uint32_t* out, uint32_t n_cols, uint32_t log_n, uint64_t stride,
               hipStream_t s);
 void d2_powers(const uint32_t* alpha, uint32_t* beta, uint32_t k_cons, hipStream_t s);
+void d2_weights(const uint32_t* alpha, const uint32_t* cons, uint32_t* w, uint32_t k_cons, hipStream_t s);
 void d2_trace_col(const uint32_t* seed_words, uint32_t* out, uint32_t col, uint32_t log_n, hipStream_t s);
 void d2_fill_columns(const uint32_t* seed_words, uint32_t* out, const uint64_t* d_off, uint32_t n_cols,
                      uint64_t total, hipStream_t s);
