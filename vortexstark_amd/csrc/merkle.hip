@@ -616,7 +616,18 @@ void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, con
                    hipEvent_t sub_begin, hipEvent_t sub_end) {
   check_leaf(L, T.log_n);
   require(T.base != nullptr, "merkle_commit: null tree buffer");
-  const uint32_t tile_log = T.log_n >= BIG_TREE_LOG ? MAX_TILE_LOG : (T.log_n < 10 ? T.log_n : 10);
+  // Leaves read from several columns (and the FRI folds, which read two
+  // values of four columns per leaf) keep 4 leaves per lane at every size:
+  // lane t then reads rows 4t..4t+3, so each 16-byte load of a wavefront is
+  // one contiguous run. With 16 leaves per lane a wavefront's load would
+  // touch 64 lines at a 256-byte (folds: 512-byte) stride and the rest of
+  // each line is used only after the lane's next hash groups, by which time
+  // L2 has evicted it (measured: line folds fetched each byte about 4 times).
+  const bool multi = L.kind == LEAF_D2_QUOTIENT || L.kind == LEAF_LIFTED_RLC || L.kind == LEAF_LIFTED ||
+                     (L.kind == LEAF_COLS && L.n_cols > 1) || L.kind == LEAF_FOLD_LINE ||
+                     L.kind == LEAF_FOLD_CIRCLE_Q;
+  const uint32_t tile_log =
+      T.log_n >= BIG_TREE_LOG && !multi ? MAX_TILE_LOG : (T.log_n < 10 ? T.log_n : 10);
   const uint32_t lpt = lane_log_for(tile_log);
   require(store_lo >= T.lo && store_lo >= lpt && store_lo <= tile_log,
           "merkle_commit: stored levels must start at or above the lane level");
