@@ -706,7 +706,7 @@ int vx_merkle_root_lifted(const uint32_t* const* h_cols, const uint32_t* h_log_s
     DeviceBuf table;
     LeafSrc L = lifted_src(h_cols, h_log_sizes, n_cols, lg, table);
     // only the levels from the tile size up are stored (the tile roots and above)
-    const uint32_t tile = lg >= 21 ? 12 : (lg < 10 ? lg : 10);
+    const uint32_t tile = merkle_tile_log(L, lg);
     TreeLevels T{};
     uint64_t w = 0;
     T.log_n = lg;

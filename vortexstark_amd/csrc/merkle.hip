@@ -24,6 +24,7 @@
 // needed. FRI layers of <= 2^10 values are committed by one workgroup in a
 // single launch (k_fri_tail). Blake2s is VALU-bound here.
 #include "vx_blake2s_x4.h"
+#include "vx_channel_x4.h"
 #include "vx_d2.h"
 #include "vx_internal.h"
 
@@ -31,10 +32,11 @@ namespace vx {
 
 static constexpr uint32_t SUB_THREADS = 256;
 static constexpr uint32_t BIG_TREE_LOG = 21;  // >= this: 16 leaves per lane, 4096-leaf tiles
+static constexpr uint32_t SMALL_TREE_LOG = 16;  // <= this: 1 leaf per lane, 256-leaf tiles
 static constexpr uint32_t MAX_TILE_LOG = 12;
 static constexpr uint32_t RED_THREADS = 512;  // tree-top reduce: 1024 nodes, 10 levels per workgroup
 static constexpr uint32_t RED_NLEV_MAX = 10;
-static constexpr uint32_t TAIL_THREADS = 256;
+static constexpr uint32_t TAIL_THREADS = 1u << TAIL_MAX_LOG;  // one lane per leaf of the largest tail layer
 
 struct G4 {
   uint32_t w[4][4];  // words of 4 consecutive leaves (W <= 4)
@@ -97,7 +99,7 @@ __device__ __forceinline__ void leaf_load(const LeafSrc& L, uint64_t i, Raw& r) 
 // the new layer).
 template <uint32_t KIND, uint32_t NC>
 __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const Raw& r, const qm31& aq,
-                                             const qm31& af, G4& g) {
+                                             const qm31& af, G4& g, bool st = true) {
   if constexpr (KIND == LEAF_COLS && NC == 1) {
     g.w[0][0] = r.a[0].x;
     g.w[1][0] = r.a[0].y;
@@ -138,8 +140,10 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
 #pragma unroll
       for (int c = 0; c < 4; c++) g.w[j][c] = o.v[c];
     }
+    if (st) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+      for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+    }
   } else if constexpr (KIND == LEAF_FOLD_LINE) {
     uint32_t in[4][8], t[4];
 #pragma unroll
@@ -156,8 +160,10 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
 #pragma unroll
       for (int c = 0; c < 4; c++) g.w[j][c] = o.v[c];
     }
+    if (st) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+      for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+    }
   }
 }
 
@@ -173,7 +179,7 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
 // use plus the wrapped row i + 4 of column b. Q is stored (SoA) for the fold.
 __device__ __forceinline__ uint64_t fold64(uint64_t x) { return (x & 0xFFFFFFFFull) + ((x >> 32) << 1); }
 
-__device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g) {
+__device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g, bool store = true) {
   const uint32_t* ev = L.col[0];
   const uint64_t st = L.col_stride;
   const uint64_t i4 = (i + 4) & L.src_mask;
@@ -216,8 +222,10 @@ __device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g) 
   for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int c = 0; c < 4; c++) g.w[j][c] = d2_mod_p(acc[j][c]);
+  if (store) {
 #pragma unroll
-  for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+    for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+  }
 }
 
 // Lifted leaf (cuda/merkle_leaves_lifted.cu:26-98): Blake2s of the words of
@@ -258,7 +266,7 @@ __device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint
 // components accumulate in 64 bits as in d2_produce (fold after every four
 // columns); four columns' loads are in flight together. Q is stored (SoA)
 // for the fold.
-__device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i, G4& g) {
+__device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i, G4& g, bool store = true) {
   const uint32_t S = L.lift_log, n = L.lift_n;
   uint64_t acc[4][4];
 #pragma unroll
@@ -297,8 +305,10 @@ __device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i,
   for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int c = 0; c < 4; c++) g.w[j][c] = d2_mod_p(acc[j][c]);
+  if (store) {
 #pragma unroll
-  for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+    for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+  }
 }
 
 // Rows [i, i+4) of up to 16 columns, one 16-byte load per column, all
@@ -391,6 +401,55 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
   }
 }
 
+// One leaf per lane (small, latency-bound trees): the producers work on
+// groups of four leaves, so each lane computes the words of its group (the
+// group's four lanes load the same addresses) and hashes leaf idx & 3; only
+// the group's first lane stores produced values (fold layer, quotient).
+template <uint32_t KIND, uint32_t NC>
+__device__ __forceinline__ void lane_leaf(const LeafSrc& L, uint64_t idx, uint32_t out[8]) {
+  constexpr bool kGeneric = KIND == LEAF_COLS && NC == 0;
+  if constexpr (KIND == LEAF_LIFTED) {
+    lifted_leaf(L, idx, out);
+  } else {
+    const uint64_t i = idx & ~3ull;
+    const uint32_t j = (uint32_t)idx & 3u;
+    const bool first = j == 0;
+    auto pick = [j](uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
+      return j == 0 ? a0 : (j == 1 ? a1 : (j == 2 ? a2 : a3));
+    };
+    G4 g;
+    if constexpr (kGeneric) {
+      GCols gc;
+      cols_load(L, i, gc);
+      uint32_t w[16];
+#pragma unroll
+      for (uint32_t c = 0; c < 16; c++) w[c] = pick(gc.c[c].x, gc.c[c].y, gc.c[c].z, gc.c[c].w);
+      b2_hash_words(w, L.n_cols, out);
+      return;
+    } else if constexpr (KIND == LEAF_D2_QUOTIENT) {
+      d2_produce(L, i, g, first);
+    } else if constexpr (KIND == LEAF_LIFTED_RLC) {
+      lifted_rlc_produce(L, i, g, first);
+    } else {
+      qm31 aq{}, af{};
+      if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
+      if constexpr (KIND == LEAF_FOLD_LINE) af = ldq(L.alpha_f);
+      if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) af = qm_mul(ldq(L.alpha_f), aq);
+      Raw r;
+      leaf_load<KIND, NC>(L, i, r);
+      leaf_produce<KIND, NC>(L, i, r, aq, af, g, first);
+    }
+    if constexpr (KIND == LEAF_COLS && NC == 1) {
+      b2_leaf1(pick(g.w[0][0], g.w[1][0], g.w[2][0], g.w[3][0]), out);
+    } else {
+      uint32_t w[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) w[c] = pick(g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+      b2_leaf4(w[0], w[1], w[2], w[3], out);
+    }
+  }
+}
+
 __device__ __forceinline__ void store_node(uint32_t* p, const uint32_t n[8]) {
   st4(p, n[0], n[1], n[2], n[3]);
   st4(p + 4, n[4], n[5], n[6], n[7]);
@@ -406,20 +465,24 @@ __device__ __forceinline__ void lds_get(const uint4* s, uint32_t idx, uint32_t n
 }
 
 // Transcript step fused into the kernel that produced a root
-// (src/prover.rs:321-334 and the mix/draw pairs that follow each commit).
-__device__ __forceinline__ void finish_root(const RootHook& h, const uint32_t node[8]) {
-  if (h.root_out)
+// (src/prover.rs:321-334 and the mix/draw pairs that follow each commit),
+// run by the quad of lanes 0..3, which all hold the root: the channel's
+// compressions are quad hashes (about half the dependent chain of one lane).
+// Lane 0 writes the results.
+__device__ __forceinline__ void finish_root_q0(const RootHook& h, const uint32_t node[8], uint32_t q) {
+  if (h.root_out && q == 0)
     for (int k = 0; k < 8; k++) h.root_out[k] = node[k];
   if (h.ch) {
     Channel c;
     if (h.init) ch_init(c);
     else c = *h.ch;
-    ch_mix_digest(c, node);
+    ch_mix_digest_q0(c, node, q);
     if (h.alpha_out) {
-      qm31 a = ch_draw_felt(c);
-      for (int k = 0; k < 4; k++) h.alpha_out[k] = a.v[k];
+      const qm31 a = ch_draw_felt_q0(c, q);
+      if (q == 0)
+        for (int k = 0; k < 4; k++) h.alpha_out[k] = a.v[k];
     }
-    *h.ch = c;
+    if (q == 0) *h.ch = c;
   }
 }
 
@@ -430,7 +493,7 @@ __device__ __forceinline__ void finish_root(const RootHook& h, const uint32_t no
 // lanes of a quad (b2_node_x4): those levels are a dependent chain with most
 // lanes idle, so halving the instructions per compression halves their
 // latency. (The throughput instances do without: the quad path raises their
-// VGPR count.) Lane 0 returns the top node.
+// VGPR count.) Lanes 0..3 return the top node.
 template <uint32_t CAP, bool QUAD>
 __device__ __forceinline__ void lds_merge(uint4 (*buf)[2 * CAP], uint32_t nthr, uint32_t lvl0,
                                           const TreeLevels& T, uint64_t group, uint32_t group_log,
@@ -470,7 +533,7 @@ __device__ __forceinline__ void lds_merge(uint4 (*buf)[2 * CAP], uint32_t nthr, 
     cur ^= 1;
     __syncthreads();
   }
-  if (tid == 0) lds_get(buf[cur], 0, node);
+  if (tid < 4) lds_get(buf[cur], 0, node);
 }
 
 // ROOT instances (the whole tree in one workgroup) carry the transcript step;
@@ -486,15 +549,16 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(LeafSrc L, TreeLevels T
   uint32_t node[8];
   if (tid < nthr) {
     const uint64_t first = (tile << tile_log) + ((uint64_t)tid << LPT_LOG);
-    lane_subtree<KIND, NC, LPT_LOG>(L, first, node);
+    if constexpr (LPT_LOG == 0) lane_leaf<KIND, NC>(L, first, node);
+    else lane_subtree<KIND, NC, LPT_LOG>(L, first, node);
     lds_put(buf[0], tid, node);
     if (T.base && LPT_LOG >= store_lo)
       store_node(T.base + T.off[LPT_LOG] + 8 * ((tile << (tile_log - LPT_LOG)) + tid), node);
   }
   __syncthreads();
-  lds_merge<SUB_THREADS, ROOT || LPT_LOG == 2>(buf, nthr, LPT_LOG, T, tile, tile_log, store_lo, node);
+  lds_merge<SUB_THREADS, ROOT || LPT_LOG <= 2>(buf, nthr, LPT_LOG, T, tile, tile_log, store_lo, node);
   if constexpr (ROOT) {
-    if (tid == 0) finish_root(hook, node);
+    if (tid < 4) finish_root_q0(hook, node, tid);
   }
 }
 
@@ -508,7 +572,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_tree_reduce(TreeLevels T, uint3
   const uint32_t nthr = 1u << (nlev - 1);
   const uint64_t wg = blockIdx.x;
   uint32_t node[8];
-  if (tid < nthr) {
+  if (4 * nthr <= RED_THREADS) {  // few nodes: the first level by quads too
+    const uint32_t nid = tid >> 2, q = tid & 3;
+    if (nid < nthr) {
+      const uint32_t* in = T.base + T.off[from] + 8 * ((wg << nlev) + 2 * (uint64_t)nid);
+      uint32_t m[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) m[k] = in[k];
+      uint32_t lo, hi;
+      b2_node_x4(m, q, lo, hi);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(buf[0]) + 8 * nid;
+      dst[q] = lo;
+      dst[4 + q] = hi;
+      uint32_t* g = T.base + T.off[from + 1] + 8 * ((wg << (nlev - 1)) + nid);
+      g[q] = lo;
+      g[4 + q] = hi;
+    }
+  } else if (tid < nthr) {
     const uint32_t* in = T.base + T.off[from] + 8 * ((wg << nlev) + 2 * (uint64_t)tid);
     uint32_t l[8], r[8];
     for (int k = 0; k < 8; k++) {
@@ -522,7 +602,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_tree_reduce(TreeLevels T, uint3
   __syncthreads();
   lds_merge<RED_THREADS, true>(buf, nthr, from + 1, T, wg, from + nlev, 0, node);
   if constexpr (ROOT) {
-    if (tid == 0) finish_root(hook, node);
+    if (tid < 4) finish_root_q0(hook, node, tid);
   }
 }
 
@@ -546,13 +626,14 @@ static void launch_subtree_lpt(const LeafSrc& L, const TreeLevels& T, uint32_t t
 }
 
 template <uint32_t KIND, uint32_t NC>
-static void launch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t tile_log, uint32_t store_lo,
-                           const RootHook& hook, hipStream_t s) {
-  const uint32_t lpt = lane_log_for(tile_log);
-  require(tile_log >= lpt && (tile_log - lpt) <= 8, "merkle: tile shape out of range");
+static void launch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t tile_log, uint32_t lpt,
+                           uint32_t store_lo, const RootHook& hook, hipStream_t s) {
+  require(tile_log >= lpt && (tile_log - lpt) <= 8 && (lpt == 0 || (lpt >= 2 && lpt <= 4)),
+          "merkle: tile shape out of range");
   if (lpt == 4) launch_subtree_lpt<KIND, NC, 4>(L, T, tile_log, store_lo, hook, s);
   else if (lpt == 3) launch_subtree_lpt<KIND, NC, 3>(L, T, tile_log, store_lo, hook, s);
-  else launch_subtree_lpt<KIND, NC, 2>(L, T, tile_log, store_lo, hook, s);
+  else if (lpt == 2) launch_subtree_lpt<KIND, NC, 2>(L, T, tile_log, store_lo, hook, s);
+  else launch_subtree_lpt<KIND, NC, 0>(L, T, tile_log, store_lo, hook, s);
   VX_HIP(hipGetLastError());
 }
 
@@ -595,27 +676,26 @@ static void check_leaf(const LeafSrc& L, uint32_t log_n) {
   }
 }
 
-static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t tile_log, uint32_t store_lo,
-                             const RootHook& hook, hipStream_t s) {
+static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t tile_log, uint32_t lpt,
+                             uint32_t store_lo, const RootHook& hook, hipStream_t s) {
   switch (L.kind) {
     case LEAF_COLS:
-      if (L.n_cols == 1) launch_subtree<LEAF_COLS, 1>(L, T, tile_log, store_lo, hook, s);
-      else if (L.n_cols == 4) launch_subtree<LEAF_COLS, 4>(L, T, tile_log, store_lo, hook, s);
-      else launch_subtree<LEAF_COLS, 0>(L, T, tile_log, store_lo, hook, s);
+      if (L.n_cols == 1) launch_subtree<LEAF_COLS, 1>(L, T, tile_log, lpt, store_lo, hook, s);
+      else if (L.n_cols == 4) launch_subtree<LEAF_COLS, 4>(L, T, tile_log, lpt, store_lo, hook, s);
+      else launch_subtree<LEAF_COLS, 0>(L, T, tile_log, lpt, store_lo, hook, s);
       break;
-    case LEAF_QUOTIENT: launch_subtree<LEAF_QUOTIENT, 4>(L, T, tile_log, store_lo, hook, s); break;
-    case LEAF_FOLD_CIRCLE_Q: launch_subtree<LEAF_FOLD_CIRCLE_Q, 4>(L, T, tile_log, store_lo, hook, s); break;
-    case LEAF_FOLD_LINE: launch_subtree<LEAF_FOLD_LINE, 4>(L, T, tile_log, store_lo, hook, s); break;
-    case LEAF_D2_QUOTIENT: launch_subtree<LEAF_D2_QUOTIENT, 4>(L, T, tile_log, store_lo, hook, s); break;
-    case LEAF_LIFTED: launch_subtree<LEAF_LIFTED, 0>(L, T, tile_log, store_lo, hook, s); break;
-    case LEAF_LIFTED_RLC: launch_subtree<LEAF_LIFTED_RLC, 4>(L, T, tile_log, store_lo, hook, s); break;
+    case LEAF_QUOTIENT: launch_subtree<LEAF_QUOTIENT, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
+    case LEAF_FOLD_CIRCLE_Q: launch_subtree<LEAF_FOLD_CIRCLE_Q, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
+    case LEAF_FOLD_LINE: launch_subtree<LEAF_FOLD_LINE, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
+    case LEAF_D2_QUOTIENT: launch_subtree<LEAF_D2_QUOTIENT, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
+    case LEAF_LIFTED: launch_subtree<LEAF_LIFTED, 0>(L, T, tile_log, lpt, store_lo, hook, s); break;
+    case LEAF_LIFTED_RLC: launch_subtree<LEAF_LIFTED_RLC, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
   }
 }
 
-void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, const RootHook& hook, hipStream_t s,
-                   hipEvent_t sub_begin, hipEvent_t sub_end) {
-  check_leaf(L, T.log_n);
-  require(T.base != nullptr, "merkle_commit: null tree buffer");
+// Tile (leaves per subtree workgroup, log) and leaves per lane (log) of a
+// tree of 2^log_n leaves of the given kind.
+static void tree_shape(const LeafSrc& L, uint32_t log_n, uint32_t& tile_log, uint32_t& lpt) {
   // Leaves read from several columns (and the FRI folds, which read two
   // values of four columns per leaf) keep 4 leaves per lane at every size:
   // lane t then reads rows 4t..4t+3, so each 16-byte load of a wavefront is
@@ -626,13 +706,34 @@ void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, con
   const bool multi = L.kind == LEAF_D2_QUOTIENT || L.kind == LEAF_LIFTED_RLC || L.kind == LEAF_LIFTED ||
                      (L.kind == LEAF_COLS && L.n_cols > 1) || L.kind == LEAF_FOLD_LINE ||
                      L.kind == LEAF_FOLD_CIRCLE_Q;
-  const uint32_t tile_log =
-      T.log_n >= BIG_TREE_LOG && !multi ? MAX_TILE_LOG : (T.log_n < 10 ? T.log_n : 10);
-  const uint32_t lpt = lane_log_for(tile_log);
+  // Small trees are latency-bound: one leaf per lane and 256-leaf tiles
+  // make a workgroup's dependent chain one leaf hash plus the tile's levels
+  // (quad hashes from 64 nodes down) instead of 7 hashes plus 8 levels.
+  if (log_n <= SMALL_TREE_LOG) {
+    tile_log = log_n < 8 ? log_n : 8;
+    lpt = 0;
+  } else {
+    tile_log = log_n >= BIG_TREE_LOG && !multi ? MAX_TILE_LOG : 10;
+    lpt = lane_log_for(tile_log);
+  }
+}
+
+uint32_t merkle_tile_log(const LeafSrc& L, uint32_t log_n) {
+  uint32_t tile_log, lpt;
+  tree_shape(L, log_n, tile_log, lpt);
+  return tile_log;
+}
+
+void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, const RootHook& hook, hipStream_t s,
+                   hipEvent_t sub_begin, hipEvent_t sub_end) {
+  check_leaf(L, T.log_n);
+  require(T.base != nullptr, "merkle_commit: null tree buffer");
+  uint32_t tile_log, lpt;
+  tree_shape(L, T.log_n, tile_log, lpt);
   require(store_lo >= T.lo && store_lo >= lpt && store_lo <= tile_log,
           "merkle_commit: stored levels must start at or above the lane level");
   if (sub_begin) VX_HIP(hipEventRecord(sub_begin, s));
-  dispatch_subtree(L, T, tile_log, store_lo, hook, s);
+  dispatch_subtree(L, T, tile_log, lpt, store_lo, hook, s);
   if (sub_end) VX_HIP(hipEventRecord(sub_end, s));
   uint32_t from = tile_log;
   while (from < T.log_n) {
@@ -655,15 +756,17 @@ void merkle_tiled(const LeafSrc& L, uint32_t log_n, uint32_t tile_log, uint32_t*
   T.log_n = log_n;
   T.lo = tile_log;
   T.off[tile_log] = 0;
-  dispatch_subtree(L, T, tile_log, tile_log, RootHook{}, s);
+  dispatch_subtree(L, T, tile_log, lane_log_for(tile_log), tile_log, RootHook{}, s);
 }
 
 // ------------------------------------------------------------ FRI tail
 // FRI line-fold layers of at most 2^TAIL_MAX_LOG values, committed one after
-// another by a single workgroup: fold (previous layer from LDS), leaf hashes,
-// tree levels in LDS (levels >= lo stored for the decommitment), root into
-// the proof, then mix the root and draw the next alpha (src/prover.rs:
-// 527-544). Each layer's dependent chain is about L + 4 compressions.
+// another by a single 1024-lane workgroup: fold (previous layer from LDS),
+// one leaf hash per lane, tree levels in LDS (quad hashes once at most 256
+// parents are left; levels >= lo stored for the decommitment), root into the
+// proof, then the quad of lanes 0..3 mixes the root and draws the next alpha
+// (src/prover.rs:527-544). A layer of 2^L values is a dependent chain of
+// two one-lane hashes, L - 1 quad hashes and the two quad channel hashes.
 __global__ __launch_bounds__(TAIL_THREADS) void k_fri_tail(FriTailArgs a) {
   __shared__ uint32_t vals[2][4][1u << TAIL_MAX_LOG];
   __shared__ uint4 hashes[2u << TAIL_MAX_LOG];
@@ -677,7 +780,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_fri_tail(FriTailArgs a) {
       for (int k = 0; k < 4; k++) s_alpha[k] = T.alpha[k];
     __syncthreads();
     const qm31 al = {{s_alpha[0], s_alpha[1], s_alpha[2], s_alpha[3]}};
-    for (uint32_t i = tid; i < n; i += TAIL_THREADS) {
+    if (tid < n) {
+      const uint32_t i = tid;
       qm31 f0, f1;
       if (li == 0) {
         f0 = {{a.in[0][2 * i], a.in[1][2 * i], a.in[2][2 * i], a.in[3][2 * i]}};
@@ -698,41 +802,56 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_fri_tail(FriTailArgs a) {
     }
     __syncthreads();
     for (uint32_t lvl = 1; lvl <= L; lvl++) {
-      const uint32_t cnt = n >> lvl;  // <= 512 = 2 parents per lane
-      const uint32_t i0 = tid, i1 = tid + TAIL_THREADS;
-      uint32_t p0[8], p1[8], l[8], r[8];
-      if (i0 < cnt) {
-        lds_get(hashes, 2 * i0, l);
-        lds_get(hashes, 2 * i0 + 1, r);
-        b2_hash_node(l, r, p0);
-      }
-      if (i1 < cnt) {
-        lds_get(hashes, 2 * i1, l);
-        lds_get(hashes, 2 * i1 + 1, r);
-        b2_hash_node(l, r, p1);
-      }
-      __syncthreads();  // every child read before any parent overwrites it
-      if (i0 < cnt) {
-        lds_put(hashes, i0, p0);
-        if (lvl >= T.lo) store_node(T.tree + T.off[lvl] + 8 * i0, p0);
-      }
-      if (i1 < cnt) {
-        lds_put(hashes, i1, p1);
-        if (lvl >= T.lo) store_node(T.tree + T.off[lvl] + 8 * i1, p1);
+      const uint32_t cnt = n >> lvl;
+      if (4 * cnt <= TAIL_THREADS) {
+        const uint32_t nid = tid >> 2, q = tid & 3;
+        uint32_t lo = 0, hi = 0;
+        if (nid < cnt) {
+          const uint32_t* src = reinterpret_cast<const uint32_t*>(hashes) + 16 * nid;
+          uint32_t m[16];
+#pragma unroll
+          for (int k = 0; k < 16; k++) m[k] = src[k];
+          b2_node_x4(m, q, lo, hi);
+        }
+        __syncthreads();  // every child read before any parent overwrites it
+        if (nid < cnt) {
+          uint32_t* dst = reinterpret_cast<uint32_t*>(hashes) + 8 * nid;
+          dst[q] = lo;
+          dst[4 + q] = hi;
+          if (lvl >= T.lo) {
+            uint32_t* g = T.tree + T.off[lvl] + 8 * nid;
+            g[q] = lo;
+            g[4 + q] = hi;
+          }
+        }
+      } else {
+        uint32_t p[8], l[8], r[8];
+        if (tid < cnt) {
+          lds_get(hashes, 2 * tid, l);
+          lds_get(hashes, 2 * tid + 1, r);
+          b2_hash_node(l, r, p);
+        }
+        __syncthreads();
+        if (tid < cnt) {
+          lds_put(hashes, tid, p);
+          if (lvl >= T.lo) store_node(T.tree + T.off[lvl] + 8 * tid, p);
+        }
       }
       __syncthreads();
     }
-    if (tid == 0) {
+    if (tid < 4) {
       uint32_t root[8];
       lds_get(hashes, 0, root);
-      for (int k = 0; k < 8; k++) T.root_out[k] = root[k];
+      if (tid == 0)
+        for (int k = 0; k < 8; k++) T.root_out[k] = root[k];
       Channel c = *a.ch;
-      ch_mix_digest(c, root);
+      ch_mix_digest_q0(c, root, tid);
       if (T.next_alpha) {
-        const qm31 nx = ch_draw_felt(c);
-        for (int k = 0; k < 4; k++) T.next_alpha[k] = nx.v[k];
+        const qm31 nx = ch_draw_felt_q0(c, tid);
+        if (tid == 0)
+          for (int k = 0; k < 4; k++) T.next_alpha[k] = nx.v[k];
       }
-      *a.ch = c;
+      if (tid == 0) *a.ch = c;
     }
     __syncthreads();
     cur ^= 1;
@@ -834,7 +953,7 @@ void merkle_root_small(const LeafSrc& L, uint32_t log_n, uint32_t* root_out, hip
   T.log_n = log_n;
   RootHook hook{};
   hook.root_out = root_out;
-  dispatch_subtree(L, T, log_n, MAX_LEVELS, hook, s);
+  dispatch_subtree(L, T, log_n, lane_log_for(log_n), MAX_LEVELS, hook, s);
 }
 
 // parents[j] = H(leaf(2j), leaf(2j+1)) (cuda/blake2s.cu:195-240).

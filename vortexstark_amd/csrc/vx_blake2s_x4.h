@@ -39,16 +39,17 @@ __device__ __forceinline__ uint32_t sel4(bool q0, bool q1, uint32_t A, uint32_t 
   return q1 ? hi : lo;
 }
 
-// Node hash Blake2s(m[0..15]) (t = 64, final block) by the four lanes of a
-// quad, q = lane & 3; all four lanes pass the same 16 message words. Returns
-// output words q and 4 + q in lo / hi.
-__device__ __forceinline__ void b2_node_x4(const uint32_t m[16], uint32_t q, uint32_t& lo, uint32_t& hi) {
+// Single-block Blake2s(m[0..15]) of `t` message bytes (the final block;
+// bytes past t are zero in m) by the four lanes of a quad, q = lane & 3; all
+// four lanes pass the same 16 message words. Returns output words q and
+// 4 + q in lo / hi.
+__device__ __forceinline__ void b2_hash_x4(const uint32_t m[16], uint32_t t, uint32_t q, uint32_t& lo, uint32_t& hi) {
   const bool q0 = q & 1, q1 = q & 2;
   const uint32_t iv_lo = sel4(q0, q1, VX_B2_IV0, VX_B2_IV1, VX_B2_IV2, VX_B2_IV3);
   const uint32_t iv_hi = sel4(q0, q1, VX_B2_IV4, VX_B2_IV5, VX_B2_IV6, VX_B2_IV7);
   const uint32_t h_lo = q == 0 ? (VX_B2_IV0 ^ 0x01010020u) : iv_lo, h_hi = iv_hi;
   uint32_t a = h_lo, b = h_hi, c = iv_lo;
-  uint32_t d = iv_hi ^ (q == 0 ? 64u : 0u) ^ (q == 2 ? 0xFFFFFFFFu : 0u);  // v12 ^= t, v14 ^= final
+  uint32_t d = iv_hi ^ (q == 0 ? t : 0u) ^ (q == 2 ? 0xFFFFFFFFu : 0u);  // v12 ^= t, v14 ^= final
 #pragma unroll
   for (int r = 0; r < 10; r++) {
     uint32_t x = sel4(q0, q1, m[B2_SIGMA[r][0]], m[B2_SIGMA[r][2]], m[B2_SIGMA[r][4]], m[B2_SIGMA[r][6]]);
@@ -66,6 +67,22 @@ __device__ __forceinline__ void b2_node_x4(const uint32_t m[16], uint32_t q, uin
   }
   lo = h_lo ^ a ^ c;
   hi = h_hi ^ b ^ d;
+}
+
+// Merkle node hash (t = 64).
+__device__ __forceinline__ void b2_node_x4(const uint32_t m[16], uint32_t q, uint32_t& lo, uint32_t& hi) {
+  b2_hash_x4(m, 64u, q, lo, hi);
+}
+
+// The 8 output words of a quad hash in every lane of the quad: word k
+// (k < 4) is lane k's lo, word 4 + k its hi. The quad must be lanes 0..3 of
+// the wavefront (readlane reads fixed lanes).
+__device__ __forceinline__ void quad0_gather(uint32_t lo, uint32_t hi, uint32_t out[8]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    out[k] = (uint32_t)__builtin_amdgcn_readlane((int)lo, k);
+    out[4 + k] = (uint32_t)__builtin_amdgcn_readlane((int)hi, k);
+  }
 }
 
 }  // namespace vx

@@ -211,6 +211,9 @@ struct RootHook {
 // Hashes all 2^log_n leaves and stores tree levels [store_lo, log_n] into
 // `tl` (store_lo >= tl.lo, and >= the lane level: 4 for trees of >= 2^21
 // leaves, else 2). Optional events bracket the subtree kernel.
+// log2 of the leaves one subtree workgroup commits for this tree (the
+// lowest level merkle_commit can store above the lane level).
+uint32_t merkle_tile_log(const LeafSrc& leaf, uint32_t log_n);
 void merkle_commit(const LeafSrc& leaf, const TreeLevels& tl, uint32_t store_lo, const RootHook& hook,
                    hipStream_t s, hipEvent_t sub_begin = nullptr, hipEvent_t sub_end = nullptr);
 // FRI line-fold layers of <= 2^TAIL_MAX_LOG values, committed by one
