@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--pow-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-batch", action="store_true")

@@ -173,6 +173,10 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
     // stride between lanes; it goes through LDS instead, so that every
     // 16-byte access of a wavefront is part of one contiguous 1 KiB run.
     const bool staged = a.cbits == 0 && a.j0 == 0 && pos < 4;
+    // LDS word of element s: swz(own(t, 0, pos)) ^ swz(s << pos) (own's bit
+    // groups are disjoint, so the swizzle splits over them); the second term
+    // is wave-uniform, so each access costs one XOR with a scalar.
+    const uint32_t lb = rx_swz(rx_own(t, 0, pos));
     if (ri == 0 && staged) {
 #pragma unroll
       for (uint32_t g = 0; g < 8; g++) {
@@ -185,7 +189,7 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
       }
       __syncthreads();
 #pragma unroll
-      for (uint32_t s = 0; s < 32; s++) v[s] = lds[rx_swz(rx_own(t, s, pos))];
+      for (uint32_t s = 0; s < 32; s++) v[s] = lds[lb ^ rx_swz(s << pos)];
     } else if (ri == 0) {
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) {
@@ -195,11 +199,14 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
       }
     } else {
 #pragma unroll
-      for (uint32_t s = 0; s < 32; s++) v[s] = lds[rx_swz(rx_own(t, s, pos))];
+      for (uint32_t s = 0; s < 32; s++) v[s] = lds[lb ^ rx_swz(s << pos)];
     }
     const uint32_t qlo = r_rb0 + a.cbits - pos, qhi = qlo + r_nb;  // register bits to butterfly
-    // all of the round's twiddles are requested before the first butterfly,
-    // so their latencies overlap (one wait per round instead of one per bit)
+    // Each bit's twiddles are loaded just before its butterflies. (Loading
+    // the whole round's 31 words up front overlaps their latencies within a
+    // workgroup but costs 26 more VGPRs: 154 per lane fits one 512-lane
+    // workgroup per CU, 128 fits the two that LDS allows, and the second
+    // workgroup hides the latency better: the LDE ran 0.60 -> 0.41 ms.)
     uint32_t w4[1], w3[2], w2[4], w1[8], w0[16];
 #define VX_RX_LOAD(Q, W)                                                                          \
   if ((Q) >= qlo && (Q) < qhi) {                                                                  \
@@ -207,26 +214,27 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
     const uint64_t base = (H << (13 - epos)) | ((uint64_t)(t >> pos) << (4 - (Q)));               \
     rx_load_tw<(Q)>(a.tw[epos - a.cbits], base, W);                                               \
   }
-    VX_RX_LOAD(4, w4) VX_RX_LOAD(3, w3) VX_RX_LOAD(2, w2) VX_RX_LOAD(1, w1) VX_RX_LOAD(0, w0)
-#undef VX_RX_LOAD
 #define VX_RX_APPLY(Q, W) \
   if ((Q) >= qlo && (Q) < qhi) rx_apply<FWD, (Q)>(v, W);
     if (FWD) {
-      VX_RX_APPLY(4, w4) VX_RX_APPLY(3, w3) VX_RX_APPLY(2, w2) VX_RX_APPLY(1, w1) VX_RX_APPLY(0, w0)
+      VX_RX_LOAD(4, w4) VX_RX_APPLY(4, w4) VX_RX_LOAD(3, w3) VX_RX_APPLY(3, w3) VX_RX_LOAD(2, w2)
+      VX_RX_APPLY(2, w2) VX_RX_LOAD(1, w1) VX_RX_APPLY(1, w1) VX_RX_LOAD(0, w0) VX_RX_APPLY(0, w0)
     } else {
-      VX_RX_APPLY(0, w0) VX_RX_APPLY(1, w1) VX_RX_APPLY(2, w2) VX_RX_APPLY(3, w3) VX_RX_APPLY(4, w4)
+      VX_RX_LOAD(0, w0) VX_RX_APPLY(0, w0) VX_RX_LOAD(1, w1) VX_RX_APPLY(1, w1) VX_RX_LOAD(2, w2)
+      VX_RX_APPLY(2, w2) VX_RX_LOAD(3, w3) VX_RX_APPLY(3, w3) VX_RX_LOAD(4, w4) VX_RX_APPLY(4, w4)
     }
 #undef VX_RX_APPLY
+#undef VX_RX_LOAD
     if (ri + 1 < a.n_rounds) {
 #pragma unroll
-      for (uint32_t s = 0; s < 32; s++) lds[rx_swz(rx_own(t, s, pos))] = v[s];
+      for (uint32_t s = 0; s < 32; s++) lds[lb ^ rx_swz(s << pos)] = v[s];
       __syncthreads();
     } else if (staged) {
       // each lane rewrites exactly the words it read this round, so no
       // barrier is needed before these stores
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++)
-        lds[rx_swz(rx_own(t, s, pos))] = a.scale != 1u ? m31_mul(v[s], a.scale) : v[s];
+        lds[lb ^ rx_swz(s << pos)] = a.scale != 1u ? m31_mul(v[s], a.scale) : v[s];
       __syncthreads();
 #pragma unroll
       for (uint32_t g = 0; g < 8; g++) {
