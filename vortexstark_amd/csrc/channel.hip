@@ -168,4 +168,27 @@ void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_
   VX_HIP(hipGetLastError());
 }
 
+// Proof download by the GPU itself: the workgroups store the proof
+// straight into pinned host memory (mapped into the device's address space),
+// 16 bytes per lane per step, instead of a DMA copy whose start-up cost
+// dominated for 1-2 MB proofs.
+__global__ __launch_bounds__(256) void k_copy_to_host(uint32_t* dst, const uint32_t* src, uint64_t words) {
+  const uint64_t n16 = words >> 2;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint4* d4 = reinterpret_cast<uint4*>(dst);
+  const uint4* s4 = reinterpret_cast<const uint4*>(src);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) d4[i] = s4[i];
+  const uint64_t tail = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tail < (words & 3)) dst[4 * n16 + tail] = src[4 * n16 + tail];
+}
+
+void copy_to_host(uint32_t* host_dst, const uint32_t* dev_src, uint64_t words, hipStream_t s) {
+  require((reinterpret_cast<uintptr_t>(host_dst) & 15u) == 0 && (reinterpret_cast<uintptr_t>(dev_src) & 15u) == 0,
+          "copy_to_host: 16-byte aligned buffers");
+  const uint64_t blocks = ((words >> 2) + 255) / 256;
+  const uint32_t grid = (uint32_t)(blocks < 1 ? 1 : (blocks > 512 ? 512 : blocks));
+  hipLaunchKernelGGL(k_copy_to_host, dim3(grid), dim3(256), 0, s, host_dst, dev_src, words);
+  VX_HIP(hipGetLastError());
+}
+
 }  // namespace vx

@@ -296,8 +296,10 @@ StarkProver::StarkProver(uint32_t log_n, uint32_t blowup_bits, uint32_t n_querie
   d_nonce_ = reinterpret_cast<uint64_t*>(reinterpret_cast<uint8_t*>(d_prefix_) + 64);
   d_inputs_ = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(d_prefix_) + 128);
   proof_ = DeviceBuf(layout_.total_words * 4);
-  for (auto& h : host_proof_)
-    VX_HIP(hipHostMalloc(reinterpret_cast<void**>(&h), layout_.total_words * 4, hipHostMallocDefault));
+  for (int k = 0; k < 2; k++) {
+    VX_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_proof_[k]), layout_.total_words * 4, hipHostMallocDefault));
+    VX_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_proof_dev_[k]), host_proof_[k], 0));
+  }
   device_bytes_ += small_.bytes + proof_.bytes;
   for (auto& e : ev_) VX_HIP(hipEventCreate(&e));
   tab_ = ProverTables::get(log_n_, blowup_, stream_);
@@ -653,7 +655,7 @@ void StarkProver::enqueue(uint32_t a, uint32_t b, bool timed) {
     VX_HIP(hipGraphLaunch(graph_, stream_));
   }
   cur_ ^= 1;
-  VX_HIP(hipMemcpyAsync(host_proof_[cur_], proof_.p, layout_.total_words * 4, hipMemcpyDeviceToHost, stream_));
+  copy_to_host(host_proof_dev_[cur_], proof_.u32(), layout_.total_words, stream_);
   if (timed) VX_HIP(hipEventRecord(ev_[9], stream_));
 }
 

@@ -156,6 +156,7 @@ class StarkProver {
   DeviceBuf descs_;
   DeviceBuf proof_;
   uint8_t* host_proof_[2] = {nullptr, nullptr};
+  uint32_t* host_proof_dev_[2] = {nullptr, nullptr};  // the same pinned buffers as the device addresses them
   uint32_t cur_ = 0;
   hipEvent_t ev_[12] = {};
   bool timed_ = false;

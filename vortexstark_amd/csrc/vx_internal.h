@@ -298,6 +298,9 @@ void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, 
 void grind_pow_batch(const uint32_t* d_prefix, uint64_t* d_result, uint32_t pow_bits,
                      uint64_t batch_offset, uint64_t n_nonces, hipStream_t s);
 void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_t s);
+// Copy `words` words from device memory into pinned host memory
+// (hipHostMalloc) with a kernel; both 16-byte aligned.
+void copy_to_host(uint32_t* host_dst, const uint32_t* dev_src, uint64_t words, hipStream_t s);
 
 // ------------------------------------- synthetic multi-column AIR (d2.hip)
 void d2_trace(const uint32_t* seed_words, uint32_t* out, uint32_t n_cols, uint32_t log_n, uint64_t stride,
