@@ -360,6 +360,20 @@ uint64_t vx_server_resident_bytes(const vx_server *s);
 int vx_server_run_trace(vx_server *s, const vx_request *reqs, const double *arrival_s, uint32_t n,
                         uint8_t *const *outs, const size_t *caps, size_t *lens, double *times);
 
+/* ------------------------------------------------------------ autotuner
+ * No counterpart in the reference. The first commitment of each key (leaf
+ * kind, columns, constraints, log size, stored levels) on a device is timed
+ * with every valid subtree shape (leaves per workgroup and per lane) and the
+ * fastest is cached; all shapes give the same tree, so proofs do not change.
+ * Provers run one eager proof before capturing their graph so that every
+ * commitment of the class is tuned. enabled = 0 uses the built-in shapes.
+ * cache_file (may be NULL): entries are loaded from it and appended to it. */
+int vx_tuner_configure(int enabled, const char *cache_file);
+/* Up to `capacity` cached entries, 9 words each: device, leaf kind, n_cols,
+ * n_cons, lifted columns, log_n, stored-from level, tile log, leaves-per-lane
+ * log; *n = the number of entries (may exceed capacity). */
+int vx_tuner_entries(uint32_t *out, uint32_t capacity, uint32_t *n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@ byte strings (src/bin/stark_cli.rs:1092-1120); v1 when pow_bits == 0.
     ok, why = verify(proof)       # verifier::verify
 """
 import ctypes as C
+import os
 
 from . import _lib
 from ._lib import VortexError, load
@@ -241,6 +242,27 @@ def request(log_n, a=1, b=1, blowup_bits=BLOWUP_BITS, n_queries=N_QUERIES, pow_b
     d2 = air == AIR_D2
     return _lib.Request(air, log_n, blowup_bits, n_queries, pow_bits, a if not d2 else 0, b if not d2 else 0,
                         n_cols if d2 else 0, n_cons if d2 else 0, 0, seed if d2 else 0, air_seed if d2 else 0)
+
+
+TUNER_FIELDS = ("device", "leaf_kind", "n_cols", "n_cons", "lift_n", "log_n", "store_lo", "tile_log", "lane_log")
+
+
+def tuner(enabled=True, cache_file=None):
+    """Configure the Merkle-shape autotuner (process-wide): when enabled, the
+    first commitment of each (leaf kind, columns, size) is timed with every
+    valid subtree shape and the fastest is kept; cache_file persists the
+    choices across processes. Proof bytes never depend on it."""
+    _lib.call("vx_tuner_configure", 1 if enabled else 0,
+              None if cache_file is None else os.fsencode(cache_file))
+
+
+def tuner_entries():
+    """The cached tuning choices, one dict per key."""
+    n = C.c_uint32()
+    _lib.call("vx_tuner_entries", None, 0, C.byref(n))
+    buf = (C.c_uint32 * (9 * max(n.value, 1)))()
+    _lib.call("vx_tuner_entries", buf, n.value, C.byref(n))
+    return [dict(zip(TUNER_FIELDS, buf[9 * i:9 * i + 9])) for i in range(n.value)]
 
 
 def request_size(req):

@@ -71,6 +71,8 @@ SIGNATURES = {
     "vx_prove": (ip, [u32, u32, u32, u32, u32, u32, vp, sz, C.POINTER(sz)]),
     "vx_verify": (ip, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
     "vx_prover_set_graphs": (ip, [vp, ip]),
+    "vx_tuner_configure": (ip, [ip, C.c_char_p]),
+    "vx_tuner_entries": (ip, [C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32)]),
     "vx_d2_prover_create": (ip, [u32, u32, u32, u64, u32, u32, u32, vp, C.POINTER(vp)]),
     "vx_d2_proof_size": (sz, [u32, u32, u32, u32]),
     "vx_m31_fill_random": (ip, [vp, u32, u64, u32, vp]),

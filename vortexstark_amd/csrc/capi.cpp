@@ -8,6 +8,7 @@
 
 #include "prover.h"
 #include "batch.h"
+#include "tune.h"
 #include "server.h"
 #include "vx_d2.h"
 
@@ -560,6 +561,24 @@ int vx_prover_set_graphs(vx_prover* p, int on) {
   return guard([&] {
     require(p != nullptr, "null prover");
     p->impl.set_graphs(on != 0);
+  });
+}
+
+// ------------------------------------------------------------ autotuner
+int vx_tuner_configure(int enabled, const char* cache_file) {
+  return guard([&] { tuner_configure(enabled != 0, cache_file); });
+}
+
+int vx_tuner_entries(uint32_t* out, uint32_t capacity, uint32_t* n) {
+  return guard([&] {
+    require(n != nullptr && (out != nullptr || capacity == 0), "tuner: null argument");
+    const std::vector<TunedEntry> e = tuner_entries();
+    *n = (uint32_t)e.size();
+    for (uint32_t i = 0; i < e.size() && i < capacity; i++) {
+      const TunedEntry& t = e[i];
+      const uint32_t w[9] = {t.device, t.kind, t.n_cols, t.n_cons, t.lift_n, t.log_n, t.store_lo, t.tile_log, t.lpt};
+      for (int k = 0; k < 9; k++) out[9 * i + k] = w[k];
+    }
   });
 }
 

@@ -693,9 +693,9 @@ static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t til
   }
 }
 
-// Tile (leaves per subtree workgroup, log) and leaves per lane (log) of a
-// tree of 2^log_n leaves of the given kind.
-static void tree_shape(const LeafSrc& L, uint32_t log_n, uint32_t& tile_log, uint32_t& lpt) {
+// Built-in tile (leaves per subtree workgroup, log) and leaves per lane
+// (log) of a tree of 2^log_n leaves of the given kind.
+TreeShape merkle_default_shape(const LeafSrc& L, uint32_t log_n) {
   // Leaves read from several columns (and the FRI folds, which read two
   // values of four columns per leaf) keep 4 leaves per lane at every size:
   // lane t then reads rows 4t..4t+3, so each 16-byte load of a wavefront is
@@ -709,27 +709,33 @@ static void tree_shape(const LeafSrc& L, uint32_t log_n, uint32_t& tile_log, uin
   // Small trees are latency-bound: one leaf per lane and 256-leaf tiles
   // make a workgroup's dependent chain one leaf hash plus the tile's levels
   // (quad hashes from 64 nodes down) instead of 7 hashes plus 8 levels.
-  if (log_n <= SMALL_TREE_LOG) {
-    tile_log = log_n < 8 ? log_n : 8;
-    lpt = 0;
-  } else {
-    tile_log = log_n >= BIG_TREE_LOG && !multi ? MAX_TILE_LOG : 10;
-    lpt = lane_log_for(tile_log);
-  }
+  if (log_n <= SMALL_TREE_LOG) return {log_n < 8 ? log_n : 8, 0};
+  const uint32_t tile_log = log_n >= BIG_TREE_LOG && !multi ? MAX_TILE_LOG : 10;
+  return {tile_log, lane_log_for(tile_log)};
 }
 
-uint32_t merkle_tile_log(const LeafSrc& L, uint32_t log_n) {
-  uint32_t tile_log, lpt;
-  tree_shape(L, log_n, tile_log, lpt);
-  return tile_log;
+std::vector<TreeShape> merkle_shapes(uint32_t log_n, uint32_t store_lo) {
+  std::vector<TreeShape> out;
+  const TreeShape all[4] = {{log_n < 8 ? log_n : 8, 0}, {10, 2}, {11, 3}, {12, 4}};
+  for (const TreeShape& t : all)
+    if (t.tile_log <= log_n && t.lpt <= store_lo && store_lo <= t.tile_log) out.push_back(t);
+  return out;
 }
+
+uint32_t merkle_tile_log(const LeafSrc& L, uint32_t log_n) { return merkle_default_shape(L, log_n).tile_log; }
 
 void merkle_commit(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, const RootHook& hook, hipStream_t s,
                    hipEvent_t sub_begin, hipEvent_t sub_end) {
+  merkle_commit_shaped(L, T, store_lo, hook, s, tuned_shape(L, T, store_lo, s), sub_begin, sub_end);
+}
+
+void merkle_commit_shaped(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, const RootHook& hook,
+                          hipStream_t s, const TreeShape& shape, hipEvent_t sub_begin, hipEvent_t sub_end) {
   check_leaf(L, T.log_n);
   require(T.base != nullptr, "merkle_commit: null tree buffer");
-  uint32_t tile_log, lpt;
-  tree_shape(L, T.log_n, tile_log, lpt);
+  const uint32_t tile_log = shape.tile_log, lpt = shape.lpt;
+  require(tile_log <= T.log_n && tile_log <= MAX_TILE_LOG && tile_log >= lpt && tile_log - lpt <= 8,
+          "merkle_commit: invalid tree shape");
   require(store_lo >= T.lo && store_lo >= lpt && store_lo <= tile_log,
           "merkle_commit: stored levels must start at or above the lane level");
   if (sub_begin) VX_HIP(hipEventRecord(sub_begin, s));

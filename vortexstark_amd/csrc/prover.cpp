@@ -643,6 +643,14 @@ void StarkProver::capture() {
   VX_HIP(e);
 }
 
+void StarkProver::tune() {
+  if (tuned_) return;
+  set_inputs(d_inputs_, proof_.u32(), air_.kind == AIR_D2 ? 9 : (air_.kind == AIR_BATCH ? 7 : 3), 1, 1, stream_);
+  record(false, stream_);
+  VX_HIP(hipStreamSynchronize(stream_));
+  tuned_ = true;
+}
+
 void StarkProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   const bool seeded = air_.kind != AIR_FIBONACCI;  // (a, b) = the trace seed's words
   require(seeded || (a < P && b < P), "prove: public inputs must be canonical M31");
@@ -651,7 +659,12 @@ void StarkProver::enqueue(uint32_t a, uint32_t b, bool timed) {
   if (timed || !use_graph_) {
     record(timed, stream_);
   } else {
-    if (!graph_) capture();
+    if (!graph_) {
+      tune();
+      set_inputs(d_inputs_, proof_.u32(), air_.kind == AIR_D2 ? 9 : (air_.kind == AIR_BATCH ? 7 : 3), a, b,
+                 stream_);
+      capture();
+    }
     VX_HIP(hipGraphLaunch(graph_, stream_));
   }
   cur_ ^= 1;

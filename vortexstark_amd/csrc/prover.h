@@ -98,10 +98,17 @@ class StarkProver {
   uint64_t device_bytes() const { return device_bytes_; }
   StageTimes last_times();
   void set_graphs(bool on) { use_graph_ = on; }
-  // Capture the proof graph now rather than on the first untimed proof.
+  // Capture the proof graph now rather than on the first untimed proof
+  // (after one eager proof that lets the autotuner time every commitment).
   void prepare_graph() {
-    if (use_graph_ && !graph_) capture();
+    if (use_graph_ && !graph_) {
+      tune();
+      capture();
+    }
   }
+  // One eager proof of dummy inputs: every merkle_commit of this class is
+  // timed by the autotuner (if enabled) before a graph freezes the shapes.
+  void tune();
   bool graphs() const { return use_graph_; }
   // Device bytes a prover of this class allocates for itself (tables excluded).
   static uint64_t estimate_bytes(uint32_t log_n, uint32_t blowup_bits, uint32_t n_queries, uint32_t pow_bits,
@@ -161,6 +168,7 @@ class StarkProver {
   hipEvent_t ev_[12] = {};
   bool timed_ = false;
   bool use_graph_ = true;
+  bool tuned_ = false;
   hipGraphExec_t graph_ = nullptr;
 };
 

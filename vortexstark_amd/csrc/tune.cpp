@@ -1,0 +1,160 @@
+// Autotuner for the Merkle subtree schedule.
+//
+// A commitment's speed depends on its tile shape (leaves per workgroup and
+// per lane, TreeShape) in ways that change with the leaf kind, the number of
+// columns and the tree size: small trees are latency-bound and want one leaf
+// per lane, large single-column trees want 16 leaves per lane, multi-column
+// leaves and FRI folds want 4 (coalesced loads). Instead of trusting one
+// table, the first commitment of each key (leaf kind, columns, constraints,
+// log_n, stored levels) on a device is timed with every valid shape, twice
+// each (the second run counts), on the caller's stream and with the
+// caller's buffers, and the fastest is cached for the process. All shapes
+// produce the same tree, so the choice never changes a proof. Measuring
+// needs a stream synchronisation, so it never happens while the stream is
+// being captured into a graph (the provers run one eager proof first, see
+// StarkProver::tune). An optional cache file keeps the choices across
+// processes: one line "kind n_cols n_cons lift_n log_n store_lo tile_log lpt"
+// per key.
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+
+#include "tune.h"
+#include "vx_internal.h"
+
+namespace vx {
+
+namespace {
+
+using Key = std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>;
+
+struct Tuner {
+  std::mutex mu;
+  bool enabled = true;
+  std::string file;
+  std::map<Key, TreeShape> cache;
+  uint64_t measured = 0;
+};
+
+Tuner& tuner() {
+  static Tuner t;
+  return t;
+}
+
+Key key_of(int dev, const LeafSrc& L, uint32_t log_n, uint32_t store_lo) {
+  const bool cols = L.kind == LEAF_COLS || L.kind == LEAF_D2_QUOTIENT;
+  return Key{dev,
+             L.kind,
+             cols ? L.n_cols : 0u,
+             L.kind == LEAF_D2_QUOTIENT ? L.n_cons : 0u,
+             (L.kind == LEAF_LIFTED || L.kind == LEAF_LIFTED_RLC) ? L.lift_n : 0u,
+             log_n,
+             store_lo};
+}
+
+void append_line(const std::string& file, const Key& k, const TreeShape& s) {
+  if (file.empty()) return;
+  FILE* f = std::fopen(file.c_str(), "a");
+  if (!f) return;  // the cache file is an optimisation: a read-only path only costs re-measuring
+  std::fprintf(f, "%u %u %u %u %u %u %u %u\n", (unsigned)std::get<1>(k), std::get<2>(k), std::get<3>(k),
+               std::get<4>(k), std::get<5>(k), std::get<6>(k), s.tile_log, s.lpt);
+  std::fclose(f);
+}
+
+TreeShape measure(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, hipStream_t s,
+                  const std::vector<TreeShape>& cands) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VX_HIP(hipEventCreate(&e0));
+  VX_HIP(hipEventCreate(&e1));
+  TreeShape best = cands[0];
+  float best_ms = 1e30f;
+  try {
+    for (const TreeShape& c : cands) {
+      float ms = 0;
+      for (int rep = 0; rep < 2; rep++) {
+        VX_HIP(hipEventRecord(e0, s));
+        merkle_commit_shaped(L, T, store_lo, RootHook{}, s, c);
+        VX_HIP(hipEventRecord(e1, s));
+        VX_HIP(hipEventSynchronize(e1));
+        VX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  VX_HIP(hipEventDestroy(e0));
+  VX_HIP(hipEventDestroy(e1));
+  return best;
+}
+
+}  // namespace
+
+TreeShape tuned_shape(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, hipStream_t s) {
+  const TreeShape dflt = merkle_default_shape(L, T.log_n);
+  Tuner& t = tuner();
+  int dev = 0;
+  VX_HIP(hipGetDevice(&dev));
+  const Key k = key_of(dev, L, T.log_n, store_lo);
+  {
+    std::lock_guard<std::mutex> g(t.mu);
+    if (!t.enabled) return dflt;
+    auto it = t.cache.find(k);
+    if (it != t.cache.end()) return it->second;
+  }
+  const std::vector<TreeShape> cands = merkle_shapes(T.log_n, store_lo);
+  if (cands.size() < 2) return dflt;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  VX_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone) return dflt;
+  const TreeShape best = measure(L, T, store_lo, s, cands);
+  std::lock_guard<std::mutex> g(t.mu);
+  auto ins = t.cache.emplace(k, best);
+  if (ins.second) {
+    t.measured++;
+    append_line(t.file, k, best);
+  }
+  return ins.first->second;
+}
+
+void tuner_configure(bool enabled, const char* cache_file) {
+  Tuner& t = tuner();
+  int dev = 0;
+  VX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(t.mu);
+  t.enabled = enabled;
+  t.file = cache_file ? cache_file : "";
+  if (t.file.empty()) return;
+  FILE* f = std::fopen(t.file.c_str(), "r");
+  if (!f) return;
+  unsigned v[8];
+  while (std::fscanf(f, "%u %u %u %u %u %u %u %u", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]) == 8) {
+    const TreeShape sh{v[6], v[7]};
+    // keep only shapes this build accepts for that key
+    bool ok = false;
+    for (const TreeShape& c : merkle_shapes(v[4], v[5])) ok = ok || (c.tile_log == sh.tile_log && c.lpt == sh.lpt);
+    if (ok) t.cache[Key{dev, v[0], v[1], v[2], v[3], v[4], v[5]}] = sh;
+  }
+  std::fclose(f);
+}
+
+std::vector<TunedEntry> tuner_entries() {
+  Tuner& t = tuner();
+  std::lock_guard<std::mutex> g(t.mu);
+  std::vector<TunedEntry> out;
+  for (const auto& kv : t.cache) {
+    const Key& k = kv.first;
+    out.push_back({(uint32_t)std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k), std::get<4>(k),
+                   std::get<5>(k), std::get<6>(k), kv.second.tile_log, kv.second.lpt});
+  }
+  return out;
+}
+
+}  // namespace vx

@@ -208,14 +208,37 @@ struct RootHook {
   uint32_t* alpha_out;
   uint32_t init;
 };
-// Hashes all 2^log_n leaves and stores tree levels [store_lo, log_n] into
-// `tl` (store_lo >= tl.lo, and >= the lane level: 4 for trees of >= 2^21
-// leaves, else 2). Optional events bracket the subtree kernel.
-// log2 of the leaves one subtree workgroup commits for this tree (the
-// lowest level merkle_commit can store above the lane level).
+// Subtree schedule of one commitment: 2^tile_log leaves per workgroup and
+// 2^lpt leaves per lane (lpt 0: one leaf per lane, tile levels hashed by
+// quads; 2..4: a lane folds its leaves to one node in registers). Every
+// valid shape produces the same tree; only the speed differs.
+struct TreeShape {
+  uint32_t tile_log, lpt;
+};
+// The built-in choice for a tree of 2^log_n leaves of this kind.
+TreeShape merkle_default_shape(const LeafSrc& leaf, uint32_t log_n);
+// The shapes merkle_commit accepts for this tree when levels from store_lo
+// up are stored (store_lo between lpt and tile_log).
+std::vector<TreeShape> merkle_shapes(uint32_t log_n, uint32_t store_lo);
+// log2 of the leaves one subtree workgroup commits with the built-in shape
+// (the lowest level merkle_commit can store above the lane level).
 uint32_t merkle_tile_log(const LeafSrc& leaf, uint32_t log_n);
+// Hashes all 2^log_n leaves and stores tree levels [store_lo, log_n] into
+// `tl` (store_lo >= tl.lo, between the shape's lane level and tile size).
+// The shape comes from the autotuner (tune.cpp) when it is enabled, else
+// the built-in choice. Optional events bracket the subtree kernel.
 void merkle_commit(const LeafSrc& leaf, const TreeLevels& tl, uint32_t store_lo, const RootHook& hook,
                    hipStream_t s, hipEvent_t sub_begin = nullptr, hipEvent_t sub_end = nullptr);
+// The same with an explicit shape (validated).
+void merkle_commit_shaped(const LeafSrc& leaf, const TreeLevels& tl, uint32_t store_lo, const RootHook& hook,
+                          hipStream_t s, const TreeShape& shape, hipEvent_t sub_begin = nullptr,
+                          hipEvent_t sub_end = nullptr);
+// Autotuner (tune.cpp): the fastest shape for this commitment, measured on
+// the stream the first time its key (leaf kind, column / constraint counts,
+// log_n, store_lo) is seen, and cached for the process (and in the cache
+// file when one is configured). Never measures while the stream is being
+// captured into a graph; then the cached or built-in shape is returned.
+TreeShape tuned_shape(const LeafSrc& leaf, const TreeLevels& tl, uint32_t store_lo, hipStream_t s);
 // FRI line-fold layers of <= 2^TAIL_MAX_LOG values, committed by one
 // workgroup in one launch.
 constexpr uint32_t TAIL_MAX_LOG = 10;
