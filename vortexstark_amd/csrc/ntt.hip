@@ -498,33 +498,60 @@ struct BitrevJob {
   uint32_t log_n;
   uint64_t first_tile;
 };
-static constexpr uint32_t BR_T = 5;
+// Columns of log size >= 12 move in 64 x 64 tiles (256-byte runs on both
+// sides), smaller ones (10, 11) in 32 x 32 tiles.
+static constexpr uint32_t BR_BIG = 6, BR_SMALL = 5, BR_MIN_LOG = 2 * BR_SMALL;
+static constexpr uint32_t BR_WGS = 2048;  // persistent workgroups (8 per CU)
 
-__global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint32_t n_jobs) {
-  __shared__ uint32_t tile[32][33];
-  const uint64_t b = blockIdx.x;
-  uint32_t lo = 0, hi = n_jobs;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (ld_uniform(&jobs[mid].first_tile) <= b) lo = mid;
-    else hi = mid;
-  }
-  const BitrevJob J = ld_uniform(jobs + lo);
-  const uint32_t E = J.log_n, mb = E - 2 * BR_T;
-  const uint64_t m = b - J.first_tile;
+__host__ __device__ __forceinline__ uint32_t br_tbits(uint32_t log_n) {
+  return log_n >= 2 * BR_BIG ? BR_BIG : BR_SMALL;
+}
+
+// One tile m of column J: index i = (a << (E - T)) | (m << T) | b maps to
+// rev(i) = (rev_T(b) << (E - T)) | (rev(m) << T) | rev_T(a).
+template <uint32_t T>
+__device__ __forceinline__ void br_tile(const uint32_t* src, uint32_t* dst, uint32_t E, uint64_t m,
+                                        uint32_t (*tile)[(1u << BR_BIG) + 1]) {
+  constexpr uint32_t W = 1u << T, ROWS = 256 / W;
+  const uint32_t mb = E - 2 * T;
   const uint64_t rm = mb ? (__brevll(m) >> (64 - mb)) : 0;
   const uint32_t t = threadIdx.x;
 #pragma unroll
-  for (uint32_t r = 0; r < 4; r++) {
-    const uint32_t a = (t >> 5) + 8 * r, bb = t & 31;
-    tile[a][bb] = J.src[((uint64_t)a << (E - BR_T)) | (m << BR_T) | bb];
+  for (uint32_t r = 0; r < W / ROWS; r++) {
+    const uint32_t a = (t / W) + ROWS * r, b = t % W;
+    tile[a][b] = src[((uint64_t)a << (E - T)) | (m << T) | b];
   }
   __syncthreads();
 #pragma unroll
-  for (uint32_t r = 0; r < 4; r++) {
-    const uint32_t bb = (t >> 5) + 8 * r, a2 = t & 31;  // a2 = rev5(a)
-    const uint32_t a = __brev(a2) >> (32 - BR_T), rb = __brev(bb) >> (32 - BR_T);
-    J.dst[((uint64_t)rb << (E - BR_T)) | (rm << BR_T) | a2] = tile[a][bb];
+  for (uint32_t r = 0; r < W / ROWS; r++) {
+    const uint32_t ob = (t / W) + ROWS * r, oa = t % W;
+    const uint32_t a = __brev(oa) >> (32 - T), b = __brev(ob) >> (32 - T);
+    dst[((uint64_t)ob << (E - T)) | (rm << T) | oa] = tile[a][b];
+  }
+  __syncthreads();  // the next tile's writes must not overtake these reads
+}
+
+// Workgroup w runs the contiguous tiles [w chunk, (w + 1) chunk) of the
+// whole list, so its job changes rarely and is found by one binary search
+// and then a forward scan (scalar loads).
+__global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint32_t n_jobs, uint64_t total,
+                                                     uint64_t chunk) {
+  __shared__ uint32_t tile[1u << BR_BIG][(1u << BR_BIG) + 1];
+  const uint64_t b0 = (uint64_t)blockIdx.x * chunk;
+  if (b0 >= total) return;
+  const uint64_t b1 = b0 + chunk < total ? b0 + chunk : total;
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ld_uniform(&jobs[mid].first_tile) <= b0) lo = mid;
+    else hi = mid;
+  }
+  for (uint64_t b = b0; b < b1; b++) {
+    while (lo + 1 < n_jobs && ld_uniform(&jobs[lo + 1].first_tile) <= b) lo++;
+    uniform_t<BitrevJob>& J = as_uniform(jobs + lo);
+    const uint32_t E = J.log_n;
+    if (E >= 2 * BR_BIG) br_tile<BR_BIG>(J.src, J.dst, E, b - J.first_tile, tile);
+    else br_tile<BR_SMALL>(J.src, J.dst, E, b - J.first_tile, tile);
   }
 }
 
@@ -533,10 +560,10 @@ void bitrev_batch_build(const uint32_t* const* src, uint32_t* const* dst, const 
   std::vector<BitrevJob> jobs;
   tiles = 0;
   for (uint32_t c = 0; c < n; c++) {
-    require(log_n[c] >= 2 * BR_T && log_n[c] <= 31 && src[c] && dst[c] && src[c] != dst[c],
+    require(log_n[c] >= BR_MIN_LOG && log_n[c] <= 31 && src[c] && dst[c] && src[c] != dst[c],
             "bitrev_batch: out-of-place columns of log size >= 10");
     jobs.push_back({src[c], dst[c], log_n[c], tiles});
-    tiles += 1ull << (log_n[c] - 2 * BR_T);
+    tiles += 1ull << (log_n[c] - 2 * br_tbits(log_n[c]));
   }
   count = n;
   out.assign(reinterpret_cast<const uint8_t*>(jobs.data()),
@@ -544,10 +571,11 @@ void bitrev_batch_build(const uint32_t* const* src, uint32_t* const* dst, const 
 }
 
 void bitrev_batch_launch(const uint8_t* d_jobs, uint32_t count, uint64_t tiles, hipStream_t s) {
-  if (!count) return;
-  require(tiles < (1ull << 31), "bitrev_batch: too many tiles");
-  hipLaunchKernelGGL(k_bitrev_list, dim3((uint32_t)tiles), dim3(256), 0, s, reinterpret_cast<const BitrevJob*>(d_jobs),
-                     count);
+  if (!count || !tiles) return;
+  const uint64_t chunk = (tiles + BR_WGS - 1) / BR_WGS;
+  const uint32_t grid = (uint32_t)((tiles + chunk - 1) / chunk);
+  hipLaunchKernelGGL(k_bitrev_list, dim3(grid), dim3(256), 0, s, reinterpret_cast<const BitrevJob*>(d_jobs), count,
+                     tiles, chunk);
   VX_HIP(hipGetLastError());
 }
 
