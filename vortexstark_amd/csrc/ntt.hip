@@ -516,17 +516,18 @@ __device__ __forceinline__ void br_tile(const uint32_t* src, uint32_t* dst, uint
   const uint32_t mb = E - 2 * T;
   const uint64_t rm = mb ? (__brevll(m) >> (64 - mb)) : 0;
   const uint32_t t = threadIdx.x;
+  // row r of a lane is `ROWS r` rows below its first: a uniform step
+  const uint64_t step = (uint64_t)ROWS << (E - T);
+  const uint32_t* sp = src + (((uint64_t)(t / W) << (E - T)) | (m << T) | (t % W));
 #pragma unroll
-  for (uint32_t r = 0; r < W / ROWS; r++) {
-    const uint32_t a = (t / W) + ROWS * r, b = t % W;
-    tile[a][b] = src[((uint64_t)a << (E - T)) | (m << T) | b];
-  }
+  for (uint32_t r = 0; r < W / ROWS; r++) tile[(t / W) + ROWS * r][t % W] = sp[r * step];
   __syncthreads();
+  uint32_t* dp = dst + (((uint64_t)(t / W) << (E - T)) | (rm << T) | (t % W));
+  const uint32_t a = __brev(t % W) >> (32 - T);
 #pragma unroll
   for (uint32_t r = 0; r < W / ROWS; r++) {
-    const uint32_t ob = (t / W) + ROWS * r, oa = t % W;
-    const uint32_t a = __brev(oa) >> (32 - T), b = __brev(ob) >> (32 - T);
-    dst[((uint64_t)ob << (E - T)) | (rm << T) | oa] = tile[a][b];
+    const uint32_t b = __brev((t / W) + ROWS * r) >> (32 - T);
+    dp[r * step] = tile[a][b];
   }
   __syncthreads();  // the next tile's writes must not overtake these reads
 }
