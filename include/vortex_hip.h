@@ -360,6 +360,41 @@ uint64_t vx_server_resident_bytes(const vx_server *s);
 int vx_server_run_trace(vx_server *s, const vx_request *reqs, const double *arrival_s, uint32_t n,
                         uint8_t *const *outs, const size_t *caps, size_t *lens, double *times);
 
+/* --------------------------------------------------------- fused commits
+ * The fused entry points named by the boundary plan: one call hashes every
+ * leaf (producing the leaf words on the fly where noted), stores tree levels
+ * store_lo..log_n into d_levels (level l at word offset
+ * 8 * sum_{store_lo <= k < l} 2^(log_n - k), 8 words per node; size from
+ * vx_merkle_levels_words) and writes the root (8 words) to d_root. They
+ * replace the reference's leaf-hash / tiled-subtree / reduce-to-root /
+ * host-upper-tree sequence (src/merkle.rs:236-451, cuda/blake2s.cu:305-532).
+ * Alphas are host [4] words, as in cuda_fold_line_soa. The calls that take
+ * alphas return after their work completes. */
+uint64_t vx_merkle_levels_words(uint32_t log_n, uint32_t store_lo);
+/* MerkleTree::commit_root_only_with_subtrees (src/merkle.rs:386-451) over
+ * 1..16 columns of 2^log_n rows (leaf = the row's words). */
+int vx_merkle_commit_tiled(const uint32_t *const *h_cols, uint32_t n_cols, uint32_t log_n, uint32_t store_lo,
+                           uint32_t *d_levels, uint32_t *d_root, vx_stream_t stream);
+/* cuda_fibonacci_quotient + MerkleTree::commit_root_soa4_with_subtrees
+ * (src/prover.rs:341-365): the QM31 quotient q_i = alpha (e[i+2] - e[i+1] -
+ * e[i]) of a 2^log_n evaluation, committed without being stored. */
+int vx_quotient_commit(const uint32_t *d_eval, const uint32_t *alpha, uint32_t log_n, uint32_t store_lo,
+                       uint32_t *d_levels, uint32_t *d_root, vx_stream_t stream);
+/* fold_circle_into_line_with_twiddles of that quotient (dst = 0) + the
+ * layer's commitment (src/prover.rs:388-413): writes line layer 0
+ * (2^(log_n-1) QM31, SoA o0..o3) and commits it (tree of log_n - 1 levels). */
+int vx_circle_fold_commit(const uint32_t *d_eval, const uint32_t *alpha_q, const uint32_t *alpha_f,
+                          const uint32_t *d_twiddles, uint32_t *o0, uint32_t *o1, uint32_t *o2, uint32_t *o3,
+                          uint32_t log_n, uint32_t store_lo, uint32_t *d_levels, uint32_t *d_root,
+                          vx_stream_t stream);
+/* fold_line_with_twiddles + the new layer's commitment (src/prover.rs:
+ * 452-493): folds 2^(log_half_n + 1) SoA values into 2^log_half_n and
+ * commits them. */
+int vx_fold_line_commit(const uint32_t *i0, const uint32_t *i1, const uint32_t *i2, const uint32_t *i3,
+                        const uint32_t *d_twiddles, uint32_t *o0, uint32_t *o1, uint32_t *o2, uint32_t *o3,
+                        const uint32_t *alpha, uint32_t log_half_n, uint32_t store_lo, uint32_t *d_levels,
+                        uint32_t *d_root, vx_stream_t stream);
+
 /* ------------------------------------------------------------ autotuner
  * No counterpart in the reference. The first commitment of each key (leaf
  * kind, columns, constraints, log size, stored levels) on a device is timed

@@ -276,6 +276,86 @@ def test_merkle_tiled(ncols, log_n):
     assert np.array_equal(host(roots).reshape(-1, 8), level)
 
 
+def _tree_levels(leaves):
+    """level l of the tree over the given leaf hashes, l = 0 .. log n"""
+    levels = [leaves]
+    while len(levels[-1]) > 1:
+        levels.append(po.hash_nodes(levels[-1]))
+    return levels
+
+
+def _check_commit(d_levels, d_root, levels, lo):
+    got = host(d_levels)
+    off = 0
+    for lv in levels[lo:]:
+        n = 8 * len(lv)
+        assert np.array_equal(got[off:off + n].reshape(-1, 8), lv)
+        off += n
+    assert off == got.size
+    assert np.array_equal(host(d_root), levels[-1][0])
+
+
+def _levels_buf(log_n, lo):
+    return empty(vx.load().vx_merkle_levels_words(log_n, lo))
+
+
+@pytest.mark.parametrize("ncols,log_n,lo", [(1, 12, 4), (3, 15, 4), (16, 18, 6), (2, 22, 10)])
+def test_merkle_commit_tiled(ncols, log_n, lo):
+    cols = [rnd(1 << log_n, 700 + log_n + k) for k in range(ncols)]
+    dcols = [dev(c) for c in cols]
+    lv, root = _levels_buf(log_n, lo), empty(8)
+    call("vx_merkle_commit_tiled", _col_ptr_array(dcols), ncols, log_n, lo, ptr(lv), ptr(root), NULL)
+    _check_commit(lv, root, _tree_levels(po.leaf_hashes(cols)), lo)
+
+
+@pytest.mark.parametrize("log_n", [10, 16, 21])
+def test_quotient_and_circle_fold_commits(log_n):
+    n = 1 << log_n
+    ev, tw = rnd(n, 900 + log_n), rnd(n // 2, 901)
+    aq, af = rnd(4, 902), rnd(4, 903)
+    d_ev, d_tw = dev(ev), dev(tw)
+    lv, root = _levels_buf(log_n, 4), empty(8)
+    call("vx_quotient_commit", ptr(d_ev), hp(aq), log_n, 4, ptr(lv), ptr(root), NULL)
+    q = po.quotient(ev, aq)
+    _check_commit(lv, root, _tree_levels(po.leaf_hashes(q)), 4)
+    outs = [empty(n // 2) for _ in range(4)]
+    lv2, root2 = _levels_buf(log_n - 1, 4), empty(8)
+    call("vx_circle_fold_commit", ptr(d_ev), hp(aq), hp(af), ptr(d_tw), *[ptr(o) for o in outs], log_n, 4,
+         ptr(lv2), ptr(root2), NULL)
+    af2 = np.ascontiguousarray(po.qm31_mul(af, af))
+    want = po.fold_circle_into_line([np.zeros(n // 2, np.uint32) for _ in range(4)], q, tw, af, af2)
+    for o, w in zip(outs, want):
+        assert np.array_equal(host(o), w)
+    _check_commit(lv2, root2, _tree_levels(po.leaf_hashes(want)), 4)
+
+
+@pytest.mark.parametrize("log_half", [9, 14, 19])
+def test_fold_line_commit(log_half):
+    half = 1 << log_half
+    cols = [rnd(2 * half, 950 + k) for k in range(4)]
+    tw, alpha = rnd(half, 960), rnd(4, 961)
+    dcols, dtw = [dev(c) for c in cols], dev(tw)
+    outs = [empty(half) for _ in range(4)]
+    lv, root = _levels_buf(log_half, 4), empty(8)
+    call("vx_fold_line_commit", *[ptr(c) for c in dcols], ptr(dtw), *[ptr(o) for o in outs], hp(alpha), log_half, 4,
+         ptr(lv), ptr(root), NULL)
+    want = po.fold_line(cols, tw, alpha)
+    for o, w in zip(outs, want):
+        assert np.array_equal(host(o), w)
+    _check_commit(lv, root, _tree_levels(po.leaf_hashes(want)), 4)
+
+
+def test_fused_commits_reject_bad_shapes():
+    lv, root = empty(64), empty(8)
+    col = dev(rnd(1 << 10, 1))
+    with pytest.raises(vx.VortexError, match="bad tree shape"):
+        call("vx_merkle_commit_tiled", _col_ptr_array([col]), 1, 10, 11, ptr(lv), ptr(root), NULL)
+    with pytest.raises(vx.VortexError, match="1..16 columns"):
+        call("vx_merkle_commit_tiled", _col_ptr_array([col]), 0, 10, 4, ptr(lv), ptr(root), NULL)
+    assert vx.load().vx_merkle_levels_words(10, 11) == 0
+    sync()
+
+
 @pytest.mark.parametrize("bits", [1, 8, 12, 16, 20])
 def test_grind_smallest_nonce(bits):
     ch = po.PyChannel()

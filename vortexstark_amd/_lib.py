@@ -72,6 +72,11 @@ SIGNATURES = {
     "vx_verify": (ip, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
     "vx_prover_set_graphs": (ip, [vp, ip]),
     "vx_tuner_configure": (ip, [ip, C.c_char_p]),
+    "vx_merkle_levels_words": (u64, [u32, u32]),
+    "vx_merkle_commit_tiled": (ip, [vp, u32, u32, u32, vp, vp, vp]),
+    "vx_quotient_commit": (ip, [vp, vp, u32, u32, vp, vp, vp]),
+    "vx_circle_fold_commit": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
+    "vx_fold_line_commit": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
     "vx_tuner_entries": (ip, [C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32)]),
     "vx_d2_prover_create": (ip, [u32, u32, u32, u64, u32, u32, u32, vp, C.POINTER(vp)]),
     "vx_d2_proof_size": (sz, [u32, u32, u32, u32]),

@@ -564,6 +564,126 @@ int vx_prover_set_graphs(vx_prover* p, int on) {
   });
 }
 
+// ------------------------------------------------------- fused commits
+// Tree levels store_lo..log_n back to back from d_levels (8 words per node).
+static TreeLevels caller_levels(uint32_t* d_levels, uint32_t log_n, uint32_t store_lo) {
+  require(d_levels != nullptr && log_n >= 2 && log_n <= 31 && store_lo >= 2 && store_lo <= log_n,
+          "commit: bad tree shape (need 2 <= store_lo <= log_n <= 31)");
+  TreeLevels T{};
+  T.base = d_levels;
+  T.log_n = log_n;
+  T.lo = store_lo;
+  uint64_t w = 0;
+  for (uint32_t l = store_lo; l <= log_n; l++) {
+    T.off[l] = w;
+    w += 8ull << (log_n - l);
+  }
+  return T;
+}
+
+uint64_t vx_merkle_levels_words(uint32_t log_n, uint32_t store_lo) {
+  if (log_n > 31 || store_lo > log_n) return 0;
+  uint64_t w = 0;
+  for (uint32_t l = store_lo; l <= log_n; l++) w += 8ull << (log_n - l);
+  return w;
+}
+
+// Alphas arrive as host [4] words (the reference passes them by value); the
+// fused producers read them from device memory, so they are staged in a
+// small buffer and the call waits for its stream before returning.
+struct StagedAlphas {
+  DeviceBuf buf{64};
+  uint32_t* put(int k, const uint32_t* a, hipStream_t s) {
+    require(a != nullptr, "commit: null alpha");
+    uint32_t* d = buf.u32() + 4 * k;
+    VX_HIP(hipMemcpyAsync(d, a, 16, hipMemcpyHostToDevice, s));
+    return d;
+  }
+};
+
+int vx_merkle_commit_tiled(const uint32_t* const* h_cols, uint32_t n_cols, uint32_t log_n, uint32_t store_lo,
+                           uint32_t* d_levels, uint32_t* d_root, vx_stream_t st) {
+  return guard([&] {
+    require(d_root != nullptr && n_cols >= 1 && n_cols <= 16, "commit_tiled: 1..16 columns and a root");
+    const TreeLevels T = caller_levels(d_levels, log_n, store_lo);
+    RootHook hook{};
+    hook.root_out = d_root;
+    merkle_commit(cols_leaf(h_cols, n_cols), T, store_lo, hook, S(st));
+  });
+}
+
+int vx_quotient_commit(const uint32_t* d_eval, const uint32_t* alpha, uint32_t log_n, uint32_t store_lo,
+                       uint32_t* d_levels, uint32_t* d_root, vx_stream_t st) {
+  return guard([&] {
+    require(d_eval != nullptr && d_root != nullptr, "quotient_commit: null pointer");
+    const TreeLevels T = caller_levels(d_levels, log_n, store_lo);
+    StagedAlphas a;
+    LeafSrc L{};
+    L.kind = LEAF_QUOTIENT;
+    L.col[0] = d_eval;
+    L.alpha_q = a.put(0, alpha, S(st));
+    L.src_mask = (1ull << log_n) - 1;
+    RootHook hook{};
+    hook.root_out = d_root;
+    merkle_commit(L, T, store_lo, hook, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_circle_fold_commit(const uint32_t* d_eval, const uint32_t* alpha_q, const uint32_t* alpha_f,
+                          const uint32_t* d_twiddles, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3,
+                          uint32_t log_n, uint32_t store_lo, uint32_t* d_levels, uint32_t* d_root, vx_stream_t st) {
+  return guard([&] {
+    require(d_eval && d_twiddles && o0 && o1 && o2 && o3 && d_root && log_n >= 3,
+            "circle_fold_commit: null pointer or log_n < 3");
+    const TreeLevels T = caller_levels(d_levels, log_n - 1, store_lo);
+    StagedAlphas a;
+    LeafSrc L{};
+    L.kind = LEAF_FOLD_CIRCLE_Q;
+    L.col[0] = d_eval;
+    L.alpha_q = a.put(0, alpha_q, S(st));
+    L.alpha_f = a.put(1, alpha_f, S(st));
+    L.tw = d_twiddles;
+    L.src_mask = (1ull << log_n) - 1;
+    uint32_t* out[4] = {o0, o1, o2, o3};
+    for (int c = 0; c < 4; c++) {
+      require(aligned16(out[c]), "circle_fold_commit: outputs must be 16-byte aligned");
+      L.out[c] = out[c];
+    }
+    RootHook hook{};
+    hook.root_out = d_root;
+    merkle_commit(L, T, store_lo, hook, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_fold_line_commit(const uint32_t* i0, const uint32_t* i1, const uint32_t* i2, const uint32_t* i3,
+                        const uint32_t* d_twiddles, uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3,
+                        const uint32_t* alpha, uint32_t log_half_n, uint32_t store_lo, uint32_t* d_levels,
+                        uint32_t* d_root, vx_stream_t st) {
+  return guard([&] {
+    require(i0 && i1 && i2 && i3 && d_twiddles && o0 && o1 && o2 && o3 && d_root, "fold_line_commit: null pointer");
+    const TreeLevels T = caller_levels(d_levels, log_half_n, store_lo);
+    StagedAlphas a;
+    LeafSrc L{};
+    L.kind = LEAF_FOLD_LINE;
+    const uint32_t* in[4] = {i0, i1, i2, i3};
+    uint32_t* out[4] = {o0, o1, o2, o3};
+    for (int c = 0; c < 4; c++) {
+      require(aligned16(in[c]) && aligned16(out[c]), "fold_line_commit: columns must be 16-byte aligned");
+      L.col[c] = in[c];
+      L.out[c] = out[c];
+    }
+    L.tw = d_twiddles;
+    require(aligned16(d_twiddles), "fold_line_commit: twiddles must be 16-byte aligned");
+    L.alpha_f = a.put(0, alpha, S(st));
+    RootHook hook{};
+    hook.root_out = d_root;
+    merkle_commit(L, T, store_lo, hook, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
 // ------------------------------------------------------------ autotuner
 int vx_tuner_configure(int enabled, const char* cache_file) {
   return guard([&] { tuner_configure(enabled != 0, cache_file); });

@@ -106,7 +106,11 @@ TreeShape measure(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, hipS
 }  // namespace
 
 TreeShape tuned_shape(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, hipStream_t s) {
-  const TreeShape dflt = merkle_default_shape(L, T.log_n);
+  TreeShape dflt = merkle_default_shape(L, T.log_n);
+  const std::vector<TreeShape> cands = merkle_shapes(T.log_n, store_lo);
+  // a caller storing lower levels than the built-in shape allows gets the
+  // first shape that does (merkle_commit rejects the call if none does)
+  if (!cands.empty() && !(dflt.lpt <= store_lo && store_lo <= dflt.tile_log)) dflt = cands[0];
   Tuner& t = tuner();
   int dev = 0;
   VX_HIP(hipGetDevice(&dev));
@@ -117,7 +121,6 @@ TreeShape tuned_shape(const LeafSrc& L, const TreeLevels& T, uint32_t store_lo, 
     auto it = t.cache.find(k);
     if (it != t.cache.end()) return it->second;
   }
-  const std::vector<TreeShape> cands = merkle_shapes(T.log_n, store_lo);
   if (cands.size() < 2) return dflt;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   VX_HIP(hipStreamIsCapturing(s, &cs));
