@@ -99,7 +99,7 @@ __device__ __forceinline__ void leaf_load(const LeafSrc& L, uint64_t i, Raw& r) 
 // the new layer).
 template <uint32_t KIND, uint32_t NC>
 __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const Raw& r, const qm31& aq,
-                                             const qm31& af, G4& g, bool st = true) {
+                                             const qm31& af, const qm31_mat& am, G4& g, bool st = true) {
   if constexpr (KIND == LEAF_COLS && NC == 1) {
     g.w[0][0] = r.a[0].x;
     g.w[1][0] = r.a[0].y;
@@ -156,7 +156,7 @@ __device__ __forceinline__ void leaf_produce(const LeafSrc& L, uint64_t i, const
     for (int j = 0; j < 4; j++) {
       qm31 f0 = {{in[0][2 * j], in[1][2 * j], in[2][2 * j], in[3][2 * j]}};
       qm31 f1 = {{in[0][2 * j + 1], in[1][2 * j + 1], in[2][2 * j + 1], in[3][2 * j + 1]}};
-      qm31 o = fri_fold_pair(f0, f1, af, t[j]);
+      qm31 o = fri_fold_pair_m(f0, f1, am, t[j]);
 #pragma unroll
       for (int c = 0; c < 4; c++) g.w[j][c] = o.v[c];
     }
@@ -353,8 +353,9 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
                            KIND == LEAF_LIFTED_RLC;  // no raw-load prefetch
   constexpr uint32_t G = 1u << (LPT_LOG - 2);
   qm31 aq{}, af{};
+  qm31_mat am{};
   if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
-  if constexpr (KIND == LEAF_FOLD_LINE) af = ldq(L.alpha_f);
+  if constexpr (KIND == LEAF_FOLD_LINE) am = qm_left_matrix(ld_uniform(reinterpret_cast<const qm31*>(L.alpha_f)));
   if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) af = qm_mul(ldq(L.alpha_f), aq);
   Raw cur, nxt;
   if constexpr (!kDirect) leaf_load<KIND, NC>(L, first, cur);
@@ -366,7 +367,7 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
     GCols gc;
     if constexpr (!kDirect) {
       if (gi + 1 < G) leaf_load<KIND, NC>(L, i + 4, nxt);
-      leaf_produce<KIND, NC>(L, i, cur, aq, af, g);
+      leaf_produce<KIND, NC>(L, i, cur, aq, af, am, g);
     }
     if constexpr (kGeneric) cols_load(L, i, gc);
     if constexpr (KIND == LEAF_D2_QUOTIENT) d2_produce(L, i, g);
@@ -432,12 +433,14 @@ __device__ __forceinline__ void lane_leaf(const LeafSrc& L, uint64_t idx, uint32
       lifted_rlc_produce(L, i, g, first);
     } else {
       qm31 aq{}, af{};
+      qm31_mat am{};
       if constexpr (KIND == LEAF_QUOTIENT || KIND == LEAF_FOLD_CIRCLE_Q) aq = ldq(L.alpha_q);
-      if constexpr (KIND == LEAF_FOLD_LINE) af = ldq(L.alpha_f);
+      if constexpr (KIND == LEAF_FOLD_LINE)
+        am = qm_left_matrix(ld_uniform(reinterpret_cast<const qm31*>(L.alpha_f)));
       if constexpr (KIND == LEAF_FOLD_CIRCLE_Q) af = qm_mul(ldq(L.alpha_f), aq);
       Raw r;
       leaf_load<KIND, NC>(L, i, r);
-      leaf_produce<KIND, NC>(L, i, r, aq, af, g, first);
+      leaf_produce<KIND, NC>(L, i, r, aq, af, am, g, first);
     }
     if constexpr (KIND == LEAF_COLS && NC == 1) {
       b2_leaf1(pick(g.w[0][0], g.w[1][0], g.w[2][0], g.w[3][0]), out);
@@ -785,7 +788,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_fri_tail(FriTailArgs a) {
     if (tid == 0)
       for (int k = 0; k < 4; k++) s_alpha[k] = T.alpha[k];
     __syncthreads();
-    const qm31 al = {{s_alpha[0], s_alpha[1], s_alpha[2], s_alpha[3]}};
+    const qm31_mat am = qm_left_matrix(qm31{{s_alpha[0], s_alpha[1], s_alpha[2], s_alpha[3]}});
     if (tid < n) {
       const uint32_t i = tid;
       qm31 f0, f1;
@@ -797,7 +800,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_fri_tail(FriTailArgs a) {
         f0 = {{vals[p][0][2 * i], vals[p][1][2 * i], vals[p][2][2 * i], vals[p][3][2 * i]}};
         f1 = {{vals[p][0][2 * i + 1], vals[p][1][2 * i + 1], vals[p][2][2 * i + 1], vals[p][3][2 * i + 1]}};
       }
-      const qm31 r = fri_fold_pair(f0, f1, al, T.tw[i]);
+      const qm31 r = fri_fold_pair_m(f0, f1, am, T.tw[i]);
       for (int c = 0; c < 4; c++) {
         vals[cur][c][i] = r.v[c];
         T.out[c][i] = r.v[c];

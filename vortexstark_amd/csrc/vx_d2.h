@@ -30,11 +30,7 @@ VX_HD uint64_t d2_mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 // x mod (2^31 - 1) of a 64-bit x: fold at bit 31 twice, then canonicalise.
-VX_HD uint32_t d2_mod_p(uint64_t x) {
-  uint64_t s = (x & P) + (x >> 31);   // < 2^34
-  uint32_t r = (uint32_t)((s & P) + (s >> 31));  // < 2^31 + 8
-  return m31_canon(r);
-}
+VX_HD uint32_t d2_mod_p(uint64_t x) { return m31_reduce64(x); }
 VX_HD uint32_t d2_value(uint64_t seed, uint32_t col, uint64_t row) {
   return d2_mod_p(d2_mix64(seed + D2_GOLDEN * ((((uint64_t)col << 32) | row) + 1)));
 }

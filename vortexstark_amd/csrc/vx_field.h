@@ -96,4 +96,42 @@ VX_HD qm31 fri_fold_pair(qm31 f0, qm31 f1, qm31 alpha, uint32_t tw) {
   return qm_add(qm_add(f0, f1), qm_mul(alpha, qm_scale(qm_sub(f0, f1), tw)));
 }
 
+// x mod P of any 64-bit x: fold at bit 31 twice, then canonicalise.
+VX_HD uint32_t m31_reduce64(uint64_t x) {
+  const uint64_t s = (x & P) + (x >> 31);               // < 2^34
+  const uint32_t r = (uint32_t)((s & P) + (s >> 31));  // < 2^31 + 8
+  return m31_canon(r);
+}
+
+// Left multiplication by a fixed QM31 x as a 4x4 matrix over M31 (u^2 =
+// 2 + i): (x y)_k = sum_j m[k][j] y_j, from expanding qm_mul:
+//   r0 = x0 y0 - x1 y1 + (2 x2 - x3) y2 - (x2 + 2 x3) y3
+//   r1 = x1 y0 + x0 y1 + (x2 + 2 x3) y2 + (2 x2 - x3) y3
+//   r2 = x2 y0 - x3 y1 + x0 y2 - x1 y3
+//   r3 = x3 y0 + x2 y1 + x1 y2 + x0 y3
+struct qm31_mat {
+  uint32_t m[4][4];
+};
+VX_HD qm31_mat qm_left_matrix(qm31 x) {
+  const uint32_t x0 = x.v[0], x1 = x.v[1], x2 = x.v[2], x3 = x.v[3];
+  const uint32_t a = m31_sub(m31_add(x2, x2), x3), b = m31_add(x2, m31_add(x3, x3));
+  return {{{x0, m31_neg(x1), a, m31_neg(b)}, {x1, x0, b, a}, {x2, m31_neg(x3), x0, m31_neg(x1)}, {x3, x2, x1, x0}}};
+}
+
+// The same fold with alpha given as its matrix: each component is a dot
+// product of four M31 products accumulated in 64 bits (4 P^2 < 2^64) and
+// reduced once, then tw * s + (f0 + f1), again reduced once.
+VX_HD qm31 fri_fold_pair_m(qm31 f0, qm31 f1, const qm31_mat& am, uint32_t tw) {
+  uint32_t d[4];
+  for (int j = 0; j < 4; j++) d[j] = m31_sub(f0.v[j], f1.v[j]);
+  qm31 r;
+  for (int k = 0; k < 4; k++) {
+    uint64_t acc = 0;
+    for (int j = 0; j < 4; j++) acc += (uint64_t)am.m[k][j] * d[j];
+    const uint32_t s = m31_reduce64(acc);
+    r.v[k] = m31_reduce64((uint64_t)tw * s + f0.v[k] + f1.v[k]);
+  }
+  return r;
+}
+
 }  // namespace vx
