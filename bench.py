@@ -290,6 +290,82 @@ def batched(vx, steps, n_cols=96, seed=42, blowup_bits=2):
     return out
 
 
+def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
+    """stwo-basis circle NTT (the stwo plugin's PolyOps): evaluate and
+    interpolate of one 2^log_n column resident in HBM, asynchronous calls
+    back to back on one stream, and the synchronous batched call over
+    batch_cols separately allocated 2^batch_log columns. The HBM figure counts
+    what the passes must move: each pass reads and writes the column once
+    (two passes at 2^24: a strided 10-bit and a contiguous 14-bit one)."""
+    import ctypes as C
+
+    import numpy as np
+    from vortexstark_amd import stwo
+    from vortexstark_amd.device import DeviceBuffer
+
+    call = vx._lib.call
+    n = 1 << log_n
+    cache = stwo.StwoNttCache(log_n)
+    x = np.random.default_rng(7).integers(0, P, n, dtype=np.uint32)
+    d = DeviceBuffer.from_host(x)
+    ptr, tw, itw = C.c_void_p(d.as_ptr()), C.c_void_p(cache.d_fwd_twiddles.as_ptr()), \
+        C.c_void_p(cache.d_inv_twiddles.as_ptr())
+    res = {}
+    for name, fn, t in (("evaluate", "vx_stwo_ntt_evaluate_async", tw),
+                        ("interpolate", "vx_stwo_ntt_interpolate_async", itw)):
+        for _ in range(2):
+            call(fn, ptr, t, n, 1, 0, None)
+        call("vx_device_sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            call(fn, ptr, t, n, 1, 0, None)
+        call("vx_device_sync")
+        res[name] = (time.perf_counter() - t0) * 1000.0 / steps
+    # evaluate and interpolate ran `steps + 2` times each: the column is back
+    roundtrip_ok = bool(np.array_equal(d.to_host(), x))
+    passes = 1 if log_n <= 14 else 2
+    bytes_moved = passes * 8 * n
+    m = 1 << batch_log
+    cols = [DeviceBuffer.from_host(np.random.default_rng(c).integers(0, P, m, dtype=np.uint32))
+            for c in range(batch_cols)]
+    cache_b = stwo.StwoNttCache(batch_log)
+    stwo.evaluate_stwo_batch(cols, cache_b)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stwo.evaluate_stwo_batch(cols, cache_b)
+    batch_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    out = {
+        "workload": f"stwo-basis evaluate / interpolate, one 2^{log_n} column in HBM",
+        "evaluate_ms": round(res["evaluate"], 4),
+        "interpolate_ms": round(res["interpolate"], 4),
+        "passes": passes,
+        "hbm_bytes_per_transform": bytes_moved,
+        "evaluate_hbm_frac": round(bytes_moved / (res["evaluate"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "interpolate_hbm_frac": round(bytes_moved / (res["interpolate"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roundtrip_ok": roundtrip_ok,
+        "batch": {"columns": batch_cols, "log_n": batch_log, "ms_per_call": round(batch_ms, 4),
+                  "columns_per_s": round(batch_cols / (batch_ms * 1e-3), 1), "synchronous": True},
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+
+        fwd, inv = po.stwo_twiddles(log_n)
+        t0 = time.perf_counter()
+        ev = po.stwo_evaluate(x, fwd)
+        te = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        back = po.stwo_interpolate(ev, inv)
+        ti = time.perf_counter() - t0
+        out["cpu_baseline"] = {"evaluate_ms": round(te * 1000.0, 1), "interpolate_ms": round(ti * 1000.0, 1),
+                               "cores": 1, "kind": "port", "roundtrip_ok": bool(np.array_equal(back, x)),
+                               "sample": f"one full 2^{log_n} evaluate and interpolate, C oracle, 1 thread"}
+    for b in cols:
+        b.free()
+    d.free()
+    return out
+
+
 def copy_calibration(vx, mib=1024):
     """A plain device-to-device copy of known size (1 GiB, four times the
     256 MB MALL, so the source streams from HBM): its FETCH_SIZE / WRITE_SIZE
@@ -322,6 +398,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--no-stwo", action="store_true")
+    ap.add_argument("--stwo-only", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: one 1 GiB device copy (counter calibration), "
                          "1 warmup + 2 timed proofs, no CPU baseline")
@@ -341,6 +419,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batched_commit": batched(vx, args.steps)}), flush=True)
+        return
+    if args.stwo_only:
+        print(json.dumps({"stwo_ntt": stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)}), flush=True)
         return
     if args.profile:
         copy_calibration(vx)
@@ -421,6 +502,8 @@ def main():
         out["serving"] = serving(vx, args.serve_requests, args.slots)
     if not args.no_batch and not args.profile:
         out["batched_commit"] = batched(vx, args.steps)
+    if not args.no_stwo and not args.profile:
+        out["stwo_ntt"] = stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(log_n, max(log_n - 4, 10))
     prover.close()

@@ -76,7 +76,8 @@ int vx_circle_ntt_interpolate(uint32_t *d_data, const uint32_t *d_itwiddles, con
 
 /* Replaces cuda_circle_ntt_evaluate_batch / _interpolate_batch
  * (ffi.rs:281-301; the default FORGE variants ffi.rs:407-428 take a HOST
- * array of column pointers, as these do). Columns of equal size n. */
+ * array of column pointers, as these do). Columns of equal size n, 16-byte
+ * aligned; each pass is one launch over all columns. Synchronous. */
 int vx_circle_ntt_evaluate_batch(uint32_t *const *h_columns, const uint32_t *d_twiddles,
                                  const uint32_t *d_circle_twids, const uint32_t *h_layer_offsets,
                                  const uint32_t *h_layer_sizes, uint32_t n_line_layers, uint32_t n,
@@ -105,6 +106,46 @@ int vx_bit_reverse_m31(uint32_t *d_data, uint32_t log_n, vx_stream_t stream);
 /* Replaces cuda_bit_reverse_qm31 (ffi.rs:1184, cuda/bit_reverse_wide.cu:21-52):
  * out-of-place, 4 words per element. Async. */
 int vx_bit_reverse_qm31(const uint32_t *d_in, uint32_t *d_out, uint32_t n, uint32_t log_n, vx_stream_t stream);
+
+/* --------------------------------------------- stwo-basis NTT (stwo plugin)
+ * The stwo Backend plugin's PolyOps transforms (crates/cuda-backend/src/
+ * poly_ops.rs:361-381 and its evaluate/extend paths), in stwo's basis
+ * f = sum c_i * prod_{bit b of i} m_b (m_0 = y, m_1 = x, m_b = 2 m_{b-1}^2 - 1),
+ * values in bit-reversed CircleDomain order. Twiddles are stwo's flat buffer
+ * of n/2 words (src/ntt.rs:296-354): line layer k at n/2 - 2^(log2(n)-1-k),
+ * then a pad word 1; the circle twiddles are derived from line layer 0.
+ * n is a power of two in [4, 2^30]; columns 16-byte aligned. At n = 4 the
+ * circle layer uses the pad word as its y twiddle, as the reference does. */
+
+/* stwo forward (d_tw) and, if d_itw is not NULL, inverse twiddle buffers of
+ * 2^(log_n - 1) words each for a 2^log_n-point domain, computed on the device
+ * (replaces the host loop of StwoNttCache::new, src/ntt.rs:296-354, and the
+ * plugin's CpuBackend::precompute_twiddles upload, poly_ops.rs:102-127). Async. */
+int vx_stwo_twiddles(uint32_t log_n, uint32_t *d_tw, uint32_t *d_itw, vx_stream_t stream);
+
+/* Replaces cuda_stwo_ntt_evaluate (src/cuda/ffi.rs:235-236,
+ * cuda/circle_ntt_stwo.cu:268-337): coefficients -> values. Synchronous like
+ * the reference. */
+int vx_stwo_ntt_evaluate(uint32_t *d_data, const uint32_t *d_twiddles, uint32_t n, vx_stream_t stream);
+/* Replaces cuda_stwo_ntt_interpolate (ffi.rs:237-238, circle_ntt_stwo.cu:
+ * 339-402): values -> coefficients, including the 2^((30 log n) mod 31) scale.
+ * Synchronous. */
+int vx_stwo_ntt_interpolate(uint32_t *d_data, const uint32_t *d_itwiddles, uint32_t n, vx_stream_t stream);
+/* Replace cuda_stwo_ntt_batch_evaluate / _batch_interpolate (ffi.rs:239-256,
+ * cuda/circle_ntt_stwo_batch.cu:239-372): n_cols columns of n words at the
+ * device addresses in the HOST array h_columns, each pass one launch for all
+ * columns. n_cols = 0 is a no-op. Synchronous. */
+int vx_stwo_ntt_batch_evaluate(uint32_t *const *h_columns, const uint32_t *d_twiddles, uint32_t n, uint32_t n_cols,
+                               vx_stream_t stream);
+int vx_stwo_ntt_batch_interpolate(uint32_t *const *h_columns, const uint32_t *d_itwiddles, uint32_t n,
+                                  uint32_t n_cols, vx_stream_t stream);
+/* No counterpart in the reference: the same transforms, asynchronous, over
+ * n_cols (1..65535) columns `stride` words apart (stride >= n when n_cols > 1),
+ * for callers that keep columns in one allocation and order work by stream. */
+int vx_stwo_ntt_evaluate_async(uint32_t *d_data, const uint32_t *d_twiddles, uint32_t n, uint32_t n_cols,
+                               uint64_t stride, vx_stream_t stream);
+int vx_stwo_ntt_interpolate_async(uint32_t *d_data, const uint32_t *d_itwiddles, uint32_t n, uint32_t n_cols,
+                                  uint64_t stride, vx_stream_t stream);
 
 /* ------------------------------------------------------------- twiddles */
 

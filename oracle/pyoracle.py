@@ -97,6 +97,10 @@ def lib(omp=False):
         "or_batch_proof_size": (sz, [_u32p, u32, u32, u32]),
         "or_build_leaves_lifted": (None, [C.c_void_p, _u32p, u32, u32, _u32p]),
         "or_merkle_root_lifted": (None, [C.c_void_p, _u32p, u32, _u32p]),
+        "or_stwo_twiddles": (None, [u32, _u32p, _u32p]),
+        "or_stwo_ntt_evaluate": (None, [_u32p, _u32p, u32]),
+        "or_stwo_ntt_interpolate": (None, [_u32p, _u32p, u32]),
+        "or_stwo_eval_at_point": (None, [_u32p, u32, _u32p, _u32p, _u32p]),
     }
     for name_, (res, args) in sig.items():
         f = getattr(L, name_)
@@ -431,3 +435,55 @@ def batch_verify(proof: bytes, blowup_bits=2, n_queries=80):
 
 def batch_proof_size(log_sizes, blowup_bits=2, n_queries=80):
     return lib().or_batch_proof_size(u32(log_sizes), len(log_sizes), blowup_bits, n_queries)
+
+
+# ---------------- stwo-basis NTT (stwo plugin PolyOps) ----------------
+
+def stwo_twiddles(log_n):
+    """(forward, inverse) flat twiddle buffers of n/2 words for a domain of 2^log_n points."""
+    assert 2 <= log_n <= 30
+    fwd = np.zeros(1 << (log_n - 1), np.uint32)
+    inv = np.zeros_like(fwd)
+    lib().or_stwo_twiddles(log_n, fwd, inv)
+    return fwd, inv
+
+
+def _log2(n):
+    log_n = n.bit_length() - 1
+    assert n == 1 << log_n and log_n >= 2
+    return log_n
+
+
+def stwo_evaluate(coeffs, tw=None, omp=False):
+    d = u32(coeffs).copy()
+    log_n = _log2(d.size)
+    tw = stwo_twiddles(log_n)[0] if tw is None else u32(tw)
+    lib(omp).or_stwo_ntt_evaluate(d, tw, log_n)
+    return d
+
+
+def stwo_interpolate(values, itw=None, omp=False):
+    d = u32(values).copy()
+    log_n = _log2(d.size)
+    itw = stwo_twiddles(log_n)[1] if itw is None else u32(itw)
+    lib(omp).or_stwo_ntt_interpolate(d, itw, log_n)
+    return d
+
+
+def stwo_eval_at_point(coeffs, zx, zy):
+    """f(z) for stwo-basis coefficients at a QM31 circle point z = (zx, zy)."""
+    c = u32(coeffs)
+    out = np.zeros(4, np.uint32)
+    lib().or_stwo_eval_at_point(c, c.size.bit_length() - 1, u32(zx), u32(zy), out)
+    return tuple(int(v) for v in out)
+
+
+def stwo_domain_point(log_n, j):
+    """Point j of the canonic CircleDomain of 2^log_n points (half_odds(log_n - 1)
+    then its conjugates)."""
+    ho = coset(HALF_ODDS, log_n - 1)
+    half = 1 << (log_n - 1)
+    if j < half:
+        return coset_at(ho, j)
+    x, y = coset_at(ho, j - half)
+    return x, (P - y) % P

@@ -1817,3 +1817,97 @@ done:
     free(beta);
     return ok;
 }
+
+/* ======================================================================
+ * stwo-basis circle NTT (the stwo Backend plugin's PolyOps).
+ * Twiddles: src/ntt.rs:296-354 (StwoNttCache::new, which restates stwo's
+ * slow_precompute_twiddles, garrick247/stwo-fork rev 5c41bf4a, for the
+ * half_odds(log_n - 1) coset). Transform: cuda/circle_ntt_stwo.cu:30-56
+ * (line layer), :202-248 (circle layer), :268-402 (layer order and the
+ * 1/n scale). The fused shared-memory and radix-4 kernels of that file
+ * (:71-200) apply the same butterflies with the same twiddle index per
+ * butterfly, so one layer at a time gives identical values.
+ * Basis and evaluation order: src/oods.rs:133-161 (eval_at_oods_from_coeffs)
+ * and src/ntt.rs:517-548 (value k is f at CircleDomain point bitrev(k)).
+ * ====================================================================== */
+void or_stwo_twiddles(uint32_t log_n, uint32_t *fwd, uint32_t *inv) {
+    uint64_t tc = 1ull << (log_n - 1), pos = 0;
+    coset_t c = coset_make(OR_HALF_ODDS, log_n - 1);
+    uint32_t *xs = (uint32_t *)malloc((tc > 1 ? tc / 2 : 1) * 4);
+    while (c.log_size > 0) {
+        /* x of the first half of the coset, stored at bit-reversed positions */
+        uint32_t layer_log = c.log_size - 1;
+        uint64_t half = 1ull << layer_log;
+        cpt p = c.init;
+        for (uint64_t j = 0; j < half; j++) {
+            xs[j] = p.x;
+            p = c_mul(p, c.step);
+        }
+        for (uint64_t j = 0; j < half; j++) fwd[pos + bitrev(j, layer_log)] = xs[j];
+        pos += half;
+        c.init = c_mul(c.init, c.init);
+        c.step = c_mul(c.step, c.step);
+        c.log_size--;
+    }
+    fwd[pos] = 1; /* pos == tc - 1: the pad */
+    free(xs);
+    /* every entry is the x of a point of order >= 8, never 0 */
+    for (uint64_t i = 0; i < tc; i++) inv[i] = m31_inv(fwd[i]);
+}
+
+/* circle layer twiddle of butterfly t: (y, -y, -x, x)[t & 3] with
+ * (x, y) = (tw[2(t >> 2)], tw[2(t >> 2) + 1]) (circle_ntt_stwo.cu:214-234) */
+static uint32_t *stwo_circle_table(const uint32_t *tw, uint64_t half_n) {
+    uint32_t *ct = (uint32_t *)malloc(half_n * 4);
+    for (uint64_t t = 0; t < half_n; t++) {
+        uint32_t x = tw[2 * (t >> 2)], y = tw[2 * (t >> 2) + 1];
+        uint32_t r = t & 3;
+        ct[t] = r == 0 ? y : r == 1 ? m31_neg(y) : r == 2 ? m31_neg(x) : x;
+    }
+    return ct;
+}
+
+void or_stwo_ntt_evaluate(uint32_t *d, const uint32_t *tw, uint32_t log_n) {
+    uint64_t n = 1ull << log_n, half_n = n / 2;
+    uint32_t nl = log_n - 1;
+    for (int k = (int)nl - 1; k >= 0; k--)
+        or_ntt_layer(d, tw + (half_n - (1ull << (nl - (uint32_t)k))), (uint32_t)k + 1, n, 1);
+    uint32_t *ct = stwo_circle_table(tw, half_n);
+    or_ntt_layer(d, ct, 0, n, 1);
+    free(ct);
+}
+
+void or_stwo_ntt_interpolate(uint32_t *d, const uint32_t *itw, uint32_t log_n) {
+    uint64_t n = 1ull << log_n, half_n = n / 2;
+    uint32_t nl = log_n - 1;
+    uint32_t *ct = stwo_circle_table(itw, half_n);
+    or_ntt_layer(d, ct, 0, n, 0);
+    free(ct);
+    for (uint32_t k = 0; k < nl; k++) or_ntt_layer(d, itw + (half_n - (1ull << (nl - k))), k + 1, n, 0);
+    uint32_t e = (30u * log_n) % 31u;
+    uint32_t inv_n = e == 0 ? 1u : (1u << e);
+    for (uint64_t i = 0; i < n; i++) d[i] = m31_mul(d[i], inv_n);
+}
+
+/* src/oods.rs:133-161, 258-290: f(z) = sum_i c_i prod_{bit b of i} m_b with
+ * m_0 = z.y, m_1 = z.x, m_b = 2 m_{b-1}^2 - 1 (src/oods.rs:96-99); folded
+ * from the lowest bit up (exact, so the order does not matter). */
+void or_stwo_eval_at_point(const uint32_t *coeffs, uint32_t log_n, const uint32_t zx[4], const uint32_t zy[4],
+                           uint32_t out[4]) {
+    uint64_t n = 1ull << log_n;
+    qm31 *v = (qm31 *)malloc(n * sizeof(qm31));
+    for (uint64_t i = 0; i < n; i++) v[i] = (qm31){{coeffs[i], 0, 0, 0}};
+    qm31 x = q_load(zx), m = q_load(zy);
+    const qm31 one = {{1, 0, 0, 0}};
+    for (uint32_t b = 0; b < log_n; b++) {
+        if (b == 1) m = x;
+        if (b >= 2) {
+            qm31 m2 = q_mul(m, m);
+            m = q_sub(q_add(m2, m2), one);
+        }
+        uint64_t half = n >> (b + 1);
+        for (uint64_t i = 0; i < half; i++) v[i] = q_add(v[2 * i], q_mul(m, v[2 * i + 1]));
+    }
+    memcpy(out, v[0].v, 16);
+    free(v);
+}

@@ -154,6 +154,14 @@ int or_batch_verify(const uint8_t *bytes, size_t len, uint32_t blowup_bits, uint
                     size_t err_cap);
 size_t or_batch_proof_size(const uint32_t *log_sizes, uint32_t n_cols, uint32_t blowup_bits, uint32_t n_queries);
 
+/* ---- stwo-basis circle NTT (stwo plugin PolyOps): src/ntt.rs:296-354,
+ *      cuda/circle_ntt_stwo.cu, src/oods.rs:133-161; log_n >= 2 ---- */
+void or_stwo_twiddles(uint32_t log_n, uint32_t *fwd, uint32_t *inv);
+void or_stwo_ntt_evaluate(uint32_t *data, const uint32_t *tw, uint32_t log_n);
+void or_stwo_ntt_interpolate(uint32_t *data, const uint32_t *itw, uint32_t log_n);
+void or_stwo_eval_at_point(const uint32_t *coeffs, uint32_t log_n, const uint32_t zx[4], const uint32_t zy[4],
+                           uint32_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,191 @@
+"""stwo-basis circle NTT (the stwo plugin's PolyOps) on the GPU, through the
+C-ABI, against the oracle (pinned by tests/test_oracle_stwo.py).
+
+Every output is bit-identical to the oracle's: the device-built twiddle
+buffers, evaluate / interpolate from n = 4 (the reference's pad-word case)
+through the register-radix sizes, the batched calls over separately allocated
+columns (one launch per pass) and the strided asynchronous variant. At 2^24
+the checks are size-independent: the roundtrip and the direct sum at sampled
+CircleDomain points."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import vortexstark_amd as vx
+from gpu_util import DevBuf, dev, empty, host, ptr, sync
+from vortexstark_amd import stwo
+from vortexstark_amd._lib import VortexError, call
+from vortexstark_amd.device import DeviceBuffer
+
+pytestmark = pytest.mark.gpu
+P = po.P
+NULL = None
+
+
+def rnd(n, seed):
+    return np.random.default_rng(seed).integers(0, P, int(n), dtype=np.uint32)
+
+
+def bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2)
+
+
+def cols_arg(ptrs):
+    return (C.c_void_p * len(ptrs))(*ptrs)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib_loaded():
+    vx.load()
+    print("stwo ntt tests: library loaded", flush=True)
+
+
+def dev_twiddles(log_n):
+    half = 1 << (log_n - 1)
+    tw, itw = empty(half), empty(half)
+    call("vx_stwo_twiddles", log_n, ptr(tw), ptr(itw), NULL)
+    sync()
+    return tw, itw
+
+
+@pytest.mark.parametrize("log_n", [2, 3, 4, 7, 13, 14, 16, 22])
+def test_twiddles_match_oracle(log_n):
+    tw, itw = dev_twiddles(log_n)
+    fwd, inv = po.stwo_twiddles(log_n)
+    assert np.array_equal(host(tw), fwd)
+    assert np.array_equal(host(itw), inv)
+
+
+@pytest.mark.parametrize("log_n", [2, 3, 4, 5, 6, 9, 12, 13, 14, 15, 16, 17, 19, 20, 23])
+def test_evaluate_interpolate_match_oracle(log_n):
+    print(f"stwo ntt log_n={log_n}", flush=True)
+    n = 1 << log_n
+    tw, itw = dev_twiddles(log_n)
+    fwd, inv = po.stwo_twiddles(log_n)
+    x = rnd(n, 100 + log_n)
+    d = dev(x)
+    call("vx_stwo_ntt_evaluate", ptr(d), ptr(tw), n, NULL)
+    ev = host(d)
+    assert np.array_equal(ev, po.stwo_evaluate(x, fwd, omp=True))
+    y = rnd(n, 200 + log_n)
+    d2 = dev(y)
+    call("vx_stwo_ntt_interpolate", ptr(d2), ptr(itw), n, NULL)
+    assert np.array_equal(host(d2), po.stwo_interpolate(y, inv, omp=True))
+    call("vx_stwo_ntt_interpolate", ptr(d), ptr(itw), n, NULL)
+    assert np.array_equal(host(d), x)
+
+
+def test_host_built_twiddles_give_the_same_values():
+    # the plugin uploads host-computed twiddles (poly_ops.rs:102-127)
+    log_n = 15
+    fwd, inv = po.stwo_twiddles(log_n)
+    x = rnd(1 << log_n, 7)
+    d = dev(x)
+    call("vx_stwo_ntt_evaluate", ptr(d), ptr(dev(fwd)), 1 << log_n, NULL)
+    assert np.array_equal(host(d), po.stwo_evaluate(x, fwd, omp=True))
+
+
+@pytest.mark.parametrize("log_n,n_cols", [(3, 4), (10, 5), (14, 3), (16, 7), (18, 2)])
+def test_batch_over_separate_columns(log_n, n_cols):
+    n = 1 << log_n
+    fwd, inv = po.stwo_twiddles(log_n)
+    tw, itw = dev_twiddles(log_n)
+    xs = [rnd(n, 300 + c) for c in range(n_cols)]
+    bufs = [dev(x) for x in xs]
+    call("vx_stwo_ntt_batch_evaluate", cols_arg([b.ptr for b in bufs]), ptr(tw), n, n_cols, NULL)
+    for x, b in zip(xs, bufs):
+        assert np.array_equal(host(b), po.stwo_evaluate(x, fwd, omp=True))
+    call("vx_stwo_ntt_batch_interpolate", cols_arg([b.ptr for b in bufs]), ptr(itw), n, n_cols, NULL)
+    for x, b in zip(xs, bufs):
+        assert np.array_equal(host(b), x)
+
+
+def test_batch_of_more_columns_than_one_grid_row():
+    # 70000 four-point columns 16 bytes apart in one allocation: two chunks
+    n_cols, n = 70000, 4
+    x = rnd(n * n_cols, 9)
+    big = dev(x)
+    tw, _ = dev_twiddles(2)
+    call("vx_stwo_ntt_batch_evaluate", cols_arg([big.ptr + 16 * c for c in range(n_cols)]), ptr(tw), n, n_cols,
+         NULL)
+    got = host(big).reshape(n_cols, n)
+    fwd, _ = po.stwo_twiddles(2)
+    for c in list(range(0, n_cols, 997)) + [65534, 65535, 65536, n_cols - 1]:
+        assert np.array_equal(got[c], po.stwo_evaluate(x[n * c:n * c + n], fwd)), c
+
+
+def test_batch_without_columns_is_a_no_op():
+    tw, _ = dev_twiddles(4)
+    call("vx_stwo_ntt_batch_evaluate", None, ptr(tw), 16, 0, NULL)
+
+
+def test_strided_async_batch():
+    log_n, n_cols = 15, 3
+    n, stride = 1 << log_n, (1 << log_n) + 64
+    fwd, inv = po.stwo_twiddles(log_n)
+    tw, itw = dev_twiddles(log_n)
+    x = rnd(stride * n_cols, 12)
+    d = dev(x)
+    call("vx_stwo_ntt_evaluate_async", ptr(d), ptr(tw), n, n_cols, stride, NULL)
+    sync()
+    got = host(d)
+    for c in range(n_cols):
+        col = x[c * stride:c * stride + n]
+        assert np.array_equal(got[c * stride:c * stride + n], po.stwo_evaluate(col, fwd, omp=True))
+        assert np.array_equal(got[c * stride + n:(c + 1) * stride], x[c * stride + n:(c + 1) * stride])
+    call("vx_stwo_ntt_interpolate_async", ptr(d), ptr(itw), n, n_cols, stride, NULL)
+    sync()
+    assert np.array_equal(host(d), x)
+
+
+def test_full_size_roundtrip_and_sampled_points():
+    log_n = 24
+    n = 1 << log_n
+    print("stwo ntt log_n=24 roundtrip", flush=True)
+    cache = stwo.StwoNttCache(log_n)
+    x = rnd(n, 24)
+    d = DeviceBuffer.from_host(x)
+    stwo.evaluate_stwo(d, cache)
+    ev = d.to_host()
+    for k in (0, 1, 12345, n - 1):
+        zx, zy = po.stwo_domain_point(log_n, bitrev(k, log_n))
+        assert po.stwo_eval_at_point(x, (zx, 0, 0, 0), (zy, 0, 0, 0)) == (int(ev[k]), 0, 0, 0), k
+    stwo.interpolate_stwo(d, cache)
+    assert np.array_equal(d.to_host(), x)
+
+
+def test_mirror_reference_vector_log8():
+    # src/ntt.rs:517-548 through the Python mirror of StwoNttCache / evaluate_stwo
+    log_n = 8
+    cache = stwo.StwoNttCache(log_n)
+    c = np.zeros(1 << log_n, np.uint32)
+    c[:4] = [42, 17, 5, 99]
+    d = DeviceBuffer.from_host(c)
+    stwo.evaluate_stwo(d, cache)
+    ev = d.to_host()
+    for k in range(1 << log_n):
+        zx, zy = po.stwo_domain_point(log_n, bitrev(k, log_n))
+        assert po.stwo_eval_at_point(c, (zx, 0, 0, 0), (zy, 0, 0, 0)) == (int(ev[k]), 0, 0, 0)
+    cols = [DeviceBuffer.from_host(rnd(1 << log_n, s)) for s in range(3)]
+    orig = [b.to_host() for b in cols]
+    stwo.evaluate_stwo_batch(cols, cache)
+    stwo.interpolate_stwo_batch(cols, cache)
+    assert all(np.array_equal(b.to_host(), o) for b, o in zip(cols, orig))
+
+
+def test_bad_arguments_are_rejected():
+    tw, _ = dev_twiddles(4)
+    d = dev(rnd(16, 1))
+    for n in (2, 1, 24, 0):
+        with pytest.raises(VortexError):
+            call("vx_stwo_ntt_evaluate", ptr(d), ptr(tw), n, NULL)
+    with pytest.raises(VortexError):
+        call("vx_stwo_ntt_evaluate", ptr(d), NULL, 16, NULL)
+    with pytest.raises(VortexError):  # misaligned column
+        call("vx_stwo_ntt_batch_evaluate", cols_arg([d.ptr + 4]), ptr(tw), 16, 1, NULL)
+    with pytest.raises(VortexError):
+        call("vx_stwo_twiddles", 1, ptr(tw), NULL, NULL)
+    # nothing was written by the rejected calls
+    assert np.array_equal(host(d), rnd(16, 1))

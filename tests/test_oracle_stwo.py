@@ -1,0 +1,143 @@
+"""Pins the oracle's stwo-basis circle NTT (oracle/vx_oracle.c, or_stwo_*).
+
+The reference holds no vectors for this path (its own checks compare with the
+stwo crate, which is not vendored), so the restatement is pinned by what the
+mathematics fixes:
+  * evaluation at a point against the basis definition, term by term
+    (1, y, x, xy, 2x^2 - 1, ...; src/oods.rs:133-161), with an independent
+    pure-Python QM31;
+  * the FFT against that direct sum at every point of the CircleDomain in
+    bit-reversed order, for full random coefficient vectors (every layer's
+    twiddles matter), which is also the reference's own check
+    (src/ntt.rs:517-548, coefficients 42, 17, 5, 99 at log_n 8);
+  * interpolation of the coordinate functions x and y, the roundtrip
+    (src/ntt.rs:923-935) and the twiddle/inverse-twiddle pairing.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import pyref
+
+P = po.P
+
+
+def bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def m31_point(x, y):
+    return (x, 0, 0, 0), (y, 0, 0, 0)
+
+
+def basis_value(idx, zx, zy):
+    """prod over the set bits b of idx of m_b: m_0 = y, m_1 = x, m_b = 2 m_{b-1}^2 - 1."""
+    r, m, b = (1, 0, 0, 0), zx, 0
+    while idx >> b:
+        f = zy if b == 0 else m
+        if b >= 2:
+            m2 = pyref.qm_mul(m, m)
+            m = pyref.qm_sub(pyref.qm_add(m2, m2), (1, 0, 0, 0))
+            f = m
+        if (idx >> b) & 1:
+            r = pyref.qm_mul(r, f)
+        b += 1
+    return r
+
+
+def test_eval_at_point_closed_forms():
+    rng = random.Random(3)
+    zx = tuple(rng.randrange(P) for _ in range(4))
+    zy = tuple(rng.randrange(P) for _ in range(4))
+    x2 = pyref.qm_mul(zx, zx)
+    dx = pyref.qm_sub(pyref.qm_add(x2, x2), (1, 0, 0, 0))  # 2x^2 - 1
+    ddx = pyref.qm_sub(pyref.qm_add(pyref.qm_mul(dx, dx), pyref.qm_mul(dx, dx)), (1, 0, 0, 0))
+    expect = {0: (1, 0, 0, 0), 1: zy, 2: zx, 3: pyref.qm_mul(zx, zy), 4: dx, 5: pyref.qm_mul(dx, zy),
+              6: pyref.qm_mul(dx, zx), 8: ddx, 15: pyref.qm_mul(pyref.qm_mul(ddx, dx), pyref.qm_mul(zx, zy))}
+    for idx, v in expect.items():
+        c = np.zeros(16, np.uint32)
+        c[idx] = 1
+        assert po.stwo_eval_at_point(c, zx, zy) == v, idx
+    # linear in the coefficients, and the general term
+    c = np.array([rng.randrange(P) for _ in range(32)], np.uint32)
+    acc = (0, 0, 0, 0)
+    for i in range(32):
+        acc = pyref.qm_add(acc, pyref.qm_scale(basis_value(i, zx, zy), int(c[i])))
+    assert po.stwo_eval_at_point(c, zx, zy) == acc
+
+
+def test_twiddle_buffer_shape():
+    for log_n in range(2, 12):
+        fwd, inv = po.stwo_twiddles(log_n)
+        assert fwd.size == 1 << (log_n - 1) and fwd[-1] == 1
+        assert all(int(a) * int(b) % P == 1 for a, b in zip(fwd, inv))
+        assert np.all(fwd < P) and np.all(fwd > 0)
+
+
+def test_reference_vector_eval_domain_log8():
+    # src/ntt.rs:517-548: value k is f at CircleDomain point bitrev(k)
+    log_n = 8
+    c = np.zeros(1 << log_n, np.uint32)
+    c[:4] = [42, 17, 5, 99]
+    ev = po.stwo_evaluate(c)
+    for k in range(1 << log_n):
+        zx, zy = m31_point(*po.stwo_domain_point(log_n, bitrev(k, log_n)))
+        assert po.stwo_eval_at_point(c, zx, zy) == (int(ev[k]), 0, 0, 0), k
+
+
+@pytest.mark.parametrize("log_n", [3, 4, 5, 7, 9])
+def test_evaluate_is_the_direct_sum_at_every_point(log_n):
+    rng = np.random.default_rng(log_n)
+    c = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+    ev = po.stwo_evaluate(c)
+    for k in range(1 << log_n):
+        zx, zy = m31_point(*po.stwo_domain_point(log_n, bitrev(k, log_n)))
+        assert po.stwo_eval_at_point(c, zx, zy) == (int(ev[k]), 0, 0, 0), (log_n, k)
+
+
+def test_four_points_use_the_pad_word():
+    """At n = 4 the twiddle buffer is [x0, 1]: the first line layer has one
+    word, so the circle layer's (x, y) pair is (x0, pad 1) and its twiddles are
+    (1, -1, -x0, x0) (cuda/circle_ntt_stwo.cu:214-234). The reference's values
+    there are this closed form, not the CircleDomain evaluation."""
+    fwd, _ = po.stwo_twiddles(2)
+    x0 = int(fwd[0])
+    assert int(fwd[1]) == 1
+    rng = random.Random(5)
+    for _ in range(20):
+        c = [rng.randrange(P) for _ in range(4)]
+        a0, a2 = (c[0] + x0 * c[2]) % P, (c[0] - x0 * c[2]) % P  # line layer, twiddle x0
+        a1, a3 = (c[1] + x0 * c[3]) % P, (c[1] - x0 * c[3]) % P
+        want = [(a0 + a1) % P, (a0 - a1) % P, (a2 - a3) % P, (a2 + a3) % P]  # circle twiddles 1, -1
+        assert po.stwo_evaluate(np.array(c, np.uint32)).tolist() == want
+
+
+@pytest.mark.parametrize("log_n", [3, 4, 6, 10])
+def test_interpolate_coordinate_functions(log_n):
+    n = 1 << log_n
+    pts = [po.stwo_domain_point(log_n, bitrev(k, log_n)) for k in range(n)]
+    for f, idx in ((lambda p: 1, 0), (lambda p: p[1], 1), (lambda p: p[0], 2),
+                   (lambda p: p[0] * p[1] % P, 3), (lambda p: (2 * p[0] * p[0] - 1) % P, 4)):
+        coef = po.stwo_interpolate(np.array([f(p) for p in pts], np.uint32))
+        want = np.zeros(n, np.uint32)
+        want[idx] = 1
+        assert np.array_equal(coef, want), (log_n, idx)
+
+
+def test_roundtrip():
+    # src/ntt.rs:923-935
+    d = np.array([(i * 7 + 3) % P for i in range(64)], np.uint32)
+    assert np.array_equal(po.stwo_interpolate(po.stwo_evaluate(d)), d)
+    rng = np.random.default_rng(11)
+    for log_n in (2, 3, 8, 13, 15):
+        d = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+        assert np.array_equal(po.stwo_evaluate(po.stwo_interpolate(d)), d)
+        assert np.array_equal(po.stwo_interpolate(po.stwo_evaluate(d)), d)
+
+
+def test_differs_from_the_fibonacci_basis():
+    # the stwo basis is not the Fibonacci prover's (src/ntt.rs:626-631)
+    c = np.arange(1, 65, dtype=np.uint32)
+    assert not np.array_equal(po.stwo_evaluate(c), po.evaluate(c, po.coset(po.HALF_COSET, 6)))

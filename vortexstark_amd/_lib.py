@@ -84,6 +84,13 @@ SIGNATURES = {
     "vx_batch_prover_create": (ip, [vp, u32, u32, u32, u32, vp, C.POINTER(vp)]),
     "vx_batch_proof_size": (sz, [vp, u32, u32, u32]),
     "vx_circle_ntt_lde_batch": (ip, [vp, vp, vp, vp, u32, u32, vp]),
+    "vx_stwo_twiddles": (ip, [u32, vp, vp, vp]),
+    "vx_stwo_ntt_evaluate": (ip, [vp, vp, u32, vp]),
+    "vx_stwo_ntt_interpolate": (ip, [vp, vp, u32, vp]),
+    "vx_stwo_ntt_batch_evaluate": (ip, [vp, vp, u32, u32, vp]),
+    "vx_stwo_ntt_batch_interpolate": (ip, [vp, vp, u32, u32, vp]),
+    "vx_stwo_ntt_evaluate_async": (ip, [vp, vp, u32, u32, u64, vp]),
+    "vx_stwo_ntt_interpolate_async": (ip, [vp, vp, u32, u32, u64, vp]),
     "vx_bit_reverse_m31_batch": (ip, [vp, vp, vp, u32, vp]),
     "vx_build_leaves_lifted": (ip, [vp, vp, u32, u32, vp, vp]),
     "vx_merkle_root_lifted": (ip, [vp, vp, u32, vp, vp]),

@@ -147,17 +147,36 @@ int vx_circle_ntt_interpolate(uint32_t* d, const uint32_t* tw, const uint32_t* c
   });
 }
 
+// Columns of a batched in-place transform: the host pointer array is checked
+// and copied to the device, in chunks of at most 65535 columns (one grid row
+// each), and every pass runs over a whole chunk in one launch. Returns after
+// the stream has finished, so the staging buffer outlives the kernels.
+static void ntt_cols(uint32_t* const* cols, uint32_t n_cols, const TwiddleRef& r, bool forward, hipStream_t s) {
+  if (n_cols == 0) return;
+  require(cols != nullptr, "null column array");
+  for (uint32_t c = 0; c < n_cols; c++)
+    require(cols[c] != nullptr && (reinterpret_cast<uintptr_t>(cols[c]) & 15u) == 0,
+            "ntt batch: columns must be non-null and 16-byte aligned");
+  constexpr uint32_t CHUNK = 65535;
+  DeviceBuf d(sizeof(uint32_t*) * (n_cols < CHUNK ? n_cols : CHUNK));
+  for (uint32_t c0 = 0; c0 < n_cols; c0 += CHUNK) {
+    const uint32_t m = n_cols - c0 < CHUNK ? n_cols - c0 : CHUNK;
+    VX_HIP(hipMemcpyAsync(d.p, cols + c0, sizeof(uint32_t*) * m, hipMemcpyHostToDevice, s));
+    uint32_t* const* dc = static_cast<uint32_t* const*>(d.p);
+    if (forward)
+      ntt_forward_cols(dc, m, r, s);
+    else
+      ntt_inverse_cols(dc, m, r, s);
+    VX_HIP(hipStreamSynchronize(s));
+  }
+}
+
 int vx_circle_ntt_evaluate_batch(uint32_t* const* cols, const uint32_t* tw, const uint32_t* ctw,
                                  const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
                                  uint32_t n_cols, vx_stream_t st) {
   return guard([&] {
-    require(n_cols == 0 || cols != nullptr, "null column array");
     TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
-    for (uint32_t c = 0; c < n_cols; c++) {
-      require(cols[c] != nullptr, "null column");
-      ntt_forward(cols[c], cols[c], n - 1, r, r.log_n, S(st));
-    }
-    VX_HIP(hipStreamSynchronize(S(st)));
+    ntt_cols(cols, n_cols, r, true, S(st));
   });
 }
 
@@ -165,13 +184,85 @@ int vx_circle_ntt_interpolate_batch(uint32_t* const* cols, const uint32_t* tw, c
                                     const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
                                     uint32_t n_cols, vx_stream_t st) {
   return guard([&] {
-    require(n_cols == 0 || cols != nullptr, "null column array");
     TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
-    for (uint32_t c = 0; c < n_cols; c++) {
-      require(cols[c] != nullptr, "null column");
-      ntt_inverse(cols[c], r, S(st));
-    }
+    ntt_cols(cols, n_cols, r, false, S(st));
+  });
+}
+
+// stwo flat twiddle buffer of an n-point transform (n/2 words, src/ntt.rs:
+// 296-354): line layer k at n/2 - 2^(log_n - 1 - k), circle twiddles derived
+// from line layer 0 (cuda/circle_ntt_stwo.cu:202-248).
+static TwiddleRef stwo_ref(const uint32_t* tw, uint32_t n) {
+  require(tw != nullptr, "stwo ntt: null twiddles");
+  require(pow2(n) && n >= 4 && n <= (1u << 30), "stwo ntt: n must be a power of two in [4, 2^30]");
+  TwiddleRef r{};
+  r.line = tw;
+  r.log_n = log2u(n);
+  r.stwo_circle = 1;
+  const uint32_t nl = r.log_n - 1;
+  for (uint32_t k = 0; k < nl; k++) r.off[k] = (n >> 1) - (1u << (nl - k));
+  return r;
+}
+
+int vx_stwo_twiddles(uint32_t log_n, uint32_t* d_tw, uint32_t* d_itw, vx_stream_t st) {
+  return guard([&] {
+    require(d_tw != nullptr, "stwo_twiddles: null output");
+    require(log_n >= 2 && log_n <= 30, "stwo_twiddles: log_n must be 2..30");
+    stwo_twiddles(log_n, d_tw, d_itw, S(st));
+  });
+}
+
+int vx_stwo_ntt_evaluate(uint32_t* d, const uint32_t* tw, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(d != nullptr, "null data");
+    TwiddleRef r = stwo_ref(tw, n);
+    ntt_forward(d, d, n - 1, r, r.log_n, S(st));
     VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_stwo_ntt_interpolate(uint32_t* d, const uint32_t* itw, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(d != nullptr, "null data");
+    TwiddleRef r = stwo_ref(itw, n);
+    ntt_inverse(d, r, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_stwo_ntt_evaluate_async(uint32_t* d, const uint32_t* tw, uint32_t n, uint32_t n_cols, uint64_t stride,
+                               vx_stream_t st) {
+  return guard([&] {
+    require(d != nullptr && n_cols >= 1 && n_cols <= 65535, "stwo ntt: bad batch");
+    TwiddleRef r = stwo_ref(tw, n);
+    ntt_forward(d, d, n - 1, r, r.log_n, S(st), n_cols, stride, stride);
+  });
+}
+
+int vx_stwo_ntt_interpolate_async(uint32_t* d, const uint32_t* itw, uint32_t n, uint32_t n_cols, uint64_t stride,
+                                  vx_stream_t st) {
+  return guard([&] {
+    require(d != nullptr && n_cols >= 1 && n_cols <= 65535, "stwo ntt: bad batch");
+    TwiddleRef r = stwo_ref(itw, n);
+    ntt_inverse(d, r, S(st), n_cols, stride);
+  });
+}
+
+int vx_stwo_ntt_batch_evaluate(uint32_t* const* cols, const uint32_t* tw, uint32_t n, uint32_t n_cols,
+                               vx_stream_t st) {
+  return guard([&] {
+    if (n_cols == 0) return;
+    TwiddleRef r = stwo_ref(tw, n);
+    ntt_cols(cols, n_cols, r, true, S(st));
+  });
+}
+
+int vx_stwo_ntt_batch_interpolate(uint32_t* const* cols, const uint32_t* itw, uint32_t n, uint32_t n_cols,
+                                  vx_stream_t st) {
+  return guard([&] {
+    if (n_cols == 0) return;
+    TwiddleRef r = stwo_ref(itw, n);
+    ntt_cols(cols, n_cols, r, false, S(st));
   });
 }
 

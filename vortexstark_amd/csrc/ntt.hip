@@ -22,19 +22,39 @@ static constexpr uint32_t NTT_THREADS = 512;
 static constexpr uint32_t NTT_TILE_LOG = 14;   // words per tile (64 KiB of LDS)
 static constexpr uint32_t NTT_STRIDED_MAX = 10; // bits per strided pass (>= 16 columns)
 
+// Kernel variants (template MODE bits):
+//   NTT_STWO  bit 0 takes stwo's circle twiddles, derived from line layer 0:
+//             butterfly t uses +-L0[(t >> 1) ^ 1], negated when t & 3 is 1 or 2
+//             (that is (y, -y, -x, x)[t & 3] of the pair (L0[2p], L0[2p + 1]),
+//             p = t >> 2, cuda/circle_ntt_stwo.cu:214-234);
+//   NTT_COLS  column blockIdx.y of the batch is cols[blockIdx.y] (in place).
+enum : int { NTT_STWO = 1, NTT_COLS = 2 };
+
 struct PassArgs {
   const uint32_t* tw[16];  // twiddle table of bit j0 + b, for local bit b
   uint32_t j0, k, cbits;
   uint64_t src_mask;
   uint32_t scale;          // output multiplier (1 = none)
   uint64_t dst_stride, src_stride;  // words between the columns of a batch (blockIdx.y)
+  uint32_t* const* cols;   // NTT_COLS: device array of column pointers
 };
 
-template <bool FWD>
+// stwo circle twiddle of circle butterfly h from line layer 0.
+__device__ __forceinline__ uint32_t stwo_circle_tw(const uint32_t* l0, uint64_t h) {
+  const uint32_t x = l0[(h >> 1) ^ 1];
+  return ((h ^ (h >> 1)) & 1) ? m31_neg(x) : x;
+}
+
+template <bool FWD, int MODE>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(uint32_t* dst, const uint32_t* src, PassArgs a) {
   extern __shared__ uint32_t lds[];
-  dst += blockIdx.y * a.dst_stride;
-  src += blockIdx.y * a.src_stride;
+  if constexpr ((MODE & NTT_COLS) != 0) {
+    dst = ld_uniform(a.cols + blockIdx.y);
+    src = dst;
+  } else {
+    dst += blockIdx.y * a.dst_stride;
+    src += blockIdx.y * a.src_stride;
+  }
   const uint32_t cmask = (1u << a.cbits) - 1;
   const uint32_t E = 1u << (a.k + a.cbits);
   const uint32_t low_tiles_log = a.j0 - a.cbits;
@@ -58,7 +78,11 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(uint32_t* dst, const u
       const uint32_t c = u & cmask, rr = u >> a.cbits;
       const uint32_t r0 = ((rr >> b) << (b + 1)) | (rr & bmask);
       const uint64_t h = (hi << (a.k - b - 1)) | (r0 >> (b + 1));
-      const uint32_t w = tw[h];
+      uint32_t w;
+      if constexpr ((MODE & NTT_STWO) != 0)
+        w = a.j0 + b == 0 ? stwo_circle_tw(tw, h) : tw[h];
+      else
+        w = tw[h];
       const uint32_t i0 = (r0 << a.cbits) | c, i1 = ((r0 | (1u << b)) << a.cbits) | c;
       const uint32_t v0 = lds[i0], v1 = lds[i1];
       if (FWD) {
@@ -105,6 +129,7 @@ struct RadixArgs {
   uint64_t src_mask;
   uint32_t scale;
   uint64_t dst_stride, src_stride;
+  uint32_t* const* cols;   // NTT_COLS: device array of column pointers
 };
 
 __device__ __forceinline__ uint32_t rx_own(uint32_t t, uint32_t s, uint32_t pos) {
@@ -135,6 +160,31 @@ __device__ __forceinline__ void rx_load_tw(const uint32_t* table, uint64_t base,
   for (uint32_t m = 0; m < NW; m++) w[m] = p[m];
 }
 
+// stwo circle bit (register bit 0 at tile bit 0): the 16 circle twiddles
+// from `base` (a multiple of 16) are +-L0[base / 2 + ((m >> 1) ^ 1)], m < 16,
+// negated when m & 3 is 1 or 2: 8 consecutive words of line layer 0.
+__device__ __forceinline__ void rx_load_stwo_circle(const uint32_t* l0, uint64_t base, uint32_t (&w)[16]) {
+  uint32_t x[8];
+  rx_load_tw<1>(l0, base >> 1, x);
+#pragma unroll
+  for (uint32_t m = 0; m < 16; m++) {
+    const uint32_t v = x[(m >> 1) ^ 1];
+    w[m] = ((m ^ (m >> 1)) & 1) ? m31_neg(v) : v;
+  }
+}
+
+template <int Q, int MODE>
+__device__ __forceinline__ void rx_load_bit(const uint32_t* table, uint32_t epos, uint64_t base,
+                                            uint32_t (&w)[1u << (4 - Q)]) {
+  if constexpr (Q == 0 && (MODE & NTT_STWO) != 0) {
+    if (epos == 0) {
+      rx_load_stwo_circle(table, base, w);
+      return;
+    }
+  }
+  rx_load_tw<Q>(table, base, w);
+}
+
 template <bool FWD, int Q>
 __device__ __forceinline__ void rx_apply(uint32_t (&v)[32], const uint32_t (&w)[1u << (4 - Q)]) {
 #pragma unroll
@@ -156,7 +206,7 @@ __device__ __forceinline__ void rx_apply(uint32_t (&v)[32], const uint32_t (&w)[
 
 // One 16384-element tile of a pass; `a` may live in kernel arguments or in
 // global memory (work-list jobs), `lds` holds 2^14 words.
-template <bool FWD, typename Args>
+template <bool FWD, int MODE, typename Args>
 __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, const Args& a, uint64_t tile,
                                            uint32_t* lds) {
   const uint32_t t = threadIdx.x;
@@ -212,7 +262,7 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   if ((Q) >= qlo && (Q) < qhi) {                                                                  \
     const uint32_t epos = pos + (Q);                                                              \
     const uint64_t base = (H << (13 - epos)) | ((uint64_t)(t >> pos) << (4 - (Q)));               \
-    rx_load_tw<(Q)>(a.tw[epos - a.cbits], base, W);                                               \
+    rx_load_bit<(Q), MODE>(a.tw[epos - a.cbits], epos, base, W);                                  \
   }
 #define VX_RX_APPLY(Q, W) \
   if ((Q) >= qlo && (Q) < qhi) rx_apply<FWD, (Q)>(v, W);
@@ -253,12 +303,17 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   }
 }
 
-template <bool FWD>
+template <bool FWD, int MODE>
 __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const uint32_t* src, RadixArgs a) {
   __shared__ uint32_t lds[1u << RX_E_BITS];
-  dst += blockIdx.y * a.dst_stride;
-  src += blockIdx.y * a.src_stride;
-  radix_tile<FWD>(dst, src, a, blockIdx.x, lds);
+  if constexpr ((MODE & NTT_COLS) != 0) {
+    dst = ld_uniform(a.cols + blockIdx.y);
+    src = dst;
+  } else {
+    dst += blockIdx.y * a.dst_stride;
+    src += blockIdx.y * a.src_stride;
+  }
+  radix_tile<FWD, MODE>(dst, src, a, blockIdx.x, lds);
 }
 
 // Work list: the tiles of many passes (columns of different sizes) in one
@@ -282,17 +337,34 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix_list(const RadixJob* j
     else hi = mid;
   }
   uniform_t<RadixJob>& J = as_uniform(jobs + lo);
-  radix_tile<FWD>(J.dst, J.src, J.a, b - J.first_tile, lds);
+  radix_tile<FWD, 0>(J.dst, J.src, J.a, b - J.first_tile, lds);
 }
 
 static const uint32_t* bit_table(const TwiddleRef& tw, uint32_t j) {
-  return j == 0 ? tw.circle : tw.line + tw.off[j - 1];
+  if (j == 0) return tw.stwo_circle ? tw.line + tw.off[0] : tw.circle;
+  return tw.line + tw.off[j - 1];
 }
 
 struct Batch {
   uint32_t n_cols;
   uint64_t dst_stride, src_stride;
+  uint32_t* const* cols;  // device column pointers (in place), or null for strided columns
 };
+
+using RadixKernel = void (*)(uint32_t*, const uint32_t*, RadixArgs);
+using PassKernel = void (*)(uint32_t*, const uint32_t*, PassArgs);
+static RadixKernel radix_kernel(bool fwd, int mode) {
+  static const RadixKernel k[2][4] = {
+      {k_ntt_radix<false, 0>, k_ntt_radix<false, 1>, k_ntt_radix<false, 2>, k_ntt_radix<false, 3>},
+      {k_ntt_radix<true, 0>, k_ntt_radix<true, 1>, k_ntt_radix<true, 2>, k_ntt_radix<true, 3>}};
+  return k[fwd ? 1 : 0][mode];
+}
+static PassKernel pass_kernel(bool fwd, int mode) {
+  static const PassKernel k[2][4] = {
+      {k_ntt_pass<false, 0>, k_ntt_pass<false, 1>, k_ntt_pass<false, 2>, k_ntt_pass<false, 3>},
+      {k_ntt_pass<true, 0>, k_ntt_pass<true, 1>, k_ntt_pass<true, 2>, k_ntt_pass<true, 3>}};
+  return k[fwd ? 1 : 0][mode];
+}
 
 // Arguments of a 16384-element radix pass over bits [j0, j0 + k) with
 // 2^cbits columns per row: rounds of <= 5 row bits from the top; evaluate
@@ -337,50 +409,41 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
   a.scale = scale;
   a.dst_stride = bt.dst_stride;
   a.src_stride = bt.src_stride;
+  a.cols = bt.cols;
   require(bt.n_cols >= 1 && bt.n_cols <= 65535, "ntt: 1..65535 columns per batch");
+  const int mode = (tw.stwo_circle ? NTT_STWO : 0) | (bt.cols ? NTT_COLS : 0);
   const uint32_t e_log = k + cbits;
   const uint64_t grid = 1ull << (log_n - e_log);
   if (e_log == RX_E_BITS) {
     RadixArgs r = make_radix_args(fwd, tw, j0, k, cbits, src_mask, scale);
     r.dst_stride = bt.dst_stride;
     r.src_stride = bt.src_stride;
-    if (fwd)
-      hipLaunchKernelGGL(k_ntt_radix<true>, dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0, s, dst, src, r);
-    else
-      hipLaunchKernelGGL(k_ntt_radix<false>, dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0, s, dst, src, r);
+    r.cols = bt.cols;
+    hipLaunchKernelGGL(radix_kernel(fwd, mode), dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0, s, dst, src,
+                       r);
     VX_HIP(hipGetLastError());
     return;
   }
   const uint32_t threads = e_log >= 10 ? NTT_THREADS : (e_log >= 7 ? (1u << (e_log - 1)) : 64u);
   const size_t lds = (size_t)4 << e_log;
-  if (fwd)
-    hipLaunchKernelGGL(k_ntt_pass<true>, dim3((uint32_t)grid, bt.n_cols), dim3(threads), lds, s, dst, src, a);
-  else
-    hipLaunchKernelGGL(k_ntt_pass<false>, dim3((uint32_t)grid, bt.n_cols), dim3(threads), lds, s, dst, src, a);
+  hipLaunchKernelGGL(pass_kernel(fwd, mode), dim3((uint32_t)grid, bt.n_cols), dim3(threads), lds, s, dst, src, a);
   VX_HIP(hipGetLastError());
 }
 
 static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
-void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
-                 uint32_t eff_bits, hipStream_t s, uint32_t n_cols, uint64_t dst_stride, uint64_t src_stride) {
+// Strided passes cover [NTT_TILE_LOG, eff_bits) from the top, then one
+// contiguous pass covers the low bits.
+static void forward_passes(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
+                           uint32_t eff_bits, hipStream_t s, Batch bt) {
   const uint32_t log_n = tw.log_n;
-  require(aligned16(data) && aligned16(src) && dst_stride % 4 == 0 && src_stride % 4 == 0,
-          "ntt: columns must be 16-byte aligned");
-  require(log_n >= 1 && log_n <= 31 && eff_bits >= 1 && eff_bits <= log_n, "ntt_forward: bad sizes");
-  require(((src_mask + 1) & src_mask) == 0 && src_mask + 1 >= (1ull << eff_bits),
-          "ntt_forward: source length must be a power of two covering the transformed bits");
-  require(n_cols == 1 || (dst_stride >= (1ull << log_n) && src_stride >= src_mask + 1),
-          "ntt_forward: batch columns must not overlap");
-  // Strided passes cover [NTT_TILE_LOG, eff_bits) from the top, then one
-  // contiguous pass covers the low bits.
+  const uint64_t dst_stride = bt.dst_stride;
   uint32_t top = eff_bits;
   const uint32_t contig = eff_bits < NTT_TILE_LOG ? eff_bits : NTT_TILE_LOG;
   const uint32_t strided_bits = eff_bits - contig;
   const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
   const uint32_t* cur = src;
   uint64_t mask = src_mask;
-  Batch bt{n_cols, dst_stride, src_stride};
   for (uint32_t p = 0; p < n_strided; p++) {
     uint32_t k = (top - contig) / (n_strided - p);  // balanced chunks
     uint32_t j0 = top - k;
@@ -393,18 +456,32 @@ void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const T
   launch_pass(true, data, cur, mask, tw, 0, contig, 0, 1u, bt, s);
 }
 
-void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n_cols, uint64_t stride) {
+void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
+                 uint32_t eff_bits, hipStream_t s, uint32_t n_cols, uint64_t dst_stride, uint64_t src_stride) {
   const uint32_t log_n = tw.log_n;
-  require(aligned16(data) && stride % 4 == 0, "ntt: columns must be 16-byte aligned");
-  require(log_n >= 1 && log_n <= 31, "ntt_inverse: bad size");
-  require(n_cols == 1 || stride >= (1ull << log_n), "ntt_inverse: batch columns must not overlap");
+  require(aligned16(data) && aligned16(src) && dst_stride % 4 == 0 && src_stride % 4 == 0,
+          "ntt: columns must be 16-byte aligned");
+  require(log_n >= 1 && log_n <= 31 && eff_bits >= 1 && eff_bits <= log_n, "ntt_forward: bad sizes");
+  require(((src_mask + 1) & src_mask) == 0 && src_mask + 1 >= (1ull << eff_bits),
+          "ntt_forward: source length must be a power of two covering the transformed bits");
+  require(n_cols == 1 || (dst_stride >= (1ull << log_n) && src_stride >= src_mask + 1),
+          "ntt_forward: batch columns must not overlap");
+  forward_passes(data, src, src_mask, tw, eff_bits, s, Batch{n_cols, dst_stride, src_stride, nullptr});
+}
+
+void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s) {
+  require(d_cols != nullptr && tw.log_n >= 1 && tw.log_n <= 31, "ntt_forward_cols: bad arguments");
+  forward_passes(nullptr, nullptr, (1ull << tw.log_n) - 1, tw, tw.log_n, s, Batch{n_cols, 0, 0, d_cols});
+}
+
+static void inverse_passes(uint32_t* data, const TwiddleRef& tw, hipStream_t s, const Batch& bt) {
+  const uint32_t log_n = tw.log_n;
   const uint32_t e = (30u * log_n) % 31u;
   const uint32_t inv_n = e == 0 ? 1u : (1u << e);
   const uint64_t mask = (1ull << log_n) - 1;
   const uint32_t contig = log_n < NTT_TILE_LOG ? log_n : NTT_TILE_LOG;
   const uint32_t strided_bits = log_n - contig;
   const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
-  const Batch bt{n_cols, stride, stride};
   launch_pass(false, data, data, mask, tw, 0, contig, 0, n_strided == 0 ? inv_n : 1u, bt, s);
   uint32_t bot = contig;
   for (uint32_t p = 0; p < n_strided; p++) {
@@ -413,6 +490,19 @@ void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n
     launch_pass(false, data, data, mask, tw, bot, k, NTT_TILE_LOG - k, last ? inv_n : 1u, bt, s);
     bot += k;
   }
+}
+
+void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n_cols, uint64_t stride) {
+  const uint32_t log_n = tw.log_n;
+  require(aligned16(data) && stride % 4 == 0, "ntt: columns must be 16-byte aligned");
+  require(log_n >= 1 && log_n <= 31, "ntt_inverse: bad size");
+  require(n_cols == 1 || stride >= (1ull << log_n), "ntt_inverse: batch columns must not overlap");
+  inverse_passes(data, tw, s, Batch{n_cols, stride, stride, nullptr});
+}
+
+void ntt_inverse_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s) {
+  require(d_cols != nullptr && tw.log_n >= 1 && tw.log_n <= 31, "ntt_inverse_cols: bad arguments");
+  inverse_passes(nullptr, tw, s, Batch{n_cols, 0, 0, d_cols});
 }
 
 // ---------------------------------------------------------------------------
@@ -434,7 +524,8 @@ void lde_batch_build(const LdeBatchCol* cols, uint32_t n, uint32_t blowup_bits, 
   for (uint32_t c = 0; c < n; c++) {
     const LdeBatchCol& C = cols[c];
     const uint32_t L = C.log_n, E = L + blowup_bits;
-    require(L >= RX_E_BITS && L <= 24 && E <= 31 && C.itw.log_n == L && C.etw.log_n == E && C.coef && C.eval,
+    require(L >= RX_E_BITS && L <= 24 && E <= 31 && C.itw.log_n == L && C.etw.log_n == E && C.coef && C.eval &&
+                !C.itw.stwo_circle && !C.etw.stwo_circle,
             "lde_batch: columns need 14 <= log_n <= 24 and matching twiddles");
     require(aligned16(C.coef) && aligned16(C.eval) && (!C.in || aligned16(C.in)),
             "lde_batch: columns must be 16-byte aligned");

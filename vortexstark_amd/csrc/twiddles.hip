@@ -97,4 +97,38 @@ void fold_twiddle_sources(cpt init, cpt step, uint32_t* out, uint64_t n, uint32_
   VX_HIP(hipGetLastError());
 }
 
+// stwo flat twiddle buffer of a 2^log_n-point transform (src/ntt.rs:296-354):
+// level l is the half_odds(log_n - 1) coset doubled l times; its 2^b words
+// (b = log_n - 2 - l) sit at [n/2 - 2^(b+1), n/2 - 2^b) and hold, at position
+// bitrev_b(j), the x of that coset's point j, which is pi^l(x(init * step^j))
+// with pi(x) = 2x^2 - 1 (doubling a point squares it). The last word is 1.
+__global__ void k_stwo_twiddles(cpt init, cpt step, uint32_t* out, uint32_t log_n) {
+  const uint64_t half = 1ull << (log_n - 1);
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= half) return;
+  if (i == half - 1) {
+    out[i] = 1u;
+    return;
+  }
+  const uint64_t r = half - 1 - i;          // in [2^b, 2^(b+1)) for level size 2^b
+  const uint32_t b = 63u - (uint32_t)__clzll(r);
+  const uint64_t pos = i + (2ull << b) - half;
+  const uint64_t j = b == 0 ? 0 : (__brevll(pos) >> (64 - b));
+  uint32_t x = c_mul(init, c_pow(step, j)).x;
+  for (uint32_t l = log_n - 2 - b; l > 0; l--) {
+    const uint32_t x2 = m31_mul(x, x);
+    x = m31_sub(m31_add(x2, x2), 1u);
+  }
+  out[i] = x;
+}
+void stwo_twiddles(uint32_t log_n, uint32_t* tw, uint32_t* itw, hipStream_t s) {
+  require(log_n >= 2 && log_n <= 31, "stwo_twiddles: log_n must be 2..31");
+  const coset c = make_coset(HALF_ODDS, log_n - 1);
+  const uint64_t half = 1ull << (log_n - 1);
+  hipLaunchKernelGGL(k_stwo_twiddles, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, s, c.init, c.step, tw,
+                     log_n);
+  VX_HIP(hipGetLastError());
+  if (itw) batch_inverse(tw, itw, half, s);  // entries are x of points of order >= 8: never 0
+}
+
 }  // namespace vx

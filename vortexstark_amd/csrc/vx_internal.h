@@ -73,11 +73,17 @@ __device__ __forceinline__ T ld_uniform(const T* p) {
 // --------------------------------------------------------- NTT (ntt.hip)
 // Reference-layout twiddle tables of a coset (src/circle.rs:284-347):
 //   line: layer l occupies [off[l], off[l] + (N >> (l+1))), circle: N/2 y-coords.
+// stwo layout (src/ntt.rs:296-354, one flat buffer of N/2 words): line layer l
+// at off[l] = N/2 - 2^(log_n - 1 - l), and stwo_circle set: the circle
+// twiddle of butterfly t is derived from line layer 0 as
+// (y, -y, -x, x)[t & 3] with (x, y) = (line[2(t >> 2)], line[2(t >> 2) + 1]),
+// i.e. +-line[(t >> 1) ^ 1] (cuda/circle_ntt_stwo.cu:202-248); `circle` is unused.
 struct TwiddleRef {
   const uint32_t* line;
   const uint32_t* circle;
   uint32_t off[32];
   uint32_t log_n;
+  uint32_t stwo_circle;
 };
 
 // Multi-bit butterfly pass plan entry.
@@ -100,6 +106,11 @@ void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const T
                  uint32_t eff_bits, hipStream_t s, uint32_t n_cols = 1, uint64_t dst_stride = 0,
                  uint64_t src_stride = 0);
 void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n_cols = 1, uint64_t stride = 0);
+// In-place transforms of n_cols columns of 2^log_n words each at arbitrary
+// (16-byte aligned) addresses: d_cols is a device array of column pointers.
+// Same passes as ntt_forward / ntt_inverse, one launch per pass for all columns.
+void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s);
+void ntt_inverse_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s);
 // Uneven batched LDE (columns of log sizes 14..24): host-built work lists
 // (lde_batch_build) uploaded by the caller, then four launches.
 struct LdeBatchCol {
@@ -138,6 +149,9 @@ void extract_and_squash(const uint32_t* in, uint32_t* tw_out, uint32_t* sq_out, 
 void batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s);
 void fold_twiddle_sources(cpt init, cpt step, uint32_t* out, uint64_t n, uint32_t log_n,
                           bool extract_y, hipStream_t s);
+// stwo flat forward twiddles (n/2 words) of a 2^log_n-point transform and,
+// if itw is set, their inverses (src/ntt.rs:296-354).
+void stwo_twiddles(uint32_t log_n, uint32_t* tw, uint32_t* itw, hipStream_t s);
 
 // ------------------------------------------------- misc kernels (fri.hip)
 void fib_trace(uint32_t a, uint32_t b, uint32_t* out, uint32_t log_n, hipStream_t s);
