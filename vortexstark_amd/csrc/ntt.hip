@@ -303,6 +303,14 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   }
 }
 
+// Workgroup b runs on XCD b mod 8 (round-robin dispatch). A strided pass's
+// tiles are 64-byte row segments, so tiles t and t ^ 1 share every 128-byte
+// line; giving each XCD a contiguous range of tiles makes the two run back
+// to back on one XCD, where the second finds the lines in that XCD's L2.
+__device__ __forceinline__ uint64_t xcd_tile(uint32_t b, uint32_t grid) {
+  return (grid & 7u) == 0 ? (uint64_t)(b & 7u) * (grid >> 3) + (b >> 3) : b;
+}
+
 template <bool FWD, int MODE>
 __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const uint32_t* src, RadixArgs a) {
   __shared__ uint32_t lds[1u << RX_E_BITS];
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const u
     dst += blockIdx.y * a.dst_stride;
     src += blockIdx.y * a.src_stride;
   }
-  radix_tile<FWD, MODE>(dst, src, a, blockIdx.x, lds);
+  radix_tile<FWD, MODE>(dst, src, a, xcd_tile(blockIdx.x, gridDim.x), lds);
 }
 
 // Work list: the tiles of many passes (columns of different sizes) in one
