@@ -334,6 +334,23 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     for _ in range(steps):
         stwo.evaluate_stwo_batch(cols, cache_b)
     batch_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    # PolyOps::eval_at_point of the same column (now values, any 2^24 words
+    # serve) and barycentric_eval_at_point over 2^22 values with the plugin's
+    # 256 blocks; synchronous calls, as the plugin makes them
+    zx, zy = (3, 1, 4, 1), (5, 9, 2, 6)
+    stwo.eval_at_point(d, n, zx, zy)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stwo.eval_at_point(d, n, zx, zy)
+    eval_pt_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    nb = 1 << 22
+    e = DeviceBuffer.from_host(np.random.default_rng(8).integers(0, P, nb, dtype=np.uint32))
+    w = DeviceBuffer.from_host(np.random.default_rng(9).integers(0, P, 4 * nb, dtype=np.uint32))
+    stwo.barycentric_eval(e, w, nb)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stwo.barycentric_eval(e, w, nb)
+    bary_ms = (time.perf_counter() - t0) * 1000.0 / steps
     out = {
         "workload": f"stwo-basis evaluate / interpolate, one 2^{log_n} column in HBM",
         "evaluate_ms": round(res["evaluate"], 4),
@@ -345,6 +362,10 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
         "roundtrip_ok": roundtrip_ok,
         "batch": {"columns": batch_cols, "log_n": batch_log, "ms_per_call": round(batch_ms, 4),
                   "columns_per_s": round(batch_cols / (batch_ms * 1e-3), 1), "synchronous": True},
+        "eval_at_point": {"log_n": log_n, "ms_per_call": round(eval_pt_ms, 4),
+                          "read_gbs": round(4 * n / (eval_pt_ms * 1e-3) / 1e9, 1), "synchronous": True},
+        "barycentric": {"n": nb, "blocks": 256, "ms_per_call": round(bary_ms, 4),
+                        "read_gbs": round(20 * nb / (bary_ms * 1e-3) / 1e9, 1), "synchronous": True},
     }
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -360,9 +381,8 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
         out["cpu_baseline"] = {"evaluate_ms": round(te * 1000.0, 1), "interpolate_ms": round(ti * 1000.0, 1),
                                "cores": 1, "kind": "port", "roundtrip_ok": bool(np.array_equal(back, x)),
                                "sample": f"one full 2^{log_n} evaluate and interpolate, C oracle, 1 thread"}
-    for b in cols:
+    for b in cols + [e, w, d]:
         b.free()
-    d.free()
     return out
 
 

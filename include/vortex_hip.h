@@ -147,6 +147,24 @@ int vx_stwo_ntt_evaluate_async(uint32_t *d_data, const uint32_t *d_twiddles, uin
 int vx_stwo_ntt_interpolate_async(uint32_t *d_data, const uint32_t *d_itwiddles, uint32_t n, uint32_t n_cols,
                                   uint64_t stride, vx_stream_t stream);
 
+/* Replaces cuda_eval_at_point (src/cuda/ffi.rs:334-341, cuda/circle_ntt.cu:
+ * 186-231,429-468; the plugin's PolyOps::eval_at_point, crates/cuda-backend/
+ * src/poly_ops.rs:382-443): f(z) of n stwo-basis coefficients, given the
+ * log2(n) QM31 folding factors (device, 4 words each, factors[0] folding the
+ * top coefficient bit: the mappings [z.y, z.x, 2 z.x^2 - 1, ...] reversed).
+ * The QM31 result is written to the HOST array h_result[4]. The scratch
+ * buffers need max(4, n / 1024) words each (the reference's callers pass
+ * 2n). n is a power of two; coefficients 16-byte aligned. Synchronous. */
+int vx_eval_at_point(const uint32_t *d_coeffs, const uint32_t *d_folding_factors, uint32_t *h_result, uint32_t n,
+                     uint32_t *d_scratch1, uint32_t *d_scratch2, vx_stream_t stream);
+/* Replaces cuda_barycentric_eval (ffi.rs:1037-1043, cuda/barycentric_eval.cu:
+ * 41-110): d_out[4b..4b+4) = sum of d_evals[i] * d_weights[i] (QM31, 4 words
+ * each) over i = b*256 + t + k*n_blocks*256 (t < 256), the reference's
+ * partition, so each partial equals the reference's; the caller adds the
+ * n_blocks partials. Weights 16-byte aligned. Synchronous. */
+int vx_barycentric_eval(const uint32_t *d_evals, const uint32_t *d_weights, uint32_t n, uint32_t *d_out,
+                        uint32_t n_blocks, vx_stream_t stream);
+
 /* ------------------------------------------------------------- twiddles */
 
 /* Replaces cuda_compute_coset_points (ffi.rs:453-458, cuda/twiddles.cu:80-105,182-194). */

@@ -101,6 +101,8 @@ def lib(omp=False):
         "or_stwo_ntt_evaluate": (None, [_u32p, _u32p, u32]),
         "or_stwo_ntt_interpolate": (None, [_u32p, _u32p, u32]),
         "or_stwo_eval_at_point": (None, [_u32p, u32, _u32p, _u32p, _u32p]),
+        "or_eval_at_point_factors": (None, [_u32p, _u32p, u32, _u32p]),
+        "or_barycentric_partials": (None, [_u32p, _u32p, u64, u32, _u32p]),
     }
     for name_, (res, args) in sig.items():
         f = getattr(L, name_)
@@ -487,3 +489,31 @@ def stwo_domain_point(log_n, j):
         return coset_at(ho, j)
     x, y = coset_at(ho, j - half)
     return x, (P - y) % P
+
+
+def folding_factors(zx, zy, log_n):
+    """The plugin's factors for a point (poly_ops.rs:400-410): [z.y, z.x,
+    2 z.x^2 - 1, ...] (log_n entries) reversed, flattened to 4 words each."""
+    maps = [tuple(int(v) for v in zy)]
+    x = tuple(int(v) for v in zx)
+    one = (1, 0, 0, 0)
+    for _ in range(1, log_n):
+        maps.append(x)
+        x2 = qm31_mul(x, x)
+        x = tuple((2 * int(a) - b) % P for a, b in zip(x2, one))
+    maps.reverse()
+    return np.array([w for m in maps for w in m] or [0], np.uint32)
+
+
+def eval_at_point_factors(coeffs, factors):
+    c = u32(coeffs)
+    out = np.zeros(4, np.uint32)
+    lib().or_eval_at_point_factors(c, u32(factors), c.size.bit_length() - 1, out)
+    return tuple(int(v) for v in out)
+
+
+def barycentric_partials(evals, weights, n_blocks):
+    e = u32(evals)
+    out = np.zeros(4 * n_blocks, np.uint32)
+    lib().or_barycentric_partials(e, u32(weights), e.size, n_blocks, out)
+    return out

@@ -1911,3 +1911,41 @@ void or_stwo_eval_at_point(const uint32_t *coeffs, uint32_t log_n, const uint32_
     memcpy(out, v[0].v, 16);
     free(v);
 }
+
+/* cuda/circle_ntt.cu:186-231, 429-468 (cuda_eval_at_point): fold the
+ * coefficient vector in halves, level l combining v[t] + factors[l] * v[t + half]. */
+void or_eval_at_point_factors(const uint32_t *coeffs, const uint32_t *factors, uint32_t log_n, uint32_t out[4]) {
+    uint64_t n = 1ull << log_n;
+    if (log_n == 0) {
+        out[0] = coeffs[0]; out[1] = out[2] = out[3] = 0;
+        return;
+    }
+    uint64_t half = n / 2;
+    qm31 *v = (qm31 *)malloc(half * sizeof(qm31));
+    qm31 f = q_load(factors);
+    for (uint64_t t = 0; t < half; t++) {
+        qm31 bf = q_scale(f, coeffs[t + half]);
+        v[t] = bf;
+        v[t].v[0] = m31_add(coeffs[t], bf.v[0]);
+    }
+    for (uint32_t level = 1; level < log_n; level++) {
+        half /= 2;
+        f = q_load(factors + 4 * level);
+        for (uint64_t t = 0; t < half; t++) v[t] = q_add(v[t], q_mul(v[t + half], f));
+    }
+    memcpy(out, v[0].v, 16);
+    free(v);
+}
+
+/* cuda/barycentric_eval.cu:41-110: out[b] = sum of evals[i] * weights[i]
+ * over i = b*256 + t + k*n_blocks*256, t < 256. */
+void or_barycentric_partials(const uint32_t *evals, const uint32_t *weights, uint64_t n, uint32_t n_blocks,
+                             uint32_t *out) {
+    for (uint32_t b = 0; b < n_blocks; b++) {
+        qm31 acc = {{0, 0, 0, 0}};
+        for (uint64_t base = (uint64_t)b * 256; base < n; base += (uint64_t)n_blocks * 256)
+            for (uint64_t i = base; i < base + 256 && i < n; i++)
+                acc = q_add(acc, q_scale(q_load(weights + 4 * i), evals[i]));
+        memcpy(out + 4 * (size_t)b, acc.v, 16);
+    }
+}

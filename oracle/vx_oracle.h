@@ -161,6 +161,11 @@ void or_stwo_ntt_evaluate(uint32_t *data, const uint32_t *tw, uint32_t log_n);
 void or_stwo_ntt_interpolate(uint32_t *data, const uint32_t *itw, uint32_t log_n);
 void or_stwo_eval_at_point(const uint32_t *coeffs, uint32_t log_n, const uint32_t zx[4], const uint32_t zy[4],
                            uint32_t out[4]);
+/* cuda/circle_ntt.cu:186-231,429-468 (cuda_eval_at_point, folding factors given);
+ * cuda/barycentric_eval.cu:41-110 (per-block partial sums) */
+void or_eval_at_point_factors(const uint32_t *coeffs, const uint32_t *factors, uint32_t log_n, uint32_t out[4]);
+void or_barycentric_partials(const uint32_t *evals, const uint32_t *weights, uint64_t n, uint32_t n_blocks,
+                             uint32_t *out);
 
 #ifdef __cplusplus
 }

@@ -189,3 +189,68 @@ def test_bad_arguments_are_rejected():
         call("vx_stwo_twiddles", 1, ptr(tw), NULL, NULL)
     # nothing was written by the rejected calls
     assert np.array_equal(host(d), rnd(16, 1))
+
+
+# ---- evaluation at a point (PolyOps::eval_at_point) and barycentric sums
+
+def eval_gpu(coeffs_dev, factors, n):
+    words = max(4, n // 1024)  # the documented minimum scratch
+    s1, s2 = empty(words), empty(words)
+    res = np.zeros(4, np.uint32)
+    call("vx_eval_at_point", ptr(coeffs_dev), ptr(dev(factors)), C.c_void_p(res.ctypes.data), n, ptr(s1), ptr(s2),
+         NULL)
+    return tuple(int(v) for v in res)
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 2, 3, 4, 5, 8, 11, 12, 13, 16, 20, 21, 24])
+def test_eval_at_point_matches_oracle(log_n):
+    print(f"eval_at_point log_n={log_n}", flush=True)
+    rng = np.random.default_rng(500 + log_n)
+    n = 1 << log_n
+    c = rng.integers(0, P, n, dtype=np.uint32)
+    zx, zy = (tuple(int(v) for v in rng.integers(0, P, 4)) for _ in range(2))
+    f = po.folding_factors(zx, zy, log_n)
+    assert eval_gpu(dev(c), f, n) == po.eval_at_point_factors(c, f)
+
+
+def test_eval_at_point_agrees_with_evaluate():
+    # value k of the stwo NTT is f at CircleDomain point bitrev(k)
+    log_n = 20
+    n = 1 << log_n
+    c = rnd(n, 77)
+    d = dev(c)
+    cache = stwo.StwoNttCache(log_n)
+    e = DeviceBuffer.from_host(c)
+    stwo.evaluate_stwo(e, cache)
+    ev = e.to_host()
+    for k in (0, 3, 99999, n - 1):
+        x, y = po.stwo_domain_point(log_n, bitrev(k, log_n))
+        assert eval_gpu(d, po.folding_factors((x, 0, 0, 0), (y, 0, 0, 0), log_n), n) == (int(ev[k]), 0, 0, 0)
+
+
+@pytest.mark.parametrize("n,n_blocks", [(0, 1), (1, 1), (255, 1), (256, 1), (1000, 4), (4097, 3),
+                                        (1 << 20, 256), (1 << 20, 7), (12345, 300)])
+def test_barycentric_partials_match_oracle(n, n_blocks):
+    rng = np.random.default_rng(n + n_blocks)
+    e = rng.integers(0, P, max(n, 1), dtype=np.uint32)[:n]
+    w = rng.integers(0, P, max(4 * n, 4), dtype=np.uint32)[:4 * n]
+    de, dw = dev(e if n else np.zeros(1, np.uint32)), dev(w if n else np.zeros(4, np.uint32))
+    out = empty(4 * n_blocks)
+    call("vx_barycentric_eval", ptr(de), ptr(dw), n, ptr(out), n_blocks, NULL)
+    assert np.array_equal(host(out), po.barycentric_partials(e, w, n_blocks))
+
+
+def test_mirror_eval_at_point_and_barycentric():
+    log_n = 14
+    n = 1 << log_n
+    c = rnd(n, 31)
+    zx, zy = (5, 6, 7, 8), (9, 10, 11, 12)
+    got = stwo.eval_at_point(DeviceBuffer.from_host(c), n, zx, zy)
+    assert got == po.stwo_eval_at_point(c, zx, zy)
+    assert stwo.folding_factors(zx, zy, log_n) == po.folding_factors(zx, zy, log_n).tolist()
+    e, w = rnd(3000, 32), rnd(12000, 33)
+    total = tuple(int(v) for v in po.barycentric_partials(e, w, 1))
+    assert stwo.barycentric_eval(DeviceBuffer.from_host(e), DeviceBuffer.from_host(w), 3000) == total
+    with pytest.raises(VortexError):
+        call("vx_eval_at_point", ptr(dev(c)), ptr(dev(np.zeros(56, np.uint32))), C.c_void_p(np.zeros(4, np.uint32).ctypes.data),
+             n - 1, ptr(empty(64)), ptr(empty(64)), NULL)

@@ -141,3 +141,44 @@ def test_differs_from_the_fibonacci_basis():
     # the stwo basis is not the Fibonacci prover's (src/ntt.rs:626-631)
     c = np.arange(1, 65, dtype=np.uint32)
     assert not np.array_equal(po.stwo_evaluate(c), po.evaluate(c, po.coset(po.HALF_COSET, 6)))
+
+
+# ---- evaluation from folding factors (cuda_eval_at_point) and barycentric sums
+
+def test_factor_fold_equals_point_evaluation():
+    # two restatements: halves folded with factors[0] first (circle_ntt.cu:
+    # 429-468) against adjacent pairs folded from z.y up (src/oods.rs)
+    rng = np.random.default_rng(21)
+    for log_n in range(0, 11):
+        zx = tuple(int(v) for v in rng.integers(0, P, 4))
+        zy = tuple(int(v) for v in rng.integers(0, P, 4))
+        c = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+        f = po.folding_factors(zx, zy, log_n)
+        assert po.eval_at_point_factors(c, f) == po.stwo_eval_at_point(c, zx, zy), log_n
+    # factors[0] folds the top coefficient bit: a unit vector at the top index
+    # evaluates to factors[0]
+    f = np.arange(1, 4 * 6 + 1, dtype=np.uint32)
+    c = np.zeros(64, np.uint32)
+    c[32] = 1
+    assert po.eval_at_point_factors(c, f) == (1, 2, 3, 4)
+    c[:] = 0
+    c[1] = 1
+    assert po.eval_at_point_factors(c, f) == tuple(range(21, 25))
+
+
+def test_barycentric_partials():
+    rng = np.random.default_rng(22)
+    for n, nb in ((1, 1), (300, 2), (1000, 3), (5000, 7)):
+        e = rng.integers(0, P, n, dtype=np.uint32)
+        w = rng.integers(0, P, 4 * n, dtype=np.uint32)
+        parts = po.barycentric_partials(e, w, nb).reshape(nb, 4).astype(object)
+        total = [sum(int(e[i]) * int(w[4 * i + k]) for i in range(n)) % P for k in range(4)]
+        assert [int(v) % P for v in parts.sum(axis=0) % P] == total
+        # the partition: element i lands in block (i // 256) % n_blocks
+        for i in (0, n - 1, n // 2):
+            u = np.zeros(n, np.uint32)
+            u[i] = 1
+            p1 = po.barycentric_partials(u, w, nb).reshape(nb, 4)
+            b = (i // 256) % nb
+            assert p1[b].tolist() == w[4 * i:4 * i + 4].tolist()
+            assert not np.any(np.delete(p1, b, axis=0))

@@ -85,6 +85,8 @@ SIGNATURES = {
     "vx_batch_proof_size": (sz, [vp, u32, u32, u32]),
     "vx_circle_ntt_lde_batch": (ip, [vp, vp, vp, vp, u32, u32, vp]),
     "vx_stwo_twiddles": (ip, [u32, vp, vp, vp]),
+    "vx_eval_at_point": (ip, [vp, vp, vp, u32, vp, vp, vp]),
+    "vx_barycentric_eval": (ip, [vp, vp, u32, vp, u32, vp]),
     "vx_stwo_ntt_evaluate": (ip, [vp, vp, u32, vp]),
     "vx_stwo_ntt_interpolate": (ip, [vp, vp, u32, vp]),
     "vx_stwo_ntt_batch_evaluate": (ip, [vp, vp, u32, u32, vp]),

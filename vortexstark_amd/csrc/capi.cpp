@@ -248,6 +248,28 @@ int vx_stwo_ntt_interpolate_async(uint32_t* d, const uint32_t* itw, uint32_t n, 
   });
 }
 
+int vx_eval_at_point(const uint32_t* d_coeffs, const uint32_t* d_factors, uint32_t* h_result, uint32_t n,
+                     uint32_t* d_scratch1, uint32_t* d_scratch2, vx_stream_t st) {
+  return guard([&] {
+    require(d_coeffs && d_factors && h_result && d_scratch1 && d_scratch2, "eval_at_point: null argument");
+    require(pow2(n) && n <= (1u << 31), "eval_at_point: n must be a power of two");
+    require(n < 16 || (reinterpret_cast<uintptr_t>(d_coeffs) & 15u) == 0,
+            "eval_at_point: coefficients must be 16-byte aligned");
+    const uint32_t* r = eval_at_point(d_coeffs, d_factors, log2u(n), d_scratch1, d_scratch2, S(st));
+    VX_HIP(hipMemcpyAsync(h_result, r, 16, hipMemcpyDeviceToHost, S(st)));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+int vx_barycentric_eval(const uint32_t* d_evals, const uint32_t* d_weights, uint32_t n, uint32_t* d_out,
+                        uint32_t n_blocks, vx_stream_t st) {
+  return guard([&] {
+    require(d_out != nullptr && (n == 0 || (d_evals && d_weights)), "barycentric_eval: null argument");
+    barycentric_partials(d_evals, d_weights, n, d_out, n_blocks, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
 int vx_stwo_ntt_batch_evaluate(uint32_t* const* cols, const uint32_t* tw, uint32_t n, uint32_t n_cols,
                                vx_stream_t st) {
   return guard([&] {

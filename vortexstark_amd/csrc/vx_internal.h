@@ -142,6 +142,17 @@ void bit_reverse_m31(uint32_t* data, uint32_t log_n, hipStream_t s);
 void bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint32_t log_n, hipStream_t s);
 void zero_pad(const uint32_t* src, uint32_t* dst, uint64_t src_n, uint64_t dst_n, hipStream_t s);
 
+// ------------------------------------------- point evaluation (oods.hip)
+// f(z) of 2^log_n stwo-basis coefficients from the QM31 folding factors
+// (factors[0] folds the top coefficient bit). Uses s1 and s2 (each at least
+// max(4, 2^log_n / 1024) words) as scratch; returns the one holding the QM31.
+const uint32_t* eval_at_point(const uint32_t* coeffs, const uint32_t* factors, uint32_t log_n, uint32_t* s1,
+                              uint32_t* s2, hipStream_t s);
+// Per-block partial sums of evals[i] * weights[i] over the reference's
+// grid-stride partition (n_blocks blocks of 256 lanes); out: n_blocks QM31.
+void barycentric_partials(const uint32_t* evals, const uint32_t* weights, uint32_t n, uint32_t* out,
+                          uint32_t n_blocks, hipStream_t s);
+
 // ----------------------------------------------- twiddles (twiddles.hip)
 void coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n, hipStream_t s);
 void extract_and_squash(const uint32_t* in, uint32_t* tw_out, uint32_t* sq_out, uint64_t half_n,

@@ -70,3 +70,58 @@ def interpolate_stwo_batch(columns, cache, stream=None):
     """In-place interpolate of equal-size columns, one launch per pass for all of them."""
     call("vx_stwo_ntt_batch_interpolate", _cols(columns), _ptr(cache.d_inv_twiddles), cache.n, len(columns),
          _stream(stream))
+
+
+# ---- PolyOps::eval_at_point / barycentric_eval_at_point (poly_ops.rs:382-500)
+
+P = 0x7FFFFFFF
+
+
+def qm31_mul(x, y):
+    """QM31 product, words [a.re, a.im, b.re, b.im] with u^2 = 2 + i."""
+    def cm(a, b):
+        return ((a[0] * b[0] - a[1] * b[1]) % P, (a[0] * b[1] + a[1] * b[0]) % P)
+    a, b, c, d = (x[0], x[1]), (x[2], x[3]), (y[0], y[1]), (y[2], y[3])
+    ac, bd, ad, bc = cm(a, c), cm(b, d), cm(a, d), cm(b, c)
+    bdu = cm(bd, (2, 1))
+    return ((ac[0] + bdu[0]) % P, (ac[1] + bdu[1]) % P, (ad[0] + bc[0]) % P, (ad[1] + bc[1]) % P)
+
+
+def folding_factors(zx, zy, log_n):
+    """[z.y, z.x, 2 z.x^2 - 1, ...] (log_n entries) reversed, as the plugin
+    builds them (poly_ops.rs:400-410); 4 words per factor."""
+    maps = [tuple(int(v) % P for v in zy)]
+    x = tuple(int(v) % P for v in zx)
+    for _ in range(1, log_n):
+        maps.append(x)
+        x2 = qm31_mul(x, x)
+        x = ((2 * x2[0] - 1) % P, 2 * x2[1] % P, 2 * x2[2] % P, 2 * x2[3] % P)
+    maps.reverse()
+    return [w for m in maps for w in m]
+
+
+def eval_at_point(d_coeffs, n, zx, zy, stream=None):
+    """f(z) of n stwo-basis coefficients on the device at the QM31 point
+    z = (zx, zy) (4-word tuples); returns the QM31 as a 4-tuple."""
+    log_n = n.bit_length() - 1
+    if n <= 0 or n != 1 << log_n:
+        raise ValueError("n must be a power of two")
+    f = folding_factors(zx, zy, log_n) or [0, 0, 0, 0]
+    d_f = DeviceBuffer.from_host(f)
+    words = max(4, n // 1024)
+    s1, s2 = DeviceBuffer(words), DeviceBuffer(words)
+    res = (C.c_uint32 * 4)()
+    call("vx_eval_at_point", _ptr(d_coeffs), _ptr(d_f), res, n, _ptr(s1), _ptr(s2), _stream(stream))
+    return tuple(res)
+
+
+def barycentric_eval(d_evals, d_weights, n, stream=None):
+    """sum_i evals[i] * weights[i] (weights QM31, 4 words each): at most 256
+    blocks of partial sums on the device, added here (poly_ops.rs:445-500)."""
+    if n == 0:
+        return (0, 0, 0, 0)
+    n_blocks = min((n + 255) // 256, 256)
+    d_out = DeviceBuffer(4 * n_blocks)
+    call("vx_barycentric_eval", _ptr(d_evals), _ptr(d_weights), n, _ptr(d_out), n_blocks, _stream(stream))
+    parts = d_out.to_host().reshape(n_blocks, 4).astype(object)
+    return tuple(int(v) % P for v in parts.sum(axis=0))
