@@ -152,9 +152,10 @@ int vx_stwo_ntt_interpolate_async(uint32_t *d_data, const uint32_t *d_itwiddles,
  * src/poly_ops.rs:382-443): f(z) of n stwo-basis coefficients, given the
  * log2(n) QM31 folding factors (device, 4 words each, factors[0] folding the
  * top coefficient bit: the mappings [z.y, z.x, 2 z.x^2 - 1, ...] reversed).
- * The QM31 result is written to the HOST array h_result[4]. The scratch
- * buffers need max(4, n / 1024) words each (the reference's callers pass
- * 2n). n is a power of two; coefficients 16-byte aligned. Synchronous. */
+ * The QM31 result is written to the HOST array h_result[4]. Each scratch
+ * buffer needs 4 words for n < 2^14, else max(80, n / 1024) words (the
+ * reference's callers pass 2n). n is a power of two; coefficients 16-byte
+ * aligned. Synchronous. */
 int vx_eval_at_point(const uint32_t *d_coeffs, const uint32_t *d_folding_factors, uint32_t *h_result, uint32_t n,
                      uint32_t *d_scratch1, uint32_t *d_scratch2, vx_stream_t stream);
 /* Replaces cuda_barycentric_eval (ffi.rs:1037-1043, cuda/barycentric_eval.cu:

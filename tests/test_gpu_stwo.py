@@ -194,7 +194,7 @@ def test_bad_arguments_are_rejected():
 # ---- evaluation at a point (PolyOps::eval_at_point) and barycentric sums
 
 def eval_gpu(coeffs_dev, factors, n):
-    words = max(4, n // 1024)  # the documented minimum scratch
+    words = 4 if n < (1 << 14) else max(80, n // 1024)  # the documented minimum scratch
     s1, s2 = empty(words), empty(words)
     res = np.zeros(4, np.uint32)
     call("vx_eval_at_point", ptr(coeffs_dev), ptr(dev(factors)), C.c_void_p(res.ctypes.data), n, ptr(s1), ptr(s2),
@@ -202,7 +202,7 @@ def eval_gpu(coeffs_dev, factors, n):
     return tuple(int(v) for v in res)
 
 
-@pytest.mark.parametrize("log_n", [0, 1, 2, 3, 4, 5, 8, 11, 12, 13, 16, 20, 21, 24])
+@pytest.mark.parametrize("log_n", [0, 1, 2, 3, 4, 5, 8, 11, 12, 13, 14, 15, 16, 17, 20, 21, 24])
 def test_eval_at_point_matches_oracle(log_n):
     print(f"eval_at_point log_n={log_n}", flush=True)
     rng = np.random.default_rng(500 + log_n)

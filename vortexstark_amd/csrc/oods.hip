@@ -9,12 +9,12 @@
 //
 // MI355X design: the reference folds level by level (log_n launches, each
 // reading and writing a QM31 vector, 16 bytes per element at the first
-// level). Here the M31 coefficients are read once. Lane t multiplies its
-// 16 consecutive coefficients by the 16 products of F[0..3] (built per
-// workgroup in LDS), accumulating 4 products per 64-bit sum before a
-// reduction. A 256-lane workgroup then folds its lanes in LDS with F[4..11],
-// and the 4096-coefficient block sums are folded 256 at a time with the
-// next 8 factors.
+// level). Here the M31 coefficients are read once, by k_eval_big (n >= 2^14:
+// 64 coefficients per lane, coalesced loads, the basis as scalars) or
+// k_eval_blocks (smaller n: 16 consecutive coefficients per lane against the
+// 16 products of F[0..3] in LDS). Products are accumulated 4 per 64-bit sum
+// before a reduction; a workgroup folds its lanes in LDS, and the block sums
+// are folded 256 at a time with the next 8 factors.
 #include "vx_internal.h"
 
 namespace vx {
@@ -91,6 +91,69 @@ __global__ __launch_bounds__(EV_WG) void k_eval_tree(const uint32_t* __restrict_
   if (threadIdx.x == 0) reinterpret_cast<qm31*>(out)[blockIdx.x] = r;
 }
 
+// ---- n >= 2^14: 64 coefficients per lane, basis in scalar registers.
+// A workgroup covers 2^14 coefficients. Lane t reads the 16 uint4 at
+// t + 256 m (m < 16), fully coalesced, so coefficient r + 4t + 1024m sits in
+// component r of load m: bits 0-1 = r, 2-9 = t, 10-13 = m. The setup kernel
+// tabulates E[r + 4 m01] = prod of F[0], F[1], F[10], F[11] over the set
+// bits of (r, m01) and D[m23] over F[12], F[13]; lanes read them as
+// wave-uniform scalars. Per lane: four 16-term sums w[m23] against E, then
+// w0 + D1 w1 + D2 w2 + D3 w3; the 256 lanes are folded with F[2..9].
+// (Per 2^14 coefficients a wave issues about 1600 VALU instructions, a
+// quarter of the 16-per-lane kernel's; at 2^24 this kernel takes 27 us
+// against 50 us. Looping each workgroup over several chunks with the next
+// chunk's loads in flight measured the same 27 us.)
+static constexpr uint32_t EV_BIG_LOG = 14;
+
+__global__ void k_eval_setup(const uint32_t* factors, uint32_t log_n, uint32_t* table) {
+  const uint32_t q = threadIdx.x;  // 0..15: E[q]; 16..19: D[q - 16]
+  qm31 b{{1u, 0u, 0u, 0u}};
+  if (q < 16) {
+    const uint32_t bits[4] = {0, 1, 10, 11};
+    for (uint32_t k = 0; k < 4; k++)
+      if ((q >> k) & 1u) b = qm_mul(b, ld_factor(factors, log_n, bits[k]));
+  } else if (q < 20) {
+    for (uint32_t k = 0; k < 2; k++)
+      if (((q - 16) >> k) & 1u) b = qm_mul(b, ld_factor(factors, log_n, 12 + k));
+  }
+  if (q < 20) reinterpret_cast<qm31*>(table)[q] = b;
+}
+
+__global__ __launch_bounds__(EV_WG) void k_eval_big(const uint32_t* __restrict__ c, const uint32_t* table,
+                                                    const uint32_t* factors, uint32_t log_n,
+                                                    uint32_t* __restrict__ part) {
+  __shared__ qm31 q[EV_WG];
+  const uint32_t t = threadIdx.x;
+  const uint4* p = reinterpret_cast<const uint4*>(c) + ((uint64_t)blockIdx.x << (EV_BIG_LOG - 2)) + t;
+  uint4 x[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; m++) x[m] = p[m * EV_WG];
+  uniform_t<qm31>* E = (uniform_t<qm31>*)table;
+  qm31 w[4];
+#pragma unroll
+  for (uint32_t m23 = 0; m23 < 4; m23++) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      uint32_t r = 0;
+#pragma unroll
+      for (uint32_t m01 = 0; m01 < 4; m01++) {
+        const uint4 v = x[m01 + 4 * m23];
+        // 4 products < 4 P^2 < 2^64
+        const uint64_t acc = (uint64_t)v.x * E[4 * m01].v[k] + (uint64_t)v.y * E[4 * m01 + 1].v[k] +
+                             (uint64_t)v.z * E[4 * m01 + 2].v[k] + (uint64_t)v.w * E[4 * m01 + 3].v[k];
+        r = m31_add(r, m31_reduce64(acc));
+      }
+      w[m23].v[k] = r;
+    }
+  }
+  qm31 v = w[0];
+#pragma unroll
+  for (uint32_t m23 = 1; m23 < 4; m23++)
+    v = qm_add(v, qm_mul(ld_uniform(reinterpret_cast<const qm31*>(table) + 16 + m23), w[m23]));
+  const qm31 s = lds_fold(q, v, factors, log_n, 2);
+  if (t == 0) reinterpret_cast<qm31*>(part)[blockIdx.x] = s;
+}
+
 // n < 16: one lane.
 __global__ void k_eval_small(const uint32_t* c, const uint32_t* factors, uint32_t log_n, uint32_t* out) {
   qm31 v[8];
@@ -116,14 +179,25 @@ const uint32_t* eval_at_point(const uint32_t* c, const uint32_t* factors, uint32
     VX_HIP(hipGetLastError());
     return s1;
   }
-  const uint32_t lanes_log = log_n - EV_LANE_LOG;
-  const uint32_t wg_log = lanes_log < EV_WG_LOG ? lanes_log : EV_WG_LOG;
-  uint32_t groups_log = lanes_log - wg_log;
-  hipLaunchKernelGGL(k_eval_blocks, dim3(1u << groups_log), dim3(1u << wg_log), 0, s, c, factors, log_n, s1);
-  VX_HIP(hipGetLastError());
-  uint32_t fb = EV_LANE_LOG + wg_log;
+  uint32_t groups_log, fb;
   uint32_t* in = s1;
   uint32_t* out = s2;
+  if (log_n >= EV_BIG_LOG) {
+    // the 20-entry table sits in s2 until the first tree level overwrites it
+    hipLaunchKernelGGL(k_eval_setup, dim3(1), dim3(64), 0, s, factors, log_n, s2);
+    VX_HIP(hipGetLastError());
+    groups_log = log_n - EV_BIG_LOG;
+    hipLaunchKernelGGL(k_eval_big, dim3(1u << groups_log), dim3(EV_WG), 0, s, c, s2, factors, log_n, s1);
+    VX_HIP(hipGetLastError());
+    fb = EV_BIG_LOG;
+  } else {
+    const uint32_t lanes_log = log_n - EV_LANE_LOG;
+    const uint32_t wg_log = lanes_log < EV_WG_LOG ? lanes_log : EV_WG_LOG;
+    groups_log = lanes_log - wg_log;
+    hipLaunchKernelGGL(k_eval_blocks, dim3(1u << groups_log), dim3(1u << wg_log), 0, s, c, factors, log_n, s1);
+    VX_HIP(hipGetLastError());
+    fb = EV_LANE_LOG + wg_log;
+  }
   while (groups_log > 0) {
     const uint32_t w = groups_log < EV_WG_LOG ? groups_log : EV_WG_LOG;
     groups_log -= w;

@@ -144,8 +144,9 @@ void zero_pad(const uint32_t* src, uint32_t* dst, uint64_t src_n, uint64_t dst_n
 
 // ------------------------------------------- point evaluation (oods.hip)
 // f(z) of 2^log_n stwo-basis coefficients from the QM31 folding factors
-// (factors[0] folds the top coefficient bit). Uses s1 and s2 (each at least
-// max(4, 2^log_n / 1024) words) as scratch; returns the one holding the QM31.
+// (factors[0] folds the top coefficient bit). Uses s1 and s2 as scratch (each
+// at least 4 words below 2^14 coefficients, else max(80, 2^log_n / 1024)
+// words); returns the one holding the QM31.
 const uint32_t* eval_at_point(const uint32_t* coeffs, const uint32_t* factors, uint32_t log_n, uint32_t* s1,
                               uint32_t* s2, hipStream_t s);
 // Per-block partial sums of evals[i] * weights[i] over the reference's

@@ -108,7 +108,7 @@ def eval_at_point(d_coeffs, n, zx, zy, stream=None):
         raise ValueError("n must be a power of two")
     f = folding_factors(zx, zy, log_n) or [0, 0, 0, 0]
     d_f = DeviceBuffer.from_host(f)
-    words = max(4, n // 1024)
+    words = 4 if n < (1 << 14) else max(80, n // 1024)
     s1, s2 = DeviceBuffer(words), DeviceBuffer(words)
     res = (C.c_uint32 * 4)()
     call("vx_eval_at_point", _ptr(d_coeffs), _ptr(d_f), res, n, _ptr(s1), _ptr(s2), _stream(stream))
