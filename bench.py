@@ -351,6 +351,53 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     for _ in range(steps):
         stwo.barycentric_eval(e, w, nb)
     bary_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    # QuotientOps at 2^22 rows: accumulate_numerators over 16 columns, then
+    # compute_quotients_combine of 2 accumulators (asynchronous calls)
+    qr, qc = 1 << 22, 16
+    qcols = [DeviceBuffer.from_host(np.random.default_rng(40 + i).integers(0, P, qr, dtype=np.uint32))
+             for i in range(qc)]
+
+    def dptrs(bufs):
+        return DeviceBuffer.from_host(np.array([b.as_ptr() for b in bufs], np.uint64).view(np.uint32))
+
+    d_cols = dptrs(qcols)
+    d_idx = DeviceBuffer.from_host(np.arange(qc, dtype=np.uint32))
+    d_b = DeviceBuffer.from_host(np.random.default_rng(50).integers(0, P, 4 * qc, dtype=np.uint32))
+    d_c = DeviceBuffer.from_host(np.random.default_rng(51).integers(0, P, 4 * qc, dtype=np.uint32))
+    accs = [[DeviceBuffer(qr) for _ in range(4)] for _ in range(2)]
+    vp = C.c_void_p
+
+    def acc_call(o):
+        call("vx_accumulate_numerators", vp(d_cols.as_ptr()), vp(d_idx.as_ptr()), vp(d_b.as_ptr()),
+             vp(d_c.as_ptr()), qc, qr, *[vp(x.as_ptr()) for x in o], None)
+
+    for o in accs:
+        acc_call(o)
+    call("vx_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        acc_call(accs[0])
+    call("vx_device_sync")
+    accnum_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    rng = np.random.default_rng(52)
+    d_sp = [DeviceBuffer.from_host(rng.integers(0, P, 8, dtype=np.uint32)) for _ in range(3)]
+    d_nu = [dptrs([accs[j][k] for j in range(2)]) for k in range(4)]
+    d_logs = DeviceBuffer.from_host(np.array([22, 22], np.uint32))
+    d_xs, d_ys = (DeviceBuffer.from_host(rng.integers(0, P, qr, dtype=np.uint32)) for _ in range(2))
+    q_out = [DeviceBuffer(qr) for _ in range(4)]
+
+    def comb_call():
+        call("vx_compute_quotients_combine", *[vp(x.as_ptr()) for x in d_sp], *[vp(x.as_ptr()) for x in d_nu],
+             vp(d_logs.as_ptr()), 2, vp(d_xs.as_ptr()), vp(d_ys.as_ptr()), 22, qr,
+             *[vp(x.as_ptr()) for x in q_out], None)
+
+    comb_call()
+    call("vx_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        comb_call()
+    call("vx_device_sync")
+    comb_ms = (time.perf_counter() - t0) * 1000.0 / steps
     out = {
         "workload": f"stwo-basis evaluate / interpolate, one 2^{log_n} column in HBM",
         "evaluate_ms": round(res["evaluate"], 4),
@@ -366,6 +413,10 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
                           "read_gbs": round(4 * n / (eval_pt_ms * 1e-3) / 1e9, 1), "synchronous": True},
         "barycentric": {"n": nb, "blocks": 256, "ms_per_call": round(bary_ms, 4),
                         "read_gbs": round(20 * nb / (bary_ms * 1e-3) / 1e9, 1), "synchronous": True},
+        "accumulate_numerators": {"rows": qr, "columns": qc, "ms_per_call": round(accnum_ms, 4),
+                                  "hbm_gbs": round(4 * qr * (qc + 4) / (accnum_ms * 1e-3) / 1e9, 1)},
+        "quotients_combine": {"rows": qr, "accumulators": 2, "ms_per_call": round(comb_ms, 4),
+                              "hbm_gbs": round(4 * qr * (2 + 8 + 4) / (comb_ms * 1e-3) / 1e9, 1)},
     }
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -381,7 +432,7 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
         out["cpu_baseline"] = {"evaluate_ms": round(te * 1000.0, 1), "interpolate_ms": round(ti * 1000.0, 1),
                                "cores": 1, "kind": "port", "roundtrip_ok": bool(np.array_equal(back, x)),
                                "sample": f"one full 2^{log_n} evaluate and interpolate, C oracle, 1 thread"}
-    for b in cols + [e, w, d]:
+    for b in cols + [e, w, d] + qcols + [x for a in accs for x in a] + q_out:
         b.free()
     return out
 

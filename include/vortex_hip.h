@@ -166,6 +166,46 @@ int vx_eval_at_point(const uint32_t *d_coeffs, const uint32_t *d_folding_factors
 int vx_barycentric_eval(const uint32_t *d_evals, const uint32_t *d_weights, uint32_t n, uint32_t *d_out,
                         uint32_t n_blocks, vx_stream_t stream);
 
+/* ------------------------------------- stwo plugin QuotientOps (quotients)
+ * Replaces cuda_accumulate_numerators (src/cuda/ffi.rs:1059-1070,
+ * cuda/quotient.cu:13-50,240-257), called by the plugin's
+ * QuotientOps::accumulate_numerators (crates/cuda-backend/src/
+ * quotient_ops.rs:120-208): for row < n_rows,
+ *   out[row] = sum_i (c_i * d_col_ptrs[d_col_indices[i]][row] - b_i)
+ * with b_i, c_i QM31 (4 words each) and the result as 4 SoA columns.
+ * d_col_ptrs, d_col_indices, d_b, d_c are DEVICE arrays. Async. */
+int vx_accumulate_numerators(const uint32_t *const *d_col_ptrs, const uint32_t *d_col_indices, const uint32_t *d_b,
+                             const uint32_t *d_c, uint32_t n_batch_cols, uint32_t n_rows, uint32_t *out0,
+                             uint32_t *out1, uint32_t *out2, uint32_t *out3, vx_stream_t stream);
+/* Replaces cuda_accumulate_numerators_dual (ffi.rs:1072-1089, quotient.cu:
+ * 56-131,259-282): the z sums over all n_cols_z columns and, from the same
+ * column reads, the z_next sums over the columns whose d_zn_coeff_idx[i] is
+ * not 0xFFFFFFFF, with coefficients b_zn / c_zn at that index. The reference
+ * stages at most 128 columns in shared memory; here any count. Async. */
+int vx_accumulate_numerators_dual(const uint32_t *const *d_col_ptrs, const uint32_t *d_col_indices_z,
+                                  const uint32_t *d_zn_coeff_idx, const uint32_t *d_b_z, const uint32_t *d_c_z,
+                                  const uint32_t *d_b_zn, const uint32_t *d_c_zn, uint32_t n_cols_z, uint32_t n_rows,
+                                  uint32_t *out_z0, uint32_t *out_z1, uint32_t *out_z2, uint32_t *out_z3,
+                                  uint32_t *out_zn0, uint32_t *out_zn1, uint32_t *out_zn2, uint32_t *out_zn3,
+                                  vx_stream_t stream);
+/* Replaces cuda_compute_quotients_combine (ffi.rs:1091-1124, quotient.cu:
+ * 153-233,284-312; QuotientOps::compute_quotients_and_combine,
+ * quotient_ops.rs:210-342): for row < n_rows with domain point
+ * d = (d_xs[row], d_ys[row]),
+ *   out[row] = sum_j (numer_j[lifted(row)] - first_linear_acc_j * d.y) / den_j,
+ *   den_j = (Re p_j.x - d.x) Im p_j.y - (Re p_j.y - d.y) Im p_j.x   (CM31),
+ *   lifted(row) = ((row >> (lifting_log_size - acc_log_sizes[j] + 1)) << 1) + (row & 1),
+ * each CM31 half of the QM31 numerator divided by den_j (zero inverts to
+ * zero). Sample points, accumulators, pointer arrays and log sizes are
+ * DEVICE arrays; n_rows <= 2^lifting_log_size. Async. */
+int vx_compute_quotients_combine(const uint32_t *d_sample_points_x, const uint32_t *d_sample_points_y,
+                                 const uint32_t *d_first_linear_acc, const uint32_t *const *d_numer_ptrs0,
+                                 const uint32_t *const *d_numer_ptrs1, const uint32_t *const *d_numer_ptrs2,
+                                 const uint32_t *const *d_numer_ptrs3, const uint32_t *d_acc_log_sizes,
+                                 uint32_t n_accs, const uint32_t *d_domain_xs, const uint32_t *d_domain_ys,
+                                 uint32_t lifting_log_size, uint32_t n_rows, uint32_t *out0, uint32_t *out1,
+                                 uint32_t *out2, uint32_t *out3, vx_stream_t stream);
+
 /* ------------------------------------------------------------- twiddles */
 
 /* Replaces cuda_compute_coset_points (ffi.rs:453-458, cuda/twiddles.cu:80-105,182-194). */

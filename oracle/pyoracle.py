@@ -103,6 +103,10 @@ def lib(omp=False):
         "or_stwo_eval_at_point": (None, [_u32p, u32, _u32p, _u32p, _u32p]),
         "or_eval_at_point_factors": (None, [_u32p, _u32p, u32, _u32p]),
         "or_barycentric_partials": (None, [_u32p, _u32p, u64, u32, _u32p]),
+        "or_accumulate_numerators": (None, [C.c_void_p, _u32p, _u32p, _u32p, u32, u64, _u32p, _u32p, _u32p,
+                                            _u32p]),
+        "or_quotients_combine": (None, [_u32p, _u32p, _u32p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        _u32p, u32, _u32p, _u32p, u32, u64, _u32p, _u32p, _u32p, _u32p]),
     }
     for name_, (res, args) in sig.items():
         f = getattr(L, name_)
@@ -516,4 +520,24 @@ def barycentric_partials(evals, weights, n_blocks):
     e = u32(evals)
     out = np.zeros(4 * n_blocks, np.uint32)
     lib().or_barycentric_partials(e, u32(weights), e.size, n_blocks, out)
+    return out
+
+
+def accumulate_numerators(cols, col_indices, b, c, n_rows):
+    """out[k][row] = sum_i (c_i * cols[col_indices[i]][row] - b_i), QM31 as 4 SoA columns."""
+    cs = [u32(x) for x in cols]
+    out = [np.zeros(n_rows, np.uint32) for _ in range(4)]
+    lib().or_accumulate_numerators(_ptrs(cs), u32(col_indices), u32(b) if len(b) else np.zeros(1, np.uint32),
+                                   u32(c) if len(c) else np.zeros(1, np.uint32), len(col_indices), n_rows, *out)
+    return out
+
+
+def quotients_combine(sp_x, sp_y, fla, numers, acc_log_sizes, domain_xs, domain_ys, lifting_log_size):
+    """numers: per accumulator, 4 SoA columns of 2^acc_log_sizes[j] words."""
+    n_rows = len(domain_xs)
+    ns = [[u32(col) for col in acc] for acc in numers]
+    out = [np.zeros(n_rows, np.uint32) for _ in range(4)]
+    lib().or_quotients_combine(u32(sp_x), u32(sp_y), u32(fla), *[_ptrs([acc[k] for acc in ns]) for k in range(4)],
+                               u32(acc_log_sizes), len(ns), u32(domain_xs), u32(domain_ys), lifting_log_size,
+                               n_rows, *out)
     return out

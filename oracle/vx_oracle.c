@@ -1949,3 +1949,46 @@ void or_barycentric_partials(const uint32_t *evals, const uint32_t *weights, uin
         memcpy(out + 4 * (size_t)b, acc.v, 16);
     }
 }
+
+/* ======================================================================
+ * stwo plugin QuotientOps: cuda/quotient.cu:13-50 (accumulate_numerators),
+ * :56-131 (the fused z / z_next form), :153-233 (compute_quotients_combine).
+ * Column and numerator pointer arrays are given as host arrays here.
+ * ====================================================================== */
+void or_accumulate_numerators(const uint32_t *const *cols, const uint32_t *col_indices, const uint32_t *b,
+                              const uint32_t *c, uint32_t n_batch_cols, uint64_t n_rows, uint32_t *o0,
+                              uint32_t *o1, uint32_t *o2, uint32_t *o3) {
+    for (uint64_t row = 0; row < n_rows; row++) {
+        qm31 acc = {{0, 0, 0, 0}};
+        for (uint32_t i = 0; i < n_batch_cols; i++) {
+            uint32_t f = cols[col_indices[i]][row];
+            acc = q_add(acc, q_sub(q_scale(q_load(c + 4 * i), f), q_load(b + 4 * i)));
+        }
+        o0[row] = acc.v[0]; o1[row] = acc.v[1]; o2[row] = acc.v[2]; o3[row] = acc.v[3];
+    }
+}
+
+void or_quotients_combine(const uint32_t *sp_x, const uint32_t *sp_y, const uint32_t *first_linear_acc,
+                          const uint32_t *const *numer0, const uint32_t *const *numer1,
+                          const uint32_t *const *numer2, const uint32_t *const *numer3,
+                          const uint32_t *acc_log_sizes, uint32_t n_accs, const uint32_t *domain_xs,
+                          const uint32_t *domain_ys, uint32_t lifting_log_size, uint64_t n_rows, uint32_t *o0,
+                          uint32_t *o1, uint32_t *o2, uint32_t *o3) {
+    for (uint64_t row = 0; row < n_rows; row++) {
+        uint32_t dx = domain_xs[row], dy = domain_ys[row];
+        qm31 quot = {{0, 0, 0, 0}};
+        for (uint32_t j = 0; j < n_accs; j++) {
+            cm31 prx = {sp_x[4 * j], sp_x[4 * j + 1]}, pix = {sp_x[4 * j + 2], sp_x[4 * j + 3]};
+            cm31 pry = {sp_y[4 * j], sp_y[4 * j + 1]}, piy = {sp_y[4 * j + 2], sp_y[4 * j + 3]};
+            cm31 denom = cm_sub(cm_mul(cm_sub(prx, (cm31){dx, 0}), piy), cm_mul(cm_sub(pry, (cm31){dy, 0}), pix));
+            cm31 den_inv = cm_inv(denom);
+            uint32_t lr = lifting_log_size - acc_log_sizes[j];
+            uint64_t li = ((row >> (lr + 1)) << 1) + (row & 1);
+            qm31 pn = {{numer0[j][li], numer1[j][li], numer2[j][li], numer3[j][li]}};
+            qm31 full = q_sub(pn, q_scale(q_load(first_linear_acc + 4 * j), dy));
+            cm31 ra = cm_mul(q_a(full), den_inv), rb = cm_mul(q_b(full), den_inv);
+            quot = q_add(quot, q_make(ra, rb));
+        }
+        o0[row] = quot.v[0]; o1[row] = quot.v[1]; o2[row] = quot.v[2]; o3[row] = quot.v[3];
+    }
+}

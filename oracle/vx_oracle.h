@@ -167,6 +167,17 @@ void or_eval_at_point_factors(const uint32_t *coeffs, const uint32_t *factors, u
 void or_barycentric_partials(const uint32_t *evals, const uint32_t *weights, uint64_t n, uint32_t n_blocks,
                              uint32_t *out);
 
+/* ---- stwo plugin QuotientOps: cuda/quotient.cu (pointer arrays on the host) ---- */
+void or_accumulate_numerators(const uint32_t *const *cols, const uint32_t *col_indices, const uint32_t *b,
+                              const uint32_t *c, uint32_t n_batch_cols, uint64_t n_rows, uint32_t *o0,
+                              uint32_t *o1, uint32_t *o2, uint32_t *o3);
+void or_quotients_combine(const uint32_t *sp_x, const uint32_t *sp_y, const uint32_t *first_linear_acc,
+                          const uint32_t *const *numer0, const uint32_t *const *numer1,
+                          const uint32_t *const *numer2, const uint32_t *const *numer3,
+                          const uint32_t *acc_log_sizes, uint32_t n_accs, const uint32_t *domain_xs,
+                          const uint32_t *domain_ys, uint32_t lifting_log_size, uint64_t n_rows, uint32_t *o0,
+                          uint32_t *o1, uint32_t *o2, uint32_t *o3);
+
 #ifdef __cplusplus
 }
 #endif

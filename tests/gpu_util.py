@@ -59,6 +59,22 @@ def ptr(b):
     return C.c_void_p(b.ptr)
 
 
+class Pin:
+    """Keeps device buffers alive for as long as the Pin is referenced.
+
+    `ptr(dev(x))` passed straight into a call frees the buffer before the
+    call runs (the temporary's last reference ends when ptr returns), so a
+    kernel would read freed memory. Tests pass `pin(dev(x))` instead and keep
+    `pin` until after the synchronisation that ends the kernels."""
+
+    def __init__(self):
+        self.bufs = []
+
+    def __call__(self, b):
+        self.bufs.append(b)
+        return ptr(b)
+
+
 def sync():
     call("vx_device_sync")
 

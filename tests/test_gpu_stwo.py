@@ -14,7 +14,7 @@ import pytest
 
 import pyoracle as po
 import vortexstark_amd as vx
-from gpu_util import DevBuf, dev, empty, host, ptr, sync
+from gpu_util import Pin, dev, empty, host, ptr, sync
 from vortexstark_amd import stwo
 from vortexstark_amd._lib import VortexError, call
 from vortexstark_amd.device import DeviceBuffer
@@ -82,8 +82,8 @@ def test_host_built_twiddles_give_the_same_values():
     log_n = 15
     fwd, inv = po.stwo_twiddles(log_n)
     x = rnd(1 << log_n, 7)
-    d = dev(x)
-    call("vx_stwo_ntt_evaluate", ptr(d), ptr(dev(fwd)), 1 << log_n, NULL)
+    d, tw = dev(x), dev(fwd)
+    call("vx_stwo_ntt_evaluate", ptr(d), ptr(tw), 1 << log_n, NULL)
     assert np.array_equal(host(d), po.stwo_evaluate(x, fwd, omp=True))
 
 
@@ -197,8 +197,8 @@ def eval_gpu(coeffs_dev, factors, n):
     words = 4 if n < (1 << 14) else max(80, n // 1024)  # the documented minimum scratch
     s1, s2 = empty(words), empty(words)
     res = np.zeros(4, np.uint32)
-    call("vx_eval_at_point", ptr(coeffs_dev), ptr(dev(factors)), C.c_void_p(res.ctypes.data), n, ptr(s1), ptr(s2),
-         NULL)
+    d_f = dev(factors)
+    call("vx_eval_at_point", ptr(coeffs_dev), ptr(d_f), C.c_void_p(res.ctypes.data), n, ptr(s1), ptr(s2), NULL)
     return tuple(int(v) for v in res)
 
 
@@ -251,6 +251,7 @@ def test_mirror_eval_at_point_and_barycentric():
     e, w = rnd(3000, 32), rnd(12000, 33)
     total = tuple(int(v) for v in po.barycentric_partials(e, w, 1))
     assert stwo.barycentric_eval(DeviceBuffer.from_host(e), DeviceBuffer.from_host(w), 3000) == total
+    pin, res = Pin(), np.zeros(4, np.uint32)
     with pytest.raises(VortexError):
-        call("vx_eval_at_point", ptr(dev(c)), ptr(dev(np.zeros(56, np.uint32))), C.c_void_p(np.zeros(4, np.uint32).ctypes.data),
-             n - 1, ptr(empty(64)), ptr(empty(64)), NULL)
+        call("vx_eval_at_point", pin(dev(c)), pin(dev(np.zeros(56, np.uint32))), C.c_void_p(res.ctypes.data),
+             n - 1, pin(empty(64)), pin(empty(64)), NULL)

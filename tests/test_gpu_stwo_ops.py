@@ -1,0 +1,127 @@
+"""The stwo plugin's QuotientOps on the GPU through the C-ABI, bit-exact
+against the oracle (cuda/quotient.cu restated in oracle/vx_oracle.c, pinned
+by tests/test_oracle_stwo.py)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import vortexstark_amd as vx
+from gpu_util import Pin, dev, empty, host, ptr, sync
+from vortexstark_amd._lib import VortexError, call
+
+pytestmark = pytest.mark.gpu
+P = po.P
+NULL = None
+NONE = 0xFFFFFFFF
+
+
+def rnd(n, seed):
+    return np.random.default_rng(seed).integers(0, P, int(n), dtype=np.uint32)
+
+
+def dev_ptrs(bufs):
+    """device array of the device addresses of `bufs`"""
+    return dev(np.array([b.ptr for b in bufs], np.uint64).view(np.uint32))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib_loaded():
+    vx.load()
+    print("stwo quotient-op tests: library loaded", flush=True)
+
+
+@pytest.mark.parametrize("n_rows,idx", [(1, [0]), (100, [2, 0, 2]), (4096, []), (4096, [5, 1, 3, 3, 0]),
+                                        (1 << 20, [0, 1, 2, 3, 4, 5, 0, 1, 2])])
+def test_accumulate_numerators(n_rows, idx):
+    print(f"accumulate_numerators rows={n_rows} cols={len(idx)}", flush=True)
+    cols = [rnd(n_rows, 10 + c) for c in range(6)]
+    bufs = [dev(c) for c in cols]
+    b, c = rnd(4 * len(idx), 1), rnd(4 * len(idx), 2)
+    outs = [empty(n_rows) for _ in range(4)]
+    pin = Pin()
+    call("vx_accumulate_numerators", pin(dev_ptrs(bufs)), pin(dev(np.array(idx or [0], np.uint32))),
+         pin(dev(b if len(idx) else np.zeros(4, np.uint32))), pin(dev(c if len(idx) else np.zeros(4, np.uint32))),
+         len(idx), n_rows, *[ptr(o) for o in outs], NULL)
+    sync()
+    want = po.accumulate_numerators(cols, idx, b, c, n_rows)
+    for k in range(4):
+        assert np.array_equal(host(outs[k]), want[k]), k
+
+
+@pytest.mark.parametrize("n_rows", [256, 5000, 1 << 18])
+def test_accumulate_numerators_dual(n_rows):
+    cols = [rnd(n_rows, 20 + c) for c in range(5)]
+    bufs = [dev(c) for c in cols]
+    idx_z = [4, 0, 1, 3, 2, 0]
+    zn_idx = [1, NONE, 0, NONE, 2, 3]  # z_next coefficients for 4 of the 6 columns
+    b_z, c_z = rnd(24, 3), rnd(24, 4)
+    b_zn, c_zn = rnd(16, 5), rnd(16, 6)
+    oz = [empty(n_rows) for _ in range(4)]
+    on = [empty(n_rows) for _ in range(4)]
+    pin = Pin()
+    call("vx_accumulate_numerators_dual", pin(dev_ptrs(bufs)), pin(dev(np.array(idx_z, np.uint32))),
+         pin(dev(np.array(zn_idx, np.uint32))), pin(dev(b_z)), pin(dev(c_z)), pin(dev(b_zn)), pin(dev(c_zn)),
+         len(idx_z), n_rows, *[ptr(o) for o in oz + on], NULL)
+    sync()
+    want_z = po.accumulate_numerators(cols, idx_z, b_z, c_z, n_rows)
+    sel = [i for i, z in enumerate(zn_idx) if z != NONE]
+    bn = np.concatenate([b_zn[4 * zn_idx[i]:4 * zn_idx[i] + 4] for i in sel])
+    cn = np.concatenate([c_zn[4 * zn_idx[i]:4 * zn_idx[i] + 4] for i in sel])
+    want_n = po.accumulate_numerators(cols, [idx_z[i] for i in sel], bn, cn, n_rows)
+    for k in range(4):
+        assert np.array_equal(host(oz[k]), want_z[k])
+        assert np.array_equal(host(on[k]), want_n[k])
+
+
+def run_combine(sx, sy, fla, numers, logs, xs, ys, L):
+    n_rows = len(xs)
+    nb = [[dev(col) for col in acc] for acc in numers]
+    outs = [empty(n_rows) for _ in range(4)]
+    pin = Pin()
+    call("vx_compute_quotients_combine", pin(dev(sx)), pin(dev(sy)), pin(dev(fla)),
+         *[pin(dev_ptrs([acc[k] for acc in nb])) for k in range(4)], pin(dev(np.array(logs, np.uint32))),
+         len(numers), pin(dev(xs)), pin(dev(ys)), L, n_rows, *[ptr(o) for o in outs], NULL)
+    sync()
+    return [host(o) for o in outs]
+
+
+@pytest.mark.parametrize("L,logs", [(1, [1]), (5, [5, 4]), (12, [12, 11, 9]), (20, [20, 19]), (21, [21, 20, 17])])
+def test_compute_quotients_combine(L, logs):
+    print(f"quotients_combine L={L} accs={len(logs)}", flush=True)
+    rng = np.random.default_rng(L)
+    n = 1 << L
+    sx, sy, fla = (rng.integers(0, P, 4 * len(logs), dtype=np.uint32) for _ in range(3))
+    numers = [[rng.integers(0, P, 1 << g, dtype=np.uint32) for _ in range(4)] for g in logs]
+    xs, ys = rnd(n, 7), rnd(n, 8)
+    got = run_combine(sx, sy, fla, numers, logs, xs, ys, L)
+    want = po.quotients_combine(sx, sy, fla, numers, logs, xs, ys, L)
+    for k in range(4):
+        assert np.array_equal(got[k], want[k]), k
+
+
+def test_combine_zero_denominators_and_partial_rows():
+    # Im p.x = 0, Im p.y = 1, Re p.x = (x of row 5, 0): the denominator
+    # Re p.x - d.x vanishes on the rows sharing that x, and only there
+    L, n = 11, 2000  # n_rows below 2^L: a partial last workgroup
+    xs, ys = rnd(n, 9), rnd(n, 10)
+    xs[[17, 1000, 1999]] = xs[5]
+    sx = np.array([xs[5], 0, 0, 0, 11, 12, 13, 14], np.uint32)
+    sy = np.array([3, 4, 1, 0, 15, 16, 17, 18], np.uint32)
+    fla = rnd(8, 11)
+    numers = [[rnd(1 << L, 20 + k) for k in range(4)], [rnd(1 << (L - 2), 30 + k) for k in range(4)]]
+    got = run_combine(sx, sy, fla, numers, [L, L - 2], xs, ys, L)
+    want = po.quotients_combine(sx, sy, fla, numers, [L, L - 2], xs, ys, L)
+    for k in range(4):
+        assert np.array_equal(got[k], want[k]), k
+
+
+def test_quotient_ops_reject_bad_arguments():
+    pin = Pin()
+    small, rows = empty(8), empty(64)
+    with pytest.raises(VortexError):  # more rows than the lifting domain
+        call("vx_compute_quotients_combine", *[pin(small)] * 8, 1, pin(rows), pin(rows), 5, 64, *[pin(rows)] * 4,
+             NULL)
+    with pytest.raises(VortexError):
+        call("vx_accumulate_numerators", NULL, NULL, NULL, NULL, 2, 16, *[pin(rows)] * 4, NULL)

@@ -182,3 +182,76 @@ def test_barycentric_partials():
             b = (i // 256) % nb
             assert p1[b].tolist() == w[4 * i:4 * i + 4].tolist()
             assert not np.any(np.delete(p1, b, axis=0))
+
+
+# ---- QuotientOps (cuda/quotient.cu)
+
+def test_accumulate_numerators_closed_forms():
+    rng = np.random.default_rng(41)
+    n = 64
+    cols = [rng.integers(0, P, n, dtype=np.uint32) for _ in range(3)]
+    b = rng.integers(0, P, 12, dtype=np.uint32)
+    zero = [np.zeros(n, np.uint32)] * 3
+    out = po.accumulate_numerators(zero, [0, 1, 2], b, rng.integers(0, P, 12, dtype=np.uint32), n)
+    for k in range(4):  # zero columns: -sum b
+        assert np.all(out[k] == (-sum(int(b[4 * i + k]) for i in range(3))) % P)
+    one = np.tile(np.array([1, 0, 0, 0], np.uint32), 3)
+    out = po.accumulate_numerators(cols, [2, 0, 2], np.zeros(12, np.uint32), one, n)
+    want = (2 * cols[2].astype(np.int64) + cols[0]) % P
+    assert np.array_equal(out[0], want) and not any(np.any(out[k]) for k in (1, 2, 3))
+    u = np.tile(np.array([0, 0, 1, 0], np.uint32), 2)  # c = u moves the sum to word 2
+    out = po.accumulate_numerators(cols, [1, 1], np.zeros(8, np.uint32), u, n)
+    assert np.array_equal(out[2], (2 * cols[1].astype(np.int64)) % P) and not np.any(out[0])
+
+
+def _cm_mul(x, y):
+    return pyref.cm_mul(x, y)
+
+
+def test_quotients_combine_inverts_the_denominator():
+    rng = np.random.default_rng(42)
+    L, n = 6, 64
+    for lift_lr in (0, 1, 3):  # accumulator smaller than the lifting domain by 2^lr
+        acc_log = L - lift_lr
+        sx = [int(v) for v in rng.integers(0, P, 4)]
+        sy = [int(v) for v in rng.integers(0, P, 4)]
+        fla = [int(v) for v in rng.integers(0, P, 4)]
+        numer = [rng.integers(0, P, 1 << acc_log, dtype=np.uint32) for _ in range(4)]
+        dxs = rng.integers(0, P, n, dtype=np.uint32)
+        dys = rng.integers(0, P, n, dtype=np.uint32)
+        out = po.quotients_combine(sx, sy, fla, [numer], [acc_log], dxs, dys, L)
+        for row in range(n):
+            dx, dy = int(dxs[row]), int(dys[row])
+            li = ((row >> (lift_lr + 1)) << 1) + (row & 1)
+            full = [(int(numer[k][li]) - fla[k] * dy) % P for k in range(4)]
+            # the line denominator through the sample point and its conjugate
+            den = pyref.qm_sub(_cm_mul(((sx[0] - dx) % P, sx[1]), (sy[2], sy[3])) + (0, 0),
+                               _cm_mul(((sy[0] - dy) % P, sy[1]), (sx[2], sx[3])) + (0, 0))[:2]
+            q = [int(out[k][row]) for k in range(4)]
+            assert _cm_mul((q[0], q[1]), den) == (full[0], full[1]), (lift_lr, row)
+            assert _cm_mul((q[2], q[3]), den) == (full[2], full[3]), (lift_lr, row)
+
+
+def test_quotients_combine_special_point_and_sum_over_samples():
+    rng = np.random.default_rng(43)
+    L, n = 5, 32
+    dxs = rng.integers(0, P, n, dtype=np.uint32)
+    dys = rng.integers(0, P, n, dtype=np.uint32)
+    # Im(p.x) = 0 and Im(p.y) = 1: the denominator is Re(p.x) - d.x
+    sx = [int(v) for v in rng.integers(0, P, 2)] + [0, 0]
+    sy = [int(v) for v in rng.integers(0, P, 2)] + [1, 0]
+    numer = [rng.integers(0, P, n, dtype=np.uint32) for _ in range(4)]
+    out = po.quotients_combine(sx, sy, [0, 0, 0, 0], [numer], [L], dxs, dys, L)
+    for row in range(n):
+        den = ((sx[0] - int(dxs[row])) % P, sx[1])
+        for h in (0, 2):
+            assert _cm_mul((int(out[h][row]), int(out[h + 1][row])), den) == (int(numer[h][row]), int(numer[h + 1][row]))
+    # two accumulators give the sum of the two single-accumulator quotients
+    sx2 = [int(v) for v in rng.integers(0, P, 4)]
+    sy2 = [int(v) for v in rng.integers(0, P, 4)]
+    numer2 = [rng.integers(0, P, n // 2, dtype=np.uint32) for _ in range(4)]
+    a = po.quotients_combine(sx, sy, [1, 2, 3, 4], [numer], [L], dxs, dys, L)
+    b = po.quotients_combine(sx2, sy2, [5, 6, 7, 8], [numer2], [L - 1], dxs, dys, L)
+    ab = po.quotients_combine(sx + sx2, sy + sy2, [1, 2, 3, 4, 5, 6, 7, 8], [numer, numer2], [L, L - 1], dxs, dys, L)
+    for k in range(4):
+        assert np.array_equal(ab[k], (a[k].astype(np.int64) + b[k]) % P)

@@ -270,6 +270,57 @@ int vx_barycentric_eval(const uint32_t* d_evals, const uint32_t* d_weights, uint
   });
 }
 
+int vx_accumulate_numerators(const uint32_t* const* d_col_ptrs, const uint32_t* d_col_indices, const uint32_t* d_b,
+                             const uint32_t* d_c, uint32_t n_batch_cols, uint32_t n_rows, uint32_t* o0, uint32_t* o1,
+                             uint32_t* o2, uint32_t* o3, vx_stream_t st) {
+  return guard([&] {
+    if (n_rows == 0) return;
+    require(o0 && o1 && o2 && o3, "accumulate_numerators: null output");
+    require(n_batch_cols == 0 || (d_col_ptrs && d_col_indices && d_b && d_c), "accumulate_numerators: null input");
+    uint32_t* const oz[4] = {o0, o1, o2, o3};
+    accumulate_numerators(d_col_ptrs, d_col_indices, nullptr, d_b, d_c, nullptr, nullptr, n_batch_cols, n_rows,
+                          false, oz, oz, S(st));
+  });
+}
+
+int vx_accumulate_numerators_dual(const uint32_t* const* d_col_ptrs, const uint32_t* d_col_indices_z,
+                                  const uint32_t* d_zn_coeff_idx, const uint32_t* d_b_z, const uint32_t* d_c_z,
+                                  const uint32_t* d_b_zn, const uint32_t* d_c_zn, uint32_t n_cols_z, uint32_t n_rows,
+                                  uint32_t* z0, uint32_t* z1, uint32_t* z2, uint32_t* z3, uint32_t* n0, uint32_t* n1,
+                                  uint32_t* n2, uint32_t* n3, vx_stream_t st) {
+  return guard([&] {
+    if (n_rows == 0) return;
+    require(z0 && z1 && z2 && z3 && n0 && n1 && n2 && n3, "accumulate_numerators_dual: null output");
+    require(n_cols_z == 0 || (d_col_ptrs && d_col_indices_z && d_zn_coeff_idx && d_b_z && d_c_z && d_b_zn && d_c_zn),
+            "accumulate_numerators_dual: null input");
+    uint32_t* const oz[4] = {z0, z1, z2, z3};
+    uint32_t* const on[4] = {n0, n1, n2, n3};
+    accumulate_numerators(d_col_ptrs, d_col_indices_z, d_zn_coeff_idx, d_b_z, d_c_z, d_b_zn, d_c_zn, n_cols_z,
+                          n_rows, true, oz, on, S(st));
+  });
+}
+
+int vx_compute_quotients_combine(const uint32_t* d_sp_x, const uint32_t* d_sp_y, const uint32_t* d_fla,
+                                 const uint32_t* const* d_numer0, const uint32_t* const* d_numer1,
+                                 const uint32_t* const* d_numer2, const uint32_t* const* d_numer3,
+                                 const uint32_t* d_acc_log_sizes, uint32_t n_accs, const uint32_t* d_xs,
+                                 const uint32_t* d_ys, uint32_t lifting_log_size, uint32_t n_rows, uint32_t* o0,
+                                 uint32_t* o1, uint32_t* o2, uint32_t* o3, vx_stream_t st) {
+  return guard([&] {
+    if (n_rows == 0) return;
+    require(o0 && o1 && o2 && o3 && d_xs && d_ys, "compute_quotients_combine: null argument");
+    require(n_accs == 0 || (d_sp_x && d_sp_y && d_fla && d_numer0 && d_numer1 && d_numer2 && d_numer3 &&
+                            d_acc_log_sizes),
+            "compute_quotients_combine: null argument");
+    require(lifting_log_size <= 31 && (uint64_t)n_rows <= (1ull << lifting_log_size),
+            "compute_quotients_combine: n_rows must not exceed 2^lifting_log_size");
+    const uint32_t* const* nu[4] = {d_numer0, d_numer1, d_numer2, d_numer3};
+    uint32_t* const out[4] = {o0, o1, o2, o3};
+    quotients_combine(d_sp_x, d_sp_y, d_fla, nu, d_acc_log_sizes, n_accs, d_xs, d_ys, lifting_log_size, n_rows, out,
+                      S(st));
+  });
+}
+
 int vx_stwo_ntt_batch_evaluate(uint32_t* const* cols, const uint32_t* tw, uint32_t n, uint32_t n_cols,
                                vx_stream_t st) {
   return guard([&] {

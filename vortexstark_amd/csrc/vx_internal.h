@@ -154,6 +154,19 @@ const uint32_t* eval_at_point(const uint32_t* coeffs, const uint32_t* factors, u
 void barycentric_partials(const uint32_t* evals, const uint32_t* weights, uint32_t n, uint32_t* out,
                           uint32_t n_blocks, hipStream_t s);
 
+// ----------------------------------- stwo plugin QuotientOps (quotient.hip)
+// Device pointer arrays, as the reference takes them. dual selects the fused
+// z / z_next form (zn_idx, b_zn, c_zn used); otherwise only oz is written.
+void accumulate_numerators(const uint32_t* const* cols, const uint32_t* idx_z, const uint32_t* zn_idx,
+                           const uint32_t* b_z, const uint32_t* c_z, const uint32_t* b_zn, const uint32_t* c_zn,
+                           uint32_t n_cols, uint32_t n_rows, bool dual, uint32_t* const oz[4],
+                           uint32_t* const on[4],
+                           hipStream_t s);
+void quotients_combine(const uint32_t* sp_x, const uint32_t* sp_y, const uint32_t* fla,
+                       const uint32_t* const* const numer[4], const uint32_t* acc_log_sizes, uint32_t n_accs,
+                       const uint32_t* xs, const uint32_t* ys, uint32_t lifting_log_size, uint32_t n_rows,
+                       uint32_t* const out[4], hipStream_t s);
+
 // ----------------------------------------------- twiddles (twiddles.hip)
 void coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n, hipStream_t s);
 void extract_and_squash(const uint32_t* in, uint32_t* tw_out, uint32_t* sq_out, uint64_t half_n,
