@@ -298,6 +298,13 @@ int vx_grind_pow(const uint32_t *d_prefixed_digest, uint64_t *d_result, uint32_t
  * trailing_zeros(Blake2s(prefix || nonce)[0..16]) >= pow_bits
  * (Channel::verify_pow_nonce, src/channel.rs:140-168). pow_bits <= 32. */
 int vx_grind_pow_min(const uint32_t *d_prefixed_digest, uint32_t pow_bits, uint64_t *d_result, vx_stream_t stream);
+/* The stwo plugin's grind as one call (GrindOps for Blake2sChannelGeneric<false>,
+ * crates/cuda-backend/src/grind_ops.rs:21-135, which loops on the host over
+ * cuda_grind_pow batches of 2^20): *d_result = the smallest valid nonce of
+ * the form (hi << 32) | low with low < 2^20, hi = 0, 1, ... (the order the
+ * plugin and stwo's SIMD backend search); 0 when pow_bits = 0. pow_bits <= 32.
+ * Async, except for pow_bits = 0. */
+int vx_grind_pow_stwo(const uint32_t *d_prefixed_digest, uint32_t pow_bits, uint64_t *d_result, vx_stream_t stream);
 
 /* --------------------------------------------- prove / verify (src/prover.rs) */
 

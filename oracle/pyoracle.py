@@ -76,6 +76,7 @@ def lib(omp=False):
         "or_channel_mix_u64": (None, [C.POINTER(Channel), u64]),
         "or_channel_verify_pow_nonce": (C.c_int, [C.POINTER(Channel), u32, u64]),
         "or_grind_pow": (u64, [C.POINTER(Channel), u32]),
+        "or_grind_pow_stwo": (u64, [C.POINTER(Channel), u32]),
         "or_prove": (C.c_int, [u32, u32, u32, u32, u32, u32,
                                C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz)]),
         "or_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
@@ -325,6 +326,17 @@ class PyChannel:
 
     def grind(self, bits):
         return int(lib().or_grind_pow(C.byref(self.c), bits))
+
+    def grind_stwo(self, bits):
+        return int(lib().or_grind_pow_stwo(C.byref(self.c), bits))
+
+    def pow_prefix(self, bits):
+        """Blake2s(0x12345678 LE || 0^12 || state || bits LE) (src/channel.rs:140-150) as 8 words."""
+        msg = np.zeros(13, np.uint32)
+        msg[0] = 0x12345678
+        msg[4:12] = np.frombuffer(self.state, dtype=np.uint32)
+        msg[12] = bits
+        return np.frombuffer(blake2s(msg.tobytes()), dtype=np.uint32).copy()
 
 
 # ---- synthetic degree-2 multi-column AIR (KRAK v3) ----

@@ -624,6 +624,17 @@ uint64_t or_grind_pow(const or_channel *ch, uint32_t pow_bits) {
         if (pow_tz(prefix, n) >= pow_bits) return n;
 }
 
+/* The stwo plugin's grind (crates/cuda-backend/src/grind_ops.rs:71-128):
+ * nonces (hi << 32) | low, low < 2^20, for hi = 0, 1, ...; the smallest
+ * valid low of the first hi that has one. */
+uint64_t or_grind_pow_stwo(const or_channel *ch, uint32_t pow_bits) {
+    uint8_t prefix[32];
+    pow_prefix(ch, pow_bits, prefix);
+    for (uint64_t hi = 0;; hi++)
+        for (uint64_t low = 0; low < (1u << 20); low++)
+            if (pow_tz(prefix, (hi << 32) | low) >= pow_bits) return (hi << 32) | low;
+}
+
 /* ======================================================================
  * Proof bytes: KRAK v1 (src/bin/stark_cli.rs:1051-1110). pow_bits > 0 selects
  * the KRAK v2 extension: header gains blowup_bits, pow_bits, nonce(lo,hi);

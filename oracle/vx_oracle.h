@@ -112,6 +112,8 @@ uint64_t or_channel_draw_number(or_channel *ch, uint64_t bound);
 void or_channel_mix_u64(or_channel *ch, uint64_t value);
 int or_channel_verify_pow_nonce(const or_channel *ch, uint32_t pow_bits, uint64_t nonce);
 uint64_t or_grind_pow(const or_channel *ch, uint32_t pow_bits);
+/* the stwo plugin's nonce family (hi << 32) | low, low < 2^20 (grind_ops.rs:71-128) */
+uint64_t or_grind_pow_stwo(const or_channel *ch, uint32_t pow_bits);
 
 /* ---- prove / verify (src/prover.rs:264-619, src/verifier.rs:39-310,
  *      src/bin/stark_cli.rs:1051-1110) ---- */

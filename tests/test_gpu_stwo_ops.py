@@ -125,3 +125,37 @@ def test_quotient_ops_reject_bad_arguments():
              NULL)
     with pytest.raises(VortexError):
         call("vx_accumulate_numerators", NULL, NULL, NULL, NULL, 2, 16, *[pin(rows)] * 4, NULL)
+
+
+# ---- GrindOps (grind_ops.rs:21-135): the (hi << 32) | low nonce family
+
+@pytest.mark.parametrize("seed,bits", [(0, 1), (0, 8), (3, 16), (0, 21), (1, 21), (2, 21)])
+def test_grind_pow_stwo(seed, bits):
+    print(f"grind stwo seed={seed} bits={bits}", flush=True)
+    ch = po.PyChannel()
+    ch.mix_digest([seed, 1, 2, 3, 4, 5, 6, 7])
+    dp, res = dev(ch.pow_prefix(bits)), empty(2)
+    call("vx_grind_pow_stwo", ptr(dp), bits, ptr(res), NULL)
+    sync()
+    got = host(res)
+    nonce = int(got[0]) | (int(got[1]) << 32)
+    assert nonce == ch.grind_stwo(bits)
+    assert ch.verify_pow_nonce(bits, nonce)
+    # the plugin's own loop: reference-style batches of 2^20 at (hi << 32)
+    for hi in range(nonce >> 32):
+        r = dev(np.array([0xFFFFFFFF, 0xFFFFFFFF], np.uint32))
+        call("vx_grind_pow", ptr(dp), ptr(r), bits, hi << 32, 1 << 20, NULL)
+        sync()
+        assert host(r).tolist() == [0xFFFFFFFF, 0xFFFFFFFF]
+    r = dev(np.array([0xFFFFFFFF, 0xFFFFFFFF], np.uint32))
+    call("vx_grind_pow", ptr(dp), ptr(r), bits, (nonce >> 32) << 32, 1 << 20, NULL)
+    sync()
+    g = host(r)
+    assert int(g[0]) | (int(g[1]) << 32) == nonce
+
+
+def test_grind_pow_stwo_zero_bits():
+    res = dev(np.array([7, 7], np.uint32))
+    dp = dev(np.zeros(8, np.uint32))
+    call("vx_grind_pow_stwo", ptr(dp), 0, ptr(res), NULL)
+    assert host(res).tolist() == [0, 0]

@@ -255,3 +255,28 @@ def test_quotients_combine_special_point_and_sum_over_samples():
     ab = po.quotients_combine(sx + sx2, sy + sy2, [1, 2, 3, 4, 5, 6, 7, 8], [numer, numer2], [L, L - 1], dxs, dys, L)
     for k in range(4):
         assert np.array_equal(ab[k], (a[k].astype(np.int64) + b[k]) % P)
+
+
+# ---- GrindOps nonce family (grind_ops.rs:71-128)
+
+def _pow_ok(prefix_words, bits, nonce):
+    import hashlib
+    h = hashlib.blake2s(prefix_words.tobytes() + nonce.to_bytes(8, "little")).digest()
+    return int.from_bytes(h[:16], "little") % (1 << bits) == 0
+
+
+def test_grind_stwo_family():
+    # small bits: the answer lies in hi = 0, where both searches see 0, 1, 2, ...
+    for bits in (4, 9, 14):
+        ch = po.PyChannel()
+        ch.mix_digest([bits, 9, 8, 7, 6, 5, 4, 3])
+        assert ch.grind_stwo(bits) == ch.grind(bits)
+    # 21 bits on this state: no low < 2^20 passes for hi = 0, so the answer
+    # is (1 << 32) | low; checked with hashlib over every earlier nonce
+    ch = po.PyChannel()
+    ch.mix_digest([1, 1, 2, 3, 4, 5, 6, 7])
+    n = ch.grind_stwo(21)
+    pre = ch.pow_prefix(21)
+    assert n >> 32 == 1 and _pow_ok(pre, 21, n)
+    assert not any(_pow_ok(pre, 21, low) for low in range(1 << 20))
+    assert not any(_pow_ok(pre, 21, (1 << 32) | low) for low in range(n & 0xFFFFFFFF))

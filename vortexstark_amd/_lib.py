@@ -87,6 +87,7 @@ SIGNATURES = {
     "vx_stwo_twiddles": (ip, [u32, vp, vp, vp]),
     "vx_eval_at_point": (ip, [vp, vp, vp, u32, vp, vp, vp]),
     "vx_barycentric_eval": (ip, [vp, vp, u32, vp, u32, vp]),
+    "vx_grind_pow_stwo": (ip, [vp, u32, vp, vp]),
     "vx_accumulate_numerators": (ip, [vp, vp, vp, vp, u32, u32, vp, vp, vp, vp, vp]),
     "vx_accumulate_numerators_dual": (ip, [vp, vp, vp, vp, vp, vp, vp, u32, u32] + [vp] * 8 + [vp]),
     "vx_compute_quotients_combine": (ip, [vp] * 8 + [u32, vp, vp, u32, u32, vp, vp, vp, vp, vp]),

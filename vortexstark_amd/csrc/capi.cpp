@@ -554,6 +554,19 @@ int vx_grind_pow(const uint32_t* prefix, uint64_t* result, uint32_t pow_bits, ui
   });
 }
 
+int vx_grind_pow_stwo(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
+  return guard([&] {
+    require(prefix && result, "grind_pow_stwo: null pointer");
+    if (pow_bits == 0) {
+      const uint64_t zero = 0;
+      VX_HIP(hipMemcpyAsync(result, &zero, 8, hipMemcpyHostToDevice, S(st)));
+      VX_HIP(hipStreamSynchronize(S(st)));
+      return;
+    }
+    grind_pow_stwo(prefix, pow_bits, result, S(st));
+  });
+}
+
 int vx_grind_pow_min(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
   return guard([&] {
     require(prefix && result, "grind_pow_min: null pointer");

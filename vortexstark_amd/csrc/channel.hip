@@ -100,6 +100,12 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// The stwo plugin's nonce family (crates/cuda-backend/src/grind_ops.rs:
+// 71-128): index i stands for nonce ((i >> 20) << 32) | (i & (2^20 - 1)), a
+// map that keeps the order, so the smallest index is the smallest nonce.
+__device__ __forceinline__ uint64_t stwo_nonce(uint64_t i) { return ((i >> 20) << 32) | (i & 0xFFFFFull); }
+
+template <bool STWO>
 __global__ __launch_bounds__(256) void k_grind(const uint32_t* prefix_dev, uint32_t bits,
                                               unsigned long long* best, uint64_t start, uint64_t count) {
   uint32_t prefix[8];
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(256) void k_grind(const uint32_t* prefix_dev, uint3
     if (base > cur_best) break;
     const uint64_t n = base + threadIdx.x;
     uint64_t hit = ~0ull;
-    if (n < end && pow_trailing_zeros(prefix, n) >= bits) hit = n;
+    if (n < end && pow_trailing_zeros(prefix, STWO ? stwo_nonce(n) : n) >= bits) hit = n;
     hit = wave_min_u64(hit);
     if ((threadIdx.x & 63) == 0 && hit != ~0ull) atomicMin(best, (unsigned long long)hit);
   }
@@ -124,13 +130,28 @@ __global__ __launch_bounds__(256) void k_grind(const uint32_t* prefix_dev, uint3
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
+__global__ void k_stwo_index_to_nonce(unsigned long long* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && *p != ~0ull) *p = stwo_nonce(*p);
+}
+
+// Whole search over the stwo family: the same increasing-index scan, then
+// the winning index is turned into its nonce. Indices up to 2^44 cover
+// hi < 2^24.
+void grind_pow_stwo(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, hipStream_t s) {
+  require(pow_bits >= 1 && pow_bits <= 32, "grind_pow_stwo: pow_bits must be 1..32");
+  auto* best = reinterpret_cast<unsigned long long*>(d_result);
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, best, ~0ull);
+  hipLaunchKernelGGL(k_grind<true>, dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull, GRIND_LIMIT);
+  hipLaunchKernelGGL(k_stwo_index_to_nonce, dim3(1), dim3(64), 0, s, best);
+  VX_HIP(hipGetLastError());
+}
 
 void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, hipStream_t s) {
   require(pow_bits >= 1 && pow_bits <= 32, "grind_pow: pow_bits must be 1..32");
   auto* best = reinterpret_cast<unsigned long long*>(d_result);
   hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, best, ~0ull);
   // 256 CUs x 8 workgroups: every lane loops until its nonce exceeds the best hit.
-  hipLaunchKernelGGL(k_grind, dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull, GRIND_LIMIT);
+  hipLaunchKernelGGL(k_grind<false>, dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull, GRIND_LIMIT);
   VX_HIP(hipGetLastError());
 }
 
@@ -141,7 +162,7 @@ void grind_pow_batch(const uint32_t* d_prefix, uint64_t* d_result, uint32_t pow_
   require(pow_bits <= 128 && n > 0, "grind_pow_batch: bad arguments");
   uint64_t blocks = (n + 255) / 256;
   uint32_t grid = (uint32_t)(blocks < 4096 ? blocks : 4096);
-  hipLaunchKernelGGL(k_grind, dim3(grid), dim3(256), 0, s, d_prefix, pow_bits,
+  hipLaunchKernelGGL(k_grind<false>, dim3(grid), dim3(256), 0, s, d_prefix, pow_bits,
                      reinterpret_cast<unsigned long long*>(d_result), batch_offset, n);
   VX_HIP(hipGetLastError());
 }
