@@ -206,6 +206,38 @@ int vx_compute_quotients_combine(const uint32_t *d_sample_points_x, const uint32
                                  uint32_t lifting_log_size, uint32_t n_rows, uint32_t *out0, uint32_t *out1,
                                  uint32_t *out2, uint32_t *out3, vx_stream_t stream);
 
+/* ------------------------------------- stwo plugin column ops (columns)
+ * Replace cuda_m31_add / cuda_m31_mul (src/cuda/ffi.rs:212-213; the plugin's
+ * AccumulationOps::accumulate, accumulation_ops.rs:11-31): out = a + b (or
+ * a * b) element-wise over n M31 words. Async. */
+int vx_m31_add(const uint32_t *d_a, const uint32_t *d_b, uint32_t *d_out, uint32_t n, vx_stream_t stream);
+int vx_m31_mul(const uint32_t *d_a, const uint32_t *d_b, uint32_t *d_out, uint32_t n, vx_stream_t stream);
+/* Replaces cuda_accumulate_lift (ffi.rs:1186, cuda/accumulate_lift.cu:14-40):
+ * col[i] += curr[((i >> (log_ratio + 1)) << 1) | (i & 1)] for i < col_n. Async. */
+int vx_accumulate_lift(uint32_t *d_col, const uint32_t *d_curr, uint32_t col_n, uint32_t log_ratio,
+                       vx_stream_t stream);
+/* No counterpart as one call in the reference: the plugin's
+ * AccumulationOps::lift_and_accumulate (accumulation_ops.rs:44-85), which
+ * runs accumulate_lift per column and channel with a synchronisation after
+ * each column. Columns in ascending log size (h_log_sizes, 1..31, HOST);
+ * h_cols is a HOST array of 4 x n_cols device pointers, channel-major
+ * (h_cols[c * n_cols + k] = channel c of column k). The largest (last)
+ * column receives the lifted sum of all columns in one pass; the others
+ * are only read. Synchronous. */
+int vx_lift_and_accumulate(uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
+                           vx_stream_t stream);
+/* Replace cuda_gather_u32 / cuda_gather_u256 (ffi.rs:980-984, cuda/gather.cu)
+ * and their bounds-checked forms cuda_gather_u32_forge / _u256_forge
+ * (ffi.rs:986-1002, cuda/forge/gather_forge.cu:82-116): dst[i] = src[idx[i]]
+ * with 1 or 8 words per element; the checked forms skip elements whose
+ * source words are not all below src_len (words). Async. */
+int vx_gather_u32(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst, uint32_t n, vx_stream_t stream);
+int vx_gather_u32_checked(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst, uint32_t n,
+                          uint32_t src_len, vx_stream_t stream);
+int vx_gather_u256(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst, uint32_t n, vx_stream_t stream);
+int vx_gather_u256_checked(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst, uint32_t n,
+                           uint32_t src_len, vx_stream_t stream);
+
 /* ------------------------------------------------------------- twiddles */
 
 /* Replaces cuda_compute_coset_points (ffi.rs:453-458, cuda/twiddles.cu:80-105,182-194). */

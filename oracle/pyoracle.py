@@ -77,6 +77,8 @@ def lib(omp=False):
         "or_channel_verify_pow_nonce": (C.c_int, [C.POINTER(Channel), u32, u64]),
         "or_grind_pow": (u64, [C.POINTER(Channel), u32]),
         "or_grind_pow_stwo": (u64, [C.POINTER(Channel), u32]),
+        "or_accumulate_lift": (None, [_u32p, _u32p, u64, u32]),
+        "or_lift_and_accumulate": (None, [C.c_void_p, _u32p, u32]),
         "or_prove": (C.c_int, [u32, u32, u32, u32, u32, u32,
                                C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz)]),
         "or_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
@@ -553,3 +555,18 @@ def quotients_combine(sp_x, sp_y, fla, numers, acc_log_sizes, domain_xs, domain_
                                u32(acc_log_sizes), len(ns), u32(domain_xs), u32(domain_ys), lifting_log_size,
                                n_rows, *out)
     return out
+
+
+def accumulate_lift(col, curr, log_ratio):
+    out = u32(col).copy()
+    lib().or_accumulate_lift(out, u32(curr), out.size, log_ratio)
+    return out
+
+
+def lift_and_accumulate(cols):
+    """One channel of the plugin's lift_and_accumulate; returns every column
+    after the loop (the last is the result)."""
+    cs = [u32(c).copy() for c in cols]
+    logs = np.array([c.size.bit_length() - 1 for c in cs], np.uint32)
+    lib().or_lift_and_accumulate(_ptrs(cs), logs, len(cs))
+    return cs

@@ -2003,3 +2003,22 @@ void or_quotients_combine(const uint32_t *sp_x, const uint32_t *sp_y, const uint
         o0[row] = quot.v[0]; o1[row] = quot.v[1]; o2[row] = quot.v[2]; o3[row] = quot.v[3];
     }
 }
+
+/* ---- stwo plugin column ops: cuda/accumulate_lift.cu:14-40 and the
+ *      plugin's lift_and_accumulate loop (accumulation_ops.rs:44-85) ---- */
+void or_accumulate_lift(uint32_t *col, const uint32_t *curr, uint64_t col_n, uint32_t log_ratio) {
+    for (uint64_t i = 0; i < col_n; i++)
+        col[i] = m31_add(col[i], curr[((i >> (log_ratio + 1)) << 1) | (i & 1)]);
+}
+/* one channel: curr = zeros(2); for each column (ascending):
+ * col += lift(curr); curr = col. Columns are updated in place. */
+void or_lift_and_accumulate(uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols) {
+    uint32_t zeros[2] = {0, 0};
+    const uint32_t *curr = zeros;
+    uint32_t curr_log = 1;
+    for (uint32_t k = 0; k < n_cols; k++) {
+        or_accumulate_lift(cols[k], curr, 1ull << log_sizes[k], log_sizes[k] - curr_log);
+        curr = cols[k];
+        curr_log = log_sizes[k];
+    }
+}

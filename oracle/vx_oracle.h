@@ -112,6 +112,9 @@ uint64_t or_channel_draw_number(or_channel *ch, uint64_t bound);
 void or_channel_mix_u64(or_channel *ch, uint64_t value);
 int or_channel_verify_pow_nonce(const or_channel *ch, uint32_t pow_bits, uint64_t nonce);
 uint64_t or_grind_pow(const or_channel *ch, uint32_t pow_bits);
+/* stwo plugin column ops: cuda/accumulate_lift.cu:14-40, accumulation_ops.rs:44-85 (one channel) */
+void or_accumulate_lift(uint32_t *col, const uint32_t *curr, uint64_t col_n, uint32_t log_ratio);
+void or_lift_and_accumulate(uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols);
 /* the stwo plugin's nonce family (hi << 32) | low, low < 2^20 (grind_ops.rs:71-128) */
 uint64_t or_grind_pow_stwo(const or_channel *ch, uint32_t pow_bits);
 

@@ -159,3 +159,73 @@ def test_grind_pow_stwo_zero_bits():
     dp = dev(np.zeros(8, np.uint32))
     call("vx_grind_pow_stwo", ptr(dp), 0, ptr(res), NULL)
     assert host(res).tolist() == [0, 0]
+
+
+# ---- column ops (column.rs, accumulation_ops.rs)
+
+@pytest.mark.parametrize("n", [1, 3, 4, 1000, 1 << 20])
+def test_m31_add_mul(n):
+    a, b = rnd(n, 70), rnd(n, 71)
+    a[:1] = P - 1
+    b[:1] = P - 1
+    da, db, do = dev(a), dev(b), empty(n)
+    call("vx_m31_add", ptr(da), ptr(db), ptr(do), n, NULL)
+    sync()
+    assert np.array_equal(host(do), (a.astype(np.uint64) + b) % P)
+    call("vx_m31_mul", ptr(da), ptr(db), ptr(do), n, NULL)
+    sync()
+    assert np.array_equal(host(do), (a.astype(np.uint64) * b) % P)
+    if n >= 8:  # unaligned operands take the word path
+        d1 = dev(np.concatenate([[0], a]).astype(np.uint32))
+        call("vx_m31_add", C.c_void_p(d1.ptr + 4), ptr(db), ptr(do), n, NULL)
+        sync()
+        assert np.array_equal(host(do), (a.astype(np.uint64) + b) % P)
+
+
+@pytest.mark.parametrize("col_log,lr", [(1, 0), (5, 2), (20, 3), (21, 0)])
+def test_accumulate_lift(col_log, lr):
+    col, curr = rnd(1 << col_log, 72), rnd(1 << (col_log - lr), 73)
+    dc, du = dev(col), dev(curr)
+    call("vx_accumulate_lift", ptr(dc), ptr(du), 1 << col_log, lr, NULL)
+    sync()
+    assert np.array_equal(host(dc), po.accumulate_lift(col, curr, lr))
+
+
+@pytest.mark.parametrize("logs", [[4], [1, 3, 3, 5, 8], [10, 12, 16, 20], [14] * 5 + [22]])
+def test_lift_and_accumulate(logs):
+    print(f"lift_and_accumulate logs={logs}", flush=True)
+    chans = [[rnd(1 << L, 100 * c + k) for k, L in enumerate(logs)] for c in range(4)]
+    bufs = [[dev(x) for x in ch] for ch in chans]
+    arr = (C.c_void_p * (4 * len(logs)))(*[b.ptr for ch in bufs for b in ch])
+    call("vx_lift_and_accumulate", arr, (C.c_uint32 * len(logs))(*logs), len(logs), NULL)
+    for c in range(4):
+        want = po.lift_and_accumulate(chans[c])
+        assert np.array_equal(host(bufs[c][-1]), want[-1]), c
+        for k in range(len(logs) - 1):  # smaller columns are only read
+            assert np.array_equal(host(bufs[c][k]), chans[c][k])
+    with pytest.raises(VortexError):  # sizes must ascend
+        call("vx_lift_and_accumulate", arr, (C.c_uint32 * 2)(5, 4), 2, NULL)
+
+
+@pytest.mark.parametrize("words", [1, 8])
+def test_gather(words):
+    n_src, n = 5000, 3000
+    src = rnd(n_src * words, 74)
+    idx = np.random.default_rng(75).integers(0, n_src, n, dtype=np.uint32)
+    fn = "vx_gather_u32" if words == 1 else "vx_gather_u256"
+    ds, di, dd = dev(src), dev(idx), empty(n * words)
+    call(fn, ptr(ds), ptr(di), ptr(dd), n, NULL)
+    sync()
+    assert np.array_equal(host(dd).reshape(n, words), src.reshape(n_src, words)[idx])
+    # checked form: indices at or beyond src_len (in words) leave dst as it was
+    idx2 = idx.copy()
+    idx2[::7] = n_src + 5
+    idx2[1] = n_src - 1
+    before = rnd(n * words, 76)
+    di2, dd2 = dev(idx2), dev(before)
+    call(fn + "_checked", ptr(ds), ptr(di2), ptr(dd2), n, n_src * words, NULL)
+    sync()
+    got = host(dd2).reshape(n, words)
+    ok = idx2 < n_src
+    assert np.array_equal(got[ok], src.reshape(n_src, words)[idx2[ok]])
+    assert np.array_equal(got[~ok], before.reshape(n, words)[~ok])

@@ -280,3 +280,31 @@ def test_grind_stwo_family():
     assert n >> 32 == 1 and _pow_ok(pre, 21, n)
     assert not any(_pow_ok(pre, 21, low) for low in range(1 << 20))
     assert not any(_pow_ok(pre, 21, (1 << 32) | low) for low in range(n & 0xFFFFFFFF))
+
+
+# ---- column ops: lifted accumulation (accumulate_lift.cu, accumulation_ops.rs)
+
+def test_accumulate_lift_closed_form():
+    rng = np.random.default_rng(61)
+    curr = rng.integers(0, P, 8, dtype=np.uint32)
+    col = rng.integers(0, P, 32, dtype=np.uint32)
+    out = po.accumulate_lift(col, curr, 2)
+    # i -> ((i >> 3) << 1) | (i & 1): rows 0..7 read curr[0], curr[1], curr[0], ...
+    src = np.array([((i >> 3) << 1) | (i & 1) for i in range(32)])
+    assert np.array_equal(out, (col.astype(np.int64) + curr[src]) % P)
+    assert np.array_equal(po.accumulate_lift(col, col, 0), (2 * col.astype(np.int64)) % P)
+
+
+def test_lift_and_accumulate_composes():
+    # the loop's last column equals last + the sum of every smaller column
+    # lifted directly (lifts by r1 then r2 equal the lift by r1 + r2)
+    rng = np.random.default_rng(62)
+    logs = [1, 3, 3, 5, 8]
+    cols = [rng.integers(0, P, 1 << L, dtype=np.uint32) for L in logs]
+    last = po.lift_and_accumulate(cols)[-1]
+    L = logs[-1]
+    want = cols[-1].astype(np.int64)
+    for c, l in zip(cols[:-1], logs[:-1]):
+        idx = np.array([((i >> (L - l + 1)) << 1) | (i & 1) for i in range(1 << L)])
+        want = (want + c[idx]) % P
+    assert np.array_equal(last, want)

@@ -88,6 +88,14 @@ SIGNATURES = {
     "vx_eval_at_point": (ip, [vp, vp, vp, u32, vp, vp, vp]),
     "vx_barycentric_eval": (ip, [vp, vp, u32, vp, u32, vp]),
     "vx_grind_pow_stwo": (ip, [vp, u32, vp, vp]),
+    "vx_m31_add": (ip, [vp, vp, vp, u32, vp]),
+    "vx_m31_mul": (ip, [vp, vp, vp, u32, vp]),
+    "vx_accumulate_lift": (ip, [vp, vp, u32, u32, vp]),
+    "vx_lift_and_accumulate": (ip, [vp, vp, u32, vp]),
+    "vx_gather_u32": (ip, [vp, vp, vp, u32, vp]),
+    "vx_gather_u32_checked": (ip, [vp, vp, vp, u32, u32, vp]),
+    "vx_gather_u256": (ip, [vp, vp, vp, u32, vp]),
+    "vx_gather_u256_checked": (ip, [vp, vp, vp, u32, u32, vp]),
     "vx_accumulate_numerators": (ip, [vp, vp, vp, vp, u32, u32, vp, vp, vp, vp, vp]),
     "vx_accumulate_numerators_dual": (ip, [vp, vp, vp, vp, vp, vp, vp, u32, u32] + [vp] * 8 + [vp]),
     "vx_compute_quotients_combine": (ip, [vp] * 8 + [u32, vp, vp, u32, u32, vp, vp, vp, vp, vp]),

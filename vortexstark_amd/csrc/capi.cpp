@@ -321,6 +321,76 @@ int vx_compute_quotients_combine(const uint32_t* d_sp_x, const uint32_t* d_sp_y,
   });
 }
 
+int vx_m31_add(const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(n == 0 || (a && b && out), "m31_add: null pointer");
+    m31_binop(a, b, out, n, false, S(st));
+  });
+}
+
+int vx_m31_mul(const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(n == 0 || (a && b && out), "m31_mul: null pointer");
+    m31_binop(a, b, out, n, true, S(st));
+  });
+}
+
+int vx_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio, vx_stream_t st) {
+  return guard([&] {
+    require(col_n == 0 || (col && curr), "accumulate_lift: null pointer");
+    require(log_ratio <= 30, "accumulate_lift: log_ratio must be <= 30");
+    accumulate_lift(col, curr, col_n, log_ratio, S(st));
+  });
+}
+
+int vx_lift_and_accumulate(uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols, vx_stream_t st) {
+  return guard([&] {
+    require(h_cols && h_log_sizes && n_cols >= 1, "lift_and_accumulate: null argument");
+    const uint32_t L = h_log_sizes[n_cols - 1];
+    for (uint32_t k = 0; k < n_cols; k++) {
+      require(h_log_sizes[k] >= 1 && h_log_sizes[k] <= 31 && (k == 0 || h_log_sizes[k] >= h_log_sizes[k - 1]),
+              "lift_and_accumulate: log sizes must be 1..31 and ascending");
+      for (uint32_t c = 0; c < 4; c++) require(h_cols[c * n_cols + k] != nullptr, "lift_and_accumulate: null column");
+    }
+    const uint32_t n_src = n_cols - 1;
+    // device tables: src pointers (channel-major), shifts, the 4 last columns
+    std::vector<uint64_t> tab(4 * (size_t)n_src + 4 + (n_src + 1) / 2 + 1, 0);
+    for (uint32_t c = 0; c < 4; c++)
+      for (uint32_t k = 0; k < n_src; k++) tab[c * n_src + k] = reinterpret_cast<uint64_t>(h_cols[c * n_cols + k]);
+    for (uint32_t c = 0; c < 4; c++) tab[4 * n_src + c] = reinterpret_cast<uint64_t>(h_cols[c * n_cols + n_src]);
+    uint32_t* shifts = reinterpret_cast<uint32_t*>(tab.data() + 4 * n_src + 4);
+    for (uint32_t k = 0; k < n_src; k++) shifts[k] = L - h_log_sizes[k] + 1;
+    DeviceBuf d(tab.size() * 8);
+    VX_HIP(hipMemcpyAsync(d.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, S(st)));
+    const uint64_t* dp = static_cast<const uint64_t*>(d.p);
+    lift_accumulate(reinterpret_cast<const uint32_t* const*>(dp), reinterpret_cast<const uint32_t*>(dp + 4 * n_src + 4),
+                    n_src, reinterpret_cast<uint32_t* const*>(dp + 4 * n_src), 1u << L, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));
+  });
+}
+
+static int gather_call(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, uint32_t words,
+                       bool checked, uint64_t src_len, vx_stream_t st) {
+  return guard([&] {
+    require(n == 0 || (src && idx && dst), "gather: null pointer");
+    gather(src, idx, dst, n, words, checked, src_len, S(st));
+  });
+}
+int vx_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, vx_stream_t st) {
+  return gather_call(src, idx, dst, n, 1, false, 0, st);
+}
+int vx_gather_u32_checked(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, uint32_t src_len,
+                          vx_stream_t st) {
+  return gather_call(src, idx, dst, n, 1, true, src_len, st);
+}
+int vx_gather_u256(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, vx_stream_t st) {
+  return gather_call(src, idx, dst, n, 8, false, 0, st);
+}
+int vx_gather_u256_checked(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, uint32_t src_len,
+                           vx_stream_t st) {
+  return gather_call(src, idx, dst, n, 8, true, src_len, st);
+}
+
 int vx_stwo_ntt_batch_evaluate(uint32_t* const* cols, const uint32_t* tw, uint32_t n, uint32_t n_cols,
                                vx_stream_t st) {
   return guard([&] {

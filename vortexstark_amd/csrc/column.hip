@@ -1,0 +1,127 @@
+// Column operations of the stwo plugin (crates/cuda-backend/src/column.rs,
+// accumulation_ops.rs): element-wise M31 add / mul, the lifted accumulation
+// and index gathers.
+//
+// Reference kernels: cuda_m31_add / cuda_m31_mul (src/cuda/ffi.rs:212-213),
+// cuda/accumulate_lift.cu:14-40, cuda/gather.cu and the bounds-checked
+// cuda/forge/gather_forge.cu:82-116. The element-wise kernels move 16 bytes
+// per lane (uint4) when the buffers allow it; lift_and_accumulate fuses the
+// plugin's per-column loop into one pass over the largest column.
+#include "vx_internal.h"
+
+namespace vx {
+
+static constexpr uint32_t COL_THREADS = 256;
+
+static bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+template <bool MUL>
+__global__ void k_m31_binop4(const uint4* a, const uint4* b, uint4* out, uint64_t n4) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  const uint4 x = a[i], y = b[i];
+  out[i] = MUL ? make_uint4(m31_mul(x.x, y.x), m31_mul(x.y, y.y), m31_mul(x.z, y.z), m31_mul(x.w, y.w))
+               : make_uint4(m31_add(x.x, y.x), m31_add(x.y, y.y), m31_add(x.z, y.z), m31_add(x.w, y.w));
+}
+template <bool MUL>
+__global__ void k_m31_binop(const uint32_t* a, const uint32_t* b, uint32_t* out, uint64_t from, uint64_t n) {
+  const uint64_t i = from + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = MUL ? m31_mul(a[i], b[i]) : m31_add(a[i], b[i]);
+}
+
+void m31_binop(const uint32_t* a, const uint32_t* b, uint32_t* out, uint64_t n, bool mul, hipStream_t s) {
+  if (n == 0) return;
+  uint64_t from = 0;
+  if (al16(a) && al16(b) && al16(out) && n >= 4) {
+    const uint64_t n4 = n / 4;
+    const dim3 g((uint32_t)((n4 + COL_THREADS - 1) / COL_THREADS));
+    if (mul)
+      hipLaunchKernelGGL(k_m31_binop4<true>, g, dim3(COL_THREADS), 0, s, reinterpret_cast<const uint4*>(a),
+                         reinterpret_cast<const uint4*>(b), reinterpret_cast<uint4*>(out), n4);
+    else
+      hipLaunchKernelGGL(k_m31_binop4<false>, g, dim3(COL_THREADS), 0, s, reinterpret_cast<const uint4*>(a),
+                         reinterpret_cast<const uint4*>(b), reinterpret_cast<uint4*>(out), n4);
+    from = n4 * 4;
+  }
+  if (from < n) {
+    const dim3 g((uint32_t)((n - from + COL_THREADS - 1) / COL_THREADS));
+    if (mul)
+      hipLaunchKernelGGL(k_m31_binop<true>, g, dim3(COL_THREADS), 0, s, a, b, out, from, n);
+    else
+      hipLaunchKernelGGL(k_m31_binop<false>, g, dim3(COL_THREADS), 0, s, a, b, out, from, n);
+  }
+  VX_HIP(hipGetLastError());
+}
+
+// col[i] += curr[((i >> (log_ratio + 1)) << 1) | (i & 1)]
+__global__ void k_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= col_n) return;
+  col[i] = m31_add(col[i], curr[((i >> (log_ratio + 1)) << 1) | (i & 1u)]);
+}
+
+void accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio, hipStream_t s) {
+  if (col_n == 0) return;
+  hipLaunchKernelGGL(k_accumulate_lift, dim3((col_n + COL_THREADS - 1) / COL_THREADS), dim3(COL_THREADS), 0, s,
+                     col, curr, col_n, log_ratio);
+  VX_HIP(hipGetLastError());
+}
+
+// The plugin's lift_and_accumulate (accumulation_ops.rs:44-85) applies
+// accumulate_lift column by column, smallest first, each onto the running
+// sum. Lifting composes: ((j >> (r2 + 1)) << 1) | (j & 1) of
+// j = ((i >> (r1 + 1)) << 1) | (i & 1) is the lift of i by r1 + r2. So the
+// largest column's final value is
+//   last[i] + sum_k col_k[((i >> (L - L_k + 1)) << 1) | (i & 1)],
+// computed here in one pass per channel (blockIdx.y) over the largest
+// column; the smaller columns are read, not written.
+__global__ void k_lift_accumulate(const uint32_t* const* d_src, const uint32_t* d_shift, uint32_t n_src,
+                                  uint32_t* const* d_last, uint32_t last_n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t ch = blockIdx.y;
+  if (i >= last_n) return;
+  uint32_t* last = ld_uniform(d_last + ch);
+  uint32_t acc = last[i];
+  for (uint32_t k = 0; k < n_src; k++) {
+    const uint32_t* c = ld_uniform(d_src + (uint64_t)ch * n_src + k);
+    const uint32_t sh = ld_uniform(d_shift + k);
+    acc = m31_add(acc, c[((i >> sh) << 1) | (i & 1u)]);
+  }
+  last[i] = acc;
+}
+
+void lift_accumulate(const uint32_t* const* d_src, const uint32_t* d_shift, uint32_t n_src, uint32_t* const* d_last,
+                     uint32_t last_n, hipStream_t s) {
+  if (last_n == 0) return;
+  hipLaunchKernelGGL(k_lift_accumulate, dim3((last_n + COL_THREADS - 1) / COL_THREADS, 4), dim3(COL_THREADS), 0, s,
+                     d_src, d_shift, n_src, d_last, last_n);
+  VX_HIP(hipGetLastError());
+}
+
+// dst[i] = src[idx[i]] (W words per element); with src_len set, elements
+// whose source words are not all below src_len are skipped, as the
+// bounds-checked reference variant does.
+template <uint32_t W>
+__global__ void k_gather(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, uint64_t src_len,
+                         int checked) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t off = (uint64_t)idx[i] * W;
+  if (checked && off + W - 1 >= src_len) return;
+#pragma unroll
+  for (uint32_t w = 0; w < W; w++) dst[(uint64_t)i * W + w] = src[off + w];
+}
+
+void gather(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, uint32_t words, bool checked,
+            uint64_t src_len, hipStream_t s) {
+  if (n == 0) return;
+  const dim3 g((n + COL_THREADS - 1) / COL_THREADS);
+  if (words == 1)
+    hipLaunchKernelGGL(k_gather<1>, g, dim3(COL_THREADS), 0, s, src, idx, dst, n, src_len, checked ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_gather<8>, g, dim3(COL_THREADS), 0, s, src, idx, dst, n, src_len, checked ? 1 : 0);
+  VX_HIP(hipGetLastError());
+}
+
+}  // namespace vx

@@ -167,6 +167,18 @@ void quotients_combine(const uint32_t* sp_x, const uint32_t* sp_y, const uint32_
                        const uint32_t* xs, const uint32_t* ys, uint32_t lifting_log_size, uint32_t n_rows,
                        uint32_t* const out[4], hipStream_t s);
 
+// --------------------------------- stwo plugin column ops (column.hip)
+void m31_binop(const uint32_t* a, const uint32_t* b, uint32_t* out, uint64_t n, bool mul, hipStream_t s);
+void accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio, hipStream_t s);
+// For each of 4 channels ch: last[ch][i] += sum_k src[ch * n_src + k][((i >> shift[k]) << 1) | (i & 1)]
+// (device arrays: src pointers channel-major, shifts, the 4 last-column pointers).
+void lift_accumulate(const uint32_t* const* d_src, const uint32_t* d_shift, uint32_t n_src, uint32_t* const* d_last,
+                     uint32_t last_n, hipStream_t s);
+// dst[i] = src[idx[i]] with `words` (1 or 8) words per element; checked skips
+// elements whose source words are not all below src_len.
+void gather(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, uint32_t words, bool checked,
+            uint64_t src_len, hipStream_t s);
+
 // ----------------------------------------------- twiddles (twiddles.hip)
 void coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n, hipStream_t s);
 void extract_and_squash(const uint32_t* in, uint32_t* tw_out, uint32_t* sq_out, uint64_t half_n,
