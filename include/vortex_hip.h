@@ -238,6 +238,14 @@ int vx_gather_u256(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst
 int vx_gather_u256_checked(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst, uint32_t n,
                            uint32_t src_len, vx_stream_t stream);
 
+/* Replaces cuda_pack_leaves (ffi.rs:1187-1195, cuda/pack_leaves.cu:13-35; the
+ * plugin's PackLeavesOps::pack_leaves_input, merkle_ops.rs:197-228): 4 M31
+ * columns of N words (N a multiple of 16) into 64 columns of N / 16 words,
+ * out[16 c + (p mod 16)][p / 16] = in[c][p]. d_input_col_ptrs (4) and
+ * d_output_col_ptrs (64) are DEVICE arrays of device pointers. Async. */
+int vx_pack_leaves(const uint32_t *const *d_input_col_ptrs, uint32_t *const *d_output_col_ptrs, uint32_t n,
+                   vx_stream_t stream);
+
 /* ------------------------------------------------------------- twiddles */
 
 /* Replaces cuda_compute_coset_points (ffi.rs:453-458, cuda/twiddles.cu:80-105,182-194). */

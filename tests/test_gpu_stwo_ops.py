@@ -229,3 +229,19 @@ def test_gather(words):
     ok = idx2 < n_src
     assert np.array_equal(got[ok], src.reshape(n_src, words)[idx2[ok]])
     assert np.array_equal(got[~ok], before.reshape(n, words)[~ok])
+
+
+@pytest.mark.parametrize("n", [16, 4096, 1 << 20])
+def test_pack_leaves(n):
+    ins = [rnd(n, 80 + c) for c in range(4)]
+    din = [dev(x) for x in ins]
+    dout = [empty(n // 16) for _ in range(64)]
+    pi, po_ = dev_ptrs(din), dev_ptrs(dout)
+    call("vx_pack_leaves", ptr(pi), ptr(po_), n, NULL)
+    sync()
+    for c in range(4):
+        for m in range(16):
+            # out[16 c + m][r] = in[c][16 r + m]
+            assert np.array_equal(host(dout[16 * c + m]), ins[c][m::16]), (c, m)
+    with pytest.raises(VortexError):
+        call("vx_pack_leaves", ptr(pi), ptr(po_), n + 8, NULL)

@@ -96,6 +96,7 @@ SIGNATURES = {
     "vx_gather_u32_checked": (ip, [vp, vp, vp, u32, u32, vp]),
     "vx_gather_u256": (ip, [vp, vp, vp, u32, vp]),
     "vx_gather_u256_checked": (ip, [vp, vp, vp, u32, u32, vp]),
+    "vx_pack_leaves": (ip, [vp, vp, u32, vp]),
     "vx_accumulate_numerators": (ip, [vp, vp, vp, vp, u32, u32, vp, vp, vp, vp, vp]),
     "vx_accumulate_numerators_dual": (ip, [vp, vp, vp, vp, vp, vp, vp, u32, u32] + [vp] * 8 + [vp]),
     "vx_compute_quotients_combine": (ip, [vp] * 8 + [u32, vp, vp, u32, u32, vp, vp, vp, vp, vp]),

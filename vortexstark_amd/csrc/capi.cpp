@@ -391,6 +391,14 @@ int vx_gather_u256_checked(const uint32_t* src, const uint32_t* idx, uint32_t* d
   return gather_call(src, idx, dst, n, 8, true, src_len, st);
 }
 
+int vx_pack_leaves(const uint32_t* const* d_in, uint32_t* const* d_out, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(n % 16 == 0, "pack_leaves: N must be a multiple of 16");
+    require(n == 0 || (d_in && d_out), "pack_leaves: null pointer array");
+    pack_leaves(d_in, d_out, n, S(st));
+  });
+}
+
 int vx_stwo_ntt_batch_evaluate(uint32_t* const* cols, const uint32_t* tw, uint32_t n, uint32_t n_cols,
                                vx_stream_t st) {
   return guard([&] {

@@ -1,6 +1,6 @@
 // Column operations of the stwo plugin (crates/cuda-backend/src/column.rs,
 // accumulation_ops.rs): element-wise M31 add / mul, the lifted accumulation
-// and index gathers.
+// index gathers and the 16-way leaf packing.
 //
 // Reference kernels: cuda_m31_add / cuda_m31_mul (src/cuda/ffi.rs:212-213),
 // cuda/accumulate_lift.cu:14-40, cuda/gather.cu and the bounds-checked
@@ -121,6 +121,43 @@ void gather(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n,
     hipLaunchKernelGGL(k_gather<1>, g, dim3(COL_THREADS), 0, s, src, idx, dst, n, src_len, checked ? 1 : 0);
   else
     hipLaunchKernelGGL(k_gather<8>, g, dim3(COL_THREADS), 0, s, src, idx, dst, n, src_len, checked ? 1 : 0);
+  VX_HIP(hipGetLastError());
+}
+
+// The plugin's PackLeavesOps (merkle_ops.rs:197-228, cuda/pack_leaves.cu:
+// 13-35): 4 M31 columns of N words into 64 columns of N / 16 words,
+// out[16 c + (p mod 16)][p / 16] = in[c][p]. A lane takes the 16 consecutive
+// words of one output row (four 16-byte loads when the column is aligned)
+// and writes one word to each of the 16 output columns of its coordinate;
+// neighbouring lanes write neighbouring words.
+__global__ void k_pack_leaves(const uint32_t* const* in, uint32_t* const* out, uint32_t rows) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;  // output row
+  const uint32_t c = blockIdx.y;                             // coordinate
+  if (r >= rows) return;
+  const uint32_t* src = ld_uniform(in + c) + (uint64_t)r * 16;
+  uint32_t v[16];
+  if ((reinterpret_cast<uintptr_t>(src) & 15u) == 0) {
+#pragma unroll
+    for (uint32_t m = 0; m < 4; m++) {
+      const uint4 x = reinterpret_cast<const uint4*>(src)[m];
+      v[4 * m] = x.x;
+      v[4 * m + 1] = x.y;
+      v[4 * m + 2] = x.z;
+      v[4 * m + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t m = 0; m < 16; m++) v[m] = src[m];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < 16; m++) ld_uniform(out + 16 * c + m)[r] = v[m];
+}
+
+void pack_leaves(const uint32_t* const* d_in, uint32_t* const* d_out, uint32_t n, hipStream_t s) {
+  const uint32_t rows = n / 16;
+  if (rows == 0) return;
+  hipLaunchKernelGGL(k_pack_leaves, dim3((rows + COL_THREADS - 1) / COL_THREADS, 4), dim3(COL_THREADS), 0, s, d_in,
+                     d_out, rows);
   VX_HIP(hipGetLastError());
 }
 

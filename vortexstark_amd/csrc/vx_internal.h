@@ -178,6 +178,9 @@ void lift_accumulate(const uint32_t* const* d_src, const uint32_t* d_shift, uint
 // elements whose source words are not all below src_len.
 void gather(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n, uint32_t words, bool checked,
             uint64_t src_len, hipStream_t s);
+// 4 columns of n (a multiple of 16) words into 64 columns of n / 16:
+// out[16 c + (p mod 16)][p / 16] = in[c][p] (device pointer arrays).
+void pack_leaves(const uint32_t* const* d_in, uint32_t* const* d_out, uint32_t n, hipStream_t s);
 
 // ----------------------------------------------- twiddles (twiddles.hip)
 void coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n, hipStream_t s);
