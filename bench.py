@@ -398,6 +398,30 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
         comb_call()
     call("vx_device_sync")
     comb_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    # OODS-style: 64 columns of 2^20 coefficients at one point, one batched
+    # call against 64 single calls (each synchronous, as the plugin's
+    # per-column eval_at_point)
+    oc, ol = 64, 20
+    ocols = [DeviceBuffer.from_host(np.random.default_rng(90 + i).integers(0, P, 1 << ol, dtype=np.uint32))
+             for i in range(oc)]
+    d_f = DeviceBuffer.from_host(stwo.folding_factors(zx, zy, ol))
+    d_res = DeviceBuffer(4 * oc)
+    o_ptrs = (C.c_void_p * oc)(*[b.as_ptr() for b in ocols])
+    o_logs = (C.c_uint32 * oc)(*([ol] * oc))
+
+    def batch_call():
+        call("vx_eval_at_point_batch", o_ptrs, o_logs, oc, vp(d_f.as_ptr()), ol, vp(d_res.as_ptr()), None)
+
+    batch_call()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch_call()
+    oods_batch_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    batch_vals = d_res.to_host().reshape(oc, 4)
+    t0 = time.perf_counter()
+    singles = [stwo.eval_at_point(b, 1 << ol, zx, zy) for b in ocols]
+    oods_single_ms = (time.perf_counter() - t0) * 1000.0
+    oods_same = all(tuple(int(v) for v in batch_vals[i]) == singles[i] for i in range(oc))
     out = {
         "workload": f"stwo-basis evaluate / interpolate, one 2^{log_n} column in HBM",
         "evaluate_ms": round(res["evaluate"], 4),
@@ -415,6 +439,10 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
                         "read_gbs": round(20 * nb / (bary_ms * 1e-3) / 1e9, 1), "synchronous": True},
         "accumulate_numerators": {"rows": qr, "columns": qc, "ms_per_call": round(accnum_ms, 4),
                                   "hbm_gbs": round(4 * qr * (qc + 4) / (accnum_ms * 1e-3) / 1e9, 1)},
+        "eval_at_point_batch": {"columns": oc, "log_n": ol, "ms_per_call": round(oods_batch_ms, 4),
+                                "ms_as_single_calls": round(oods_single_ms, 3),
+                                "read_gbs": round(4 * oc * (1 << ol) / (oods_batch_ms * 1e-3) / 1e9, 1),
+                                "same_as_single_calls": oods_same},
         "quotients_combine": {"rows": qr, "accumulators": 2, "ms_per_call": round(comb_ms, 4),
                               "hbm_gbs": round(4 * qr * (2 + 8 + 4) / (comb_ms * 1e-3) / 1e9, 1)},
     }
@@ -432,7 +460,7 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
         out["cpu_baseline"] = {"evaluate_ms": round(te * 1000.0, 1), "interpolate_ms": round(ti * 1000.0, 1),
                                "cores": 1, "kind": "port", "roundtrip_ok": bool(np.array_equal(back, x)),
                                "sample": f"one full 2^{log_n} evaluate and interpolate, C oracle, 1 thread"}
-    for b in cols + [e, w, d] + qcols + [x for a in accs for x in a] + q_out:
+    for b in cols + [e, w, d] + qcols + [x for a in accs for x in a] + q_out + ocols:
         b.free()
     return out
 

@@ -158,6 +158,20 @@ int vx_stwo_ntt_interpolate_async(uint32_t *d_data, const uint32_t *d_itwiddles,
  * aligned. Synchronous. */
 int vx_eval_at_point(const uint32_t *d_coeffs, const uint32_t *d_folding_factors, uint32_t *h_result, uint32_t n,
                      uint32_t *d_scratch1, uint32_t *d_scratch2, vx_stream_t stream);
+/* Many polynomials at one point (the OODS step). Replaces
+ * cuda_batch_eval_at_point (cuda/batch_eval_at_point.cu:11-100), which is
+ * limited to 64 coefficients per polynomial and marks larger ones with
+ * 0xFFFFFFFF; here any power-of-two size up to 2^max_log_n. Column k has
+ * 2^h_log_sizes[k] coefficients and uses the last h_log_sizes[k] of the
+ * max_log_n factors (d_folding_factors, the factors of a 2^max_log_n
+ * polynomial, as for vx_eval_at_point). h_cols and h_log_sizes are HOST
+ * arrays (the reference passes device arrays); columns of 4 or more words
+ * 16-byte aligned. d_results: n_cols QM31 (device). One launch for every
+ * 2^14-coefficient chunk of every column, then one per fold level.
+ * Synchronous. */
+int vx_eval_at_point_batch(const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
+                           const uint32_t *d_folding_factors, uint32_t max_log_n, uint32_t *d_results,
+                           vx_stream_t stream);
 /* Replaces cuda_barycentric_eval (ffi.rs:1037-1043, cuda/barycentric_eval.cu:
  * 41-110): d_out[4b..4b+4) = sum of d_evals[i] * d_weights[i] (QM31, 4 words
  * each) over i = b*256 + t + k*n_blocks*256 (t < 256), the reference's

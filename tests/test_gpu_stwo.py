@@ -255,3 +255,23 @@ def test_mirror_eval_at_point_and_barycentric():
     with pytest.raises(VortexError):
         call("vx_eval_at_point", pin(dev(c)), pin(dev(np.zeros(56, np.uint32))), C.c_void_p(res.ctypes.data),
              n - 1, pin(empty(64)), pin(empty(64)), NULL)
+
+
+@pytest.mark.parametrize("logs,max_log_n", [([0, 1, 2, 3, 5, 10, 13, 14, 15, 17, 22, 14, 2], 22),
+                                             ([16, 16, 16, 16], 24), ([4], 4), ([20] * 3 + [23], 23)])
+def test_eval_at_point_batch(logs, max_log_n):
+    print(f"eval_at_point_batch logs={logs} max={max_log_n}", flush=True)
+    rng = np.random.default_rng(sum(logs) + max_log_n)
+    zx, zy = (tuple(int(v) for v in rng.integers(0, P, 4)) for _ in range(2))
+    cols = [rng.integers(0, P, 1 << L, dtype=np.uint32) for L in logs]
+    bufs = [dev(c) for c in cols]
+    f = dev(po.folding_factors(zx, zy, max_log_n))
+    res = empty(4 * len(logs))
+    call("vx_eval_at_point_batch", (C.c_void_p * len(logs))(*[b.ptr for b in bufs]),
+         (C.c_uint32 * len(logs))(*logs), len(logs), ptr(f), max_log_n, ptr(res), NULL)
+    got = host(res).reshape(-1, 4)
+    for k, c in enumerate(cols):
+        assert tuple(int(v) for v in got[k]) == po.stwo_eval_at_point(c, zx, zy), (k, logs[k])
+    with pytest.raises(VortexError):  # a column larger than the factors cover
+        call("vx_eval_at_point_batch", (C.c_void_p * 1)(bufs[0].ptr), (C.c_uint32 * 1)(max_log_n + 1), 1, ptr(f),
+             max_log_n, ptr(res), NULL)

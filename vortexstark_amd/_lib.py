@@ -86,6 +86,7 @@ SIGNATURES = {
     "vx_circle_ntt_lde_batch": (ip, [vp, vp, vp, vp, u32, u32, vp]),
     "vx_stwo_twiddles": (ip, [u32, vp, vp, vp]),
     "vx_eval_at_point": (ip, [vp, vp, vp, u32, vp, vp, vp]),
+    "vx_eval_at_point_batch": (ip, [vp, vp, u32, vp, u32, vp, vp]),
     "vx_barycentric_eval": (ip, [vp, vp, u32, vp, u32, vp]),
     "vx_grind_pow_stwo": (ip, [vp, u32, vp, vp]),
     "vx_m31_add": (ip, [vp, vp, vp, u32, vp]),

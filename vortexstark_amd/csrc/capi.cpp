@@ -261,6 +261,16 @@ int vx_eval_at_point(const uint32_t* d_coeffs, const uint32_t* d_factors, uint32
   });
 }
 
+int vx_eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                           const uint32_t* d_factors, uint32_t max_log_n, uint32_t* d_results, vx_stream_t st) {
+  return guard([&] {
+    if (n_cols == 0) return;
+    require(h_cols && h_log_sizes && d_results, "eval_at_point_batch: null argument");
+    require(max_log_n == 0 || d_factors, "eval_at_point_batch: null factors");
+    eval_at_point_batch(h_cols, h_log_sizes, n_cols, d_factors, max_log_n, d_results, S(st));
+  });
+}
+
 int vx_barycentric_eval(const uint32_t* d_evals, const uint32_t* d_weights, uint32_t n, uint32_t* d_out,
                         uint32_t n_blocks, vx_stream_t st) {
   return guard([&] {

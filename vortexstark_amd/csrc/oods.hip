@@ -15,6 +15,7 @@
 // 16 products of F[0..3] in LDS). Products are accumulated 4 per 64-bit sum
 // before a reduction; a workgroup folds its lanes in LDS, and the block sums
 // are folded 256 at a time with the next 8 factors.
+#include "prover.h"
 #include "vx_internal.h"
 
 namespace vx {
@@ -209,6 +210,212 @@ const uint32_t* eval_at_point(const uint32_t* c, const uint32_t* factors, uint32
     out = tmp;
   }
   return in;
+}
+
+// ---------------------------------------------------------------------------
+// Many columns at one point (the OODS step evaluates every committed column
+// at z; cuda/batch_eval_at_point.cu:11-100 restricts this to n <= 64). A
+// column of 2^L coefficients uses the last L of the max_log_n factors, so
+// coefficient bit b of every column is folded by the same
+// F[b] = factors[max_log_n - 1 - b]. Columns are cut into chunks of 2^14
+// coefficients (shorter columns are one chunk, the words past their end
+// read as zero, which no factor can change), and one launch runs every
+// chunk of every column with the k_eval_big arithmetic (work list: each
+// workgroup binary-searches its column). Per column, the chunk sums are
+// then folded 256 at a time with F[14..], one launch per level for all
+// columns.
+struct EvalChunkJob {
+  const uint32_t* col;
+  uint32_t log_n;
+  uint32_t first_chunk;  // jobs sorted by first_chunk
+};
+struct EvalFoldJob {
+  uint32_t in;   // first QM31 of the group in the input level
+  uint32_t out;  // its QM31 in the output level
+  uint32_t n;    // sums in the group (a power of two, <= 256)
+  uint32_t fb;   // coefficient bit folded by the group's level 0
+};
+
+__device__ __forceinline__ qm31 factor_or_one(const uint32_t* factors, uint32_t max_log_n, uint32_t b) {
+  return b < max_log_n ? ld_factor(factors, max_log_n, b) : qm31{{1u, 0u, 0u, 0u}};
+}
+
+// table: E[16] over bits {0,1,10,11}, D[4] over {12,13}, F[2..9] at 20..27
+__global__ void k_eval_batch_setup(const uint32_t* factors, uint32_t max_log_n, uint32_t* table) {
+  const uint32_t q = threadIdx.x;
+  qm31 b{{1u, 0u, 0u, 0u}};
+  if (q < 16) {
+    const uint32_t bits[4] = {0, 1, 10, 11};
+    for (uint32_t k = 0; k < 4; k++)
+      if ((q >> k) & 1u) b = qm_mul(b, factor_or_one(factors, max_log_n, bits[k]));
+  } else if (q < 20) {
+    for (uint32_t k = 0; k < 2; k++)
+      if (((q - 16) >> k) & 1u) b = qm_mul(b, factor_or_one(factors, max_log_n, 12 + k));
+  } else if (q < 28) {
+    b = factor_or_one(factors, max_log_n, 2 + (q - 20));
+  } else {
+    return;
+  }
+  reinterpret_cast<qm31*>(table)[q] = b;
+}
+
+// 256-lane LDS fold of q[0..256) with the factors f[0..8) (uniform).
+__device__ qm31 lds_fold_table(qm31* q, qm31 v, uniform_t<qm31>* f) {
+  const uint32_t t = threadIdx.x;
+  q[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < EV_WG_LOG; j++) {
+    const qm31 fj{{f[j].v[0], f[j].v[1], f[j].v[2], f[j].v[3]}};
+    if ((t & ((2u << j) - 1)) == 0) q[t] = qm_add(q[t], qm_mul(fj, q[t + (1u << j)]));
+    __syncthreads();
+  }
+  return q[0];
+}
+
+__global__ __launch_bounds__(EV_WG) void k_eval_batch_chunks(const EvalChunkJob* jobs, uint32_t n_jobs,
+                                                             const uint32_t* table, uint32_t* part) {
+  __shared__ qm31 q[EV_WG];
+  const uint32_t b = blockIdx.x, t = threadIdx.x;
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ld_uniform(&jobs[mid].first_chunk) <= b) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t* col = ld_uniform(&jobs[lo].col);
+  const uint32_t L = ld_uniform(&jobs[lo].log_n);
+  const uint64_t chunk = b - ld_uniform(&jobs[lo].first_chunk);
+  const uint64_t n = 1ull << L;
+  uint4 x[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; m++) {
+    const uint64_t w = (chunk << EV_BIG_LOG) + 4ull * (t + m * EV_WG);  // first word of the load
+    if (L >= 2) {
+      x[m] = w < n ? reinterpret_cast<const uint4*>(col)[w >> 2] : make_uint4(0u, 0u, 0u, 0u);
+    } else {
+      x[m] = make_uint4(w < n ? col[w] : 0u, w + 1 < n ? col[w + 1] : 0u, 0u, 0u);
+    }
+  }
+  uniform_t<qm31>* E = (uniform_t<qm31>*)table;
+  qm31 w4[4];
+#pragma unroll
+  for (uint32_t m23 = 0; m23 < 4; m23++) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      uint32_t r = 0;
+#pragma unroll
+      for (uint32_t m01 = 0; m01 < 4; m01++) {
+        const uint4 v = x[m01 + 4 * m23];
+        const uint64_t acc = (uint64_t)v.x * E[4 * m01].v[k] + (uint64_t)v.y * E[4 * m01 + 1].v[k] +
+                             (uint64_t)v.z * E[4 * m01 + 2].v[k] + (uint64_t)v.w * E[4 * m01 + 3].v[k];
+        r = m31_add(r, m31_reduce64(acc));
+      }
+      w4[m23].v[k] = r;
+    }
+  }
+  qm31 v = w4[0];
+#pragma unroll
+  for (uint32_t m23 = 1; m23 < 4; m23++)
+    v = qm_add(v, qm_mul(ld_uniform(reinterpret_cast<const qm31*>(table) + 16 + m23), w4[m23]));
+  const qm31 sum = lds_fold_table(q, v, (uniform_t<qm31>*)table + 20);
+  if (t == 0) reinterpret_cast<qm31*>(part)[b] = sum;
+}
+
+__global__ __launch_bounds__(EV_WG) void k_eval_batch_fold(const EvalFoldJob* jobs, const uint32_t* factors,
+                                                           uint32_t max_log_n, const uint32_t* in, uint32_t* out) {
+  __shared__ qm31 q[EV_WG];
+  const uint32_t t = threadIdx.x;
+  uniform_t<EvalFoldJob>& J = as_uniform(jobs + blockIdx.x);
+  const uint32_t n = J.n, fb = J.fb;
+  qm31 f[EV_WG_LOG];
+#pragma unroll
+  for (uint32_t j = 0; j < EV_WG_LOG; j++) f[j] = factor_or_one(factors, max_log_n, fb + j);
+  q[t] = t < n ? reinterpret_cast<const qm31*>(in)[J.in + t] : qm31{{0u, 0u, 0u, 0u}};
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < EV_WG_LOG; j++) {
+    if ((t & ((2u << j) - 1)) == 0) q[t] = qm_add(q[t], qm_mul(f[j], q[t + (1u << j)]));
+    __syncthreads();
+  }
+  if (t == 0) reinterpret_cast<qm31*>(out)[J.out] = q[0];
+}
+
+// result[k] = QM31 at src[k] (device pointers, one per column)
+__global__ void k_copy_qm31(const uint32_t* const* src, uint32_t* results, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  reinterpret_cast<qm31*>(results)[k] = *reinterpret_cast<const qm31*>(src[k]);
+}
+
+void eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                         const uint32_t* factors, uint32_t max_log_n, uint32_t* results, hipStream_t s) {
+  if (n_cols == 0) return;
+  require(max_log_n <= 31, "eval_at_point_batch: max_log_n must be <= 31");
+  std::vector<EvalChunkJob> jobs(n_cols);
+  std::vector<uint32_t> count(n_cols), off(n_cols);
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < n_cols; k++) {
+    const uint32_t L = h_log_sizes[k];
+    require(h_cols[k] != nullptr && L <= max_log_n, "eval_at_point_batch: null column or log size above max_log_n");
+    require(L < 2 || (reinterpret_cast<uintptr_t>(h_cols[k]) & 15u) == 0,
+            "eval_at_point_batch: columns must be 16-byte aligned");
+    count[k] = L > EV_BIG_LOG ? 1u << (L - EV_BIG_LOG) : 1u;
+    off[k] = (uint32_t)total;
+    jobs[k] = {h_cols[k], L, (uint32_t)total};
+    total += count[k];
+  }
+  require(total < (1ull << 31), "eval_at_point_batch: too many chunks");
+  // every level's sums stay allocated until the results are copied: a column
+  // that is down to one sum keeps it where it was written
+  std::vector<DeviceBuf> keep;
+  keep.emplace_back(sizeof(EvalChunkJob) * n_cols);
+  VX_HIP(hipMemcpyAsync(keep.back().p, jobs.data(), sizeof(EvalChunkJob) * n_cols, hipMemcpyHostToDevice, s));
+  const EvalChunkJob* d_jobs = static_cast<const EvalChunkJob*>(keep.back().p);
+  keep.emplace_back(28 * 16);
+  uint32_t* d_table = keep.back().u32();
+  keep.emplace_back(total * 16);
+  uint32_t* level = keep.back().u32();
+  hipLaunchKernelGGL(k_eval_batch_setup, dim3(1), dim3(64), 0, s, factors, max_log_n, d_table);
+  hipLaunchKernelGGL(k_eval_batch_chunks, dim3((uint32_t)total), dim3(EV_WG), 0, s, d_jobs, n_cols, d_table, level);
+  VX_HIP(hipGetLastError());
+  std::vector<const uint32_t*> final_src(n_cols, nullptr);
+  for (uint32_t k = 0; k < n_cols; k++)
+    if (count[k] == 1) final_src[k] = level + 4ull * off[k];
+  uint32_t fb = EV_BIG_LOG;
+  for (;;) {
+    std::vector<EvalFoldJob> fj;
+    uint32_t next = 0;
+    for (uint32_t k = 0; k < n_cols; k++) {
+      if (count[k] <= 1) continue;
+      const uint32_t w = count[k] < EV_WG ? count[k] : EV_WG;
+      const uint32_t groups = count[k] / w;
+      for (uint32_t g = 0; g < groups; g++) fj.push_back({off[k] + g * w, next + g, w, fb});
+      off[k] = next;
+      count[k] = groups;
+      next += groups;
+    }
+    if (fj.empty()) break;
+    keep.emplace_back(sizeof(EvalFoldJob) * fj.size());
+    VX_HIP(hipMemcpyAsync(keep.back().p, fj.data(), sizeof(EvalFoldJob) * fj.size(), hipMemcpyHostToDevice, s));
+    const EvalFoldJob* d_fj = static_cast<const EvalFoldJob*>(keep.back().p);
+    keep.emplace_back((size_t)next * 16);
+    uint32_t* out = keep.back().u32();
+    hipLaunchKernelGGL(k_eval_batch_fold, dim3((uint32_t)fj.size()), dim3(EV_WG), 0, s, d_fj, factors, max_log_n,
+                       level, out);
+    VX_HIP(hipGetLastError());
+    level = out;
+    for (uint32_t k = 0; k < n_cols; k++)
+      if (count[k] == 1 && final_src[k] == nullptr) final_src[k] = level + 4ull * off[k];
+    fb += EV_WG_LOG;
+  }
+  keep.emplace_back(sizeof(uint32_t*) * n_cols);
+  VX_HIP(hipMemcpyAsync(keep.back().p, final_src.data(), sizeof(uint32_t*) * n_cols, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_copy_qm31, dim3((n_cols + 255) / 256), dim3(256), 0, s,
+                     static_cast<const uint32_t* const*>(keep.back().p), results, n_cols);
+  VX_HIP(hipGetLastError());
+  // the staging buffers are freed on return: the caller's stream must be done
+  VX_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------

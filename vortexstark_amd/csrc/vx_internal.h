@@ -149,6 +149,11 @@ void zero_pad(const uint32_t* src, uint32_t* dst, uint64_t src_n, uint64_t dst_n
 // words); returns the one holding the QM31.
 const uint32_t* eval_at_point(const uint32_t* coeffs, const uint32_t* factors, uint32_t log_n, uint32_t* s1,
                               uint32_t* s2, hipStream_t s);
+// Columns k < n_cols (HOST arrays of device pointers and log sizes <=
+// max_log_n) at one point: results[k] (QM31, device) = f_k(z), column k
+// using the last log_size_k of the max_log_n factors. Synchronous.
+void eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                         const uint32_t* factors, uint32_t max_log_n, uint32_t* results, hipStream_t s);
 // Per-block partial sums of evals[i] * weights[i] over the reference's
 // grid-stride partition (n_blocks blocks of 256 lanes); out: n_blocks QM31.
 void barycentric_partials(const uint32_t* evals, const uint32_t* weights, uint32_t n, uint32_t* out,
