@@ -323,6 +323,30 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
         res[name] = (time.perf_counter() - t0) * 1000.0 / steps
     # evaluate and interpolate ran `steps + 2` times each: the column is back
     roundtrip_ok = bool(np.array_equal(d.to_host(), x))
+    # PolyOps::evaluate of 2^(log_n - 2) coefficients on the 2^log_n domain:
+    # fused (replication layers skipped) against the plugin's extend
+    # (zero-fill + copy) followed by the full transform
+    cn = n >> 2
+    d_co = DeviceBuffer.from_host(x[:cn])
+
+    def lde_fused():
+        call("vx_stwo_ntt_evaluate_lde", ptr, C.c_void_p(d_co.as_ptr()), cn, tw, n, None)
+
+    def lde_plugin():
+        call("vx_memset", ptr, 0, 4 * n)
+        call("vx_memcpy", ptr, C.c_void_p(d_co.as_ptr()), 4 * cn, 3)
+        call("vx_stwo_ntt_evaluate_async", ptr, tw, n, 1, 0, None)
+
+    lde_ms = {}
+    for name, fn in (("fused", lde_fused), ("extend_then_evaluate", lde_plugin)):
+        fn()
+        call("vx_device_sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        call("vx_device_sync")
+        lde_ms[name] = round((time.perf_counter() - t0) * 1000.0 / steps, 4)
+    d_co.free()
     passes = 1 if log_n <= 14 else 2
     bytes_moved = passes * 8 * n
     m = 1 << batch_log
@@ -431,6 +455,7 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
         "evaluate_hbm_frac": round(bytes_moved / (res["evaluate"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "interpolate_hbm_frac": round(bytes_moved / (res["interpolate"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roundtrip_ok": roundtrip_ok,
+        "evaluate_lde_ms": {"coeffs": cn, "domain": n, **lde_ms},
         "batch": {"columns": batch_cols, "log_n": batch_log, "ms_per_call": round(batch_ms, 4),
                   "columns_per_s": round(batch_cols / (batch_ms * 1e-3), 1), "synchronous": True},
         "eval_at_point": {"log_n": log_n, "ms_per_call": round(eval_pt_ms, 4),

@@ -139,6 +139,22 @@ int vx_stwo_ntt_batch_evaluate(uint32_t *const *h_columns, const uint32_t *d_twi
                                vx_stream_t stream);
 int vx_stwo_ntt_batch_interpolate(uint32_t *const *h_columns, const uint32_t *d_itwiddles, uint32_t n,
                                   uint32_t n_cols, vx_stream_t stream);
+/* The plugin's PolyOps::evaluate on a larger domain (poly_ops.rs:652-672:
+ * extend = zero-fill + copy of the coefficients, then cuda_stwo_ntt_evaluate
+ * over all n words) in one call: d_eval[0..n) = evaluate(d_coeffs[0..coeff_n)
+ * zero-padded to n), d_twiddles being the n-point domain's. The top
+ * log2(n / coeff_n) layers only replicate the coefficients and are not run,
+ * and there is no zero-fill or copy pass. coeff_n a power of two in [2, n];
+ * d_eval must not alias d_coeffs. Async. */
+int vx_stwo_ntt_evaluate_lde(uint32_t *d_eval, const uint32_t *d_coeffs, uint32_t coeff_n,
+                             const uint32_t *d_twiddles, uint32_t n, vx_stream_t stream);
+/* The same for the plugin's evaluate_polynomials groups (poly_ops.rs:674-760:
+ * per target size, allocate + zero-fill + copy each column, then
+ * cuda_stwo_ntt_batch_evaluate): n_cols columns of coeff_n coefficients
+ * (HOST array h_coeffs of device pointers) into n-word evaluations (HOST
+ * array h_eval), every pass one launch for all columns. Synchronous. */
+int vx_stwo_ntt_evaluate_lde_batch(uint32_t *const *h_eval, const uint32_t *const *h_coeffs, uint32_t coeff_n,
+                                   uint32_t n_cols, const uint32_t *d_twiddles, uint32_t n, vx_stream_t stream);
 /* No counterpart in the reference: the same transforms, asynchronous, over
  * n_cols (1..65535) columns `stride` words apart (stride >= n when n_cols > 1),
  * for callers that keep columns in one allocation and order work by stream. */

@@ -275,3 +275,35 @@ def test_eval_at_point_batch(logs, max_log_n):
     with pytest.raises(VortexError):  # a column larger than the factors cover
         call("vx_eval_at_point_batch", (C.c_void_p * 1)(bufs[0].ptr), (C.c_uint32 * 1)(max_log_n + 1), 1, ptr(f),
              max_log_n, ptr(res), NULL)
+
+
+@pytest.mark.parametrize("coeff_log,log_n", [(1, 2), (3, 5), (10, 12), (14, 16), (18, 20), (20, 20), (22, 24)])
+def test_stwo_evaluate_lde(coeff_log, log_n):
+    print(f"stwo lde {coeff_log} -> {log_n}", flush=True)
+    n = 1 << log_n
+    c = rnd(1 << coeff_log, 400 + coeff_log)
+    tw, _ = dev_twiddles(log_n)
+    d_c, d_e = dev(c), empty(n)
+    call("vx_stwo_ntt_evaluate_lde", ptr(d_e), ptr(d_c), 1 << coeff_log, ptr(tw), n, NULL)
+    sync()
+    padded = np.zeros(n, np.uint32)
+    padded[:c.size] = c
+    fwd, _ = po.stwo_twiddles(log_n)
+    assert np.array_equal(host(d_e), po.stwo_evaluate(padded, fwd, omp=True))
+    assert np.array_equal(host(d_c), c)  # the coefficients are only read
+
+
+@pytest.mark.parametrize("coeff_log,log_n,n_cols", [(12, 14, 3), (16, 18, 5), (19, 21, 2)])
+def test_stwo_evaluate_lde_batch(coeff_log, log_n, n_cols):
+    n = 1 << log_n
+    cs = [rnd(1 << coeff_log, 500 + k) for k in range(n_cols)]
+    dcs = [dev(c) for c in cs]
+    des = [empty(n) for _ in range(n_cols)]
+    tw, _ = dev_twiddles(log_n)
+    call("vx_stwo_ntt_evaluate_lde_batch", cols_arg([d.ptr for d in des]), cols_arg([d.ptr for d in dcs]),
+         1 << coeff_log, n_cols, ptr(tw), n, NULL)
+    fwd, _ = po.stwo_twiddles(log_n)
+    for c, d in zip(cs, des):
+        padded = np.zeros(n, np.uint32)
+        padded[:c.size] = c
+        assert np.array_equal(host(d), po.stwo_evaluate(padded, fwd, omp=True))

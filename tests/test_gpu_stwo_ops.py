@@ -245,3 +245,26 @@ def test_pack_leaves(n):
             assert np.array_equal(host(dout[16 * c + m]), ins[c][m::16]), (c, m)
     with pytest.raises(VortexError):
         call("vx_pack_leaves", ptr(pi), ptr(po_), n + 8, NULL)
+
+
+def test_python_mirror_of_the_plugin_ops():
+    from vortexstark_amd import stwo
+    from vortexstark_amd.device import DeviceBuffer
+    n = 4096
+    cols = [rnd(n, 200 + i) for i in range(3)]
+    dcols = [DeviceBuffer.from_host(c) for c in cols]
+    b, c = rnd(8, 210).tolist(), rnd(8, 211).tolist()
+    out = stwo.accumulate_numerators(dcols, [2, 0], b, c, n)
+    want = po.accumulate_numerators(cols, [2, 0], b, c, n)
+    assert all(np.array_equal(o.to_host(), w) for o, w in zip(out, want))
+    ch = po.PyChannel()
+    ch.mix_digest([7, 7, 7, 7, 7, 7, 7, 7])
+    assert stwo.grind(ch.pow_prefix(12), 12) == ch.grind_stwo(12)
+    chans = [[DeviceBuffer.from_host(rnd(1 << L, 300 + 10 * k + L)) for L in (3, 6)] for k in range(4)]
+    host_chans = [[b_.to_host() for b_ in ch_] for ch_ in chans]
+    stwo.lift_and_accumulate(chans, [3, 6])
+    for k in range(4):
+        assert np.array_equal(chans[k][1].to_host(), po.lift_and_accumulate(host_chans[k])[1])
+    zx, zy = (1, 2, 3, 4), (5, 6, 7, 8)
+    vals = stwo.eval_at_point_batch(dcols, [12, 12, 12], zx, zy)
+    assert vals == [po.stwo_eval_at_point(x, zx, zy) for x in cols]

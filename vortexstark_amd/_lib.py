@@ -106,6 +106,8 @@ SIGNATURES = {
     "vx_stwo_ntt_batch_evaluate": (ip, [vp, vp, u32, u32, vp]),
     "vx_stwo_ntt_batch_interpolate": (ip, [vp, vp, u32, u32, vp]),
     "vx_stwo_ntt_evaluate_async": (ip, [vp, vp, u32, u32, u64, vp]),
+    "vx_stwo_ntt_evaluate_lde": (ip, [vp, vp, u32, vp, u32, vp]),
+    "vx_stwo_ntt_evaluate_lde_batch": (ip, [vp, vp, u32, u32, vp, u32, vp]),
     "vx_stwo_ntt_interpolate_async": (ip, [vp, vp, u32, u32, u64, vp]),
     "vx_bit_reverse_m31_batch": (ip, [vp, vp, vp, u32, vp]),
     "vx_build_leaves_lifted": (ip, [vp, vp, u32, u32, vp, vp]),

@@ -409,6 +409,46 @@ int vx_pack_leaves(const uint32_t* const* d_in, uint32_t* const* d_out, uint32_t
   });
 }
 
+int vx_stwo_ntt_evaluate_lde(uint32_t* d_eval, const uint32_t* d_coeffs, uint32_t coeff_n, const uint32_t* tw,
+                             uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    require(d_eval && d_coeffs && d_eval != d_coeffs, "stwo lde: null or aliasing buffers");
+    require(pow2(coeff_n) && coeff_n >= 2 && coeff_n <= n, "stwo lde: coeff_n must be a power of two in [2, n]");
+    TwiddleRef r = stwo_ref(tw, n);
+    ntt_forward(d_eval, d_coeffs, coeff_n - 1, r, log2u(coeff_n), S(st));
+  });
+}
+
+int vx_stwo_ntt_evaluate_lde_batch(uint32_t* const* h_eval, const uint32_t* const* h_coeffs, uint32_t coeff_n,
+                                   uint32_t n_cols, const uint32_t* tw, uint32_t n, vx_stream_t st) {
+  return guard([&] {
+    if (n_cols == 0) return;
+    require(h_eval && h_coeffs, "stwo lde batch: null column array");
+    require(pow2(coeff_n) && coeff_n >= 2 && coeff_n <= n, "stwo lde batch: coeff_n must be a power of two in [2, n]");
+    TwiddleRef r = stwo_ref(tw, n);
+    for (uint32_t c = 0; c < n_cols; c++) {
+      require(h_eval[c] && h_coeffs[c] && h_eval[c] != h_coeffs[c], "stwo lde batch: null or aliasing column");
+      require(((reinterpret_cast<uintptr_t>(h_eval[c]) | reinterpret_cast<uintptr_t>(h_coeffs[c])) & 15u) == 0,
+              "stwo lde batch: columns must be 16-byte aligned");
+    }
+    constexpr uint32_t CHUNK = 65535;
+    const uint32_t m0 = n_cols < CHUNK ? n_cols : CHUNK;
+    DeviceBuf d(2 * sizeof(uint32_t*) * m0);
+    for (uint32_t c0 = 0; c0 < n_cols; c0 += CHUNK) {
+      const uint32_t m = n_cols - c0 < CHUNK ? n_cols - c0 : CHUNK;
+      std::vector<const void*> p(2 * (size_t)m);
+      for (uint32_t c = 0; c < m; c++) {
+        p[c] = h_eval[c0 + c];
+        p[m + c] = h_coeffs[c0 + c];
+      }
+      VX_HIP(hipMemcpyAsync(d.p, p.data(), p.size() * sizeof(void*), hipMemcpyHostToDevice, S(st)));
+      uint32_t* const* dst = static_cast<uint32_t* const*>(d.p);
+      ntt_forward_cols(dst, m, r, S(st), reinterpret_cast<const uint32_t* const*>(dst + m), log2u(coeff_n));
+      VX_HIP(hipStreamSynchronize(S(st)));
+    }
+  });
+}
+
 int vx_stwo_ntt_batch_evaluate(uint32_t* const* cols, const uint32_t* tw, uint32_t n, uint32_t n_cols,
                                vx_stream_t st) {
   return guard([&] {

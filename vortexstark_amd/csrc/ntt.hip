@@ -36,7 +36,8 @@ struct PassArgs {
   uint64_t src_mask;
   uint32_t scale;          // output multiplier (1 = none)
   uint64_t dst_stride, src_stride;  // words between the columns of a batch (blockIdx.y)
-  uint32_t* const* cols;   // NTT_COLS: device array of column pointers
+  uint32_t* const* cols;   // NTT_COLS: device array of column pointers (destinations)
+  const uint32_t* const* src_cols;  // NTT_COLS: source columns, or null to read the destinations
 };
 
 // stwo circle twiddle of circle butterfly h from line layer 0.
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(uint32_t* dst, const u
   extern __shared__ uint32_t lds[];
   if constexpr ((MODE & NTT_COLS) != 0) {
     dst = ld_uniform(a.cols + blockIdx.y);
-    src = dst;
+    src = a.src_cols ? ld_uniform(a.src_cols + blockIdx.y) : dst;
   } else {
     dst += blockIdx.y * a.dst_stride;
     src += blockIdx.y * a.src_stride;
@@ -129,7 +130,8 @@ struct RadixArgs {
   uint64_t src_mask;
   uint32_t scale;
   uint64_t dst_stride, src_stride;
-  uint32_t* const* cols;   // NTT_COLS: device array of column pointers
+  uint32_t* const* cols;   // NTT_COLS: device array of column pointers (destinations)
+  const uint32_t* const* src_cols;  // NTT_COLS: source columns, or null to read the destinations
 };
 
 __device__ __forceinline__ uint32_t rx_own(uint32_t t, uint32_t s, uint32_t pos) {
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const u
   __shared__ uint32_t lds[1u << RX_E_BITS];
   if constexpr ((MODE & NTT_COLS) != 0) {
     dst = ld_uniform(a.cols + blockIdx.y);
-    src = dst;
+    src = a.src_cols ? ld_uniform(a.src_cols + blockIdx.y) : dst;
   } else {
     dst += blockIdx.y * a.dst_stride;
     src += blockIdx.y * a.src_stride;
@@ -356,7 +358,8 @@ static const uint32_t* bit_table(const TwiddleRef& tw, uint32_t j) {
 struct Batch {
   uint32_t n_cols;
   uint64_t dst_stride, src_stride;
-  uint32_t* const* cols;  // device column pointers (in place), or null for strided columns
+  uint32_t* const* cols;  // device column pointers, or null for strided columns
+  const uint32_t* const* src_cols;  // with cols: the first pass's source columns (null: in place)
 };
 
 using RadixKernel = void (*)(uint32_t*, const uint32_t*, RadixArgs);
@@ -418,6 +421,7 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
   a.dst_stride = bt.dst_stride;
   a.src_stride = bt.src_stride;
   a.cols = bt.cols;
+  a.src_cols = bt.src_cols;
   require(bt.n_cols >= 1 && bt.n_cols <= 65535, "ntt: 1..65535 columns per batch");
   const int mode = (tw.stwo_circle ? NTT_STWO : 0) | (bt.cols ? NTT_COLS : 0);
   const uint32_t e_log = k + cbits;
@@ -427,6 +431,7 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
     r.dst_stride = bt.dst_stride;
     r.src_stride = bt.src_stride;
     r.cols = bt.cols;
+    r.src_cols = bt.src_cols;
     hipLaunchKernelGGL(radix_kernel(fwd, mode), dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0, s, dst, src,
                        r);
     VX_HIP(hipGetLastError());
@@ -460,6 +465,7 @@ static void forward_passes(uint32_t* data, const uint32_t* src, uint64_t src_mas
     cur = data;
     mask = (1ull << log_n) - 1;
     bt.src_stride = dst_stride;
+    bt.src_cols = nullptr;  // later passes work in place on the destinations
   }
   launch_pass(true, data, cur, mask, tw, 0, contig, 0, 1u, bt, s);
 }
@@ -474,12 +480,15 @@ void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const T
           "ntt_forward: source length must be a power of two covering the transformed bits");
   require(n_cols == 1 || (dst_stride >= (1ull << log_n) && src_stride >= src_mask + 1),
           "ntt_forward: batch columns must not overlap");
-  forward_passes(data, src, src_mask, tw, eff_bits, s, Batch{n_cols, dst_stride, src_stride, nullptr});
+  forward_passes(data, src, src_mask, tw, eff_bits, s, Batch{n_cols, dst_stride, src_stride, nullptr, nullptr});
 }
 
-void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s) {
+void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s,
+                      const uint32_t* const* d_src_cols, uint32_t src_log_n) {
   require(d_cols != nullptr && tw.log_n >= 1 && tw.log_n <= 31, "ntt_forward_cols: bad arguments");
-  forward_passes(nullptr, nullptr, (1ull << tw.log_n) - 1, tw, tw.log_n, s, Batch{n_cols, 0, 0, d_cols});
+  const uint32_t eff = d_src_cols ? src_log_n : tw.log_n;
+  require(eff >= 1 && eff <= tw.log_n, "ntt_forward_cols: source size must be 2..2^log_n");
+  forward_passes(nullptr, nullptr, (1ull << eff) - 1, tw, eff, s, Batch{n_cols, 0, 0, d_cols, d_src_cols});
 }
 
 static void inverse_passes(uint32_t* data, const TwiddleRef& tw, hipStream_t s, const Batch& bt) {
@@ -505,12 +514,12 @@ void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n
   require(aligned16(data) && stride % 4 == 0, "ntt: columns must be 16-byte aligned");
   require(log_n >= 1 && log_n <= 31, "ntt_inverse: bad size");
   require(n_cols == 1 || stride >= (1ull << log_n), "ntt_inverse: batch columns must not overlap");
-  inverse_passes(data, tw, s, Batch{n_cols, stride, stride, nullptr});
+  inverse_passes(data, tw, s, Batch{n_cols, stride, stride, nullptr, nullptr});
 }
 
 void ntt_inverse_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s) {
   require(d_cols != nullptr && tw.log_n >= 1 && tw.log_n <= 31, "ntt_inverse_cols: bad arguments");
-  inverse_passes(nullptr, tw, s, Batch{n_cols, 0, 0, d_cols});
+  inverse_passes(nullptr, tw, s, Batch{n_cols, 0, 0, d_cols, nullptr});
 }
 
 // ---------------------------------------------------------------------------

@@ -109,7 +109,10 @@ void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n
 // In-place transforms of n_cols columns of 2^log_n words each at arbitrary
 // (16-byte aligned) addresses: d_cols is a device array of column pointers.
 // Same passes as ntt_forward / ntt_inverse, one launch per pass for all columns.
-void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s);
+// With d_src_cols, the forward transform reads 2^src_log_n coefficients per
+// column from there, zero-padded (the LDE: the replication layers are skipped).
+void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s,
+                      const uint32_t* const* d_src_cols = nullptr, uint32_t src_log_n = 0);
 void ntt_inverse_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s);
 // Uneven batched LDE (columns of log sizes 14..24): host-built work lists
 // (lde_batch_build) uploaded by the caller, then four launches.

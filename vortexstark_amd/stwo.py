@@ -125,3 +125,61 @@ def barycentric_eval(d_evals, d_weights, n, stream=None):
     call("vx_barycentric_eval", _ptr(d_evals), _ptr(d_weights), n, _ptr(d_out), n_blocks, _stream(stream))
     parts = d_out.to_host().reshape(n_blocks, 4).astype(object)
     return tuple(int(v) % P for v in parts.sum(axis=0))
+
+
+# ---- QuotientOps, GrindOps, column and accumulation ops
+
+def _dev_ptr_array(bufs):
+    """A device array holding the device addresses of `bufs` (kept by the caller)."""
+    import numpy as np
+    return DeviceBuffer.from_host(np.array([b.as_ptr() for b in bufs], np.uint64).view(np.uint32))
+
+
+def eval_at_point_batch(columns, log_sizes, zx, zy, stream=None):
+    """f_k(z) for every column (the OODS step): one call for all columns;
+    returns a list of QM31 4-tuples."""
+    import numpy as np
+    max_log = max(log_sizes)
+    d_f = DeviceBuffer.from_host(folding_factors(zx, zy, max_log) or [0, 0, 0, 0])
+    d_res = DeviceBuffer(4 * len(columns))
+    ptrs = (C.c_void_p * len(columns))(*[b.as_ptr() for b in columns])
+    logs = (C.c_uint32 * len(columns))(*log_sizes)
+    call("vx_eval_at_point_batch", ptrs, logs, len(columns), _ptr(d_f), max_log, _ptr(d_res), _stream(stream))
+    return [tuple(int(v) for v in r) for r in d_res.to_host().reshape(-1, 4)]
+
+
+def accumulate_numerators(columns, col_indices, b, c, n_rows, stream=None):
+    """QuotientOps::accumulate_numerators (quotient_ops.rs:120-208): the 4
+    SoA output columns of sum_i (c_i * columns[col_indices[i]][row] - b_i);
+    b, c are flat QM31 word lists. Returns 4 DeviceBuffers."""
+    import numpy as np
+    d_cols = _dev_ptr_array(columns)
+    d_idx = DeviceBuffer.from_host(np.array(col_indices or [0], np.uint32))
+    d_b = DeviceBuffer.from_host(np.array(b or [0, 0, 0, 0], np.uint32))
+    d_c = DeviceBuffer.from_host(np.array(c or [0, 0, 0, 0], np.uint32))
+    out = [DeviceBuffer(n_rows) for _ in range(4)]
+    call("vx_accumulate_numerators", _ptr(d_cols), _ptr(d_idx), _ptr(d_b), _ptr(d_c), len(col_indices), n_rows,
+         *[_ptr(o) for o in out], _stream(stream))
+    call("vx_stream_synchronize", _stream(stream))  # the staging buffers above are freed on return
+    return out
+
+
+def grind(prefixed_digest_words, pow_bits, stream=None):
+    """GrindOps (grind_ops.rs:21-135): the smallest valid nonce (hi << 32) | low,
+    low < 2^20, for the 8-word prefixed digest."""
+    import numpy as np
+    d_p = DeviceBuffer.from_host(np.asarray(prefixed_digest_words, np.uint32))
+    d_r = DeviceBuffer(2)
+    call("vx_grind_pow_stwo", _ptr(d_p), pow_bits, _ptr(d_r), _stream(stream))
+    call("vx_stream_synchronize", _stream(stream))
+    r = d_r.to_host()
+    return int(r[0]) | (int(r[1]) << 32)
+
+
+def lift_and_accumulate(channels, log_sizes, stream=None):
+    """AccumulationOps::lift_and_accumulate (accumulation_ops.rs:44-85):
+    channels[c][k] is coordinate c of column k (ascending sizes); the last
+    column of every channel receives the lifted sum, in place."""
+    flat = [b.as_ptr() for ch in channels for b in ch]
+    call("vx_lift_and_accumulate", (C.c_void_p * len(flat))(*flat), (C.c_uint32 * len(log_sizes))(*log_sizes),
+         len(log_sizes), _stream(stream))
