@@ -176,7 +176,8 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
 
     reqs = serving_requests(vx, n_requests, seed)
     plain = vx.Server(n_slots=1)
-    batch = vx.Server(n_slots=slots)
+    # about 3 GB of resident provers per slot for this mix (48 GB at 16 slots)
+    batch = vx.Server(n_slots=slots, mem_budget=max(64 << 30, slots * (4 << 30)))
     classes = sorted({(r.air, r.log_n, r.blowup_bits, r.pow_bits) for r in reqs})
     warm = [vx.request(ln, air=air, blowup_bits=bb, pow_bits=pw) for (air, ln, bb, pw) in classes]
     for w in warm:

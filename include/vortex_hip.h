@@ -375,6 +375,20 @@ int vx_grind_pow_min(const uint32_t *d_prefixed_digest, uint32_t pow_bits, uint6
  * plugin and stwo's SIMD backend search); 0 when pow_bits = 0. pow_bits <= 32.
  * Async, except for pow_bits = 0. */
 int vx_grind_pow_stwo(const uint32_t *d_prefixed_digest, uint32_t pow_bits, uint64_t *d_result, vx_stream_t stream);
+/* Replaces cuda_grind_pow_m31_output (ffi.rs:1196-1204,
+ * cuda/grind_m31_output.cu:15-87): vx_grind_pow with every word of the
+ * Blake2s output reduced mod P before the trailing-zero count (stwo's
+ * Blake2sChannelGeneric<true>). The prefixed digest is the caller's, already
+ * reduced as the plugin does (grind_ops.rs:160-180). Smallest valid nonce of
+ * the batch, as vx_grind_pow. */
+int vx_grind_pow_m31_output(const uint32_t *d_prefixed_digest, uint64_t *d_result, uint32_t pow_bits,
+                            uint64_t batch_offset, uint32_t n_threads, vx_stream_t stream);
+/* The plugin's GrindOps for Blake2sChannelGeneric<true> as one call
+ * (grind_ops.rs:154-245, which loops on the host over
+ * cuda_grind_pow_m31_output batches of 2^20): vx_grind_pow_stwo's search
+ * with the M31-reduced test. 0 when pow_bits = 0. pow_bits <= 32. */
+int vx_grind_pow_stwo_m31(const uint32_t *d_prefixed_digest, uint32_t pow_bits, uint64_t *d_result,
+                          vx_stream_t stream);
 
 /* --------------------------------------------- prove / verify (src/prover.rs) */
 

@@ -77,6 +77,8 @@ def lib(omp=False):
         "or_channel_verify_pow_nonce": (C.c_int, [C.POINTER(Channel), u32, u64]),
         "or_grind_pow": (u64, [C.POINTER(Channel), u32]),
         "or_grind_pow_stwo": (u64, [C.POINTER(Channel), u32]),
+        "or_pow_tz_words": (u32, [_u32p, u64, C.c_int]),
+        "or_grind_pow_stwo_words": (u64, [_u32p, u32, C.c_int]),
         "or_accumulate_lift": (None, [_u32p, _u32p, u64, u32]),
         "or_lift_and_accumulate": (None, [C.c_void_p, _u32p, u32]),
         "or_prove": (C.c_int, [u32, u32, u32, u32, u32, u32,
@@ -332,6 +334,11 @@ class PyChannel:
     def grind_stwo(self, bits):
         return int(lib().or_grind_pow_stwo(C.byref(self.c), bits))
 
+    def pow_prefix_m31(self, bits):
+        """The Blake2sChannelGeneric<true> prefix: pow_prefix with every word
+        reduced mod P (grind_ops.rs:160-180)."""
+        return self.pow_prefix(bits) % np.uint32(P)
+
     def pow_prefix(self, bits):
         """Blake2s(0x12345678 LE || 0^12 || state || bits LE) (src/channel.rs:140-150) as 8 words."""
         msg = np.zeros(13, np.uint32)
@@ -458,6 +465,16 @@ def batch_proof_size(log_sizes, blowup_bits=2, n_queries=80):
 
 
 # ---------------- stwo-basis NTT (stwo plugin PolyOps) ----------------
+
+def pow_tz_words(prefix, nonce, m31_out=False):
+    """Trailing zeros of Blake2s(prefix || nonce)[0..16], words reduced mod P first with m31_out."""
+    return int(lib().or_pow_tz_words(u32(prefix), nonce, int(m31_out)))
+
+
+def grind_stwo_words(prefix, bits, m31_out=False):
+    """The stwo plugin's grind over a given prefixed digest (either channel flavour)."""
+    return int(lib().or_grind_pow_stwo_words(u32(prefix), bits, int(m31_out)))
+
 
 def stwo_twiddles(log_n):
     """(forward, inverse) flat twiddle buffers of n/2 words for a domain of 2^log_n points."""

@@ -635,6 +635,31 @@ uint64_t or_grind_pow_stwo(const or_channel *ch, uint32_t pow_bits) {
             if (pow_tz(prefix, (hi << 32) | low) >= pow_bits) return (hi << 32) | low;
 }
 
+/* The grind test over a caller-given 8-word prefixed digest. m31_out
+ * reduces every output word mod P before the trailing-zero count, as stwo's
+ * Blake2sChannelGeneric<true> does (grind_ops.rs:154-180; the device form
+ * cuda/grind_m31_output.cu:44-63). The reduction here is the plain x % P. */
+uint32_t or_pow_tz_words(const uint32_t prefix[8], uint64_t nonce, int m31_out) {
+    uint8_t in[40], r[32];
+    for (int i = 0; i < 8; i++) put32(in + 4 * i, prefix[i]);
+    put32(in + 32, (uint32_t)nonce);
+    put32(in + 36, (uint32_t)(nonce >> 32));
+    or_blake2s_single(in, 40, r);
+    for (int i = 0; i < 4; i++) {
+        uint32_t w = le32(r + 4 * i);
+        if (m31_out) w %= 0x7FFFFFFFu;
+        if (w) return 32u * (uint32_t)i + (uint32_t)__builtin_ctz(w);
+    }
+    return 128;
+}
+/* The plugin's search order (grind_ops.rs:71-128, :190-240) over a given
+ * prefixed digest. */
+uint64_t or_grind_pow_stwo_words(const uint32_t prefix[8], uint32_t pow_bits, int m31_out) {
+    for (uint64_t hi = 0;; hi++)
+        for (uint64_t low = 0; low < (1u << 20); low++)
+            if (or_pow_tz_words(prefix, (hi << 32) | low, m31_out) >= pow_bits) return (hi << 32) | low;
+}
+
 /* ======================================================================
  * Proof bytes: KRAK v1 (src/bin/stark_cli.rs:1051-1110). pow_bits > 0 selects
  * the KRAK v2 extension: header gains blowup_bits, pow_bits, nonce(lo,hi);

@@ -117,6 +117,10 @@ void or_accumulate_lift(uint32_t *col, const uint32_t *curr, uint64_t col_n, uin
 void or_lift_and_accumulate(uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols);
 /* the stwo plugin's nonce family (hi << 32) | low, low < 2^20 (grind_ops.rs:71-128) */
 uint64_t or_grind_pow_stwo(const or_channel *ch, uint32_t pow_bits);
+/* the same search over a caller-given prefixed digest; m31_out reduces the
+ * hash words mod P first (Blake2sChannelGeneric<true>, grind_ops.rs:154-245) */
+uint32_t or_pow_tz_words(const uint32_t prefix[8], uint64_t nonce, int m31_out);
+uint64_t or_grind_pow_stwo_words(const uint32_t prefix[8], uint32_t pow_bits, int m31_out);
 
 /* ---- prove / verify (src/prover.rs:264-619, src/verifier.rs:39-310,
  *      src/bin/stark_cli.rs:1051-1110) ---- */

@@ -159,6 +159,39 @@ def test_grind_pow_stwo_zero_bits():
     dp = dev(np.zeros(8, np.uint32))
     call("vx_grind_pow_stwo", ptr(dp), 0, ptr(res), NULL)
     assert host(res).tolist() == [0, 0]
+    res = dev(np.array([7, 7], np.uint32))
+    call("vx_grind_pow_stwo_m31", ptr(dp), 0, ptr(res), NULL)
+    assert host(res).tolist() == [0, 0]
+
+
+# GrindOps for Blake2sChannelGeneric<true> (grind_ops.rs:154-245,
+# cuda/grind_m31_output.cu): hash words reduced mod P before the count
+@pytest.mark.parametrize("seed,bits", [(0, 1), (0, 8), (3, 16), (0, 21), (1, 21), (4, 22)])
+def test_grind_pow_stwo_m31(seed, bits):
+    print(f"grind stwo m31 seed={seed} bits={bits}", flush=True)
+    ch = po.PyChannel()
+    ch.mix_digest([seed, 9, 2, 3, 4, 5, 6, 7])
+    pre = ch.pow_prefix_m31(bits)
+    dp, res = dev(pre), empty(2)
+    call("vx_grind_pow_stwo_m31", ptr(dp), bits, ptr(res), NULL)
+    sync()
+    got = host(res)
+    nonce = int(got[0]) | (int(got[1]) << 32)
+    assert nonce == po.grind_stwo_words(pre, bits, True)
+    assert po.pow_tz_words(pre, nonce, True) >= bits
+    # the plugin's loop over cuda_grind_pow_m31_output batches of 2^20
+    for hi in range((nonce >> 32) + 1):
+        r = dev(np.array([0xFFFFFFFF, 0xFFFFFFFF], np.uint32))
+        call("vx_grind_pow_m31_output", ptr(dp), ptr(r), bits, hi << 32, 1 << 20, NULL)
+        sync()
+        g = host(r)
+        want = nonce if hi == nonce >> 32 else (1 << 64) - 1
+        assert int(g[0]) | (int(g[1]) << 32) == want
+    # the same prefix under the raw test finds the raw flavour's nonce
+    call("vx_grind_pow_stwo", ptr(dp), bits, ptr(res), NULL)
+    sync()
+    got = host(res)
+    assert int(got[0]) | (int(got[1]) << 32) == po.grind_stwo_words(pre, bits, False)
 
 
 # ---- column ops (column.rs, accumulation_ops.rs)
@@ -260,6 +293,7 @@ def test_python_mirror_of_the_plugin_ops():
     ch = po.PyChannel()
     ch.mix_digest([7, 7, 7, 7, 7, 7, 7, 7])
     assert stwo.grind(ch.pow_prefix(12), 12) == ch.grind_stwo(12)
+    assert stwo.grind(ch.pow_prefix_m31(12), 12, m31_output=True) == po.grind_stwo_words(ch.pow_prefix_m31(12), 12, True)
     chans = [[DeviceBuffer.from_host(rnd(1 << L, 300 + 10 * k + L)) for L in (3, 6)] for k in range(4)]
     host_chans = [[b_.to_host() for b_ in ch_] for ch_ in chans]
     stwo.lift_and_accumulate(chans, [3, 6])

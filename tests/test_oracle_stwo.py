@@ -282,6 +282,42 @@ def test_grind_stwo_family():
     assert not any(_pow_ok(pre, 21, (1 << 32) | low) for low in range(n & 0xFFFFFFFF))
 
 
+def _tz_hashlib(prefix_words, nonce, m31_out):
+    import hashlib
+    h = hashlib.blake2s(np.asarray(prefix_words, np.uint32).tobytes() + nonce.to_bytes(8, "little")).digest()
+    words = [int.from_bytes(h[4 * i:4 * i + 4], "little") for i in range(4)]
+    if m31_out:
+        words = [w % P for w in words]
+    v = sum(w << (32 * i) for i, w in enumerate(words))
+    return 128 if v == 0 else (v & -v).bit_length() - 1
+
+
+def test_grind_m31_output_oracle():
+    # Blake2sChannelGeneric<true> (grind_ops.rs:154-245): hash words reduced
+    # mod P before the count; hashlib computes the same test independently
+    rng = np.random.default_rng(64)
+    differs = 0
+    for _ in range(4):
+        pre = rng.integers(0, P, 8, dtype=np.uint32)
+        for n in list(range(200)) + [(1 << 32) | 5, (7 << 32) | 0xFFFFF]:
+            a, b = po.pow_tz_words(pre, n, True), po.pow_tz_words(pre, n, False)
+            assert a == _tz_hashlib(pre, n, True) and b == _tz_hashlib(pre, n, False)
+            differs += a != b
+    assert differs > 100  # the reduction changes the count whenever word 0 >= P
+    # the search: valid, and every earlier nonce of hi = 0 fails
+    for bits in (6, 11):
+        pre = rng.integers(0, P, 8, dtype=np.uint32)
+        n = po.grind_stwo_words(pre, bits, True)
+        assert n < (1 << 20) and _tz_hashlib(pre, n, True) >= bits
+        assert all(_tz_hashlib(pre, k, True) < bits for k in range(n))
+    # the raw flavour over a channel's prefix is the channel-based search
+    ch = po.PyChannel()
+    ch.mix_digest([5, 4, 3, 2, 1, 0, 9, 8])
+    assert po.grind_stwo_words(ch.pow_prefix(13), 13, False) == ch.grind_stwo(13)
+    pm = ch.pow_prefix_m31(13)
+    assert pm.dtype == np.uint32 and np.all(pm < P)
+
+
 # ---- column ops: lifted accumulation (accumulate_lift.cu, accumulation_ops.rs)
 
 def test_accumulate_lift_closed_form():

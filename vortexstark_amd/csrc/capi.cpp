@@ -678,21 +678,36 @@ int vx_grind_pow(const uint32_t* prefix, uint64_t* result, uint32_t pow_bits, ui
                  uint32_t n_threads, vx_stream_t st) {
   return guard([&] {
     require(prefix && result && n_threads > 0, "grind_pow: bad arguments");
-    grind_pow_batch(prefix, result, pow_bits, batch_offset, n_threads, S(st));
+    grind_pow_batch(prefix, result, pow_bits, batch_offset, n_threads, false, S(st));
   });
 }
 
-int vx_grind_pow_stwo(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
+int vx_grind_pow_m31_output(const uint32_t* prefix, uint64_t* result, uint32_t pow_bits, uint64_t batch_offset,
+                            uint32_t n_threads, vx_stream_t st) {
   return guard([&] {
-    require(prefix && result, "grind_pow_stwo: null pointer");
-    if (pow_bits == 0) {
-      const uint64_t zero = 0;
-      VX_HIP(hipMemcpyAsync(result, &zero, 8, hipMemcpyHostToDevice, S(st)));
-      VX_HIP(hipStreamSynchronize(S(st)));
-      return;
-    }
-    grind_pow_stwo(prefix, pow_bits, result, S(st));
+    require(prefix && result && n_threads > 0, "grind_pow_m31_output: bad arguments");
+    grind_pow_batch(prefix, result, pow_bits, batch_offset, n_threads, true, S(st));
   });
+}
+
+static void grind_stwo_call(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, bool m31_out,
+                            hipStream_t s) {
+  require(prefix && result, "grind_pow_stwo: null pointer");
+  if (pow_bits == 0) {
+    const uint64_t zero = 0;
+    VX_HIP(hipMemcpyAsync(result, &zero, 8, hipMemcpyHostToDevice, s));
+    VX_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  grind_pow_stwo(prefix, pow_bits, result, m31_out, s);
+}
+
+int vx_grind_pow_stwo(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
+  return guard([&] { grind_stwo_call(prefix, pow_bits, result, false, S(st)); });
+}
+
+int vx_grind_pow_stwo_m31(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
+  return guard([&] { grind_stwo_call(prefix, pow_bits, result, true, S(st)); });
 }
 
 int vx_grind_pow_min(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {

@@ -105,7 +105,7 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 // map that keeps the order, so the smallest index is the smallest nonce.
 __device__ __forceinline__ uint64_t stwo_nonce(uint64_t i) { return ((i >> 20) << 32) | (i & 0xFFFFFull); }
 
-template <bool STWO>
+template <bool STWO, bool M31_OUT = false>
 __global__ __launch_bounds__(256) void k_grind(const uint32_t* prefix_dev, uint32_t bits,
                                               unsigned long long* best, uint64_t start, uint64_t count) {
   uint32_t prefix[8];
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_grind(const uint32_t* prefix_dev, uint3
     if (base > cur_best) break;
     const uint64_t n = base + threadIdx.x;
     uint64_t hit = ~0ull;
-    if (n < end && pow_trailing_zeros(prefix, STWO ? stwo_nonce(n) : n) >= bits) hit = n;
+    if (n < end && pow_trailing_zeros<M31_OUT>(prefix, STWO ? stwo_nonce(n) : n) >= bits) hit = n;
     hit = wave_min_u64(hit);
     if ((threadIdx.x & 63) == 0 && hit != ~0ull) atomicMin(best, (unsigned long long)hit);
   }
@@ -137,11 +137,18 @@ __global__ void k_stwo_index_to_nonce(unsigned long long* p) {
 // Whole search over the stwo family: the same increasing-index scan, then
 // the winning index is turned into its nonce. Indices up to 2^44 cover
 // hi < 2^24.
-void grind_pow_stwo(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, hipStream_t s) {
+// m31_out selects the Blake2sChannelGeneric<true> test (output words
+// reduced mod P before the trailing-zero count).
+void grind_pow_stwo(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, bool m31_out, hipStream_t s) {
   require(pow_bits >= 1 && pow_bits <= 32, "grind_pow_stwo: pow_bits must be 1..32");
   auto* best = reinterpret_cast<unsigned long long*>(d_result);
   hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, best, ~0ull);
-  hipLaunchKernelGGL(k_grind<true>, dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull, GRIND_LIMIT);
+  if (m31_out)
+    hipLaunchKernelGGL((k_grind<true, true>), dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull,
+                       GRIND_LIMIT);
+  else
+    hipLaunchKernelGGL((k_grind<true, false>), dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull,
+                       GRIND_LIMIT);
   hipLaunchKernelGGL(k_stwo_index_to_nonce, dim3(1), dim3(64), 0, s, best);
   VX_HIP(hipGetLastError());
 }
@@ -158,12 +165,17 @@ void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, 
 // Reference-style batch (cuda/grind.cu:94-110): test nonces
 // [batch_offset, batch_offset + n) and atomicMin hits into *d_result.
 void grind_pow_batch(const uint32_t* d_prefix, uint64_t* d_result, uint32_t pow_bits, uint64_t batch_offset,
-                     uint64_t n, hipStream_t s) {
+                     uint64_t n, bool m31_out, hipStream_t s) {
   require(pow_bits <= 128 && n > 0, "grind_pow_batch: bad arguments");
   uint64_t blocks = (n + 255) / 256;
   uint32_t grid = (uint32_t)(blocks < 4096 ? blocks : 4096);
-  hipLaunchKernelGGL(k_grind<false>, dim3(grid), dim3(256), 0, s, d_prefix, pow_bits,
-                     reinterpret_cast<unsigned long long*>(d_result), batch_offset, n);
+  auto* best = reinterpret_cast<unsigned long long*>(d_result);
+  if (m31_out)
+    hipLaunchKernelGGL((k_grind<false, true>), dim3(grid), dim3(256), 0, s, d_prefix, pow_bits, best, batch_offset,
+                       n);
+  else
+    hipLaunchKernelGGL((k_grind<false, false>), dim3(grid), dim3(256), 0, s, d_prefix, pow_bits, best, batch_offset,
+                       n);
   VX_HIP(hipGetLastError());
 }
 

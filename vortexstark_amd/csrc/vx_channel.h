@@ -101,6 +101,10 @@ VX_HD void ch_pow_prefix(const Channel& c, uint32_t pow_bits, uint32_t out[8]) {
 }
 
 // Trailing zeros of the first 16 bytes (u128 LE) of Blake2s(prefix || nonce).
+// With M31_OUT every output word is first reduced mod P, as stwo's
+// Blake2sChannelGeneric<true> hashes (cuda/grind_m31_output.cu:44-63): a
+// 32-bit word x becomes (x & P) + (x >> 31), less P when that reaches P.
+template <bool M31_OUT = false>
 VX_HD uint32_t pow_trailing_zeros(const uint32_t prefix[8], uint64_t nonce) {
   uint32_t m[16] = {0};
   for (int i = 0; i < 8; i++) m[i] = prefix[i];
@@ -109,8 +113,14 @@ VX_HD uint32_t pow_trailing_zeros(const uint32_t prefix[8], uint64_t nonce) {
   uint32_t h[8];
   b2_init(h);
   b2_compress(h, m, 40u, true);
-  for (int i = 0; i < 4; i++)
-    if (h[i]) return 32u * i + (uint32_t)__builtin_ctz(h[i]);
+  for (int i = 0; i < 4; i++) {
+    uint32_t w = h[i];
+    if (M31_OUT) {
+      w = (w & P) + (w >> 31);
+      w = w >= P ? w - P : w;
+    }
+    if (w) return 32u * i + (uint32_t)__builtin_ctz(w);
+  }
   return 128u;
 }
 

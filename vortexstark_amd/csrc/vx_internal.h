@@ -380,10 +380,12 @@ void channel_draw_queries(const QueryArgs& q, hipStream_t s);
 // Deterministic PoW: smallest nonce >= 0 (linear order) whose Blake2s has
 // >= pow_bits trailing zeros (src/channel.rs:140-168).
 void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, hipStream_t s);
-// The same search over the stwo plugin's nonces (hi << 32) | low, low < 2^20.
-void grind_pow_stwo(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, hipStream_t s);
+// The same search over the stwo plugin's nonces (hi << 32) | low, low < 2^20;
+// m31_out reduces the hash words mod P before the count (stwo's
+// Blake2sChannelGeneric<true>).
+void grind_pow_stwo(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, bool m31_out, hipStream_t s);
 void grind_pow_batch(const uint32_t* d_prefix, uint64_t* d_result, uint32_t pow_bits,
-                     uint64_t batch_offset, uint64_t n_nonces, hipStream_t s);
+                     uint64_t batch_offset, uint64_t n_nonces, bool m31_out, hipStream_t s);
 void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_t s);
 // Copy `words` words from device memory into pinned host memory
 // (hipHostMalloc) with a kernel; both 16-byte aligned.

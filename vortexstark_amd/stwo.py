@@ -164,13 +164,17 @@ def accumulate_numerators(columns, col_indices, b, c, n_rows, stream=None):
     return out
 
 
-def grind(prefixed_digest_words, pow_bits, stream=None):
+def grind(prefixed_digest_words, pow_bits, stream=None, m31_output=False):
     """GrindOps (grind_ops.rs:21-135): the smallest valid nonce (hi << 32) | low,
-    low < 2^20, for the 8-word prefixed digest."""
+    low < 2^20, for the 8-word prefixed digest. m31_output selects the
+    Blake2sChannelGeneric<true> flavour (grind_ops.rs:154-245): the hash words
+    are reduced mod P before the trailing-zero count, and the caller passes
+    the prefixed digest already reduced the same way."""
     import numpy as np
     d_p = DeviceBuffer.from_host(np.asarray(prefixed_digest_words, np.uint32))
     d_r = DeviceBuffer(2)
-    call("vx_grind_pow_stwo", _ptr(d_p), pow_bits, _ptr(d_r), _stream(stream))
+    call("vx_grind_pow_stwo_m31" if m31_output else "vx_grind_pow_stwo", _ptr(d_p), pow_bits, _ptr(d_r),
+         _stream(stream))
     call("vx_stream_synchronize", _stream(stream))
     r = d_r.to_host()
     return int(r[0]) | (int(r[1]) << 32)
