@@ -217,6 +217,31 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
     }
 
 
+def serving_mix_profile(vx):
+    """Every class of the serving mix proved eagerly (no graph, so each
+    dispatch is visible to a kernel trace), built-in tree shapes: 7 proofs per
+    Fibonacci class (log_n 14..22) and 9 per synthetic class (log_n 14..20),
+    the mix's 1:1 weighting of the two AIRs over 9 and 7 classes; PoW 20."""
+    vx.tuner(False)
+    t0 = time.perf_counter()
+    n = 0
+    for ln in range(14, 23):
+        p = vx.Prover(ln, pow_bits=20, graphs=False)
+        for k in range(7):
+            p.prove(1, 1000 + k)
+        n += 7
+        p.close()
+        print(f"fibonacci log_n={ln} done", file=sys.stderr, flush=True)
+    for ln in range(14, 21):
+        p = vx.D2Prover(ln, pow_bits=20, graphs=False)
+        for k in range(9):
+            p.prove(77 + k)
+        n += 9
+        p.close()
+        print(f"synthetic log_n={ln} done", file=sys.stderr, flush=True)
+    return {"proofs": n, "wall_s": round(time.perf_counter() - t0, 3)}
+
+
 def batched(vx, steps, n_cols=96, seed=42, blowup_bits=2):
     """The batched uneven-commit workload: n_cols columns with log sizes drawn
     from {14..22} by the seed, values from the synthetic generator; one step =
@@ -534,6 +559,9 @@ def main():
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-stwo", action="store_true")
     ap.add_argument("--stwo-only", action="store_true")
+    ap.add_argument("--profile-serving-mix", action="store_true",
+                    help="eager proofs of every serving class (7 per Fibonacci class, 9 per synthetic class) "
+                         "for a kernel-trace profile of the mix")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: one 1 GiB device copy (counter calibration), "
                          "1 warmup + 2 timed proofs, no CPU baseline")
@@ -548,6 +576,9 @@ def main():
     import vortexstark_amd as vx
 
     vx.load()
+    if args.profile_serving_mix:
+        print(json.dumps({"serving_mix_profile": serving_mix_profile(vx)}), flush=True)
+        return
     if args.serve_only:
         print(json.dumps({"serving": serving(vx, args.serve_requests, args.slots)}), flush=True)
         return
