@@ -206,6 +206,19 @@ __device__ __forceinline__ void rx_apply(uint32_t (&v)[32], const uint32_t (&w)[
   }
 }
 
+// Pass arguments with the round layout fixed at compile time (defined
+// below); their tiles load a round's twiddles before its butterflies.
+template <bool FWD, int SHAPE>
+struct FixedArgs;
+template <typename A>
+struct rx_is_fixed {
+  static constexpr bool value = false;
+};
+template <bool F, int S>
+struct rx_is_fixed<FixedArgs<F, S>> {
+  static constexpr bool value = true;
+};
+
 // One 16384-element tile of a pass; `a` may live in kernel arguments or in
 // global memory (work-list jobs), `lds` holds 2^14 words.
 template <bool FWD, int MODE, typename Args>
@@ -255,11 +268,14 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
       for (uint32_t s = 0; s < 32; s++) v[s] = lds[lb ^ rx_swz(s << pos)];
     }
     const uint32_t qlo = r_rb0 + a.cbits - pos, qhi = qlo + r_nb;  // register bits to butterfly
-    // Each bit's twiddles are loaded just before its butterflies. (Loading
-    // the whole round's 31 words up front overlaps their latencies within a
-    // workgroup but costs 26 more VGPRs: 154 per lane fits one 512-lane
-    // workgroup per CU, 128 fits the two that LDS allows, and the second
-    // workgroup hides the latency better: the LDE ran 0.60 -> 0.41 ms.)
+    // The generic kernel loads each bit's twiddles just before its
+    // butterflies. (Loading the whole round's 31 words up front overlaps
+    // their latencies within a workgroup but costs 26 more VGPRs: 154 per
+    // lane fits one 512-lane workgroup per CU, 128 fits the two that LDS
+    // allows, and the second workgroup hides the latency better: the LDE ran
+    // 0.60 -> 0.41 ms.) The fixed-shape kernels need about 100 VGPRs, so
+    // they load the round's twiddles up front and still fit two workgroups.
+    constexpr bool PRELOAD = rx_is_fixed<Args>::value;
     uint32_t w4[1], w3[2], w2[4], w1[8], w0[16];
 #define VX_RX_LOAD(Q, W)                                                                          \
   if ((Q) >= qlo && (Q) < qhi) {                                                                  \
@@ -269,7 +285,13 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   }
 #define VX_RX_APPLY(Q, W) \
   if ((Q) >= qlo && (Q) < qhi) rx_apply<FWD, (Q)>(v, W);
-    if (FWD) {
+    if (PRELOAD && FWD) {
+      VX_RX_LOAD(4, w4) VX_RX_LOAD(3, w3) VX_RX_LOAD(2, w2) VX_RX_LOAD(1, w1) VX_RX_LOAD(0, w0)
+      VX_RX_APPLY(4, w4) VX_RX_APPLY(3, w3) VX_RX_APPLY(2, w2) VX_RX_APPLY(1, w1) VX_RX_APPLY(0, w0)
+    } else if (PRELOAD) {
+      VX_RX_LOAD(0, w0) VX_RX_LOAD(1, w1) VX_RX_LOAD(2, w2) VX_RX_LOAD(3, w3) VX_RX_LOAD(4, w4)
+      VX_RX_APPLY(0, w0) VX_RX_APPLY(1, w1) VX_RX_APPLY(2, w2) VX_RX_APPLY(3, w3) VX_RX_APPLY(4, w4)
+    } else if (FWD) {
       VX_RX_LOAD(4, w4) VX_RX_APPLY(4, w4) VX_RX_LOAD(3, w3) VX_RX_APPLY(3, w3) VX_RX_LOAD(2, w2)
       VX_RX_APPLY(2, w2) VX_RX_LOAD(1, w1) VX_RX_APPLY(1, w1) VX_RX_LOAD(0, w0) VX_RX_APPLY(0, w0)
     } else {
@@ -325,11 +347,11 @@ __device__ __forceinline__ uint64_t xcd_tile(uint32_t b, uint32_t grid) {
 // A/B against the generic kernel on one box: 2^24 stwo evaluate 0.125 ->
 // 0.111 ms, Fibonacci LDE (2^24 -> 2^26) 0.412 -> 0.346 ms, the batched
 // 96-column LDE 2.23 -> 2.02 ms. The work-list kernel uses the contiguous
-// shape for its first and last stages (every column's 14-bit pass).
+// shape for its first and last stages (every column's 14-bit pass). Loading
+// each round's twiddles up front (radix_tile) then took the Fibonacci LDE
+// 0.350 -> 0.338 ms.
 enum : int { RX_GENERIC = 0, RX_CONTIG = 1, RX_STRIDED = 2 };
 
-template <bool FWD, int SHAPE>
-struct FixedArgs;
 template <bool FWD>
 struct FixedArgs<FWD, RX_CONTIG> {
   static constexpr uint32_t k = 14, cbits = 0, j0 = 0, n_rounds = 3;
