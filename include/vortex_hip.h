@@ -227,7 +227,9 @@ int vx_accumulate_numerators_dual(const uint32_t *const *d_col_ptrs, const uint3
  *   lifted(row) = ((row >> (lifting_log_size - acc_log_sizes[j] + 1)) << 1) + (row & 1),
  * each CM31 half of the QM31 numerator divided by den_j (zero inverts to
  * zero). Sample points, accumulators, pointer arrays and log sizes are
- * DEVICE arrays; n_rows <= 2^lifting_log_size. Async. */
+ * DEVICE arrays; n_rows <= 2^lifting_log_size. lifted(row) is clamped to
+ * the accumulator's last row, so an accumulator of log size 0, or one larger
+ * than the lifting domain, is read in bounds. Async. */
 int vx_compute_quotients_combine(const uint32_t *d_sample_points_x, const uint32_t *d_sample_points_y,
                                  const uint32_t *d_first_linear_acc, const uint32_t *const *d_numer_ptrs0,
                                  const uint32_t *const *d_numer_ptrs1, const uint32_t *const *d_numer_ptrs2,

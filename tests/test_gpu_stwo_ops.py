@@ -117,6 +117,21 @@ def test_combine_zero_denominators_and_partial_rows():
         assert np.array_equal(got[k], want[k]), k
 
 
+def test_combine_single_row_accumulator():
+    # log size 0 lies outside the lifting formula; the kernel clamps the
+    # index to the one row, which reads like a 2-row accumulator [v, v]
+    L, n = 11, 1 << 11
+    rng = np.random.default_rng(5)
+    sx, sy, fla = (rng.integers(0, P, 8, dtype=np.uint32) for _ in range(3))
+    xs, ys = rnd(n, 12), rnd(n, 13)
+    big = [rnd(1 << L, 40 + k) for k in range(4)]
+    one = [rnd(1, 50 + k) for k in range(4)]
+    got = run_combine(sx, sy, fla, [big, one], [L, 0], xs, ys, L)
+    want = po.quotients_combine(sx, sy, fla, [big, [np.repeat(c, 2) for c in one]], [L, 1], xs, ys, L)
+    for k in range(4):
+        assert np.array_equal(got[k], want[k]), k
+
+
 def test_quotient_ops_reject_bad_arguments():
     pin = Pin()
     small, rows = empty(8), empty(64)

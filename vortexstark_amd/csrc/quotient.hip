@@ -176,9 +176,13 @@ __global__ __launch_bounds__(QC_THREADS) void k_quotients_combine(QcArgs a) {
     const qm31 py = ld_uniform(reinterpret_cast<const qm31*>(a.sp_y) + j);
     const qm31 al = ld_uniform(reinterpret_cast<const qm31*>(a.fla) + j);
     // an accumulator larger than the lifting domain (a caller error) reads
-    // its first rows instead of out of bounds
+    // its first rows instead of out of bounds; so does one of a single row
+    // (log size 0, which the lifting formula does not cover): every index is
+    // clamped to the accumulator's last row
     const uint32_t alog = ld_uniform(a.acc_log_sizes + j);
     const uint32_t lr = alog <= a.lifting_log_size ? a.lifting_log_size - alog : 0u;
+    const uint32_t sh = lr < 31u ? lr + 1u : 31u;
+    const uint32_t last = alog < 31u ? (1u << alog) - 1u : 0x7FFFFFFFu;
     const uint32_t* nu[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) nu[k] = ld_uniform(a.numer[k] + j);
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(QC_THREADS) void k_quotients_combine(QcArgs a) {
       const uint32_t row = base + r * QC_THREADS;
       if (row >= a.n_rows) continue;
       const cm31 di{m31_mul(den[r].a, inv_n), m31_mul(m31_neg(den[r].b), inv_n)};  // conj / norm
-      const uint32_t li = ((row >> (lr + 1)) << 1) + (row & 1u);
+      const uint32_t li = min(((row >> sh) << 1) + (row & 1u), last);
       qm31 full;
 #pragma unroll
       for (int k = 0; k < 4; k++) full.v[k] = m31_sub(nu[k][li], m31_mul(al.v[k], dy[r]));
