@@ -479,6 +479,20 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     singles = [stwo.eval_at_point(b, 1 << ol, zx, zy) for b in ocols]
     oods_single_ms = (time.perf_counter() - t0) * 1000.0
     oods_same = all(tuple(int(v) for v in batch_vals[i]) == singles[i] for i in range(oc))
+    # half-coset natural -> canonic bit-reversed order of one 2^log_n column
+    # (asynchronous calls back to back): reads and writes the column once
+    d_perm = DeviceBuffer(n)
+
+    def perm_call():
+        call("vx_permute_hc_to_canonic_brt", ptr, vp(d_perm.as_ptr()), n, log_n, None)
+
+    perm_call()
+    call("vx_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        perm_call()
+    call("vx_device_sync")
+    perm_ms = (time.perf_counter() - t0) * 1000.0 / steps
     out = {
         "workload": f"stwo-basis evaluate / interpolate, one 2^{log_n} column in HBM",
         "evaluate_ms": round(res["evaluate"], 4),
@@ -505,6 +519,9 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
                                 "same_as_single_calls": oods_same},
         "quotients_combine": {"rows": qr, "accumulators": 2, "ms_per_call": round(comb_ms, 4),
                               "hbm_gbs": round(4 * qr * (2 + 8 + 4) / (comb_ms * 1e-3) / 1e9, 1)},
+        "permute_hc_to_canonic_brt": {"log_n": log_n, "ms_per_call": round(perm_ms, 4),
+                                      "hbm_gbs": round(8 * n / (perm_ms * 1e-3) / 1e9, 1),
+                                      "hbm_frac": round(8 * n / (perm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

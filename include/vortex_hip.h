@@ -270,6 +270,17 @@ int vx_gather_u256(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst
 int vx_gather_u256_checked(const uint32_t *d_src, const uint32_t *d_idx, uint32_t *d_dst, uint32_t n,
                            uint32_t src_len, vx_stream_t stream);
 
+/* Replace cuda_permute_hc_to_canonic_brt and
+ * cuda_permute_canonic_brt_to_hc_natural (ffi.rs:317-330, cuda/permute.cu:
+ * 30-57; Coset::permute_hc_natural_to_canonic_brt and its inverse,
+ * src/circle.rs:192-220): for k < n = 2^log_n, with
+ * cn = k / 2 (k even) or n - 1 - k / 2 (k odd) and j = bit-reverse(cn),
+ * the first sets dst[j] = src[k], the second dst[k] = src[j]. Out of place
+ * (src != dst). log_n <= 31. Async. */
+int vx_permute_hc_to_canonic_brt(const uint32_t *d_src, uint32_t *d_dst, uint32_t n, uint32_t log_n,
+                                 vx_stream_t stream);
+int vx_permute_canonic_brt_to_hc_natural(const uint32_t *d_src, uint32_t *d_dst, uint32_t n, uint32_t log_n,
+                                         vx_stream_t stream);
 /* Replaces cuda_pack_leaves (ffi.rs:1187-1195, cuda/pack_leaves.cu:13-35; the
  * plugin's PackLeavesOps::pack_leaves_input, merkle_ops.rs:197-228): 4 M31
  * columns of N words (N a multiple of 16) into 64 columns of N / 16 words,

@@ -77,6 +77,8 @@ def lib(omp=False):
         "or_channel_verify_pow_nonce": (C.c_int, [C.POINTER(Channel), u32, u64]),
         "or_grind_pow": (u64, [C.POINTER(Channel), u32]),
         "or_grind_pow_stwo": (u64, [C.POINTER(Channel), u32]),
+        "or_permute_hc_to_canonic_brt": (None, [_u32p, _u32p, u32]),
+        "or_permute_canonic_brt_to_hc_natural": (None, [_u32p, _u32p, u32]),
         "or_pow_tz_words": (u32, [_u32p, u64, C.c_int]),
         "or_grind_pow_stwo_words": (u64, [_u32p, u32, C.c_int]),
         "or_accumulate_lift": (None, [_u32p, _u32p, u64, u32]),
@@ -466,6 +468,22 @@ def batch_proof_size(log_sizes, blowup_bits=2, n_queries=80):
 
 # ---------------- stwo-basis NTT (stwo plugin PolyOps) ----------------
 
+def permute_hc_to_canonic_brt(src):
+    """Values in half_coset(log_n) natural order -> canonic bit-reversed order (src/circle.rs:192-204)."""
+    s = u32(src)
+    out = np.zeros_like(s)
+    lib().or_permute_hc_to_canonic_brt(s, out, _log2_any(s.size))
+    return out
+
+
+def permute_canonic_brt_to_hc_natural(src):
+    """The inverse (src/circle.rs:206-220)."""
+    s = u32(src)
+    out = np.zeros_like(s)
+    lib().or_permute_canonic_brt_to_hc_natural(s, out, _log2_any(s.size))
+    return out
+
+
 def pow_tz_words(prefix, nonce, m31_out=False):
     """Trailing zeros of Blake2s(prefix || nonce)[0..16], words reduced mod P first with m31_out."""
     return int(lib().or_pow_tz_words(u32(prefix), nonce, int(m31_out)))
@@ -483,6 +501,12 @@ def stwo_twiddles(log_n):
     inv = np.zeros_like(fwd)
     lib().or_stwo_twiddles(log_n, fwd, inv)
     return fwd, inv
+
+
+def _log2_any(n):
+    log_n = n.bit_length() - 1
+    assert n == 1 << log_n
+    return log_n
 
 
 def _log2(n):

@@ -402,6 +402,24 @@ void or_bit_reverse_m31(uint32_t *d, uint32_t log_n) {
     }
 }
 
+/* half-coset natural order -> canonic bit-reversed order and back
+ * (src/circle.rs:192-220, cuda/permute.cu:30-57): natural index k of
+ * half_coset(log_n) is canonic index cn = k / 2 (k even) or n - 1 - k / 2
+ * (k odd), stored at bit-reverse(cn). */
+static uint64_t hc_to_brt(uint64_t k, uint32_t log_n) {
+    uint64_t n = 1ull << log_n;
+    uint64_t cn = (k & 1) ? n - 1 - k / 2 : k / 2;
+    return bitrev(cn, log_n);
+}
+void or_permute_hc_to_canonic_brt(const uint32_t *src, uint32_t *dst, uint32_t log_n) {
+    uint64_t n = 1ull << log_n;
+    for (uint64_t k = 0; k < n; k++) dst[hc_to_brt(k, log_n)] = src[k];
+}
+void or_permute_canonic_brt_to_hc_natural(const uint32_t *src, uint32_t *dst, uint32_t log_n) {
+    uint64_t n = 1ull << log_n;
+    for (uint64_t k = 0; k < n; k++) dst[k] = src[hc_to_brt(k, log_n)];
+}
+
 /* ======================================================================
  * AIR: src/air.rs:30-45 ; constraint: cuda/constraint.cu:10-36, 81-90
  * ====================================================================== */

@@ -317,3 +317,51 @@ def test_python_mirror_of_the_plugin_ops():
     zx, zy = (1, 2, 3, 4), (5, 6, 7, 8)
     vals = stwo.eval_at_point_batch(dcols, [12, 12, 12], zx, zy)
     assert vals == [po.stwo_eval_at_point(x, zx, zy) for x in cols]
+
+
+# ---- half-coset natural <-> canonic bit-reversed order (cuda/permute.cu,
+# src/circle.rs:192-220): scatter path up to log_n 10, tiles from 11
+
+@pytest.mark.parametrize("log_n", [0, 1, 2, 5, 10, 11, 12, 13, 16, 20, 24])
+def test_permute_hc_brt(log_n):
+    print(f"permute log_n={log_n}", flush=True)
+    n = 1 << log_n
+    src = rnd(n, 90 + log_n)
+    ds, dd, dk = dev(src), empty(n), empty(n)
+    call("vx_permute_hc_to_canonic_brt", ptr(ds), ptr(dd), n, log_n, NULL)
+    call("vx_permute_canonic_brt_to_hc_natural", ptr(dd), ptr(dk), n, log_n, NULL)
+    sync()
+    got = host(dd)
+    assert np.array_equal(got, po.permute_hc_to_canonic_brt(src))
+    assert np.array_equal(host(dk), src)
+
+
+def test_permute_rejects_bad_arguments():
+    d1, d2 = empty(1 << 8), empty(1 << 8)
+    with pytest.raises(VortexError):  # n not 2^log_n
+        call("vx_permute_hc_to_canonic_brt", ptr(d1), ptr(d2), 200, 8, NULL)
+    with pytest.raises(VortexError):  # in place
+        call("vx_permute_canonic_brt_to_hc_natural", ptr(d1), ptr(d1), 256, 8, NULL)
+
+
+def test_permute_then_stwo_interpolate():
+    # a polynomial's values at the half-coset points, natural order, through
+    # the permutation and the stwo inverse NTT give back its coefficients
+    from vortexstark_amd import stwo
+    from vortexstark_amd.device import DeviceBuffer
+    log_n = 12
+    coeffs = rnd(1 << log_n, 97)
+    cache = stwo.StwoNttCache(log_n)
+    d = DeviceBuffer.from_host(coeffs)
+    stwo.evaluate_stwo(d, cache)                         # canonic bit-reversed values
+    hc = stwo.permute_canonic_brt_to_hc_natural(d, log_n)  # half-coset natural order
+    back = stwo.permute_hc_to_canonic_brt(hc, log_n)
+    stwo.interpolate_stwo(back, cache)
+    vx._lib.call("vx_device_sync")
+    assert np.array_equal(back.to_host(), coeffs)
+    # the natural-order values are f at half_coset(log_n).at(k)
+    hcs = po.coset(po.HALF_COSET, log_n)
+    vals = hc.to_host()
+    for k in (0, 1, 2, 3, 1000, 4095):
+        x, y = po.coset_at(hcs, k)
+        assert po.stwo_eval_at_point(coeffs, (x, 0, 0, 0), (y, 0, 0, 0)) == (int(vals[k]), 0, 0, 0), k

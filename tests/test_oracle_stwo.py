@@ -344,3 +344,51 @@ def test_lift_and_accumulate_composes():
         idx = np.array([((i >> (L - l + 1)) << 1) | (i & 1) for i in range(1 << L)])
         want = (want + c[idx]) % P
     assert np.array_equal(last, want)
+
+
+# ---- half-coset natural <-> canonic bit-reversed order (src/circle.rs:192-220)
+
+def _brev(x, bits):
+    return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def test_permute_hc_brt_literal_and_geometry():
+    for L in range(0, 11):
+        n = 1 << L
+        src = np.random.default_rng(70 + L).integers(0, P, n, dtype=np.uint32)
+        # the reference loop, restated here independently of the C oracle
+        want = np.zeros(n, np.uint32)
+        for k in range(n):
+            cn = n - 1 - k // 2 if k & 1 else k // 2
+            want[_brev(cn, L)] = src[k]
+        got = po.permute_hc_to_canonic_brt(src)
+        assert np.array_equal(got, want), L
+        assert np.array_equal(po.permute_canonic_brt_to_hc_natural(got), src), L
+        # both directions are one permutation (an involution)
+        assert np.array_equal(po.permute_canonic_brt_to_hc_natural(src), got), L
+        if L >= 1:
+            # the geometry: point k of half_coset(L) is point cn(k) of the
+            # canonic CircleDomain (half_odds(L - 1), then its conjugates)
+            hc = po.coset(po.HALF_COSET, L)
+            for k in range(n):
+                cn = n - 1 - k // 2 if k & 1 else k // 2
+                assert po.coset_at(hc, k) == po.stwo_domain_point(L, cn)
+
+
+def test_permute_then_stwo_interpolate_matches_direct_evaluation():
+    # values of a polynomial at the half-coset points, in natural order, go
+    # through the permutation into the stwo NTT's input order: interpolate
+    # then gives back the coefficients (the reference's
+    # test_permute_then_stwo_intt_eval_domain, src/ntt.rs:451-500); from
+    # 8 points up, where the stwo NTT is the CircleDomain transform
+    for L in (3, 5, 8):
+        n = 1 << L
+        coeffs = np.random.default_rng(80 + L).integers(0, P, n, dtype=np.uint32)
+        hc = po.coset(po.HALF_COSET, L)
+        vals = np.zeros(n, np.uint32)
+        for k in range(n):
+            x, y = po.coset_at(hc, k)
+            v = po.stwo_eval_at_point(coeffs, (x, 0, 0, 0), (y, 0, 0, 0))
+            assert list(v[1:]) == [0, 0, 0]
+            vals[k] = v[0]
+        assert np.array_equal(po.stwo_interpolate(po.permute_hc_to_canonic_brt(vals)), coeffs), L

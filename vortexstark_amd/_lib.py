@@ -90,6 +90,8 @@ SIGNATURES = {
     "vx_barycentric_eval": (ip, [vp, vp, u32, vp, u32, vp]),
     "vx_grind_pow_stwo": (ip, [vp, u32, vp, vp]),
     "vx_grind_pow_stwo_m31": (ip, [vp, u32, vp, vp]),
+    "vx_permute_hc_to_canonic_brt": (ip, [vp, vp, u32, u32, vp]),
+    "vx_permute_canonic_brt_to_hc_natural": (ip, [vp, vp, u32, u32, vp]),
     "vx_grind_pow_m31_output": (ip, [vp, vp, u32, u64, u32, vp]),
     "vx_m31_add": (ip, [vp, vp, vp, u32, vp]),
     "vx_m31_mul": (ip, [vp, vp, vp, u32, vp]),

@@ -401,6 +401,22 @@ int vx_gather_u256_checked(const uint32_t* src, const uint32_t* idx, uint32_t* d
   return gather_call(src, idx, dst, n, 8, true, src_len, st);
 }
 
+static int permute_call(const char* what, const uint32_t* src, uint32_t* dst, uint32_t n, uint32_t log_n,
+                        vx_stream_t st) {
+  return guard([&] {
+    require(log_n <= 31 && (uint64_t)n == (1ull << log_n), (std::string(what) + ": n must be 2^log_n, log_n <= 31").c_str());
+    require(src && dst && src != dst, (std::string(what) + ": null or aliasing buffers").c_str());
+    permute_hc_brt(src, dst, log_n, S(st));
+  });
+}
+int vx_permute_hc_to_canonic_brt(const uint32_t* src, uint32_t* dst, uint32_t n, uint32_t log_n, vx_stream_t st) {
+  return permute_call("permute_hc_to_canonic_brt", src, dst, n, log_n, st);
+}
+int vx_permute_canonic_brt_to_hc_natural(const uint32_t* src, uint32_t* dst, uint32_t n, uint32_t log_n,
+                                         vx_stream_t st) {
+  return permute_call("permute_canonic_brt_to_hc_natural", src, dst, n, log_n, st);
+}
+
 int vx_pack_leaves(const uint32_t* const* d_in, uint32_t* const* d_out, uint32_t n, vx_stream_t st) {
   return guard([&] {
     require(n % 16 == 0, "pack_leaves: N must be a multiple of 16");

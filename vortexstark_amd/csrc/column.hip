@@ -161,4 +161,76 @@ void pack_leaves(const uint32_t* const* d_in, uint32_t* const* d_out, uint32_t n
   VX_HIP(hipGetLastError());
 }
 
+// Half-coset natural order <-> canonic bit-reversed order (cuda/permute.cu:
+// 30-57, src/circle.rs:192-220). Natural index k of half_coset(log_n) is
+// canonic index cn = k / 2 (k even) or n - 1 - k / 2 (k odd), stored at
+// rev(cn). With h = log_n - 1, M = 2^h - 1 and i = 2m + p this is
+//   F(i) = ((rev_h(m) ^ (p ? M : 0)) << 1) | p,
+// an involution (rev_h(M) = M), so the reference's inverse,
+// dst[k] = src[F(k)], is dst[F(j)] = src[j] again: both entry points run the
+// one permutation dst[F(i)] = src[i].
+__device__ __forceinline__ uint32_t perm_f(uint32_t i, uint32_t h) {
+  const uint32_t m = i >> 1, p = i & 1u;
+  const uint32_t r = h ? __brev(m) >> (32 - h) : 0u;
+  const uint32_t M = h ? 0xFFFFFFFFu >> (32 - h) : 0u;
+  return ((r ^ (p ? M : 0u)) << 1) | p;
+}
+
+// Small columns (log_n <= 10): one element per lane, scattered.
+__global__ void k_permute_scatter(const uint32_t* src, uint32_t* dst, uint32_t n, uint32_t h) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[perm_f(i, h)] = src[i];
+}
+
+// log_n >= 11: i = (a, mid, b, p) with a, b of PM_T bits and mid of
+// mb = h - 2 PM_T bits; F(i) = (rev b ^ f, rev mid ^ f', rev a ^ f, p), f
+// and f' all ones when p = 1. A run of 64 consecutive destinations (fixed
+// B, MIDd) takes its even words from source middle value rev(MIDd) and its
+// odd words from the complement of that, so a workgroup loads the pair of
+// 32 x 64 tiles {mid, ~mid} (32 runs of 64 consecutive source words each)
+// and writes 64 full destination runs of 64 words: 256-byte runs on both
+// sides. With mb = 0 there is one tile, which pairs with itself.
+static constexpr uint32_t PM_T = 5, PM_W = 1u << PM_T;
+__global__ __launch_bounds__(256) void k_permute_tiles(const uint32_t* src, uint32_t* dst, uint32_t h) {
+  __shared__ uint32_t tile[2][PM_W][2 * PM_W + 1];
+  const uint32_t mb = h - 2 * PM_T;
+  const uint32_t t = threadIdx.x, e = t & 63u;
+  const uint32_t mmask = mb ? 0xFFFFFFFFu >> (32 - mb) : 0u;
+  const uint32_t mid0 = blockIdx.x, mid1 = mid0 ^ mmask;  // blockIdx.x < 2^(mb - 1): mid0's top bit is 0
+  const uint32_t ntiles = mb ? 2u : 1u;
+  const uint32_t ash = mb + PM_T + 1;  // position of a (source) and B (destination)
+#pragma unroll
+  for (uint32_t q = 0; q < 2; q++) {
+    if (q >= ntiles) break;
+    const uint64_t base = (uint64_t)(q ? mid1 : mid0) << (PM_T + 1);
+#pragma unroll
+    for (uint32_t r = t >> 6; r < PM_W; r += 4) tile[q][r][e] = src[((uint64_t)r << ash) | base | e];
+  }
+  __syncthreads();
+  const uint32_t rmid0 = mb ? __brev(mid0) >> (32 - mb) : 0u;
+  const uint32_t x = e >> 1, p = e & 1u, flip = p ? PM_W - 1 : 0u;
+  const uint32_t a = __brev(x ^ flip) >> (32 - PM_T);
+#pragma unroll 4
+  for (uint32_t run = t >> 6; run < ntiles * PM_W; run += 4) {
+    const uint32_t B = run & (PM_W - 1), which = run >> PM_T;
+    const uint32_t midd = which ? rmid0 ^ mmask : rmid0;
+    const uint32_t b = __brev(B ^ flip) >> (32 - PM_T);
+    dst[((uint64_t)B << ash) | ((uint64_t)midd << (PM_T + 1)) | e] = tile[mb ? (p ^ which) : 0u][a][2 * b + p];
+  }
+}
+
+void permute_hc_brt(const uint32_t* src, uint32_t* dst, uint32_t log_n, hipStream_t s) {
+  const uint32_t h = log_n ? log_n - 1 : 0;
+  if (log_n <= 2 * PM_T) {
+    const uint32_t n = 1u << log_n;
+    hipLaunchKernelGGL(k_permute_scatter, dim3((n + COL_THREADS - 1) / COL_THREADS), dim3(COL_THREADS), 0, s, src,
+                       dst, n, h);
+  } else {
+    const uint32_t mb = h - 2 * PM_T;
+    hipLaunchKernelGGL(k_permute_tiles, dim3(mb ? 1u << (mb - 1) : 1u), dim3(256), 0, s, src, dst, h);
+  }
+  VX_HIP(hipGetLastError());
+}
+
 }  // namespace vx

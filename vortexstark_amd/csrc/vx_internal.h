@@ -188,6 +188,9 @@ void gather(const uint32_t* src, const uint32_t* idx, uint32_t* dst, uint32_t n,
             uint64_t src_len, hipStream_t s);
 // 4 columns of n (a multiple of 16) words into 64 columns of n / 16:
 // out[16 c + (p mod 16)][p / 16] = in[c][p] (device pointer arrays).
+// Half-coset natural <-> canonic bit-reversed order; one permutation for
+// both directions (it is an involution). Out of place.
+void permute_hc_brt(const uint32_t* src, uint32_t* dst, uint32_t log_n, hipStream_t s);
 void pack_leaves(const uint32_t* const* d_in, uint32_t* const* d_out, uint32_t n, hipStream_t s);
 
 // ----------------------------------------------- twiddles (twiddles.hip)

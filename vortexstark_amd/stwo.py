@@ -187,3 +187,21 @@ def lift_and_accumulate(channels, log_sizes, stream=None):
     flat = [b.as_ptr() for ch in channels for b in ch]
     call("vx_lift_and_accumulate", (C.c_void_p * len(flat))(*flat), (C.c_uint32 * len(log_sizes))(*log_sizes),
          len(log_sizes), _stream(stream))
+
+
+def permute_hc_to_canonic_brt(d_src, log_n, stream=None):
+    """Coset::permute_hc_natural_to_canonic_brt on the device (src/circle.rs:
+    192-204, cuda_permute_hc_to_canonic_brt): values in half_coset(log_n)
+    natural order into the canonic bit-reversed order the stwo NTT takes.
+    Returns a new DeviceBuffer."""
+    out = DeviceBuffer(1 << log_n)
+    call("vx_permute_hc_to_canonic_brt", _ptr(d_src), _ptr(out), 1 << log_n, log_n, _stream(stream))
+    return out
+
+
+def permute_canonic_brt_to_hc_natural(d_src, log_n, stream=None):
+    """The inverse (src/circle.rs:206-220, cuda_permute_canonic_brt_to_hc_natural).
+    Returns a new DeviceBuffer."""
+    out = DeviceBuffer(1 << log_n)
+    call("vx_permute_canonic_brt_to_hc_natural", _ptr(d_src), _ptr(out), 1 << log_n, log_n, _stream(stream))
+    return out
