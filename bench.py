@@ -479,6 +479,20 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     singles = [stwo.eval_at_point(b, 1 << ol, zx, zy) for b in ocols]
     oods_single_ms = (time.perf_counter() - t0) * 1000.0
     oods_same = all(tuple(int(v) for v in batch_vals[i]) == singles[i] for i in range(oc))
+    # the OODS step over the same 64 columns (trace 2^20): draw z, both
+    # points' values for every column, mix them into the channel
+    d_chan = DeviceBuffer.from_host(np.arange(9, dtype=np.uint32))
+    d_opts, d_ovals = DeviceBuffer(16), DeviceBuffer(8 * oc)
+
+    def oods_call():
+        call("vx_oods_sample", vp(d_chan.as_ptr()), o_ptrs, o_logs, oc, ol, vp(d_opts.as_ptr()),
+             vp(d_ovals.as_ptr()), None)
+
+    oods_call()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        oods_call()
+    oods_step_ms = (time.perf_counter() - t0) * 1000.0 / steps
     # half-coset natural -> canonic bit-reversed order of one 2^log_n column
     # (asynchronous calls back to back): reads and writes the column once
     d_perm = DeviceBuffer(n)
@@ -519,6 +533,9 @@ def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
                                 "same_as_single_calls": oods_same},
         "quotients_combine": {"rows": qr, "accumulators": 2, "ms_per_call": round(comb_ms, 4),
                               "hbm_gbs": round(4 * qr * (2 + 8 + 4) / (comb_ms * 1e-3) / 1e9, 1)},
+        "oods_sample": {"columns": oc, "log_n": ol, "ms_per_call": round(oods_step_ms, 4),
+                        "read_gbs": round(2 * 4 * oc * (1 << ol) / (oods_step_ms * 1e-3) / 1e9, 1),
+                        "synchronous": True},
         "permute_hc_to_canonic_brt": {"log_n": log_n, "ms_per_call": round(perm_ms, 4),
                                       "hbm_gbs": round(8 * n / (perm_ms * 1e-3) / 1e9, 1),
                                       "hbm_frac": round(8 * n / (perm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},

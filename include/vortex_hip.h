@@ -188,6 +188,21 @@ int vx_eval_at_point(const uint32_t *d_coeffs, const uint32_t *d_folding_factors
 int vx_eval_at_point_batch(const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
                            const uint32_t *d_folding_factors, uint32_t max_log_n, uint32_t *d_results,
                            vx_stream_t stream);
+/* The OODS step (src/oods.rs:1-19): on a DEVICE channel of 9 words (the
+ * 32-byte state as 8 LE words, then n_draws), draw t (Channel::draw_felt)
+ * and the point z = ((1 - t^2)/(1 + t^2), 2t/(1 + t^2)) (OodsPoint::
+ * from_channel, :44-52), z_next = z * step with step the generator of the
+ * 2^trace_log_n subgroup (next_step, :74-81); evaluate every stwo-basis
+ * coefficient column (HOST arrays of device pointers and log sizes <= 30) at
+ * z and at z_next; mix the 2 n_cols values into the channel
+ * (flatten_for_channel order, :395-406: all f_k(z), then all f_k(z_next);
+ * Channel::mix_felts, src/channel.rs:46-59). Writes d_points[16] =
+ * (z.x, z.y, z_next.x, z_next.y) and d_values[8 n_cols] in that order; the
+ * channel is updated in place. No counterpart kernel in the reference (its
+ * OODS step runs on the host); the per-column evaluation is
+ * vx_eval_at_point_batch. Synchronous. */
+int vx_oods_sample(uint32_t *d_channel, const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
+                   uint32_t trace_log_n, uint32_t *d_points, uint32_t *d_values, vx_stream_t stream);
 /* Replaces cuda_barycentric_eval (ffi.rs:1037-1043, cuda/barycentric_eval.cu:
  * 41-110): d_out[4b..4b+4) = sum of d_evals[i] * d_weights[i] (QM31, 4 words
  * each) over i = b*256 + t + k*n_blocks*256 (t < 256), the reference's

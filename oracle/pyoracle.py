@@ -78,6 +78,7 @@ def lib(omp=False):
         "or_grind_pow": (u64, [C.POINTER(Channel), u32]),
         "or_grind_pow_stwo": (u64, [C.POINTER(Channel), u32]),
         "or_permute_hc_to_canonic_brt": (None, [_u32p, _u32p, u32]),
+        "or_oods_sample": (None, [C.POINTER(Channel), C.c_void_p, _u32p, u32, u32, _u32p, _u32p]),
         "or_permute_canonic_brt_to_hc_natural": (None, [_u32p, _u32p, u32]),
         "or_pow_tz_words": (u32, [_u32p, u64, C.c_int]),
         "or_grind_pow_stwo_words": (u64, [_u32p, u32, C.c_int]),
@@ -335,6 +336,17 @@ class PyChannel:
 
     def grind_stwo(self, bits):
         return int(lib().or_grind_pow_stwo(C.byref(self.c), bits))
+
+    def oods_sample(self, cols, trace_log):
+        """The OODS step (src/oods.rs): returns (points (16 words: z.x, z.y,
+        z_next.x, z_next.y), values (f_k(z) for every column, then f_k(z_next)))
+        and leaves the channel after the mix."""
+        cs = [u32(c) for c in cols]
+        logs = u32([_log2_any(c.size) for c in cs] or [0])
+        pts = np.zeros(16, np.uint32)
+        vals = np.zeros(max(1, 8 * len(cs)), np.uint32)
+        lib().or_oods_sample(C.byref(self.c), _ptrs(cs) if cs else None, logs, len(cs), trace_log, pts, vals)
+        return pts, vals[:8 * len(cs)]
 
     def pow_prefix_m31(self, bits):
         """The Blake2sChannelGeneric<true> prefix: pow_prefix with every word

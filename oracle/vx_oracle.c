@@ -2065,3 +2065,37 @@ void or_lift_and_accumulate(uint32_t *const *cols, const uint32_t *log_sizes, ui
         curr_log = log_sizes[k];
     }
 }
+
+/* The OODS step (src/oods.rs): t = draw_felt, z = ((1 - t^2)/(1 + t^2),
+ * 2t/(1 + t^2)) (:44-52), z_next = z * step, step = generator doubled
+ * 31 - trace_log times (:74-81, circle.rs:26-29), every column evaluated at
+ * z and z_next (or_stwo_eval_at_point, the basis sum), then the values
+ * mixed into the channel, all f_k(z) first (:395-406, channel.rs:46-59).
+ * points: z.x, z.y, z_next.x, z_next.y; values: 8 n_cols words. */
+void or_oods_sample(or_channel *ch, const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
+                    uint32_t trace_log, uint32_t points[16], uint32_t *values) {
+    uint32_t tw[4];
+    or_channel_draw_felt(ch, tw);
+    const qm31 t = q_load(tw), one = {{1, 0, 0, 0}};
+    const qm31 t2 = q_mul(t, t);
+    const qm31 inv = q_inv(q_add(one, t2));
+    const qm31 zx = q_mul(q_sub(one, t2), inv), zy = q_mul(q_add(t, t), inv);
+    uint32_t st[2] = {2u, 1268011823u};
+    for (uint32_t i = 0; i < 31 - trace_log; i++) {
+        uint32_t sq[2];
+        or_circle_mul(st, st, sq);
+        st[0] = sq[0];
+        st[1] = sq[1];
+    }
+    const qm31 nx = q_sub(q_scale(zx, st[0]), q_scale(zy, st[1]));
+    const qm31 ny = q_add(q_scale(zx, st[1]), q_scale(zy, st[0]));
+    const qm31 pts[4] = {zx, zy, nx, ny};
+    for (int p = 0; p < 4; p++)
+        for (int k = 0; k < 4; k++) points[4 * p + k] = pts[p].v[k];
+    for (uint32_t c = 0; c < n_cols; c++) {
+        or_stwo_eval_at_point(cols[c], log_sizes[c], zx.v, zy.v, values + 4 * c);
+        or_stwo_eval_at_point(cols[c], log_sizes[c], nx.v, ny.v, values + 4 * (n_cols + c));
+    }
+    or_channel_mix_felts(ch, values, 2 * (size_t)n_cols);
+}
+

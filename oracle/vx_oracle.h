@@ -171,6 +171,10 @@ void or_stwo_ntt_interpolate(uint32_t *data, const uint32_t *itw, uint32_t log_n
 /* half-coset natural <-> canonic bit-reversed order (src/circle.rs:192-220) */
 void or_permute_hc_to_canonic_brt(const uint32_t *src, uint32_t *dst, uint32_t log_n);
 void or_permute_canonic_brt_to_hc_natural(const uint32_t *src, uint32_t *dst, uint32_t log_n);
+/* the OODS step of src/oods.rs: draw z, z_next = z * step(trace_log),
+ * evaluate every column at both, mix the values (all at z first) */
+void or_oods_sample(or_channel *ch, const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
+                    uint32_t trace_log, uint32_t points[16], uint32_t *values);
 void or_stwo_eval_at_point(const uint32_t *coeffs, uint32_t log_n, const uint32_t zx[4], const uint32_t zy[4],
                            uint32_t out[4]);
 /* cuda/circle_ntt.cu:186-231,429-468 (cuda_eval_at_point, folding factors given);

@@ -365,3 +365,42 @@ def test_permute_then_stwo_interpolate():
     for k in (0, 1, 2, 3, 1000, 4095):
         x, y = po.coset_at(hcs, k)
         assert po.stwo_eval_at_point(coeffs, (x, 0, 0, 0), (y, 0, 0, 0)) == (int(vals[k]), 0, 0, 0), k
+
+
+# ---- the OODS step (src/oods.rs) on a device channel
+
+def _chan_words(ch):
+    return np.concatenate([np.frombuffer(ch.state, np.uint32), [ch.c.n_draws]]).astype(np.uint32)
+
+
+@pytest.mark.parametrize("trace_log,logs", [(0, [3]), (5, [0, 1, 5, 5]), (20, [14, 16, 18, 20, 20]),
+                                            (31, [12, 22]), (8, [])])
+def test_oods_sample(trace_log, logs):
+    print(f"oods_sample trace_log={trace_log} logs={logs}", flush=True)
+    cols = [rnd(1 << L, 200 + i) for i, L in enumerate(logs)]
+    ch = po.PyChannel()
+    ch.mix_digest([trace_log, 8, 6, 7, 5, 3, 0, 9])
+    d_ch = dev(_chan_words(ch))
+    dcols = [dev(c) for c in cols]
+    d_pts, d_vals = empty(16), empty(max(1, 8 * len(cols)))
+    ptrs = (C.c_void_p * max(1, len(cols)))(*[c.ptr for c in dcols])
+    lg = (C.c_uint32 * max(1, len(cols)))(*logs)
+    call("vx_oods_sample", ptr(d_ch), ptrs, lg, len(cols), trace_log, ptr(d_pts), ptr(d_vals), NULL)
+    pts, vals = ch.oods_sample(cols, trace_log)
+    assert np.array_equal(host(d_pts), pts)
+    if cols:
+        assert np.array_equal(host(d_vals)[:8 * len(cols)], vals)
+    assert np.array_equal(host(d_ch), _chan_words(ch))
+
+
+def test_oods_sample_mirror():
+    from vortexstark_amd import stwo
+    from vortexstark_amd.device import DeviceBuffer
+    cols = [rnd(1 << L, 300 + L) for L in (10, 13)]
+    ch = po.PyChannel()
+    ch.mix_digest([1, 2, 3, 4, 5, 6, 7, 8])
+    d_ch = DeviceBuffer.from_host(_chan_words(ch))
+    pts, vals = stwo.oods_sample(d_ch, [DeviceBuffer.from_host(c) for c in cols], [10, 13], 13)
+    want_pts, want_vals = ch.oods_sample(cols, 13)
+    assert np.array_equal(pts, want_pts) and np.array_equal(vals, want_vals)
+    assert np.array_equal(d_ch.to_host(), _chan_words(ch))

@@ -392,3 +392,66 @@ def test_permute_then_stwo_interpolate_matches_direct_evaluation():
             assert list(v[1:]) == [0, 0, 0]
             vals[k] = v[0]
         assert np.array_equal(po.stwo_interpolate(po.permute_hc_to_canonic_brt(vals)), coeffs), L
+
+
+# ---- the OODS step (src/oods.rs): point from the channel, z_next, values, mix
+
+def _qm(v):
+    return tuple(int(x) for x in v)
+
+
+def _on_circle(x, y):
+    return pyref.qm_add(pyref.qm_mul(x, x), pyref.qm_mul(y, y)) == (1, 0, 0, 0)
+
+
+def _channel(seed):
+    ch = po.PyChannel()
+    ch.mix_digest([seed, 3, 1, 4, 1, 5, 9, 2])
+    return ch
+
+
+def test_oods_sample_oracle():
+    rng = np.random.default_rng(66)
+    cols = [rng.integers(0, P, 1 << L, dtype=np.uint32) for L in (0, 1, 3, 6, 6, 9)]
+    for trace_log in (0, 3, 9, 20, 31):
+        ch = _channel(trace_log)
+        pts, vals = ch.oods_sample(cols, trace_log)
+        zx, zy, nx, ny = (_qm(pts[4 * i:4 * i + 4]) for i in range(4))
+        assert _on_circle(zx, zy) and _on_circle(nx, ny)
+        # z is the rational point of the drawn t: y = t (1 + x); the
+        # reference channel here is pyref's hashlib restatement
+        ref = pyref.Channel()
+        ref.mix_digest([trace_log, 3, 1, 4, 1, 5, 9, 2])
+        t = ref.draw_felt()
+        assert zy == pyref.qm_mul(t, pyref.qm_add((1, 0, 0, 0), zx))
+        # z_next: z moved by the 2^trace_log subgroup's generator, so
+        # 2^trace_log steps come back to z (trace_log 0: z itself)
+        if trace_log <= 9:
+            gx, gy = po.coset_at(po.coset(po.SUBGROUP, trace_log), 1) if trace_log else (1, 0)
+            x, y = zx, zy
+            for _ in range(1 << trace_log):
+                x, y = (pyref.qm_sub(pyref.qm_scale(x, gx), pyref.qm_scale(y, gy)),
+                        pyref.qm_add(pyref.qm_scale(x, gy), pyref.qm_scale(y, gx)))
+                if (x, y) == (zx, zy):
+                    break
+            assert (x, y) == (zx, zy)
+            if trace_log:
+                x1 = pyref.qm_sub(pyref.qm_scale(zx, gx), pyref.qm_scale(zy, gy))
+                y1 = pyref.qm_add(pyref.qm_scale(zx, gy), pyref.qm_scale(zy, gx))
+                assert (x1, y1) == (nx, ny)
+            else:
+                assert (nx, ny) == (zx, zy)
+        # values: every column at z, then every column at z_next
+        n = len(cols)
+        for k, c in enumerate(cols):
+            assert _qm(vals[4 * k:4 * k + 4]) == po.stwo_eval_at_point(c, zx, zy)
+            assert _qm(vals[4 * (n + k):4 * (n + k) + 4]) == po.stwo_eval_at_point(c, nx, ny)
+        # the channel: draw, then mix all the values in that order
+        ref.mix_felts([_qm(vals[4 * i:4 * i + 4]) for i in range(2 * n)])
+        assert ref.state == ch.state and ch.c.n_draws == 0
+    # no columns: the draw only
+    ch, ref = _channel(1), pyref.Channel()
+    ref.mix_digest([1, 3, 1, 4, 1, 5, 9, 2])
+    ch.oods_sample([], 5)
+    ref.draw_felt()
+    assert ch.state == ref.state and ch.c.n_draws == ref.n_draws

@@ -91,6 +91,7 @@ SIGNATURES = {
     "vx_grind_pow_stwo": (ip, [vp, u32, vp, vp]),
     "vx_grind_pow_stwo_m31": (ip, [vp, u32, vp, vp]),
     "vx_permute_hc_to_canonic_brt": (ip, [vp, vp, u32, u32, vp]),
+    "vx_oods_sample": (ip, [vp, vp, vp, u32, u32, vp, vp, vp]),
     "vx_permute_canonic_brt_to_hc_natural": (ip, [vp, vp, u32, u32, vp]),
     "vx_grind_pow_m31_output": (ip, [vp, vp, u32, u64, u32, vp]),
     "vx_m31_add": (ip, [vp, vp, vp, u32, vp]),

@@ -271,6 +271,18 @@ int vx_eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_
   });
 }
 
+int vx_oods_sample(uint32_t* d_channel, const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                   uint32_t trace_log_n, uint32_t* d_points, uint32_t* d_values, vx_stream_t st) {
+  return guard([&] {
+    require(d_channel && d_points && (n_cols == 0 || (h_cols && h_log_sizes && d_values)),
+            "oods_sample: null argument");
+    require(trace_log_n <= 31, "oods_sample: trace_log_n must be <= 31");
+    for (uint32_t k = 0; k < n_cols; k++)
+      require(h_cols[k] && h_log_sizes[k] <= 30, "oods_sample: null column or log size above 30");
+    oods_sample(d_channel, h_cols, h_log_sizes, n_cols, trace_log_n, d_points, d_values, S(st));
+  });
+}
+
 int vx_barycentric_eval(const uint32_t* d_evals, const uint32_t* d_weights, uint32_t n, uint32_t* d_out,
                         uint32_t n_blocks, vx_stream_t st) {
   return guard([&] {

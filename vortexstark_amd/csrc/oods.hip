@@ -468,4 +468,77 @@ void barycentric_partials(const uint32_t* ev, const uint32_t* w, uint32_t n, uin
   VX_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------- OODS step
+// src/oods.rs:44-81: t drawn from the channel, z = ((1 - t^2) / (1 + t^2),
+// 2t / (1 + t^2)) on the QM31 circle, z_next = z * step with step the
+// generator of the 2^trace_log subgroup (circle.rs generator doubled
+// 31 - trace_log times); then both points' folding factors for max_log:
+// [z.y, z.x, 2 z.x^2 - 1, ...] (max_log entries) reversed.
+__device__ void oods_factors(qm31 zx, qm31 zy, uint32_t max_log, uint32_t* out) {
+  qm31 x = zx;
+  for (uint32_t i = 0; i < max_log; i++) {
+    const qm31 m = i == 0 ? zy : x;
+    if (i > 0) {
+      const qm31 x2 = qm_mul(x, x);
+      x = qm_sub(qm_add(x2, x2), qm31{{1u, 0u, 0u, 0u}});
+    }
+    for (int k = 0; k < 4; k++) out[4 * (max_log - 1 - i) + k] = m.v[k];
+  }
+}
+
+__global__ void k_oods_point(uint32_t* ch_words, uint32_t trace_log, uint32_t max_log, uint32_t* points, uint32_t* fz,
+                             uint32_t* fzn) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Channel c;
+  for (int i = 0; i < 8; i++) c.state[i] = ch_words[i];
+  c.n_draws = ch_words[8];
+  const qm31 t = ch_draw_felt(c);
+  for (int i = 0; i < 8; i++) ch_words[i] = c.state[i];
+  ch_words[8] = c.n_draws;
+  const qm31 one{{1u, 0u, 0u, 0u}};
+  const qm31 t2 = qm_mul(t, t);
+  const qm31 inv = qm_inv(qm_add(one, t2));
+  const qm31 zx = qm_mul(qm_sub(one, t2), inv), zy = qm_mul(qm_add(t, t), inv);
+  const cpt st = c_rdouble(c_generator(), 31 - trace_log);
+  const qm31 nx = qm_sub(qm_scale(zx, st.x), qm_scale(zy, st.y));
+  const qm31 ny = qm_add(qm_scale(zx, st.y), qm_scale(zy, st.x));
+  const qm31 pts[4] = {zx, zy, nx, ny};
+  for (int p = 0; p < 4; p++)
+    for (int k = 0; k < 4; k++) points[4 * p + k] = pts[p].v[k];
+  oods_factors(zx, zy, max_log, fz);
+  oods_factors(nx, ny, max_log, fzn);
+}
+
+// Channel::mix_felts (src/channel.rs:46-59) of the sampled values: msg holds
+// 8 free words, then the 8 n_cols value words; the state goes in front and
+// state || values is hashed as one message.
+__global__ void k_oods_mix(uint32_t* ch_words, uint32_t* msg, uint32_t n_felts) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || n_felts == 0) return;
+  for (int i = 0; i < 8; i++) msg[i] = ch_words[i];
+  uint32_t h[8];
+  b2_hash_long(msg, 32u + 16u * n_felts, h);
+  for (int i = 0; i < 8; i++) ch_words[i] = h[i];
+  ch_words[8] = 0;
+}
+
+void oods_sample(uint32_t* d_channel, const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                 uint32_t trace_log, uint32_t* d_points, uint32_t* d_values, hipStream_t s) {
+  uint32_t max_log = 0;
+  for (uint32_t k = 0; k < n_cols; k++) max_log = h_log_sizes[k] > max_log ? h_log_sizes[k] : max_log;
+  const uint32_t fw = 4 * (max_log ? max_log : 1);
+  DeviceBuf fac(8ull * fw), msg(4ull * (8 + 8ull * n_cols));
+  uint32_t* fz = static_cast<uint32_t*>(fac.p);
+  uint32_t* m = static_cast<uint32_t*>(msg.p);
+  hipLaunchKernelGGL(k_oods_point, dim3(1), dim3(64), 0, s, d_channel, trace_log, max_log, d_points, fz, fz + fw);
+  VX_HIP(hipGetLastError());
+  if (n_cols) {
+    eval_at_point_batch(h_cols, h_log_sizes, n_cols, fz, max_log, m + 8, s);
+    eval_at_point_batch(h_cols, h_log_sizes, n_cols, fz + fw, max_log, m + 8 + 4 * n_cols, s);
+    hipLaunchKernelGGL(k_oods_mix, dim3(1), dim3(64), 0, s, d_channel, m, 2 * n_cols);
+    VX_HIP(hipGetLastError());
+    VX_HIP(hipMemcpyAsync(d_values, m + 8, 32ull * n_cols, hipMemcpyDeviceToDevice, s));
+  }
+  VX_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace vx

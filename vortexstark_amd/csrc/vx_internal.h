@@ -157,6 +157,12 @@ const uint32_t* eval_at_point(const uint32_t* coeffs, const uint32_t* factors, u
 // using the last log_size_k of the max_log_n factors. Synchronous.
 void eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
                          const uint32_t* factors, uint32_t max_log_n, uint32_t* results, hipStream_t s);
+// The OODS step of src/oods.rs on a device channel (9 words: state, n_draws):
+// draw z, z_next = z * step(trace_log); points = (z.x, z.y, z_next.x,
+// z_next.y); values = f_k(z) for every column, then f_k(z_next); the 2 n_cols
+// values mixed into the channel. Synchronous.
+void oods_sample(uint32_t* d_channel, const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                 uint32_t trace_log, uint32_t* d_points, uint32_t* d_values, hipStream_t s);
 // Per-block partial sums of evals[i] * weights[i] over the reference's
 // grid-stride partition (n_blocks blocks of 256 lanes); out: n_blocks QM31.
 void barycentric_partials(const uint32_t* evals, const uint32_t* weights, uint32_t n, uint32_t* out,

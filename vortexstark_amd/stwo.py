@@ -205,3 +205,18 @@ def permute_canonic_brt_to_hc_natural(d_src, log_n, stream=None):
     out = DeviceBuffer(1 << log_n)
     call("vx_permute_canonic_brt_to_hc_natural", _ptr(d_src), _ptr(out), 1 << log_n, log_n, _stream(stream))
     return out
+
+
+def oods_sample(d_channel, columns, log_sizes, trace_log_n, stream=None):
+    """The OODS step of src/oods.rs on the device: d_channel is a 9-word
+    DeviceBuffer (channel state as 8 LE words, then n_draws), updated in
+    place; columns are stwo-basis coefficient DeviceBuffers. Returns
+    (points, values): z.x, z.y, z_next.x, z_next.y (16 words) and every
+    f_k(z) then every f_k(z_next) (4 words each)."""
+    n = len(columns)
+    d_pts = DeviceBuffer(16)
+    d_vals = DeviceBuffer(max(1, 8 * n))
+    ptrs = (C.c_void_p * max(1, n))(*[c.as_ptr() for c in columns])
+    logs = (C.c_uint32 * max(1, n))(*log_sizes)
+    call("vx_oods_sample", _ptr(d_channel), ptrs, logs, n, trace_log_n, _ptr(d_pts), _ptr(d_vals), _stream(stream))
+    return d_pts.to_host(), d_vals.to_host()[:8 * n]
