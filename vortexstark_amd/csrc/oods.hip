@@ -273,8 +273,13 @@ __device__ qm31 lds_fold_table(qm31* q, qm31 v, uniform_t<qm31>* f) {
   return q[0];
 }
 
+// NP points (1, or 2 for the OODS step's z and z_next) from one read of the
+// coefficients: table holds NP 28-entry tables, part NP arrays of
+// part_stride QM31.
+template <uint32_t NP>
 __global__ __launch_bounds__(EV_WG) void k_eval_batch_chunks(const EvalChunkJob* jobs, uint32_t n_jobs,
-                                                             const uint32_t* table, uint32_t* part) {
+                                                             const uint32_t* table, uint32_t* part,
+                                                             uint32_t part_stride) {
   __shared__ qm31 q[EV_WG];
   const uint32_t b = blockIdx.x, t = threadIdx.x;
   uint32_t lo = 0, hi = n_jobs;
@@ -297,35 +302,47 @@ __global__ __launch_bounds__(EV_WG) void k_eval_batch_chunks(const EvalChunkJob*
       x[m] = make_uint4(w < n ? col[w] : 0u, w + 1 < n ? col[w + 1] : 0u, 0u, 0u);
     }
   }
-  uniform_t<qm31>* E = (uniform_t<qm31>*)table;
-  qm31 w4[4];
 #pragma unroll
-  for (uint32_t m23 = 0; m23 < 4; m23++) {
+  for (uint32_t pt = 0; pt < NP; pt++) {
+    const uint32_t* tab = table + 28 * 4 * pt;
+    uniform_t<qm31>* E = (uniform_t<qm31>*)tab;
+    qm31 w4[4];
 #pragma unroll
-    for (uint32_t k = 0; k < 4; k++) {
-      uint32_t r = 0;
+    for (uint32_t m23 = 0; m23 < 4; m23++) {
 #pragma unroll
-      for (uint32_t m01 = 0; m01 < 4; m01++) {
-        const uint4 v = x[m01 + 4 * m23];
-        const uint64_t acc = (uint64_t)v.x * E[4 * m01].v[k] + (uint64_t)v.y * E[4 * m01 + 1].v[k] +
-                             (uint64_t)v.z * E[4 * m01 + 2].v[k] + (uint64_t)v.w * E[4 * m01 + 3].v[k];
-        r = m31_add(r, m31_reduce64(acc));
+      for (uint32_t k = 0; k < 4; k++) {
+        uint32_t r = 0;
+#pragma unroll
+        for (uint32_t m01 = 0; m01 < 4; m01++) {
+          const uint4 v = x[m01 + 4 * m23];
+          const uint64_t acc = (uint64_t)v.x * E[4 * m01].v[k] + (uint64_t)v.y * E[4 * m01 + 1].v[k] +
+                               (uint64_t)v.z * E[4 * m01 + 2].v[k] + (uint64_t)v.w * E[4 * m01 + 3].v[k];
+          r = m31_add(r, m31_reduce64(acc));
+        }
+        w4[m23].v[k] = r;
       }
-      w4[m23].v[k] = r;
     }
-  }
-  qm31 v = w4[0];
+    qm31 v = w4[0];
 #pragma unroll
-  for (uint32_t m23 = 1; m23 < 4; m23++)
-    v = qm_add(v, qm_mul(ld_uniform(reinterpret_cast<const qm31*>(table) + 16 + m23), w4[m23]));
-  const qm31 sum = lds_fold_table(q, v, (uniform_t<qm31>*)table + 20);
-  if (t == 0) reinterpret_cast<qm31*>(part)[b] = sum;
+    for (uint32_t m23 = 1; m23 < 4; m23++)
+      v = qm_add(v, qm_mul(ld_uniform(reinterpret_cast<const qm31*>(tab) + 16 + m23), w4[m23]));
+    const qm31 sum = lds_fold_table(q, v, (uniform_t<qm31>*)tab + 20);
+    if (t == 0) reinterpret_cast<qm31*>(part)[(uint64_t)pt * part_stride + b] = sum;
+    if (NP > 1) __syncthreads();  // q is reused by the next point's fold
+  }
 }
 
+// blockIdx.y = point: its factors, input and output arrays lie f_stride,
+// in_stride and out_stride words after the previous point's.
 __global__ __launch_bounds__(EV_WG) void k_eval_batch_fold(const EvalFoldJob* jobs, const uint32_t* factors,
-                                                           uint32_t max_log_n, const uint32_t* in, uint32_t* out) {
+                                                           uint32_t max_log_n, const uint32_t* in, uint32_t* out,
+                                                           uint32_t f_stride, uint32_t in_stride,
+                                                           uint32_t out_stride) {
   __shared__ qm31 q[EV_WG];
   const uint32_t t = threadIdx.x;
+  factors += (uint64_t)blockIdx.y * f_stride;
+  in += (uint64_t)blockIdx.y * in_stride;
+  out += (uint64_t)blockIdx.y * out_stride;
   uniform_t<EvalFoldJob>& J = as_uniform(jobs + blockIdx.x);
   const uint32_t n = J.n, fb = J.fb;
   qm31 f[EV_WG_LOG];
@@ -348,9 +365,14 @@ __global__ void k_copy_qm31(const uint32_t* const* src, uint32_t* results, uint3
   reinterpret_cast<qm31*>(results)[k] = *reinterpret_cast<const qm31*>(src[k]);
 }
 
-void eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
-                         const uint32_t* factors, uint32_t max_log_n, uint32_t* results, hipStream_t s) {
+// np points (1 or 2): factors holds np tables of 4 max_log_n words back to
+// back (4 words when max_log_n is 0), results np arrays of n_cols QM31.
+static void eval_batch_points(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                              const uint32_t* factors, uint32_t np, uint32_t max_log_n, uint32_t* results,
+                              hipStream_t s) {
   if (n_cols == 0) return;
+  require(np == 1 || np == 2, "eval_at_point_batch: one or two points");
+  const uint32_t f_stride = 4 * (max_log_n ? max_log_n : 1);
   require(max_log_n <= 31, "eval_at_point_batch: max_log_n must be <= 31");
   std::vector<EvalChunkJob> jobs(n_cols);
   std::vector<uint32_t> count(n_cols), off(n_cols);
@@ -372,16 +394,28 @@ void eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_si
   keep.emplace_back(sizeof(EvalChunkJob) * n_cols);
   VX_HIP(hipMemcpyAsync(keep.back().p, jobs.data(), sizeof(EvalChunkJob) * n_cols, hipMemcpyHostToDevice, s));
   const EvalChunkJob* d_jobs = static_cast<const EvalChunkJob*>(keep.back().p);
-  keep.emplace_back(28 * 16);
+  keep.emplace_back(np * 28 * 16);
   uint32_t* d_table = keep.back().u32();
-  keep.emplace_back(total * 16);
+  keep.emplace_back(np * total * 16);
   uint32_t* level = keep.back().u32();
-  hipLaunchKernelGGL(k_eval_batch_setup, dim3(1), dim3(64), 0, s, factors, max_log_n, d_table);
-  hipLaunchKernelGGL(k_eval_batch_chunks, dim3((uint32_t)total), dim3(EV_WG), 0, s, d_jobs, n_cols, d_table, level);
+  uint32_t level_stride = 4 * (uint32_t)total;  // words between the points' arrays
+  for (uint32_t pt = 0; pt < np; pt++)
+    hipLaunchKernelGGL(k_eval_batch_setup, dim3(1), dim3(64), 0, s, factors + pt * f_stride, max_log_n,
+                       d_table + 28 * 4 * pt);
+  if (np == 2)
+    hipLaunchKernelGGL(k_eval_batch_chunks<2>, dim3((uint32_t)total), dim3(EV_WG), 0, s, d_jobs, n_cols, d_table,
+                       level, (uint32_t)total);
+  else
+    hipLaunchKernelGGL(k_eval_batch_chunks<1>, dim3((uint32_t)total), dim3(EV_WG), 0, s, d_jobs, n_cols, d_table,
+                       level, (uint32_t)total);
   VX_HIP(hipGetLastError());
-  std::vector<const uint32_t*> final_src(n_cols, nullptr);
+  // final_src[pt * n_cols + k]: where column k's sum for point pt ends up
+  std::vector<const uint32_t*> final_src((size_t)np * n_cols, nullptr);
+  auto settle = [&](uint32_t k, const uint32_t* lv, uint32_t stride) {
+    for (uint32_t pt = 0; pt < np; pt++) final_src[(size_t)pt * n_cols + k] = lv + (uint64_t)pt * stride + 4ull * off[k];
+  };
   for (uint32_t k = 0; k < n_cols; k++)
-    if (count[k] == 1) final_src[k] = level + 4ull * off[k];
+    if (count[k] == 1) settle(k, level, level_stride);
   uint32_t fb = EV_BIG_LOG;
   for (;;) {
     std::vector<EvalFoldJob> fj;
@@ -399,23 +433,30 @@ void eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_si
     keep.emplace_back(sizeof(EvalFoldJob) * fj.size());
     VX_HIP(hipMemcpyAsync(keep.back().p, fj.data(), sizeof(EvalFoldJob) * fj.size(), hipMemcpyHostToDevice, s));
     const EvalFoldJob* d_fj = static_cast<const EvalFoldJob*>(keep.back().p);
-    keep.emplace_back((size_t)next * 16);
+    keep.emplace_back((size_t)np * next * 16);
     uint32_t* out = keep.back().u32();
-    hipLaunchKernelGGL(k_eval_batch_fold, dim3((uint32_t)fj.size()), dim3(EV_WG), 0, s, d_fj, factors, max_log_n,
-                       level, out);
+    hipLaunchKernelGGL(k_eval_batch_fold, dim3((uint32_t)fj.size(), np), dim3(EV_WG), 0, s, d_fj, factors,
+                       max_log_n, level, out, f_stride, level_stride, 4 * next);
     VX_HIP(hipGetLastError());
     level = out;
+    level_stride = 4 * next;
     for (uint32_t k = 0; k < n_cols; k++)
-      if (count[k] == 1 && final_src[k] == nullptr) final_src[k] = level + 4ull * off[k];
+      if (count[k] == 1 && final_src[k] == nullptr) settle(k, level, level_stride);
     fb += EV_WG_LOG;
   }
-  keep.emplace_back(sizeof(uint32_t*) * n_cols);
-  VX_HIP(hipMemcpyAsync(keep.back().p, final_src.data(), sizeof(uint32_t*) * n_cols, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_copy_qm31, dim3((n_cols + 255) / 256), dim3(256), 0, s,
-                     static_cast<const uint32_t* const*>(keep.back().p), results, n_cols);
+  const uint32_t n_out = np * n_cols;
+  keep.emplace_back(sizeof(uint32_t*) * n_out);
+  VX_HIP(hipMemcpyAsync(keep.back().p, final_src.data(), sizeof(uint32_t*) * n_out, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_copy_qm31, dim3((n_out + 255) / 256), dim3(256), 0, s,
+                     static_cast<const uint32_t* const*>(keep.back().p), results, n_out);
   VX_HIP(hipGetLastError());
   // the staging buffers are freed on return: the caller's stream must be done
   VX_HIP(hipStreamSynchronize(s));
+}
+
+void eval_at_point_batch(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                         const uint32_t* factors, uint32_t max_log_n, uint32_t* results, hipStream_t s) {
+  eval_batch_points(h_cols, h_log_sizes, n_cols, factors, 1, max_log_n, results, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -532,8 +573,8 @@ void oods_sample(uint32_t* d_channel, const uint32_t* const* h_cols, const uint3
   hipLaunchKernelGGL(k_oods_point, dim3(1), dim3(64), 0, s, d_channel, trace_log, max_log, d_points, fz, fz + fw);
   VX_HIP(hipGetLastError());
   if (n_cols) {
-    eval_at_point_batch(h_cols, h_log_sizes, n_cols, fz, max_log, m + 8, s);
-    eval_at_point_batch(h_cols, h_log_sizes, n_cols, fz + fw, max_log, m + 8 + 4 * n_cols, s);
+    // both points from one read of every column: values at z, then at z_next
+    eval_batch_points(h_cols, h_log_sizes, n_cols, fz, 2, max_log, m + 8, s);
     hipLaunchKernelGGL(k_oods_mix, dim3(1), dim3(64), 0, s, d_channel, m, 2 * n_cols);
     VX_HIP(hipGetLastError());
     VX_HIP(hipMemcpyAsync(d_values, m + 8, 32ull * n_cols, hipMemcpyDeviceToDevice, s));
