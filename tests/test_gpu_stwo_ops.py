@@ -336,6 +336,16 @@ def test_permute_hc_brt(log_n):
     assert np.array_equal(host(dk), src)
 
 
+def test_permute_unaligned_buffers():
+    # buffers off 16-byte alignment take the one-word-per-lane path
+    log_n, n = 12, 1 << 12
+    src = rnd(n, 95)
+    ds, dd = dev(np.concatenate([[7], src]).astype(np.uint32)), empty(n + 1)
+    call("vx_permute_hc_to_canonic_brt", C.c_void_p(ds.ptr + 4), C.c_void_p(dd.ptr + 4), n, log_n, NULL)
+    sync()
+    assert np.array_equal(host(dd)[1:], po.permute_hc_to_canonic_brt(src))
+
+
 def test_permute_rejects_bad_arguments():
     d1, d2 = empty(1 << 8), empty(1 << 8)
     with pytest.raises(VortexError):  # n not 2^log_n

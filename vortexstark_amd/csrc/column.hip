@@ -176,7 +176,8 @@ __device__ __forceinline__ uint32_t perm_f(uint32_t i, uint32_t h) {
   return ((r ^ (p ? M : 0u)) << 1) | p;
 }
 
-// Small columns (log_n <= 10): one element per lane, scattered.
+// Small columns (log_n <= 10) and buffers not 16-byte aligned: one element
+// per lane, scattered.
 __global__ void k_permute_scatter(const uint32_t* src, uint32_t* dst, uint32_t n, uint32_t h) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -190,39 +191,56 @@ __global__ void k_permute_scatter(const uint32_t* src, uint32_t* dst, uint32_t n
 // odd words from the complement of that, so a workgroup loads the pair of
 // 32 x 64 tiles {mid, ~mid} (32 runs of 64 consecutive source words each)
 // and writes 64 full destination runs of 64 words: 256-byte runs on both
-// sides. With mb = 0 there is one tile, which pairs with itself.
+// sides, 16 bytes per lane. With mb = 0 there is one tile, which pairs with
+// itself.
 static constexpr uint32_t PM_T = 5, PM_W = 1u << PM_T;
 __global__ __launch_bounds__(256) void k_permute_tiles(const uint32_t* src, uint32_t* dst, uint32_t h) {
   __shared__ uint32_t tile[2][PM_W][2 * PM_W + 1];
   const uint32_t mb = h - 2 * PM_T;
-  const uint32_t t = threadIdx.x, e = t & 63u;
+  const uint32_t t = threadIdx.x, u = t & 15u;  // u: 16-byte column of a 64-word row or run
   const uint32_t mmask = mb ? 0xFFFFFFFFu >> (32 - mb) : 0u;
   const uint32_t mid0 = blockIdx.x, mid1 = mid0 ^ mmask;  // blockIdx.x < 2^(mb - 1): mid0's top bit is 0
   const uint32_t ntiles = mb ? 2u : 1u;
   const uint32_t ash = mb + PM_T + 1;  // position of a (source) and B (destination)
+  // 16 lanes per 64-word row, 16 rows per pass, one 16-byte load each
 #pragma unroll
   for (uint32_t q = 0; q < 2; q++) {
     if (q >= ntiles) break;
     const uint64_t base = (uint64_t)(q ? mid1 : mid0) << (PM_T + 1);
 #pragma unroll
-    for (uint32_t r = t >> 6; r < PM_W; r += 4) tile[q][r][e] = src[((uint64_t)r << ash) | base | e];
+    for (uint32_t r = t >> 4; r < PM_W; r += 16) {
+      const uint4 v = *reinterpret_cast<const uint4*>(src + (((uint64_t)r << ash) | base | (4 * u)));
+      tile[q][r][4 * u] = v.x;
+      tile[q][r][4 * u + 1] = v.y;
+      tile[q][r][4 * u + 2] = v.z;
+      tile[q][r][4 * u + 3] = v.w;
+    }
   }
   __syncthreads();
   const uint32_t rmid0 = mb ? __brev(mid0) >> (32 - mb) : 0u;
-  const uint32_t x = e >> 1, p = e & 1u, flip = p ? PM_W - 1 : 0u;
-  const uint32_t a = __brev(x ^ flip) >> (32 - PM_T);
+  // destination word e = 4u + k of a run is entry (x = e >> 1, p = e & 1)
+  uint32_t a[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) {
+    const uint32_t e = 4 * u + k, flip = (e & 1u) ? PM_W - 1 : 0u;
+    a[k] = __brev((e >> 1) ^ flip) >> (32 - PM_T);
+  }
 #pragma unroll 4
-  for (uint32_t run = t >> 6; run < ntiles * PM_W; run += 4) {
+  for (uint32_t run = t >> 4; run < ntiles * PM_W; run += 16) {
     const uint32_t B = run & (PM_W - 1), which = run >> PM_T;
     const uint32_t midd = which ? rmid0 ^ mmask : rmid0;
-    const uint32_t b = __brev(B ^ flip) >> (32 - PM_T);
-    dst[((uint64_t)B << ash) | ((uint64_t)midd << (PM_T + 1)) | e] = tile[mb ? (p ^ which) : 0u][a][2 * b + p];
+    const uint32_t be = __brev(B) >> (32 - PM_T), bo = __brev(B ^ (PM_W - 1)) >> (32 - PM_T);
+    const uint32_t te = mb ? which : 0u, to = mb ? which ^ 1u : 0u;  // tiles of the even / odd words
+    const uint4 v = make_uint4(tile[te][a[0]][2 * be], tile[to][a[1]][2 * bo + 1], tile[te][a[2]][2 * be],
+                               tile[to][a[3]][2 * bo + 1]);
+    *reinterpret_cast<uint4*>(dst + (((uint64_t)B << ash) | ((uint64_t)midd << (PM_T + 1)) | (4 * u))) = v;
   }
 }
 
 void permute_hc_brt(const uint32_t* src, uint32_t* dst, uint32_t log_n, hipStream_t s) {
   const uint32_t h = log_n ? log_n - 1 : 0;
-  if (log_n <= 2 * PM_T) {
+  const bool aligned = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0;
+  if (log_n <= 2 * PM_T || !aligned) {
     const uint32_t n = 1u << log_n;
     hipLaunchKernelGGL(k_permute_scatter, dim3((n + COL_THREADS - 1) / COL_THREADS), dim3(COL_THREADS), 0, s, src,
                        dst, n, h);
