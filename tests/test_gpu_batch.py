@@ -62,6 +62,22 @@ def test_bit_reverse_batch():
         assert np.array_equal(host(d), bitrev(v))
 
 
+def test_bit_reverse_batch_unaligned_columns():
+    # columns off 16-byte alignment take the 4-byte path, aligned ones in the
+    # same list the 16-byte path
+    rng = np.random.default_rng(13)
+    logs = [12, 10, 14]
+    vals = [rng.integers(0, po.P, size=1 << L, dtype=np.uint32) for L in logs]
+    pad_src, off = [1, 0, 1], [1, 0, 0]  # words before each column in its buffer
+    src = [dev(np.concatenate([[1] * p, v]).astype(np.uint32)) for p, v in zip(pad_src, vals)]
+    dst = [empty((1 << L) + 1) for L in logs]
+    sp = (C.c_void_p * 3)(*[b.ptr + 4 * p for b, p in zip(src, pad_src)])
+    dp = (C.c_void_p * 3)(*[b.ptr + 4 * o for b, o in zip(dst, off)])
+    vx._lib.call("vx_bit_reverse_m31_batch", sp, dp, u32s(logs), len(logs), None)
+    for v, d, o in zip(vals, dst, off):
+        assert np.array_equal(host(d)[o:o + v.size], bitrev(v))
+
+
 @pytest.mark.parametrize("logs", [[4], [3, 3, 4, 6], [2] * 7 + [4] * 9 + [6] * 16, [5, 6, 8, 8, 9, 12, 12, 13]])
 def test_lifted_leaves_and_root(logs):
     rng = np.random.default_rng(len(logs))

@@ -698,6 +698,7 @@ struct BitrevJob {
   const uint32_t* src;
   uint32_t* dst;
   uint32_t log_n;
+  uint32_t vec;  // both columns 16-byte aligned: 16 bytes per lane
   uint64_t first_tile;
 };
 // Columns of log size >= 12 move in 64 x 64 tiles (256-byte runs on both
@@ -734,6 +735,40 @@ __device__ __forceinline__ void br_tile(const uint32_t* src, uint32_t* dst, uint
   __syncthreads();  // the next tile's writes must not overtake these reads
 }
 
+// The same tile with 16 bytes per lane on both sides: W / 4 lanes per row,
+// and a lane writes destination words rev(a) = 4u .. 4u + 3 of its row,
+// taken from four tile rows a.
+template <uint32_t T>
+__device__ __forceinline__ void br_tile_vec(const uint32_t* src, uint32_t* dst, uint32_t E, uint64_t m,
+                                            uint32_t (*tile)[(1u << BR_BIG) + 1]) {
+  constexpr uint32_t W = 1u << T, LPR = W / 4, ROWS = 256 / LPR;
+  const uint32_t mb = E - 2 * T;
+  const uint64_t rm = mb ? (__brevll(m) >> (64 - mb)) : 0;
+  const uint32_t t = threadIdx.x, c4 = 4 * (t % LPR);
+  const uint64_t step = (uint64_t)ROWS << (E - T);
+  const uint32_t* sp = src + (((uint64_t)(t / LPR) << (E - T)) | (m << T) | c4);
+#pragma unroll
+  for (uint32_t r = 0; r < W / ROWS; r++) {
+    const uint4 v = *reinterpret_cast<const uint4*>(sp + r * step);
+    uint32_t* row = tile[(t / LPR) + ROWS * r] + c4;
+    row[0] = v.x;
+    row[1] = v.y;
+    row[2] = v.z;
+    row[3] = v.w;
+  }
+  __syncthreads();
+  uint32_t* dp = dst + (((uint64_t)(t / LPR) << (E - T)) | (rm << T) | c4);
+  uint32_t a[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) a[k] = __brev(c4 + k) >> (32 - T);
+#pragma unroll
+  for (uint32_t r = 0; r < W / ROWS; r++) {
+    const uint32_t b = __brev((t / LPR) + ROWS * r) >> (32 - T);
+    *reinterpret_cast<uint4*>(dp + r * step) = make_uint4(tile[a[0]][b], tile[a[1]][b], tile[a[2]][b], tile[a[3]][b]);
+  }
+  __syncthreads();  // the next tile's writes must not overtake these reads
+}
+
 // Workgroup w runs the contiguous tiles [w chunk, (w + 1) chunk) of the
 // whole list, so its job changes rarely and is found by one binary search
 // and then a forward scan (scalar loads).
@@ -753,8 +788,13 @@ __global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint
     while (lo + 1 < n_jobs && ld_uniform(&jobs[lo + 1].first_tile) <= b) lo++;
     uniform_t<BitrevJob>& J = as_uniform(jobs + lo);
     const uint32_t E = J.log_n;
-    if (E >= 2 * BR_BIG) br_tile<BR_BIG>(J.src, J.dst, E, b - J.first_tile, tile);
-    else br_tile<BR_SMALL>(J.src, J.dst, E, b - J.first_tile, tile);
+    if (J.vec) {
+      if (E >= 2 * BR_BIG) br_tile_vec<BR_BIG>(J.src, J.dst, E, b - J.first_tile, tile);
+      else br_tile_vec<BR_SMALL>(J.src, J.dst, E, b - J.first_tile, tile);
+    } else {
+      if (E >= 2 * BR_BIG) br_tile<BR_BIG>(J.src, J.dst, E, b - J.first_tile, tile);
+      else br_tile<BR_SMALL>(J.src, J.dst, E, b - J.first_tile, tile);
+    }
   }
 }
 
@@ -765,7 +805,8 @@ void bitrev_batch_build(const uint32_t* const* src, uint32_t* const* dst, const 
   for (uint32_t c = 0; c < n; c++) {
     require(log_n[c] >= BR_MIN_LOG && log_n[c] <= 31 && src[c] && dst[c] && src[c] != dst[c],
             "bitrev_batch: out-of-place columns of log size >= 10");
-    jobs.push_back({src[c], dst[c], log_n[c], tiles});
+    const bool vec = ((reinterpret_cast<uintptr_t>(src[c]) | reinterpret_cast<uintptr_t>(dst[c])) & 15u) == 0;
+    jobs.push_back({src[c], dst[c], log_n[c], vec ? 1u : 0u, tiles});
     tiles += 1ull << (log_n[c] - 2 * br_tbits(log_n[c]));
   }
   count = n;

@@ -69,6 +69,37 @@ __device__ __forceinline__ void b2_hash_x4(const uint32_t m[16], uint32_t t, uin
   hi = h_hi ^ b ^ d;
 }
 
+// One compression of a chained hash by the four lanes of a quad: lane q
+// holds chaining words q and 4 + q in h_lo / h_hi and updates them; t is the
+// byte count up to and including this block, last marks the final block.
+// All four lanes pass the same 16 message words. Same function as
+// b2_compress.
+__device__ __forceinline__ void b2_compress_x4(uint32_t& h_lo, uint32_t& h_hi, const uint32_t m[16], uint32_t t,
+                                               bool last, uint32_t q) {
+  const bool q0 = q & 1, q1 = q & 2;
+  uint32_t a = h_lo, b = h_hi;
+  uint32_t c = sel4(q0, q1, VX_B2_IV0, VX_B2_IV1, VX_B2_IV2, VX_B2_IV3);
+  uint32_t d = sel4(q0, q1, VX_B2_IV4, VX_B2_IV5, VX_B2_IV6, VX_B2_IV7) ^ (q == 0 ? t : 0u) ^
+               (q == 2 && last ? 0xFFFFFFFFu : 0u);  // v12 ^= t, v14 ^= final
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint32_t x = sel4(q0, q1, m[B2_SIGMA[r][0]], m[B2_SIGMA[r][2]], m[B2_SIGMA[r][4]], m[B2_SIGMA[r][6]]);
+    uint32_t y = sel4(q0, q1, m[B2_SIGMA[r][1]], m[B2_SIGMA[r][3]], m[B2_SIGMA[r][5]], m[B2_SIGMA[r][7]]);
+    VX_B2_G(a, b, c, d, x, y);
+    b = quad_rot(b, VX_DPP_ROT1);
+    c = quad_rot(c, VX_DPP_ROT2);
+    d = quad_rot(d, VX_DPP_ROT3);
+    x = sel4(q0, q1, m[B2_SIGMA[r][8]], m[B2_SIGMA[r][10]], m[B2_SIGMA[r][12]], m[B2_SIGMA[r][14]]);
+    y = sel4(q0, q1, m[B2_SIGMA[r][9]], m[B2_SIGMA[r][11]], m[B2_SIGMA[r][13]], m[B2_SIGMA[r][15]]);
+    VX_B2_G(a, b, c, d, x, y);
+    b = quad_rot(b, VX_DPP_ROT3);
+    c = quad_rot(c, VX_DPP_ROT2);
+    d = quad_rot(d, VX_DPP_ROT1);
+  }
+  h_lo ^= a ^ c;
+  h_hi ^= b ^ d;
+}
+
 // Merkle node hash (t = 64).
 __device__ __forceinline__ void b2_node_x4(const uint32_t m[16], uint32_t q, uint32_t& lo, uint32_t& hi) {
   b2_hash_x4(m, 64u, q, lo, hi);
