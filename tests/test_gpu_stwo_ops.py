@@ -50,6 +50,25 @@ def test_accumulate_numerators(n_rows, idx):
         assert np.array_equal(host(outs[k]), want[k]), k
 
 
+@pytest.mark.parametrize("n_rows", [1001, 4096])
+def test_accumulate_numerators_unaligned_column(n_rows):
+    # column 1 starts off 16-byte alignment (word loads inside the 4-row
+    # path); 1001 rows (not a multiple of 4) take the strided-row kernel
+    cols = [rnd(n_rows, 60 + c) for c in range(3)]
+    bufs = [dev(cols[0]), dev(np.concatenate([[5], cols[1]]).astype(np.uint32)), dev(cols[2])]
+    addrs = np.array([bufs[0].ptr, bufs[1].ptr + 4, bufs[2].ptr], np.uint64).view(np.uint32)
+    idx = [1, 0, 2, 1]
+    b, c = rnd(16, 61), rnd(16, 62)
+    outs = [empty(n_rows) for _ in range(4)]
+    pin = Pin()
+    call("vx_accumulate_numerators", pin(dev(addrs)), pin(dev(np.array(idx, np.uint32))), pin(dev(b)), pin(dev(c)),
+         len(idx), n_rows, *[ptr(o) for o in outs], NULL)
+    sync()
+    want = po.accumulate_numerators(cols, idx, b, c, n_rows)
+    for k in range(4):
+        assert np.array_equal(host(outs[k]), want[k]), k
+
+
 @pytest.mark.parametrize("n_rows", [256, 5000, 1 << 18])
 def test_accumulate_numerators_dual(n_rows):
     cols = [rnd(n_rows, 20 + c) for c in range(5)]

@@ -50,13 +50,19 @@ __device__ __forceinline__ void qn_reduce(QnAcc& s) {
 // Each step takes QN_GROUP columns: their indices, pointers and
 // coefficients are wave-uniform scalar loads issued together, then the
 // QN_GROUP x QN_ROWS column values are independent loads in flight at once.
-template <bool DUAL>
+// VEC: a lane takes 4 consecutive rows instead (n_rows a multiple of 4,
+// outputs 16-byte aligned): one 16-byte load per column when the column is
+// aligned (a wave-uniform check), one 16-byte store per output word.
+template <bool DUAL, bool VEC>
 __global__ __launch_bounds__(QN_THREADS) void k_accumulate_numerators(
     const uint32_t* const* cols, const uint32_t* idx_z, const uint32_t* zn_idx, const uint32_t* b_z,
     const uint32_t* c_z, const uint32_t* b_zn, const uint32_t* c_zn, uint32_t n_cols, uint32_t n_rows,
     uint32_t* oz0, uint32_t* oz1, uint32_t* oz2, uint32_t* oz3, uint32_t* on0, uint32_t* on1, uint32_t* on2,
     uint32_t* on3) {
-  const uint32_t row0 = blockIdx.x * (QN_THREADS * QN_ROWS) + threadIdx.x;
+  const uint32_t row0 = VEC ? QN_ROWS * (blockIdx.x * QN_THREADS + threadIdx.x)
+                           : blockIdx.x * (QN_THREADS * QN_ROWS) + threadIdx.x;
+  const uint32_t rstep = VEC ? 1u : QN_THREADS;  // rows of a lane: row0 + rr * rstep
+  if (VEC && row0 >= n_rows) return;
   QnAcc z{}, n{};
   for (uint32_t i0 = 0; i0 < n_cols; i0 += QN_GROUP) {
     const uint32_t cnt = n_cols - i0 < QN_GROUP ? n_cols - i0 : QN_GROUP;
@@ -87,12 +93,21 @@ __global__ __launch_bounds__(QN_THREADS) void k_accumulate_numerators(
     }
     uint32_t f[QN_GROUP][QN_ROWS];
 #pragma unroll
-    for (uint32_t g = 0; g < QN_GROUP; g++)
+    for (uint32_t g = 0; g < QN_GROUP; g++) {
+      if (VEC && g < cnt && (reinterpret_cast<uintptr_t>(col[g]) & 15u) == 0) {
+        const uint4 v = *reinterpret_cast<const uint4*>(col[g] + row0);
+        f[g][0] = v.x;
+        f[g][1] = v.y;
+        f[g][2] = v.z;
+        f[g][3] = v.w;
+        continue;
+      }
 #pragma unroll
       for (uint32_t rr = 0; rr < QN_ROWS; rr++) {
-        const uint32_t row = row0 + rr * QN_THREADS;
+        const uint32_t row = row0 + rr * rstep;
         f[g][rr] = (g < cnt && row < n_rows) ? col[g][row] : 0u;
       }
+    }
 #pragma unroll
     for (uint32_t g = 0; g < QN_GROUP; g++) {
       if (g >= cnt) break;
@@ -114,6 +129,19 @@ __global__ __launch_bounds__(QN_THREADS) void k_accumulate_numerators(
   }
   uint32_t* const oz[4] = {oz0, oz1, oz2, oz3};
   uint32_t* const on[4] = {on0, on1, on2, on3};
+  if constexpr (VEC) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      *reinterpret_cast<uint4*>(oz[k] + row0) =
+          make_uint4(m31_sub(z.r[0][k], z.b[k]), m31_sub(z.r[1][k], z.b[k]), m31_sub(z.r[2][k], z.b[k]),
+                     m31_sub(z.r[3][k], z.b[k]));
+      if constexpr (DUAL)
+        *reinterpret_cast<uint4*>(on[k] + row0) =
+            make_uint4(m31_sub(n.r[0][k], n.b[k]), m31_sub(n.r[1][k], n.b[k]), m31_sub(n.r[2][k], n.b[k]),
+                       m31_sub(n.r[3][k], n.b[k]));
+    }
+    return;
+  }
 #pragma unroll
   for (uint32_t rr = 0; rr < QN_ROWS; rr++) {
     const uint32_t row = row0 + rr * QN_THREADS;
@@ -133,12 +161,18 @@ void accumulate_numerators(const uint32_t* const* cols, const uint32_t* idx_z, c
   if (n_rows == 0) return;
   const uint32_t per_block = QN_THREADS * QN_ROWS;
   const dim3 grid((n_rows + per_block - 1) / per_block);
-  if (dual)
-    hipLaunchKernelGGL(k_accumulate_numerators<true>, grid, dim3(QN_THREADS), 0, s, cols, idx_z, zn_idx, b_z, c_z,
-                       b_zn, c_zn, n_cols, n_rows, oz[0], oz[1], oz[2], oz[3], on[0], on[1], on[2], on[3]);
-  else
-    hipLaunchKernelGGL(k_accumulate_numerators<false>, grid, dim3(QN_THREADS), 0, s, cols, idx_z, zn_idx, b_z, c_z,
-                       b_zn, c_zn, n_cols, n_rows, oz[0], oz[1], oz[2], oz[3], oz[0], oz[1], oz[2], oz[3]);
+  uintptr_t bits = 0;
+  for (int k = 0; k < 4; k++) bits |= reinterpret_cast<uintptr_t>(oz[k]) | (dual ? reinterpret_cast<uintptr_t>(on[k]) : 0);
+  const bool vec = n_rows % QN_ROWS == 0 && (bits & 15u) == 0;
+  uint32_t* const* o2 = dual ? on : oz;
+#define VX_QN_LAUNCH(D, V)                                                                                     \
+  hipLaunchKernelGGL((k_accumulate_numerators<D, V>), grid, dim3(QN_THREADS), 0, s, cols, idx_z, zn_idx, b_z, \
+                     c_z, b_zn, c_zn, n_cols, n_rows, oz[0], oz[1], oz[2], oz[3], o2[0], o2[1], o2[2], o2[3])
+  if (dual && vec) VX_QN_LAUNCH(true, true);
+  else if (dual) VX_QN_LAUNCH(true, false);
+  else if (vec) VX_QN_LAUNCH(false, true);
+  else VX_QN_LAUNCH(false, false);
+#undef VX_QN_LAUNCH
   VX_HIP(hipGetLastError());
 }
 
