@@ -102,9 +102,9 @@ void fold_twiddle_sources(cpt init, cpt step, uint32_t* out, uint64_t n, uint32_
 // (b = log_n - 2 - l) sit at [n/2 - 2^(b+1), n/2 - 2^b) and hold, at position
 // bitrev_b(j), the x of that coset's point j, which is pi^l(x(init * step^j))
 // with pi(x) = 2x^2 - 1 (doubling a point squares it). The last word is 1.
-__global__ void k_stwo_twiddles(cpt init, cpt step, uint32_t* out, uint32_t log_n) {
+__global__ void k_stwo_twiddles(cpt init, cpt step, uint32_t* out, uint32_t log_n, uint64_t i_from) {
   const uint64_t half = 1ull << (log_n - 1);
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i = i_from + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= half) return;
   if (i == half - 1) {
     out[i] = 1u;
@@ -121,12 +121,58 @@ __global__ void k_stwo_twiddles(cpt init, cpt step, uint32_t* out, uint32_t log_
   }
   out[i] = x;
 }
+// Levels of 2^b >= 16 words, 16 words per lane: lane g of level b writes
+// positions pos = g + m 2^(b-4), m < 16, whose bit reverses are
+// (rev_(b-4)(g) << 4) | rev_4(m). So the lane forms one point
+// init * (step^16)^rev(g) and takes each word's x from it and step^rev_4(m)
+// (a 16-entry table) with two multiplications, instead of a power of step
+// per word; for each m the lanes of a wave write consecutive words. Lane
+// u + 1 in [2^(b-4), 2^(b-3)) belongs to level b.
+struct StwoTwArgs {
+  cpt init, step16;
+  cpt s[16];  // step^k
+};
+__global__ void k_stwo_twiddles_levels(StwoTwArgs a, uint32_t* out, uint32_t log_n) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (u >= (1ull << (log_n - 5))) return;
+  const uint32_t b4 = 63u - (uint32_t)__clzll(u);  // b - 4
+  const uint64_t g = u - (1ull << b4);
+  const uint64_t jb = b4 ? (__brevll(g) >> (64 - b4)) : 0;
+  const cpt base = c_mul(a.init, c_pow(a.step16, jb));
+  const uint64_t half = 1ull << (log_n - 1);
+  uint32_t* o = out + (half - (2ull << (b4 + 4))) + g;
+  const uint32_t l = log_n - 6 - b4;  // doublings: log_n - 2 - b
+#pragma unroll
+  for (uint32_t m = 0; m < 16; m++) {
+    const cpt sm = a.s[((m & 1u) << 3) | ((m & 2u) << 1) | ((m & 4u) >> 1) | ((m & 8u) >> 3)];
+    uint32_t x = m31_sub(m31_mul(base.x, sm.x), m31_mul(base.y, sm.y));
+    for (uint32_t k = 0; k < l; k++) {
+      const uint32_t x2 = m31_mul(x, x);
+      x = m31_sub(m31_add(x2, x2), 1u);
+    }
+    o[(uint64_t)m << b4] = x;
+  }
+}
+
 void stwo_twiddles(uint32_t log_n, uint32_t* tw, uint32_t* itw, hipStream_t s) {
   require(log_n >= 2 && log_n <= 31, "stwo_twiddles: log_n must be 2..31");
   const coset c = make_coset(HALF_ODDS, log_n - 1);
   const uint64_t half = 1ull << (log_n - 1);
-  hipLaunchKernelGGL(k_stwo_twiddles, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, s, c.init, c.step, tw,
-                     log_n);
+  // levels 0..3 and the pad word (the last 16 words, or all of a small
+  // buffer) one word per lane
+  const uint64_t from = log_n >= 6 ? half - 16 : 0;
+  hipLaunchKernelGGL(k_stwo_twiddles, dim3((uint32_t)((half - from + 255) / 256)), dim3(256), 0, s, c.init, c.step,
+                     tw, log_n, from);
+  if (log_n >= 6) {
+    StwoTwArgs a{};
+    a.init = c.init;
+    a.s[0] = c_identity();
+    for (int k = 1; k < 16; k++) a.s[k] = c_mul(a.s[k - 1], c.step);
+    a.step16 = c_mul(a.s[15], c.step);
+    const uint64_t lanes = (1ull << (log_n - 5)) - 1;
+    hipLaunchKernelGGL(k_stwo_twiddles_levels, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, s, a, tw,
+                       log_n);
+  }
   VX_HIP(hipGetLastError());
   if (itw) batch_inverse(tw, itw, half, s);  // entries are x of points of order >= 8: never 0
 }
