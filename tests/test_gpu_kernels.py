@@ -75,7 +75,8 @@ def test_batch_inverse(n):
     assert np.array_equal(host(d), po.batch_inverse(x))
 
 
-@pytest.mark.parametrize("kind,log_n,use_y", [(po.HALF_COSET, 4, True), (po.HALF_COSET, 18, True),
+@pytest.mark.parametrize("kind,log_n,use_y", [(po.HALF_COSET, 4, True), (po.HALF_COSET, 5, False),
+                                              (po.HALF_ODDS, 6, True), (po.HALF_COSET, 18, True),
                                               (po.HALF_ODDS, 9, False), (po.HALF_ODDS, 20, False),
                                               (po.SUBGROUP, 12, True)])
 def test_fold_twiddle_sources(kind, log_n, use_y):

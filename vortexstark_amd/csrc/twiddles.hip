@@ -88,12 +88,50 @@ __global__ void k_fold_tw_sources(cpt init, cpt step, uint32_t* out, uint64_t ha
   cpt p = c_mul(init, c_pow(step, idx));
   out[i] = use_y ? p.y : p.x;
 }
+// The same for h = log_n - 1 >= 4, 16 words per lane: bitrev_log_n(2i) is
+// rev_h(i), and for i = g + m 2^(h-4) (m < 16) that is
+// (rev_(h-4)(g) << 4) | rev_4(m): one power of step^16 per lane, then two
+// multiplications per word with a 16-entry table of step^k; for each m the
+// lanes of a wave write consecutive words.
+struct StepTable {
+  cpt init, step16;
+  cpt s[16];  // step^k
+};
+__global__ void k_fold_tw_sources16(StepTable a, uint32_t* out, uint32_t h, int use_y) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t hb = h - 4;
+  if (g >= (1ull << hb)) return;
+  const uint64_t jb = hb ? (__brevll(g) >> (64 - hb)) : 0;
+  const cpt base = c_mul(a.init, c_pow(a.step16, jb));
+#pragma unroll
+  for (uint32_t m = 0; m < 16; m++) {
+    const cpt sm = a.s[((m & 1u) << 3) | ((m & 2u) << 1) | ((m & 4u) >> 1) | ((m & 8u) >> 3)];
+    out[g + ((uint64_t)m << hb)] = use_y ? m31_add(m31_mul(base.x, sm.y), m31_mul(base.y, sm.x))
+                                         : m31_sub(m31_mul(base.x, sm.x), m31_mul(base.y, sm.y));
+  }
+}
+
+static StepTable step_table(cpt init, cpt step) {
+  StepTable a{};
+  a.init = init;
+  a.s[0] = c_identity();
+  for (int k = 1; k < 16; k++) a.s[k] = c_mul(a.s[k - 1], step);
+  a.step16 = c_mul(a.s[15], step);
+  return a;
+}
+
 void fold_twiddle_sources(cpt init, cpt step, uint32_t* out, uint64_t n, uint32_t log_n, bool extract_y,
                           hipStream_t s) {
   require(log_n >= 1 && log_n <= 31 && n == (1ull << log_n), "fold_twiddle_sources: n must be 2^log_n");
   uint64_t half = n / 2;
-  hipLaunchKernelGGL(k_fold_tw_sources, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, s, init, step, out,
-                     half, log_n, extract_y ? 1 : 0);
+  if (log_n >= 5) {
+    const uint64_t lanes = 1ull << (log_n - 5);
+    hipLaunchKernelGGL(k_fold_tw_sources16, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, s,
+                       step_table(init, step), out, log_n - 1, extract_y ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL(k_fold_tw_sources, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, s, init, step, out,
+                       half, log_n, extract_y ? 1 : 0);
+  }
   VX_HIP(hipGetLastError());
 }
 
