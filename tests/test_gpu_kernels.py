@@ -40,6 +40,18 @@ def test_fibonacci_trace(log_n, a, b):
     assert np.array_equal(host(out), po.fibonacci_trace(a, b, log_n))
 
 
+@pytest.mark.parametrize("n", [1, 48, 1000, 4096 + 16])
+def test_coset_points_any_length(n):
+    # lengths that are not powers of two: multiples of 16 take the 16-per-lane
+    # kernel, the others the run-per-lane one
+    c = po.coset(po.HALF_ODDS, 14)
+    xs, ys = empty(n), empty(n)
+    call("vx_compute_coset_points", int(c[0]), int(c[1]), int(c[2]), int(c[3]), ptr(xs), ptr(ys), n, NULL)
+    hx, hy = host(xs), host(ys)
+    for i in sorted({0, min(1, n - 1), n // 3, n // 2, n - 1}):
+        assert (int(hx[i]), int(hy[i])) == po.coset_at(c, i), i
+
+
 @pytest.mark.parametrize("kind,log_n", [(po.HALF_COSET, 3), (po.HALF_COSET, 11), (po.HALF_ODDS, 10),
                                         (po.SUBGROUP, 12), (po.HALF_COSET, 20)])
 def test_coset_points_and_squash(kind, log_n):

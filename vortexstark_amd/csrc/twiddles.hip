@@ -23,11 +23,33 @@ __global__ void k_coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, u
     p = c_mul(p, step);
   }
 }
+// n a multiple of 16, G = n / 16 lanes: lane g writes points g + m G,
+// m < 16, stepping by step^G, so for each m the lanes of a wave write
+// consecutive words (the kernel above writes runs of PTS_PER_THREAD words
+// per lane, PTS_PER_THREAD words apart).
+__global__ void k_coset_points16(cpt init, cpt step, cpt step_g, uint32_t* xs, uint32_t* ys, uint64_t g_lanes) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g_lanes) return;
+  cpt p = c_mul(init, c_pow(step, g));
+#pragma unroll
+  for (uint32_t m = 0; m < 16; m++) {
+    const uint64_t i = g + m * g_lanes;
+    if (xs) xs[i] = p.x;
+    if (ys) ys[i] = p.y;
+    p = c_mul(p, step_g);
+  }
+}
 void coset_points(cpt init, cpt step, uint32_t* xs, uint32_t* ys, uint64_t n, hipStream_t s) {
   if (n == 0) return;
-  uint64_t threads = (n + PTS_PER_THREAD - 1) / PTS_PER_THREAD;
-  hipLaunchKernelGGL(k_coset_points, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, init, step, xs,
-                     ys, n);
+  if (n % 16 == 0) {
+    const uint64_t g_lanes = n / 16;
+    hipLaunchKernelGGL(k_coset_points16, dim3((uint32_t)((g_lanes + 255) / 256)), dim3(256), 0, s, init, step,
+                       c_pow(step, g_lanes), xs, ys, g_lanes);
+  } else {
+    uint64_t threads = (n + PTS_PER_THREAD - 1) / PTS_PER_THREAD;
+    hipLaunchKernelGGL(k_coset_points, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, init, step, xs,
+                       ys, n);
+  }
   VX_HIP(hipGetLastError());
 }
 
