@@ -69,18 +69,20 @@ void extract_and_squash(const uint32_t* in, uint32_t* tw, uint32_t* sq, uint64_t
   VX_HIP(hipGetLastError());
 }
 
-// Batch inverse, 16 elements per thread: prefix products over the non-zero
-// entries, one Fermat inversion, then unwind. Zeros map to zero.
+// Batch inverse, 16 elements per lane: prefix products over the non-zero
+// entries, one Fermat inversion, then unwind. Zeros map to zero. With
+// G = ceil(n / 16) lanes, lane g takes elements g + m G (m < 16), so for
+// each m the lanes of a wave read and write consecutive words.
 static constexpr uint32_t INV_CHUNK = 16;
-__global__ void k_batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n) {
-  uint64_t start = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * INV_CHUNK;
-  if (start >= n) return;
-  uint32_t cnt = (uint32_t)(n - start < INV_CHUNK ? n - start : INV_CHUNK);
+__global__ void k_batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t g_lanes) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g_lanes) return;
   uint32_t v[INV_CHUNK], pre[INV_CHUNK];
   uint32_t acc = 1;
 #pragma unroll
   for (uint32_t i = 0; i < INV_CHUNK; i++) {
-    v[i] = i < cnt ? in[start + i] : 0u;
+    const uint64_t e = g + i * g_lanes;
+    v[i] = e < n ? in[e] : 0u;
     pre[i] = acc;  // product of the non-zero entries before i
     if (v[i] != 0) acc = m31_mul(acc, v[i]);
   }
@@ -92,13 +94,15 @@ __global__ void k_batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n) {
       r = m31_mul(inv, pre[i]);
       inv = m31_mul(inv, v[i]);
     }
-    if ((uint32_t)i < cnt) out[start + i] = r;
+    const uint64_t e = g + (uint64_t)i * g_lanes;
+    if (e < n) out[e] = r;
   }
 }
 void batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s) {
   if (n == 0) return;
-  uint64_t threads = (n + INV_CHUNK - 1) / INV_CHUNK;
-  hipLaunchKernelGGL(k_batch_inverse, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, in, out, n);
+  const uint64_t g_lanes = (n + INV_CHUNK - 1) / INV_CHUNK;
+  hipLaunchKernelGGL(k_batch_inverse, dim3((uint32_t)((g_lanes + 255) / 256)), dim3(256), 0, s, in, out, n,
+                     g_lanes);
   VX_HIP(hipGetLastError());
 }
 
