@@ -75,17 +75,30 @@ int vx_circle_ntt_interpolate(uint32_t *d_data, const uint32_t *d_itwiddles, con
                               uint32_t n_line_layers, uint32_t n, vx_stream_t stream);
 
 /* Replaces cuda_circle_ntt_evaluate_batch / _interpolate_batch
- * (ffi.rs:281-301; the default FORGE variants ffi.rs:407-428 take a HOST
- * array of column pointers, as these do). Columns of equal size n, 16-byte
+ * (ffi.rs:281-301, callers src/ntt.rs:226-240,270-281): d_columns is a
+ * DEVICE array of n_cols device column pointers, as the reference uploads it
+ * (a host array is accepted too: the pointers are read back with the copy
+ * direction inferred from the address). Columns of equal size n, 16-byte
  * aligned; each pass is one launch over all columns. Synchronous. */
-int vx_circle_ntt_evaluate_batch(uint32_t *const *h_columns, const uint32_t *d_twiddles,
+int vx_circle_ntt_evaluate_batch(uint32_t *const *d_columns, const uint32_t *d_twiddles,
                                  const uint32_t *d_circle_twids, const uint32_t *h_layer_offsets,
                                  const uint32_t *h_layer_sizes, uint32_t n_line_layers, uint32_t n,
                                  uint32_t n_cols, vx_stream_t stream);
-int vx_circle_ntt_interpolate_batch(uint32_t *const *h_columns, const uint32_t *d_itwiddles,
+int vx_circle_ntt_interpolate_batch(uint32_t *const *d_columns, const uint32_t *d_itwiddles,
                                     const uint32_t *d_circle_itwids, const uint32_t *h_layer_offsets,
                                     const uint32_t *h_layer_sizes, uint32_t n_line_layers, uint32_t n,
                                     uint32_t n_cols, vx_stream_t stream);
+/* Replace cuda_circle_ntt_evaluate_batch_forge / _interpolate_batch_forge
+ * (ffi.rs:407-428, the reference's default-feature build, src/ntt.rs:
+ * 208-222): the same transforms over a HOST array of device column pointers. */
+int vx_circle_ntt_evaluate_batch_host(uint32_t *const *h_columns, const uint32_t *d_twiddles,
+                                      const uint32_t *d_circle_twids, const uint32_t *h_layer_offsets,
+                                      const uint32_t *h_layer_sizes, uint32_t n_line_layers, uint32_t n,
+                                      uint32_t n_cols, vx_stream_t stream);
+int vx_circle_ntt_interpolate_batch_host(uint32_t *const *h_columns, const uint32_t *d_itwiddles,
+                                         const uint32_t *d_circle_itwids, const uint32_t *h_layer_offsets,
+                                         const uint32_t *h_layer_sizes, uint32_t n_line_layers, uint32_t n,
+                                         uint32_t n_cols, vx_stream_t stream);
 
 /* Replaces cuda_circle_ntt_layer (ffi.rs:303-309, circle_ntt.cu:405-417):
  * one butterfly layer with stride 2^layer_idx (0 = circle layer). Async. */
@@ -352,11 +365,17 @@ int vx_fold_circle_into_line_soa(uint32_t *d_dst0, uint32_t *d_dst1, uint32_t *d
 
 /* --------------------------------------------------------------- Merkle */
 
-/* Replaces cuda_merkle_hash_leaves (ffi.rs:777-782, cuda/blake2s.cu:13-61,536-545):
- * leaf = Blake2s(min(n_cols,16) LE words). h_columns is a HOST array of
- * device column pointers (the reference uploads the same array first). */
-int vx_merkle_hash_leaves(const uint32_t *const *h_columns, uint32_t *d_hashes, uint32_t n_cols,
+/* Replaces cuda_merkle_hash_leaves (ffi.rs:777-782, cuda/blake2s.cu:13-61,536-545;
+ * callers src/merkle.rs:32-34,60,146-158): leaf = Blake2s(min(n_cols,16) LE
+ * words). d_columns is a DEVICE array of device column pointers, as the
+ * reference uploads it (a host array is accepted too). Reading the pointers
+ * back waits for the stream's earlier work; the hashing is asynchronous. */
+int vx_merkle_hash_leaves(const uint32_t *const *d_columns, uint32_t *d_hashes, uint32_t n_cols,
                           uint32_t n_leaves, vx_stream_t stream);
+/* The same leaves from a HOST array of device column pointers, fully
+ * asynchronous. No reference counterpart (the reference always uploads). */
+int vx_merkle_hash_leaves_host(const uint32_t *const *h_columns, uint32_t *d_hashes, uint32_t n_cols,
+                               uint32_t n_leaves, vx_stream_t stream);
 /* Replace cuda_merkle_hash_leaves_forge_single / _quad (ffi.rs:797-815). */
 int vx_merkle_hash_leaves_single(const uint32_t *d_col, uint32_t *d_hashes, uint32_t n_leaves, vx_stream_t stream);
 int vx_merkle_hash_leaves_quad(const uint32_t *d_c0, const uint32_t *d_c1, const uint32_t *d_c2,
@@ -376,12 +395,19 @@ int vx_merkle_hash_leaves_merge_soa4(const uint32_t *d_c0, const uint32_t *d_c1,
                                      const uint32_t *d_c3, uint32_t *d_parents, uint32_t n_leaves,
                                      vx_stream_t stream);
 /* Replace cuda_merkle_tiled_soa4[_stream] / cuda_merkle_tiled_generic
- * (ffi.rs:844-864, blake2s.cu:305-532,600-644): the root of every 1024-leaf
- * tile, n_leaves a multiple of 1024. */
+ * (ffi.rs:844-864, blake2s.cu:305-532,600-644; callers src/merkle.rs:
+ * 398-407): the root of every 1024-leaf tile, n_leaves any positive multiple
+ * of 1024. vx_merkle_tiled_generic takes a DEVICE array of device column
+ * pointers, as the reference does (a host array is accepted too; reading the
+ * pointers back waits for the stream's earlier work). */
 int vx_merkle_tiled_soa4(const uint32_t *d_c0, const uint32_t *d_c1, const uint32_t *d_c2, const uint32_t *d_c3,
                          uint32_t *d_subtree_roots, uint32_t n_leaves, vx_stream_t stream);
-int vx_merkle_tiled_generic(const uint32_t *const *h_columns, uint32_t *d_subtree_roots, uint32_t n_cols,
+int vx_merkle_tiled_generic(const uint32_t *const *d_columns, uint32_t *d_subtree_roots, uint32_t n_cols,
                             uint32_t n_leaves, vx_stream_t stream);
+/* The same tile roots from a HOST array of device column pointers, fully
+ * asynchronous. No reference counterpart (the reference always uploads). */
+int vx_merkle_tiled_generic_host(const uint32_t *const *h_columns, uint32_t *d_subtree_roots, uint32_t n_cols,
+                                 uint32_t n_leaves, vx_stream_t stream);
 
 /* ------------------------------------------------------------------ PoW */
 
@@ -573,6 +599,11 @@ int vx_server_wait(vx_server *s, uint64_t ticket, size_t *out_len, double times[
 /* 1 if the ticket has completed, 0 if not, < 0 on error. */
 int vx_server_done(vx_server *s, uint64_t ticket);
 uint64_t vx_server_resident_bytes(const vx_server *s);
+/* Scheduler counters: out[0] provers created, out[1] provers evicted,
+ * out[2] admissions postponed until a running proof or a proof copy-out
+ * released memory. No reference counterpart (the reference keeps no
+ * resident provers). */
+int vx_server_stats(const vx_server *s, uint64_t out[3]);
 /* Replay an arrival trace: request i is submitted arrival_s[i] seconds
  * (sorted) after the call starts; returns when all completed. times[3i..3i+2]
  * = submit, start, completion, in seconds from the call's start. */

@@ -10,6 +10,7 @@ import pytest
 import pyoracle as po
 import vortexstark_amd as vx
 from vortexstark_amd import _lib
+from krak import Krak
 from test_oracle import tamper_cases
 
 
@@ -46,6 +47,32 @@ def test_host_verifier_rejects_tampering():
     for name, bad in tamper_cases(proof).items():
         ok, why = vx.verify(bad)
         assert not ok and why, name
+
+
+@pytest.mark.parametrize("extra", [1, 2, 3])
+def test_host_verifier_rejects_extra_fri_layers(extra):
+    # Appending copies of the last FRI decommitment (no root covers them)
+    # must be rejected, not read past the drawn FRI alphas.
+    proof = po.prove(1, 1, 6)
+    k = Krak(proof)
+    last = k.fri[-1]
+    start = last["values"] - 1
+    words = list(k.words)
+    words[k.off_n_fri_decom] += extra
+    words += words[start:] * extra
+    bad = struct.pack(f"<{len(words)}I", *words)
+    ok, why = vx.verify(bad)
+    assert not ok and "FRI decommitments" in why, why
+
+
+def test_host_verifier_rejects_missing_fri_layers():
+    proof = po.prove(1, 1, 6)
+    k = Krak(proof)
+    start = k.fri[-1]["values"] - 1
+    words = list(k.words[:start])
+    words[k.off_n_fri_decom] -= 1
+    ok, why = vx.verify(struct.pack(f"<{len(words)}I", *words))
+    assert not ok and why
 
 
 def test_host_verifier_rejects_wrong_params_and_bad_pow():

@@ -138,6 +138,34 @@ def test_ntt_interpolate(log_n):
     assert np.array_equal(host(d), po.interpolate(v, c))
 
 
+@pytest.mark.parametrize("log_n,n_cols", [(2, 3), (5, 1), (10, 7), (16, 4), (20, 2)])
+@pytest.mark.parametrize("variant", ["device_array", "host_array"])
+def test_ntt_batch_evaluate_and_interpolate(log_n, n_cols, variant):
+    # cuda_circle_ntt_evaluate_batch / _interpolate_batch (ffi.rs:281-301,
+    # device pointer array) and the FORGE entry points (ffi.rs:407-428, host
+    # array): every column equals the oracle's single-column transform
+    c = po.coset(po.HALF_COSET, log_n)
+    n = 1 << log_n
+    vals = [rnd(n, 900 + 31 * log_n + k) for k in range(n_cols)]
+    cols = [dev(v) for v in vals]
+    suffix = "" if variant == "device_array" else "_host"
+    arr = _dev_ptr_array(cols) if variant == "device_array" else None
+    colptr = ptr(arr) if arr is not None else _col_ptr_array(cols)
+    fl, fc, offs, sizes = _tables(c)
+    call("vx_circle_ntt_evaluate_batch" + suffix, colptr, ptr(fl), ptr(fc), hp(offs), hp(sizes), log_n - 1, n,
+         n_cols, NULL)
+    for d, v in zip(cols, vals):
+        assert np.array_equal(host(d), po.evaluate(v, c))
+    il, ic, ioffs, isizes = _itables(c)
+    evals = [host(d) for d in cols]
+    call("vx_circle_ntt_interpolate_batch" + suffix, colptr, ptr(il), ptr(ic), hp(ioffs), hp(isizes), log_n - 1,
+         n, n_cols, NULL)
+    for d, e, v in zip(cols, evals, vals):
+        got = host(d)
+        assert np.array_equal(got, po.interpolate(e, c))
+        assert np.array_equal(got, v)
+
+
 def test_ntt_roundtrip_2p26():
     # size-independent property at the LDE size of the log_n = 24 proof
     log_n = 26
@@ -237,17 +265,45 @@ def _col_ptr_array(dcols):
     return (C.c_void_p * len(dcols))(*[t.data_ptr() for t in dcols])
 
 
+def _dev_ptr_array(dcols):
+    """The column pointers uploaded to a device array, as the reference's
+    callers do before cuda_merkle_hash_leaves / cuda_merkle_tiled_generic
+    (src/merkle.rs:32-34): 64-bit words in device memory."""
+    words = np.array([t.data_ptr() for t in dcols], dtype=np.uint64).view(np.uint32)
+    return dev(words)
+
+
 @pytest.mark.parametrize("ncols,n", [(1, 64), (4, 1000), (7, 4096), (16, 256), (19, 128)])
 def test_merkle_leaves_and_nodes(ncols, n):
     cols = [rnd(n, ncols * 10 + k) for k in range(ncols)]
     dcols = [dev(c) for c in cols]
-    hashes = empty(8 * n)
-    call("vx_merkle_hash_leaves", _col_ptr_array(dcols), ptr(hashes), ncols, n, NULL)
     want = po.leaf_hashes(cols[:16])
+    darr = _dev_ptr_array(dcols)
+    hashes = empty(8 * n)
+    call("vx_merkle_hash_leaves", ptr(darr), ptr(hashes), ncols, n, NULL)
+    sync()
     assert np.array_equal(host(hashes).reshape(-1, 8), want)
+    hashes2 = empty(8 * n)
+    call("vx_merkle_hash_leaves_host", _col_ptr_array(dcols), ptr(hashes2), ncols, n, NULL)
+    sync()
+    assert np.array_equal(host(hashes2).reshape(-1, 8), want)
     par = empty(8 * (n // 2))
     call("vx_merkle_hash_nodes", ptr(hashes), ptr(par), n // 2, NULL)
     assert np.array_equal(host(par).reshape(-1, 8), po.hash_nodes(want))
+
+
+@pytest.mark.parametrize("n", [1, 64, 1000, 1 << 16])
+def test_merkle_hash_leaves_single_and_quad(n):
+    # cuda_merkle_hash_leaves_forge_single / _quad (ffi.rs:797-815): the
+    # 1- and 4-column leaf rules of cuda_merkle_hash_leaves
+    cols = [rnd(n, 700 + n + k) for k in range(4)]
+    dcols = [dev(c) for c in cols]
+    h1, h4 = empty(8 * n), empty(8 * n)
+    call("vx_merkle_hash_leaves_single", ptr(dcols[0]), ptr(h1), n, NULL)
+    call("vx_merkle_hash_leaves_quad", *[ptr(c) for c in dcols], ptr(h4), n, NULL)
+    sync()
+    assert np.array_equal(host(h1).reshape(-1, 8), po.leaf_hashes(cols[:1]))
+    assert np.array_equal(host(h4).reshape(-1, 8), po.leaf_hashes(cols))
 
 
 @pytest.mark.parametrize("n_nodes", [1, 2, 64, 512, 1024])
@@ -273,20 +329,44 @@ def test_merkle_small_soa4_and_merge(n):
     assert np.array_equal(host(par).reshape(-1, 8), po.hash_nodes(po.leaf_hashes(cols)))
 
 
-@pytest.mark.parametrize("ncols,log_n", [(1, 10), (1, 16), (4, 12), (4, 18), (3, 11)])
-def test_merkle_tiled(ncols, log_n):
-    n = 1 << log_n
-    cols = [rnd(n, 400 + log_n + k) for k in range(ncols)]
+def _tile_roots(cols, n):
+    """root of every 1024-leaf tile (the oracle's tree, tile by tile)"""
+    out = []
+    for t in range(n // 1024):
+        level = po.leaf_hashes([c[1024 * t:1024 * (t + 1)] for c in cols])
+        for _ in range(10):
+            level = po.hash_nodes(level)
+        out.append(level[0])
+    return np.array(out, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("ncols,n", [(1, 1 << 10), (1, 1 << 16), (4, 1 << 12), (4, 1 << 18), (3, 1 << 11),
+                                     (2, 3 * 1024), (5, 7 * 1024), (4, 5 * 1024), (17, 13 * 1024)])
+def test_merkle_tiled(ncols, n):
+    # cuda_merkle_tiled_generic accepts any multiple of 1024 leaves
+    # (blake2s.cu:631-643: n_leaves / 1024 blocks)
+    cols = [rnd(n, 400 + n + k) for k in range(ncols)]
     dcols = [dev(c) for c in cols]
-    roots = empty(8 * (n // 1024))
-    if ncols == 4:
-        call("vx_merkle_tiled_soa4", *[ptr(c) for c in dcols], ptr(roots), n, NULL)
-    else:
-        call("vx_merkle_tiled_generic", _col_ptr_array(dcols), ptr(roots), ncols, n, NULL)
-    level = po.leaf_hashes(cols)
-    for _ in range(10):
-        level = po.hash_nodes(level)
-    assert np.array_equal(host(roots).reshape(-1, 8), level)
+    want = _tile_roots(cols[:16], n)
+    variants = ["generic", "generic_host"] + (["soa4"] if ncols == 4 else [])
+    darr = _dev_ptr_array(dcols)
+    for v in variants:
+        roots = empty(8 * (n // 1024))
+        if v == "soa4":
+            call("vx_merkle_tiled_soa4", *[ptr(c) for c in dcols], ptr(roots), n, NULL)
+        elif v == "generic":
+            call("vx_merkle_tiled_generic", ptr(darr), ptr(roots), ncols, n, NULL)
+        else:
+            call("vx_merkle_tiled_generic_host", _col_ptr_array(dcols), ptr(roots), ncols, n, NULL)
+        sync()
+        assert np.array_equal(host(roots).reshape(-1, 8), want), v
+
+
+def test_merkle_tiled_rejects_partial_tiles():
+    d = dev(rnd(2048, 1))
+    roots = empty(16)
+    with pytest.raises(vx.VortexError, match="multiple of 1024"):
+        call("vx_merkle_tiled_generic_host", _col_ptr_array([d]), ptr(roots), 1, 1536, NULL)
 
 
 def _tree_levels(leaves):

@@ -97,6 +97,29 @@ def test_server_evicts_to_fit_budget():
     srv.close()
 
 
+def test_server_two_slots_tight_budget_alternating():
+    # Regression test of the eviction / copy-out order: two slots and a budget
+    # that holds one 2^16 class, classes alternating, all queued at once, so
+    # proofs finish while a request of a new class waits for memory. A prover
+    # whose finished proof is not yet copied out must not be evicted.
+    one = vx.Prover(16)
+    budget = int(2.2 * one.device_bytes)
+    one.close()
+    srv = vx.Server(n_slots=2, mem_budget=budget)
+    classes = [(16, 0), (14, 0), (16, 6), (14, 6)]
+    reqs = [vx.request(ln, a=i + 1, b=3 * i + 2, pow_bits=pw) for i, (ln, pw) in enumerate(classes * 4)]
+    tickets = [srv.submit(r) for r in reqs]
+    got = [srv.wait(t)[0] for t in tickets]
+    st = srv.stats
+    resident = srv.resident_bytes
+    srv.close()
+    print("server stats", st, "resident", resident, "budget", budget, flush=True)
+    assert st["evicted"] > 0
+    assert resident <= budget
+    for r, proof in zip(reqs, got):
+        assert proof == alone(r.log_n, r.a, r.b, r.blowup_bits, r.pow_bits), (r.log_n, r.a, r.pow_bits)
+
+
 def test_server_rejects_what_cannot_fit():
     srv = vx.Server(n_slots=1, mem_budget=1 << 20)
     t = srv.submit(vx.request(16))

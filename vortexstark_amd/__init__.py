@@ -329,6 +329,14 @@ class Server:
     def resident_bytes(self):
         return int(load().vx_server_resident_bytes(self._h))
 
+    @property
+    def stats(self):
+        """{'created', 'evicted', 'deferred'}: provers built, provers evicted,
+        admissions postponed until a running proof or a copy-out freed memory."""
+        out = (C.c_uint64 * 3)()
+        _lib.call("vx_server_stats", self._h, out)
+        return {"created": out[0], "evicted": out[1], "deferred": out[2]}
+
     def close(self):
         if self._h:
             _lib.call("vx_server_destroy", self._h)

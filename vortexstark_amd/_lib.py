@@ -33,6 +33,8 @@ SIGNATURES = {
     "vx_circle_ntt_interpolate": (ip, [vp, vp, vp, vp, vp, u32, u32, vp]),
     "vx_circle_ntt_evaluate_batch": (ip, [vp, vp, vp, vp, vp, u32, u32, u32, vp]),
     "vx_circle_ntt_interpolate_batch": (ip, [vp, vp, vp, vp, vp, u32, u32, u32, vp]),
+    "vx_circle_ntt_evaluate_batch_host": (ip, [vp, vp, vp, vp, vp, u32, u32, u32, vp]),
+    "vx_circle_ntt_interpolate_batch_host": (ip, [vp, vp, vp, vp, vp, u32, u32, u32, vp]),
     "vx_circle_ntt_layer": (ip, [vp, vp, u32, u32, ip, vp]),
     "vx_circle_ntt_evaluate_lde": (ip, [vp, vp, u32, vp, vp, vp, u32, u32, vp]),
     "vx_bit_reverse_m31": (ip, [vp, u32, vp]),
@@ -49,6 +51,7 @@ SIGNATURES = {
     "vx_fold_line_soa": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp]),
     "vx_fold_circle_into_line_soa": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp]),
     "vx_merkle_hash_leaves": (ip, [vp, vp, u32, u32, vp]),
+    "vx_merkle_hash_leaves_host": (ip, [vp, vp, u32, u32, vp]),
     "vx_merkle_hash_leaves_single": (ip, [vp, vp, u32, vp]),
     "vx_merkle_hash_leaves_quad": (ip, [vp, vp, vp, vp, vp, u32, vp]),
     "vx_merkle_hash_nodes": (ip, [vp, vp, u32, vp]),
@@ -57,6 +60,7 @@ SIGNATURES = {
     "vx_merkle_hash_leaves_merge_soa4": (ip, [vp, vp, vp, vp, vp, u32, vp]),
     "vx_merkle_tiled_soa4": (ip, [vp, vp, vp, vp, vp, u32, vp]),
     "vx_merkle_tiled_generic": (ip, [vp, vp, u32, u32, vp]),
+    "vx_merkle_tiled_generic_host": (ip, [vp, vp, u32, u32, vp]),
     "vx_grind_pow": (ip, [vp, vp, u32, u64, u32, vp]),
     "vx_grind_pow_min": (ip, [vp, u32, vp, vp]),
     "vx_proof_size": (sz, [u32, u32, u32, u32]),
@@ -131,6 +135,7 @@ SIGNATURES = {
     "vx_server_wait": (ip, [vp, u64, C.POINTER(sz), C.POINTER(C.c_double)]),
     "vx_server_done": (ip, [vp, u64]),
     "vx_server_resident_bytes": (u64, [vp]),
+    "vx_server_stats": (ip, [vp, C.POINTER(u64)]),
     "vx_server_run_trace": (ip, [vp, vp, C.POINTER(C.c_double), u32, C.POINTER(vp), C.POINTER(sz),
                                  C.POINTER(sz), C.POINTER(C.c_double)]),
 }

@@ -147,6 +147,20 @@ int vx_circle_ntt_interpolate(uint32_t* d, const uint32_t* tw, const uint32_t* c
   });
 }
 
+// Column-pointer arrays in the reference's batched calls are device memory
+// the caller uploaded first (src/merkle.rs:32-34, src/ntt.rs:226-240). The
+// pointers become kernel arguments here, so they are read back to the host:
+// after the stream's earlier work (which may have written them), with the
+// direction inferred from the address, so a host array is accepted too.
+static std::vector<uint32_t*> fetch_ptrs(const void* arr, uint32_t n, hipStream_t s) {
+  require(arr != nullptr, "null column-pointer array");
+  std::vector<uint32_t*> h(n);
+  if (n == 0) return h;
+  VX_HIP(hipMemcpyAsync(h.data(), arr, sizeof(void*) * n, hipMemcpyDefault, s));
+  VX_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
 // Columns of a batched in-place transform: the host pointer array is checked
 // and copied to the device, in chunks of at most 65535 columns (one grid row
 // each), and every pass runs over a whole chunk in one launch. Returns after
@@ -171,21 +185,43 @@ static void ntt_cols(uint32_t* const* cols, uint32_t n_cols, const TwiddleRef& r
   }
 }
 
-int vx_circle_ntt_evaluate_batch(uint32_t* const* cols, const uint32_t* tw, const uint32_t* ctw,
+int vx_circle_ntt_evaluate_batch(uint32_t* const* d_cols, const uint32_t* tw, const uint32_t* ctw,
                                  const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
                                  uint32_t n_cols, vx_stream_t st) {
   return guard([&] {
     TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
-    ntt_cols(cols, n_cols, r, true, S(st));
+    if (n_cols == 0) return;
+    std::vector<uint32_t*> cols = fetch_ptrs(d_cols, n_cols, S(st));
+    ntt_cols(cols.data(), n_cols, r, true, S(st));
   });
 }
 
-int vx_circle_ntt_interpolate_batch(uint32_t* const* cols, const uint32_t* tw, const uint32_t* ctw,
+int vx_circle_ntt_interpolate_batch(uint32_t* const* d_cols, const uint32_t* tw, const uint32_t* ctw,
                                     const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
                                     uint32_t n_cols, vx_stream_t st) {
   return guard([&] {
     TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
-    ntt_cols(cols, n_cols, r, false, S(st));
+    if (n_cols == 0) return;
+    std::vector<uint32_t*> cols = fetch_ptrs(d_cols, n_cols, S(st));
+    ntt_cols(cols.data(), n_cols, r, false, S(st));
+  });
+}
+
+int vx_circle_ntt_evaluate_batch_host(uint32_t* const* h_cols, const uint32_t* tw, const uint32_t* ctw,
+                                      const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
+                                      uint32_t n_cols, vx_stream_t st) {
+  return guard([&] {
+    TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
+    ntt_cols(h_cols, n_cols, r, true, S(st));
+  });
+}
+
+int vx_circle_ntt_interpolate_batch_host(uint32_t* const* h_cols, const uint32_t* tw, const uint32_t* ctw,
+                                         const uint32_t* offs, const uint32_t* sizes, uint32_t nl, uint32_t n,
+                                         uint32_t n_cols, vx_stream_t st) {
+  return guard([&] {
+    TwiddleRef r = tw_ref(tw, ctw, offs, nl, n);
+    ntt_cols(h_cols, n_cols, r, false, S(st));
   });
 }
 
@@ -633,23 +669,33 @@ static LeafSrc cols_leaf(const uint32_t* const* cols, uint32_t n_cols) {
   return L;
 }
 
-int vx_merkle_hash_leaves(const uint32_t* const* cols, uint32_t* hashes, uint32_t n_cols, uint32_t n_leaves,
-                          vx_stream_t st) {
+int vx_merkle_hash_leaves_host(const uint32_t* const* cols, uint32_t* hashes, uint32_t n_cols, uint32_t n_leaves,
+                               vx_stream_t st) {
   return guard([&] {
     require(hashes != nullptr, "hash_leaves: null output");
     merkle_leaf_hashes(cols_leaf(cols, n_cols), n_leaves, hashes, S(st));
   });
 }
 
+int vx_merkle_hash_leaves(const uint32_t* const* d_cols, uint32_t* hashes, uint32_t n_cols, uint32_t n_leaves,
+                          vx_stream_t st) {
+  return guard([&] {
+    require(hashes != nullptr && n_cols >= 1, "hash_leaves: null output or no column");
+    // only the first 16 columns enter a leaf (cuda/blake2s.cu:50-54)
+    std::vector<uint32_t*> cols = fetch_ptrs(d_cols, n_cols > 16 ? 16 : n_cols, S(st));
+    merkle_leaf_hashes(cols_leaf(cols.data(), n_cols), n_leaves, hashes, S(st));
+  });
+}
+
 int vx_merkle_hash_leaves_single(const uint32_t* col, uint32_t* hashes, uint32_t n_leaves, vx_stream_t st) {
   const uint32_t* cols[1] = {col};
-  return vx_merkle_hash_leaves(cols, hashes, 1, n_leaves, st);
+  return vx_merkle_hash_leaves_host(cols, hashes, 1, n_leaves, st);
 }
 
 int vx_merkle_hash_leaves_quad(const uint32_t* c0, const uint32_t* c1, const uint32_t* c2, const uint32_t* c3,
                                uint32_t* hashes, uint32_t n_leaves, vx_stream_t st) {
   const uint32_t* cols[4] = {c0, c1, c2, c3};
-  return vx_merkle_hash_leaves(cols, hashes, 4, n_leaves, st);
+  return vx_merkle_hash_leaves_host(cols, hashes, 4, n_leaves, st);
 }
 
 int vx_merkle_hash_nodes(const uint32_t* ch, uint32_t* par, uint32_t n_parents, vx_stream_t st) {
@@ -686,20 +732,42 @@ int vx_merkle_hash_leaves_merge_soa4(const uint32_t* c0, const uint32_t* c1, con
   });
 }
 
-int vx_merkle_tiled_generic(const uint32_t* const* cols, uint32_t* roots, uint32_t n_cols, uint32_t n_leaves,
+// Root of every 1024-leaf tile of n_leaves (a multiple of 1024): the tiles
+// are covered by one subtree launch per set bit of n_leaves / 1024, largest
+// first, each over its own row range (tile roots stay in tile order).
+static void tiled_roots(const LeafSrc& L0, uint32_t n_leaves, uint32_t* roots, hipStream_t s) {
+  require(roots != nullptr && n_leaves >= 1024 && n_leaves % 1024 == 0,
+          "tiled: n_leaves must be a positive multiple of 1024");
+  for (uint32_t c = 0; c < L0.n_cols; c++) require(aligned16(L0.col[c]), "tiled: columns must be 16-byte aligned");
+  const uint32_t tiles = n_leaves / 1024;
+  uint64_t done = 0;
+  for (int b = 31; b >= 0; b--) {
+    if (!((tiles >> b) & 1u)) continue;
+    LeafSrc L = L0;
+    for (uint32_t c = 0; c < L.n_cols; c++) L.col[c] = L0.col[c] + done * 1024;
+    merkle_tiled(L, 10 + (uint32_t)b, 10, roots + 8 * done, s);
+    done += 1ull << b;
+  }
+}
+
+int vx_merkle_tiled_generic_host(const uint32_t* const* cols, uint32_t* roots, uint32_t n_cols, uint32_t n_leaves,
+                                 vx_stream_t st) {
+  return guard([&] { tiled_roots(cols_leaf(cols, n_cols), n_leaves, roots, S(st)); });
+}
+
+int vx_merkle_tiled_generic(const uint32_t* const* d_cols, uint32_t* roots, uint32_t n_cols, uint32_t n_leaves,
                             vx_stream_t st) {
   return guard([&] {
-    require(roots && pow2(n_leaves) && n_leaves >= 1024, "tiled: n_leaves must be a power of two >= 1024");
-    LeafSrc L = cols_leaf(cols, n_cols);
-    for (uint32_t c = 0; c < L.n_cols; c++) require(aligned16(L.col[c]), "tiled: columns must be 16-byte aligned");
-    merkle_tiled(L, log2u(n_leaves), 10, roots, S(st));
+    require(n_cols >= 1, "tiled: need at least one column");
+    std::vector<uint32_t*> cols = fetch_ptrs(d_cols, n_cols > 16 ? 16 : n_cols, S(st));
+    tiled_roots(cols_leaf(cols.data(), n_cols), n_leaves, roots, S(st));
   });
 }
 
 int vx_merkle_tiled_soa4(const uint32_t* c0, const uint32_t* c1, const uint32_t* c2, const uint32_t* c3,
                          uint32_t* roots, uint32_t n_leaves, vx_stream_t st) {
   const uint32_t* cols[4] = {c0, c1, c2, c3};
-  return vx_merkle_tiled_generic(cols, roots, 4, n_leaves, st);
+  return vx_merkle_tiled_generic_host(cols, roots, 4, n_leaves, st);
 }
 
 int vx_grind_pow(const uint32_t* prefix, uint64_t* result, uint32_t pow_bits, uint64_t batch_offset,
@@ -1285,6 +1353,16 @@ int vx_server_done(vx_server* s, uint64_t ticket) {
 }
 
 uint64_t vx_server_resident_bytes(const vx_server* s) { return s ? s->impl.resident_bytes() : 0; }
+
+int vx_server_stats(const vx_server* s, uint64_t out[3]) {
+  return guard([&] {
+    require(s && out, "null argument");
+    const ServerStats st = s->impl.stats();
+    out[0] = st.created;
+    out[1] = st.evicted;
+    out[2] = st.deferred;
+  });
+}
 
 int vx_server_run_trace(vx_server* s, const vx_request* reqs, const double* arrival_s, uint32_t n,
                         uint8_t* const* outs, const size_t* caps, size_t* lens, double* times) {

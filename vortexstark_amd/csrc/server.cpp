@@ -1,8 +1,5 @@
-// Continuous-batching proof server (see server.h).
+// HIP provers for the proof server's scheduler (see server.h, server_core.h).
 #include "server.h"
-
-#include <algorithm>
-#include <cstring>
 
 #include "vx_d2.h"
 
@@ -21,16 +18,31 @@ AirSpec air_of(const ProofRequest& r) {
   return a;
 }
 
+// A resident prover of one size class on a slot's stream. Its completion
+// event is recorded behind each proof (graph replay + download kernel).
 class Served final : public ServedProver {
  public:
   Served(const ProofRequest& r, hipStream_t s, bool graphs)
       : p_(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, s, air_of(r)) {
+    VX_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
     p_.set_graphs(graphs);
     if (graphs) p_.prepare_graph();
+  }
+  ~Served() override {
+    (void)hipStreamSynchronize(p_.stream());
+    if (done_) (void)hipEventDestroy(done_);
   }
   void start(const ProofRequest& r) override {
     if (r.air == AIR_D2) p_.enqueue((uint32_t)r.seed, (uint32_t)(r.seed >> 32), false);
     else p_.enqueue(r.a, r.b, false);
+    VX_HIP(hipEventRecord(done_, p_.stream()));
+  }
+  int poll(std::string* err) override {
+    const hipError_t e = hipEventQuery(done_);
+    if (e == hipErrorNotReady) return 0;
+    if (e == hipSuccess) return 1;
+    if (err) *err = hipGetErrorString(e);
+    return -1;
   }
   const uint8_t* result() const override { return p_.host_proof(); }
   size_t result_bytes() const override { return p_.proof_bytes(); }
@@ -38,6 +50,7 @@ class Served final : public ServedProver {
 
  private:
   StarkProver p_;
+  hipEvent_t done_ = nullptr;
 };
 
 void check_request(const ProofRequest& r) {
@@ -52,123 +65,40 @@ void check_request(const ProofRequest& r) {
 
 }  // namespace
 
-ProofServer::ProofServer(uint32_t n_slots, uint64_t mem_budget_bytes, bool graphs)
-    : budget_(mem_budget_bytes), graphs_(graphs), epoch_(std::chrono::steady_clock::now()) {
+ProofServer::ProofServer(uint32_t n_slots, uint64_t mem_budget_bytes, bool graphs) : graphs_(graphs) {
   require(n_slots >= 1 && n_slots <= 64, "server: n_slots must be 1..64");
   VX_HIP(hipGetDevice(&device_));
-  slots_.resize(n_slots);
-  for (auto& sl : slots_) {
-    VX_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-    VX_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  streams_.assign(n_slots, nullptr);
+  try {
+    for (auto& s : streams_) VX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    ServerHooks h;
+    h.create = [this](uint32_t slot, const ProofRequest& r) -> std::unique_ptr<ServedProver> {
+      VX_HIP(hipSetDevice(device_));
+      return std::make_unique<Served>(r, streams_[slot], graphs_);
+    };
+    h.prover_bytes = [](const ProofRequest& r) {
+      return StarkProver::estimate_bytes(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, air_of(r));
+    };
+    h.table_bytes = [](const ProofRequest& r) { return ProverTables::estimate_bytes(r.log_n, r.blowup_bits); };
+    h.thread_init = [this] { VX_HIP(hipSetDevice(device_)); };
+    core_ = std::make_unique<ServerCore>(n_slots, mem_budget_bytes, std::move(h));
+  } catch (...) {
+    for (auto& s : streams_)
+      if (s) (void)hipStreamDestroy(s);
+    throw;
   }
-  thread_ = std::thread([this] { loop(); });
 }
 
 ProofServer::~ProofServer() {
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    stop_ = true;
-  }
-  cv_work_.notify_all();
-  if (thread_.joinable()) thread_.join();
-  for (auto& sl : slots_) {
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-    sl.provers.clear();
-    if (sl.done) (void)hipEventDestroy(sl.done);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
-  }
-}
-
-double ProofServer::now() const {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now() - epoch_).count();
-}
-
-ProofServer::Key ProofServer::key_of(const ProofRequest& r) {
-  const bool d2 = r.air == AIR_D2;
-  return Key{r.air, r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, d2 ? r.n_cols : 0, d2 ? r.n_cons : 0,
-             d2 ? r.air_seed : 0};
-}
-
-uint64_t ProofServer::estimate(const ProofRequest& r) {
-  return StarkProver::estimate_bytes(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, air_of(r));
-}
-
-std::unique_ptr<ServedProver> ProofServer::create(const ProofRequest& r, hipStream_t s, uint64_t* bytes) {
-  auto p = std::make_unique<Served>(r, s, graphs_);
-  *bytes = p->device_bytes();
-  return p;
-}
-
-// Evict the least recently used prover of an idle slot (freeing it then
-// waits for nothing); false if there is none.
-bool ProofServer::evict_one(uint64_t) {
-  Slot* best_slot = nullptr;
-  Key best_key{};
-  uint64_t best_use = ~0ull;
-  for (auto& sl : slots_)
-    if (!sl.ticket)
-      for (auto& [k, res] : sl.provers)
-        if (res.last_use < best_use) {
-        best_use = res.last_use;
-        best_slot = &sl;
-        best_key = k;
-      }
-  if (!best_slot) return false;
-  auto it = best_slot->provers.find(best_key);
-  resident_ -= it->second.bytes;
-  best_slot->provers.erase(it);
-  return true;
-}
-
-ServedProver* ProofServer::acquire(Slot& sl, const ProofRequest& r) {
-  std::lock_guard<std::mutex> lk(prov_mu_);
-  const Key k = key_of(r);
-  auto it = sl.provers.find(k);
-  if (it != sl.provers.end()) {
-    it->second.last_use = ++use_clock_;
-    return it->second.prover.get();
-  }
-  uint64_t tables = table_bytes_.count(k) ? 0 : ProverTables::estimate_bytes(r.log_n, r.blowup_bits);
-  const uint64_t need = estimate(r) + tables;
-  while (resident_ + need > budget_) {
-    if (!evict_one(need)) {
-      bool busy = std::any_of(slots_.begin(), slots_.end(), [](const Slot& x) { return x.ticket != 0; });
-      if (busy) return nullptr;  // retry once a running proof releases its prover
-      throw Error(ERR_ALLOC, "server: request does not fit the device-memory budget");
-    }
-  }
-  uint64_t bytes = 0;
-  auto p = create(r, sl.stream, &bytes);
-  if (tables) {
-    table_bytes_[k] = tables;
-    resident_ += tables;
-  }
-  resident_ += bytes;
-  ServedProver* raw = p.get();
-  sl.provers.emplace(k, Resident{std::move(p), bytes, ++use_clock_});
-  return raw;
+  core_.reset();  // drains the queue and destroys the provers, which use the streams
+  for (auto& s : streams_)
+    if (s) (void)hipStreamDestroy(s);
 }
 
 void ProofServer::prepare(const ProofRequest& r) {
   check_request(r);
-  // Serialise with the dispatcher: it never runs a proof of a class that
-  // is being created (the class is new), and the slot maps are locked.
   VX_HIP(hipSetDevice(device_));
-  for (auto& sl : slots_) {
-    std::lock_guard<std::mutex> lk(prov_mu_);
-    const Key k = key_of(r);
-    if (sl.provers.count(k)) continue;
-    uint64_t tables = table_bytes_.count(k) ? 0 : ProverTables::estimate_bytes(r.log_n, r.blowup_bits);
-    require(resident_ + estimate(r) + tables <= budget_, "server: prepare exceeds the device-memory budget");
-    uint64_t bytes = 0;
-    auto p = create(r, sl.stream, &bytes);
-    if (tables) {
-      table_bytes_[k] = tables;
-      resident_ += tables;
-    }
-    resident_ += bytes;
-    sl.provers.emplace(k, Resident{std::move(p), bytes, ++use_clock_});
-  }
+  core_->prepare(r);
 }
 
 uint64_t ProofServer::submit(const ProofRequest& r, uint8_t* out, size_t cap) {
@@ -176,152 +106,7 @@ uint64_t ProofServer::submit(const ProofRequest& r, uint8_t* out, size_t cap) {
   require(out != nullptr, "server: null output buffer");
   const size_t need = proof_size(r.log_n, r.blowup_bits, r.n_queries, r.pow_bits, air_of(r));
   require(cap >= need, "server: output buffer smaller than the proof (see vx_proof_size)");
-  uint64_t t;
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    require(!stop_, "server: stopped");
-    t = next_ticket_++;
-    Job& j = jobs_[t];
-    j.req = r;
-    j.out = out;
-    j.cap = cap;
-    j.t.submit_s = now();
-    incoming_.push_back(t);
-  }
-  cv_work_.notify_one();
-  return t;
-}
-
-bool ProofServer::done(uint64_t ticket) {
-  std::lock_guard<std::mutex> lk(mu_);
-  auto it = jobs_.find(ticket);
-  require(it != jobs_.end(), "server: unknown ticket");
-  return it->second.state >= 2;
-}
-
-bool ProofServer::wait(uint64_t ticket, size_t* len, JobTimes* t, std::string& err) {
-  std::unique_lock<std::mutex> lk(mu_);
-  auto it = jobs_.find(ticket);
-  require(it != jobs_.end(), "server: unknown ticket");
-  Job* j = &it->second;  // element references survive rehashing, iterators do not
-  cv_done_.wait(lk, [&] { return j->state >= 2; });
-  const bool ok = j->state == 2;
-  if (len) *len = j->len;
-  if (t) *t = j->t;
-  err = j->err;
-  jobs_.erase(ticket);
-  return ok;
-}
-
-// Dispatcher: polls the slots' completion events, starts queued requests on
-// free slots, then copies finished proofs out of the provers' pinned buffers
-// (after the slot has already been refilled, so the GPU does not wait for it).
-void ProofServer::loop() {
-  (void)hipSetDevice(device_);
-  std::deque<uint64_t> queue;
-  struct Finished {
-    uint64_t ticket;
-    const uint8_t* src;
-    size_t len;
-    double t;
-    std::string err;
-  };
-  for (;;) {
-    {
-      std::unique_lock<std::mutex> lk(mu_);
-      auto busy = [&] {
-        return std::any_of(slots_.begin(), slots_.end(), [](const Slot& x) { return x.ticket != 0; });
-      };
-      if (queue.empty() && incoming_.empty() && !busy()) {
-        if (stop_) return;
-        cv_work_.wait(lk, [&] { return stop_ || !incoming_.empty(); });
-        if (stop_ && incoming_.empty()) return;
-      }
-      while (!incoming_.empty()) {
-        queue.push_back(incoming_.front());
-        incoming_.pop_front();
-      }
-    }
-    bool progress = false;
-    std::vector<Finished> fin;
-    for (auto& sl : slots_) {
-      if (!sl.ticket) continue;
-      hipError_t e = hipEventQuery(sl.done);
-      if (e == hipErrorNotReady) continue;
-      Finished f{sl.ticket, nullptr, 0, now(), ""};
-      if (e == hipSuccess) {
-        f.src = sl.active->result();
-        f.len = sl.active->result_bytes();
-      } else {
-        f.err = std::string("server: proof failed on the device: ") + hipGetErrorString(e);
-      }
-      fin.push_back(std::move(f));
-      sl.ticket = 0;
-      sl.active = nullptr;
-      progress = true;
-    }
-    while (!queue.empty()) {
-      Slot* sl = nullptr;
-      const uint64_t tk = queue.front();
-      ProofRequest r;
-      {
-        std::lock_guard<std::mutex> lk(mu_);
-        r = jobs_[tk].req;
-      }
-      {  // prefer a free slot that already holds this class
-        std::lock_guard<std::mutex> lk(prov_mu_);
-        for (auto& x : slots_)
-          if (!x.ticket && x.provers.count(key_of(r))) {
-            sl = &x;
-            break;
-          }
-      }
-      if (!sl)
-        for (auto& x : slots_)
-          if (!x.ticket) {
-            sl = &x;
-            break;
-          }
-      if (!sl) break;
-      std::string err;
-      try {
-        ServedProver* p = acquire(*sl, r);
-        if (!p) break;  // waits for memory
-        {
-          std::lock_guard<std::mutex> lk(mu_);
-          Job& j = jobs_[tk];
-          j.state = 1;
-          j.t.start_s = now();
-        }
-        p->start(r);
-        VX_HIP(hipEventRecord(sl->done, sl->stream));
-        sl->ticket = tk;
-        sl->active = p;
-      } catch (const std::exception& ex) {
-        err = ex.what();
-      }
-      if (!err.empty()) fin.push_back(Finished{tk, nullptr, 0, now(), err});
-      queue.pop_front();
-      progress = true;
-    }
-    if (!fin.empty()) {
-      for (auto& f : fin) {
-        Job* j;
-        {
-          std::lock_guard<std::mutex> lk(mu_);
-          j = &jobs_[f.ticket];
-        }
-        if (f.err.empty()) std::memcpy(j->out, f.src, f.len);
-        std::lock_guard<std::mutex> lk(mu_);
-        j->len = f.len;
-        j->t.done_s = f.t;
-        j->err = f.err;
-        j->state = f.err.empty() ? 2 : 3;
-      }
-      cv_done_.notify_all();
-    }
-    if (!progress) std::this_thread::yield();
-  }
+  return core_->submit(r, out, cap);
 }
 
 }  // namespace vx

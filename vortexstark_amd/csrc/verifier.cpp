@@ -165,6 +165,13 @@ std::string replay_fri(Channel& ch, FriPart& f, uint32_t le, uint32_t nq, bool p
 // verifier.rs:119-144,153-233: FRI auth paths and fold equations.
 std::string check_fri_layers(const FriPart& f, uint32_t le, uint32_t nq) {
   const uint32_t n_fd = (uint32_t)f.fd.size();
+  // The prover decommits every committed FRI layer. Extra layers carry no
+  // root to check them against, and the fold equations below take one alpha
+  // per layer (n_fri + 1 were drawn), so any other count is rejected here;
+  // the reference indexes fri_alphas[layer + 1] unchecked and panics past
+  // n_fri + 1 (src/verifier.rs:185-196).
+  if (n_fd != f.n_fri)
+    return "Expected " + std::to_string(f.n_fri) + " FRI decommitments, got " + std::to_string(n_fd);
   std::vector<uint64_t> fidx(nq);
   for (uint32_t i = 0; i < nq; i++) fidx[i] = f.qidx[i] / 2;
   for (uint32_t k = 0; k < n_fd && k < f.n_fri; k++) {

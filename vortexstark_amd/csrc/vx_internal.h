@@ -10,28 +10,14 @@
 
 #include "vx_channel.h"
 #include "vx_circle.h"
+#include "vx_error.h"
 #include "vx_field.h"
 
 namespace vx {
 
 // ---------------------------------------------------------------- errors
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-enum Status : int {
-  OK = 0,
-  ERR_ARG = 1,      // invalid argument (shape, size, null pointer)
-  ERR_HIP = 2,      // HIP runtime error
-  ERR_ALLOC = 3,    // device allocation failed
-  ERR_VERIFY = 4,   // proof rejected
-  ERR_INTERNAL = 5,
-};
 void hip_check(hipError_t e, const char* what);
 #define VX_HIP(call) ::vx::hip_check((call), #call)
-inline void require(bool cond, const char* what) {
-  if (!cond) throw Error(ERR_ARG, what);
-}
 // Last error message of the calling thread (C-ABI vx_last_error).
 void set_last_error(const std::string& msg);
 
