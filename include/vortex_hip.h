@@ -547,6 +547,18 @@ int vx_build_leaves_lifted(const uint32_t *const *h_cols, const uint32_t *h_log_
  * Blake2s(left || right). */
 int vx_merkle_root_lifted(const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
                           uint32_t *d_root, vx_stream_t stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same root as vx_merkle_root_lifted, computed with
+ * no device allocation. The column table and the stored tree levels live in
+ * a static 8 MiB device scratch array per GPU (held until the process ends);
+ * shapes that need more (a table of 16 B per column plus about
+ * 64 B x 2^(log_max - 10) of tree levels, i.e. log_max above 26) are rejected
+ * with VX_ERR_ARG. At most 4096 columns. Asynchronous on `stream`: the root
+ * is in d_root once the stream reaches it. Every call on one GPU uses the
+ * same scratch, so calls that overlap on one GPU race: serialise them. */
+int vx_merkle_root_lifted_noalloc(const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
+                                  uint32_t *d_root, vx_stream_t stream);
 
 /* Resident batch (the batched-commit workload): inputs, coefficients and
  * evaluations of n_cols columns (log sizes sorted ascending, 14..24) kept in

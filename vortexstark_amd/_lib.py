@@ -121,6 +121,7 @@ SIGNATURES = {
     "vx_bit_reverse_m31_batch": (ip, [vp, vp, vp, u32, vp]),
     "vx_build_leaves_lifted": (ip, [vp, vp, u32, u32, vp, vp]),
     "vx_merkle_root_lifted": (ip, [vp, vp, u32, vp, vp]),
+    "vx_merkle_root_lifted_noalloc": (ip, [vp, vp, u32, vp, vp]),
     "vx_batch_create": (ip, [vp, u32, u32, ip, vp, C.POINTER(vp)]),
     "vx_batch_destroy": (ip, [vp]),
     "vx_batch_column": (ip, [vp, u32, ip, C.POINTER(vp), C.POINTER(u64)]),

@@ -1279,6 +1279,11 @@ int vx_merkle_root_lifted(const uint32_t* const* h_cols, const uint32_t* h_log_s
   });
 }
 
+int vx_merkle_root_lifted_noalloc(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                                  uint32_t* d_root, vx_stream_t st) {
+  return guard([&] { merkle_root_lifted_noalloc(h_cols, h_log_sizes, n_cols, d_root, S(st)); });
+}
+
 // ------------------------------------------------------------------ serving
 struct vx_server {
   ProofServer impl;

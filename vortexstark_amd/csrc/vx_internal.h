@@ -327,6 +327,11 @@ void merkle_root_small(const LeafSrc& leaf, uint32_t log_n, uint32_t* root_out, 
 void merkle_leaf_pairs(const LeafSrc& leaf, uint64_t n_leaves, uint32_t* parents, hipStream_t s);
 // Every leaf hash of a lifted tree (cuda_build_leaves_lifted semantics).
 void merkle_lifted_leaves(const LeafSrc& leaf, uint32_t* hashes, hipStream_t s);
+// Root of a lifted tree with the column table and the stored tree levels in
+// a file-scope device scratch array (noalloc.hip): no device allocation.
+// Asynchronous; calls that overlap on one GPU share the scratch and race.
+void merkle_root_lifted_noalloc(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
+                                uint32_t* d_root, hipStream_t s);
 
 // Decommitment of one tree at the query positions (src/prover.rs:551-618),
 // written straight into the KRAK byte buffer.
