@@ -11,8 +11,11 @@ Prints one JSON line (the driver's contract). `value` is milliseconds per
 proof (lower is better). `roofline` covers the dominant kernel, the trace-tree
 Blake2s subtree hash, which is VALU-bound; its duration comes from HIP events
 recorded around that kernel on the prover's stream. `cpu_baseline` times the
-C oracle (a single-threaded restatement of the same algorithm, same bytes) on
-a full log_n = 20 proof on this host and scales it to one log_n = 24 proof.
+C oracle (a restatement of the same algorithm producing the same bytes) on
+this host: config A (log_n 16) median of 5 proofs single-threaded and with
+OpenMP, and one full log_n = 24 proof with OpenMP (its `value`), each proof's
+digest compared with the GPU's. Sub-objects time config A on the GPU, config
+B with PoW 26 (KRAK v2) and config C (log_n 30, blowup 1, PoW 26).
 """
 import argparse
 import hashlib
@@ -90,24 +93,152 @@ def golden_digest(log_n, blowup, pow_bits):
     return None
 
 
-def cpu_baseline(log_n_target, sample_log_n):
-    """Single-threaded oracle proof on this host, scaled to the target size."""
+def host_info():
+    """Host CPU and the oracle's compiler flags (oracle/Makefile)."""
+    import platform
+    import re
+    import subprocess
+
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            m = re.search(r"model name\s*:\s*(.+)", f.read())
+            model = m.group(1).strip() if m else None
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, subprocess.SubprocessError, IndexError):
+        cc = "gcc"
+    return {"nproc": os.cpu_count(), "cpu_model": model or platform.processor(),
+            "compiler": f"{cc}: -O3 -march=x86-64-v2 (liboracle.so); the same plus -fopenmp (liboracle_omp.so)"}
+
+
+def omp_threads():
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+def cpu_baseline(gpu_a, gpu_b_sha, runs=5):
+    """The C oracle (the same algorithm and bytes as the GPU path) on this
+    host: config A (log_n 16, blowup 4, 80 queries; PoW 0 and 10) as the
+    median of `runs` proofs after one warm-up, single-threaded and with
+    OpenMP; config B (log_n 24, PoW 0) once with OpenMP. `value` is config
+    B's time, the same workload as the headline. Every CPU proof's SHA-256 is
+    compared with the GPU proof of the same statement."""
+    import statistics
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     a1 = fib_a1(42)
-    t0 = time.perf_counter()
-    pyoracle.prove(1, a1, sample_log_n)
-    secs = time.perf_counter() - t0
-    scale = float(1 << (log_n_target - sample_log_n))
+    threads = omp_threads()
+
+    def timed(log_n, pw, omp):
+        t0 = time.perf_counter()
+        data = pyoracle.prove(1, a1, log_n, 2, 80, pw, omp=omp)
+        return (time.perf_counter() - t0) * 1000.0, hashlib.sha256(data).hexdigest()
+
+    config_a = {}
+    for pw in (0, 10):
+        row = {"runs": runs}
+        for omp in (False, True):
+            timed(16, pw, omp)
+            ts, shas = zip(*(timed(16, pw, omp) for _ in range(runs)))
+            row["omp_ms_median" if omp else "single_thread_ms_median"] = round(statistics.median(ts), 2)
+            row["omp_sha256_matches_gpu" if omp else "single_thread_sha256_matches_gpu"] = all(
+                h == gpu_a[pw]["proof_sha256"] for h in shas)
+        row["gpu_ms"] = gpu_a[pw]["ms_per_proof"]
+        row["speedup_gpu_vs_omp"] = round(row["omp_ms_median"] / row["gpu_ms"], 1)
+        config_a[f"pow{pw}"] = row
+    b_ms, b_sha = timed(24, 0, True)
     return {
-        "value": round(secs * 1000.0 * scale, 1),
+        "value": round(b_ms, 1),
         "unit": "ms",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"one full log_n={sample_log_n} proof ({secs:.2f} s, single thread, oracle/liboracle.so), "
-                  f"scaled x{int(scale)} (linear in the domain size) to one log_n={log_n_target} proof",
+        "sample": f"one full config-B proof (log_n 24, blowup 4, 80 queries) by the C oracle (oracle/liboracle_omp.so) "
+                  f"on {threads} OpenMP threads of this host, bytes compared with the GPU proof",
+        "config_b_sha256_matches_gpu": b_sha == gpu_b_sha,
+        "config_a": config_a,
+        "host": host_info(),
     }
+
+
+def prove_leg(vx, log_n, bb, pw, steps, warmup):
+    """ms per proof (graph replays, host clock over `steps` proofs), one timed
+    proof's stage breakdown, the digest against the committed oracle digest
+    when there is one, and the host verifier's verdict."""
+    a0, a1 = 1, fib_a1(42)
+    prover = vx.Prover(log_n, bb, vx.N_QUERIES, pw)
+    try:
+        for _ in range(warmup):
+            prover.prove(a0, a1)
+        vx._lib.call("vx_device_sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            proof = prover.prove(a0, a1)
+        vx._lib.call("vx_device_sync")
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        prover.enqueue(a0, a1, timed=True)
+        prover.wait()
+        stages = prover.stage_times()
+        dev_bytes = prover.device_bytes
+    finally:
+        prover.close()
+    digest = hashlib.sha256(proof).hexdigest()
+    want = golden_digest(log_n, bb, pw)
+    ok, why = vx.verify(proof, bb)
+    out = {"log_n": log_n, "blowup_bits": bb, "pow_bits": pw, "n_queries": vx.N_QUERIES,
+           "proof_format": "KRAK v2" if pw else "KRAK v1", "steps": steps, "ms_per_proof": round(ms, 3),
+           "stages_ms": {k: round(v, 4) for k, v in stages.items()}, "proof_bytes": len(proof),
+           "proof_sha256": digest, "bit_exact_vs_oracle": (digest == want) if want else None, "verified": ok,
+           "device_bytes": dev_bytes}
+    if not ok:
+        out["verify_error"] = why
+    return out
+
+
+# VALU instructions per second measured on the box by a chain-of-instructions
+# probe (v_alignbit_b32 + v_xor_b32, 8 independent chains per lane): the
+# engine clock runs above the nominal 2.4 GHz under integer load, so
+# fractions are also reported against this rate.
+VALU_MEASURED_TOPS = 41.2
+# Lane instructions the trace-tree subtree kernel issues per compression
+# (SQ_INSTS_VALU x 64 / compressions, profiles/fib_log24_valu_counters.md):
+# 960 of the compression function plus loads, addressing and stores.
+ISSUED_PER_COMPRESSION = 997.1
+
+
+def proof_valu_ops(air, log_n, blowup_bits, pow_bits, n_cols=34, n_cons=35):
+    """Algorithmic VALU instructions of one proof: Blake2s compressions of
+    every tree (a leaf of <= 16 words is one compression, a node one) and of
+    the expected PoW search (2^pow_bits nonces), times 960; the NTT
+    butterflies (interpolate on 2^log_n, LDE of log_n layers on 2^le points),
+    times 12; for the multi-column AIR the quotient's 35 constraints at about
+    14 instructions per constraint and row (one M31 product, two QM31 x M31
+    lazy multiply-adds). Folds, decommitment and transcript are left out
+    (under 1 %)."""
+    n, N = 1 << log_n, 1 << (log_n + blowup_bits)
+    le = log_n + blowup_bits
+    trees = (2 * N - 1) * (2 if air == 0 else 1 + -(-n_cols // 16))  # trace tree(s) + quotient tree
+    fri = sum(2 * (1 << (le - 1 - k)) - 1 for k in range(le - 3))
+    comp = trees + fri + ((1 << pow_bits) if pow_bits else 0)
+    w = 1 if air == 0 else n_cols
+    butterflies = w * log_n * (n // 2 + N // 2)
+    quot = 0 if air == 0 else N * n_cons * 14
+    return comp * OPS_PER_COMPRESSION + butterflies * OPS_PER_BUTTERFLY + quot
+
+
+def serving_floor():
+    """The serving mix's VALU floor in proofs/s: the peak VALU rate over the
+    mean algorithmic instructions per proof of the mix (half Fibonacci with
+    log_n uniform in 14..22, half the 34-column AIR with log_n uniform in
+    14..20; blowup 4, PoW 20)."""
+    fib = [proof_valu_ops(0, L, 2, 20) for L in range(14, 23)]
+    d2 = [proof_valu_ops(1, L, 2, 20) for L in range(14, 21)]
+    mean = 0.5 * sum(fib) / len(fib) + 0.5 * sum(d2) / len(d2)
+    return {"ops_per_proof_mean": round(mean), "floor_proofs_per_s": round(VALU_PEAK_TOPS * 1e12 / mean, 1),
+            "floor_proofs_per_s_measured_rate": round(VALU_MEASURED_TOPS * 1e12 / mean, 1)}
 
 
 def percentile(xs, q):
@@ -199,6 +330,11 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
     open_plain, o_plain = run_server(vx, plain, reqs, arrivals)
     open_batch, o_batch = run_server(vx, batch, reqs, arrivals)
     identical = d_plain == d_batch == o_plain == o_batch
+    roof = serving_floor()
+    roof["bound"] = "valu"
+    roof["achieved_proofs_per_s"] = sat_batch["proofs_per_s"]
+    roof["frac"] = round(sat_batch["proofs_per_s"] / roof["floor_proofs_per_s"], 4)
+    roof["plain_loop_frac"] = round(sat_plain["proofs_per_s"] / roof["floor_proofs_per_s"], 4)
     resident = batch.resident_bytes
     plain.close()
     batch.close()
@@ -214,6 +350,7 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
         "p99_ratio": round(open_batch["p99_ms"] / open_plain["p99_ms"], 3),
         "identical_to_alone": identical,
         "resident_bytes": resident,
+        "roofline": roof,
     }
 
 
@@ -585,6 +722,8 @@ def main():
     ap.add_argument("--blowup-bits", type=int, default=2)
     ap.add_argument("--pow-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the config A (log_n 16), config B PoW 26 and config C (log_n 30) legs")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--no-serve", action="store_true")
@@ -681,6 +820,8 @@ def main():
             "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "TOP/s",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "frac_vs_measured_rate": round(achieved / VALU_MEASURED_TOPS, 4),
+            "issued_frac": round(compressions * ISSUED_PER_COMPRESSION / (k_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
             "traffic": profiled_traffic() if (log_n, bb) == (24, 2) else None,
             "kernel_ms": round(k_ms, 4),
             "work": f"{compressions} Blake2s compressions x {OPS_PER_COMPRESSION} int ops",
@@ -701,14 +842,25 @@ def main():
     }
     if not ok:
         out["verify_error"] = why
+    headline = (log_n, bb, pw) == (24, 2, 0) and not args.profile
+    gpu_a = None
+    if headline and not (args.no_configs and args.no_cpu_baseline):
+        # config A on the GPU (the CPU baseline's workload), PoW 0 and 10
+        gpu_a = {p: prove_leg(vx, 16, 2, p, 20, 3) for p in (0, 10)}
+        out["config_a_gpu"] = {f"pow{p}": v for p, v in gpu_a.items()}
+    if headline and not args.no_configs:
+        # config B as specified (PoW 26, KRAK v2) and config C (log_n 30,
+        # blowup 1: the whole 2^31-point circle group, everything resident)
+        out["config_b_pow26"] = prove_leg(vx, 24, 2, 26, 5, 1)
+        out["config_c"] = prove_leg(vx, 30, 1, 26, 3, 1)
     if not args.no_serve and not args.profile:
         out["serving"] = serving(vx, args.serve_requests, args.slots)
     if not args.no_batch and not args.profile:
         out["batched_commit"] = batched(vx, args.steps)
     if not args.no_stwo and not args.profile:
         out["stwo_ntt"] = stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(log_n, max(log_n - 4, 10))
+    if not args.no_cpu_baseline and gpu_a is not None:
+        out["cpu_baseline"] = cpu_baseline(gpu_a, digest)
     prover.close()
     print(json.dumps(out), flush=True)
 
