@@ -360,7 +360,10 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
   Raw cur, nxt;
   if constexpr (!kDirect) leaf_load<KIND, NC>(L, first, cur);
   uint32_t s2[8], s3[8];
-#pragma unroll
+  // Not unrolled: a group's body is seven compressions (about 7,000
+  // instructions), so two or four copies would exceed what the instruction
+  // cache holds well; the branches on gi are uniform.
+#pragma unroll 1
   for (uint32_t gi = 0; gi < G; gi++) {
     const uint64_t i = first + 4 * gi;
     G4 g;

@@ -230,9 +230,10 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   const uint64_t low_base = (tile & ((1ull << low_tiles_log) - 1)) << a.cbits;
   const uint64_t H = tile >> low_tiles_log;
   const uint64_t hi_base = H << (a.j0 + a.k);
-  uint32_t v[32];
-#pragma unroll
-  for (uint32_t ri = 0; ri < a.n_rounds; ri++) {
+  // One round of the pass; the fixed shapes unroll their constant round
+  // count, the generic kernel loops over its runtime one.
+  auto round = [&](uint32_t ri) __attribute__((always_inline)) {
+    uint32_t v[32];
     const uint32_t pos = a.round[ri].pos, r_rb0 = a.round[ri].rb0, r_nb = a.round[ri].nb;
     // A contiguous pass (cbits = 0) whose round keeps each lane's 32 elements
     // next to each other (pos < 4) would read or write HBM with a 128-byte
@@ -325,6 +326,12 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
         dst[gi] = a.scale != 1u ? m31_mul(v[s], a.scale) : v[s];
       }
     }
+  };
+  if constexpr (rx_is_fixed<Args>::value) {
+#pragma unroll
+    for (uint32_t ri = 0; ri < Args::n_rounds; ri++) round(ri);
+  } else {
+    for (uint32_t ri = 0; ri < a.n_rounds; ri++) round(ri);
   }
 }
 
