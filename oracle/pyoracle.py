@@ -98,6 +98,7 @@ def lib(omp=False):
         "or_d2_verify": (C.c_int, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
         "or_d2_proof_size": (sz, [u32, u32, u32, u32]),
         "or_lde": (None, [_u32p, u32, u32, _u32p]),
+        "or_ntt_evaluate_at": (None, [_u32p, u32, _u32p, C.POINTER(C.c_uint64), u32, _u32p]),
         "or_d2_column": (None, [u64, u32, u32, _u32p]),
         "or_batch_prove": (C.c_int, [u64, _u32p, u32, u32, u32, u32, C.POINTER(C.POINTER(C.c_uint8)),
                                      C.POINTER(sz)]),
@@ -411,6 +412,18 @@ def d2_proof_size(log_n, n_cols=D2_COLS, blowup_bits=2, n_queries=80):
 
 
 # ---- batched uneven columns ----
+def evaluate_at(coeffs, dom, idx, omp=True):
+    """Outputs idx of evaluate(zero-padded coeffs, dom) without the whole
+    transform: O(len(coeffs)) per group of indices sharing bits >= 2."""
+    c = u32(coeffs)
+    log_n = c.size.bit_length() - 1
+    assert c.size == 1 << log_n
+    ix = np.ascontiguousarray(np.asarray(idx, dtype=np.uint64))
+    out = np.zeros(ix.size, np.uint32)
+    lib(omp).or_ntt_evaluate_at(c, log_n, u32(dom), ix.ctypes.data_as(C.POINTER(C.c_uint64)), ix.size, out)
+    return out
+
+
 def lde(values, blowup_bits=2):
     v = u32(values)
     log_n = v.size.bit_length() - 1

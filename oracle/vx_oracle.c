@@ -393,6 +393,119 @@ void or_ntt_interpolate(uint32_t *d, const uint32_t *line_itw, const uint32_t *c
     uint32_t inv_n = e == 0 ? 1u : (1u << e);
     for (uint64_t i = 0; i < n; i++) d[i] = m31_mul(d[i], inv_n);
 }
+/* Single outputs of the forward transform above (or_ntt_evaluate on the
+ * domain of coset dom, 2^dom_log points) applied to the zero-padded
+ * coefficients c[0 .. 2^log_n), each in O(2^log_n) instead of the whole
+ * O(N log N) transform.
+ *
+ * Unrolling the butterflies of cuda/circle_ntt.cu:74-117 from output i back
+ * to the input: layer j pairs positions that differ in bit j; the output's
+ * bits above j fix the twiddle index h = i >> (j + 1) and bit j of i picks
+ * + or -, so input k reaches output i with the factor
+ *     prod over bits j set in k of  s_j(i) = (-1)^{i_j} w_j(i),
+ * w_0(i) = y(p_{i >> 1}) (circle layer), w_j(i) = x(pi^{j-1}(p_{2^j (i >> (j+1))}))
+ * for j >= 1 (line layer j - 1), p_t = init * step^t and pi(x) = 2x^2 - 1:
+ * the closed form of the tables or_compute_twiddles builds (src/circle.rs:
+ * 284-347, circle_tw[h] = y(p_h), line layer l entry h = x of p_{2^{l+1} h}
+ * squashed l times). Bits j >= log_n of k are zero (zero padding), so only
+ * the first log_n layers contribute (the others replicate).
+ *
+ * Outputs that share all bits above bit 1 share s_j for j >= 2, so they are
+ * evaluated together: for each such group the 4 partial sums over the
+ * coefficients with each value of bits 0..1 are accumulated once (2^log_n
+ * multiply-adds per group), then combined per output with its own s_0, s_1. */
+static uint32_t ntt_twiddle_at(coset_t c, uint32_t j, uint64_t i) {
+    if (j == 0) return coset_at(c, i >> 1).y;
+    uint32_t x = coset_at(c, (i >> (j + 1)) << j).x;
+    for (uint32_t r = 1; r < j; r++) {
+        uint32_t x2 = m31_mul(x, x);
+        x = m31_sub(m31_add(x2, x2), 1);
+    }
+    return x;
+}
+static uint32_t ntt_sign_twiddle(coset_t c, uint32_t j, uint64_t i) {
+    uint32_t w = ntt_twiddle_at(c, j, i);
+    return ((i >> j) & 1) ? m31_neg(w) : w;
+}
+/* table[t] = prod over bits b of t set of f[b], t < 2^nb */
+static void product_table(const uint32_t *f, uint32_t nb, uint32_t *table) {
+    table[0] = 1;
+    for (uint32_t b = 0; b < nb; b++)
+        for (uint64_t t = 0; t < (1ull << b); t++) table[t + (1ull << b)] = m31_mul(table[t], f[b]);
+}
+static inline uint64_t fold62(uint64_t v) { return (v & P) + (v >> 31); }
+static int cmp_u64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : (x > y);
+}
+void or_ntt_evaluate_at(const uint32_t *c, uint32_t log_n, const uint32_t dom5[5], const uint64_t *idx,
+                        uint32_t n_idx, uint32_t *out) {
+    if (n_idx == 0) return;
+    coset_t d = coset_unpack(dom5);
+    const uint32_t m = log_n < 2 ? log_n : 2;       /* bits combined per output */
+    const uint32_t B = log_n < 16 ? log_n : 16;     /* bits per coefficient block */
+    const uint32_t mid_bits = B - m, hi_bits = log_n - B;
+    const uint64_t n_low = 1ull << m, n_mid = 1ull << mid_bits, n_hi = 1ull << hi_bits;
+    /* distinct groups (output index >> m) */
+    uint64_t *grp = (uint64_t *)malloc(8 * (size_t)n_idx);
+    for (uint32_t k = 0; k < n_idx; k++) grp[k] = idx[k] >> m;
+    qsort(grp, n_idx, 8, cmp_u64);
+    uint32_t ng = 0;
+    for (uint32_t k = 0; k < n_idx; k++)
+        if (k == 0 || grp[k] != grp[ng - 1]) grp[ng++] = grp[k];
+    uint32_t *wmid = (uint32_t *)malloc(4 * (size_t)ng * n_mid);
+    uint32_t *whi = (uint32_t *)malloc(4 * (size_t)ng * n_hi);
+    uint64_t *acc = (uint64_t *)calloc((size_t)ng * n_low, 8);
+    for (uint32_t g = 0; g < ng; g++) {
+        uint32_t f[32];
+        const uint64_t base = grp[g] << m;
+        for (uint32_t j = m; j < B; j++) f[j - m] = ntt_sign_twiddle(d, j, base);
+        product_table(f, mid_bits, wmid + (size_t)g * n_mid);
+        for (uint32_t j = B; j < log_n; j++) f[j - B] = ntt_sign_twiddle(d, j, base);
+        product_table(f, hi_bits, whi + (size_t)g * n_hi);
+    }
+#pragma omp parallel
+    {
+        uint64_t *part = (uint64_t *)calloc((size_t)ng * n_low, 8);
+#pragma omp for schedule(static)
+        for (int64_t h = 0; h < (int64_t)n_hi; h++) {
+            const uint32_t *blk = c + ((uint64_t)h << B);
+            for (uint32_t g = 0; g < ng; g++) {
+                const uint32_t *wm = wmid + (size_t)g * n_mid;
+                uint64_t a[4] = {0, 0, 0, 0};
+                for (uint64_t t = 0; t < n_mid; t++)
+                    for (uint64_t lo = 0; lo < n_low; lo++)
+                        a[lo] += fold62((uint64_t)blk[(t << m) | lo] * wm[t]);
+                const uint32_t wh = whi[(size_t)g * n_hi + (uint64_t)h];
+                for (uint64_t lo = 0; lo < n_low; lo++)
+                    part[(size_t)g * n_low + lo] += fold62((uint64_t)(uint32_t)(a[lo] % P) * wh);
+            }
+        }
+#pragma omp critical
+        for (size_t k = 0; k < (size_t)ng * n_low; k++) acc[k] = (acc[k] + part[k] % P) % P;
+        free(part);
+    }
+    for (uint32_t k = 0; k < n_idx; k++) {
+        uint64_t lo_g = idx[k] >> m;
+        uint32_t g = 0, hi = ng;
+        while (hi - g > 1) { /* binary search in the sorted distinct groups */
+            uint32_t mid = (g + hi) / 2;
+            if (grp[mid] <= lo_g) g = mid; else hi = mid;
+        }
+        uint32_t f[2], tab[4];
+        for (uint32_t j = 0; j < m; j++) f[j] = ntt_sign_twiddle(d, j, idx[k]);
+        product_table(f, m, tab);
+        uint32_t v = 0;
+        for (uint64_t lo = 0; lo < n_low; lo++)
+            v = m31_add(v, m31_mul((uint32_t)acc[(size_t)g * n_low + lo], tab[lo]));
+        out[k] = v;
+    }
+    free(grp);
+    free(wmid);
+    free(whi);
+    free(acc);
+}
+
 /* circle_ntt.cu:234-244 */
 void or_bit_reverse_m31(uint32_t *d, uint32_t log_n) {
     uint64_t n = 1ull << log_n;

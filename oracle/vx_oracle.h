@@ -24,6 +24,9 @@
  *                         SplitMix64 reference output, closed forms
  *   lifted Merkle         Python hashlib over the lifted rows; the LDE is the
  *                         pinned single-column LDE per column
+ *   ntt_evaluate_at       the full transform (or_ntt_evaluate, or_lde) at
+ *                         every output for small sizes and three coset kinds,
+ *                         and the roundtrip to the trace values
  */
 #ifndef VX_ORACLE_H
 #define VX_ORACLE_H
@@ -150,6 +153,11 @@ size_t or_d2_proof_size(uint32_t log_n, uint32_t n_cols, uint32_t blowup_bits, u
 /* ---- batched uneven columns: per-column LDE (src/prover.rs:280-318) and the
  *      lifted Merkle tree (cuda/merkle_leaves_lifted.cu:26-98) ---- */
 void or_lde(const uint32_t *in, uint32_t log_n, uint32_t blowup_bits, uint32_t *out);
+/* Outputs idx[0..n_idx) of or_ntt_evaluate on the domain of coset dom applied
+ * to the zero-padded coefficients c[0 .. 2^log_n), O(2^log_n) per group of
+ * outputs sharing bits >= 2 (closed-form twiddles, product basis). */
+void or_ntt_evaluate_at(const uint32_t *c, uint32_t log_n, const uint32_t dom5[5], const uint64_t *idx,
+                        uint32_t n_idx, uint32_t *out);
 void or_build_leaves_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
                             uint32_t lifting_log, uint32_t *hashes);
 void or_merkle_root_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,

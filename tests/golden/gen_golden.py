@@ -17,10 +17,13 @@ Sources, per section:
                the batched-columns proof (KRAK v4): pyref.prove_batch at tiny
                sizes, the C oracle at the workload's sizes
 
-Run from the repo root:  python tests/golden/gen_golden.py [--big]
+  serving_e    one request of every class of the serving mix (Fibonacci
+               log_n 14..22, 34-column AIR log_n 14..20, PoW 20), C oracle
+
+Run from the repo root:  python tests/golden/gen_golden.py [--big | --serving]
 (--big adds the log_n = 20, 24 and 25 (blowup 1) oracle digests, the
 log_n = 24 PoW-26 digest and the log_n = 20 multi-column AIR digest, which
-take minutes.)
+take minutes; --serving recomputes only serving_e and keeps the rest.)
 """
 import hashlib
 import json
@@ -57,8 +60,48 @@ def fib_a1(seed):
             return r
 
 
+def serving_requests_e(seed=20261203):
+    """One request of every class of the serving mix: Fibonacci (a0 = 1,
+    a1 from SplitMix64) at log_n 14..22 and the 34-column synthetic AIR
+    (35 constraints, air seed 0xD2A1, trace seed from SplitMix64) at log_n
+    14..20; blowup 4, 80 queries, PoW 20."""
+    out = []
+    s = seed
+    for log_n in range(14, 23):
+        s += 1
+        out.append({"air": 0, "log_n": log_n, "a": 1, "b": fib_a1(s), "blowup_bits": 2, "n_queries": 80,
+                    "pow_bits": 20})
+    for log_n in range(14, 21):
+        s += 1
+        out.append({"air": 1, "log_n": log_n, "seed": next(splitmix64(s)), "n_cols": 34, "n_cons": 35,
+                    "air_seed": 0xD2A1, "blowup_bits": 2, "n_queries": 80, "pow_bits": 20})
+    return out
+
+
+def serving_digests():
+    import pyoracle
+    rows = []
+    for r in serving_requests_e():
+        if r["air"] == 0:
+            data = pyoracle.prove(r["a"], r["b"], r["log_n"], blowup_bits=2, n_queries=80, pow_bits=20, omp=True)
+        else:
+            data = pyoracle.d2_prove(r["seed"], r["log_n"], 34, 35, 0xD2A1, 2, 80, 20, omp=True)
+        rows.append(dict(r, len=len(data), sha256=hashlib.sha256(data).hexdigest()))
+        print("serving digest", r["air"], r["log_n"], len(data), flush=True)
+    return rows
+
+
 def main():
     big = "--big" in sys.argv
+    out = os.path.join(HERE, "golden.json")
+    if "--serving" in sys.argv:
+        # only (re)compute the serving-mix digests, keep everything else
+        g = json.load(open(out))
+        g["serving_e"] = serving_digests()
+        with open(out, "w") as f:
+            json.dump(g, f, separators=(",", ":"))
+        print("wrote", out, os.path.getsize(out), "bytes")
+        return
     g = {}
     rng = random.Random(20261202)
 
@@ -189,7 +232,8 @@ def main():
                                     "sha256": hashlib.sha256(data).hexdigest()})
         print("oracle batch proof", n_cols, pw, len(data), flush=True)
 
-    out = os.path.join(HERE, "golden.json")
+    if os.path.exists(out):
+        g["serving_e"] = json.load(open(out)).get("serving_e", [])
     if not big and os.path.exists(out):
         old = json.load(open(out))
         keep = [p for p in old.get("proofs_c", []) if p["log_n"] >= 20]

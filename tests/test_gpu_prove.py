@@ -4,7 +4,9 @@ Small sizes: proof bytes equal the oracle's bytes and the independent Python
 restatement's digests. Full size (log_n = 20, 24, 25 at blowup 1, and 24 with
 26 PoW bits): the SHA-256 of the proof equals the oracle digest committed in
 tests/golden/golden.json (computed by oracle/liboracle_omp.so), and the
-verifier accepts. log_n = 30 (blowup 1) is checked by properties only."""
+verifier accepts. log_n = 30 (blowup 1, PoW 26): the interpolated coefficients
+reproduce the trace, and every trace opening and quotient value of the proof
+equals the oracle's single-output evaluation of the same LDE."""
 import hashlib
 
 import numpy as np
@@ -82,6 +84,75 @@ def test_log30_blowup1_proof_properties(golden):
     bad[len(bad) // 2] ^= 1
     assert not vx.verify(bytes(bad), 1)[0]
     p.close()
+
+
+def fib_at(a0, a1, i):
+    """Trace value i of a[k+2] = a[k+1] + a[k] (mod P) by 2x2 matrix powers."""
+    def mul(x, y):
+        return [[(x[0][0] * y[0][0] + x[0][1] * y[1][0]) % po.P, (x[0][0] * y[0][1] + x[0][1] * y[1][1]) % po.P],
+                [(x[1][0] * y[0][0] + x[1][1] * y[1][0]) % po.P, (x[1][0] * y[0][1] + x[1][1] * y[1][1]) % po.P]]
+    r, m = [[1, 0], [0, 1]], [[1, 1], [1, 0]]
+    while i:
+        if i & 1:
+            r = mul(r, m)
+        m = mul(m, m)
+        i >>= 1
+    return (r[1][0] * a1 + r[1][1] * a0) % po.P   # M^i = [[F(i+1), F(i)], [F(i), F(i-1)]]
+
+
+def test_fib_at_matches_recurrence():
+    t = po.fibonacci_trace(3, 7, 6)
+    assert [fib_at(3, 7, i) for i in range(64)] == [int(x) for x in t]
+
+
+def test_log30_blowup1_pow26_openings_match_oracle(golden):
+    """Config C at its own size: log_n = 30, blowup 1 (the whole 2^31-point
+    circle group), 80 queries, PoW 26. The oracle cannot run the whole proof,
+    so it evaluates single outputs of the same linear maps in O(n) each
+    (po.evaluate_at): (1) the GPU's interpolated coefficients, transformed
+    forward on the trace domain, give the Fibonacci trace at 16 points; (2)
+    the trace openings (values at q, siblings at q ^ 1) equal the LDE of
+    those coefficients at every query; (3) the quotient openings equal
+    alpha * (e(i + 2) - e(i + 1) - e(i)) from the oracle's LDE values, with
+    alpha drawn from the proof's trace root by the Python channel."""
+    import pyref
+
+    ws = golden["workload_seed"]
+    log_n, bb, pw = 30, 1, 26
+    n, N = 1 << log_n, 1 << (log_n + bb)
+    p = vx.Prover(log_n, bb, 80, pw)
+    proof = p.prove(ws["a0"], ws["a1"])
+    print("log30 proof", len(proof), flush=True)
+    assert vx.verify(proof, bb) == (True, "")
+    ptr, words = p.buffer(0)
+    assert words == n
+    coef = d2h(ptr, words)
+    p.close()
+    k = Krak(proof)
+    assert (k.version, k.log_n, k.pow_bits) == (2, 30, 26)
+
+    rng = np.random.default_rng(30)
+    tpts = np.concatenate([[0, 1, 2, n - 1], rng.choice(n, size=12, replace=False)]).astype(np.uint64)
+    got_t = po.evaluate_at(coef, po.coset(po.HALF_COSET, log_n), tpts)
+    assert [int(x) for x in got_t] == [fib_at(ws["a0"], ws["a1"], int(i)) for i in tpts]
+    print("coefficients reproduce the trace", flush=True)
+
+    qs = [int(q) for q in k.queries]
+    pts = sorted({((q & ~1) + d) % N for q in qs for d in range(4)})
+    ev = dict(zip(pts, (int(x) for x in po.evaluate_at(coef, po.eval_domain(log_n + bb), np.array(pts, np.uint64)))))
+    print("oracle LDE values at", len(pts), "points", flush=True)
+    w = k.words
+    tv, ts = k.trace["values"], k.trace["siblings"]
+    for j, q in enumerate(qs):
+        assert w[tv + j] == ev[q] and w[ts + j] == ev[q ^ 1], (j, q)
+    ch = pyref.Channel()
+    ch.mix_digest(k.root(k.off_trace_root))
+    alpha = ch.draw_felt()
+    qv, qsib = k.quotient["values"], k.quotient["siblings"]
+    for j, q in enumerate(qs):
+        for off, i in ((qv, q), (qsib, q ^ 1)):
+            c = (ev[(i + 2) % N] - ev[(i + 1) % N] - ev[i]) % po.P
+            assert w[off + 4 * j:off + 4 * j + 4] == [a * c % po.P for a in alpha], (j, i)
 
 
 def test_intermediate_lde_matches_oracle():

@@ -4,6 +4,7 @@ Every proof produced through a HIP graph or through the server, alone or in a
 mix of concurrent requests, must equal byte for byte the proof of the same
 request made alone by the kernel-by-kernel prover; that prover in turn equals
 the oracle (test_gpu_prove.py)."""
+import hashlib
 import random
 
 import pytest
@@ -130,3 +131,35 @@ def test_server_rejects_what_cannot_fit():
     with pytest.raises(vx.VortexError):
         srv2 = vx.Server(n_slots=1)
         srv2.submit(vx.request(16, a=vx.P))  # non-canonical input
+
+
+def test_serving_mix_classes_match_oracle_digests(golden):
+    # Every class of the serving workload (Fibonacci log_n 14..22 and the
+    # 34-column AIR log_n 14..20, blowup 4, 80 queries, PoW 20), each request
+    # twice, streamed in shuffled order on a Poisson trace through a 16-slot
+    # server: every proof must equal the oracle's (SHA-256 committed by
+    # tests/golden/gen_golden.py --serving).
+    rows = golden["serving_e"]
+    assert len(rows) == 16
+    reqs, want = [], []
+    for r in rows:
+        if r["air"] == 0:
+            q = vx.request(r["log_n"], a=r["a"], b=r["b"], blowup_bits=r["blowup_bits"], pow_bits=r["pow_bits"])
+        else:
+            q = vx.request(r["log_n"], air=vx.AIR_D2, seed=r["seed"], n_cols=r["n_cols"], n_cons=r["n_cons"],
+                           air_seed=r["air_seed"], blowup_bits=r["blowup_bits"], pow_bits=r["pow_bits"])
+        reqs.append(q)
+        want.append(r["sha256"])
+    order = list(range(len(reqs))) * 2
+    rng = random.Random(5)
+    rng.shuffle(order)
+    t, arrivals = 0.0, []
+    for _ in order:
+        t += rng.expovariate(400.0)
+        arrivals.append(t)
+    srv = vx.Server(n_slots=16)
+    proofs, _ = srv.run_trace([reqs[i] for i in order], arrivals)
+    srv.close()
+    for k, (i, proof) in enumerate(zip(order, proofs)):
+        print("serving request", k, "class", rows[i]["air"], rows[i]["log_n"], flush=True)
+        assert hashlib.sha256(proof).hexdigest() == want[i], (rows[i]["air"], rows[i]["log_n"])

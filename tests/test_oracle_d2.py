@@ -77,6 +77,24 @@ def test_d2_oracle_digests_stable(golden):
         assert hashlib.sha256(data).hexdigest() == g["sha256"], g
 
 
+def test_serving_fixture_is_the_generated_request_list(golden):
+    # the committed serving-mix digests belong to exactly the requests the
+    # generator draws, and the two smallest classes reproduce from the oracle
+    import gen_golden
+
+    rows = golden["serving_e"]
+    reqs = gen_golden.serving_requests_e()
+    assert [{k: v for k, v in r.items() if k not in ("len", "sha256")} for r in rows] == reqs
+    fib14 = next(r for r in rows if r["air"] == 0 and r["log_n"] == 14)
+    data = po.prove(fib14["a"], fib14["b"], 14, blowup_bits=2, n_queries=80, pow_bits=20, omp=True)
+    assert hashlib.sha256(data).hexdigest() == fib14["sha256"]
+    d14 = next(r for r in rows if r["air"] == 1 and r["log_n"] == 14)
+    data = po.d2_prove(d14["seed"], 14, 34, 35, 0xD2A1, 2, 80, 20, omp=True)
+    assert hashlib.sha256(data).hexdigest() == d14["sha256"]
+    ok, why = vx.verify(data, 2)
+    assert ok, why
+
+
 def words(b):
     return list(struct.unpack(f"<{len(b) // 4}I", b))
 
