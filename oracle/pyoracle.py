@@ -99,6 +99,12 @@ def lib(omp=False):
         "or_d2_proof_size": (sz, [u32, u32, u32, u32]),
         "or_lde": (None, [_u32p, u32, u32, _u32p]),
         "or_ntt_evaluate_at": (None, [_u32p, u32, _u32p, C.POINTER(C.c_uint64), u32, _u32p]),
+        "or_cairo_vanishing": (None, [_u32p, _u32p, _u32p, u32, u32, C.c_int]),
+        "or_axpy_m31": (None, [u32, _u32p, _u32p, C.c_uint64]),
+        "or_cairo_quotient": (None, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _u32p,
+                                     _u32p, _u32p, _u32p, C.c_uint64, u32, C.c_uint64, C.c_int]),
+        "or_cairo_quotient_chunk": (None, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                           _u32p, _u32p, _u32p, _u32p, C.c_uint64, C.c_uint64, C.c_uint64, u32]),
         "or_d2_column": (None, [u64, u32, u32, _u32p]),
         "or_batch_prove": (C.c_int, [u64, _u32p, u32, u32, u32, u32, C.POINTER(C.POINTER(C.c_uint8)),
                                      C.POINTER(sz)]),
@@ -421,6 +427,51 @@ def evaluate_at(coeffs, dom, idx, omp=True):
     ix = np.ascontiguousarray(np.asarray(idx, dtype=np.uint64))
     out = np.zeros(ix.size, np.uint32)
     lib(omp).or_ntt_evaluate_at(c, log_n, u32(dom), ix.ctypes.data_as(C.POINTER(C.c_uint64)), ix.size, out)
+    return out
+
+
+def _vptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def cairo_vanishing(init, step, log_eval, log_n, inverse=True, omp=True):
+    """Z_H (or 1/Z_H) of half_coset(log_n) at eval position i = the point
+    init * step^bitrev(i) (cuda/cairo_constraint.cu:77-118, 990-1019)."""
+    out = np.zeros(1 << log_eval, np.uint32)
+    lib(omp).or_cairo_vanishing(u32(init), u32(step), out, log_eval, log_n, 1 if inverse else 0)
+    return out
+
+
+def axpy_m31(s, x, y):
+    y = u32(y).copy()
+    lib().or_axpy_m31(s, u32(x), y, y.size)
+    return y
+
+
+def cairo_quotient(cols, inter, alpha, vh_inv, trans, chal, blowup_step, n=None, chunk_offset=0, slab=False,
+                   omp=True):
+    """Quotient columns (4 x n) of cuda_cairo_quotient (slab=False: n =
+    rows of the columns) or cuda_cairo_quotient_slab (columns are slabs of
+    n + blowup_step rows; vh_inv / trans / output are global)."""
+    cols = [u32(c) for c in cols]
+    inter = [u32(c) for c in inter]
+    assert len(cols) == 34 and len(inter) == 32
+    if n is None:
+        n = cols[0].size
+    g = u32(vh_inv).size
+    out = [np.zeros(g, np.uint32) for _ in range(4)]
+    lib(omp).or_cairo_quotient(_vptrs(cols), _vptrs(inter), _vptrs(out), u32(alpha), u32(vh_inv), u32(trans),
+                               u32(chal), n, blowup_step, chunk_offset, 1 if slab else 0)
+    return out
+
+
+def cairo_quotient_chunk(cols, inter12, alpha, vh_inv, trans, chal, offset, chunk_n, blowup_step, omp=True):
+    cols = [u32(c) for c in cols]
+    inter12 = [u32(c) for c in inter12]
+    assert len(cols) == 34 and len(inter12) == 12
+    out = [np.zeros(chunk_n, np.uint32) for _ in range(4)]
+    lib(omp).or_cairo_quotient_chunk(_vptrs(cols), _vptrs(inter12), _vptrs(out), u32(alpha), u32(vh_inv),
+                                     u32(trans), u32(chal), offset, chunk_n, cols[0].size, blowup_step)
     return out
 
 

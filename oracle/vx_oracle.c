@@ -2212,3 +2212,212 @@ void or_oods_sample(or_channel *ch, const uint32_t *const *cols, const uint32_t 
     or_channel_mix_felts(ch, values, 2 * (size_t)n_cols);
 }
 
+
+/* ======================================================================
+ * Cairo VM AIR constraint quotient: cuda/cairo_constraint.cu.
+ * 34 trace columns (pc, ap, fp, inst_lo, inst_hi, 15 flags, dst_addr, dst,
+ * op0_addr, op0, op1_addr, op1, res, off0..2, dst_inv, dict key/new/active,
+ * :8-53); interaction columns as QM31 SoA (4 words each); challenges
+ * z_mem, alpha_mem, alpha_mem^2, z_rc, z_dict, alpha_dict (:179-180).
+ * ====================================================================== */
+enum {
+    CA_PC = 0, CA_AP = 1, CA_FP = 2, CA_INST_LO = 3, CA_INST_HI = 4, CA_FLAGS = 5, CA_DST_ADDR = 20,
+    CA_DST = 21, CA_OP0_ADDR = 22, CA_OP0 = 23, CA_OP1_ADDR = 24, CA_OP1 = 25, CA_RES = 26, CA_OFF0 = 27,
+    CA_OFF1 = 28, CA_OFF2 = 29, CA_DST_INV = 30, CA_DICT_KEY = 31, CA_DICT_NEW = 32, CA_DICT_ACTIVE = 33
+};
+#define CA_BIAS 0x8000u
+
+/* Z_H(x) = f_{log_n}(x) + 1 with f_0 = x, f_{k+1} = 2 f_k^2 - 1 (:64-75) */
+static uint32_t cairo_vanishing(uint32_t x, uint32_t log_n) {
+    uint32_t v = x;
+    for (uint32_t k = 0; k < log_n; k++) {
+        uint32_t s = m31_mul(v, v);
+        v = m31_sub(m31_add(s, s), 1);
+    }
+    return m31_add(v, 1);
+}
+/* :77-118, :990-1019: Z_H (or its inverse, 0 for 0) at eval position i,
+ * the point init * step^{bitrev(i)} */
+void or_cairo_vanishing(const uint32_t init[2], const uint32_t step[2], uint32_t *out, uint32_t log_eval,
+                        uint32_t log_n, int inverse) {
+    cpt in = {init[0], init[1]}, st = {step[0], step[1]};
+    uint64_t n = 1ull << log_eval;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        cpt p = c_mul(in, c_pow(st, bitrev((uint64_t)i, log_eval)));
+        uint32_t z = cairo_vanishing(p.x, log_n);
+        out[i] = inverse ? (z == 0 ? 0 : m31_inv(z)) : z;
+    }
+}
+/* :1037-1046 */
+void or_axpy_m31(uint32_t s, const uint32_t *x, uint32_t *y, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++) y[i] = m31_add(y[i], m31_mul(s, x[i]));
+}
+
+static qm31 q_from(uint32_t a) { return (qm31){{a, 0, 0, 0}}; }
+
+/* Constraints 0..30 of one row: the M31 part shared by both kernels
+ * (:243-462 and the chunk kernel's copy). Each c is weighted by the next
+ * alpha (qm31_mul_m31). */
+static qm31 cairo_m31_part(const uint32_t *const cols[34], uint64_t i, uint64_t ni, uint32_t tf,
+                           const uint32_t *alpha, int *ci) {
+    uint32_t f[15];
+    for (int j = 0; j < 15; j++) f[j] = cols[CA_FLAGS + j][i];
+    const uint32_t pc = cols[CA_PC][i], ap = cols[CA_AP][i], fp = cols[CA_FP][i];
+    const uint32_t dst = cols[CA_DST][i], op0 = cols[CA_OP0][i], op1 = cols[CA_OP1][i], res = cols[CA_RES][i];
+    const uint32_t npc = cols[CA_PC][ni], nap = cols[CA_AP][ni], nfp = cols[CA_FP][ni];
+    const uint32_t off0 = cols[CA_OFF0][i], off1 = cols[CA_OFF1][i], off2 = cols[CA_OFF2][i];
+    const uint32_t dinv = cols[CA_DST_INV][i];
+    const uint32_t op1_imm = f[2], res_add = f[5], res_mul = f[6], jabs = f[7], jrel = f[8], jnz = f[9];
+    const uint32_t ap_add = f[10], ap_add1 = f[11], call = f[12], ret = f[13], asrt = f[14];
+    uint32_t c[31];
+    int k = 0;
+    for (int j = 0; j < 15; j++) c[k++] = m31_mul(f[j], m31_sub(1, f[j]));
+    {   /* 15 result */
+        uint32_t exp = m31_add(m31_add(m31_mul(m31_sub(m31_sub(1, res_add), res_mul), op1),
+                                       m31_mul(res_add, m31_add(op0, op1))),
+                               m31_mul(res_mul, m31_mul(op0, op1)));
+        c[k++] = m31_mul(m31_sub(1, jnz), m31_sub(res, exp));
+    }
+    {   /* 16 pc update */
+        uint32_t notj = m31_sub(m31_sub(m31_sub(1, jabs), jrel), jnz);
+        uint32_t tgt = m31_add(m31_add(m31_mul(notj, m31_add(pc, m31_add(1, op1_imm))), m31_mul(jabs, res)),
+                               m31_mul(jrel, m31_add(pc, res)));
+        uint32_t nonjnz = m31_mul(m31_sub(1, jnz), m31_sub(npc, tgt));
+        uint32_t jz = m31_mul(jnz, m31_mul(dst, m31_sub(npc, m31_add(pc, op1))));
+        c[k++] = m31_mul(m31_add(nonjnz, jz), tf);
+    }
+    /* 17 ap update, 18 fp update, 19 assert_eq */
+    c[k++] = m31_mul(m31_sub(nap, m31_add(m31_add(m31_add(ap, m31_mul(ap_add, res)), ap_add1), m31_mul(call, 2))), tf);
+    c[k++] = m31_mul(m31_sub(nfp, m31_add(m31_add(m31_mul(m31_sub(m31_sub(1, call), ret), fp),
+                                                 m31_mul(call, m31_add(ap, 2))), m31_mul(ret, dst))), tf);
+    c[k++] = m31_mul(asrt, m31_sub(dst, res));
+    /* 20..22 operand addresses, offsets biased by 2^15 */
+    c[k++] = m31_sub(cols[CA_DST_ADDR][i],
+                     m31_sub(m31_add(m31_add(m31_mul(m31_sub(1, f[0]), ap), m31_mul(f[0], fp)), off0), CA_BIAS));
+    c[k++] = m31_sub(cols[CA_OP0_ADDR][i],
+                     m31_sub(m31_add(m31_add(m31_mul(m31_sub(1, f[1]), ap), m31_mul(f[1], fp)), off1), CA_BIAS));
+    {
+        uint32_t dflt = m31_sub(m31_sub(m31_sub(1, op1_imm), f[3]), f[4]);
+        uint32_t base = m31_add(m31_add(m31_add(m31_mul(op1_imm, pc), m31_mul(f[3], fp)), m31_mul(f[4], ap)),
+                                m31_mul(dflt, op0));
+        c[k++] = m31_sub(cols[CA_OP1_ADDR][i], m31_sub(m31_add(base, off2), CA_BIAS));
+    }
+    /* 23 jnz fall-through, 24 jnz inverse */
+    c[k++] = m31_mul(m31_mul(jnz, m31_mul(m31_sub(1, m31_mul(dst, dinv)), m31_sub(npc, m31_add(pc, m31_add(1, op1_imm))))),
+                     tf);
+    c[k++] = m31_mul(jnz, m31_mul(dst, m31_sub(1, m31_mul(dst, dinv))));
+    /* 25..29 exclusivity */
+    c[k++] = m31_mul(op1_imm, f[3]);
+    c[k++] = m31_mul(op1_imm, f[4]);
+    c[k++] = m31_mul(f[3], f[4]);
+    c[k++] = m31_add(m31_add(m31_mul(jabs, jrel), m31_mul(jabs, jnz)), m31_mul(jrel, jnz));
+    c[k++] = m31_add(m31_add(m31_mul(call, ret), m31_mul(call, asrt)), m31_mul(ret, asrt));
+    {   /* 30 instruction word: 2^31 = 1, 2^32 = 2, 2^(48+j) = 2^(17+j), 2^62 = 1 */
+        uint32_t rhs = m31_add(m31_add(off0, m31_mul(off1, 1u << 16)), m31_mul(off2, 2));
+        for (int j = 0; j < 14; j++) rhs = m31_add(rhs, m31_mul(f[j], 1u << (17 + j)));
+        rhs = m31_add(rhs, f[14]);
+        c[k++] = m31_sub(m31_add(cols[CA_INST_LO][i], cols[CA_INST_HI][i]), rhs);
+    }
+    qm31 acc = {{0, 0, 0, 0}};
+    for (int j = 0; j < k; j++) acc = q_add(acc, q_scale(q_load(alpha + 4 * (*ci)++), c[j]));
+    return acc;
+}
+
+/* QM31 column value at row i of a SoA group of 4 columns */
+static qm31 q_at(const uint32_t *const g[4], uint64_t i) { return (qm31){{g[0][i], g[1][i], g[2][i], g[3][i]}}; }
+
+/* cuda_cairo_quotient / _slab (:157-580, wrappers :860-960): 40 weighted
+ * sub-constraints; LogUp (31a-d), range check (32a-c) and dict (34) in
+ * polynomial form over the interaction columns. inter: 32 column pointers in
+ * FFI order (S_logup, T1, T2, T3, S_rc, U1, U2, S_dict; 4 words each).
+ * Full mode: rows 0..n of n, next row (i + blowup_step) mod n. Slab mode:
+ * the columns are slabs of n + blowup_step rows starting at global row
+ * chunk_offset, next row i + blowup_step; vh_inv, trans and out are global. */
+void or_cairo_quotient(const uint32_t *const cols[34], const uint32_t *const inter[32], uint32_t *const out[4],
+                       const uint32_t alpha[160], const uint32_t *vh_inv, const uint32_t *trans,
+                       const uint32_t chal[24], uint64_t n, uint32_t blowup_step, uint64_t chunk_offset,
+                       int slab_mode) {
+    const qm31 z_mem = q_load(chal), a_mem = q_load(chal + 4), a_mem2 = q_load(chal + 8);
+    const qm31 z_rc = q_load(chal + 12), z_dict = q_load(chal + 16), a_dict = q_load(chal + 20);
+    const qm31 one = {{1, 0, 0, 0}};
+#pragma omp parallel for schedule(static)
+    for (int64_t si = 0; si < (int64_t)n; si++) {
+        const uint64_t i = (uint64_t)si;
+        const uint64_t g = slab_mode ? chunk_offset + i : i;
+        const uint64_t ni = slab_mode ? i + blowup_step : (i + blowup_step) % n;
+        const uint32_t tf = trans[g];
+        int ci = 0;
+        qm31 acc = cairo_m31_part(cols, i, ni, tf, alpha, &ci);
+        const uint32_t pc = cols[CA_PC][i], dst = cols[CA_DST][i], op0 = cols[CA_OP0][i], op1 = cols[CA_OP1][i];
+        const qm31 s = q_at(inter + 0, i), sn = q_at(inter + 0, ni);
+        const qm31 t1 = q_at(inter + 4, i), t2 = q_at(inter + 8, i), t3 = q_at(inter + 12, i);
+        const qm31 d0 = q_sub(z_mem, q_add(q_from(pc), q_add(q_scale(a_mem, cols[CA_INST_LO][i]),
+                                                              q_scale(a_mem2, cols[CA_INST_HI][i]))));
+        const qm31 d1 = q_sub(z_mem, q_add(q_from(cols[CA_DST_ADDR][i]), q_scale(a_mem, dst)));
+        const qm31 d2 = q_sub(z_mem, q_add(q_from(cols[CA_OP0_ADDR][i]), q_scale(a_mem, op0)));
+        const qm31 d3 = q_sub(z_mem, q_add(q_from(cols[CA_OP1_ADDR][i]), q_scale(a_mem, op1)));
+        qm31 cq[8];
+        cq[0] = q_sub(q_mul(q_sub(t1, s), d0), one);
+        cq[1] = q_sub(q_mul(q_sub(t2, t1), d1), one);
+        cq[2] = q_sub(q_mul(q_sub(t3, t2), d2), one);
+        cq[3] = q_scale(q_sub(q_mul(q_sub(sn, t3), d3), one), tf);
+        const qm31 r = q_at(inter + 16, i), rn = q_at(inter + 16, ni);
+        const qm31 u1 = q_at(inter + 20, i), u2 = q_at(inter + 24, i);
+        cq[4] = q_sub(q_mul(q_sub(u1, r), q_sub(z_rc, q_from(cols[CA_OFF0][i]))), one);
+        cq[5] = q_sub(q_mul(q_sub(u2, u1), q_sub(z_rc, q_from(cols[CA_OFF1][i]))), one);
+        cq[6] = q_scale(q_sub(q_mul(q_sub(rn, u2), q_sub(z_rc, q_from(cols[CA_OFF2][i]))), one), tf);
+        for (int k = 0; k < 7; k++) acc = q_add(acc, q_mul(q_load(alpha + 4 * ci++), cq[k]));
+        const uint32_t act = cols[CA_DICT_ACTIVE][i];
+        acc = q_add(acc, q_scale(q_load(alpha + 4 * ci++), m31_mul(act, m31_sub(1, act))));
+        const qm31 ds = q_at(inter + 28, i), dsn = q_at(inter + 28, ni);
+        const qm31 entry = q_add(q_from(cols[CA_DICT_KEY][i]), q_scale(a_dict, cols[CA_DICT_NEW][i]));
+        const qm31 c34 = q_scale(q_sub(q_mul(q_sub(dsn, ds), q_sub(z_dict, entry)), q_from(act)), tf);
+        acc = q_add(acc, q_mul(q_load(alpha + 4 * ci++), c34));
+        const qm31 q = q_scale(acc, vh_inv[g]);
+        for (int k = 0; k < 4; k++) out[k][g] = q.v[k];
+    }
+}
+
+/* cuda_cairo_quotient_chunk (:582-850): rows offset .. offset + chunk_n of
+ * global_n, 35 weighted constraints; LogUp (31) and range check (32) as one
+ * step constraint each with the inverse-sum delta (:120-153); outputs at the
+ * chunk-local row. inter: 12 pointers (S_logup, S_rc, S_dict; 4 words each). */
+void or_cairo_quotient_chunk(const uint32_t *const cols[34], const uint32_t *const inter[12],
+                             uint32_t *const out[4], const uint32_t alpha[140], const uint32_t *vh_inv,
+                             const uint32_t *trans, const uint32_t chal[24], uint64_t offset, uint64_t chunk_n,
+                             uint64_t global_n, uint32_t blowup_step) {
+    const qm31 z_mem = q_load(chal), a_mem = q_load(chal + 4), a_mem2 = q_load(chal + 8);
+    const qm31 z_rc = q_load(chal + 12), z_dict = q_load(chal + 16), a_dict = q_load(chal + 20);
+#pragma omp parallel for schedule(static)
+    for (int64_t sl = 0; sl < (int64_t)chunk_n; sl++) {
+        const uint64_t i = offset + (uint64_t)sl;
+        const uint64_t ni = (i + blowup_step) % global_n;
+        const uint32_t tf = trans[i];
+        int ci = 0;
+        qm31 acc = cairo_m31_part(cols, i, ni, tf, alpha, &ci);
+        const uint32_t pc = cols[CA_PC][i], dst = cols[CA_DST][i], op0 = cols[CA_OP0][i], op1 = cols[CA_OP1][i];
+        const qm31 d0 = q_sub(z_mem, q_add(q_from(pc), q_add(q_scale(a_mem, cols[CA_INST_LO][i]),
+                                                              q_scale(a_mem2, cols[CA_INST_HI][i]))));
+        const qm31 d1 = q_sub(z_mem, q_add(q_from(cols[CA_DST_ADDR][i]), q_scale(a_mem, dst)));
+        const qm31 d2 = q_sub(z_mem, q_add(q_from(cols[CA_OP0_ADDR][i]), q_scale(a_mem, op0)));
+        const qm31 d3 = q_sub(z_mem, q_add(q_from(cols[CA_OP1_ADDR][i]), q_scale(a_mem, op1)));
+        const qm31 dl = q_add(q_add(q_add(q_inv(d0), q_inv(d1)), q_inv(d2)), q_inv(d3));
+        const qm31 c31 = q_scale(q_sub(q_sub(q_at(inter + 0, ni), q_at(inter + 0, i)), dl), tf);
+        acc = q_add(acc, q_mul(q_load(alpha + 4 * ci++), c31));
+        const qm31 dr = q_add(q_add(q_inv(q_sub(z_rc, q_from(cols[CA_OFF0][i]))),
+                                    q_inv(q_sub(z_rc, q_from(cols[CA_OFF1][i])))),
+                              q_inv(q_sub(z_rc, q_from(cols[CA_OFF2][i]))));
+        const qm31 c32 = q_scale(q_sub(q_sub(q_at(inter + 4, ni), q_at(inter + 4, i)), dr), tf);
+        acc = q_add(acc, q_mul(q_load(alpha + 4 * ci++), c32));
+        const uint32_t act = cols[CA_DICT_ACTIVE][i];
+        acc = q_add(acc, q_scale(q_load(alpha + 4 * ci++), m31_mul(act, m31_sub(1, act))));
+        const qm31 entry = q_add(q_from(cols[CA_DICT_KEY][i]), q_scale(a_dict, cols[CA_DICT_NEW][i]));
+        const qm31 c34 = q_scale(q_sub(q_mul(q_sub(q_at(inter + 8, ni), q_at(inter + 8, i)), q_sub(z_dict, entry)),
+                                       q_from(act)),
+                                 tf);
+        acc = q_add(acc, q_mul(q_load(alpha + 4 * ci++), c34));
+        const qm31 q = q_scale(acc, vh_inv[i]);
+        for (int k = 0; k < 4; k++) out[k][sl] = q.v[k];
+    }
+}

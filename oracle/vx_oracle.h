@@ -24,6 +24,10 @@
  *                         SplitMix64 reference output, closed forms
  *   lifted Merkle         Python hashlib over the lifted rows; the LDE is the
  *                         pinned single-column LDE per column
+ *   cairo quotient        independent Python restatement (tests/golden/
+ *                         pyref_cairo.py) of the reference's host-side
+ *                         evaluation; valid Cairo rows give zero, one changed
+ *                         value gives nonzero; vanishing at the trace domain
  *   ntt_evaluate_at       the full transform (or_ntt_evaluate, or_lde) at
  *                         every output for small sizes and three coset kinds,
  *                         and the roundtrip to the trace values
@@ -156,6 +160,20 @@ void or_lde(const uint32_t *in, uint32_t log_n, uint32_t blowup_bits, uint32_t *
 /* Outputs idx[0..n_idx) of or_ntt_evaluate on the domain of coset dom applied
  * to the zero-padded coefficients c[0 .. 2^log_n), O(2^log_n) per group of
  * outputs sharing bits >= 2 (closed-form twiddles, product basis). */
+/* Cairo VM AIR quotient (cuda/cairo_constraint.cu): 34 trace columns,
+ * interaction columns as QM31 SoA, 40 (full / slab) or 35 (chunk) weighted
+ * constraints, divided by Z_H. */
+void or_cairo_vanishing(const uint32_t init[2], const uint32_t step[2], uint32_t *out, uint32_t log_eval,
+                        uint32_t log_n, int inverse);
+void or_axpy_m31(uint32_t s, const uint32_t *x, uint32_t *y, uint64_t n);
+void or_cairo_quotient(const uint32_t *const cols[34], const uint32_t *const inter[32], uint32_t *const out[4],
+                       const uint32_t alpha[160], const uint32_t *vh_inv, const uint32_t *trans,
+                       const uint32_t chal[24], uint64_t n, uint32_t blowup_step, uint64_t chunk_offset,
+                       int slab_mode);
+void or_cairo_quotient_chunk(const uint32_t *const cols[34], const uint32_t *const inter[12],
+                             uint32_t *const out[4], const uint32_t alpha[140], const uint32_t *vh_inv,
+                             const uint32_t *trans, const uint32_t chal[24], uint64_t offset, uint64_t chunk_n,
+                             uint64_t global_n, uint32_t blowup_step);
 void or_ntt_evaluate_at(const uint32_t *c, uint32_t log_n, const uint32_t dom5[5], const uint64_t *idx,
                         uint32_t n_idx, uint32_t *out);
 void or_build_leaves_lifted(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
