@@ -560,6 +560,83 @@ int vx_merkle_root_lifted(const uint32_t *const *h_cols, const uint32_t *h_log_s
 int vx_merkle_root_lifted_noalloc(const uint32_t *const *h_cols, const uint32_t *h_log_sizes, uint32_t n_cols,
                                   uint32_t *d_root, vx_stream_t stream);
 
+/* ------------------------------------------------------ Cairo VM AIR quotient */
+
+/* Replaces cuda_cairo_quotient (src/cuda/ffi.rs:592-609, cuda/cairo_constraint.cu:
+ * 157-580, caller src/cairo_air/prover.rs:2046-2066). Q[i] = (sum of the 40
+ * alpha-weighted constraints of row i) * vh_inv[i] into out0..out3 (QM31 SoA),
+ * i < n; the next row is (i + blowup_step) mod n. d_trace_cols is a DEVICE
+ * array of the 34 trace-column pointers (pc, ap, fp, inst_lo, inst_hi, 15
+ * flags, dst_addr, dst, op0_addr, op0, op1_addr, op1, res, off0..2, dst_inv,
+ * dict key / new / active), read back after the stream's earlier work.
+ * Interaction columns: S_logup, T1, T2, T3, S_rc, U1, U2, S_dict (4 SoA words
+ * each). d_alpha: 40 QM31; d_challenges: z_mem, alpha_mem, alpha_mem^2, z_rc,
+ * z_dict, alpha_dict (24 words). Asynchronous. */
+int vx_cairo_quotient(const uint32_t *const *d_trace_cols, const uint32_t *s_logup0, const uint32_t *s_logup1,
+                      const uint32_t *s_logup2, const uint32_t *s_logup3, const uint32_t *t1l0, const uint32_t *t1l1,
+                      const uint32_t *t1l2, const uint32_t *t1l3, const uint32_t *t2l0, const uint32_t *t2l1,
+                      const uint32_t *t2l2, const uint32_t *t2l3, const uint32_t *t3l0, const uint32_t *t3l1,
+                      const uint32_t *t3l2, const uint32_t *t3l3, const uint32_t *s_rc0, const uint32_t *s_rc1,
+                      const uint32_t *s_rc2, const uint32_t *s_rc3, const uint32_t *u1r0, const uint32_t *u1r1,
+                      const uint32_t *u1r2, const uint32_t *u1r3, const uint32_t *u2r0, const uint32_t *u2r1,
+                      const uint32_t *u2r2, const uint32_t *u2r3, const uint32_t *s_dict0, const uint32_t *s_dict1,
+                      const uint32_t *s_dict2, const uint32_t *s_dict3, uint32_t *out0, uint32_t *out1,
+                      uint32_t *out2, uint32_t *out3, const uint32_t *d_alpha, const uint32_t *d_vh_inv,
+                      const uint32_t *d_trans_factor, const uint32_t *d_challenges, uint32_t n,
+                      uint32_t blowup_step, vx_stream_t stream);
+/* Replaces cuda_cairo_quotient_slab (ffi.rs:611-638): the columns are slabs
+ * of chunk_n + blowup_step rows starting at global row chunk_offset (next
+ * row = local row + blowup_step); vh_inv, trans_factor and out are global
+ * and indexed by chunk_offset + local row. */
+int vx_cairo_quotient_slab(const uint32_t *const *d_trace_cols, const uint32_t *s_logup0, const uint32_t *s_logup1,
+                           const uint32_t *s_logup2, const uint32_t *s_logup3, const uint32_t *t1l0,
+                           const uint32_t *t1l1, const uint32_t *t1l2, const uint32_t *t1l3, const uint32_t *t2l0,
+                           const uint32_t *t2l1, const uint32_t *t2l2, const uint32_t *t2l3, const uint32_t *t3l0,
+                           const uint32_t *t3l1, const uint32_t *t3l2, const uint32_t *t3l3, const uint32_t *s_rc0,
+                           const uint32_t *s_rc1, const uint32_t *s_rc2, const uint32_t *s_rc3, const uint32_t *u1r0,
+                           const uint32_t *u1r1, const uint32_t *u1r2, const uint32_t *u1r3, const uint32_t *u2r0,
+                           const uint32_t *u2r1, const uint32_t *u2r2, const uint32_t *u2r3,
+                           const uint32_t *s_dict0, const uint32_t *s_dict1, const uint32_t *s_dict2,
+                           const uint32_t *s_dict3, uint32_t *out0, uint32_t *out1, uint32_t *out2, uint32_t *out3,
+                           const uint32_t *d_alpha, const uint32_t *d_vh_inv, const uint32_t *d_trans_factor,
+                           const uint32_t *d_challenges, uint32_t chunk_n, uint32_t blowup_step,
+                           uint32_t chunk_offset, vx_stream_t stream);
+/* Replaces cuda_cairo_quotient_chunk (ffi.rs:640-655, cairo_constraint.cu:
+ * 582-850): rows offset .. offset + chunk_n of global_n in the chunked form
+ * (35 weighted constraints; the LogUp and range-check steps as S_next - S
+ * minus the sum of inverse denominators), output at the chunk-local row. */
+int vx_cairo_quotient_chunk(const uint32_t *const *d_trace_cols, const uint32_t *s_logup0,
+                            const uint32_t *s_logup1, const uint32_t *s_logup2, const uint32_t *s_logup3,
+                            const uint32_t *s_rc0, const uint32_t *s_rc1, const uint32_t *s_rc2,
+                            const uint32_t *s_rc3, const uint32_t *s_dict0, const uint32_t *s_dict1,
+                            const uint32_t *s_dict2, const uint32_t *s_dict3, uint32_t *out0, uint32_t *out1,
+                            uint32_t *out2, uint32_t *out3, const uint32_t *d_alpha, const uint32_t *d_vh_inv,
+                            const uint32_t *d_trans_factor, const uint32_t *d_challenges, uint32_t offset,
+                            uint32_t chunk_n, uint32_t global_n, uint32_t blowup_step, vx_stream_t stream);
+/* Fused form of vx_cairo_quotient followed by the commitment of its four
+ * output columns (the reference runs cuda_cairo_quotient, then commits the
+ * quotient columns, src/cairo_air/prover.rs:2046-2066 and the quotient
+ * commitment after it): one subtree launch computes Q of rows 0..2^log_n
+ * (next row (i + blowup_step) mod 2^log_n), stores it in out0..out3
+ * (16-byte aligned) and hashes QM31 leaves Blake2s(4 words); tree levels
+ * store_lo..log_n land in d_levels (vx_merkle_levels_words(log_n, store_lo)
+ * words), the root in d_root. h_inter is a HOST array of the 32 interaction
+ * column pointers in vx_cairo_quotient's order. Synchronous. */
+int vx_cairo_quotient_commit(const uint32_t *const *d_trace_cols, const uint32_t *const *h_inter, uint32_t *out0,
+                             uint32_t *out1, uint32_t *out2, uint32_t *out3, const uint32_t *d_alpha,
+                             const uint32_t *d_vh_inv, const uint32_t *d_trans_factor, const uint32_t *d_challenges,
+                             uint32_t log_n, uint32_t blowup_step, uint32_t store_lo, uint32_t *d_levels,
+                             uint32_t *d_root, vx_stream_t stream);
+/* Replace cuda_compute_vanishing_inv / cuda_compute_vanishing (ffi.rs:570-586,
+ * cairo_constraint.cu:77-118,990-1019): Z_H of half_coset(log_n) (or its
+ * inverse, 0 for 0) at the point initial * step^bitrev(i) for i < 2^log_eval. */
+int vx_compute_vanishing_inv(uint32_t initial_x, uint32_t initial_y, uint32_t step_x, uint32_t step_y,
+                             uint32_t *d_out, uint32_t log_eval, uint32_t log_n, vx_stream_t stream);
+int vx_compute_vanishing(uint32_t initial_x, uint32_t initial_y, uint32_t step_x, uint32_t step_y, uint32_t *d_out,
+                         uint32_t log_eval, uint32_t log_n, vx_stream_t stream);
+/* Replaces cuda_axpy_m31 (ffi.rs:588-590, cairo_constraint.cu:1037-1057): y += scalar * x. */
+int vx_axpy_m31(uint32_t scalar, const uint32_t *d_x, uint32_t *d_y, uint32_t n, vx_stream_t stream);
+
 /* Resident batch (the batched-commit workload): inputs, coefficients and
  * evaluations of n_cols columns (log sizes sorted ascending, 14..24) kept in
  * HBM; each run is four work-list LDE launches, one bit-reverse launch and one

@@ -17,13 +17,16 @@ Sources, per section:
                the batched-columns proof (KRAK v4): pyref.prove_batch at tiny
                sizes, the C oracle at the workload's sizes
 
+  cairo_commit_c  the Cairo-shaped commitment at trace log_n 20 (lo / hi / dict
+               trace trees and the quotient tree), C oracle
   serving_e    one request of every class of the serving mix (Fibonacci
                log_n 14..22, 34-column AIR log_n 14..20, PoW 20), C oracle
 
-Run from the repo root:  python tests/golden/gen_golden.py [--big | --serving]
+Run from the repo root:  python tests/golden/gen_golden.py [--big | --serving | --cairo]
 (--big adds the log_n = 20, 24 and 25 (blowup 1) oracle digests, the
 log_n = 24 PoW-26 digest and the log_n = 20 multi-column AIR digest, which
-take minutes; --serving recomputes only serving_e and keeps the rest.)
+take minutes; --serving / --cairo recompute only serving_e / cairo_commit_c
+and keep the rest.)
 """
 import hashlib
 import json
@@ -91,9 +94,38 @@ def serving_digests():
     return rows
 
 
+CAIRO_SEED, CAIRO_LOG_EVAL, CAIRO_STEP = 0xCA1F0, 22, 4
+
+
+def cairo_commit_digest():
+    """The Cairo-shaped commitment (trace log_n 20, blowup 4: 2^22 rows):
+    column c < 70 of the synthetic generator with seed CAIRO_SEED (34 trace
+    columns, 32 interaction columns, Z_H^-1, transition factors), alphas
+    and challenges from columns 68 and 69; roots of the lo / hi / dict trace
+    trees (columns 0-15, 16-30, 31-33) and of the quotient's QM31 tree."""
+    import pyoracle
+    L = CAIRO_LOG_EVAL
+    cols = [pyoracle.d2_column(CAIRO_SEED, c, L) for c in range(68)]
+    alpha = pyoracle.d2_column(CAIRO_SEED, 68, 8)[:160]
+    chal = pyoracle.d2_column(CAIRO_SEED, 69, 5)[:24]
+    roots = [pyoracle.merkle_root(cols[lo:hi]) for lo, hi in ((0, 16), (16, 31), (31, 34))]
+    q = pyoracle.cairo_quotient(cols[:34], cols[34:66], alpha, cols[66], cols[67], chal, CAIRO_STEP)
+    roots.append(pyoracle.merkle_root(q))
+    print("cairo commitment digest", flush=True)
+    return {"seed": CAIRO_SEED, "log_eval": L, "blowup_step": CAIRO_STEP,
+            "roots": [[int(x) for x in r] for r in roots]}
+
+
 def main():
     big = "--big" in sys.argv
     out = os.path.join(HERE, "golden.json")
+    if "--cairo" in sys.argv:
+        g = json.load(open(out))
+        g["cairo_commit_c"] = cairo_commit_digest()
+        with open(out, "w") as f:
+            json.dump(g, f, separators=(",", ":"))
+        print("wrote", out, os.path.getsize(out), "bytes")
+        return
     if "--serving" in sys.argv:
         # only (re)compute the serving-mix digests, keep everything else
         g = json.load(open(out))
@@ -234,6 +266,9 @@ def main():
 
     if os.path.exists(out):
         g["serving_e"] = json.load(open(out)).get("serving_e", [])
+        g["cairo_commit_c"] = json.load(open(out)).get("cairo_commit_c")
+    if big:
+        g["cairo_commit_c"] = cairo_commit_digest()
     if not big and os.path.exists(out):
         old = json.load(open(out))
         keep = [p for p in old.get("proofs_c", []) if p["log_n"] >= 20]

@@ -122,6 +122,13 @@ SIGNATURES = {
     "vx_build_leaves_lifted": (ip, [vp, vp, u32, u32, vp, vp]),
     "vx_merkle_root_lifted": (ip, [vp, vp, u32, vp, vp]),
     "vx_merkle_root_lifted_noalloc": (ip, [vp, vp, u32, vp, vp]),
+    "vx_cairo_quotient": (ip, [vp] + [vp] * 32 + [vp] * 4 + [vp] * 4 + [u32, u32, vp]),
+    "vx_cairo_quotient_slab": (ip, [vp] + [vp] * 32 + [vp] * 4 + [vp] * 4 + [u32, u32, u32, vp]),
+    "vx_cairo_quotient_chunk": (ip, [vp] + [vp] * 12 + [vp] * 4 + [vp] * 4 + [u32, u32, u32, u32, vp]),
+    "vx_cairo_quotient_commit": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, u32, vp, vp, vp]),
+    "vx_compute_vanishing_inv": (ip, [u32, u32, u32, u32, vp, u32, u32, vp]),
+    "vx_compute_vanishing": (ip, [u32, u32, u32, u32, vp, u32, u32, vp]),
+    "vx_axpy_m31": (ip, [u32, vp, vp, u32, vp]),
     "vx_batch_create": (ip, [vp, u32, u32, ip, vp, C.POINTER(vp)]),
     "vx_batch_destroy": (ip, [vp]),
     "vx_batch_column": (ip, [vp, u32, ip, C.POINTER(vp), C.POINTER(u64)]),

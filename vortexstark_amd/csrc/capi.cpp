@@ -10,6 +10,7 @@
 #include "batch.h"
 #include "tune.h"
 #include "server.h"
+#include "vx_cairo.h"
 #include "vx_d2.h"
 
 using namespace vx;
@@ -1282,6 +1283,147 @@ int vx_merkle_root_lifted(const uint32_t* const* h_cols, const uint32_t* h_log_s
 int vx_merkle_root_lifted_noalloc(const uint32_t* const* h_cols, const uint32_t* h_log_sizes, uint32_t n_cols,
                                   uint32_t* d_root, vx_stream_t st) {
   return guard([&] { merkle_root_lifted_noalloc(h_cols, h_log_sizes, n_cols, d_root, S(st)); });
+}
+
+// ------------------------------------------------------------ Cairo quotient
+// The 34 trace-column pointers arrive as the reference's uploaded device
+// array (read back after the stream's earlier work); interaction columns,
+// outputs, alphas, Z_H^-1, transition factors and challenges are device
+// pointers as in cuda_cairo_quotient.
+static CairoArgs cairo_args(const uint32_t* const* d_trace_cols, const uint32_t* const* inter, int n_inter,
+                            uint32_t* const out[4], const uint32_t* alpha, const uint32_t* vh, const uint32_t* tf,
+                            const uint32_t* chal, hipStream_t s) {
+  require(d_trace_cols && alpha && vh && tf && chal, "cairo_quotient: null argument");
+  std::vector<uint32_t*> cols = fetch_ptrs(d_trace_cols, CA_COLS, s);
+  CairoArgs a{};
+  for (uint32_t c = 0; c < CA_COLS; c++) {
+    require(cols[c] != nullptr, "cairo_quotient: null trace column");
+    a.col[c] = cols[c];
+  }
+  for (int k = 0; k < n_inter; k++) {
+    require(inter[k] != nullptr, "cairo_quotient: null interaction column");
+    a.inter[k] = inter[k];
+  }
+  for (int k = 0; k < 4; k++) {
+    require(out[k] != nullptr, "cairo_quotient: null output column");
+    a.out[k] = out[k];
+  }
+  a.alpha = alpha;
+  a.vh_inv = vh;
+  a.trans = tf;
+  a.chal = chal;
+  return a;
+}
+
+int vx_cairo_quotient(const uint32_t* const* d_trace_cols, const uint32_t* sl0, const uint32_t* sl1,
+                      const uint32_t* sl2, const uint32_t* sl3, const uint32_t* t1l0, const uint32_t* t1l1,
+                      const uint32_t* t1l2, const uint32_t* t1l3, const uint32_t* t2l0, const uint32_t* t2l1,
+                      const uint32_t* t2l2, const uint32_t* t2l3, const uint32_t* t3l0, const uint32_t* t3l1,
+                      const uint32_t* t3l2, const uint32_t* t3l3, const uint32_t* sr0, const uint32_t* sr1,
+                      const uint32_t* sr2, const uint32_t* sr3, const uint32_t* u1r0, const uint32_t* u1r1,
+                      const uint32_t* u1r2, const uint32_t* u1r3, const uint32_t* u2r0, const uint32_t* u2r1,
+                      const uint32_t* u2r2, const uint32_t* u2r3, const uint32_t* sd0, const uint32_t* sd1,
+                      const uint32_t* sd2, const uint32_t* sd3, uint32_t* out0, uint32_t* out1, uint32_t* out2,
+                      uint32_t* out3, const uint32_t* d_alpha, const uint32_t* d_vh_inv, const uint32_t* d_trans,
+                      const uint32_t* d_challenges, uint32_t n, uint32_t blowup_step, vx_stream_t st) {
+  return guard([&] {
+    require(n >= 1 && blowup_step <= n, "cairo_quotient: need n >= 1 and blowup_step <= n");
+    const uint32_t* inter[32] = {sl0,  sl1,  sl2,  sl3,  t1l0, t1l1, t1l2, t1l3, t2l0, t2l1, t2l2,
+                                 t2l3, t3l0, t3l1, t3l2, t3l3, sr0,  sr1,  sr2,  sr3,  u1r0, u1r1,
+                                 u1r2, u1r3, u2r0, u2r1, u2r2, u2r3, sd0,  sd1,  sd2,  sd3};
+    uint32_t* out[4] = {out0, out1, out2, out3};
+    CairoArgs a = cairo_args(d_trace_cols, inter, 32, out, d_alpha, d_vh_inv, d_trans, d_challenges, S(st));
+    a.wrap = n;
+    a.blowup_step = blowup_step;
+    cairo_quotient(a, n, S(st));
+  });
+}
+
+int vx_cairo_quotient_slab(const uint32_t* const* d_trace_cols, const uint32_t* sl0, const uint32_t* sl1,
+                           const uint32_t* sl2, const uint32_t* sl3, const uint32_t* t1l0, const uint32_t* t1l1,
+                           const uint32_t* t1l2, const uint32_t* t1l3, const uint32_t* t2l0, const uint32_t* t2l1,
+                           const uint32_t* t2l2, const uint32_t* t2l3, const uint32_t* t3l0, const uint32_t* t3l1,
+                           const uint32_t* t3l2, const uint32_t* t3l3, const uint32_t* sr0, const uint32_t* sr1,
+                           const uint32_t* sr2, const uint32_t* sr3, const uint32_t* u1r0, const uint32_t* u1r1,
+                           const uint32_t* u1r2, const uint32_t* u1r3, const uint32_t* u2r0, const uint32_t* u2r1,
+                           const uint32_t* u2r2, const uint32_t* u2r3, const uint32_t* sd0, const uint32_t* sd1,
+                           const uint32_t* sd2, const uint32_t* sd3, uint32_t* out0, uint32_t* out1, uint32_t* out2,
+                           uint32_t* out3, const uint32_t* d_alpha, const uint32_t* d_vh_inv,
+                           const uint32_t* d_trans, const uint32_t* d_challenges, uint32_t chunk_n,
+                           uint32_t blowup_step, uint32_t chunk_offset, vx_stream_t st) {
+  return guard([&] {
+    const uint32_t* inter[32] = {sl0,  sl1,  sl2,  sl3,  t1l0, t1l1, t1l2, t1l3, t2l0, t2l1, t2l2,
+                                 t2l3, t3l0, t3l1, t3l2, t3l3, sr0,  sr1,  sr2,  sr3,  u1r0, u1r1,
+                                 u1r2, u1r3, u2r0, u2r1, u2r2, u2r3, sd0,  sd1,  sd2,  sd3};
+    uint32_t* out[4] = {out0, out1, out2, out3};
+    CairoArgs a = cairo_args(d_trace_cols, inter, 32, out, d_alpha, d_vh_inv, d_trans, d_challenges, S(st));
+    a.g_base = chunk_offset;
+    a.o_base = chunk_offset;
+    a.blowup_step = blowup_step;
+    cairo_quotient(a, chunk_n, S(st));
+  });
+}
+
+int vx_cairo_quotient_chunk(const uint32_t* const* d_trace_cols, const uint32_t* sl0, const uint32_t* sl1,
+                            const uint32_t* sl2, const uint32_t* sl3, const uint32_t* sr0, const uint32_t* sr1,
+                            const uint32_t* sr2, const uint32_t* sr3, const uint32_t* sd0, const uint32_t* sd1,
+                            const uint32_t* sd2, const uint32_t* sd3, uint32_t* out0, uint32_t* out1,
+                            uint32_t* out2, uint32_t* out3, const uint32_t* d_alpha, const uint32_t* d_vh_inv,
+                            const uint32_t* d_trans, const uint32_t* d_challenges, uint32_t offset, uint32_t chunk_n,
+                            uint32_t global_n, uint32_t blowup_step, vx_stream_t st) {
+  return guard([&] {
+    require((uint64_t)offset + chunk_n <= global_n && blowup_step <= global_n,
+            "cairo_quotient_chunk: rows beyond global_n");
+    const uint32_t* inter[12] = {sl0, sl1, sl2, sl3, sr0, sr1, sr2, sr3, sd0, sd1, sd2, sd3};
+    uint32_t* out[4] = {out0, out1, out2, out3};
+    CairoArgs a = cairo_args(d_trace_cols, inter, 12, out, d_alpha, d_vh_inv, d_trans, d_challenges, S(st));
+    a.col_base = offset;
+    a.wrap = global_n;
+    a.g_base = offset;
+    a.blowup_step = blowup_step;
+    a.chunked = 1;
+    cairo_quotient(a, chunk_n, S(st));
+  });
+}
+
+int vx_cairo_quotient_commit(const uint32_t* const* d_trace_cols, const uint32_t* const* h_inter, uint32_t* out0,
+                             uint32_t* out1, uint32_t* out2, uint32_t* out3, const uint32_t* d_alpha,
+                             const uint32_t* d_vh_inv, const uint32_t* d_trans, const uint32_t* d_challenges,
+                             uint32_t log_n, uint32_t blowup_step, uint32_t store_lo, uint32_t* d_levels,
+                             uint32_t* d_root, vx_stream_t st) {
+  return guard([&] {
+    require(h_inter != nullptr && d_root != nullptr, "cairo_quotient_commit: null argument");
+    require(log_n >= 2 && log_n <= 31 && blowup_step <= (1ull << log_n), "cairo_quotient_commit: bad sizes");
+    const TreeLevels T = caller_levels(d_levels, log_n, store_lo);
+    uint32_t* out[4] = {out0, out1, out2, out3};
+    CairoArgs a = cairo_args(d_trace_cols, h_inter, 32, out, d_alpha, d_vh_inv, d_trans, d_challenges, S(st));
+    a.wrap = 1ull << log_n;
+    a.blowup_step = blowup_step;
+    DeviceBuf dargs(sizeof(CairoArgs));
+    VX_HIP(hipMemcpy(dargs.p, &a, sizeof(CairoArgs), hipMemcpyHostToDevice));
+    LeafSrc L{};
+    L.kind = LEAF_CAIRO_QUOTIENT;
+    L.cairo = static_cast<const CairoArgs*>(dargs.p);
+    for (int c = 0; c < 4; c++) L.out[c] = out[c];
+    RootHook hook{};
+    hook.root_out = d_root;
+    merkle_commit(L, T, store_lo, hook, S(st));
+    VX_HIP(hipStreamSynchronize(S(st)));  // the argument block outlives the kernels
+  });
+}
+
+int vx_compute_vanishing_inv(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* d_out, uint32_t log_eval,
+                             uint32_t log_n, vx_stream_t st) {
+  return guard([&] { cairo_vanishing(ix, iy, sx, sy, d_out, log_eval, log_n, true, S(st)); });
+}
+
+int vx_compute_vanishing(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* d_out, uint32_t log_eval,
+                         uint32_t log_n, vx_stream_t st) {
+  return guard([&] { cairo_vanishing(ix, iy, sx, sy, d_out, log_eval, log_n, false, S(st)); });
+}
+
+int vx_axpy_m31(uint32_t scalar, const uint32_t* d_x, uint32_t* d_y, uint32_t n, vx_stream_t st) {
+  return guard([&] { axpy_m31(scalar, d_x, d_y, n, S(st)); });
 }
 
 // ------------------------------------------------------------------ serving

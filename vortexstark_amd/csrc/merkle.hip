@@ -26,7 +26,7 @@
 #include "vx_blake2s_x4.h"
 #include "vx_channel_x4.h"
 #include "vx_d2.h"
-#include "vx_internal.h"
+#include "vx_cairo.h"
 
 namespace vx {
 
@@ -311,6 +311,31 @@ __device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i,
   }
 }
 
+// Cairo VM AIR quotient of rows [i, i+4) (vx_cairo.h), stored (SoA) for the
+// FRI and hashed as QM31 leaves.
+// The row loop is not unrolled (each row is ~2,000 instructions); row j's
+// values go to g.w[j] through uniform selects, so g stays in registers.
+__device__ __forceinline__ void cairo_produce(const LeafSrc& L, uint64_t i, G4& g) {
+  uniform_t<CairoArgs>& a = as_uniform(L.cairo);
+#pragma unroll 1
+  for (int j = 0; j < 4; j++) {
+    const qm31 q = cairo_row(a, i + j);
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) g.w[jj][c] = jj == j ? q.v[c] : g.w[jj][c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
+}
+// One leaf per lane: the lane's own row, stored and hashed.
+__device__ __forceinline__ void cairo_leaf(const LeafSrc& L, uint64_t idx, uint32_t out[8]) {
+  const qm31 q = cairo_row(as_uniform(L.cairo), idx);
+#pragma unroll
+  for (int c = 0; c < 4; c++) L.out[c][idx] = q.v[c];
+  b2_leaf4(q.v[0], q.v[1], q.v[2], q.v[3], out);
+}
+
 // Rows [i, i+4) of up to 16 columns, one 16-byte load per column, all
 // issued before the first of the four leaves is hashed.
 struct GCols {
@@ -350,7 +375,7 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
   static_assert(LPT_LOG >= 2 && LPT_LOG <= 4, "lane subtree covers 4..16 leaves");
   constexpr bool kGeneric = KIND == LEAF_COLS && NC == 0;
   constexpr bool kDirect = kGeneric || KIND == LEAF_D2_QUOTIENT || KIND == LEAF_LIFTED ||
-                           KIND == LEAF_LIFTED_RLC;  // no raw-load prefetch
+                           KIND == LEAF_LIFTED_RLC || KIND == LEAF_CAIRO_QUOTIENT;  // no raw-load prefetch
   constexpr uint32_t G = 1u << (LPT_LOG - 2);
   qm31 aq{}, af{};
   qm31_mat am{};
@@ -375,6 +400,7 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
     if constexpr (kGeneric) cols_load(L, i, gc);
     if constexpr (KIND == LEAF_D2_QUOTIENT) d2_produce(L, i, g);
     if constexpr (KIND == LEAF_LIFTED_RLC) lifted_rlc_produce(L, i, g);
+    if constexpr (KIND == LEAF_CAIRO_QUOTIENT) cairo_produce(L, i, g);
     uint32_t a[8], b[8], n1a[8], n1b[8], n2[8];
     leaf_hash<KIND, NC>(L, g, gc, i, 0, a);
     leaf_hash<KIND, NC>(L, g, gc, i, 1, b);
@@ -414,6 +440,8 @@ __device__ __forceinline__ void lane_leaf(const LeafSrc& L, uint64_t idx, uint32
   constexpr bool kGeneric = KIND == LEAF_COLS && NC == 0;
   if constexpr (KIND == LEAF_LIFTED) {
     lifted_leaf(L, idx, out);
+  } else if constexpr (KIND == LEAF_CAIRO_QUOTIENT) {
+    cairo_leaf(L, idx, out);
   } else {
     const uint64_t i = idx & ~3ull;
     const uint32_t j = (uint32_t)idx & 3u;
@@ -671,6 +699,12 @@ static void check_leaf(const LeafSrc& L, uint32_t log_n) {
       require(L.lift && L.lift_n >= 1 && L.lift_log == log_n && L.beta, "merkle: bad lifted combination source");
       for (int c = 0; c < 4; c++) require(L.out[c] != nullptr, "merkle: null quotient output");
       break;
+    case LEAF_CAIRO_QUOTIENT:
+      require(L.cairo != nullptr, "merkle: bad Cairo quotient source");
+      for (int c = 0; c < 4; c++)
+        require(L.out[c] != nullptr && (reinterpret_cast<uintptr_t>(L.out[c]) & 15u) == 0,
+                "merkle: Cairo quotient outputs must be non-null and 16-byte aligned");
+      break;
     case LEAF_D2_QUOTIENT:
       require(L.col[0] && L.cons && L.beta && L.n_cols >= 1 && L.n_cons >= 1 &&
                   L.src_mask == (1ull << log_n) - 1 && L.col_stride >= (1ull << log_n),
@@ -696,6 +730,7 @@ static void dispatch_subtree(const LeafSrc& L, const TreeLevels& T, uint32_t til
     case LEAF_D2_QUOTIENT: launch_subtree<LEAF_D2_QUOTIENT, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
     case LEAF_LIFTED: launch_subtree<LEAF_LIFTED, 0>(L, T, tile_log, lpt, store_lo, hook, s); break;
     case LEAF_LIFTED_RLC: launch_subtree<LEAF_LIFTED_RLC, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
+    case LEAF_CAIRO_QUOTIENT: launch_subtree<LEAF_CAIRO_QUOTIENT, 4>(L, T, tile_log, lpt, store_lo, hook, s); break;
   }
 }
 
@@ -710,6 +745,7 @@ TreeShape merkle_default_shape(const LeafSrc& L, uint32_t log_n) {
   // each line is used only after the lane's next hash groups, by which time
   // L2 has evicted it (measured: line folds fetched each byte about 4 times).
   const bool multi = L.kind == LEAF_D2_QUOTIENT || L.kind == LEAF_LIFTED_RLC || L.kind == LEAF_LIFTED ||
+                     L.kind == LEAF_CAIRO_QUOTIENT ||
                      (L.kind == LEAF_COLS && L.n_cols > 1) || L.kind == LEAF_FOLD_LINE ||
                      L.kind == LEAF_FOLD_CIRCLE_Q;
   // Small trees are latency-bound: one leaf per lane and 256-leaf tiles

@@ -217,6 +217,8 @@ struct TreeLevels {
   uint32_t lo;
   uint64_t off[MAX_LEVELS];
 };
+// Arguments of the Cairo VM AIR quotient (vx_cairo.h).
+struct CairoArgs;
 // Leaf producers fused into the subtree kernel.
 enum LeafKind : uint32_t {
   LEAF_COLS = 0,          // n_cols (1..16) M31 columns, leaf = Blake2s(row words)
@@ -226,6 +228,7 @@ enum LeafKind : uint32_t {
   LEAF_D2_QUOTIENT = 4,   // synthetic AIR quotient of W eval columns (vx_d2.h), writes Q (SoA)
   LEAF_LIFTED = 5,        // lifted multi-size leaf (cuda/merkle_leaves_lifted.cu:1-98)
   LEAF_LIFTED_RLC = 6,    // random linear combination of the lifted rows, writes Q (SoA)
+  LEAF_CAIRO_QUOTIENT = 7,  // Cairo VM AIR quotient of row i (cuda/cairo_constraint.cu), writes Q (SoA)
 };
 // A column of a lifted tree: 2^log_n values, log_n <= the tree's log size.
 struct LiftCol {
@@ -255,6 +258,9 @@ struct LeafSrc {
   const LiftCol* lift;
   uint32_t lift_n;
   uint32_t lift_log;
+  // LEAF_CAIRO_QUOTIENT: the quotient's arguments in device memory (full
+  // form: rows 0..2^log_n, next row wrapping); Q goes to out[0..3]
+  const CairoArgs* cairo;
 };
 // What the kernel that produces a root does with it: copy it to root_out,
 // and, with a channel, mix it (after ch_init when `init`) and draw the next
@@ -327,6 +333,11 @@ void merkle_root_small(const LeafSrc& leaf, uint32_t log_n, uint32_t* root_out, 
 void merkle_leaf_pairs(const LeafSrc& leaf, uint64_t n_leaves, uint32_t* parents, hipStream_t s);
 // Every leaf hash of a lifted tree (cuda_build_leaves_lifted semantics).
 void merkle_lifted_leaves(const LeafSrc& leaf, uint32_t* hashes, hipStream_t s);
+// Cairo VM AIR quotient (cairo.hip).
+void cairo_quotient(const CairoArgs& a, uint64_t rows, hipStream_t s);
+void cairo_vanishing(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* out, uint32_t log_eval,
+                     uint32_t log_n, bool inverse, hipStream_t s);
+void axpy_m31(uint32_t sc, const uint32_t* x, uint32_t* y, uint64_t n, hipStream_t s);
 // Root of a lifted tree with the column table and the stored tree levels in
 // a file-scope device scratch array (noalloc.hip): no device allocation.
 // Asynchronous; calls that overlap on one GPU share the scratch and race.
