@@ -460,6 +460,95 @@ def batched(vx, steps, n_cols=96, seed=42, blowup_bits=2):
     return out
 
 
+def cairo_commit(vx, steps, log_eval=24, seed=0xCA1F0, step=4):
+    """The Cairo-shaped commitment at trace log_n = log_eval - 2 (blowup 4):
+    34 trace columns as lo / hi / dict trees (16, 15, 3 columns) and the
+    Cairo VM AIR quotient (40 weighted constraints over the 34 trace and 32
+    interaction columns, divided by Z_H) committed as a QM31 tree; inputs from
+    the synthetic generator, resident in HBM. The quotient is timed alone
+    (vx_cairo_quotient, then the tree of its 4 columns) and fused with its
+    tree (vx_cairo_quotient_commit, one subtree launch)."""
+    import ctypes as C
+
+    call = vx._lib.call
+    n = 1 << log_eval
+    bufs = []
+
+    def alloc(words):
+        p = C.c_void_p()
+        call("vx_malloc", C.byref(p), 4 * words)
+        bufs.append(p)
+        return p.value
+
+    def levels_words(L, lo):
+        return sum(8 << (L - l) for l in range(lo, L + 1))
+
+    try:
+        cols = [alloc(n) for _ in range(70)]
+        for c, ptr in enumerate(cols):
+            call("vx_m31_fill_random", ptr, log_eval, seed, c, None)
+        alpha, chal = alloc(256), alloc(32)
+        call("vx_m31_fill_random", alpha, 8, seed, 68, None)
+        call("vx_m31_fill_random", chal, 5, seed, 69, None)
+        outs = [alloc(n) for _ in range(4)]
+        lv, root = alloc(levels_words(log_eval, 10)), alloc(8)
+        dcols = alloc(2 * 34)
+        arr = (C.c_uint64 * 34)(*cols[:34])
+        call("vx_memcpy", dcols, C.cast(arr, C.c_void_p), 8 * 34, 1)
+        hint = (C.c_void_p * 32)(*cols[34:66])
+        trees = [(0, 16), (16, 31), (31, 34)]
+
+        def trace_trees():
+            for lo, hi in trees:
+                hc = (C.c_void_p * (hi - lo))(*cols[lo:hi])
+                call("vx_merkle_commit_tiled", hc, hi - lo, log_eval, 10, lv, root, None)
+
+        def quotient_alone():
+            call("vx_cairo_quotient", dcols, *cols[34:66], *outs, alpha, cols[66], cols[67], chal, n, step, None)
+
+        def quotient_tree():
+            hc = (C.c_void_p * 4)(*outs)
+            call("vx_merkle_commit_tiled", hc, 4, log_eval, 10, lv, root, None)
+
+        def fused():
+            call("vx_cairo_quotient_commit", dcols, hint, *outs, alpha, cols[66], cols[67], chal, log_eval, step, 10, lv,
+                 root, None)
+
+        def timed(fn):
+            fn()
+            call("vx_device_sync")
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            call("vx_device_sync")
+            return (time.perf_counter() - t0) * 1000.0 / steps
+
+        t_trees = timed(trace_trees)
+        t_q = timed(quotient_alone)
+        t_qt = timed(quotient_tree)
+        t_fused = timed(fused)
+    finally:
+        for p in bufs:
+            vx.load().vx_free(p)
+    # unique bytes per row: 34 trace + 32 interaction + Z_H^-1 + transition
+    # factor read, 4 quotient words written (the next-row reads hit cache)
+    q_bytes = n * 4 * (34 + 32 + 2 + 4)
+    comps = 2 * n - 1
+    return {
+        "workload": f"Cairo-shaped commitment, trace log_n {log_eval - 2}, blowup 4 ({n} rows), synthetic columns",
+        "ms_trace_trees": round(t_trees, 3),
+        "ms_quotient": round(t_q, 3),
+        "ms_quotient_tree": round(t_qt, 3),
+        "ms_quotient_fused_commit": round(t_fused, 3),
+        "ms_separate_quotient_and_tree": round(t_q + t_qt, 3),
+        "quotient_hbm_gbs": round(q_bytes / (t_q * 1e-3) / 1e9, 1),
+        "quotient_hbm_frac": round(q_bytes / (t_q * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fused_tree_valu_frac": round(comps * OPS_PER_COMPRESSION / (t_fused * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
+        "steps": steps,
+        "synchronous": True,
+    }
+
+
 def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     """stwo-basis circle NTT (the stwo plugin's PolyOps): evaluate and
     interpolate of one 2^log_n column resident in HBM, asynchronous calls
@@ -731,6 +820,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-stwo", action="store_true")
+    ap.add_argument("--no-cairo", action="store_true")
+    ap.add_argument("--cairo-only", action="store_true")
     ap.add_argument("--stwo-only", action="store_true")
     ap.add_argument("--profile-serving-mix", action="store_true",
                     help="eager proofs of every serving class (7 per Fibonacci class, 9 per synthetic class) "
@@ -757,6 +848,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batched_commit": batched(vx, args.steps)}), flush=True)
+        return
+    if args.cairo_only:
+        print(json.dumps({"cairo_commit": cairo_commit(vx, args.steps)}), flush=True)
         return
     if args.stwo_only:
         print(json.dumps({"stwo_ntt": stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)}), flush=True)
@@ -857,6 +951,8 @@ def main():
         out["serving"] = serving(vx, args.serve_requests, args.slots)
     if not args.no_batch and not args.profile:
         out["batched_commit"] = batched(vx, args.steps)
+    if not args.no_cairo and not args.profile:
+        out["cairo_commit"] = cairo_commit(vx, args.steps)
     if not args.no_stwo and not args.profile:
         out["stwo_ntt"] = stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline and gpu_a is not None:
