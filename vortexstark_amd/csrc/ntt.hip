@@ -206,6 +206,16 @@ __device__ __forceinline__ void rx_apply(uint32_t (&v)[32], const uint32_t (&w)[
   }
 }
 
+// Word at a 32-bit byte offset from a wave-uniform base: the compiler
+// addresses it as scalar base + one VGPR offset, so an element costs one
+// 32-bit add instead of 64-bit index arithmetic.
+__device__ __forceinline__ uint32_t ld_at(const uint32_t* base, uint32_t byte_off) {
+  return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ void st_at(uint32_t* base, uint32_t byte_off, uint32_t v) {
+  *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 // Pass arguments with the round layout fixed at compile time (defined
 // below); their tiles load a round's twiddles before its butterflies.
 template <bool FWD, int SHAPE>
@@ -216,6 +226,16 @@ struct rx_is_fixed {
 };
 template <bool F, int S>
 struct rx_is_fixed<FixedArgs<F, S>> {
+  static constexpr bool value = true;
+};
+// The fixed strided shape (10 row bits x 16 columns) addresses its global
+// elements from a uniform tile base (ld_at / st_at).
+template <typename A>
+struct rx_is_strided {
+  static constexpr bool value = false;
+};
+template <bool F>
+struct rx_is_strided<FixedArgs<F, 2>> {
   static constexpr bool value = true;
 };
 
@@ -230,6 +250,7 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
   const uint64_t low_base = (tile & ((1ull << low_tiles_log) - 1)) << a.cbits;
   const uint64_t H = tile >> low_tiles_log;
   const uint64_t hi_base = H << (a.j0 + a.k);
+  constexpr bool kStrided = rx_is_strided<Args>::value;
   // One round of the pass; the fixed shapes unroll their constant round
   // count, the generic kernel loops over its runtime one.
   auto round = [&](uint32_t ri) __attribute__((always_inline)) {
@@ -257,6 +278,16 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
       __syncthreads();
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) v[s] = lds[lb ^ rx_swz(s << pos)];
+    } else if (ri == 0 && kStrided) {
+      // element s sits (s << (pos - cbits + j0)) words after the lane's
+      // first one; the tile spans < 2^32 bytes (checked when the shape is
+      // chosen), and masking the source index only clears bits of hi_base
+      const uint32_t* sp = src + ((hi_base & a.src_mask) | low_base);
+      const uint32_t e0 = rx_own(t, 0, pos);
+      const uint32_t lane = (((e0 >> a.cbits) << a.j0) | (e0 & cmask)) << 2;
+      const uint32_t sh = pos - a.cbits + a.j0 + 2;
+#pragma unroll
+      for (uint32_t s = 0; s < 32; s++) v[s] = ld_at(sp, lane + (s << sh));
     } else if (ri == 0) {
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) {
@@ -308,9 +339,13 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
     } else if (staged) {
       // each lane rewrites exactly the words it read this round, so no
       // barrier is needed before these stores
+      if (!FWD && a.scale != 1u) {
 #pragma unroll
-      for (uint32_t s = 0; s < 32; s++)
-        lds[lb ^ rx_swz(s << pos)] = a.scale != 1u ? m31_mul(v[s], a.scale) : v[s];
+        for (uint32_t s = 0; s < 32; s++) lds[lb ^ rx_swz(s << pos)] = m31_mul(v[s], a.scale);
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 32; s++) lds[lb ^ rx_swz(s << pos)] = v[s];
+      }
       __syncthreads();
 #pragma unroll
       for (uint32_t g = 0; g < 8; g++) {
@@ -318,12 +353,24 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
         *reinterpret_cast<uint4*>(dst + (hi_base | e)) =
             make_uint4(lds[rx_swz(e)], lds[rx_swz(e + 1)], lds[rx_swz(e + 2)], lds[rx_swz(e + 3)]);
       }
+    } else if (kStrided) {
+      uint32_t* dp = dst + (hi_base | low_base);
+      const uint32_t e0 = rx_own(t, 0, pos);
+      const uint32_t lane = (((e0 >> a.cbits) << a.j0) | (e0 & cmask)) << 2;
+      const uint32_t sh = pos - a.cbits + a.j0 + 2;
+      if (!FWD && a.scale != 1u) {
+#pragma unroll
+        for (uint32_t s = 0; s < 32; s++) st_at(dp, lane + (s << sh), m31_mul(v[s], a.scale));
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 32; s++) st_at(dp, lane + (s << sh), v[s]);
+      }
     } else {
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) {
         const uint32_t e = rx_own(t, s, pos);
         const uint64_t gi = hi_base | ((uint64_t)(e >> a.cbits) << a.j0) | (low_base + (e & cmask));
-        dst[gi] = a.scale != 1u ? m31_mul(v[s], a.scale) : v[s];
+        dst[gi] = (!FWD && a.scale != 1u) ? m31_mul(v[s], a.scale) : v[s];
       }
     }
   };
@@ -391,7 +438,8 @@ struct FixedArgs<FWD, RX_STRIDED> {
 
 static int radix_shape(const RadixArgs& r) {
   if (r.k == 14 && r.cbits == 0 && r.j0 == 0) return RX_CONTIG;
-  if (r.k == 10 && r.cbits == 4) return RX_STRIDED;
+  // the strided kernel's 32-bit element offsets need a tile span below 2^32 bytes
+  if (r.k == 10 && r.cbits == 4 && r.j0 + r.k <= 30) return RX_STRIDED;
   return RX_GENERIC;
 }
 
