@@ -188,28 +188,43 @@ __device__ __forceinline__ qm31 cairo_row(const A& args, uint64_t r) {
     rhs = m31_add(rhs, f[14]);
     ca_add_m31(acc, args, k++, m31_sub(m31_add(ca_col(args, CA_INST_LO, i), ca_col(args, CA_INST_HI, i)), rhs));
   }
-  // memory (LogUp) denominators: z - (addr + alpha value), the instruction
-  // fetch with inst_lo, inst_hi
+  // memory (LogUp) denominators d = z - (addr + alpha value) (the
+  // instruction fetch adds alpha^2 inst_hi): each component is z_k plus
+  // products with the negated uniform alphas, summed in 64 bits and
+  // reduced once
   const qm31 z_mem = ca_chal(args, 0), a_mem = ca_chal(args, 1), z_rc = ca_chal(args, 3);
-  const qm31 d0 = qm_sub(z_mem, qm_add(ca_lift(pc), qm_add(qm_scale(a_mem, ca_col(args, CA_INST_LO, i)),
-                                                          qm_scale(ca_chal(args, 2), ca_col(args, CA_INST_HI, i)))));
-  const qm31 d1 = qm_sub(z_mem, qm_add(ca_lift(ca_col(args, CA_DST_ADDR, i)), qm_scale(a_mem, dst)));
-  const qm31 d2 = qm_sub(z_mem, qm_add(ca_lift(ca_col(args, CA_OP0_ADDR, i)), qm_scale(a_mem, op0)));
-  const qm31 d3 = qm_sub(z_mem, qm_add(ca_lift(ca_col(args, CA_OP1_ADDR, i)), qm_scale(a_mem, op1)));
+  const qm31 a_mem2 = ca_chal(args, 2);
+  const uint32_t inst_lo = ca_col(args, CA_INST_LO, i), inst_hi = ca_col(args, CA_INST_HI, i);
+  auto denom = [&](uint32_t addr, uint32_t val) {
+    qm31 d;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      d.v[c] = m31_reduce64((uint64_t)z_mem.v[c] + (c == 0 ? (uint64_t)(P - addr) : 0) +
+                            (uint64_t)(P - a_mem.v[c]) * val);
+    return d;
+  };
+  qm31 d0;
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    d0.v[c] = m31_reduce64((uint64_t)z_mem.v[c] + (c == 0 ? (uint64_t)(P - pc) : 0) +
+                           (uint64_t)(P - a_mem.v[c]) * inst_lo + (uint64_t)(P - a_mem2.v[c]) * inst_hi);
+  const qm31 d1 = denom(ca_col(args, CA_DST_ADDR, i), dst);
+  const qm31 d2 = denom(ca_col(args, CA_OP0_ADDR, i), op0);
+  const qm31 d3 = denom(ca_col(args, CA_OP1_ADDR, i), op1);
   const qm31 r0 = qm_sub(z_rc, ca_lift(off0)), r1 = qm_sub(z_rc, ca_lift(off1)), r2 = qm_sub(z_rc, ca_lift(off2));
   const qm31 one{{1, 0, 0, 0}};
   if (!args.chunked) {
     // 31a..31d, 32a..32c in polynomial form over T1..T3, U1, U2
     const qm31 s = ca_inter(args, 0, i), t1 = ca_inter(args, 1, i), t2 = ca_inter(args, 2, i),
                t3 = ca_inter(args, 3, i);
-    ca_add_qm31(acc, args, k++, qm_sub(qm_mul(qm_sub(t1, s), d0), one));
-    ca_add_qm31(acc, args, k++, qm_sub(qm_mul(qm_sub(t2, t1), d1), one));
-    ca_add_qm31(acc, args, k++, qm_sub(qm_mul(qm_sub(t3, t2), d2), one));
-    ca_add_qm31(acc, args, k++, qm_scale(qm_sub(qm_mul(qm_sub(ca_inter(args, 0, ni), t3), d3), one), tf));
+    ca_add_qm31(acc, args, k++, qm_sub(qm_mul_lazy(qm_sub(t1, s), d0), one));
+    ca_add_qm31(acc, args, k++, qm_sub(qm_mul_lazy(qm_sub(t2, t1), d1), one));
+    ca_add_qm31(acc, args, k++, qm_sub(qm_mul_lazy(qm_sub(t3, t2), d2), one));
+    ca_add_qm31(acc, args, k++, qm_scale(qm_sub(qm_mul_lazy(qm_sub(ca_inter(args, 0, ni), t3), d3), one), tf));
     const qm31 rs = ca_inter(args, 4, i), u1 = ca_inter(args, 5, i), u2 = ca_inter(args, 6, i);
-    ca_add_qm31(acc, args, k++, qm_sub(qm_mul(qm_sub(u1, rs), r0), one));
-    ca_add_qm31(acc, args, k++, qm_sub(qm_mul(qm_sub(u2, u1), r1), one));
-    ca_add_qm31(acc, args, k++, qm_scale(qm_sub(qm_mul(qm_sub(ca_inter(args, 4, ni), u2), r2), one), tf));
+    ca_add_qm31(acc, args, k++, qm_sub(qm_mul_lazy(qm_sub(u1, rs), r0), one));
+    ca_add_qm31(acc, args, k++, qm_sub(qm_mul_lazy(qm_sub(u2, u1), r1), one));
+    ca_add_qm31(acc, args, k++, qm_scale(qm_sub(qm_mul_lazy(qm_sub(ca_inter(args, 4, ni), u2), r2), one), tf));
   } else {
     // 31, 32 as one step each: S_next - S - sum of inverse denominators
     const qm31 dl = qm_add(qm_add(qm_add(qm_inv(d0), qm_inv(d1)), qm_inv(d2)), qm_inv(d3));
@@ -221,10 +236,15 @@ __device__ __forceinline__ qm31 cairo_row(const A& args, uint64_t r) {
   const uint32_t act = ca_col(args, CA_DICT_ACTIVE, i);
   ca_add_m31(acc, args, k++, m31_mul(act, m31_sub(1, act)));
   const int sd = args.chunked ? 2 : 7;
-  const qm31 entry = qm_add(ca_lift(ca_col(args, CA_DICT_KEY, i)), qm_scale(ca_chal(args, 5), ca_col(args, CA_DICT_NEW, i)));
-  const qm31 c34 = qm_scale(
-      qm_sub(qm_mul(qm_sub(ca_inter(args, sd, ni), ca_inter(args, sd, i)), qm_sub(ca_chal(args, 4), entry)), ca_lift(act)),
-      tf);
+  // z_dict - (key + alpha_dict new), as the memory denominators
+  const qm31 z_d = ca_chal(args, 4), a_d = ca_chal(args, 5);
+  const uint32_t key = ca_col(args, CA_DICT_KEY, i), nv = ca_col(args, CA_DICT_NEW, i);
+  qm31 dd;
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    dd.v[c] = m31_reduce64((uint64_t)z_d.v[c] + (c == 0 ? (uint64_t)(P - key) : 0) + (uint64_t)(P - a_d.v[c]) * nv);
+  const qm31 c34 =
+      qm_scale(qm_sub(qm_mul_lazy(qm_sub(ca_inter(args, sd, ni), ca_inter(args, sd, i)), dd), ca_lift(act)), tf);
   ca_add_qm31(acc, args, k++, c34);
   const uint32_t vh = as_global(args.vh_inv)[g];
   return qm31{{m31_mul(m31_reduce64(acc.a[0]), vh), m31_mul(m31_reduce64(acc.a[1]), vh),

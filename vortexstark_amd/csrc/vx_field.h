@@ -118,6 +118,20 @@ VX_HD qm31_mat qm_left_matrix(qm31 x) {
   return {{{x0, m31_neg(x1), a, m31_neg(b)}, {x1, x0, b, a}, {x2, m31_neg(x3), x0, m31_neg(x1)}, {x3, x2, x1, x0}}};
 }
 
+// x * y for two row-varying QM31 values through the same matrix: the
+// entries need two linear combinations of x and four negations (P - v,
+// congruent to -v), each component is four products summed in 64 bits
+// (below 4 P^2 < 2^64) and reduced once: about 50 instructions instead of
+// the 16 canonical M31 products of qm_mul.
+VX_HD qm31 qm_mul_lazy(qm31 x, qm31 y) {
+  const uint32_t x0 = x.v[0], x1 = x.v[1], x2 = x.v[2], x3 = x.v[3];
+  const uint32_t a = m31_sub(m31_add(x2, x2), x3), b = m31_add(x2, m31_add(x3, x3));
+  const uint32_t nx1 = P - x1, nx3 = P - x3, nb = P - b;
+  const uint64_t y0 = y.v[0], y1 = y.v[1], y2 = y.v[2], y3 = y.v[3];
+  return {{m31_reduce64(x0 * y0 + nx1 * y1 + a * y2 + nb * y3), m31_reduce64(x1 * y0 + x0 * y1 + b * y2 + a * y3),
+           m31_reduce64(x2 * y0 + nx3 * y1 + x0 * y2 + nx1 * y3), m31_reduce64(x3 * y0 + x2 * y1 + x1 * y2 + x0 * y3)}};
+}
+
 // The same fold with alpha given as its matrix: each component is a dot
 // product of four M31 products accumulated in 64 bits (4 P^2 < 2^64) and
 // reduced once, then tw * s + (f0 + f1), again reduced once.
