@@ -511,6 +511,40 @@ int vx_prove(uint32_t a, uint32_t b, uint32_t log_n, uint32_t blowup_bits, uint3
 int vx_verify(const uint8_t *proof, size_t len, uint32_t blowup_bits, uint32_t n_queries, char *err,
               size_t err_cap);
 
+/* ------------------------------------------------------------ stwo export
+ * Host only. cairo_proof_to_json counterpart (src/stwo_export.rs:401-676)
+ * for the KRAK proofs of this library (v1..v4): the stwo-shaped JSON
+ * (config, tree_meta, commitments, sampled_values, decommitments with
+ * deduplicated hash_witness, queried_values[tree][col][query], query_indices,
+ * proof_of_work, fri_proof with fri_witness and the bit-reversed LinePoly
+ * coefficients of the last layer), plus a "krak_openings" object with the
+ * leaves each hash witness proves. *out_len = the JSON length; out may be
+ * NULL to query it, else cap must exceed it (a NUL is appended). blowup_bits
+ * is read from the proof except for v1, which does not carry it. */
+int vx_export_stwo_json(const uint8_t *proof, size_t len, uint32_t blowup_bits, char *out, size_t cap,
+                        size_t *out_len);
+/* auth_paths_to_hash_witness (src/stwo_export.rs:262-320): positions[n],
+ * paths = n auth paths of `height` 8-word hashes (leaf level first) back to
+ * back. *out_n = witness hashes; out (8 words each) may be NULL. */
+int vx_stwo_hash_witness(const uint64_t *positions, size_t n, const uint32_t *paths, uint32_t height,
+                         uint32_t *out, size_t cap, size_t *out_n);
+/* dedup_fri_witness (src/stwo_export.rs:174-206): sibling_values[n] QM31
+ * (4 words each) at positions[i] ^ 1. *out_n = witness values; out may be NULL. */
+int vx_stwo_fri_witness(const uint64_t *positions, size_t n, const uint32_t *sibling_values, uint32_t *out,
+                        size_t cap, size_t *out_n);
+/* reconstruct_root_from_witness (src/stwo_export.rs:697-780): leaves are
+ * Blake2s of the n_cols values (little-endian words) at each position,
+ * col_values column-major (n_cols x n). VX_ERR_VERIFY with the reason in err
+ * when the witness is short, long or leaves more than one root. */
+int vx_stwo_reconstruct_root(uint32_t log_height, const uint64_t *positions, size_t n, const uint32_t *col_values,
+                             uint32_t n_cols, const uint32_t *witness, size_t n_witness, uint32_t root[8], char *err,
+                             size_t err_cap);
+/* last_layer_poly_coeffs (src/fri.rs:294-335) followed by the bit-reversal
+ * of src/cairo_air/prover.rs:2896-2900: n QM31 last-layer evaluations
+ * (bit-reversed order on the half_odds line domain) -> n bit-reversed
+ * LinePoly coefficients. */
+int vx_stwo_last_layer_coeffs(const uint32_t *evals, uint32_t n, uint32_t *out);
+
 /* ------------------------------------------------- batched uneven columns
  * Many M31 columns of different log sizes through one launch per phase,
  * driven by device work lists. The reference batches only equal-size

@@ -74,6 +74,11 @@ SIGNATURES = {
     "vx_prover_buffer": (ip, [vp, ip, C.POINTER(vp), C.POINTER(u64)]),
     "vx_prove": (ip, [u32, u32, u32, u32, u32, u32, vp, sz, C.POINTER(sz)]),
     "vx_verify": (ip, [C.c_char_p, sz, u32, u32, C.c_char_p, sz]),
+    "vx_export_stwo_json": (ip, [C.c_char_p, sz, u32, vp, sz, C.POINTER(sz)]),
+    "vx_stwo_hash_witness": (ip, [vp, sz, vp, u32, vp, sz, C.POINTER(sz)]),
+    "vx_stwo_fri_witness": (ip, [vp, sz, vp, vp, sz, C.POINTER(sz)]),
+    "vx_stwo_reconstruct_root": (ip, [u32, vp, sz, vp, u32, vp, sz, vp, C.c_char_p, sz]),
+    "vx_stwo_last_layer_coeffs": (ip, [vp, u32, vp]),
     "vx_prover_set_graphs": (ip, [vp, ip]),
     "vx_tuner_configure": (ip, [ip, C.c_char_p]),
     "vx_merkle_levels_words": (u64, [u32, u32]),

@@ -8,6 +8,7 @@
 
 #include "prover.h"
 #include "batch.h"
+#include "export.h"
 #include "tune.h"
 #include "server.h"
 #include "vx_cairo.h"
@@ -972,6 +973,81 @@ int vx_verify(const uint8_t* proof, size_t len, uint32_t blowup_bits, uint32_t n
     err[err_cap - 1] = 0;
   }
   return rc;
+}
+
+// ------------------------------------------------------- stwo export
+int vx_export_stwo_json(const uint8_t* proof, size_t len, uint32_t blowup_bits, char* out, size_t cap,
+                        size_t* out_len) {
+  return guard([&] {
+    const std::string j = export_stwo_json(proof, len, blowup_bits);
+    if (out_len) *out_len = j.size();
+    if (out) {
+      require(cap > j.size(), "stwo export: output buffer too small (query the size with out = NULL)");
+      std::memcpy(out, j.data(), j.size());
+      out[j.size()] = 0;
+    }
+  });
+}
+
+static std::vector<uint64_t> positions_of(const uint64_t* p, size_t n) {
+  require(n == 0 || p != nullptr, "null positions");
+  return std::vector<uint64_t>(p, p + n);
+}
+
+int vx_stwo_hash_witness(const uint64_t* positions, size_t n, const uint32_t* paths, uint32_t height,
+                         uint32_t* out, size_t cap, size_t* out_n) {
+  return guard([&] {
+    require(height <= 64 && (n == 0 || height == 0 || paths != nullptr), "hash witness: bad paths");
+    std::vector<const uint32_t*> ps(n);
+    std::vector<size_t> words(n, 8ull * height);
+    for (size_t i = 0; i < n; i++) ps[i] = paths + 8ull * height * i;
+    const auto w = hash_witness(positions_of(positions, n), ps, words, height);
+    if (out_n) *out_n = w.size();
+    if (out) {
+      require(cap >= w.size(), "hash witness: output smaller than the witness");
+      for (size_t i = 0; i < w.size(); i++) std::memcpy(out + 8 * i, w[i].data(), 32);
+    }
+  });
+}
+
+int vx_stwo_fri_witness(const uint64_t* positions, size_t n, const uint32_t* sibling_values, uint32_t* out,
+                        size_t cap, size_t* out_n) {
+  return guard([&] {
+    require(n == 0 || sibling_values != nullptr, "fri witness: null sibling values");
+    const auto w = fri_witness(positions_of(positions, n), sibling_values);
+    if (out_n) *out_n = w.size();
+    if (out) {
+      require(cap >= w.size(), "fri witness: output smaller than the witness");
+      for (size_t i = 0; i < w.size(); i++) std::memcpy(out + 4 * i, w[i].data(), 16);
+    }
+  });
+}
+
+int vx_stwo_reconstruct_root(uint32_t log_height, const uint64_t* positions, size_t n, const uint32_t* col_values,
+                             uint32_t n_cols, const uint32_t* witness, size_t n_witness, uint32_t root[8], char* err,
+                             size_t err_cap) {
+  int rc = guard([&] {
+    require(log_height <= 64 && n_cols >= 1 && n_cols <= (1u << 16), "reconstruct root: bad shape");
+    require((n == 0 || col_values != nullptr) && (n_witness == 0 || witness != nullptr) && root != nullptr,
+            "reconstruct root: null argument");
+    const std::string why =
+        reconstruct_root(log_height, positions_of(positions, n), col_values, n_cols, witness, n_witness, root);
+    if (!why.empty()) throw Error(ERR_VERIFY, why);
+  });
+  if (err && err_cap) {
+    std::string m = rc == VX_OK ? "" : g_last_error;
+    std::strncpy(err, m.c_str(), err_cap - 1);
+    err[err_cap - 1] = 0;
+  }
+  return rc;
+}
+
+int vx_stwo_last_layer_coeffs(const uint32_t* evals, uint32_t n, uint32_t* out) {
+  return guard([&] {
+    require(evals != nullptr && out != nullptr, "last layer: null argument");
+    const auto c = last_layer_coeffs(evals, n);
+    for (uint32_t i = 0; i < n; i++) std::memcpy(out + 4 * i, c[i].data(), 16);
+  });
 }
 
 int vx_prover_set_graphs(vx_prover* p, int on) {
