@@ -228,16 +228,24 @@ template <bool F, int S>
 struct rx_is_fixed<FixedArgs<F, S>> {
   static constexpr bool value = true;
 };
-// The fixed strided shape (10 row bits x 16 columns) addresses its global
-// elements from a uniform tile base (ld_at / st_at).
+// Fixed strided shapes (k row bits x 2^(14-k) columns, shape id 16 + k)
+// address their global elements from a uniform tile base (ld_at / st_at).
+static constexpr int RX_STRIDED_K0 = 16;
 template <typename A>
 struct rx_is_strided {
   static constexpr bool value = false;
 };
-template <bool F>
-struct rx_is_strided<FixedArgs<F, 2>> {
-  static constexpr bool value = true;
+template <bool F, int S>
+struct rx_is_strided<FixedArgs<F, S>> {
+  static constexpr bool value = S > RX_STRIDED_K0;
 };
+
+// Word offset of tile element e in HBM: column bits [0, cbits) stay, row
+// bits move up to [j0, j0 + k). The map is linear over disjoint bit ranges,
+// so off(e0 | (s << pos)) = off(e0) + off(s << pos).
+__device__ __forceinline__ uint32_t rx_goff(uint32_t e, uint32_t cbits, uint32_t j0) {
+  return ((e >> cbits) << j0) | (e & ((1u << cbits) - 1));
+}
 
 // One 16384-element tile of a pass; `a` may live in kernel arguments or in
 // global memory (work-list jobs), `lds` holds 2^14 words.
@@ -279,15 +287,14 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) v[s] = lds[lb ^ rx_swz(s << pos)];
     } else if (ri == 0 && kStrided) {
-      // element s sits (s << (pos - cbits + j0)) words after the lane's
-      // first one; the tile spans < 2^32 bytes (checked when the shape is
-      // chosen), and masking the source index only clears bits of hi_base
+      // element s sits off(s << pos) words after the lane's first one (a
+      // wave-uniform step); the tile spans < 2^32 bytes and masking the
+      // source index only clears bits of hi_base (both checked when the
+      // shape is chosen)
       const uint32_t* sp = src + ((hi_base & a.src_mask) | low_base);
-      const uint32_t e0 = rx_own(t, 0, pos);
-      const uint32_t lane = (((e0 >> a.cbits) << a.j0) | (e0 & cmask)) << 2;
-      const uint32_t sh = pos - a.cbits + a.j0 + 2;
+      const uint32_t lane = rx_goff(rx_own(t, 0, pos), a.cbits, a.j0) << 2;
 #pragma unroll
-      for (uint32_t s = 0; s < 32; s++) v[s] = ld_at(sp, lane + (s << sh));
+      for (uint32_t s = 0; s < 32; s++) v[s] = ld_at(sp, lane + (rx_goff(s << pos, a.cbits, a.j0) << 2));
     } else if (ri == 0) {
 #pragma unroll
       for (uint32_t s = 0; s < 32; s++) {
@@ -355,15 +362,14 @@ __device__ __forceinline__ void radix_tile(uint32_t* dst, const uint32_t* src, c
       }
     } else if (kStrided) {
       uint32_t* dp = dst + (hi_base | low_base);
-      const uint32_t e0 = rx_own(t, 0, pos);
-      const uint32_t lane = (((e0 >> a.cbits) << a.j0) | (e0 & cmask)) << 2;
-      const uint32_t sh = pos - a.cbits + a.j0 + 2;
+      const uint32_t lane = rx_goff(rx_own(t, 0, pos), a.cbits, a.j0) << 2;
       if (!FWD && a.scale != 1u) {
 #pragma unroll
-        for (uint32_t s = 0; s < 32; s++) st_at(dp, lane + (s << sh), m31_mul(v[s], a.scale));
+        for (uint32_t s = 0; s < 32; s++)
+          st_at(dp, lane + (rx_goff(s << pos, a.cbits, a.j0) << 2), m31_mul(v[s], a.scale));
       } else {
 #pragma unroll
-        for (uint32_t s = 0; s < 32; s++) st_at(dp, lane + (s << sh), v[s]);
+        for (uint32_t s = 0; s < 32; s++) st_at(dp, lane + (rx_goff(s << pos, a.cbits, a.j0) << 2), v[s]);
       }
     } else {
 #pragma unroll
@@ -396,8 +402,9 @@ __device__ __forceinline__ uint64_t xcd_tile(uint32_t b, uint32_t grid) {
 //   RX_CONTIG   bits 0..13 (j0 = 0, cbits = 0): rounds (rb0, nb, pos) =
 //               (9, 5, 9), (4, 5, 4), (0, 4, 0) for evaluate, reversed for
 //               interpolate (make_radix_args' schedule);
-//   RX_STRIDED  10 row bits x 16 columns (cbits = 4, j0 from the pass):
-//               (5, 5, 9), (0, 5, 4), reversed for interpolate.
+//   RX_STRIDED  k row bits x 2^(14-k) columns (k = 1..10, j0 from the
+//               pass; the kernel switches on k): for k = 10, (5, 5, 9),
+//               (0, 5, 4), reversed for interpolate.
 // A/B against the generic kernel on one box: 2^24 stwo evaluate 0.125 ->
 // 0.111 ms, Fibonacci LDE (2^24 -> 2^26) 0.412 -> 0.346 ms, the batched
 // 96-column LDE 2.23 -> 2.02 ms. The work-list kernel uses the contiguous
@@ -420,11 +427,17 @@ struct FixedArgs<FWD, RX_CONTIG> {
     for (int i = 0; i < 16; i++) tw[i] = a.tw[i];
   }
 };
-template <bool FWD>
-struct FixedArgs<FWD, RX_STRIDED> {
-  static constexpr uint32_t k = 10, cbits = 4, n_rounds = 2;
-  static constexpr RadixRound round[2] = {FWD ? RadixRound{5, 5, 9} : RadixRound{0, 5, 4},
-                                          FWD ? RadixRound{0, 5, 4} : RadixRound{5, 5, 9}};
+// Strided shape with K row bits (1 <= K <= 10) and 14 - K columns, j0 from
+// the pass; make_radix_args' schedule with constant rounds: K <= 5 is one
+// round (0, K, 9); K > 5 is (K - 5, 5, 9) then (0, K - 5, 14 - K) for
+// evaluate, reversed for interpolate (K = 10 is the 16-column shape above).
+template <bool FWD, int SHAPE>
+struct FixedArgs {
+  static constexpr uint32_t k = SHAPE - RX_STRIDED_K0, cbits = RX_E_BITS - k, n_rounds = k > 5 ? 2 : 1;
+  static_assert(k >= 1 && k <= 10, "strided shapes have 1..10 row bits");
+  static constexpr RadixRound hi_r = k > 5 ? RadixRound{k - 5, 5, 9} : RadixRound{0, k, 9};
+  static constexpr RadixRound lo_r = RadixRound{0, k > 5 ? k - 5 : 0, cbits};
+  static constexpr RadixRound round[2] = {n_rounds == 1 || FWD ? hi_r : lo_r, FWD ? lo_r : hi_r};
   uint32_t j0;
   const uint32_t* tw[16];
   uint64_t src_mask;
@@ -436,10 +449,36 @@ struct FixedArgs<FWD, RX_STRIDED> {
   }
 };
 
+// A strided tile of any row count: the pass's k is uniform over the launch
+// (or over the work-list job), so the switch costs a few scalar instructions.
+template <bool FWD, int MODE, typename A>
+__device__ __forceinline__ void radix_tile_strided(uint32_t* dst, const uint32_t* src, const A& a, uint64_t tile,
+                                                   uint32_t* lds) {
+#define VX_RX_K(K)                                                  \
+  case K: {                                                         \
+    const FixedArgs<FWD, RX_STRIDED_K0 + K> f(a);                   \
+    radix_tile<FWD, MODE>(dst, src, f, tile, lds);                  \
+    break;                                                          \
+  }
+  switch (a.k) {
+    VX_RX_K(1) VX_RX_K(2) VX_RX_K(3) VX_RX_K(4) VX_RX_K(5)
+    VX_RX_K(6) VX_RX_K(7) VX_RX_K(8) VX_RX_K(9) VX_RX_K(10)
+    default: break;
+  }
+#undef VX_RX_K
+}
+
+// Strided passes (cbits = 14 - k columns) take the fixed shapes when their
+// 32-bit element offsets fit (tile span below 2^32 bytes) and the source
+// mask keeps every in-tile bit, so masking touches only the tile base.
+static bool rx_strided_ok(uint32_t j0, uint32_t k, uint32_t cbits, uint64_t src_mask) {
+  return k >= 1 && k <= 10 && cbits == RX_E_BITS - k && cbits <= j0 && j0 + k <= 30 &&
+         src_mask >= (1ull << (j0 + k)) - 1;
+}
+
 static int radix_shape(const RadixArgs& r) {
   if (r.k == 14 && r.cbits == 0 && r.j0 == 0) return RX_CONTIG;
-  // the strided kernel's 32-bit element offsets need a tile span below 2^32 bytes
-  if (r.k == 10 && r.cbits == 4 && r.j0 + r.k <= 30) return RX_STRIDED;
+  if (rx_strided_ok(r.j0, r.k, r.cbits, r.src_mask)) return RX_STRIDED;
   return RX_GENERIC;
 }
 
@@ -455,6 +494,8 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const u
   }
   if constexpr (SHAPE == RX_GENERIC) {
     radix_tile<FWD, MODE>(dst, src, a, xcd_tile(blockIdx.x, gridDim.x), lds);
+  } else if constexpr (SHAPE == RX_STRIDED) {
+    radix_tile_strided<FWD, MODE>(dst, src, a, xcd_tile(blockIdx.x, gridDim.x), lds);
   } else {
     const FixedArgs<FWD, SHAPE> f(a);
     radix_tile<FWD, MODE>(dst, src, f, xcd_tile(blockIdx.x, gridDim.x), lds);
@@ -484,6 +525,8 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix_list(const RadixJob* j
   uniform_t<RadixJob>& J = as_uniform(jobs + lo);
   if constexpr (SHAPE == RX_GENERIC) {
     radix_tile<FWD, 0>(J.dst, J.src, J.a, b - J.first_tile, lds);
+  } else if constexpr (SHAPE == RX_STRIDED) {
+    radix_tile_strided<FWD, 0>(J.dst, J.src, J.a, b - J.first_tile, lds);
   } else {
     const FixedArgs<FWD, SHAPE> f(J.a);
     radix_tile<FWD, 0>(J.dst, J.src, f, b - J.first_tile, lds);
@@ -695,6 +738,7 @@ void lde_batch_build(const LdeBatchCol* cols, uint32_t n, uint32_t blowup_bits, 
     stage[0].push_back({C.coef, C.in ? C.in : C.coef, 0,
                         make_radix_args(false, C.itw, 0, RX_E_BITS, 0, nmask, sb == 0 ? inv_n : 1u)});
     if (sb) {
+      require(rx_strided_ok(RX_E_BITS, sb, RX_E_BITS - sb, nmask), "lde_batch: strided pass shape");
       stage[1].push_back({C.coef, C.coef, 0, make_radix_args(false, C.itw, RX_E_BITS, sb, RX_E_BITS - sb, nmask, inv_n)});
       stage[2].push_back({C.eval, C.coef, 0, make_radix_args(true, C.etw, RX_E_BITS, sb, RX_E_BITS - sb, nmask, 1u)});
       stage[3].push_back({C.eval, C.eval, 0, make_radix_args(true, C.etw, 0, RX_E_BITS, 0, Emask, 1u)});
@@ -727,18 +771,19 @@ void lde_batch_launch(const uint8_t* d_jobs, const LdeBatchStages& st, hipStream
     if (!st.count[k]) continue;
     require(st.tiles[k] < (1ull << 31), "lde_batch: too many tiles");
     const RadixJob* jobs = reinterpret_cast<const RadixJob*>(d_jobs + st.offset[k]);
-    // stages 0 and 3 are the contiguous 14-bit passes of every column
+    // stages 0 and 3 are the contiguous 14-bit passes of every column,
+    // stages 1 and 2 strided passes of 1..10 row bits (checked when built)
     if (k == 3)
       hipLaunchKernelGGL((k_ntt_radix_list<true, RX_CONTIG>), dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s,
                          jobs, st.count[k]);
     else if (k == 2)
-      hipLaunchKernelGGL((k_ntt_radix_list<true, RX_GENERIC>), dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s,
+      hipLaunchKernelGGL((k_ntt_radix_list<true, RX_STRIDED>), dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s,
                          jobs, st.count[k]);
     else if (k == 0)
       hipLaunchKernelGGL((k_ntt_radix_list<false, RX_CONTIG>), dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s,
                          jobs, st.count[k]);
     else
-      hipLaunchKernelGGL((k_ntt_radix_list<false, RX_GENERIC>), dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s,
+      hipLaunchKernelGGL((k_ntt_radix_list<false, RX_STRIDED>), dim3((uint32_t)st.tiles[k]), dim3(RX_THREADS), 0, s,
                          jobs, st.count[k]);
     VX_HIP(hipGetLastError());
   }
