@@ -53,7 +53,7 @@ def test_lde_batch_matches_oracle_per_column():
 
 def test_bit_reverse_batch():
     rng = np.random.default_rng(12)
-    logs = [10, 13, 11, 16]
+    logs = [10, 13, 11, 16, 14, 15, 12]
     vals = [rng.integers(0, po.P, size=1 << L, dtype=np.uint32) for L in logs]
     src = [dev(v) for v in vals]
     dst = [empty(1 << L) for L in logs]

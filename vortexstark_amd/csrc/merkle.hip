@@ -235,9 +235,51 @@ __device__ __forceinline__ void d2_produce(const LeafSrc& L, uint64_t i, G4& g, 
 __device__ __forceinline__ uint64_t lifted_row(uint64_t idx, uint32_t S, uint32_t L) {
   return L == S ? idx : (((idx >> (S - L + 1)) << 1) | (idx & 1));
 }
+// Word at a 32-bit byte offset from a wave-uniform global base: one global
+// load with a scalar base and a VGPR offset, no 64-bit address arithmetic.
+__device__ __forceinline__ uint32_t ldg_at(const uint32_t* base, uint32_t byte_off) {
+  const __attribute__((address_space(1))) char* b = (const __attribute__((address_space(1))) char*)base;
+  return *(const __attribute__((address_space(1))) uint32_t*)(b + byte_off);
+}
+// The 16 message words of columns [base, base + 16) at leaf i's lifted rows
+// (zero past the last column). Columns come sorted by size, so the row's
+// byte offset is recomputed only where the size changes (a uniform branch);
+// a column then costs one scalar table load and one global load.
+struct LiftCursor {
+  uint32_t log, off;
+};
+__device__ __forceinline__ void lifted_block(const LiftCol* cols, uint32_t n, uint32_t S, uint32_t i, uint32_t base,
+                                             LiftCursor& cur, uint32_t (&m)[16]) {
+#pragma unroll
+  for (uint32_t j = 0; j < 16; j++) {
+    const uint32_t c = base + j;
+    m[j] = 0;
+    if (c < n) {
+      const LiftCol col = ld_uniform(cols + c);
+      if (col.log_n != cur.log) {
+        cur.log = col.log_n;
+        cur.off = col.log_n == S ? i << 2 : (((i >> (S - col.log_n + 1)) << 3) | ((i & 1u) << 2));
+      }
+      m[j] = ldg_at(col.p, cur.off);
+    }
+  }
+}
 __device__ __forceinline__ void lifted_leaf_cols(const LiftCol* cols, uint32_t n, uint32_t S, uint64_t idx,
                                                  uint32_t h[8]) {
   b2_init(h);
+  if (S <= 30) {
+    // rows and byte offsets fit 32 bits. (Loading the next block before
+    // compressing this one takes the kernel from 70 to about 250 VGPRs.)
+    const uint32_t i = (uint32_t)idx;
+    LiftCursor cur{~0u, 0u};
+    for (uint32_t base = 0; base < n; base += 16) {
+      uint32_t m[16];
+      const bool more = base + 16 < n;
+      lifted_block(cols, n, S, i, base, cur, m);
+      b2_compress(h, m, 4 * (more ? base + 16 : n), !more);
+    }
+    return;
+  }
   uint32_t t = 0;
   for (uint32_t base = 0; base < n; base += 16) {
     uint32_t m[16];

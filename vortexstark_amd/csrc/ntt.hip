@@ -802,7 +802,9 @@ struct BitrevJob {
   uint64_t first_tile;
 };
 // Columns of log size >= 12 move in 64 x 64 tiles (256-byte runs on both
-// sides), smaller ones (10, 11) in 32 x 32 tiles.
+// sides), smaller ones (10, 11) in 32 x 32 tiles. (128 x 128 tiles, 512-byte
+// runs at two workgroups per CU, took the 96-column bit reverse 0.59 ->
+// 0.61 ms.)
 static constexpr uint32_t BR_BIG = 6, BR_SMALL = 5, BR_MIN_LOG = 2 * BR_SMALL;
 static constexpr uint32_t BR_WGS = 2048;  // persistent workgroups (8 per CU)
 
