@@ -671,6 +671,46 @@ int vx_compute_vanishing(uint32_t initial_x, uint32_t initial_y, uint32_t step_x
 /* Replaces cuda_axpy_m31 (ffi.rs:588-590, cairo_constraint.cu:1037-1057): y += scalar * x. */
 int vx_axpy_m31(uint32_t scalar, const uint32_t *d_x, uint32_t *d_y, uint32_t n, vx_stream_t stream);
 
+/* ------------------------------------------------ bytecode constraint eval
+ * Replaces cuda_bytecode_constraint_eval (ffi.rs:886-902,
+ * cuda/constraint_eval.cu:97-451; called from the stwo backend's
+ * component_prover.rs:478-510): the register bytecode of
+ * crates/cuda-backend/src/constraint_eval/bytecode.rs (device pointer,
+ * n_words words) run for every row of the evaluation domain (n_rows, a
+ * power of two, bit-reversed order) over the device array d_trace_cols of
+ * n_trace_cols column pointers (a null column reads as 0); accum_c[row] +=
+ * (denom_inv[row >> log2(trace_n_rows)] * sum_k coeff_k * constraint_k)_c
+ * with coeff_k the k-th QM31 of d_random_coeff_powers. trace_col_sizes,
+ * log_expand and n_registers are accepted and unused (the first two are
+ * implied, the register file is sized from the program). Unlike the
+ * reference, a malformed program (unknown opcode, truncated instruction,
+ * column out of range, a register read before it is written, a constant
+ * that is not a canonical M31 value, row offsets with n_rows == trace_n_rows)
+ * is rejected with VX_ERR_ARG and nothing is written; any 16-bit register
+ * index is accepted. The program is read back to the host, so the call
+ * synchronises `stream` before and after the evaluation. */
+int vx_bytecode_constraint_eval(const uint32_t *d_bytecode, uint32_t n_words, const uint32_t *const *d_trace_cols,
+                                const uint32_t *d_trace_col_sizes, uint32_t n_trace_cols, uint32_t n_rows,
+                                uint32_t trace_n_rows, const uint32_t *d_random_coeff_powers,
+                                const uint32_t *d_denom_inv, uint32_t log_expand, uint32_t *accum0,
+                                uint32_t *accum1, uint32_t *accum2, uint32_t *accum3, uint32_t n_registers,
+                                vx_stream_t stream);
+/* Replaces cuda_warp_bytecode_constraint_eval (ffi.rs:908-925,
+ * cuda/constraint_eval_warp.cu), the reference's kernel for programs with
+ * more than 1,024 registers: the same evaluation as above (registers are
+ * renamed to the program's live values, so no second kernel is needed). */
+int vx_warp_bytecode_constraint_eval(const uint32_t *d_bytecode, uint32_t n_words,
+                                     const uint32_t *const *d_trace_cols, const uint32_t *d_trace_col_sizes,
+                                     uint32_t n_trace_cols, uint32_t n_rows, uint32_t trace_n_rows,
+                                     const uint32_t *d_random_coeff_powers, const uint32_t *d_denom_inv,
+                                     uint32_t log_expand, uint32_t *accum0, uint32_t *accum1, uint32_t *accum2,
+                                     uint32_t *accum3, uint32_t n_registers, vx_stream_t stream);
+/* Host only, no reference counterpart: checks a program (host pointer) as
+ * the evaluation does and reports its register file after renaming (M31
+ * slots of one word per row, QM31 slots of four) and its constraint count. */
+int vx_bytecode_slots(const uint32_t *h_bytecode, uint32_t n_words, uint32_t n_trace_cols, uint32_t *m31_slots,
+                      uint32_t *qm31_slots, uint32_t *n_constraints);
+
 /* Resident batch (the batched-commit workload): inputs, coefficients and
  * evaluations of n_cols columns (log sizes sorted ascending, 14..24) kept in
  * HBM; each run is four work-list LDE launches, one bit-reverse launch and one

@@ -122,6 +122,9 @@ def lib(omp=False):
                                             _u32p]),
         "or_quotients_combine": (None, [_u32p, _u32p, _u32p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         _u32p, u32, _u32p, _u32p, u32, u64, _u32p, _u32p, _u32p, _u32p]),
+        "or_bc_offset_row": (u32, [u32, u32, u32, C.c_int32]),
+        "or_bytecode_constraint_eval": (C.c_int, [_u32p, u32, C.c_void_p, u32, u32, u32, _u32p, _u32p, _u32p,
+                                                  _u32p, _u32p, _u32p]),
     }
     for name_, (res, args) in sig.items():
         f = getattr(L, name_)
@@ -687,3 +690,25 @@ def lift_and_accumulate(cols):
     logs = np.array([c.size.bit_length() - 1 for c in cs], np.uint32)
     lib().or_lift_and_accumulate(_ptrs(cs), logs, len(cs))
     return cs
+
+
+def bc_offset_row(i, domain_log, eval_log, offset):
+    """stwo's offset_bit_reversed_circle_domain_index (cuda/constraint_eval.cu:63-94)."""
+    return int(lib().or_bc_offset_row(i, domain_log, eval_log, offset))
+
+
+def bytecode_eval(words, cols, n_rows, trace_n_rows, coeff, denom_inv, acc=None):
+    """cuda_bytecode_constraint_eval restated (cuda/constraint_eval.cu:97-407):
+    returns the 4 accumulator columns after adding every row's weighted
+    constraint sum times denom_inv[row >> trace_log]. cols: arrays or None
+    (a null column reads as 0). Raises ValueError where the oracle rejects
+    the program (column out of range, unknown opcode, truncated)."""
+    w = u32(words)
+    keep = [None if c is None else u32(c) for c in cols]
+    ptrs = (C.c_void_p * max(len(keep), 1))(*[None if c is None else c.ctypes.data for c in keep])
+    out = [np.zeros(n_rows, np.uint32) if acc is None else u32(acc[k]).copy() for k in range(4)]
+    rc = lib().or_bytecode_constraint_eval(w, w.size, ptrs, len(keep), n_rows, trace_n_rows, u32(coeff),
+                                           u32(denom_inv), *out)
+    if rc != 0:
+        raise ValueError("oracle rejected the bytecode")
+    return out

@@ -2421,3 +2421,120 @@ void or_cairo_quotient_chunk(const uint32_t *const cols[34], const uint32_t *con
         for (int k = 0; k < 4; k++) out[k][sl] = q.v[k];
     }
 }
+
+/* ======================================================================
+ * Generic constraint evaluation (cuda/constraint_eval.cu:97-407): the
+ * reference's register bytecode interpreted row by row.
+ * ====================================================================== */
+
+static uint32_t bitrev32(uint32_t v, uint32_t bits) {
+    uint32_t r = 0;
+    for (uint32_t k = 0; k < bits; k++) r |= ((v >> k) & 1u) << (bits - 1 - k);
+    return r;
+}
+
+/* stwo's offset_bit_reversed_circle_domain_index (cuda/constraint_eval.cu:
+ * 63-94), with the step in 64 bits */
+uint32_t or_bc_offset_row(uint32_t i, uint32_t domain_log, uint32_t eval_log, int32_t offset) {
+    uint32_t prev = bitrev32(i, eval_log);
+    const int64_t half = (int64_t)1 << (eval_log - 1);
+    const int64_t step = (int64_t)offset * ((int64_t)1 << (eval_log - domain_log - 1));
+    int64_t idx;
+    if ((int64_t)prev < half) {
+        idx = ((int64_t)prev + step) % half;
+        if (idx < 0) idx += half;
+    } else {
+        idx = ((int64_t)prev - half - step) % half;
+        if (idx < 0) idx += half;
+        idx += half;
+    }
+    return bitrev32((uint32_t)idx, eval_log);
+}
+
+static uint32_t ilog2_u32(uint32_t v) {
+    uint32_t l = 0;
+    while ((1u << l) < v) l++;
+    return l;
+}
+
+int or_bytecode_constraint_eval(const uint32_t *bc, uint32_t n_words, const uint32_t *const *cols,
+                                uint32_t n_cols, uint32_t n_rows, uint32_t trace_n_rows, const uint32_t *coeff,
+                                const uint32_t *denom_inv, uint32_t *acc0, uint32_t *acc1, uint32_t *acc2,
+                                uint32_t *acc3) {
+    qm31 *regs = (qm31 *)calloc(1u << 16, sizeof(qm31));
+    if (!regs) return -1;
+    const uint32_t eval_log = ilog2_u32(n_rows), trace_log = ilog2_u32(trace_n_rows);
+    int rc = 0;
+    for (uint32_t row = 0; row < n_rows && rc == 0; row++) {
+        qm31 accq = {{0, 0, 0, 0}};
+        uint32_t k = 0, pc = 0;
+#define BC_NEXT(x)                          \
+    do {                                    \
+        if (pc >= n_words) { rc = -1; break; } \
+        (x) = bc[pc++];                     \
+    } while (0)
+        while (pc < n_words && rc == 0) {
+            const uint32_t w = bc[pc++], op = w >> 24, dst = w & 0xFFFFu;
+            uint32_t w2 = 0, w3 = 0, c[4] = {0, 0, 0, 0};
+            qm31 r = {{0, 0, 0, 0}};
+            int writes = 1;
+            switch (op) {
+            case 0x01: BC_NEXT(w2); r.v[0] = w2; break;                         /* LoadConst */
+            case 0x02: for (int j = 0; j < 4; j++) BC_NEXT(c[j]);               /* LoadSecureConst */
+                       r = q_load(c); break;
+            case 0x03: {                                                        /* LoadTrace */
+                BC_NEXT(w2); BC_NEXT(w3);
+                if (rc) break;
+                if (w2 >= n_cols) { rc = -1; break; }
+                const int32_t off = (w3 >> 31) ? -(int32_t)(w3 & 0x7FFFFFFFu) : (int32_t)(w3 & 0x7FFFFFFFu);
+                const uint32_t rr = off == 0 ? row : or_bc_offset_row(row, trace_log, eval_log, off);
+                r.v[0] = cols[w2] ? cols[w2][rr] : 0;
+                break;
+            }
+            case 0x10: BC_NEXT(w2); r.v[0] = m31_add(regs[w2 >> 16].v[0], regs[w2 & 0xFFFFu].v[0]); break;
+            case 0x11: BC_NEXT(w2); r.v[0] = m31_sub(regs[w2 >> 16].v[0], regs[w2 & 0xFFFFu].v[0]); break;
+            case 0x12: BC_NEXT(w2); r.v[0] = m31_mul(regs[w2 >> 16].v[0], regs[w2 & 0xFFFFu].v[0]); break;
+            case 0x13: BC_NEXT(w2); r.v[0] = m31_neg(regs[w2 >> 16].v[0]); break;
+            case 0x14: BC_NEXT(w2); BC_NEXT(w3); r.v[0] = m31_add(regs[w2 >> 16].v[0], w3); break;
+            case 0x15: BC_NEXT(w2); BC_NEXT(w3); r.v[0] = m31_mul(regs[w2 >> 16].v[0], w3); break;
+            case 0x20: BC_NEXT(w2); r = q_add(regs[w2 >> 16], regs[w2 & 0xFFFFu]); break;
+            case 0x21: BC_NEXT(w2); r = q_sub(regs[w2 >> 16], regs[w2 & 0xFFFFu]); break;
+            case 0x22: BC_NEXT(w2); r = q_mul(regs[w2 >> 16], regs[w2 & 0xFFFFu]); break;
+            case 0x23: BC_NEXT(w2); r = q_sub((qm31){{0, 0, 0, 0}}, regs[w2 >> 16]); break;
+            case 0x24: BC_NEXT(w2); for (int j = 0; j < 4; j++) BC_NEXT(c[j]);
+                       r = q_add(regs[w2 >> 16], q_load(c)); break;
+            case 0x25: BC_NEXT(w2); for (int j = 0; j < 4; j++) BC_NEXT(c[j]);
+                       r = q_mul(regs[w2 >> 16], q_load(c)); break;
+            case 0x28: BC_NEXT(w2); r = regs[w2 >> 16];                         /* WideAddBase */
+                       r.v[0] = m31_add(r.v[0], regs[w2 & 0xFFFFu].v[0]); break;
+            case 0x29: BC_NEXT(w2); r = q_scale(regs[w2 >> 16], regs[w2 & 0xFFFFu].v[0]); break;
+            case 0x2A: BC_NEXT(w2); for (int j = 0; j < 4; j++) BC_NEXT(c[j]);
+                       r = q_add((qm31){{regs[w2 >> 16].v[0], 0, 0, 0}}, q_load(c)); break;
+            case 0x2B: BC_NEXT(w2); for (int j = 0; j < 4; j++) BC_NEXT(c[j]);
+                       r = q_scale(q_load(c), regs[w2 >> 16].v[0]); break;
+            case 0x30: BC_NEXT(w2); r.v[0] = regs[w2 >> 16].v[0]; break;       /* Widen */
+            case 0x31: BC_NEXT(w2); BC_NEXT(w3);                                /* CombineEF */
+                       r = (qm31){{regs[w2 >> 16].v[0], regs[w2 & 0xFFFFu].v[0], regs[w3 >> 16].v[0],
+                                   regs[w3 & 0xFFFFu].v[0]}};
+                       break;
+            case 0x40:                                                          /* AddConstraint */
+                accq = q_add(accq, q_mul(regs[dst], q_load(coeff + 4 * k)));
+                k++;
+                writes = 0;
+                break;
+            default: rc = -1;
+            }
+            if (rc) break;
+            if (writes) regs[dst] = r;
+        }
+#undef BC_NEXT
+        if (rc) break;
+        const qm31 res = q_scale(accq, denom_inv[row >> trace_log]);
+        acc0[row] = m31_add(acc0[row], res.v[0]);
+        acc1[row] = m31_add(acc1[row], res.v[1]);
+        acc2[row] = m31_add(acc2[row], res.v[2]);
+        acc3[row] = m31_add(acc3[row], res.v[3]);
+    }
+    free(regs);
+    return rc;
+}

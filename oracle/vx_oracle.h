@@ -220,6 +220,19 @@ void or_quotients_combine(const uint32_t *sp_x, const uint32_t *sp_y, const uint
                           const uint32_t *domain_ys, uint32_t lifting_log_size, uint64_t n_rows, uint32_t *o0,
                           uint32_t *o1, uint32_t *o2, uint32_t *o3);
 
+/* ---- generic constraint evaluation: cuda/constraint_eval.cu:97-407 ----
+ * The reference's register bytecode run row by row with a 65,536-entry QM31
+ * register file; acc_c[row] += (denom_inv[row >> trace_log] * sum_k
+ * coeff_k * constraint_k)_c. Returns 0, or -1 (accumulators unchanged)
+ * for a column index out of range, from which the reference kernel's rows
+ * return without accumulating, and for an unknown opcode or a truncated
+ * instruction, which the reference kernel skips or reads past. */
+uint32_t or_bc_offset_row(uint32_t i, uint32_t domain_log, uint32_t eval_log, int32_t offset);
+int or_bytecode_constraint_eval(const uint32_t *bytecode, uint32_t n_words, const uint32_t *const *cols,
+                                uint32_t n_cols, uint32_t n_rows, uint32_t trace_n_rows, const uint32_t *coeff,
+                                const uint32_t *denom_inv, uint32_t *acc0, uint32_t *acc1, uint32_t *acc2,
+                                uint32_t *acc3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@
 #include "prover.h"
 #include "batch.h"
 #include "export.h"
+#include "vx_bytecode.h"
 #include "tune.h"
 #include "server.h"
 #include "vx_cairo.h"
@@ -1500,6 +1501,76 @@ int vx_compute_vanishing(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uin
 
 int vx_axpy_m31(uint32_t scalar, const uint32_t* d_x, uint32_t* d_y, uint32_t n, vx_stream_t st) {
   return guard([&] { axpy_m31(scalar, d_x, d_y, n, S(st)); });
+}
+
+// ------------------------------------------------- bytecode constraint eval
+// The program is read back to the host (it is small), checked and renamed
+// to slots, then evaluated; the call returns once the accumulators are
+// written (its temporary device buffers are released at return).
+static int bytecode_call(const uint32_t* d_bytecode, uint32_t n_words, const uint32_t* const* d_trace_cols,
+                         uint32_t n_trace_cols, uint32_t n_rows, uint32_t trace_n_rows,
+                         const uint32_t* d_coeff, const uint32_t* d_denom_inv, uint32_t* a0, uint32_t* a1,
+                         uint32_t* a2, uint32_t* a3, vx_stream_t st) {
+  return guard([&] {
+    require(n_rows >= 1 && pow2(n_rows) && trace_n_rows >= 1 && pow2(trace_n_rows) && trace_n_rows <= n_rows,
+            "bytecode eval: n_rows and trace_n_rows must be powers of two, trace_n_rows <= n_rows");
+    require(n_words == 0 || d_bytecode != nullptr, "bytecode eval: null program");
+    require(d_denom_inv && a0 && a1 && a2 && a3, "bytecode eval: null denominators or accumulators");
+    require(n_trace_cols == 0 || d_trace_cols != nullptr, "bytecode eval: null column table");
+    hipStream_t s = S(st);
+    std::vector<uint32_t> words(n_words);
+    if (n_words) {
+      VX_HIP(hipMemcpyAsync(words.data(), d_bytecode, 4ull * n_words, hipMemcpyDeviceToHost, s));
+      VX_HIP(hipStreamSynchronize(s));
+    }
+    const BcProgram p = bc_compile(words.data(), n_words, n_trace_cols);
+    require(!p.uses_offsets || n_rows > trace_n_rows,
+            "bytecode eval: row offsets need an evaluation domain larger than the trace domain");
+    require(p.n_constraints == 0 || d_coeff != nullptr, "bytecode eval: null constraint coefficients");
+    DeviceBuf ins(16 * (p.ins.size() ? p.ins.size() : 1)), kon(4 * (p.consts.size() ? p.consts.size() : 4));
+    DeviceBuf ws(bytecode_ws_bytes(p, n_rows));
+    if (!p.ins.empty()) VX_HIP(hipMemcpyAsync(ins.p, p.ins.data(), 16 * p.ins.size(), hipMemcpyHostToDevice, s));
+    if (!p.consts.empty())
+      VX_HIP(hipMemcpyAsync(kon.p, p.consts.data(), 4 * p.consts.size(), hipMemcpyHostToDevice, s));
+    uint32_t* acc[4] = {a0, a1, a2, a3};
+    bytecode_constraint_eval(p, ins.u32(), kon.u32(), d_trace_cols, n_rows, trace_n_rows, d_coeff, d_denom_inv,
+                             acc, ws.p ? ws.u32() : nullptr, s);
+    VX_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int vx_bytecode_constraint_eval(const uint32_t* d_bytecode, uint32_t n_words, const uint32_t* const* d_trace_cols,
+                                const uint32_t* d_trace_col_sizes, uint32_t n_trace_cols, uint32_t n_rows,
+                                uint32_t trace_n_rows, const uint32_t* d_random_coeff_powers,
+                                const uint32_t* d_denom_inv, uint32_t log_expand, uint32_t* accum0,
+                                uint32_t* accum1, uint32_t* accum2, uint32_t* accum3, uint32_t n_registers,
+                                vx_stream_t stream) {
+  (void)d_trace_col_sizes;  // reserved in the reference as well
+  (void)log_expand;         // implied by n_rows / trace_n_rows
+  (void)n_registers;        // the slot count comes from the program itself
+  return bytecode_call(d_bytecode, n_words, d_trace_cols, n_trace_cols, n_rows, trace_n_rows, d_random_coeff_powers,
+                       d_denom_inv, accum0, accum1, accum2, accum3, stream);
+}
+
+int vx_warp_bytecode_constraint_eval(const uint32_t* d_bytecode, uint32_t n_words,
+                                     const uint32_t* const* d_trace_cols, const uint32_t* d_trace_col_sizes,
+                                     uint32_t n_trace_cols, uint32_t n_rows, uint32_t trace_n_rows,
+                                     const uint32_t* d_random_coeff_powers, const uint32_t* d_denom_inv,
+                                     uint32_t log_expand, uint32_t* accum0, uint32_t* accum1, uint32_t* accum2,
+                                     uint32_t* accum3, uint32_t n_registers, vx_stream_t stream) {
+  return vx_bytecode_constraint_eval(d_bytecode, n_words, d_trace_cols, d_trace_col_sizes, n_trace_cols, n_rows,
+                                     trace_n_rows, d_random_coeff_powers, d_denom_inv, log_expand, accum0, accum1,
+                                     accum2, accum3, n_registers, stream);
+}
+
+int vx_bytecode_slots(const uint32_t* h_bytecode, uint32_t n_words, uint32_t n_trace_cols, uint32_t* m31_slots,
+                      uint32_t* qm31_slots, uint32_t* n_constraints) {
+  return guard([&] {
+    const BcProgram p = bc_compile(h_bytecode, n_words, n_trace_cols);
+    if (m31_slots) *m31_slots = p.m31_slots;
+    if (qm31_slots) *qm31_slots = p.qm31_slots;
+    if (n_constraints) *n_constraints = p.n_constraints;
+  });
 }
 
 // ------------------------------------------------------------------ serving

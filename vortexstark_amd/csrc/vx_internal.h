@@ -338,6 +338,19 @@ void cairo_quotient(const CairoArgs& a, uint64_t rows, hipStream_t s);
 void cairo_vanishing(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* out, uint32_t log_eval,
                      uint32_t log_n, bool inverse, hipStream_t s);
 void axpy_m31(uint32_t sc, const uint32_t* x, uint32_t* y, uint64_t n, hipStream_t s);
+
+// ------------------------------------------- bytecode constraint eval (bytecode.hip)
+// Evaluates a program renamed by bc_compile (d_ins: its 16-byte records,
+// d_consts: its QM31 constants) over n_rows rows; d_ws holds
+// bytecode_ws_bytes(p, n_rows) bytes (null when that is 0).
+struct BcProgram;
+void bytecode_constraint_eval(const BcProgram& p, const uint32_t* d_ins, const uint32_t* d_consts,
+                              const uint32_t* const* d_cols, uint32_t n_rows, uint32_t trace_n_rows,
+                              const uint32_t* coeff, const uint32_t* denom_inv, uint32_t* const acc[4],
+                              uint32_t* d_ws, hipStream_t s);
+uint32_t bytecode_row_words(const BcProgram& p);
+uint32_t bytecode_ws_rows(const BcProgram& p, uint32_t n_rows);
+uint64_t bytecode_ws_bytes(const BcProgram& p, uint32_t n_rows);
 // Root of a lifted tree with the column table and the stored tree levels in
 // a file-scope device scratch array (noalloc.hip): no device allocation.
 // Asynchronous; calls that overlap on one GPU share the scratch and race.
