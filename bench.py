@@ -549,6 +549,120 @@ def cairo_commit(vx, steps, log_eval=24, seed=0xCA1F0, step=4):
     }
 
 
+def bytecode_program(seed, n_ops, n_cols, n_constraints):
+    """A Cairo-sized constraint program in the stwo backend's register
+    bytecode: loads of every column at row offsets 0 and +1, mostly M31
+    products and sums with some QM31 work, n_constraints constraints."""
+    import random
+
+    rng = random.Random(seed)
+    words, live, wide = [], [], []
+    reg = 0
+
+    def new():
+        nonlocal reg
+        reg += 1
+        return reg - 1
+
+    for c in range(n_cols):
+        for off in (0, 1):
+            d = new()
+            words += [(0x03 << 24) | d, c, off]
+            live.append(d)
+    per = max(1, n_ops // n_constraints)
+    for i in range(n_ops):
+        d = new()
+        a = rng.choice(live[-12:])
+        b = rng.choice(live[:2 * n_cols]) if rng.random() < 0.3 else rng.choice(live[-12:])
+        kind = rng.random()
+        if kind < 0.45:
+            words += [(0x12 << 24) | d, (a << 16) | b]       # mul
+            live.append(d)
+        elif kind < 0.75:
+            words += [(rng.choice((0x10, 0x11)) << 24) | d, (a << 16) | b]  # add / sub
+            live.append(d)
+        elif kind < 0.85:
+            words += [(0x15 << 24) | d, a << 16, rng.randrange(P)]  # mul by a constant
+            live.append(d)
+        elif kind < 0.95 or not wide:
+            words += [(0x2B << 24) | d, a << 16] + [rng.randrange(P) for _ in range(4)]  # base * secure const
+            wide.append(d)
+        else:
+            words += [(0x22 << 24) | d, (rng.choice(wide[-4:]) << 16) | rng.choice(wide[-4:])]  # QM31 product
+            wide.append(d)
+        if i % per == per - 1:
+            words += [(0x40 << 24) | (wide[-1] if wide and rng.random() < 0.3 else live[-1])]
+    return words
+
+
+def bytecode_eval(vx, steps, log_eval=22, log_expand=2, n_ops=800, n_cols=34, n_constraints=40):
+    """The bytecode constraint evaluator (vx_bytecode_constraint_eval) on a
+    Cairo-sized program over 34 synthetic columns resident in HBM: one call
+    per step (each reads the program back, renames it and evaluates every
+    row). Unique HBM bytes per row: each column once (the +1 row reads hit
+    cache) and the four accumulator words read and written."""
+    import ctypes as C
+
+    import numpy as np
+
+    call = vx._lib.call
+    n = 1 << log_eval
+    words = np.array(bytecode_program(7, n_ops, n_cols, n_constraints), np.uint32)
+    m, q, k = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    call("vx_bytecode_slots", words.ctypes.data, words.size, n_cols, C.byref(m), C.byref(q), C.byref(k))
+    bufs = []
+
+    def alloc(nwords):
+        p = C.c_void_p()
+        call("vx_malloc", C.byref(p), 4 * nwords)
+        bufs.append(p)
+        return p.value
+
+    try:
+        cols = [alloc(n) for _ in range(n_cols)]
+        for c, ptr in enumerate(cols):
+            call("vx_m31_fill_random", ptr, log_eval, 0xB7C0DE, c, None)
+        table = alloc(2 * n_cols)
+        arr = (C.c_uint64 * n_cols)(*cols)
+        call("vx_memcpy", table, C.cast(arr, C.c_void_p), 8 * n_cols, 1)
+        prog = alloc(words.size)
+        call("vx_memcpy", prog, C.c_void_p(words.ctypes.data), 4 * words.size, 1)
+        coeff = alloc(4 * max(k.value, 1))
+        call("vx_m31_fill_random", coeff, 8, 0xB7C0DE, 99, None)
+        denom = alloc(1 << log_expand)
+        call("vx_m31_fill_random", denom, log_expand, 0xB7C0DE, 98, None)
+        acc = [alloc(n) for _ in range(4)]
+        for a in acc:
+            call("vx_memset", a, 0, 4 * n)
+
+        def run():
+            call("vx_bytecode_constraint_eval", prog, int(words.size), table, None, n_cols, n, n >> log_expand,
+                 coeff, denom, log_expand, *acc, 0, None)
+
+        run()
+        call("vx_device_sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        call("vx_device_sync")
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+    finally:
+        for p in bufs:
+            vx.load().vx_free(p)
+    hbm = n * (4 * n_cols + 32)
+    return {
+        "workload": f"{n_ops}-op constraint program over {n_cols} columns (offsets 0, +1), {k.value} constraints, "
+                    f"{n} rows (trace 2^{log_eval - log_expand}, blowup {1 << log_expand})",
+        "register_file_words_per_row": m.value + 4 * q.value,
+        "ms": round(ms, 3),
+        "rows_per_s": round(n / (ms * 1e-3), 1),
+        "hbm_gbs": round(hbm / (ms * 1e-3) / 1e9, 1),
+        "hbm_frac": round(hbm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "steps": steps,
+        "synchronous": True,
+    }
+
+
 def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     """stwo-basis circle NTT (the stwo plugin's PolyOps): evaluate and
     interpolate of one 2^log_n column resident in HBM, asynchronous calls
@@ -822,6 +936,8 @@ def main():
     ap.add_argument("--no-stwo", action="store_true")
     ap.add_argument("--no-cairo", action="store_true")
     ap.add_argument("--cairo-only", action="store_true")
+    ap.add_argument("--no-bytecode", action="store_true")
+    ap.add_argument("--bytecode-only", action="store_true")
     ap.add_argument("--stwo-only", action="store_true")
     ap.add_argument("--profile-serving-mix", action="store_true",
                     help="eager proofs of every serving class (7 per Fibonacci class, 9 per synthetic class) "
@@ -848,6 +964,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batched_commit": batched(vx, args.steps)}), flush=True)
+        return
+    if args.bytecode_only:
+        print(json.dumps({"bytecode_eval": bytecode_eval(vx, args.steps)}), flush=True)
         return
     if args.cairo_only:
         print(json.dumps({"cairo_commit": cairo_commit(vx, args.steps)}), flush=True)
@@ -953,6 +1072,8 @@ def main():
         out["batched_commit"] = batched(vx, args.steps)
     if not args.no_cairo and not args.profile:
         out["cairo_commit"] = cairo_commit(vx, args.steps)
+    if not args.no_bytecode and not args.profile:
+        out["bytecode_eval"] = bytecode_eval(vx, args.steps)
     if not args.no_stwo and not args.profile:
         out["stwo_ntt"] = stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline and gpu_a is not None:

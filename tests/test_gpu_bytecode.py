@@ -76,8 +76,9 @@ def test_random_programs_match_oracle(seed, log_rows, log_expand, n_ops):
 
 @pytest.mark.parametrize("live,in_lds", [(6, True), (90, False)])
 def test_register_file_in_lds_and_in_workspace(live, in_lds):
-    # `live` QM31 values held at once: 6 (29 words per row) fit the LDS
-    # register file (<= 40 words), 90 (361 words) need the workspace
+    # `live` QM31 values held at once: 6 (32 words per row with the four
+    # trace values) fit the LDS register file (<= 128 words), 90 (364 words,
+    # trace values re-read at each use) need the workspace
     import ctypes as C
     rng = random.Random(live)
     p = pb.Prog()
@@ -93,7 +94,7 @@ def test_register_file_in_lds_and_in_workspace(live, in_lds):
     words = np.array(p.encode(), np.uint32)
     m, q, k = C.c_uint32(), C.c_uint32(), C.c_uint32()
     call("vx_bytecode_slots", words.ctypes.data, words.size, 4, C.byref(m), C.byref(q), C.byref(k))
-    assert (m.value + 4 * q.value <= 40) == in_lds
+    assert (m.value + 4 * q.value <= 128) == in_lds
     n_rows = 1 << 11
     npr = np.random.default_rng(live)
     cols = [npr.integers(0, P, n_rows, dtype=np.uint32) for _ in range(4)]

@@ -114,7 +114,8 @@ def test_host_pass_renames_to_live_values():
         p.add_const(r, r - 1, 3)
     p.add_constraint(199)
     assert slots(p.encode(), 1) == (1, 0, 1)
-    # 100 values all live at once, then summed
+    # 100 loads of 3 columns, then summed: one slot per distinct (column,
+    # offset), filled by the prologue, plus the running sum
     p = pb.Prog()
     for r in range(100):
         p.load_trace(r, r % 3, 0)
@@ -124,7 +125,16 @@ def test_host_pass_renames_to_live_values():
     p.wide_mul(501, 500, 500)
     p.add_constraint(501)
     p.add_constraint(500)
-    assert slots(p.encode(), 3) == (100, 2, 2)  # 500 stays live until the last constraint
+    assert slots(p.encode(), 3) == (4, 2, 2)  # 500 stays live until the last constraint
+    # 100 computed values live at once
+    p = pb.Prog()
+    p.load_trace(1000, 0, 0)
+    for r in range(100):
+        p.mul_const(r, 1000, r + 1)
+    for r in range(1, 100):
+        p.add(0, 0, r)
+    p.add_constraint(0)
+    assert slots(p.encode(), 1) == (100, 0, 1)
 
 
 def test_host_pass_rejects_what_the_reference_would_misread():

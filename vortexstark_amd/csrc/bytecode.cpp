@@ -6,12 +6,16 @@
 // The reference kernel gives every row thread a 1,024-entry QM31 register
 // file in local memory. Here the program is straight-line code, so the live
 // range of every value is known before the launch: a value lives from the
-// instruction that writes it to its last read. A linear scan assigns each
+// instruction that writes it to its last read. Trace loads have no inputs,
+// so they are hoisted into a prologue (one value per distinct column and
+// offset, all loads of a row in flight together) and constants become
+// operands of the instructions that read them. A linear scan assigns each
 // value a slot of its type (M31: one word per row, QM31: four), reusing
 // slots whose value has died, so a wavefront's register file is as small as
 // the program's peak number of live values and fits in LDS.
 #include <algorithm>
 #include <array>
+#include <map>
 #include <set>
 #include <string>
 
@@ -53,9 +57,11 @@ uint8_t result_type(uint32_t op) {
 
 }  // namespace
 
-BcProgram bc_compile(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols) {
+namespace {
+
+// 1. decode and check
+std::vector<Dec> decode(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols) {
   require(w != nullptr || n_words == 0, "bytecode: null program");
-  // 1. decode
   std::vector<Dec> ops;
   uint32_t pc = 0;
   auto take = [&]() -> uint32_t {
@@ -143,14 +149,43 @@ BcProgram bc_compile(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols)
     ops.push_back(d);
   }
 
+  return ops;
+}
+
+// 2-3. values and slots. remat: trace loads take no slot; an operand of
+// kind TRACE reads the column where it is used.
+BcProgram lower(const std::vector<Dec>& ops, bool remat) {
   // 2. values: every write of a register starts a new value; find each
-  // value's type and last read
+  // value's type and last read. A trace load names the value of its
+  // (column, offset) pair, loaded in the prologue; a constant load makes the
+  // register an alias of a constant-pool entry (operand kind CONST).
   const size_t n = ops.size();
+  BcProgram p;
+  p.remat = remat;
   std::vector<int32_t> cur(1u << 16, -1);  // register -> live value
-  std::vector<uint8_t> vtype;
+  std::vector<uint8_t> vtype, vkind;       // kind: BC_K_M, BC_K_Q or BC_K_CONST
+  std::vector<uint32_t> vidx;              // constant-pool index of a constant value
   std::vector<size_t> last;
   std::vector<std::array<int32_t, 4>> src_val(n);
   std::vector<int32_t> dst_val(n, -1);
+  std::map<std::pair<uint32_t, int32_t>, int32_t> trace_val;  // (column, offset) -> value
+  std::vector<int32_t> trace_vals;                             // in table order
+  std::map<std::array<uint32_t, 4>, uint32_t> const_ix;
+  auto konst = [&](const std::array<uint32_t, 4>& k) {
+    auto it = const_ix.find(k);
+    if (it != const_ix.end()) return it->second;
+    const uint32_t ix = (uint32_t)(p.consts.size() / 4);
+    for (uint32_t x : k) p.consts.push_back(x);
+    const_ix.emplace(k, ix);
+    return ix;
+  };
+  auto new_value = [&](uint8_t type, uint8_t kind, uint32_t ix, size_t at) {
+    vtype.push_back(type);
+    vkind.push_back(kind);
+    vidx.push_back(ix);
+    last.push_back(at);
+    return (int32_t)vtype.size() - 1;
+  };
   for (size_t i = 0; i < n; i++) {
     const Dec& d = ops[i];
     for (uint32_t s = 0; s < d.n_src; s++) {
@@ -160,36 +195,66 @@ BcProgram bc_compile(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols)
       src_val[i][s] = v;
       last[v] = i;
     }
-    if (d.op != BC_ADD_CONSTRAINT) {
-      const int32_t v = (int32_t)vtype.size();
-      vtype.push_back(result_type(d.op));
-      last.push_back(i);
-      cur[d.dst] = v;
-      dst_val[i] = v;
+    if (d.op == BC_ADD_CONSTRAINT) continue;
+    int32_t v;
+    if (d.op == BC_LOAD_TRACE) {
+      auto key = std::make_pair(d.imm, d.offset);
+      auto it = trace_val.find(key);
+      if (it == trace_val.end()) {
+        const uint32_t tix = (uint32_t)(p.trace.size() / 3);
+        it = trace_val.emplace(key, new_value(0, remat ? BC_K_TRACE : BC_K_M, tix, 0)).first;
+        trace_vals.push_back(it->second);
+        p.trace.push_back(d.imm);
+        p.trace.push_back((uint32_t)d.offset);
+        p.trace.push_back(BC_NO_SLOT);  // slot, below (none when re-read at each use)
+        p.uses_offsets = p.uses_offsets || d.offset != 0;
+      }
+      v = it->second;
+    } else if (d.op == BC_LOAD_CONST) {
+      v = new_value(0, BC_K_CONST, konst({d.imm, 0u, 0u, 0u}), i);
+    } else if (d.op == BC_LOAD_SECURE_CONST) {
+      v = new_value(1, BC_K_CONST, konst({d.k[0], d.k[1], d.k[2], d.k[3]}), i);
+    } else {
+      const uint8_t t = result_type(d.op);
+      v = new_value(t, t ? BC_K_Q : BC_K_M, 0, i);
     }
+    cur[d.dst] = v;
+    dst_val[i] = v;
   }
+  if (p.trace.size() / 3 > 0xFFFFu || p.consts.size() / 4 > 0xFFFFu)
+    throw Error(ERR_ARG, "bytecode: more than 65,535 distinct trace loads or constants");
 
-  // 3. linear scan: free the slots of values read for the last time here,
-  // then give the result a free slot of its type (the device reads every
+  // 3. linear scan: the prologue's values take the first M31 slots; then,
+  // per instruction, free the slots of values read for the last time there
+  // and give the result a free slot of its type (the device reads every
   // operand before it writes the result, so the two may share a slot)
-  BcProgram p;
   std::vector<uint32_t> slot(vtype.size(), 0);
   std::set<uint32_t> free_slots[2];
   uint32_t n_slots[2] = {0, 0};
+  auto slotted = [&](int32_t v) { return vkind[v] == BC_K_M || vkind[v] == BC_K_Q; };
+  if (!remat) {
+    for (size_t t = 0; t < trace_vals.size(); t++) {
+      slot[trace_vals[t]] = n_slots[0]++;
+      p.trace[3 * t + 2] = slot[trace_vals[t]];
+    }
+  }
+  std::vector<uint8_t> dead(vtype.size(), 0);
   for (size_t i = 0; i < n; i++) {
     const Dec& d = ops[i];
+    if (d.op == BC_LOAD_TRACE || d.op == BC_LOAD_CONST || d.op == BC_LOAD_SECURE_CONST) continue;
     BcIns in{};
-    uint32_t s_slot[4] = {0, 0, 0, 0}, s_type[4] = {0, 0, 0, 0};
+    uint32_t s_ix[4] = {0, 0, 0, 0}, s_kind[4] = {0, 0, 0, 0};
     for (uint32_t s = 0; s < d.n_src; s++) {
       const int32_t v = src_val[i][s];
-      s_slot[s] = slot[v];
-      s_type[s] = vtype[v];
+      s_ix[s] = slotted(v) ? slot[v] : vidx[v];
+      s_kind[s] = vkind[v];
     }
     for (uint32_t s = 0; s < d.n_src; s++) {
       const int32_t v = src_val[i][s];
-      bool dup = false;  // the same value read twice is freed once
-      for (uint32_t t = 0; t < s; t++) dup = dup || src_val[i][t] == v;
-      if (last[v] == i && !dup) free_slots[vtype[v]].insert(slot[v]);
+      if (slotted(v) && last[v] == i && !dead[v]) {  // the same value read twice is freed once
+        dead[v] = 1;
+        free_slots[vtype[v]].insert(slot[v]);
+      }
     }
     uint32_t dslot = 0;
     if (dst_val[i] >= 0) {
@@ -204,34 +269,37 @@ BcProgram bc_compile(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols)
       if (last[dst_val[i]] == i) free_slots[t].insert(dslot);  // never read
     }
     if (n_slots[0] > 0xFFFFu || n_slots[1] > 0xFFFFu) throw Error(ERR_ARG, "bytecode: too many live values");
-    in.x = (d.op << 24) | dslot | (s_type[0] ? BC_TA : 0u) | (s_type[1] ? BC_TB : 0u) |
-           (s_type[2] ? BC_TC : 0u) | (s_type[3] ? BC_TD : 0u);
-    in.y = (s_slot[0] << 16) | s_slot[1];
+    in.x = (d.op << 24) | (s_kind[0] << BC_KA) | (s_kind[1] << BC_KB) | (s_kind[2] << BC_KC) |
+           (s_kind[3] << BC_KD) | dslot;
+    in.y = (s_ix[0] << 16) | s_ix[1];
     switch (d.op) {
-      case BC_LOAD_CONST:
       case BC_ADD_CONST:
       case BC_MUL_CONST:
-      case BC_LOAD_TRACE:
         in.z = d.imm;
         break;
       case BC_ADD_CONSTRAINT:
         in.z = p.n_constraints++;
         break;
       default:
-        if (d.has_k) {
-          in.z = (uint32_t)(p.consts.size() / 4);
-          for (uint32_t x : d.k) p.consts.push_back(x);
-        }
+        if (d.has_k) in.z = konst({d.k[0], d.k[1], d.k[2], d.k[3]});
     }
-    if (d.op == BC_LOAD_TRACE) {
-      in.w = (uint32_t)d.offset;
-      p.uses_offsets = p.uses_offsets || d.offset != 0;
-    }
-    if (d.op == BC_COMBINE_EF) in.w = (s_slot[2] << 16) | s_slot[3];
+    if (d.op == BC_COMBINE_EF) in.w = (s_ix[2] << 16) | s_ix[3];
     p.ins.push_back(in);
   }
   p.m31_slots = n_slots[0];
   p.qm31_slots = n_slots[1];
+  return p;
+}
+
+}  // namespace
+
+BcProgram bc_compile(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols) {
+  const std::vector<Dec> ops = decode(w, n_words, n_trace_cols);
+  // Trace values in slots are loaded once per row, but they are live from
+  // the prologue to their last use; when that makes the register file large
+  // (fewer wavefronts per CU to hide latency) they are re-read at each use.
+  BcProgram p = lower(ops, false);
+  if (p.m31_slots + 4 * p.qm31_slots > BC_SLOTTED_TRACE_MAX_WORDS) p = lower(ops, true);
   return p;
 }
 

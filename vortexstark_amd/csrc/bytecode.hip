@@ -7,10 +7,15 @@
 // so one row's register file is the program's peak number of live values.
 // Instructions are 16-byte records read with scalar loads: the opcode,
 // operand slots and constants are wave-uniform, the dispatch is a scalar
-// branch, and each lane only computes its row. The register file of a
-// wavefront lives in LDS when it is at most 40 words per row (10 KiB per
-// 64-row workgroup, at least 16 waves per CU); larger programs keep it in a
-// global workspace laid out slot-major ([slot][row]), so every operand
+// branch, and each lane only computes its row. When the register file stays
+// small, a prologue loads every distinct (column, offset) the program reads
+// into slots, eight loads in flight at a time; otherwise trace values are
+// read from the columns at each use (L1/L2 hits after the first), which
+// keeps the register file, and so the wavefronts per CU, what the computed
+// values need. Constants are operands.
+// A wavefront's register file lives in LDS when it is at most 128 words per
+// row (32 KiB per 64-row workgroup, 5 waves per CU); larger programs keep it
+// in a global workspace laid out slot-major ([slot][row]), so every operand
 // access of a wavefront is one contiguous 256-byte (QM31: 1 KiB) run.
 #include "vx_bytecode.h"
 #include "vx_internal.h"
@@ -18,7 +23,7 @@
 namespace vx {
 
 static constexpr uint32_t BC_LDS_ROWS = 64;       // one wavefront per workgroup
-static constexpr uint32_t BC_LDS_MAX_WORDS = 40;  // words per row kept in LDS
+static constexpr uint32_t BC_LDS_MAX_WORDS = 128;  // words per row kept in LDS
 static constexpr uint32_t BC_WS_ROWS = 256;       // workgroup of the workspace kernel
 static constexpr uint64_t BC_WS_BYTES = 1ull << 30;  // workspace per launch
 
@@ -26,6 +31,9 @@ struct BcArgs {
   const BcIns* ins;
   uint32_t n_ins;
   const uint32_t* consts;  // QM31 constants, 4 words each
+  const uint32_t* trace;   // (column, step mod half domain, slot) of each distinct trace load
+  uint32_t n_trace;
+  uint32_t remat;          // trace values have no slots: read at each use
   const uint32_t* const* cols;
   const uint32_t* coeff;   // QM31 per constraint
   const uint32_t* denom_inv;
@@ -74,39 +82,63 @@ __global__ __launch_bounds__(LDS ? BC_LDS_ROWS : BC_WS_ROWS) void k_bc_eval(BcAr
       g[3] = v.w;
     }
   };
-  // operand as M31 (component 0) or as QM31 ((v, 0, 0, 0) for an M31 slot)
-  auto get1 = [&](bool q, uint32_t s) -> uint32_t { return q ? ld_q(s).v[0] : ld_m(s); };
-  auto getq = [&](bool q, uint32_t s) -> qm31 {
-    if (q) return ld_q(s);
-    return {{ld_m(s), 0u, 0u, 0u}};
-  };
   auto konst = [&](uint32_t k) -> qm31 {
     const uint4 c = ld_uniform(reinterpret_cast<const uint4*>(a.consts) + k);
     return {{c.x, c.y, c.z, c.w}};
   };
 
+  // A row offset moves the natural index by a step s (in the first half of
+  // the domain, back by s in the conjugate half) modulo the half size; the
+  // host stores s mod half, so a move is an add and a mask.
+  const uint32_t hmask = a.eval_log ? (1u << (a.eval_log - 1)) - 1u : 0u;
+  const uint32_t nat = a.eval_log ? __builtin_bitreverse32(row) >> (32u - a.eval_log) : 0u;
+  const uint32_t second = nat & ~hmask;  // the conjugate half
+  auto trace_val = [&](uint32_t t) -> uint32_t {
+    const uint32_t* e = a.trace + 3 * t;
+    const uint32_t cix = ld_uniform(e), step = ld_uniform(e + 1);
+    const uint32_t* col = ld_uniform(a.cols + cix);
+    if (col == nullptr) return 0u;  // a null column reads as zero
+    uint32_t rr = row;
+    if (step) {
+      const uint32_t moved = second ? ((nat - step) & hmask) | second : (nat + step) & hmask;
+      rr = __builtin_bitreverse32(moved) >> (32u - a.eval_log);
+    }
+    return as_global(col)[rr];
+  };
+  // Prologue: trace values that have slots, eight loads in flight at a time.
+  if (!a.remat) {
+    for (uint32_t t0 = 0; t0 < a.n_trace; t0 += 8) {
+      uint32_t v[8];
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++) v[j] = t0 + j < a.n_trace ? trace_val(t0 + j) : 0u;
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++)
+        if (t0 + j < a.n_trace) st_m(ld_uniform(a.trace + 3 * (t0 + j) + 2), v[j]);
+    }
+  }
+
+  // operand of kind k as M31 (component 0) or as QM31 ((v, 0, 0, 0) for an
+  // M31 source); the kind is wave-uniform
+  auto get1 = [&](uint32_t k, uint32_t ix) -> uint32_t {
+    if (k == BC_K_M) return ld_m(ix);
+    if (k == BC_K_Q) return ld_q(ix).v[0];
+    if (k == BC_K_TRACE) return trace_val(ix);
+    return konst(ix).v[0];
+  };
+  auto getq = [&](uint32_t k, uint32_t ix) -> qm31 {
+    if (k == BC_K_Q) return ld_q(ix);
+    if (k == BC_K_CONST) return konst(ix);
+    return {{k == BC_K_M ? ld_m(ix) : trace_val(ix), 0u, 0u, 0u}};
+  };
+
   qm31 acc = {{0u, 0u, 0u, 0u}};
+  BcIns nxt = a.n_ins ? ld_uniform(a.ins) : BcIns{0u, 0u, 0u, 0u};
   for (uint32_t pc = 0; pc < a.n_ins; pc++) {
-    const BcIns I = ld_uniform(a.ins + pc);
+    const BcIns I = nxt;  // the next record is in flight while this one runs
+    if (pc + 1 < a.n_ins) nxt = ld_uniform(a.ins + pc + 1);
     const uint32_t op = I.x >> 24, dst = I.x & 0xFFFFu, sa = I.y >> 16, sb = I.y & 0xFFFFu;
-    const bool qa = (I.x & BC_TA) != 0, qb = (I.x & BC_TB) != 0;
-    switch (op) {
-      case BC_LOAD_CONST:
-        st_m(dst, I.z);
-        break;
-      case BC_LOAD_SECURE_CONST:
-        st_q(dst, konst(I.z));
-        break;
-      case BC_LOAD_TRACE: {
-        const uint32_t* col = ld_uniform(a.cols + I.z);
-        uint32_t v = 0;
-        if (col != nullptr) {
-          const uint32_t rr = I.w ? bc_offset_row(row, a.trace_log, a.eval_log, (int32_t)I.w) : row;
-          v = as_global(col)[rr];
-        }
-        st_m(dst, v);
-        break;
-      }
+    const uint32_t qa = (I.x >> BC_KA) & 3u, qb = (I.x >> BC_KB) & 3u;
+    switch (op) {  // trace loads and constants have no instruction of their own
       case BC_ADD:
         st_m(dst, m31_add(get1(qa, sa), get1(qb, sb)));
         break;
@@ -162,8 +194,8 @@ __global__ __launch_bounds__(LDS ? BC_LDS_ROWS : BC_WS_ROWS) void k_bc_eval(BcAr
         st_q(dst, qm31{{get1(qa, sa), 0u, 0u, 0u}});
         break;
       case BC_COMBINE_EF:
-        st_q(dst, qm31{{get1(qa, sa), get1(qb, sb), get1((I.x & BC_TC) != 0, I.w >> 16),
-                        get1((I.x & BC_TD) != 0, I.w & 0xFFFFu)}});
+        st_q(dst, qm31{{get1(qa, sa), get1(qb, sb), get1((I.x >> BC_KC) & 3u, I.w >> 16),
+                        get1((I.x >> BC_KD) & 3u, I.w & 0xFFFFu)}});
         break;
       case BC_ADD_CONSTRAINT: {
         const uint4 c = ld_uniform(reinterpret_cast<const uint4*>(a.coeff) + I.z);
@@ -182,13 +214,16 @@ __global__ __launch_bounds__(LDS ? BC_LDS_ROWS : BC_WS_ROWS) void k_bc_eval(BcAr
 }
 
 void bytecode_constraint_eval(const BcProgram& p, const uint32_t* d_ins, const uint32_t* d_consts,
-                              const uint32_t* const* d_cols, uint32_t n_rows, uint32_t trace_n_rows,
+                              const uint32_t* d_trace, const uint32_t* const* d_cols, uint32_t n_rows, uint32_t trace_n_rows,
                               const uint32_t* coeff, const uint32_t* denom_inv, uint32_t* const acc[4],
                               uint32_t* d_ws, hipStream_t s) {
   BcArgs a{};
   a.ins = reinterpret_cast<const BcIns*>(d_ins);
   a.n_ins = (uint32_t)p.ins.size();
   a.consts = d_consts;
+  a.trace = d_trace;
+  a.n_trace = (uint32_t)(p.trace.size() / 3);
+  a.remat = p.remat ? 1u : 0u;
   a.cols = d_cols;
   a.coeff = coeff;
   a.denom_inv = denom_inv;

@@ -1528,13 +1528,27 @@ static int bytecode_call(const uint32_t* d_bytecode, uint32_t n_words, const uin
             "bytecode eval: row offsets need an evaluation domain larger than the trace domain");
     require(p.n_constraints == 0 || d_coeff != nullptr, "bytecode eval: null constraint coefficients");
     DeviceBuf ins(16 * (p.ins.size() ? p.ins.size() : 1)), kon(4 * (p.consts.size() ? p.consts.size() : 4));
+    DeviceBuf tr(4 * (p.trace.size() ? p.trace.size() : 3));
     DeviceBuf ws(bytecode_ws_bytes(p, n_rows));
     if (!p.ins.empty()) VX_HIP(hipMemcpyAsync(ins.p, p.ins.data(), 16 * p.ins.size(), hipMemcpyHostToDevice, s));
     if (!p.consts.empty())
       VX_HIP(hipMemcpyAsync(kon.p, p.consts.data(), 4 * p.consts.size(), hipMemcpyHostToDevice, s));
+    // the device moves natural indices by offset * 2^(eval_log - trace_log - 1)
+    // modulo the half domain (stwo's offset_bit_reversed_circle_domain_index)
+    std::vector<uint32_t> tr_h(p.trace);
+    const uint32_t e = log2u(n_rows), d = log2u(trace_n_rows);
+    for (size_t i = 1; i < tr_h.size(); i += 3) {
+      const int32_t off = (int32_t)tr_h[i];
+      if (off == 0) continue;
+      const int64_t half = (int64_t)1 << (e - 1);
+      int64_t st = ((int64_t)off * ((int64_t)1 << (e - d - 1))) % half;
+      if (st < 0) st += half;
+      tr_h[i] = (uint32_t)st;
+    }
+    if (!tr_h.empty()) VX_HIP(hipMemcpyAsync(tr.p, tr_h.data(), 4 * tr_h.size(), hipMemcpyHostToDevice, s));
     uint32_t* acc[4] = {a0, a1, a2, a3};
-    bytecode_constraint_eval(p, ins.u32(), kon.u32(), d_trace_cols, n_rows, trace_n_rows, d_coeff, d_denom_inv,
-                             acc, ws.p ? ws.u32() : nullptr, s);
+    bytecode_constraint_eval(p, ins.u32(), kon.u32(), tr.u32(), d_trace_cols, n_rows, trace_n_rows, d_coeff,
+                             d_denom_inv, acc, ws.p ? ws.u32() : nullptr, s);
     VX_HIP(hipStreamSynchronize(s));
   });
 }

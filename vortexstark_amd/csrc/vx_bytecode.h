@@ -37,24 +37,37 @@ enum BcOp : uint32_t {
   BC_ADD_CONSTRAINT = 0x40,
 };
 
-// Compact instruction (16 bytes, one scalar load on the device). Registers
-// are renamed to typed slots: an M31 slot holds one word per row, a QM31
-// slot four. Operand type bits say how to read a source (an M31 slot read as
-// QM31 is (v, 0, 0, 0); a QM31 slot read by an M31 operation gives v[0]).
-//   x: op (bits 24..31) | a is QM31 (bit 16) | b is QM31 (bit 17) | c, d QM31 (18, 19) | dst slot (bits 0..15)
-//   y: a slot (bits 16..31) | b slot (bits 0..15)
-//   z: M31 constant, constant-pool index, flat column or constraint index
-//   w: row offset (LOAD_TRACE, signed) or c slot << 16 | d slot (COMBINE_EF)
+// Compact instruction (16 bytes, one scalar load on the device). Values are
+// renamed to typed slots (an M31 slot holds one word per row, a QM31 slot
+// four). Trace loads take no instruction: BcProgram::trace lists each
+// distinct (column, offset) once with its M31 slot, filled by a prologue, or
+// (remat) with no slot: then an operand of kind TRACE names the entry and
+// reads the column where it is used.
+// Constants take no slot: an operand of kind CONST is a constant-pool entry
+// (M31 constants as (v, 0, 0, 0)). An operand read as QM31 from an M31
+// source is (v, 0, 0, 0); read as M31 from a QM31 source it is v[0].
+//   x: op (bits 24..31) | kinds of operands a, b, c, d (2 bits each from
+//      bit 16) | dst slot (bits 0..15)
+//   y: a (bits 16..31) | b (bits 0..15)   slot or constant index
+//   z: M31 constant (ADD_CONST, MUL_CONST), constant index (QM31 constant
+//      operations) or constraint index
+//   w: c << 16 | d (COMBINE_EF)
 struct BcIns {
   uint32_t x, y, z, w;
 };
-constexpr uint32_t BC_TA = 1u << 16, BC_TB = 1u << 17, BC_TC = 1u << 18, BC_TD = 1u << 19;
+enum : uint32_t { BC_K_M = 0, BC_K_Q = 1, BC_K_TRACE = 2, BC_K_CONST = 3 };
+constexpr uint32_t BC_NO_SLOT = 0xFFFFFFFFu;
+// Largest register file (words per row) for which trace values take slots.
+constexpr uint32_t BC_SLOTTED_TRACE_MAX_WORDS = 32;
+constexpr uint32_t BC_KA = 16, BC_KB = 18, BC_KC = 20, BC_KD = 22;
 
 struct BcProgram {
   std::vector<BcIns> ins;
   std::vector<uint32_t> consts;  // QM31 constants, 4 words each
+  std::vector<uint32_t> trace;   // (column, offset, M31 slot) of each distinct trace load
   uint32_t m31_slots = 0, qm31_slots = 0, n_constraints = 0;
   bool uses_offsets = false;
+  bool remat = false;
 };
 
 // Decodes and checks the reference bytecode (n_words words) for columns

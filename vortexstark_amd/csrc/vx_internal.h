@@ -341,11 +341,12 @@ void axpy_m31(uint32_t sc, const uint32_t* x, uint32_t* y, uint64_t n, hipStream
 
 // ------------------------------------------- bytecode constraint eval (bytecode.hip)
 // Evaluates a program renamed by bc_compile (d_ins: its 16-byte records,
-// d_consts: its QM31 constants) over n_rows rows; d_ws holds
-// bytecode_ws_bytes(p, n_rows) bytes (null when that is 0).
+// d_consts: its QM31 constants, d_trace: its (column, offset) pairs) over
+// n_rows rows; d_ws holds bytecode_ws_bytes(p, n_rows) bytes (null when
+// that is 0).
 struct BcProgram;
 void bytecode_constraint_eval(const BcProgram& p, const uint32_t* d_ins, const uint32_t* d_consts,
-                              const uint32_t* const* d_cols, uint32_t n_rows, uint32_t trace_n_rows,
+                              const uint32_t* d_trace, const uint32_t* const* d_cols, uint32_t n_rows, uint32_t trace_n_rows,
                               const uint32_t* coeff, const uint32_t* denom_inv, uint32_t* const acc[4],
                               uint32_t* d_ws, hipStream_t s);
 uint32_t bytecode_row_words(const BcProgram& p);
