@@ -705,6 +705,19 @@ int vx_warp_bytecode_constraint_eval(const uint32_t *d_bytecode, uint32_t n_word
                                      const uint32_t *d_random_coeff_powers, const uint32_t *d_denom_inv,
                                      uint32_t log_expand, uint32_t *accum0, uint32_t *accum1, uint32_t *accum2,
                                      uint32_t *accum3, uint32_t n_registers, vx_stream_t stream);
+/* No reference counterpart: by default a program is compiled for gfx950 on
+ * first use (hipRTC; straight-line code, values in VGPRs; cached per device,
+ * program and domain shape) and programs of more than 50,000 instructions
+ * are interpreted; on = 0 interprets every program (same results). */
+int vx_bytecode_set_jit(int on);
+/* Host only, no reference counterpart: the HIP source the compiled tier
+ * generates for a program (host pointer) over 2^eval_log rows and a trace
+ * domain of 2^trace_log (out may be NULL to query *out_len), and a compile
+ * of it without loading (*code_bytes = the gfx950 code object's size). */
+int vx_bytecode_jit_source(const uint32_t *h_bytecode, uint32_t n_words, uint32_t n_trace_cols, uint32_t eval_log,
+                           uint32_t trace_log, char *out, size_t cap, size_t *out_len);
+int vx_bytecode_jit_compile(const uint32_t *h_bytecode, uint32_t n_words, uint32_t n_trace_cols, uint32_t eval_log,
+                            uint32_t trace_log, size_t *code_bytes);
 /* Host only, no reference counterpart: checks a program (host pointer) as
  * the evaluation does and reports its register file after renaming (M31
  * slots of one word per row, QM31 slots of four) and its constraint count. */

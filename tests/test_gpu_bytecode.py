@@ -1,5 +1,6 @@
 """The bytecode constraint evaluator on the GPU (vx_bytecode_constraint_eval,
-vx_warp_bytecode_constraint_eval through the C-ABI) against the oracle
+vx_warp_bytecode_constraint_eval through the C-ABI), compiled and
+interpreted, against the oracle
 (or_bytecode_constraint_eval, pinned by tests/test_oracle_bytecode.py): the
 reference's three kernel tests, random programs over every opcode at
 4..2^16 rows with row offsets, programs whose register file fits LDS and
@@ -40,10 +41,16 @@ def gpu_eval(words, cols, n_rows, trace_n, coeff, denom, acc=None, entry="vx_byt
 
 
 def check(prog_words, cols, n_rows, trace_n, coeff, denom, acc=None, entry="vx_bytecode_constraint_eval"):
-    got = gpu_eval(prog_words, cols, n_rows, trace_n, coeff, denom, acc, entry)
+    """both tiers (compiled and interpreted) against the oracle"""
     want = po.bytecode_eval(prog_words, cols, n_rows, trace_n, np.asarray(coeff, np.uint32).ravel(), denom, acc)
-    for c in range(4):
-        assert np.array_equal(got[c], want[c]), c
+    try:
+        for jit in (1, 0):
+            call("vx_bytecode_set_jit", jit)
+            got = gpu_eval(prog_words, cols, n_rows, trace_n, coeff, denom, acc, entry)
+            for c in range(4):
+                assert np.array_equal(got[c], want[c]), (jit, c)
+    finally:
+        call("vx_bytecode_set_jit", 1)
 
 
 def test_reference_vectors():

@@ -156,3 +156,18 @@ def test_host_pass_rejects_what_the_reference_would_misread():
     p.mul(1, 1, 0)
     p.add_constraint(1)
     assert slots(p.encode(), 1)[2] == 1
+
+
+def test_compiled_tier_sources_compile():
+    # the straight-line source of random programs (every opcode, offsets,
+    # redefinitions) compiles for gfx950 with hipRTC (no device needed)
+    for seed in range(3):
+        rng = random.Random(100 + seed)
+        prog = pb.random_program(rng, 200, 6, n_regs=20, offsets=(0, 1, -1, 2))
+        w = np.array(prog.encode(), np.uint32)
+        n = C.c_size_t()
+        _lib.call("vx_bytecode_jit_compile", w.ctypes.data, w.size, 6, 12, 10, C.byref(n))
+        assert n.value > 0
+    with pytest.raises(VortexError):  # checked like an evaluation
+        w = np.array([(0x10 << 24) | 2, (0 << 16) | 1], np.uint32)
+        _lib.call("vx_bytecode_jit_compile", w.ctypes.data, w.size, 1, 4, 4, C.byref(n))

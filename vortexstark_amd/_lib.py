@@ -78,6 +78,9 @@ SIGNATURES = {
     "vx_warp_bytecode_constraint_eval": (ip, [vp, u32, vp, vp, u32, u32, u32, vp, vp, u32, vp, vp, vp, vp, u32,
                                               vp]),
     "vx_bytecode_slots": (ip, [vp, u32, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+    "vx_bytecode_set_jit": (ip, [ip]),
+    "vx_bytecode_jit_source": (ip, [vp, u32, u32, u32, u32, vp, sz, C.POINTER(sz)]),
+    "vx_bytecode_jit_compile": (ip, [vp, u32, u32, u32, u32, C.POINTER(sz)]),
     "vx_export_stwo_json": (ip, [C.c_char_p, sz, u32, vp, sz, C.POINTER(sz)]),
     "vx_stwo_hash_witness": (ip, [vp, sz, vp, u32, vp, sz, C.POINTER(sz)]),
     "vx_stwo_fri_witness": (ip, [vp, sz, vp, vp, sz, C.POINTER(sz)]),

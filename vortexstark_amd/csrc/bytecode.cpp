@@ -304,3 +304,124 @@ BcProgram bc_compile(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols)
 }
 
 }  // namespace vx
+
+namespace vx {
+
+// Straight-line HIP source of a program for one row per lane: every value is
+// a local variable (the compiler allocates VGPRs), trace loads are read once
+// at the top, constants and the row-offset steps are literals. Returns the
+// kernel source; *n_ops = the number of instructions.
+std::string bc_jit_source(const uint32_t* w, uint32_t n_words, uint32_t n_trace_cols, uint32_t eval_log,
+                          uint32_t trace_log, size_t* n_ops) {
+  const std::vector<Dec> ops = decode(w, n_words, n_trace_cols);
+  if (n_ops) *n_ops = ops.size();
+  std::string s;
+  s.reserve(64 * ops.size() + 1024);
+  s += "typedef unsigned int uint32_t;\ntypedef unsigned long long uint64_t;\ntypedef int int32_t;\n"
+       "typedef long long int64_t;\n#include \"vx_field.h\"\nusing namespace vx;\n"
+       "__device__ __forceinline__ qm31 add_base(qm31 x, uint32_t b) {\n"
+       "  x.v[0] = m31_add(x.v[0], b);\n  return x;\n}\n"
+       "extern \"C\" __global__ __launch_bounds__(256) void bc_jit(const uint32_t* const* __restrict__ cols, "
+       "const uint32_t* __restrict__ coeff, const uint32_t* __restrict__ dinv, uint32_t* a0, uint32_t* a1, "
+       "uint32_t* a2, uint32_t* a3, uint32_t n_rows) {\n"
+       "  const uint32_t row = blockIdx.x * 256u + threadIdx.x;\n  if (row >= n_rows) return;\n"
+       "  qm31 acc = {{0u, 0u, 0u, 0u}};\n";
+  const uint32_t hmask = eval_log ? (1u << (eval_log - 1)) - 1u : 0u;
+  bool offsets = false;
+  for (const Dec& d : ops) offsets = offsets || (d.op == BC_LOAD_TRACE && d.offset != 0);
+  if (offsets) {
+    s += "  const uint32_t nat = __builtin_bitreverse32(row) >> " + std::to_string(32 - eval_log) + "u;\n";
+    s += "  const uint32_t second = nat & ~" + std::to_string(hmask) + "u;\n";
+  }
+  auto qlit = [](const uint32_t* k) {
+    return "qm31{{" + std::to_string(k[0]) + "u, " + std::to_string(k[1]) + "u, " + std::to_string(k[2]) + "u, " +
+           std::to_string(k[3]) + "u}}";
+  };
+  // register -> (value name, is QM31)
+  std::vector<std::pair<std::string, bool>> reg(1u << 16);
+  std::vector<uint8_t> defined(1u << 16, 0);
+  std::map<std::pair<uint32_t, int32_t>, std::string> trace;
+  std::map<uint32_t, std::string> colp;
+  uint32_t nv = 0, k = 0;
+  auto m = [&](uint32_t r) {  // operand as M31
+    if (!defined[r]) throw Error(ERR_ARG, "bytecode: register " + std::to_string(r) + " read before it is written");
+    return reg[r].second ? reg[r].first + ".v[0]" : reg[r].first;
+  };
+  auto q = [&](uint32_t r) {  // operand as QM31
+    if (!defined[r]) throw Error(ERR_ARG, "bytecode: register " + std::to_string(r) + " read before it is written");
+    return reg[r].second ? reg[r].first : "qm31{{" + reg[r].first + ", 0u, 0u, 0u}}";
+  };
+  for (const Dec& d : ops) {
+    std::string name = "v" + std::to_string(nv++), expr;
+    bool wide = false;
+    switch (d.op) {
+      case BC_LOAD_CONST: expr = std::to_string(d.imm) + "u"; break;
+      case BC_LOAD_SECURE_CONST: expr = qlit(d.k); wide = true; break;
+      case BC_LOAD_TRACE: {
+        auto key = std::make_pair(d.imm, d.offset);
+        auto it = trace.find(key);
+        if (it == trace.end()) {
+          if (!colp.count(d.imm)) {
+            colp[d.imm] = "c" + std::to_string(d.imm);
+            s += "  const uint32_t* " + colp[d.imm] + " = cols[" + std::to_string(d.imm) + "];\n";
+          }
+          std::string rr = "row";
+          if (d.offset != 0) {
+            const int64_t half = (int64_t)1 << (eval_log - 1);
+            int64_t st = ((int64_t)d.offset * ((int64_t)1 << (eval_log - trace_log - 1))) % half;
+            if (st < 0) st += half;
+            const std::string S = std::to_string((uint32_t)st) + "u", H = std::to_string(hmask) + "u";
+            rr = "(__builtin_bitreverse32(second ? ((nat - " + S + ") & " + H + ") | second : (nat + " + S +
+                 ") & " + H + ") >> " + std::to_string(32 - eval_log) + "u)";
+          }
+          const std::string t = "t" + std::to_string(trace.size());
+          s += "  const uint32_t " + t + " = " + colp[d.imm] + " ? " + colp[d.imm] + "[" + rr + "] : 0u;\n";
+          it = trace.emplace(key, t).first;
+        }
+        reg[d.dst] = {it->second, false};
+        defined[d.dst] = 1;
+        continue;
+      }
+      case BC_ADD: expr = "m31_add(" + m(d.src[0]) + ", " + m(d.src[1]) + ")"; break;
+      case BC_SUB: expr = "m31_sub(" + m(d.src[0]) + ", " + m(d.src[1]) + ")"; break;
+      case BC_MUL: expr = "m31_mul(" + m(d.src[0]) + ", " + m(d.src[1]) + ")"; break;
+      case BC_NEG: expr = "m31_neg(" + m(d.src[0]) + ")"; break;
+      case BC_ADD_CONST: expr = "m31_add(" + m(d.src[0]) + ", " + std::to_string(d.imm) + "u)"; break;
+      case BC_MUL_CONST: expr = "m31_mul(" + m(d.src[0]) + ", " + std::to_string(d.imm) + "u)"; break;
+      case BC_WIDE_ADD: expr = "qm_add(" + q(d.src[0]) + ", " + q(d.src[1]) + ")"; wide = true; break;
+      case BC_WIDE_SUB: expr = "qm_sub(" + q(d.src[0]) + ", " + q(d.src[1]) + ")"; wide = true; break;
+      case BC_WIDE_MUL: expr = "qm_mul(" + q(d.src[0]) + ", " + q(d.src[1]) + ")"; wide = true; break;
+      case BC_WIDE_NEG: expr = "qm_sub(qm31{{0u, 0u, 0u, 0u}}, " + q(d.src[0]) + ")"; wide = true; break;
+      case BC_WIDE_ADD_CONST: expr = "qm_add(" + q(d.src[0]) + ", " + qlit(d.k) + ")"; wide = true; break;
+      case BC_WIDE_MUL_CONST: expr = "qm_mul(" + q(d.src[0]) + ", " + qlit(d.k) + ")"; wide = true; break;
+      case BC_WIDE_ADD_BASE: expr = "add_base(" + q(d.src[0]) + ", " + m(d.src[1]) + ")"; wide = true; break;
+      case BC_WIDE_MUL_BASE: expr = "qm_scale(" + q(d.src[0]) + ", " + m(d.src[1]) + ")"; wide = true; break;
+      case BC_BASE_ADD_SECURE_CONST:
+        expr = "qm_add(qm31{{" + m(d.src[0]) + ", 0u, 0u, 0u}}, " + qlit(d.k) + ")";
+        wide = true;
+        break;
+      case BC_BASE_MUL_SECURE_CONST: expr = "qm_scale(" + qlit(d.k) + ", " + m(d.src[0]) + ")"; wide = true; break;
+      case BC_WIDEN: expr = "qm31{{" + m(d.src[0]) + ", 0u, 0u, 0u}}"; wide = true; break;
+      case BC_COMBINE_EF:
+        expr = "qm31{{" + m(d.src[0]) + ", " + m(d.src[1]) + ", " + m(d.src[2]) + ", " + m(d.src[3]) + "}}";
+        wide = true;
+        break;
+      case BC_ADD_CONSTRAINT: {
+        const std::string kk = std::to_string(4 * k++);
+        s += "  acc = qm_add(acc, qm_mul(" + q(d.src[0]) + ", qm31{{coeff[" + kk + "], coeff[" + kk + " + 1], coeff[" +
+             kk + " + 2], coeff[" + kk + " + 3]}}));\n";
+        continue;
+      }
+      default: break;
+    }
+    s += std::string("  const ") + (wide ? "qm31 " : "uint32_t ") + name + " = " + expr + ";\n";
+    reg[d.dst] = {name, wide};
+    defined[d.dst] = 1;
+  }
+  s += "  const qm31 res = qm_scale(acc, dinv[row >> " + std::to_string(trace_log) + "u]);\n"
+       "  a0[row] = m31_add(a0[row], res.v[0]);\n  a1[row] = m31_add(a1[row], res.v[1]);\n"
+       "  a2[row] = m31_add(a2[row], res.v[2]);\n  a3[row] = m31_add(a3[row], res.v[3]);\n}\n";
+  return s;
+}
+
+}  // namespace vx

@@ -1504,6 +1504,7 @@ int vx_axpy_m31(uint32_t scalar, const uint32_t* d_x, uint32_t* d_y, uint32_t n,
 }
 
 // ------------------------------------------------- bytecode constraint eval
+static constexpr size_t BC_JIT_MAX_INS = 50000;  // longer programs are interpreted
 // The program is read back to the host (it is small), checked and renamed
 // to slots, then evaluated; the call returns once the accumulators are
 // written (its temporary device buffers are released at return).
@@ -1527,6 +1528,15 @@ static int bytecode_call(const uint32_t* d_bytecode, uint32_t n_words, const uin
     require(!p.uses_offsets || n_rows > trace_n_rows,
             "bytecode eval: row offsets need an evaluation domain larger than the trace domain");
     require(p.n_constraints == 0 || d_coeff != nullptr, "bytecode eval: null constraint coefficients");
+    uint32_t* acc[4] = {a0, a1, a2, a3};
+    // compiled tier unless switched off or the program is too long to
+    // compile in reasonable time; the interpreter takes the rest
+    if (bytecode_jit_enabled() && p.ins.size() <= BC_JIT_MAX_INS) {
+      bytecode_jit_eval(words.data(), n_words, d_trace_cols, n_trace_cols, n_rows, trace_n_rows, d_coeff, d_denom_inv,
+                        acc, s);
+      VX_HIP(hipStreamSynchronize(s));
+      return;
+    }
     DeviceBuf ins(16 * (p.ins.size() ? p.ins.size() : 1)), kon(4 * (p.consts.size() ? p.consts.size() : 4));
     DeviceBuf tr(4 * (p.trace.size() ? p.trace.size() : 3));
     DeviceBuf ws(bytecode_ws_bytes(p, n_rows));
@@ -1546,7 +1556,6 @@ static int bytecode_call(const uint32_t* d_bytecode, uint32_t n_words, const uin
       tr_h[i] = (uint32_t)st;
     }
     if (!tr_h.empty()) VX_HIP(hipMemcpyAsync(tr.p, tr_h.data(), 4 * tr_h.size(), hipMemcpyHostToDevice, s));
-    uint32_t* acc[4] = {a0, a1, a2, a3};
     bytecode_constraint_eval(p, ins.u32(), kon.u32(), tr.u32(), d_trace_cols, n_rows, trace_n_rows, d_coeff,
                              d_denom_inv, acc, ws.p ? ws.u32() : nullptr, s);
     VX_HIP(hipStreamSynchronize(s));
@@ -1575,6 +1584,36 @@ int vx_warp_bytecode_constraint_eval(const uint32_t* d_bytecode, uint32_t n_word
   return vx_bytecode_constraint_eval(d_bytecode, n_words, d_trace_cols, d_trace_col_sizes, n_trace_cols, n_rows,
                                      trace_n_rows, d_random_coeff_powers, d_denom_inv, log_expand, accum0, accum1,
                                      accum2, accum3, n_registers, stream);
+}
+
+int vx_bytecode_jit_source(const uint32_t* h_bytecode, uint32_t n_words, uint32_t n_trace_cols, uint32_t eval_log,
+                           uint32_t trace_log, char* out, size_t cap, size_t* out_len) {
+  return guard([&] {
+    require(eval_log >= 1 && eval_log <= 31 && trace_log <= eval_log, "bytecode JIT: bad domain sizes");
+    (void)bc_compile(h_bytecode, n_words, n_trace_cols);  // the same checks as an evaluation
+    size_t n_ops = 0;
+    const std::string src = bc_jit_source(h_bytecode, n_words, n_trace_cols, eval_log, trace_log, &n_ops);
+    if (out_len) *out_len = src.size();
+    if (out) {
+      require(cap > src.size(), "bytecode JIT: output buffer too small");
+      std::memcpy(out, src.data(), src.size());
+      out[src.size()] = 0;
+    }
+  });
+}
+
+int vx_bytecode_jit_compile(const uint32_t* h_bytecode, uint32_t n_words, uint32_t n_trace_cols, uint32_t eval_log,
+                            uint32_t trace_log, size_t* code_bytes) {
+  return guard([&] {
+    require(eval_log >= 1 && eval_log <= 31 && trace_log <= eval_log, "bytecode JIT: bad domain sizes");
+    (void)bc_compile(h_bytecode, n_words, n_trace_cols);
+    const size_t n = bytecode_jit_code_bytes(h_bytecode, n_words, n_trace_cols, eval_log, trace_log);
+    if (code_bytes) *code_bytes = n;
+  });
+}
+
+int vx_bytecode_set_jit(int on) {
+  return guard([&] { bytecode_jit_enable(on != 0); });
 }
 
 int vx_bytecode_slots(const uint32_t* h_bytecode, uint32_t n_words, uint32_t n_trace_cols, uint32_t* m31_slots,

@@ -5,6 +5,7 @@
 // the kernel and the tests.
 #pragma once
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "vx_field.h"
@@ -77,6 +78,12 @@ struct BcProgram {
 // programs, unknown opcodes, a read of a register never written, or a
 // column index out of range.
 BcProgram bc_compile(const uint32_t* words, uint32_t n_words, uint32_t n_trace_cols);
+
+// Straight-line HIP source of the program (kernel "bc_jit": one row per
+// lane, 256 lanes per workgroup) for an evaluation of 2^eval_log rows over a
+// trace domain of 2^trace_log; *n_ops = the program's instruction count.
+std::string bc_jit_source(const uint32_t* words, uint32_t n_words, uint32_t n_trace_cols, uint32_t eval_log,
+                          uint32_t trace_log, size_t* n_ops);
 
 // stwo's offset_bit_reversed_circle_domain_index (cuda/constraint_eval.cu:
 // 63-94): the bit-reversed row `offset` trace-domain steps from row i of an

@@ -350,6 +350,16 @@ void bytecode_constraint_eval(const BcProgram& p, const uint32_t* d_ins, const u
                               const uint32_t* coeff, const uint32_t* denom_inv, uint32_t* const acc[4],
                               uint32_t* d_ws, hipStream_t s);
 uint32_t bytecode_row_words(const BcProgram& p);
+// The compiled tier (bytecode_jit.hip): the program (host words, checked by
+// bc_compile) compiled once per device and domain shape, then launched.
+void bytecode_jit_eval(const uint32_t* words, uint32_t n_words, const uint32_t* const* d_cols, uint32_t n_trace_cols,
+                       uint32_t n_rows, uint32_t trace_n_rows, const uint32_t* coeff, const uint32_t* denom_inv,
+                       uint32_t* const acc[4], hipStream_t s);
+void bytecode_jit_enable(bool on);
+// Generates and compiles (no device needed) and returns the code object's size.
+size_t bytecode_jit_code_bytes(const uint32_t* words, uint32_t n_words, uint32_t n_trace_cols, uint32_t eval_log,
+                               uint32_t trace_log);
+bool bytecode_jit_enabled();
 uint32_t bytecode_ws_rows(const BcProgram& p, uint32_t n_rows);
 uint64_t bytecode_ws_bytes(const BcProgram& p, uint32_t n_rows);
 // Root of a lifted tree with the column table and the stored tree levels in
