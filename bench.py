@@ -1061,6 +1061,8 @@ def main():
         "verified": ok,
         "verify_ms": round(verify_ms, 2),
         "device_bytes": prover.device_bytes,
+        "ntt_plans": [{k: e[k] for k in ("forward", "log_n", "bits", "n_cols", "split", "order")}
+                      for e in vx.tuner_ntt_entries()],
     }
     if not ok:
         out["verify_error"] = why

@@ -834,6 +834,18 @@ int vx_tuner_configure(int enabled, const char *cache_file);
  * n_cons, lifted columns, log_n, stored-from level, tile log, leaves-per-lane
  * log; *n = the number of entries (may exceed capacity). */
 int vx_tuner_entries(uint32_t *out, uint32_t capacity, uint32_t *n);
+/* The circle NTT's tuned plans (no counterpart in the reference, whose kernel
+ * choices are compile-time features, Cargo.toml:30-52): the first transform
+ * of each shape (direction, log_n, transformed bits, columns, basis) on a
+ * device is timed with the built-in plan, with tiles in launch order instead
+ * of XCD-contiguous ranges, and with the strided bits over one more pass;
+ * the fastest is cached (the built-in one unless another is more than 3 %
+ * faster) and every plan computes the same values. Up to `capacity` entries,
+ * 8 words each: device, forward, log_n, transformed bits, n_cols, stwo basis,
+ * split, order; *n = the number of entries. Cache-file lines
+ * "ntt fwd log_n bits n_cols stwo split order" are loaded by
+ * vx_tuner_configure. */
+int vx_tuner_ntt_entries(uint32_t *out, uint32_t capacity, uint32_t *n);
 
 #ifdef __cplusplus
 }

@@ -265,6 +265,18 @@ def tuner_entries():
     return [dict(zip(TUNER_FIELDS, buf[9 * i:9 * i + 9])) for i in range(n.value)]
 
 
+NTT_TUNER_FIELDS = ("device", "forward", "log_n", "bits", "n_cols", "stwo", "split", "order")
+
+
+def tuner_ntt_entries():
+    """The circle NTT's tuned plans, one dict per transform shape."""
+    n = C.c_uint32()
+    _lib.call("vx_tuner_ntt_entries", None, 0, C.byref(n))
+    buf = (C.c_uint32 * (8 * max(n.value, 1)))()
+    _lib.call("vx_tuner_ntt_entries", buf, n.value, C.byref(n))
+    return [dict(zip(NTT_TUNER_FIELDS, buf[8 * i:8 * i + 8])) for i in range(n.value)]
+
+
 def request_size(req):
     if req.air == AIR_D2:
         return d2_proof_size(req.log_n, req.n_cols, req.blowup_bits, req.n_queries)

@@ -94,6 +94,7 @@ SIGNATURES = {
     "vx_circle_fold_commit": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
     "vx_fold_line_commit": (ip, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
     "vx_tuner_entries": (ip, [C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "vx_tuner_ntt_entries": (ip, [C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32)]),
     "vx_d2_prover_create": (ip, [u32, u32, u32, u64, u32, u32, u32, vp, C.POINTER(vp)]),
     "vx_d2_proof_size": (sz, [u32, u32, u32, u32]),
     "vx_m31_fill_random": (ip, [vp, u32, u64, u32, vp]),

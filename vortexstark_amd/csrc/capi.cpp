@@ -1196,6 +1196,15 @@ int vx_tuner_entries(uint32_t* out, uint32_t capacity, uint32_t* n) {
   });
 }
 
+int vx_tuner_ntt_entries(uint32_t* out, uint32_t capacity, uint32_t* n) {
+  return guard([&] {
+    require(n != nullptr && (out != nullptr || capacity == 0), "tuner: null argument");
+    const std::vector<uint32_t> e = tuner_ntt_entries();
+    *n = (uint32_t)(e.size() / 8);
+    for (size_t i = 0; i < e.size() && i < 8ull * capacity; i++) out[i] = e[i];
+  });
+}
+
 // ------------------------------------------------------- batched columns
 struct vx_batch {
   BatchCommitter impl;

@@ -132,6 +132,7 @@ struct RadixArgs {
   uint64_t dst_stride, src_stride;
   uint32_t* const* cols;   // NTT_COLS: device array of column pointers (destinations)
   const uint32_t* const* src_cols;  // NTT_COLS: source columns, or null to read the destinations
+  uint32_t launch_order;   // 1: tile b runs as workgroup b (else XCD-contiguous ranges)
 };
 
 __device__ __forceinline__ uint32_t rx_own(uint32_t t, uint32_t s, uint32_t pos) {
@@ -492,13 +493,14 @@ __global__ __launch_bounds__(RX_THREADS) void k_ntt_radix(uint32_t* dst, const u
     dst += blockIdx.y * a.dst_stride;
     src += blockIdx.y * a.src_stride;
   }
+  const uint64_t tile = a.launch_order ? (uint64_t)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
   if constexpr (SHAPE == RX_GENERIC) {
-    radix_tile<FWD, MODE>(dst, src, a, xcd_tile(blockIdx.x, gridDim.x), lds);
+    radix_tile<FWD, MODE>(dst, src, a, tile, lds);
   } else if constexpr (SHAPE == RX_STRIDED) {
-    radix_tile_strided<FWD, MODE>(dst, src, a, xcd_tile(blockIdx.x, gridDim.x), lds);
+    radix_tile_strided<FWD, MODE>(dst, src, a, tile, lds);
   } else {
     const FixedArgs<FWD, SHAPE> f(a);
-    radix_tile<FWD, MODE>(dst, src, f, xcd_tile(blockIdx.x, gridDim.x), lds);
+    radix_tile<FWD, MODE>(dst, src, f, tile, lds);
   }
 }
 
@@ -591,7 +593,7 @@ static RadixArgs make_radix_args(bool fwd, const TwiddleRef& tw, uint32_t j0, ui
 
 static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_mask,
                         const TwiddleRef& tw, uint32_t j0, uint32_t k, uint32_t cbits, uint32_t scale,
-                        const Batch& bt, hipStream_t s) {
+                        const Batch& bt, hipStream_t s, uint32_t launch_order = 0) {
   const uint32_t log_n = tw.log_n;
   require(k >= 1 && k <= 16 && k + cbits <= NTT_TILE_LOG && k + cbits <= log_n && cbits <= j0 &&
               j0 + k <= log_n,
@@ -617,6 +619,7 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
     r.src_stride = bt.src_stride;
     r.cols = bt.cols;
     r.src_cols = bt.src_cols;
+    r.launch_order = launch_order;
     hipLaunchKernelGGL(radix_kernel(fwd, mode, radix_shape(r)), dim3((uint32_t)grid, bt.n_cols), dim3(RX_THREADS), 0,
                        s, dst, src, r);
     VX_HIP(hipGetLastError());
@@ -631,28 +634,37 @@ static void launch_pass(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t s
 static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 // Strided passes cover [NTT_TILE_LOG, eff_bits) from the top, then one
-// contiguous pass covers the low bits.
+// contiguous pass covers the low bits. The plan (tuned per transform shape,
+// tune.cpp) may split the strided bits over one more pass (fewer rows, wider
+// column runs per tile) and may run tiles in launch order instead of
+// XCD-contiguous ranges; every plan computes the same transform.
+static uint32_t strided_passes(uint32_t strided_bits, const NttPlan& plan) {
+  uint32_t n = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
+  if (plan.split && n && strided_bits > n) n++;
+  return n;
+}
+
 static void forward_passes(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
-                           uint32_t eff_bits, hipStream_t s, Batch bt) {
+                           uint32_t eff_bits, hipStream_t s, Batch bt, const NttPlan& plan) {
   const uint32_t log_n = tw.log_n;
   const uint64_t dst_stride = bt.dst_stride;
   uint32_t top = eff_bits;
   const uint32_t contig = eff_bits < NTT_TILE_LOG ? eff_bits : NTT_TILE_LOG;
   const uint32_t strided_bits = eff_bits - contig;
-  const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
+  const uint32_t n_strided = strided_passes(strided_bits, plan);
   const uint32_t* cur = src;
   uint64_t mask = src_mask;
   for (uint32_t p = 0; p < n_strided; p++) {
     uint32_t k = (top - contig) / (n_strided - p);  // balanced chunks
     uint32_t j0 = top - k;
-    launch_pass(true, data, cur, mask, tw, j0, k, NTT_TILE_LOG - k, 1u, bt, s);
+    launch_pass(true, data, cur, mask, tw, j0, k, NTT_TILE_LOG - k, 1u, bt, s, plan.order);
     top = j0;
     cur = data;
     mask = (1ull << log_n) - 1;
     bt.src_stride = dst_stride;
     bt.src_cols = nullptr;  // later passes work in place on the destinations
   }
-  launch_pass(true, data, cur, mask, tw, 0, contig, 0, 1u, bt, s);
+  launch_pass(true, data, cur, mask, tw, 0, contig, 0, 1u, bt, s, plan.order);
 }
 
 void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
@@ -665,7 +677,30 @@ void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const T
           "ntt_forward: source length must be a power of two covering the transformed bits");
   require(n_cols == 1 || (dst_stride >= (1ull << log_n) && src_stride >= src_mask + 1),
           "ntt_forward: batch columns must not overlap");
-  forward_passes(data, src, src_mask, tw, eff_bits, s, Batch{n_cols, dst_stride, src_stride, nullptr, nullptr});
+  const Batch bt{n_cols, dst_stride, src_stride, nullptr, nullptr};
+  const NttKey key{true, tw.log_n, eff_bits, n_cols, tw.stwo_circle != 0};
+  // out of place, the first pass reads src intact, so the candidates run on
+  // the caller's buffers; in place they run on a scratch buffer of the same
+  // shape (allocated only when this shape is measured)
+  struct Scratch {
+    void* p = nullptr;
+    ~Scratch() {
+      if (p) (void)hipFree(p);
+    }
+  } scratch;
+  const uint64_t words = (n_cols > 1 ? dst_stride : 0) * (n_cols - 1) + (1ull << tw.log_n);
+  const NttPlan plan = tuned_ntt_plan(key, s, [&](const NttPlan& p) {
+    if (src != data) {
+      forward_passes(data, src, src_mask, tw, eff_bits, s, bt, p);
+      return true;
+    }
+    if (words > NTT_TUNE_SCRATCH_MAX_WORDS) return false;
+    if (!scratch.p) VX_HIP(hipMalloc(&scratch.p, 4 * words));
+    uint32_t* t = static_cast<uint32_t*>(scratch.p);
+    forward_passes(t, t, src_mask, tw, eff_bits, s, bt, p);
+    return true;
+  });
+  forward_passes(data, src, src_mask, tw, eff_bits, s, bt, plan);
 }
 
 void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s,
@@ -673,23 +708,25 @@ void ntt_forward_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef
   require(d_cols != nullptr && tw.log_n >= 1 && tw.log_n <= 31, "ntt_forward_cols: bad arguments");
   const uint32_t eff = d_src_cols ? src_log_n : tw.log_n;
   require(eff >= 1 && eff <= tw.log_n, "ntt_forward_cols: source size must be 2..2^log_n");
-  forward_passes(nullptr, nullptr, (1ull << eff) - 1, tw, eff, s, Batch{n_cols, 0, 0, d_cols, d_src_cols});
+  const NttPlan plan = known_ntt_plan(NttKey{true, tw.log_n, eff, n_cols, tw.stwo_circle != 0});
+  forward_passes(nullptr, nullptr, (1ull << eff) - 1, tw, eff, s, Batch{n_cols, 0, 0, d_cols, d_src_cols}, plan);
 }
 
-static void inverse_passes(uint32_t* data, const TwiddleRef& tw, hipStream_t s, const Batch& bt) {
+static void inverse_passes(uint32_t* data, const TwiddleRef& tw, hipStream_t s, const Batch& bt,
+                           const NttPlan& plan) {
   const uint32_t log_n = tw.log_n;
   const uint32_t e = (30u * log_n) % 31u;
   const uint32_t inv_n = e == 0 ? 1u : (1u << e);
   const uint64_t mask = (1ull << log_n) - 1;
   const uint32_t contig = log_n < NTT_TILE_LOG ? log_n : NTT_TILE_LOG;
   const uint32_t strided_bits = log_n - contig;
-  const uint32_t n_strided = (strided_bits + NTT_STRIDED_MAX - 1) / NTT_STRIDED_MAX;
-  launch_pass(false, data, data, mask, tw, 0, contig, 0, n_strided == 0 ? inv_n : 1u, bt, s);
+  const uint32_t n_strided = strided_passes(strided_bits, plan);
+  launch_pass(false, data, data, mask, tw, 0, contig, 0, n_strided == 0 ? inv_n : 1u, bt, s, plan.order);
   uint32_t bot = contig;
   for (uint32_t p = 0; p < n_strided; p++) {
     uint32_t k = (log_n - bot) / (n_strided - p);
     bool last = p + 1 == n_strided;
-    launch_pass(false, data, data, mask, tw, bot, k, NTT_TILE_LOG - k, last ? inv_n : 1u, bt, s);
+    launch_pass(false, data, data, mask, tw, bot, k, NTT_TILE_LOG - k, last ? inv_n : 1u, bt, s, plan.order);
     bot += k;
   }
 }
@@ -699,12 +736,30 @@ void ntt_inverse(uint32_t* data, const TwiddleRef& tw, hipStream_t s, uint32_t n
   require(aligned16(data) && stride % 4 == 0, "ntt: columns must be 16-byte aligned");
   require(log_n >= 1 && log_n <= 31, "ntt_inverse: bad size");
   require(n_cols == 1 || stride >= (1ull << log_n), "ntt_inverse: batch columns must not overlap");
-  inverse_passes(data, tw, s, Batch{n_cols, stride, stride, nullptr, nullptr});
+  const Batch bt{n_cols, stride, stride, nullptr, nullptr};
+  const NttKey key{false, log_n, log_n, n_cols, tw.stwo_circle != 0};
+  // in place: the candidates are timed on a scratch buffer of the same
+  // shape (allocated only when this shape is measured)
+  struct Scratch {
+    void* p = nullptr;
+    ~Scratch() {
+      if (p) (void)hipFree(p);
+    }
+  } scratch;
+  const uint64_t words = (n_cols > 1 ? stride : 0) * (n_cols - 1) + (1ull << log_n);
+  const NttPlan plan = tuned_ntt_plan(key, s, [&](const NttPlan& p) {
+    if (words > NTT_TUNE_SCRATCH_MAX_WORDS) return false;
+    if (!scratch.p) VX_HIP(hipMalloc(&scratch.p, 4 * words));
+    inverse_passes(static_cast<uint32_t*>(scratch.p), tw, s, bt, p);
+    return true;
+  });
+  inverse_passes(data, tw, s, bt, plan);
 }
 
 void ntt_inverse_cols(uint32_t* const* d_cols, uint32_t n_cols, const TwiddleRef& tw, hipStream_t s) {
   require(d_cols != nullptr && tw.log_n >= 1 && tw.log_n <= 31, "ntt_inverse_cols: bad arguments");
-  inverse_passes(nullptr, tw, s, Batch{n_cols, 0, 0, d_cols, nullptr});
+  inverse_passes(nullptr, tw, s, Batch{n_cols, 0, 0, d_cols, nullptr},
+                 known_ntt_plan(NttKey{false, tw.log_n, tw.log_n, n_cols, tw.stwo_circle != 0}));
 }
 
 // ---------------------------------------------------------------------------

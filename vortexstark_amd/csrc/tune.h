@@ -1,5 +1,6 @@
-// Autotuner of the Merkle subtree schedule (tune.cpp); the lookup used by
-// merkle_commit is tuned_shape in vx_internal.h.
+// Autotuner of the Merkle subtree schedule and of the circle NTT's plan
+// (tune.cpp); the lookups are tuned_shape and tuned_ntt_plan in
+// vx_internal.h.
 #pragma once
 #include <stdint.h>
 
@@ -15,5 +16,8 @@ struct TunedEntry {
   uint32_t device, kind, n_cols, n_cons, lift_n, log_n, store_lo, tile_log, lpt;
 };
 std::vector<TunedEntry> tuner_entries();
+// NTT plans chosen so far: (device, fwd, log_n, eff_bits, n_cols, stwo,
+// split, order) per entry.
+std::vector<uint32_t> tuner_ntt_entries();
 
 }  // namespace vx

@@ -1,6 +1,7 @@
 // Internal interfaces between the gfx950 kernel translation units and the
 // C++ host layer (prover, C-ABI shims). Not part of the public C-ABI.
 #pragma once
+#include <functional>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -88,6 +89,25 @@ struct NttPass {
 //   inverse: interpolate, then scale by 2^((30 log_n) mod 31).
 // Batched over n_cols columns (one per blockIdx.y): column c of data starts
 // at data + c * dst_stride (stride for the inverse), of src at src + c * src_stride.
+// Tunable schedule of a transform (ntt.hip): split = 1 spreads the strided
+// bits over one more pass, order = 1 runs tiles in launch order instead of
+// XCD-contiguous ranges. Every plan computes the same values.
+struct NttPlan {
+  uint32_t split = 0, order = 0;
+};
+struct NttKey {
+  bool fwd;
+  uint32_t log_n, eff_bits, n_cols;
+  bool stwo;
+};
+// The plan of a transform shape (tune.cpp): cached, or measured now by
+// calling run(plan) for each candidate (run returns false where it cannot
+// time that shape: the built-in plan is used), or the built-in plan while the
+// stream is being captured or tuning is off.
+NttPlan tuned_ntt_plan(const NttKey& key, hipStream_t s, const std::function<bool(const NttPlan&)>& run);
+NttPlan known_ntt_plan(const NttKey& key);  // cached choice or the built-in plan, never measures
+constexpr uint64_t NTT_TUNE_SCRATCH_MAX_WORDS = 1ull << 28;
+
 void ntt_forward(uint32_t* data, const uint32_t* src, uint64_t src_mask, const TwiddleRef& tw,
                  uint32_t eff_bits, hipStream_t s, uint32_t n_cols = 1, uint64_t dst_stride = 0,
                  uint64_t src_stride = 0);
