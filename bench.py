@@ -951,6 +951,9 @@ def main():
     ap.add_argument("--profile-serving-mix", action="store_true",
                     help="eager proofs of every serving class (7 per Fibonacci class, 9 per synthetic class) "
                          "for a kernel-trace profile of the mix")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file: choices are loaded from it and new ones appended, so that "
+                         "profiled runs use the shapes an unprofiled run chose")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: one 1 GiB device copy (counter calibration), "
                          "1 warmup + 2 timed proofs, no CPU baseline")
@@ -965,6 +968,8 @@ def main():
     import vortexstark_amd as vx
 
     vx.load()
+    if args.tune_cache:
+        vx.tuner(True, args.tune_cache)
     if args.profile_serving_mix:
         print(json.dumps({"serving_mix_profile": serving_mix_profile(vx)}), flush=True)
         return

@@ -134,6 +134,17 @@ __global__ void k_stwo_index_to_nonce(unsigned long long* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0 && *p != ~0ull) *p = stwo_nonce(*p);
 }
 
+// Workgroups of a search for `bits` trailing zeros. Every lane of the grid
+// tests one nonce per round until the round passes the smallest hit, so a
+// search tests about (hit + one round of the grid) nonces; a round of a
+// quarter of the expected 2^bits nonces keeps that overshoot small (PoW 20:
+// 1024 workgroups, 2^18 lanes), and large searches use the whole GPU
+// (2048 workgroups, 8 per CU).
+static uint32_t grind_grid(uint32_t bits) {
+  const uint64_t wgs = bits >= 20 + 1 ? 2048ull : (1ull << bits) / 4 / 256;
+  return (uint32_t)(wgs < 64 ? 64 : (wgs > 2048 ? 2048 : wgs));
+}
+
 // Whole search over the stwo family: the same increasing-index scan, then
 // the winning index is turned into its nonce. Indices up to 2^44 cover
 // hi < 2^24.
@@ -144,11 +155,11 @@ void grind_pow_stwo(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_res
   auto* best = reinterpret_cast<unsigned long long*>(d_result);
   hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, best, ~0ull);
   if (m31_out)
-    hipLaunchKernelGGL((k_grind<true, true>), dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull,
-                       GRIND_LIMIT);
+    hipLaunchKernelGGL((k_grind<true, true>), dim3(grind_grid(pow_bits)), dim3(256), 0, s, d_prefix, pow_bits, best,
+                       0ull, GRIND_LIMIT);
   else
-    hipLaunchKernelGGL((k_grind<true, false>), dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull,
-                       GRIND_LIMIT);
+    hipLaunchKernelGGL((k_grind<true, false>), dim3(grind_grid(pow_bits)), dim3(256), 0, s, d_prefix, pow_bits,
+                       best, 0ull, GRIND_LIMIT);
   hipLaunchKernelGGL(k_stwo_index_to_nonce, dim3(1), dim3(64), 0, s, best);
   VX_HIP(hipGetLastError());
 }
@@ -157,8 +168,9 @@ void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, 
   require(pow_bits >= 1 && pow_bits <= 32, "grind_pow: pow_bits must be 1..32");
   auto* best = reinterpret_cast<unsigned long long*>(d_result);
   hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, best, ~0ull);
-  // 256 CUs x 8 workgroups: every lane loops until its nonce exceeds the best hit.
-  hipLaunchKernelGGL(k_grind<false>, dim3(2048), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull, GRIND_LIMIT);
+  // every lane loops until its nonce exceeds the best hit
+  hipLaunchKernelGGL(k_grind<false>, dim3(grind_grid(pow_bits)), dim3(256), 0, s, d_prefix, pow_bits, best, 0ull,
+                     GRIND_LIMIT);
   VX_HIP(hipGetLastError());
 }
 
