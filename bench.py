@@ -207,6 +207,11 @@ VALU_MEASURED_TOPS = 41.2
 # (SQ_INSTS_VALU x 64 / compressions, profiles/fib_log24_valu_counters.md):
 # 960 of the compression function plus loads, addressing and stores.
 ISSUED_PER_COMPRESSION = 997.1
+# VALU lane instructions a proof of the serving mix issues on average
+# (rocprofv3 SQ_INSTS_VALU x 64 over `bench.py --profile-serving-mix`, every
+# class of the mix proved eagerly, per-class table setup left out): the
+# issue-rate ceiling of the mix in proofs/s is the VALU rate over it.
+MIX_ISSUED_PER_PROOF = 2.493e10
 
 
 def proof_valu_ops(air, log_n, blowup_bits, pow_bits, n_cols=34, n_cons=35):
@@ -335,6 +340,13 @@ def serving(vx, n_requests, slots, seed=42, load=0.8):
     roof["achieved_proofs_per_s"] = sat_batch["proofs_per_s"]
     roof["frac"] = round(sat_batch["proofs_per_s"] / roof["floor_proofs_per_s"], 4)
     roof["plain_loop_frac"] = round(sat_plain["proofs_per_s"] / roof["floor_proofs_per_s"], 4)
+    ceiling = VALU_MEASURED_TOPS * 1e12 / MIX_ISSUED_PER_PROOF
+    roof["issued_instructions_per_proof"] = MIX_ISSUED_PER_PROOF
+    roof["issue_ceiling_proofs_per_s"] = round(ceiling, 1)
+    roof["frac_of_issue_ceiling"] = round(sat_batch["proofs_per_s"] / ceiling, 4)
+    roof["plain_loop_frac_of_issue_ceiling"] = round(sat_plain["proofs_per_s"] / ceiling, 4)
+    # a capacity ratio of 2 needs the batched server at twice the loop
+    roof["ratio_2_needs_frac_of_issue_ceiling"] = round(2 * sat_plain["proofs_per_s"] / ceiling, 4)
     resident = batch.resident_bytes
     plain.close()
     batch.close()
