@@ -36,6 +36,7 @@ static constexpr uint32_t SMALL_TREE_LOG = 16;  // <= this: 1 leaf per lane, 256
 static constexpr uint32_t MAX_TILE_LOG = 12;
 static constexpr uint32_t RED_THREADS = 512;  // tree-top reduce: 1024 nodes, 10 levels per workgroup
 static constexpr uint32_t RED_NLEV_MAX = 10;
+static constexpr uint32_t RED_QUAD_NLEV = 8;  // 4 x 2^(8-1) lanes: the first level by quads
 static constexpr uint32_t TAIL_THREADS = 1u << TAIL_MAX_LOG;  // one lane per leaf of the largest tail layer
 
 struct G4 {
@@ -825,9 +826,18 @@ void merkle_commit_shaped(const LeafSrc& L, const TreeLevels& T, uint32_t store_
   if (sub_begin) VX_HIP(hipEventRecord(sub_begin, s));
   dispatch_subtree(L, T, tile_log, lpt, store_lo, hook, s);
   if (sub_end) VX_HIP(hipEventRecord(sub_end, s));
+  // Levels above the tiles: up to RED_NLEV_MAX in one workgroup. More are
+  // split evenly over launches of at most 8 levels, so that every workgroup
+  // hashes its first level by quads (at most 128 nodes) and the lower
+  // launches spread over many CUs: the levels are a dependent chain, and a
+  // level of 512 nodes hashed one per lane on one CU takes about four quad
+  // levels' time.
+  const uint32_t above = T.log_n - tile_log;
+  const uint32_t launches = above <= RED_NLEV_MAX ? 1 : (above + RED_QUAD_NLEV - 1) / RED_QUAD_NLEV;
   uint32_t from = tile_log;
-  while (from < T.log_n) {
-    uint32_t nlev = T.log_n - from < RED_NLEV_MAX ? T.log_n - from : RED_NLEV_MAX;
+  for (uint32_t k = 0; from < T.log_n; k++) {
+    const uint32_t left = launches - k;  // levels of this launch: an even share of what is left
+    const uint32_t nlev = (T.log_n - from + left - 1) / left;
     uint64_t grid = 1ull << (T.log_n - from - nlev);
     if (from + nlev == T.log_n)
       hipLaunchKernelGGL(k_tree_reduce<true>, dim3((uint32_t)grid), dim3(RED_THREADS), 0, s, T, from, nlev, hook);
