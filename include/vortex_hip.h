@@ -847,6 +847,66 @@ int vx_tuner_entries(uint32_t *out, uint32_t capacity, uint32_t *n);
  * vx_tuner_configure. */
 int vx_tuner_ntt_entries(uint32_t *out, uint32_t capacity, uint32_t *n);
 
+/* ------------------------------------------------ GKR (stwo plugin GkrOps)
+ * The plugin's GKR sum-check operations (crates/cuda-backend/src/gkr_ops.rs
+ * over cuda/gkr.cu). Secure (QM31) columns hold element i in words
+ * 4i..4i+3, base columns one M31 word per element; every pointer is a device
+ * address and every call is asynchronous on `stream`. n is the input length
+ * in elements (even, >= 2). QM31 arguments (r, v, y_i, lambda) are 4 words
+ * in device memory, as the plugin uploads them. */
+/* Replaces cuda_gkr_fix_first_variable_base (ffi.rs:1129-1133): out[i] =
+ * r (in[i + n/2] - in[i]) + in[i] for i < n/2, M31 in, QM31 out. */
+int vx_gkr_fix_first_variable_base(const uint32_t *d_in, uint32_t *d_out, const uint32_t *d_r, uint32_t n,
+                                   vx_stream_t stream);
+/* Replaces cuda_gkr_fix_first_variable_secure (ffi.rs:1134-1138): the same
+ * over a QM31 column; d_out may equal d_in. */
+int vx_gkr_fix_first_variable_secure(const uint32_t *d_in, uint32_t *d_out, const uint32_t *d_r, uint32_t n,
+                                     vx_stream_t stream);
+/* Replaces cuda_gkr_gen_eq_evals_init (ffi.rs:1139-1140): buf[0] = v. */
+int vx_gkr_gen_eq_evals_init(uint32_t *d_buf, const uint32_t *d_v, vx_stream_t stream);
+/* Replaces cuda_gkr_gen_eq_evals_pass (ffi.rs:1141-1142): for j < cur_size,
+ * t = buf[j] y_i, buf[cur_size + j] = t, buf[j] -= t. */
+int vx_gkr_gen_eq_evals_pass(uint32_t *d_buf, const uint32_t *d_y_i, uint32_t cur_size, vx_stream_t stream);
+/* No counterpart in the reference, whose gen_eq_evals (gkr_ops.rs:129-161)
+ * calls init and then k passes with y[k-1], ..., y[0] from the host: the
+ * same 2^k entries (v times the eq polynomial of y, y[0] on the index's top
+ * bit) in one launch. d_y: k QM31 values; k <= 31. */
+int vx_gkr_gen_eq_evals(uint32_t *d_buf, const uint32_t *d_y, uint32_t k, const uint32_t *d_v, vx_stream_t stream);
+/* Replace cuda_gkr_next_layer_grand_product / _logup_generic /
+ * _logup_mult / _logup_singles (ffi.rs:1143-1159): output element i from
+ * the input pair (2i, 2i + 1); the product, or the fraction sum
+ * (n0 d1 + n1 d0, d0 d1) with QM31 numerators, M31 numerators (one word
+ * each) or unit numerators (no d_in_num). n input elements, n/2 out. */
+int vx_gkr_next_layer_grand_product(const uint32_t *d_in, uint32_t *d_out, uint32_t n, vx_stream_t stream);
+int vx_gkr_next_layer_logup_generic(const uint32_t *d_in_num, const uint32_t *d_in_den, uint32_t *d_out_num,
+                                    uint32_t *d_out_den, uint32_t n, vx_stream_t stream);
+int vx_gkr_next_layer_logup_mult(const uint32_t *d_in_num, const uint32_t *d_in_den, uint32_t *d_out_num,
+                                 uint32_t *d_out_den, uint32_t n, vx_stream_t stream);
+int vx_gkr_next_layer_logup_singles(const uint32_t *d_in_den, uint32_t *d_out_num, uint32_t *d_out_den, uint32_t n,
+                                    vx_stream_t stream);
+/* Replace cuda_gkr_sum_poly_grand_product / _logup_generic / _logup_mult /
+ * _logup_singles (ffi.rs:1160-1179), the device part of
+ * sum_as_poly_in_first_variable (gkr_ops.rs:256-330): with the pair of term
+ * i at t = 0 being elements (2i, 2i + 1), at t = 1 (2(n_terms + i),
+ * 2(n_terms + i) + 1) and at t = 2 twice t = 1 minus t = 0, writes
+ * (sum_i eq[i] F(t = 0), sum_i eq[i] F(t = 2)) as 8 words per block to
+ * d_partial_sums and the number of blocks to *n_blocks (the reference
+ * returns it); F is the pair's product or its fraction sum plus lambda
+ * times its denominator. Here every partial is folded into the first block,
+ * so *n_blocks = 1; d_partial_sums must hold ceil(n_terms / 256) blocks,
+ * as the plugin allocates. */
+int vx_gkr_sum_poly_grand_product(const uint32_t *d_eq, const uint32_t *d_layer, uint32_t *d_partial_sums,
+                                  uint32_t n_terms, uint32_t *n_blocks, vx_stream_t stream);
+int vx_gkr_sum_poly_logup_generic(const uint32_t *d_eq, const uint32_t *d_in_num, const uint32_t *d_in_den,
+                                  const uint32_t *d_lambda, uint32_t *d_partial_sums, uint32_t n_terms,
+                                  uint32_t *n_blocks, vx_stream_t stream);
+int vx_gkr_sum_poly_logup_mult(const uint32_t *d_eq, const uint32_t *d_in_num, const uint32_t *d_in_den,
+                               const uint32_t *d_lambda, uint32_t *d_partial_sums, uint32_t n_terms,
+                               uint32_t *n_blocks, vx_stream_t stream);
+int vx_gkr_sum_poly_logup_singles(const uint32_t *d_eq, const uint32_t *d_in_den, const uint32_t *d_lambda,
+                                  uint32_t *d_partial_sums, uint32_t n_terms, uint32_t *n_blocks,
+                                  vx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

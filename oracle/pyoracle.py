@@ -125,6 +125,12 @@ def lib(omp=False):
         "or_bc_offset_row": (u32, [u32, u32, u32, C.c_int32]),
         "or_bytecode_constraint_eval": (C.c_int, [_u32p, u32, C.c_void_p, u32, u32, u32, _u32p, _u32p, _u32p,
                                                   _u32p, _u32p, _u32p]),
+        "or_gkr_fix_first_variable_base": (None, [_u32p, _u32p, _u32p, u64]),
+        "or_gkr_fix_first_variable_secure": (None, [_u32p, _u32p, _u32p, u64]),
+        "or_gkr_gen_eq_evals_pass": (None, [_u32p, _u32p, u64]),
+        "or_gkr_gen_eq_evals": (None, [_u32p, u32, _u32p, _u32p]),
+        "or_gkr_next_layer": (None, [C.c_int, C.c_void_p, C.c_void_p, _u32p, _u32p, u64]),
+        "or_gkr_sum_poly": (None, [C.c_int, _u32p, C.c_void_p, C.c_void_p, _u32p, u64, _u32p]),
     }
     for name_, (res, args) in sig.items():
         f = getattr(L, name_)
@@ -712,3 +718,57 @@ def bytecode_eval(words, cols, n_rows, trace_n_rows, coeff, denom_inv, acc=None)
     if rc != 0:
         raise ValueError("oracle rejected the bytecode")
     return out
+
+
+# ---- GKR sum-check operations of the stwo plugin (cuda/gkr.cu) ----
+# Secure columns are flat uint32 arrays, 4 words per QM31 element.
+GKR_GRAND_PRODUCT, GKR_LOGUP_GENERIC, GKR_LOGUP_MULT, GKR_LOGUP_SINGLES = 0, 1, 2, 3
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data
+
+
+def gkr_fix_first_variable(col, r, secure):
+    """fold_mle_evals of the first variable (cuda/gkr.cu:20-74): n elements
+    in (M31 or QM31), n/2 QM31 out."""
+    c = u32(col)
+    n = c.size // 4 if secure else c.size
+    out = np.zeros(4 * (n // 2), np.uint32)
+    f = lib().or_gkr_fix_first_variable_secure if secure else lib().or_gkr_fix_first_variable_base
+    f(c, out, u32(r), n)
+    return out
+
+
+def gkr_gen_eq_evals(y, v):
+    """gen_eq_evals (gkr_ops.rs:129-161): y is k QM31 values (k x 4), 2^k out."""
+    yy = u32(np.asarray(y, np.uint32).reshape(-1))
+    k = yy.size // 4
+    out = np.zeros(4 << k, np.uint32)
+    lib().or_gkr_gen_eq_evals(yy if k else np.zeros(4, np.uint32), k, u32(v), out)
+    return out
+
+
+def gkr_gen_eq_evals_pass(buf, y, cur):
+    b = u32(buf).copy()
+    lib().or_gkr_gen_eq_evals_pass(b, u32(y), cur)
+    return b
+
+
+def gkr_next_layer(kind, num, den, n):
+    """next_layer (cuda/gkr.cu:120-212); returns (num, den) of n/2 elements
+    (den None for the grand product)."""
+    keep = [None if x is None else u32(x) for x in (num, den)]
+    on, od = np.zeros(2 * n, np.uint32), np.zeros(2 * n, np.uint32)
+    lib().or_gkr_next_layer(kind, _vp(keep[0]), _vp(keep[1]), on, od, n)
+    return on, (None if kind == GKR_GRAND_PRODUCT else od)
+
+
+def gkr_sum_poly(kind, eq, num, den, lam, n_terms):
+    """The device sums of sum_as_poly_in_first_variable (cuda/gkr.cu:260-540):
+    (eval_at_0, eval_at_2) as two 4-word arrays."""
+    keep = [None if x is None else u32(x) for x in (num, den)]
+    out = np.zeros(8, np.uint32)
+    lib().or_gkr_sum_poly(kind, u32(eq), _vp(keep[0]), _vp(keep[1]), u32(lam if lam is not None else [0] * 4),
+                          n_terms, out)
+    return out[:4].copy(), out[4:].copy()

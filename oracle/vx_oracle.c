@@ -2538,3 +2538,125 @@ int or_bytecode_constraint_eval(const uint32_t *bc, uint32_t n_words, const uint
     free(regs);
     return rc;
 }
+
+/* ======================================================================
+ * GKR sum-check operations of the stwo plugin: cuda/gkr.cu,
+ * crates/cuda-backend/src/gkr_ops.rs. QM31 element i of a secure column is
+ * words 4i..4i+3; a base column is one word per element.
+ * ====================================================================== */
+static inline void q_put(uint32_t *p, qm31 x) {
+    p[0] = x.v[0];
+    p[1] = x.v[1];
+    p[2] = x.v[2];
+    p[3] = x.v[3];
+}
+
+/* out[i] = r (in[i + n/2] - in[i]) + in[i], i < n/2 (cuda/gkr.cu:20-45; the
+ * fold_mle_evals of the MLE's first variable). */
+void or_gkr_fix_first_variable_base(const uint32_t *in, uint32_t *out, const uint32_t r[4], uint64_t n) {
+    const uint64_t h = n / 2;
+    const qm31 rq = q_load(r);
+    for (uint64_t i = 0; i < h; i++) {
+        qm31 o = q_scale(rq, m31_sub(in[i + h], in[i]));
+        o.v[0] = m31_add(o.v[0], in[i]);
+        q_put(out + 4 * i, o);
+    }
+}
+
+/* The same over a secure column (cuda/gkr.cu:47-74). */
+void or_gkr_fix_first_variable_secure(const uint32_t *in, uint32_t *out, const uint32_t r[4], uint64_t n) {
+    const uint64_t h = n / 2;
+    const qm31 rq = q_load(r);
+    for (uint64_t i = 0; i < h; i++) {
+        const qm31 a = q_load(in + 4 * i), b = q_load(in + 4 * (i + h));
+        q_put(out + 4 * i, q_add(q_mul(rq, q_sub(b, a)), a));
+    }
+}
+
+/* One doubling pass of gen_eq_evals (cuda/gkr.cu:84-118): for j < cur,
+ * t = buf[j] y; buf[cur + j] = t; buf[j] -= t. */
+void or_gkr_gen_eq_evals_pass(uint32_t *buf, const uint32_t y[4], uint64_t cur) {
+    const qm31 yq = q_load(y);
+    for (uint64_t j = 0; j < cur; j++) {
+        const qm31 v = q_load(buf + 4 * j), t = q_mul(v, yq);
+        q_put(buf + 4 * (cur + j), t);
+        q_put(buf + 4 * j, q_sub(v, t));
+    }
+}
+
+/* gen_eq_evals(y[0..k), v) (gkr_ops.rs:129-161): buf[0] = v, then passes
+ * with y[k-1], ..., y[0] over 1, 2, ..., 2^(k-1) entries; out has 2^k. */
+void or_gkr_gen_eq_evals(const uint32_t *y, uint32_t k, const uint32_t v[4], uint32_t *out) {
+    memcpy(out, v, 16);
+    for (uint32_t i = 0; i < k; i++) or_gkr_gen_eq_evals_pass(out, y + 4 * (k - 1 - i), 1ull << i);
+}
+
+/* next_layer (cuda/gkr.cu:120-212): kind 0 GrandProduct out[i] = in[2i] in[2i+1]
+ * (in_num is the layer); 1 LogUpGeneric, 2 LogUpMultiplicities (M31
+ * numerators, one word each), 3 LogUpSingles (numerators 1, in_num unused):
+ * (n0 d1 + n1 d0, d0 d1) of the pair (2i, 2i+1). n = input length. */
+void or_gkr_next_layer(int kind, const uint32_t *in_num, const uint32_t *in_den, uint32_t *out_num,
+                       uint32_t *out_den, uint64_t n) {
+    const uint64_t h = n / 2;
+    for (uint64_t i = 0; i < h; i++) {
+        if (kind == 0) {
+            q_put(out_num + 4 * i, q_mul(q_load(in_num + 8 * i), q_load(in_num + 8 * i + 4)));
+            continue;
+        }
+        const qm31 d0 = q_load(in_den + 8 * i), d1 = q_load(in_den + 8 * i + 4);
+        qm31 rn;
+        if (kind == 1) rn = q_add(q_mul(q_load(in_num + 8 * i), d1), q_mul(q_load(in_num + 8 * i + 4), d0));
+        else if (kind == 2) rn = q_add(q_scale(d1, in_num[2 * i]), q_scale(d0, in_num[2 * i + 1]));
+        else rn = q_add(d0, d1);
+        q_put(out_num + 4 * i, rn);
+        q_put(out_den + 4 * i, q_mul(d0, d1));
+    }
+}
+
+/* sum_as_poly_in_first_variable's device sums (cuda/gkr.cu:260-540): for
+ * i < n_terms, the pair at t = 0 is (2i, 2i+1) of the layer and at t = 1
+ * (2(n_terms + i), 2(n_terms + i) + 1); t = 2 is 2 (t = 1) - (t = 0) per
+ * element; F(pair) is the product (kind 0) or the pair's fraction sum
+ * num + lambda den (kinds 1..3, as next_layer); out[0..4) = sum_i eq[i]
+ * F(pair at 0), out[4..8) = sum_i eq[i] F(pair at 2). */
+void or_gkr_sum_poly(int kind, const uint32_t *eq, const uint32_t *in_num, const uint32_t *in_den,
+                     const uint32_t lambda[4], uint64_t n_terms, uint32_t out[8]) {
+    const qm31 lam = kind == 0 ? (qm31){{0, 0, 0, 0}} : q_load(lambda);
+    qm31 s0 = {{0, 0, 0, 0}}, s2 = {{0, 0, 0, 0}};
+    for (uint64_t i = 0; i < n_terms; i++) {
+        const uint64_t a = 2 * i, b = 2 * (n_terms + i);
+        qm31 f[2];
+        for (int t = 0; t < 2; t++) {  /* t = 0 and t = 2 */
+            qm31 num[2], den[2];
+            for (int e = 0; e < 2; e++) {
+                qm31 x0, x1, y0 = {{0, 0, 0, 0}}, y1 = {{0, 0, 0, 0}};
+                if (kind == 2) {
+                    x0 = (qm31){{in_num[a + e], 0, 0, 0}};
+                    x1 = (qm31){{in_num[b + e], 0, 0, 0}};
+                } else if (kind == 3) {
+                    x0 = x1 = (qm31){{1, 0, 0, 0}};
+                } else {
+                    x0 = q_load(in_num + 4 * (a + e));
+                    x1 = q_load(in_num + 4 * (b + e));
+                }
+                if (kind != 0) {
+                    y0 = q_load(in_den + 4 * (a + e));
+                    y1 = q_load(in_den + 4 * (b + e));
+                }
+                num[e] = t == 0 ? x0 : q_sub(q_add(x1, x1), x0);
+                den[e] = t == 0 ? y0 : q_sub(q_add(y1, y1), y0);
+            }
+            if (kind == 0) {
+                f[t] = q_mul(num[0], num[1]);
+            } else {
+                const qm31 fn = q_add(q_mul(num[0], den[1]), q_mul(num[1], den[0]));
+                f[t] = q_add(fn, q_mul(lam, q_mul(den[0], den[1])));
+            }
+        }
+        const qm31 e = q_load(eq + 4 * i);
+        s0 = q_add(s0, q_mul(e, f[0]));
+        s2 = q_add(s2, q_mul(e, f[1]));
+    }
+    q_put(out, s0);
+    q_put(out + 4, s2);
+}

@@ -233,6 +233,17 @@ int or_bytecode_constraint_eval(const uint32_t *bytecode, uint32_t n_words, cons
                                 const uint32_t *denom_inv, uint32_t *acc0, uint32_t *acc1, uint32_t *acc2,
                                 uint32_t *acc3);
 
+/* ---- GKR sum-check operations of the stwo plugin: cuda/gkr.cu (secure
+ * columns 4 words per element) ---- */
+void or_gkr_fix_first_variable_base(const uint32_t *in, uint32_t *out, const uint32_t r[4], uint64_t n);
+void or_gkr_fix_first_variable_secure(const uint32_t *in, uint32_t *out, const uint32_t r[4], uint64_t n);
+void or_gkr_gen_eq_evals_pass(uint32_t *buf, const uint32_t y[4], uint64_t cur);
+void or_gkr_gen_eq_evals(const uint32_t *y, uint32_t k, const uint32_t v[4], uint32_t *out);
+void or_gkr_next_layer(int kind, const uint32_t *in_num, const uint32_t *in_den, uint32_t *out_num,
+                       uint32_t *out_den, uint64_t n);
+void or_gkr_sum_poly(int kind, const uint32_t *eq, const uint32_t *in_num, const uint32_t *in_den,
+                     const uint32_t lambda[4], uint64_t n_terms, uint32_t out[8]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,19 @@ SIGNATURES = {
     "vx_eval_at_point_batch": (ip, [vp, vp, u32, vp, u32, vp, vp]),
     "vx_barycentric_eval": (ip, [vp, vp, u32, vp, u32, vp]),
     "vx_grind_pow_stwo": (ip, [vp, u32, vp, vp]),
+    "vx_gkr_fix_first_variable_base": (ip, [vp, vp, vp, u32, vp]),
+    "vx_gkr_fix_first_variable_secure": (ip, [vp, vp, vp, u32, vp]),
+    "vx_gkr_gen_eq_evals_init": (ip, [vp, vp, vp]),
+    "vx_gkr_gen_eq_evals_pass": (ip, [vp, vp, u32, vp]),
+    "vx_gkr_gen_eq_evals": (ip, [vp, vp, u32, vp, vp]),
+    "vx_gkr_next_layer_grand_product": (ip, [vp, vp, u32, vp]),
+    "vx_gkr_next_layer_logup_generic": (ip, [vp, vp, vp, vp, u32, vp]),
+    "vx_gkr_next_layer_logup_mult": (ip, [vp, vp, vp, vp, u32, vp]),
+    "vx_gkr_next_layer_logup_singles": (ip, [vp, vp, vp, u32, vp]),
+    "vx_gkr_sum_poly_grand_product": (ip, [vp, vp, vp, u32, vp, vp]),
+    "vx_gkr_sum_poly_logup_generic": (ip, [vp, vp, vp, vp, vp, u32, vp, vp]),
+    "vx_gkr_sum_poly_logup_mult": (ip, [vp, vp, vp, vp, vp, u32, vp, vp]),
+    "vx_gkr_sum_poly_logup_singles": (ip, [vp, vp, vp, vp, u32, vp, vp]),
     "vx_grind_pow_stwo_m31": (ip, [vp, u32, vp, vp]),
     "vx_permute_hc_to_canonic_brt": (ip, [vp, vp, u32, u32, vp]),
     "vx_oods_sample": (ip, [vp, vp, vp, u32, u32, vp, vp, vp]),

@@ -1205,6 +1205,62 @@ int vx_tuner_ntt_entries(uint32_t* out, uint32_t capacity, uint32_t* n) {
   });
 }
 
+// ------------------------------------------------------------------ GKR
+int vx_gkr_fix_first_variable_base(const uint32_t* in, uint32_t* out, const uint32_t* r, uint32_t n, vx_stream_t st) {
+  return guard([&] { gkr_fix_first_variable(in, out, r, n, false, S(st)); });
+}
+int vx_gkr_fix_first_variable_secure(const uint32_t* in, uint32_t* out, const uint32_t* r, uint32_t n,
+                                     vx_stream_t st) {
+  return guard([&] { gkr_fix_first_variable(in, out, r, n, true, S(st)); });
+}
+int vx_gkr_gen_eq_evals_init(uint32_t* buf, const uint32_t* v, vx_stream_t st) {
+  return guard([&] { gkr_gen_eq_evals_init(buf, v, S(st)); });
+}
+int vx_gkr_gen_eq_evals_pass(uint32_t* buf, const uint32_t* y, uint32_t cur, vx_stream_t st) {
+  return guard([&] { gkr_gen_eq_evals_pass(buf, y, cur, S(st)); });
+}
+int vx_gkr_gen_eq_evals(uint32_t* buf, const uint32_t* y, uint32_t k, const uint32_t* v, vx_stream_t st) {
+  return guard([&] { gkr_gen_eq_evals(buf, y, k, v, S(st)); });
+}
+int vx_gkr_next_layer_grand_product(const uint32_t* in, uint32_t* out, uint32_t n, vx_stream_t st) {
+  return guard([&] { gkr_next_layer(0, in, nullptr, out, nullptr, n, S(st)); });
+}
+int vx_gkr_next_layer_logup_generic(const uint32_t* num, const uint32_t* den, uint32_t* on, uint32_t* od, uint32_t n,
+                                    vx_stream_t st) {
+  return guard([&] { gkr_next_layer(1, num, den, on, od, n, S(st)); });
+}
+int vx_gkr_next_layer_logup_mult(const uint32_t* num, const uint32_t* den, uint32_t* on, uint32_t* od, uint32_t n,
+                                 vx_stream_t st) {
+  return guard([&] { gkr_next_layer(2, num, den, on, od, n, S(st)); });
+}
+int vx_gkr_next_layer_logup_singles(const uint32_t* den, uint32_t* on, uint32_t* od, uint32_t n, vx_stream_t st) {
+  return guard([&] { gkr_next_layer(3, nullptr, den, on, od, n, S(st)); });
+}
+static int gkr_sum_call(int kind, const uint32_t* eq, const uint32_t* num, const uint32_t* den,
+                        const uint32_t* lambda, uint32_t* partial, uint32_t n_terms, uint32_t* n_blocks,
+                        vx_stream_t st) {
+  return guard([&] {
+    require(n_blocks != nullptr, "gkr_sum_poly: null n_blocks");
+    *n_blocks = gkr_sum_poly(kind, eq, num, den, lambda, partial, n_terms, S(st));
+  });
+}
+int vx_gkr_sum_poly_grand_product(const uint32_t* eq, const uint32_t* layer, uint32_t* partial, uint32_t n_terms,
+                                  uint32_t* n_blocks, vx_stream_t st) {
+  return gkr_sum_call(0, eq, layer, nullptr, nullptr, partial, n_terms, n_blocks, st);
+}
+int vx_gkr_sum_poly_logup_generic(const uint32_t* eq, const uint32_t* num, const uint32_t* den, const uint32_t* lambda,
+                                  uint32_t* partial, uint32_t n_terms, uint32_t* n_blocks, vx_stream_t st) {
+  return gkr_sum_call(1, eq, num, den, lambda, partial, n_terms, n_blocks, st);
+}
+int vx_gkr_sum_poly_logup_mult(const uint32_t* eq, const uint32_t* num, const uint32_t* den, const uint32_t* lambda,
+                               uint32_t* partial, uint32_t n_terms, uint32_t* n_blocks, vx_stream_t st) {
+  return gkr_sum_call(2, eq, num, den, lambda, partial, n_terms, n_blocks, st);
+}
+int vx_gkr_sum_poly_logup_singles(const uint32_t* eq, const uint32_t* den, const uint32_t* lambda, uint32_t* partial,
+                                  uint32_t n_terms, uint32_t* n_blocks, vx_stream_t st) {
+  return gkr_sum_call(3, eq, nullptr, den, lambda, partial, n_terms, n_blocks, st);
+}
+
 // ------------------------------------------------------- batched columns
 struct vx_batch {
   BatchCommitter impl;

@@ -413,6 +413,20 @@ struct DecomDesc {
 void decommit(const DecomDesc* d_descs, uint32_t n_trees, const uint32_t* d_queries, uint32_t nq,
               uint32_t* proof, hipStream_t s);
 
+// ------------------------------------------------- GKR (gkr.hip)
+// The stwo plugin's GKR operations (cuda/gkr.cu); kinds 0 grand product,
+// 1 LogUp generic, 2 LogUp multiplicities, 3 LogUp singles.
+void gkr_fix_first_variable(const uint32_t* in, uint32_t* out, const uint32_t* r, uint64_t n, bool secure,
+                            hipStream_t s);
+void gkr_gen_eq_evals_init(uint32_t* buf, const uint32_t* v, hipStream_t s);
+void gkr_gen_eq_evals_pass(uint32_t* buf, const uint32_t* y, uint64_t cur, hipStream_t s);
+void gkr_gen_eq_evals(uint32_t* buf, const uint32_t* y, uint32_t k, const uint32_t* v, hipStream_t s);
+void gkr_next_layer(int kind, const uint32_t* num, const uint32_t* den, uint32_t* out_num, uint32_t* out_den,
+                    uint64_t n, hipStream_t s);
+// Returns the number of 8-word blocks written to partial (1).
+uint32_t gkr_sum_poly(int kind, const uint32_t* eq, const uint32_t* num, const uint32_t* den, const uint32_t* lambda,
+                      uint32_t* partial, uint64_t n_terms, hipStream_t s);
+
 // ------------------------------------------------- channel (channel.hip)
 // Single-lane transcript steps over a device-resident Channel.
 void channel_step(Channel* ch, bool init, const uint32_t* digest, uint32_t* alpha_out,
