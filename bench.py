@@ -684,6 +684,64 @@ def bytecode_eval(vx, steps, log_eval=22, log_expand=2, n_ops=800, n_cols=34, n_
     }
 
 
+def gkr(vx, steps, log_n=24):
+    """GKR operations of the stwo plugin (vx_gkr_*) on secure columns of
+    2^log_n QM31 elements resident in HBM, asynchronous calls back to back
+    on one stream: fold of the first variable, the LogUp next layer, the
+    round sums of a LogUp layer (n_terms = 2^(log_n - 2)) and the eq table
+    (2^log_n entries) in one launch and as the plugin's log_n passes. HBM
+    bytes are what each call must move: its inputs read once, its outputs
+    written once."""
+    import ctypes as C
+
+    import numpy as np
+    from vortexstark_amd.device import DeviceBuffer
+
+    call = vx._lib.call
+    n = 1 << log_n
+    rng = np.random.default_rng(11)
+    num = DeviceBuffer.from_host(rng.integers(0, P, 4 * n, dtype=np.uint32))
+    den = DeviceBuffer.from_host(rng.integers(0, P, 4 * n, dtype=np.uint32))
+    small = DeviceBuffer.from_host(rng.integers(0, P, 4 * 32, dtype=np.uint32))  # r, lambda, v, y
+    out_n, out_d = DeviceBuffer.zeros(2 * n), DeviceBuffer.zeros(2 * n)
+    eq = DeviceBuffer.zeros(4 * n)
+    part = DeviceBuffer.zeros(8 * (n // 4 // 256))
+    nb = C.c_uint32(0)
+    vp = lambda b, off=0: C.c_void_p(b.as_ptr() + 4 * off)
+    legs = {
+        "fix_first_variable_secure": (lambda: call("vx_gkr_fix_first_variable_secure", vp(num), vp(out_n),
+                                                   vp(small), n, None), 16 * n + 8 * n),
+        "next_layer_logup_generic": (lambda: call("vx_gkr_next_layer_logup_generic", vp(num), vp(den), vp(out_n),
+                                                  vp(out_d), n, None), 32 * n + 16 * n),
+        "sum_poly_logup_generic": (lambda: call("vx_gkr_sum_poly_logup_generic", vp(eq), vp(num), vp(den),
+                                                vp(small, 4), vp(part), n // 4, C.byref(nb), None),
+                                   16 * (n // 4) + 32 * n),
+        "gen_eq_evals": (lambda: call("vx_gkr_gen_eq_evals", vp(eq), vp(small, 12), log_n, vp(small, 8), None),
+                         16 * n),
+    }
+
+    def eq_passes():
+        call("vx_gkr_gen_eq_evals_init", vp(eq), vp(small, 8), None)
+        for i in range(log_n):
+            call("vx_gkr_gen_eq_evals_pass", vp(eq), vp(small, 12 + 4 * (log_n - 1 - i)), 1 << i, None)
+    # a pass reads and writes 2^i elements and writes 2^i more: 48 * 2^i bytes
+    legs["gen_eq_evals_plugin_passes"] = (eq_passes, 48 * (n - 1) + 16)
+    res = {"workload": f"QM31 columns of 2^{log_n} elements in HBM (sum: 2^{log_n - 2} terms of a LogUp layer)"}
+    for name, (fn, nbytes) in legs.items():
+        for _ in range(2):
+            fn()
+        call("vx_device_sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        call("vx_device_sync")
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    res["steps"] = steps
+    return res
+
+
 def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     """stwo-basis circle NTT (the stwo plugin's PolyOps): evaluate and
     interpolate of one 2^log_n column resident in HBM, asynchronous calls
@@ -960,6 +1018,8 @@ def main():
     ap.add_argument("--no-bytecode", action="store_true")
     ap.add_argument("--bytecode-only", action="store_true")
     ap.add_argument("--stwo-only", action="store_true")
+    ap.add_argument("--no-gkr", action="store_true")
+    ap.add_argument("--gkr-only", action="store_true")
     ap.add_argument("--profile-serving-mix", action="store_true",
                     help="eager proofs of every serving class (7 per Fibonacci class, 9 per synthetic class) "
                          "for a kernel-trace profile of the mix")
@@ -996,6 +1056,9 @@ def main():
         return
     if args.cairo_only:
         print(json.dumps({"cairo_commit": cairo_commit(vx, args.steps)}), flush=True)
+        return
+    if args.gkr_only:
+        print(json.dumps({"gkr": gkr(vx, args.steps)}), flush=True)
         return
     if args.stwo_only:
         print(json.dumps({"stwo_ntt": stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)}), flush=True)
@@ -1104,6 +1167,8 @@ def main():
         out["bytecode_eval"] = bytecode_eval(vx, args.steps)
     if not args.no_stwo and not args.profile:
         out["stwo_ntt"] = stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)
+    if not args.no_gkr and not args.profile:
+        out["gkr"] = gkr(vx, args.steps)
     if not args.no_cpu_baseline and gpu_a is not None:
         out["cpu_baseline"] = cpu_baseline(gpu_a, digest)
     prover.close()

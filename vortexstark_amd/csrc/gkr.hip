@@ -12,6 +12,8 @@
 // once, then across the wavefront and the workgroup; a second launch folds
 // the workgroups' partial sums into the first block, so the caller reads
 // one block.
+#include <atomic>
+
 #include "vx_internal.h"
 
 namespace vx {
@@ -19,7 +21,6 @@ namespace vx {
 namespace {
 
 constexpr uint32_t GKR_THREADS = 256;
-constexpr uint32_t GKR_SUM_WGS = 2048;  // partial sums of one launch at most
 
 template <bool A16>
 __device__ __forceinline__ qm31 ldqe(const uint32_t* p, uint64_t i) {
@@ -93,27 +94,30 @@ __global__ __launch_bounds__(GKR_THREADS) void k_gkr_eq_pass(uint32_t* buf, cons
 
 // The whole eq table in one launch. Entry x of the reference's passes is
 // v prod_i (bit i of x ? y[k-1-i] : 1 - y[k-1-i]), a product that splits
-// over the index bits: each workgroup owns the 2^kl entries with one value
-// H of the high bits, builds the low-bit factor table in LDS by the same
-// doubling passes (from 1), forms v times the high-bit factors once, and
-// writes their products. Field products are exact, so the entries equal
-// the pass-by-pass ones.
+// over the index bits: each workgroup owns GKR_EQ_HPW consecutive values H
+// of the high bits (2^kl entries each), builds the low-bit factor table in
+// LDS by the same doubling passes (from 1), forms v times the high-bit
+// factors of its H values (one lane each), and writes their products, so
+// every entry costs one product and one 16-byte store. Field products are
+// exact, so the entries equal the pass-by-pass ones.
 constexpr uint32_t GKR_EQ_LO_MAX = 10;
+constexpr uint32_t GKR_EQ_HPW = 16;
 __global__ __launch_bounds__(GKR_THREADS) void k_gkr_eq_table(uint32_t* buf, const uint32_t* y, uint32_t k,
-                                                             uint32_t kl, const uint32_t* v, bool a16) {
+                                                             uint32_t kl, uint32_t hpw, const uint32_t* v, bool a16) {
   __shared__ uint4 lo[1u << GKR_EQ_LO_MAX];
-  __shared__ uint4 hi;
+  __shared__ uint4 hi[GKR_EQ_HPW];
   const uint32_t t = threadIdx.x;
-  const uint64_t H = blockIdx.x;
-  if (t == 0) {
+  const uint64_t H0 = (uint64_t)blockIdx.x * hpw;
+  if (t < hpw) {
+    const uint64_t H = H0 + t;
     qm31 acc = ldq_arg(v);
     for (uint32_t i = kl; i < k; i++) {
       const qm31 yi = ldq_arg(y + 4 * (k - 1 - i));
       acc = qm_mul_lazy(acc, ((H >> (i - kl)) & 1) ? yi : qm_one_minus(yi));
     }
-    hi = make_uint4(acc.v[0], acc.v[1], acc.v[2], acc.v[3]);
-    lo[0] = make_uint4(1u, 0u, 0u, 0u);
+    hi[t] = make_uint4(acc.v[0], acc.v[1], acc.v[2], acc.v[3]);
   }
+  if (t == 0) lo[0] = make_uint4(1u, 0u, 0u, 0u);
   __syncthreads();
   for (uint32_t i = 0; i < kl; i++) {
     const uint32_t cur = 1u << i;
@@ -126,13 +130,12 @@ __global__ __launch_bounds__(GKR_THREADS) void k_gkr_eq_table(uint32_t* buf, con
     }
     __syncthreads();
   }
-  const qm31 hq{{hi.x, hi.y, hi.z, hi.w}};
-  const uint64_t base = H << kl;
-  for (uint32_t j = t; j < (1u << kl); j += GKR_THREADS) {
-    const uint4 e = lo[j];
-    const qm31 o = qm_mul_lazy(hq, {{e.x, e.y, e.z, e.w}});
-    if (a16) stqe<true>(buf, base + j, o);
-    else stqe<false>(buf, base + j, o);
+  const uint32_t per = hpw << kl;
+  for (uint32_t idx = t; idx < per; idx += GKR_THREADS) {
+    const uint4 h = hi[idx >> kl], e = lo[idx & ((1u << kl) - 1)];
+    const qm31 o = qm_mul_lazy({{h.x, h.y, h.z, h.w}}, {{e.x, e.y, e.z, e.w}});
+    if (a16) stqe<true>(buf, (H0 << kl) + idx, o);
+    else stqe<false>(buf, (H0 << kl) + idx, o);
   }
 }
 
@@ -177,35 +180,51 @@ __global__ __launch_bounds__(GKR_THREADS) void k_gkr_layer(const uint32_t* num, 
 // pair at t = 0 is (2i, 2i + 1), at t = 1 (2(n + i), 2(n + i) + 1), and at
 // t = 2 each element is 2 (t = 1) - (t = 0); F is the pair's product or its
 // fraction sum num + lambda den; sum0 += eq[i] F(t = 0), sum2 += eq[i] F(t = 2).
+// (Loading the next term while computing this one took the LogUp kernel
+// from 74 to 110 VGPRs and did not make it faster.)
+struct GkrTerm {
+  qm31 e, x00, x01, x10, x11, d00, d01, d10, d11;
+};
 template <int KIND, bool A16>
-__device__ __forceinline__ void gkr_term(const uint32_t* eq, const uint32_t* num, const uint32_t* den, const qm31& lam,
-                                         uint64_t n, uint64_t i, uint64_t (&s0)[4], uint64_t (&s2)[4]) {
+__device__ __forceinline__ void gkr_load(const uint32_t* eq, const uint32_t* num, const uint32_t* den, uint64_t n,
+                                         uint64_t i, GkrTerm& g) {
   const uint64_t a = 2 * i, b = 2 * (n + i);
+  g.e = ldqe<A16>(eq, i);
+  if constexpr (KIND != GKR_SINGLES) {
+    g.x00 = ld_num<KIND, A16>(num, a);
+    g.x01 = ld_num<KIND, A16>(num, a + 1);
+    g.x10 = ld_num<KIND, A16>(num, b);
+    g.x11 = ld_num<KIND, A16>(num, b + 1);
+  }
+  if constexpr (KIND != GKR_PROD) {
+    g.d00 = ldqe<A16>(den, a);
+    g.d01 = ldqe<A16>(den, a + 1);
+    g.d10 = ldqe<A16>(den, b);
+    g.d11 = ldqe<A16>(den, b + 1);
+  }
+}
+template <int KIND>
+__device__ __forceinline__ void gkr_add_term(const GkrTerm& g, const qm31& lam, uint64_t (&s0)[4], uint64_t (&s2)[4]) {
   qm31 f0, f2;
   auto twice_minus = [](const qm31& x1, const qm31& x0) { return qm_sub(qm_add(x1, x1), x0); };
   if constexpr (KIND == GKR_PROD) {
-    const qm31 x00 = ldqe<A16>(num, a), x01 = ldqe<A16>(num, a + 1);
-    const qm31 x10 = ldqe<A16>(num, b), x11 = ldqe<A16>(num, b + 1);
-    f0 = qm_mul_lazy(x00, x01);
-    f2 = qm_mul_lazy(twice_minus(x10, x00), twice_minus(x11, x01));
+    f0 = qm_mul_lazy(g.x00, g.x01);
+    f2 = qm_mul_lazy(twice_minus(g.x10, g.x00), twice_minus(g.x11, g.x01));
   } else {
-    const qm31 d00 = ldqe<A16>(den, a), d01 = ldqe<A16>(den, a + 1);
-    const qm31 d10 = ldqe<A16>(den, b), d11 = ldqe<A16>(den, b + 1);
-    const qm31 n00 = ld_num<KIND, A16>(num, a), n01 = ld_num<KIND, A16>(num, a + 1);
-    const qm31 n10 = ld_num<KIND, A16>(num, b), n11 = ld_num<KIND, A16>(num, b + 1);
-    const qm31 d20 = twice_minus(d10, d00), d21 = twice_minus(d11, d01);
-    qm31 n20, n21;
+    const qm31 d20 = twice_minus(g.d10, g.d00), d21 = twice_minus(g.d11, g.d01);
+    qm31 n00, n01, n20, n21;
     if constexpr (KIND == GKR_SINGLES) {
-      n20 = n21 = n00;  // 2 * 1 - 1
+      n00 = n01 = n20 = n21 = qm31{{1u, 0u, 0u, 0u}};  // 2 * 1 - 1 = 1
     } else {
-      n20 = twice_minus(n10, n00);
-      n21 = twice_minus(n11, n01);
+      n00 = g.x00;
+      n01 = g.x01;
+      n20 = twice_minus(g.x10, g.x00);
+      n21 = twice_minus(g.x11, g.x01);
     }
-    f0 = qm_add(frac_num<KIND>(n00, d00, n01, d01), qm_mul_lazy(lam, qm_mul_lazy(d00, d01)));
+    f0 = qm_add(frac_num<KIND>(n00, g.d00, n01, g.d01), qm_mul_lazy(lam, qm_mul_lazy(g.d00, g.d01)));
     f2 = qm_add(frac_num<KIND>(n20, d20, n21, d21), qm_mul_lazy(lam, qm_mul_lazy(d20, d21)));
   }
-  const qm31 e = ldqe<A16>(eq, i);
-  const qm31 t0 = qm_mul_lazy(e, f0), t2 = qm_mul_lazy(e, f2);
+  const qm31 t0 = qm_mul_lazy(g.e, f0), t2 = qm_mul_lazy(g.e, f2);
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     s0[c] += t0.v[c];
@@ -227,8 +246,12 @@ __global__ __launch_bounds__(GKR_THREADS) void k_gkr_sum(const uint32_t* eq, con
   const qm31 lam = KIND == GKR_PROD ? qm31{{0u, 0u, 0u, 0u}} : ldq_arg(lam_p);
   uint64_t s0[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
   const uint64_t stride = (uint64_t)gridDim.x * GKR_THREADS;
-  for (uint64_t i = (uint64_t)blockIdx.x * GKR_THREADS + threadIdx.x; i < n; i += stride)
-    gkr_term<KIND, A16>(eq, num, den, lam, n, i, s0, s2);
+  uint64_t i = (uint64_t)blockIdx.x * GKR_THREADS + threadIdx.x;
+  for (; i < n; i += stride) {
+    GkrTerm g;
+    gkr_load<KIND, A16>(eq, num, den, n, i, g);
+    gkr_add_term<KIND>(g, lam, s0, s2);
+  }
   const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
@@ -269,6 +292,27 @@ __global__ __launch_bounds__(GKR_THREADS) void k_gkr_sum_blocks(uint32_t* partia
 }
 
 bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Workgroups of kernel K resident on the whole device at once (occupancy x
+// CUs), once per device: a grid-stride launch of exactly that many runs in
+// one round, with no partly filled second round.
+template <typename K>
+uint32_t resident_wgs(K kernel) {
+  static std::atomic<uint32_t> cache[64];
+  int dev = 0;
+  VX_HIP(hipGetDevice(&dev));
+  require(dev >= 0 && dev < 64, "gkr: device index out of range");
+  uint32_t n = cache[dev].load(std::memory_order_relaxed);
+  if (!n) {
+    int per_cu = 0, cus = 0;
+    VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), GKR_THREADS,
+                                                        0));
+    VX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    n = (uint32_t)((per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1));
+    cache[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
 dim3 grid_for(uint64_t n) { return dim3((uint32_t)((n + GKR_THREADS - 1) / GKR_THREADS)); }
 
 }  // namespace
@@ -307,7 +351,9 @@ void gkr_gen_eq_evals(uint32_t* buf, const uint32_t* y, uint32_t k, const uint32
   require(buf && v && (k == 0 || y), "gkr_gen_eq_evals: null pointer");
   require(k <= 31, "gkr_gen_eq_evals: at most 31 variables");
   const uint32_t kl = k < GKR_EQ_LO_MAX ? k : GKR_EQ_LO_MAX;
-  hipLaunchKernelGGL(k_gkr_eq_table, dim3((uint32_t)(1ull << (k - kl))), dim3(GKR_THREADS), 0, s, buf, y, k, kl, v,
+  const uint64_t n_hi = 1ull << (k - kl);
+  const uint32_t hpw = (uint32_t)(n_hi < GKR_EQ_HPW ? n_hi : GKR_EQ_HPW);
+  hipLaunchKernelGGL(k_gkr_eq_table, dim3((uint32_t)(n_hi / hpw)), dim3(GKR_THREADS), 0, s, buf, y, k, kl, hpw, v,
                      al16(buf));
   VX_HIP(hipGetLastError());
 }
@@ -342,15 +388,23 @@ uint32_t gkr_sum_poly(int kind, const uint32_t* eq, const uint32_t* num, const u
   require(n_terms >= 1 && n_terms <= (1ull << 31), "gkr_sum_poly: n_terms must be 1..2^31");
   require(eq && partial && (kind == GKR_PROD ? num != nullptr : (den && lambda && (kind == GKR_SINGLES || num))),
           "gkr_sum_poly: null pointer");
-  // at most the ceil(n_terms / 256) blocks the caller's partial buffer holds
+  // at most the ceil(n_terms / 256) blocks the caller's partial buffer
+  // holds, and at most what the device holds at once
   const uint64_t need = (n_terms + GKR_THREADS - 1) / GKR_THREADS;
-  const uint32_t wgs = (uint32_t)(need < GKR_SUM_WGS ? need : GKR_SUM_WGS);
   const bool a = al16(eq) && (kind == GKR_PROD || al16(den)) && (kind != GKR_PROD && kind != GKR_GENERIC ? true : al16(num));
+  uint32_t wgs = 1;
 #define VX_GKR_SUM(K)                                                                                            \
-  if (a) hipLaunchKernelGGL((k_gkr_sum<K, true>), dim3(wgs), dim3(GKR_THREADS), 0, s, eq, num, den, lambda,        \
-                            partial, n_terms);                                                                   \
-  else hipLaunchKernelGGL((k_gkr_sum<K, false>), dim3(wgs), dim3(GKR_THREADS), 0, s, eq, num, den, lambda,         \
-                          partial, n_terms);
+  if (a) {                                                                                                       \
+    const uint64_t r = resident_wgs(k_gkr_sum<K, true>);                                                         \
+    wgs = (uint32_t)(need < r ? need : r);                                                                       \
+    hipLaunchKernelGGL((k_gkr_sum<K, true>), dim3(wgs), dim3(GKR_THREADS), 0, s, eq, num, den, lambda, partial,    \
+                       n_terms);                                                                                 \
+  } else {                                                                                                       \
+    const uint64_t r = resident_wgs(k_gkr_sum<K, false>);                                                        \
+    wgs = (uint32_t)(need < r ? need : r);                                                                       \
+    hipLaunchKernelGGL((k_gkr_sum<K, false>), dim3(wgs), dim3(GKR_THREADS), 0, s, eq, num, den, lambda, partial,   \
+                       n_terms);                                                                                 \
+  }
   switch (kind) {
     case GKR_PROD: VX_GKR_SUM(GKR_PROD) break;
     case GKR_GENERIC: VX_GKR_SUM(GKR_GENERIC) break;
