@@ -894,24 +894,34 @@ __device__ __forceinline__ void br_tile(const uint32_t* src, uint32_t* dst, uint
 
 // The same tile with 16 bytes per lane on both sides: W / 4 lanes per row,
 // and a lane writes destination words rev(a) = 4u .. 4u + 3 of its row,
-// taken from four tile rows a.
+// taken from four tile rows a. Split into the loads of a tile (into
+// registers) and the rest, so that the loop below can load the next tile of
+// a column while this one goes through LDS and out.
+constexpr uint32_t BR_VEC_LOADS = 4;  // 16-byte loads per lane of a 64 x 64 tile
 template <uint32_t T>
-__device__ __forceinline__ void br_tile_vec(const uint32_t* src, uint32_t* dst, uint32_t E, uint64_t m,
-                                            uint32_t (*tile)[(1u << BR_BIG) + 1]) {
+__device__ __forceinline__ void br_load_vec(const uint32_t* src, uint32_t E, uint64_t m, uint4 (&v)[BR_VEC_LOADS]) {
+  constexpr uint32_t W = 1u << T, LPR = W / 4, ROWS = 256 / LPR;
+  const uint32_t t = threadIdx.x, c4 = 4 * (t % LPR);
+  const uint64_t step = (uint64_t)ROWS << (E - T);
+  const uint32_t* sp = src + (((uint64_t)(t / LPR) << (E - T)) | (m << T) | c4);
+#pragma unroll
+  for (uint32_t r = 0; r < W / ROWS; r++) v[r] = *reinterpret_cast<const uint4*>(sp + r * step);
+}
+template <uint32_t T>
+__device__ __forceinline__ void br_store_vec(uint32_t* dst, uint32_t E, uint64_t m, const uint4 (&v)[BR_VEC_LOADS],
+                                             uint32_t (*tile)[(1u << BR_BIG) + 1]) {
   constexpr uint32_t W = 1u << T, LPR = W / 4, ROWS = 256 / LPR;
   const uint32_t mb = E - 2 * T;
   const uint64_t rm = mb ? (__brevll(m) >> (64 - mb)) : 0;
   const uint32_t t = threadIdx.x, c4 = 4 * (t % LPR);
   const uint64_t step = (uint64_t)ROWS << (E - T);
-  const uint32_t* sp = src + (((uint64_t)(t / LPR) << (E - T)) | (m << T) | c4);
 #pragma unroll
   for (uint32_t r = 0; r < W / ROWS; r++) {
-    const uint4 v = *reinterpret_cast<const uint4*>(sp + r * step);
     uint32_t* row = tile[(t / LPR) + ROWS * r] + c4;
-    row[0] = v.x;
-    row[1] = v.y;
-    row[2] = v.z;
-    row[3] = v.w;
+    row[0] = v[r].x;
+    row[1] = v[r].y;
+    row[2] = v[r].z;
+    row[3] = v[r].w;
   }
   __syncthreads();
   uint32_t* dp = dst + (((uint64_t)(t / LPR) << (E - T)) | (rm << T) | c4);
@@ -928,7 +938,8 @@ __device__ __forceinline__ void br_tile_vec(const uint32_t* src, uint32_t* dst, 
 
 // Workgroup w runs the contiguous tiles [w chunk, (w + 1) chunk) of the
 // whole list, so its job changes rarely and is found by one binary search
-// and then a forward scan (scalar loads).
+// and then a forward scan (scalar loads). Within a 16-byte-aligned column
+// the next tile's loads are issued before this tile is written out.
 __global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint32_t n_jobs, uint64_t total,
                                                      uint64_t chunk) {
   __shared__ uint32_t tile[1u << BR_BIG][(1u << BR_BIG) + 1];
@@ -941,16 +952,36 @@ __global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint
     if (ld_uniform(&jobs[mid].first_tile) <= b0) lo = mid;
     else hi = mid;
   }
+  uint4 cur[BR_VEC_LOADS], nxt[BR_VEC_LOADS];
+  bool have = false;  // cur holds tile b's loads
   for (uint64_t b = b0; b < b1; b++) {
     while (lo + 1 < n_jobs && ld_uniform(&jobs[lo + 1].first_tile) <= b) lo++;
     uniform_t<BitrevJob>& J = as_uniform(jobs + lo);
     const uint32_t E = J.log_n;
+    const uint64_t m = b - J.first_tile;
     if (J.vec) {
-      if (E >= 2 * BR_BIG) br_tile_vec<BR_BIG>(J.src, J.dst, E, b - J.first_tile, tile);
-      else br_tile_vec<BR_SMALL>(J.src, J.dst, E, b - J.first_tile, tile);
+      // the next tile, when it is in this column (end of the column: the
+      // next job's first tile)
+      const uint64_t end = lo + 1 < n_jobs ? ld_uniform(&jobs[lo + 1].first_tile) : total;
+      const bool pf = b + 1 < b1 && b + 1 < end;
+      if (E >= 2 * BR_BIG) {
+        if (!have) br_load_vec<BR_BIG>(J.src, E, m, cur);
+        if (pf) br_load_vec<BR_BIG>(J.src, E, m + 1, nxt);
+        br_store_vec<BR_BIG>(J.dst, E, m, cur, tile);
+      } else {
+        if (!have) br_load_vec<BR_SMALL>(J.src, E, m, cur);
+        if (pf) br_load_vec<BR_SMALL>(J.src, E, m + 1, nxt);
+        br_store_vec<BR_SMALL>(J.dst, E, m, cur, tile);
+      }
+      if (pf) {
+#pragma unroll
+        for (uint32_t r = 0; r < BR_VEC_LOADS; r++) cur[r] = nxt[r];
+      }
+      have = pf;
     } else {
-      if (E >= 2 * BR_BIG) br_tile<BR_BIG>(J.src, J.dst, E, b - J.first_tile, tile);
-      else br_tile<BR_SMALL>(J.src, J.dst, E, b - J.first_tile, tile);
+      if (E >= 2 * BR_BIG) br_tile<BR_BIG>(J.src, J.dst, E, m, tile);
+      else br_tile<BR_SMALL>(J.src, J.dst, E, m, tile);
+      have = false;
     }
   }
 }
