@@ -180,8 +180,12 @@ __global__ __launch_bounds__(GKR_THREADS) void k_gkr_layer(const uint32_t* num, 
 // pair at t = 0 is (2i, 2i + 1), at t = 1 (2(n + i), 2(n + i) + 1), and at
 // t = 2 each element is 2 (t = 1) - (t = 0); F is the pair's product or its
 // fraction sum num + lambda den; sum0 += eq[i] F(t = 0), sum2 += eq[i] F(t = 2).
-// (Loading the next term while computing this one took the LogUp kernel
-// from 74 to 110 VGPRs and did not make it faster.)
+// (A LogUp term is 751 VALU instructions for 144 bytes, so the loads and
+// the arithmetic of one grid round take about as long as each other; with
+// every wave loading, then computing, the kernel reaches about half of
+// either peak. Loading the next term into a second register set while
+// computing this one was rescheduled by the compiler into the same
+// wait-then-compute order, so it was not kept.)
 struct GkrTerm {
   qm31 e, x00, x01, x10, x11, d00, d01, d10, d11;
 };
