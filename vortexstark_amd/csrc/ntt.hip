@@ -938,8 +938,9 @@ __device__ __forceinline__ void br_store_vec(uint32_t* dst, uint32_t E, uint64_t
 
 // Workgroup w runs the contiguous tiles [w chunk, (w + 1) chunk) of the
 // whole list, so its job changes rarely and is found by one binary search
-// and then a forward scan (scalar loads). Within a 16-byte-aligned column
-// the next tile's loads are issued before this tile is written out.
+// and then a forward scan (scalar loads). Within a run of 64 x 64 tiles of a
+// 16-byte-aligned column the next tile's loads are issued before this tile
+// is written out.
 __global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint32_t n_jobs, uint64_t total,
                                                      uint64_t chunk) {
   __shared__ uint32_t tile[1u << BR_BIG][(1u << BR_BIG) + 1];
@@ -952,36 +953,40 @@ __global__ __launch_bounds__(256) void k_bitrev_list(const BitrevJob* jobs, uint
     if (ld_uniform(&jobs[mid].first_tile) <= b0) lo = mid;
     else hi = mid;
   }
-  uint4 cur[BR_VEC_LOADS], nxt[BR_VEC_LOADS];
-  bool have = false;  // cur holds tile b's loads
-  for (uint64_t b = b0; b < b1; b++) {
+  uint64_t b = b0;
+  while (b < b1) {
     while (lo + 1 < n_jobs && ld_uniform(&jobs[lo + 1].first_tile) <= b) lo++;
     uniform_t<BitrevJob>& J = as_uniform(jobs + lo);
     const uint32_t E = J.log_n;
-    const uint64_t m = b - J.first_tile;
-    if (J.vec) {
-      // the next tile, when it is in this column (end of the column: the
-      // next job's first tile)
-      const uint64_t end = lo + 1 < n_jobs ? ld_uniform(&jobs[lo + 1].first_tile) : total;
-      const bool pf = b + 1 < b1 && b + 1 < end;
-      if (E >= 2 * BR_BIG) {
-        if (!have) br_load_vec<BR_BIG>(J.src, E, m, cur);
-        if (pf) br_load_vec<BR_BIG>(J.src, E, m + 1, nxt);
+    // this workgroup's run of tiles in column J
+    const uint64_t end = lo + 1 < n_jobs ? ld_uniform(&jobs[lo + 1].first_tile) : total;
+    const uint64_t run_end = end < b1 ? end : b1;
+    if (J.vec && E >= 2 * BR_BIG) {
+      // software pipeline over the run: the next tile's loads are issued
+      // (unconditionally: the last tile reloads itself) before this tile
+      // goes through LDS and out
+      uint4 cur[BR_VEC_LOADS], nxt[BR_VEC_LOADS];
+      br_load_vec<BR_BIG>(J.src, E, b - J.first_tile, cur);
+      for (; b < run_end; b++) {
+        const uint64_t m = b - J.first_tile;
+        br_load_vec<BR_BIG>(J.src, E, b + 1 < run_end ? m + 1 : m, nxt);
         br_store_vec<BR_BIG>(J.dst, E, m, cur, tile);
-      } else {
-        if (!have) br_load_vec<BR_SMALL>(J.src, E, m, cur);
-        if (pf) br_load_vec<BR_SMALL>(J.src, E, m + 1, nxt);
-        br_store_vec<BR_SMALL>(J.dst, E, m, cur, tile);
-      }
-      if (pf) {
 #pragma unroll
         for (uint32_t r = 0; r < BR_VEC_LOADS; r++) cur[r] = nxt[r];
       }
-      have = pf;
     } else {
-      if (E >= 2 * BR_BIG) br_tile<BR_BIG>(J.src, J.dst, E, m, tile);
-      else br_tile<BR_SMALL>(J.src, J.dst, E, m, tile);
-      have = false;
+      for (; b < run_end; b++) {
+        const uint64_t m = b - J.first_tile;
+        if (J.vec) {
+          uint4 v[BR_VEC_LOADS];
+          br_load_vec<BR_SMALL>(J.src, E, m, v);
+          br_store_vec<BR_SMALL>(J.dst, E, m, v, tile);
+        } else if (E >= 2 * BR_BIG) {
+          br_tile<BR_BIG>(J.src, J.dst, E, m, tile);
+        } else {
+          br_tile<BR_SMALL>(J.src, J.dst, E, m, tile);
+        }
+      }
     }
   }
 }
