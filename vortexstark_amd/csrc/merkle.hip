@@ -314,6 +314,12 @@ __device__ __forceinline__ void lifted_leaf(const LeafSrc& L, uint64_t idx, uint
 // components accumulate in 64 bits as in d2_produce (fold after every four
 // columns); four columns' loads are in flight together. Q is stored (SoA)
 // for the fold.
+// Two words at a 32-bit byte offset from a wave-uniform global base.
+__device__ __forceinline__ uint2 ldg2_at(const uint32_t* base, uint32_t byte_off) {
+  const __attribute__((address_space(1))) char* b = (const __attribute__((address_space(1))) char*)base;
+  const __attribute__((address_space(1))) uint32_t* q = (const __attribute__((address_space(1))) uint32_t*)(b + byte_off);
+  return make_uint2(q[0], q[1]);
+}
 __device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i, G4& g, bool store = true) {
   const uint32_t S = L.lift_log, n = L.lift_n;
   uint64_t acc[4][4];
@@ -321,6 +327,41 @@ __device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i,
   for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[j][c] = 0;
+  if (S <= 30) {
+    // Rows and byte offsets fit 32 bits. Every column is two 8-byte loads
+    // (rows i, i+1 and i+2, i+3 of a column of the tree's size; rows b,
+    // b+1 twice of a smaller one) with the offsets picked by the uniform
+    // size test, so the four columns' loads of a group carry no branch and
+    // are all in flight before the first product.
+    const uint32_t i32 = (uint32_t)i;
+    for (uint32_t c0 = 0; c0 < n; c0 += 4) {
+      uint2 lo[4], hi[4];
+      uint4 bq[4];
+#pragma unroll
+      for (uint32_t h = 0; h < 4; h++) {
+        const uint32_t c = c0 + h < n ? c0 + h : n - 1;  // past the end: a column whose products are dropped
+        const LiftCol col = ld_uniform(L.lift + c);
+        bq[h] = c0 + h < n ? ld_uniform(reinterpret_cast<const uint4*>(L.beta + 4 * c)) : make_uint4(0, 0, 0, 0);
+        const bool full = col.log_n == S;
+        const uint32_t b = (i32 >> (S - col.log_n + 1)) << 3;
+        lo[h] = ldg2_at(col.p, full ? i32 << 2 : b);
+        hi[h] = ldg2_at(col.p, full ? (i32 << 2) + 8 : b);
+      }
+#pragma unroll
+      for (uint32_t h = 0; h < 4; h++) {
+        const uint32_t be[4] = {bq[h].x, bq[h].y, bq[h].z, bq[h].w};
+        const uint32_t v[4] = {lo[h].x, lo[h].y, hi[h].x, hi[h].y};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int k = 0; k < 4; k++) acc[j][k] += (uint64_t)be[k] * v[j];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[j][k] = fold64(acc[j][k]);
+    }
+  } else
   for (uint32_t c0 = 0; c0 < n; c0 += 4) {
 #pragma unroll
     for (uint32_t h = 0; h < 4; h++) {
