@@ -742,6 +742,53 @@ def gkr(vx, steps, log_n=24):
     return res
 
 
+def poseidon(vx, steps):
+    """Poseidon252 operations of the stwo plugin (the Stark-field Hades
+    permutation, one state per lane): a parent layer of 2^20 nodes, lifted
+    leaves over 32 columns of 2^16 rows (4 packed elements, 3 permutations
+    per leaf) and a grind batch of 2^22 nonces that no nonce passes (PoW
+    128), so every nonce is hashed. Reported as permutations per second."""
+    import ctypes as C
+
+    import numpy as np
+    from vortexstark_amd.device import DeviceBuffer
+
+    call = vx._lib.call
+    rng = np.random.default_rng(12)
+    # canonical field elements below 2^250 < p are valid Montgomery words
+    def felts(n):
+        w = rng.integers(0, 1 << 32, 8 * n, dtype=np.uint64).astype(np.uint32)
+        w[7::8] &= 0x03FFFFFF
+        return w
+    n_par, n_leaf, n_cols, n_grind = 1 << 20, 1 << 16, 32, 1 << 22
+    prev, par = DeviceBuffer.from_host(felts(2 * n_par)), DeviceBuffer.zeros(8 * n_par)
+    cols = [DeviceBuffer.from_host(rng.integers(0, P, n_leaf, dtype=np.uint32)) for _ in range(n_cols)]
+    ptrs = DeviceBuffer.from_host(np.array([c.as_ptr() for c in cols], np.uint64).view(np.uint32))
+    logs = DeviceBuffer.from_host(np.full(n_cols, 16, np.uint32))
+    leaves = DeviceBuffer.zeros(8 * n_leaf)
+    prefix = DeviceBuffer.from_host(felts(1))
+    best = DeviceBuffer.from_host(np.full(2, 0xFFFFFFFF, np.uint32))
+    vp = lambda b: C.c_void_p(b.as_ptr())
+    legs = {
+        "next_layer": (lambda: call("vx_build_next_layer_poseidon252", vp(prev), vp(par), n_par, None), n_par),
+        "leaves_32_cols": (lambda: call("vx_build_leaves_poseidon252", vp(ptrs), vp(logs), n_cols, 16, vp(leaves),
+                                        n_leaf, None), 3 * n_leaf),
+        "grind_batch": (lambda: call("vx_grind_pow_poseidon", vp(prefix), vp(best), 128, 0, n_grind, None), n_grind),
+    }
+    res = {"workload": "Poseidon252 (Stark field) permutations, one state per lane, inputs in HBM"}
+    for name, (fn, perms) in legs.items():
+        fn()
+        call("vx_device_sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        call("vx_device_sync")
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        res[name] = {"ms": round(ms, 4), "permutations_per_s": round(perms / (ms * 1e-3), 1)}
+    res["steps"] = steps
+    return res
+
+
 def stwo_ntt(vx, steps, log_n=24, batch_log=20, batch_cols=16, cpu=True):
     """stwo-basis circle NTT (the stwo plugin's PolyOps): evaluate and
     interpolate of one 2^log_n column resident in HBM, asynchronous calls
@@ -1020,6 +1067,8 @@ def main():
     ap.add_argument("--stwo-only", action="store_true")
     ap.add_argument("--no-gkr", action="store_true")
     ap.add_argument("--gkr-only", action="store_true")
+    ap.add_argument("--no-poseidon", action="store_true")
+    ap.add_argument("--poseidon-only", action="store_true")
     ap.add_argument("--profile-serving-mix", action="store_true",
                     help="eager proofs of every serving class (7 per Fibonacci class, 9 per synthetic class) "
                          "for a kernel-trace profile of the mix")
@@ -1059,6 +1108,9 @@ def main():
         return
     if args.gkr_only:
         print(json.dumps({"gkr": gkr(vx, args.steps)}), flush=True)
+        return
+    if args.poseidon_only:
+        print(json.dumps({"poseidon252": poseidon(vx, args.steps)}), flush=True)
         return
     if args.stwo_only:
         print(json.dumps({"stwo_ntt": stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)}), flush=True)
@@ -1169,6 +1221,8 @@ def main():
         out["stwo_ntt"] = stwo_ntt(vx, args.steps, cpu=not args.no_cpu_baseline)
     if not args.no_gkr and not args.profile:
         out["gkr"] = gkr(vx, args.steps)
+    if not args.no_poseidon and not args.profile:
+        out["poseidon252"] = poseidon(vx, args.steps)
     if not args.no_cpu_baseline and gpu_a is not None:
         out["cpu_baseline"] = cpu_baseline(gpu_a, digest)
     prover.close()

@@ -444,6 +444,42 @@ int vx_grind_pow_m31_output(const uint32_t *d_prefixed_digest, uint64_t *d_resul
 int vx_grind_pow_stwo_m31(const uint32_t *d_prefixed_digest, uint32_t pow_bits, uint64_t *d_result,
                           vx_stream_t stream);
 
+/* Poseidon252 (the Stark field p = 2^251 + 17 * 2^192 + 1). An element is
+ * 4 little-endian u64 limbs of its Montgomery form a * 2^256 mod p
+ * (starknet-ff's internal form, the plugin's CudaColumn<FieldElement252>). */
+
+/* Replaces cuda_grind_pow_poseidon (ffi.rs:1205-1212,
+ * cuda/grind_poseidon.cu:24-93): tests nonces [batch_offset, batch_offset +
+ * n_threads) against poseidon_hash(prefixed, nonce), counting the trailing
+ * zeros of the hash's low 128 bits, and atomically lowers *d_result (caller
+ * initialises it to UINT64_MAX). d_prefixed_digest_mont = poseidon_hash_many(
+ * [POW_PREFIX, digest, pow_bits]) in Montgomery form (grind_ops.rs:273-280).
+ * Unlike the reference, the result is the smallest valid nonce of the batch
+ * (no racing early exit). pow_bits <= 128. */
+int vx_grind_pow_poseidon(const uint64_t *d_prefixed_digest_mont, uint64_t *d_result, uint32_t pow_bits,
+                          uint64_t batch_offset, uint32_t n_threads, vx_stream_t stream);
+/* The plugin's GrindOps for Poseidon252Channel as one call
+ * (grind_ops.rs:263-348, which loops on the host over cuda_grind_pow_poseidon
+ * batches of 2^20 from nonce 0): *d_result = the smallest valid nonce; 0 when
+ * pow_bits = 0. pow_bits <= 40. Async, except for pow_bits = 0. */
+int vx_grind_pow_stwo_poseidon(const uint64_t *d_prefixed_digest_mont, uint32_t pow_bits, uint64_t *d_result,
+                               vx_stream_t stream);
+/* Replaces build_leaves_poseidon252 (ffi.rs:948-963,
+ * cuda/merkle_poseidon252.cu:37-123, caller merkle_ops.rs:247-311): leaf i =
+ * poseidon_hash_many of the M31 values of every column at its lifted row
+ * (row i for a column of 2^lifting_log_size rows, else
+ * ((i >> (lifting_log_size - log + 1)) << 1) | (i & 1)), packed 8 to an
+ * element, first value most significant, a short last group adding its
+ * count times 2^248. d_col_ptrs and d_col_log_sizes are DEVICE arrays of
+ * n_cols entries, as the plugin uploads them. n_leaves <= 2^lifting_log_size;
+ * d_out holds 4 * n_leaves u64. */
+int vx_build_leaves_poseidon252(const uint32_t *const *d_col_ptrs, const uint32_t *d_col_log_sizes, uint32_t n_cols,
+                                uint32_t lifting_log_size, uint64_t *d_out, uint32_t n_leaves, vx_stream_t stream);
+/* Replaces build_next_layer_poseidon252 (ffi.rs:965-974,
+ * cuda/merkle_poseidon252.cu:127-148): parent i = poseidon_hash(prev[2i],
+ * prev[2i + 1]) = permute([prev[2i], prev[2i + 1], 2])[0]. */
+int vx_build_next_layer_poseidon252(const uint64_t *d_prev, uint64_t *d_out, uint32_t n_parents, vx_stream_t stream);
+
 /* --------------------------------------------- prove / verify (src/prover.rs) */
 
 typedef struct vx_prover vx_prover;

@@ -120,6 +120,10 @@ SIGNATURES = {
     "vx_gkr_sum_poly_logup_mult": (ip, [vp, vp, vp, vp, vp, u32, vp, vp]),
     "vx_gkr_sum_poly_logup_singles": (ip, [vp, vp, vp, vp, u32, vp, vp]),
     "vx_grind_pow_stwo_m31": (ip, [vp, u32, vp, vp]),
+    "vx_grind_pow_poseidon": (ip, [vp, vp, u32, u64, u32, vp]),
+    "vx_grind_pow_stwo_poseidon": (ip, [vp, u32, vp, vp]),
+    "vx_build_leaves_poseidon252": (ip, [vp, vp, u32, u32, vp, u32, vp]),
+    "vx_build_next_layer_poseidon252": (ip, [vp, vp, u32, vp]),
     "vx_permute_hc_to_canonic_brt": (ip, [vp, vp, u32, u32, vp]),
     "vx_oods_sample": (ip, [vp, vp, vp, u32, u32, vp, vp, vp]),
     "vx_permute_canonic_brt_to_hc_natural": (ip, [vp, vp, u32, u32, vp]),

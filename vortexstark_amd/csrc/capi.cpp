@@ -809,6 +809,35 @@ int vx_grind_pow_stwo_m31(const uint32_t* prefix, uint32_t pow_bits, uint64_t* r
   return guard([&] { grind_stwo_call(prefix, pow_bits, result, true, S(st)); });
 }
 
+int vx_grind_pow_poseidon(const uint64_t* prefix_mont, uint64_t* result, uint32_t pow_bits, uint64_t batch_offset,
+                          uint32_t n_threads, vx_stream_t st) {
+  return guard([&] { poseidon_grind_batch(prefix_mont, result, pow_bits, batch_offset, n_threads, S(st)); });
+}
+
+int vx_grind_pow_stwo_poseidon(const uint64_t* prefix_mont, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
+  return guard([&] {
+    require(prefix_mont && result, "grind_pow_stwo_poseidon: null pointer");
+    if (pow_bits == 0) {
+      const uint64_t zero = 0;
+      VX_HIP(hipMemcpyAsync(result, &zero, 8, hipMemcpyHostToDevice, S(st)));
+      VX_HIP(hipStreamSynchronize(S(st)));
+      return;
+    }
+    poseidon_grind_min(prefix_mont, pow_bits, result, S(st));
+  });
+}
+
+int vx_build_leaves_poseidon252(const uint32_t* const* d_col_ptrs, const uint32_t* d_col_log_sizes, uint32_t n_cols,
+                                uint32_t lifting_log_size, uint64_t* d_out, uint32_t n_leaves, vx_stream_t st) {
+  return guard([&] {
+    poseidon_leaves(d_col_ptrs, d_col_log_sizes, n_cols, lifting_log_size, d_out, n_leaves, S(st));
+  });
+}
+
+int vx_build_next_layer_poseidon252(const uint64_t* d_prev, uint64_t* d_out, uint32_t n_parents, vx_stream_t st) {
+  return guard([&] { poseidon_next_layer(d_prev, d_out, n_parents, S(st)); });
+}
+
 int vx_grind_pow_min(const uint32_t* prefix, uint32_t pow_bits, uint64_t* result, vx_stream_t st) {
   return guard([&] {
     require(prefix && result, "grind_pow_min: null pointer");

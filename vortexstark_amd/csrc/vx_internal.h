@@ -455,6 +455,16 @@ void grind_pow(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, 
 void grind_pow_stwo(const uint32_t* d_prefix, uint32_t pow_bits, uint64_t* d_result, bool m31_out, hipStream_t s);
 void grind_pow_batch(const uint32_t* d_prefix, uint64_t* d_result, uint32_t pow_bits,
                      uint64_t batch_offset, uint64_t n_nonces, bool m31_out, hipStream_t s);
+// Poseidon252 (Stark field, Montgomery form 2^256, elements as 4 x u64):
+// the grind over nonces [batch_offset, batch_offset + n) lowering *d_result
+// to the smallest valid one; the whole search from nonce 0; lifted leaf
+// hashes; parent layers.
+void poseidon_grind_batch(const uint64_t* d_prefix_mont, uint64_t* d_result, uint32_t pow_bits, uint64_t batch_offset,
+                          uint64_t n, hipStream_t s);
+void poseidon_grind_min(const uint64_t* d_prefix_mont, uint32_t pow_bits, uint64_t* d_result, hipStream_t s);
+void poseidon_leaves(const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols, uint32_t lifting_log,
+                     uint64_t* d_out, uint32_t n_leaves, hipStream_t s);
+void poseidon_next_layer(const uint64_t* d_prev, uint64_t* d_out, uint32_t n_parents, hipStream_t s);
 void write_header(uint32_t* proof, const uint32_t* words, uint32_t n, hipStream_t s);
 // Copy `words` words from device memory into pinned host memory
 // (hipHostMalloc) with a kernel; both 16-byte aligned.
