@@ -51,28 +51,24 @@ __device__ __forceinline__ F252 f_r2() {
   return f_const(v);
 }
 
-// t in [0, 2p) -> t mod p.
+// t in [0, 2p) -> t mod p. Carry and borrow chains use the 32-bit carry
+// builtins (v_add_co / v_addc_co, v_sub_co / v_subb_co), no 64-bit temporaries.
+// t - p is computed as t + (2^256 - p), whose carry out is set exactly when
+// t >= p (an add chain; a borrow chain against p's zero words compiles to
+// three instructions per word).
 __device__ __forceinline__ void f_reduce_once(F252& t) {
-  uint32_t d[8], b = 0;
+  uint32_t d[8], c = 0;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const uint64_t s = (uint64_t)t.w[k] - pword(k) - b;
-    d[k] = (uint32_t)s;
-    b = (uint32_t)(s >> 63);
-  }
+  for (int k = 0; k < 8; k++) d[k] = __builtin_addc(t.w[k], ~pword(k) + (k == 0), c, &c);
 #pragma unroll
-  for (int k = 0; k < 8; k++) t.w[k] = b ? t.w[k] : d[k];
+  for (int k = 0; k < 8; k++) t.w[k] = c ? d[k] : t.w[k];
 }
 
 __device__ __forceinline__ F252 f_add(const F252& a, const F252& b) {
   F252 r;
-  uint64_t c = 0;
+  uint32_t c = 0;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const uint64_t s = (uint64_t)a.w[k] + b.w[k] + c;
-    r.w[k] = (uint32_t)s;
-    c = s >> 32;
-  }
+  for (int k = 0; k < 8; k++) r.w[k] = __builtin_addc(a.w[k], b.w[k], c, &c);
   f_reduce_once(r);  // a + b < 2p < 2^253: no carry out of word 7
   return r;
 }
@@ -81,67 +77,55 @@ __device__ __forceinline__ F252 f_sub(const F252& a, const F252& b) {
   F252 r;
   uint32_t br = 0;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const uint64_t s = (uint64_t)a.w[k] - b.w[k] - br;
-    r.w[k] = (uint32_t)s;
-    br = (uint32_t)(s >> 63);
-  }
+  for (int k = 0; k < 8; k++) r.w[k] = __builtin_subc(a.w[k], b.w[k], br, &br);
   const uint32_t mask = 0u - br;  // add p back on a borrow
-  uint64_t c = 0;
+  uint32_t c = 0;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const uint64_t s = (uint64_t)r.w[k] + (pword(k) & mask) + c;
-    r.w[k] = (uint32_t)s;
-    c = s >> 32;
-  }
+  for (int k = 0; k < 8; k++) r.w[k] = __builtin_addc(r.w[k], pword(k) & mask, c, &c);
   return r;
 }
 
-// Montgomery product a * b / 2^256 mod p (CIOS, one word of b per step).
-// After each step t < 2p, so t fits 9 words; the tenth holds the carry of
-// the multiply-add before the reduction.
+// Montgomery product a * b / 2^256 mod p, product scanning: column k of
+// a * b + m * p accumulates in a 64-bit word plus an overflow word (each
+// term one multiply-add into the accumulator and one add of its carry), so
+// no partial product is split or zero-extended. m_k = -(column k) mod 2^32
+// clears the column (p = 1 mod 2^32); m * p adds only at p's words 0, 6, 7.
+// The result is below 2p, then reduced once.
+__device__ __forceinline__ void mac(uint64_t& acc, uint32_t& ovf, uint32_t x, uint32_t y) {
+  const uint64_t prev = acc;
+  acc = (uint64_t)x * y + acc;  // one v_mad_u64_u32 into the accumulator
+  ovf += acc < prev;
+}
 __device__ __forceinline__ F252 f_mul(const F252& a, const F252& b) {
-  uint32_t t[10];
+  uint32_t m[8], r[8];
+  uint64_t acc = 0;
+  uint32_t ovf = 0;
 #pragma unroll
-  for (int k = 0; k < 10; k++) t[k] = 0;
+  for (int k = 0; k < 16; k++) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    uint64_t c = 0;
+    for (int j = 0; j < 8; j++)
+      if (k - j >= 0 && k - j < 8) mac(acc, ovf, a.w[j], b.w[k - j]);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const uint64_t s = (uint64_t)a.w[j] * b.w[i] + t[j] + c;
-      t[j] = (uint32_t)s;
-      c = s >> 32;
+      if (k - j == 6 && j < k) mac(acc, ovf, m[j], PW6);
+      if (k - j == 7 && j < k) mac(acc, ovf, m[j], PW7);
     }
-    uint64_t s = (uint64_t)t[8] + c;
-    t[8] = (uint32_t)s;
-    t[9] = (uint32_t)(s >> 32);
-    // m = -t[0] mod 2^32 makes word 0 of t + m p vanish (carry 1 unless
-    // t[0] = 0); shift down one word while adding m p's words 6 and 7.
-    const uint32_t m = 0u - t[0];
-    c = t[0] != 0;
-#pragma unroll
-    for (int j = 1; j < 6; j++) {
-      s = (uint64_t)t[j] + c;
-      t[j - 1] = (uint32_t)s;
-      c = s >> 32;
+    if (k < 8) {
+      m[k] = 0u - (uint32_t)acc;
+      unsigned long long c;
+      acc = __builtin_addcll(acc, (uint64_t)m[k], 0, &c);
+      ovf += (uint32_t)c;
+    } else {
+      r[k - 8] = (uint32_t)acc;
     }
-    s = (uint64_t)m * PW6 + t[6] + c;
-    t[5] = (uint32_t)s;
-    c = s >> 32;
-    s = (uint64_t)m * PW7 + t[7] + c;
-    t[6] = (uint32_t)s;
-    c = s >> 32;
-    s = (uint64_t)t[8] + c;
-    t[7] = (uint32_t)s;
-    t[8] = t[9] + (uint32_t)(s >> 32);
-    t[9] = 0;
+    acc = (acc >> 32) | ((uint64_t)ovf << 32);
+    ovf = 0;
   }
-  F252 r;
+  F252 t;
 #pragma unroll
-  for (int k = 0; k < 8; k++) r.w[k] = t[k];
-  f_reduce_once(r);  // t < 2p
-  return r;
+  for (int k = 0; k < 8; k++) t.w[k] = r[k];
+  f_reduce_once(t);
+  return t;
 }
 
 __device__ __forceinline__ F252 f_cube(const F252& x) { return f_mul(f_mul(x, x), x); }
