@@ -187,6 +187,33 @@ def test_cairo_quotient_commit_matches_oracle(log_n):
     assert np.array_equal(root, po.merkle_root(want))
 
 
+@pytest.mark.parametrize("log_n,store_lo,tile_log,lpt",
+                         [(13, 4, 12, 4), (13, 3, 11, 3), (13, 2, 10, 2), (14, 2, 8, 0), (12, 5, 12, 4)])
+def test_cairo_quotient_commit_every_tile_shape(tmp_path, log_n, store_lo, tile_log, lpt):
+    # each subtree shape pinned through the tuner's cache file, for a key
+    # (leaf kind 7, log size, stored levels) that no other test commits; the
+    # shapes with 4..16 leaves per lane compute the tile's rows first, one
+    # per lane, and hash them after the workgroup's barrier
+    import vortexstark_amd as vx
+    f = tmp_path / "tune.txt"
+    f.write_text(f"7 0 0 0 {log_n} {store_lo} {tile_log} {lpt}\n")
+    vx.tuner(True, str(f))
+    try:
+        ents = vx.tuner_entries()
+        assert any((e["leaf_kind"], e["log_n"], e["tile_log"], e["lane_log"]) == (7, log_n, tile_log, lpt)
+                   for e in ents), ents
+        n = 1 << log_n
+        cols, inter, alpha, chal, vh, tf = rand_inputs(300 + tile_log + lpt, n)
+        print("cairo commit shape", log_n, store_lo, tile_log, lpt, flush=True)
+        q, root = gpu_commit(cols, inter, alpha, chal, vh, tf, 4, log_n, store_lo)
+        want = po.cairo_quotient(cols, inter, alpha, vh, tf, chal, 4)
+        for k in range(4):
+            assert np.array_equal(q[k], want[k]), k
+        assert np.array_equal(root, po.merkle_root(want))
+    finally:
+        vx.tuner(True, None)
+
+
 def test_cairo_shaped_commitment_matches_oracle_digest(golden):
     # 34 trace columns committed as lo / hi / dict trees (16, 15, 3 columns,
     # src/cairo_air/prover.rs:341-350) and the fused quotient commitment,

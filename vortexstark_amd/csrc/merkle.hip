@@ -400,23 +400,19 @@ __device__ __forceinline__ void lifted_rlc_produce(const LeafSrc& L, uint64_t i,
   }
 }
 
-// Cairo VM AIR quotient of rows [i, i+4) (vx_cairo.h), stored (SoA) for the
-// FRI and hashed as QM31 leaves.
-// The row loop is not unrolled (each row is ~2,000 instructions); row j's
-// values go to g.w[j] through uniform selects, so g stays in registers.
-__device__ __forceinline__ void cairo_produce(const LeafSrc& L, uint64_t i, G4& g) {
-  uniform_t<CairoArgs>& a = as_uniform(L.cairo);
-#pragma unroll 1
-  for (int j = 0; j < 4; j++) {
-    const qm31 q = cairo_row(a, i + j);
+// Rows [i, i+4) of the tile's quotient, written by the workgroup's first
+// phase (k_subtree), read back from the output columns (L2-resident).
+__device__ __forceinline__ void cairo_readback(const LeafSrc& L, uint64_t i, G4& g) {
 #pragma unroll
-    for (int jj = 0; jj < 4; jj++)
-#pragma unroll
-      for (int c = 0; c < 4; c++) g.w[jj][c] = jj == j ? q.v[c] : g.w[jj][c];
+  for (int c = 0; c < 4; c++) {
+    const uint4 v = ld4(L.out[c] + i);
+    g.w[0][c] = v.x;
+    g.w[1][c] = v.y;
+    g.w[2][c] = v.z;
+    g.w[3][c] = v.w;
   }
-#pragma unroll
-  for (int c = 0; c < 4; c++) st4(L.out[c] + i, g.w[0][c], g.w[1][c], g.w[2][c], g.w[3][c]);
 }
+
 // One leaf per lane: the lane's own row, stored and hashed.
 __device__ __forceinline__ void cairo_leaf(const LeafSrc& L, uint64_t idx, uint32_t out[8]) {
   const qm31 q = cairo_row(as_uniform(L.cairo), idx);
@@ -489,7 +485,7 @@ __device__ __forceinline__ void lane_subtree(const LeafSrc& L, uint64_t first, u
     if constexpr (kGeneric) cols_load(L, i, gc);
     if constexpr (KIND == LEAF_D2_QUOTIENT) d2_produce(L, i, g);
     if constexpr (KIND == LEAF_LIFTED_RLC) lifted_rlc_produce(L, i, g);
-    if constexpr (KIND == LEAF_CAIRO_QUOTIENT) cairo_produce(L, i, g);
+    if constexpr (KIND == LEAF_CAIRO_QUOTIENT) cairo_readback(L, i, g);
     uint32_t a[8], b[8], n1a[8], n1b[8], n2[8];
     leaf_hash<KIND, NC>(L, g, gc, i, 0, a);
     leaf_hash<KIND, NC>(L, g, gc, i, 1, b);
@@ -670,6 +666,21 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(LeafSrc L, TreeLevels T
   const uint32_t tid = threadIdx.x;
   const uint64_t tile = blockIdx.x;
   uint32_t node[8];
+  if constexpr (KIND == LEAF_CAIRO_QUOTIENT && LPT_LOG >= 2) {
+    // First phase: the tile's quotient rows one per lane, so consecutive
+    // lanes read consecutive rows of every trace column (a lane computing
+    // its own 2^LPT_LOG rows would stride them); stored for the FRI, then
+    // read back by the hashing lanes after the barrier.
+    uniform_t<CairoArgs>& a = as_uniform(L.cairo);
+    const uint64_t base = tile << tile_log;
+#pragma unroll 1
+    for (uint32_t r = tid; r < (1u << tile_log); r += SUB_THREADS) {
+      const qm31 q = cairo_row(a, base + r);
+#pragma unroll
+      for (int c = 0; c < 4; c++) L.out[c][base + r] = q.v[c];
+    }
+    __syncthreads();
+  }
   if (tid < nthr) {
     const uint64_t first = (tile << tile_log) + ((uint64_t)tid << LPT_LOG);
     if constexpr (LPT_LOG == 0) lane_leaf<KIND, NC>(L, first, node);
