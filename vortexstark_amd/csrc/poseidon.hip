@@ -73,15 +73,41 @@ __device__ __forceinline__ F252 f_add(const F252& a, const F252& b) {
   return r;
 }
 
-__device__ __forceinline__ F252 f_sub(const F252& a, const F252& b) {
+// Lazy forms for the permutation's linear layer: sums without reduction
+// (every operand bound is stated where it is used), k p - x for x < k p, and
+// one reduction of a value below 16p to [0, 2p).
+__device__ __forceinline__ F252 f_add_nr(const F252& a, const F252& b) {
   F252 r;
-  uint32_t br = 0;
-#pragma unroll
-  for (int k = 0; k < 8; k++) r.w[k] = __builtin_subc(a.w[k], b.w[k], br, &br);
-  const uint32_t mask = 0u - br;  // add p back on a borrow
   uint32_t c = 0;
 #pragma unroll
-  for (int k = 0; k < 8; k++) r.w[k] = __builtin_addc(r.w[k], pword(k) & mask, c, &c);
+  for (int k = 0; k < 8; k++) r.w[k] = __builtin_addc(a.w[k], b.w[k], c, &c);
+  return r;
+}
+// 6p = 6 + 102 * 2^192 + 3 * 2^252: words 0, 6 and 7.
+__device__ __forceinline__ constexpr uint32_t p6word(int k) { return k == 0 ? 6u : k == 6 ? 0x66u : k == 7 ? 0x30000000u : 0u; }
+__device__ __forceinline__ F252 f_6p_minus(const F252& x) {  // x < 6p
+  F252 r;
+  uint32_t c = 1;  // 6p + ~x + 1
+#pragma unroll
+  for (int k = 0; k < 8; k++) r.w[k] = __builtin_addc(p6word(k), ~x.w[k], c, &c);
+  return r;
+}
+// x < 16p: with q = x >> 251, x - q p = (x mod 2^251) - q (17 * 2^192 + 1)
+// lies in (-2^200, 2^251); p is added back when it is negative.
+__device__ __forceinline__ F252 f_red16(const F252& x) {
+  const uint32_t q = x.w[7] >> 27;
+  F252 r;
+  uint32_t c = 1;  // low + ~(q + 17 q 2^192) + 1; carry out = no borrow
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t xk = k == 7 ? (x.w[7] & 0x07FFFFFFu) : x.w[k];
+    const uint32_t sk = k == 0 ? q : (k == 6 ? 17u * q : 0u);
+    r.w[k] = __builtin_addc(xk, ~sk, c, &c);
+  }
+  const uint32_t mask = c - 1u;  // all ones on a borrow
+  uint32_t d = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) r.w[k] = __builtin_addc(r.w[k], pword(k) & mask, d, &d);
   return r;
 }
 
@@ -96,7 +122,10 @@ __device__ __forceinline__ void mac(uint64_t& acc, uint32_t& ovf, uint32_t x, ui
   acc = (uint64_t)x * y + acc;  // one v_mad_u64_u32 into the accumulator
   ovf += acc < prev;
 }
-__device__ __forceinline__ F252 f_mul(const F252& a, const F252& b) {
+// Inputs below 4p give a result below 2p (a b / 2^256 < 2^250 < p/2, plus
+// m p / 2^256 < p): the permutation keeps its states in [0, 2p) and leaves
+// this product unreduced.
+__device__ __forceinline__ F252 f_mul_lazy(const F252& a, const F252& b) {
   uint32_t m[8], r[8];
   uint64_t acc = 0;
   uint32_t ovf = 0;
@@ -124,36 +153,45 @@ __device__ __forceinline__ F252 f_mul(const F252& a, const F252& b) {
   F252 t;
 #pragma unroll
   for (int k = 0; k < 8; k++) t.w[k] = r[k];
+  return t;
+}
+__device__ __forceinline__ F252 f_mul(const F252& a, const F252& b) {
+  F252 t = f_mul_lazy(a, b);
   f_reduce_once(t);
   return t;
 }
 
-__device__ __forceinline__ F252 f_cube(const F252& x) { return f_mul(f_mul(x, x), x); }
+// x below 4p -> x^3 below 2p.
+__device__ __forceinline__ F252 f_cube(const F252& x) { return f_mul_lazy(f_mul_lazy(x, x), x); }
 
-// s <- M s with M = [[3,1,1],[1,-1,1],[1,1,-2]].
+// s <- M s with M = [[3,1,1],[1,-1,1],[1,1,-2]] for s0, s1 < 3p and s2 < 2p
+// (s2 is always an S-box output): t < 8p, and each row, with 6p added
+// before a subtraction, stays below 14p before its one reduction to [0, 2p).
 __device__ __forceinline__ void mix(F252 (&s)[3]) {
-  const F252 t = f_add(f_add(s[0], s[1]), s[2]);
-  const F252 s0 = f_add(t, f_add(s[0], s[0]));
-  const F252 s1 = f_sub(t, f_add(s[1], s[1]));
-  const F252 s2 = f_sub(t, f_add(f_add(s[2], s[2]), s[2]));
+  const F252 t = f_add_nr(f_add_nr(s[0], s[1]), s[2]);
+  const F252 s0 = f_red16(f_add_nr(t, f_add_nr(s[0], s[0])));
+  const F252 s1 = f_red16(f_add_nr(t, f_6p_minus(f_add_nr(s[1], s[1]))));
+  const F252 s2 = f_red16(f_add_nr(t, f_6p_minus(f_add_nr(f_add_nr(s[2], s[2]), s[2]))));
   s[0] = s0;
   s[1] = s1;
   s[2] = s2;
 }
 
+// States in [0, 2p); a round constant (< p) brings them below 3p.
 __device__ __forceinline__ void full_round(F252 (&s)[3], uint32_t r) {
 #pragma unroll
-  for (int j = 0; j < 3; j++) s[j] = f_cube(f_add(s[j], c_ark[3 * r + j]));
+  for (int j = 0; j < 3; j++) s[j] = f_cube(f_add_nr(s[j], c_ark[3 * r + j]));
   mix(s);
 }
 
 __device__ __forceinline__ void partial_round(F252 (&s)[3], uint32_t r) {
-  s[0] = f_add(s[0], c_ark[3 * r]);
-  s[1] = f_add(s[1], c_ark[3 * r + 1]);
-  s[2] = f_cube(f_add(s[2], c_ark[3 * r + 2]));
+  s[0] = f_add_nr(s[0], c_ark[3 * r]);
+  s[1] = f_add_nr(s[1], c_ark[3 * r + 1]);
+  s[2] = f_cube(f_add_nr(s[2], c_ark[3 * r + 2]));
   mix(s);
 }
 
+// Canonical states in and out; between the rounds they stay in [0, 2p).
 // The loops stay rolled: one round's code is a few thousand instructions.
 __device__ __forceinline__ void permute(F252 (&s)[3]) {
 #pragma unroll 1
@@ -162,6 +200,8 @@ __device__ __forceinline__ void permute(F252 (&s)[3]) {
   for (uint32_t r = VX_POSEIDON_FULL_HALF; r < VX_POSEIDON_FULL_HALF + VX_POSEIDON_PARTIAL; r++) partial_round(s, r);
 #pragma unroll 1
   for (uint32_t r = VX_POSEIDON_FULL_HALF + VX_POSEIDON_PARTIAL; r < VX_POSEIDON_ROUNDS; r++) full_round(s, r);
+#pragma unroll
+  for (int j = 0; j < 3; j++) f_reduce_once(s[j]);
 }
 
 // Elements are 4 x u64 (8-byte aligned): four 8-byte accesses each.
